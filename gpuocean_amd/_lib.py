@@ -1,0 +1,113 @@
+"""ctypes binding of ``libgpuocean_amd.so`` (the C ABI in include/gpuocean_amd.h).
+
+The library is built ahead of time (``python -m gpuocean_amd.build``).  There is
+no fallback: if the library is missing or cannot be loaded, every simulator
+constructor raises.
+"""
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgpuocean_amd.so")
+
+c_int32 = ctypes.c_int32
+c_float = ctypes.c_float
+c_void_p = ctypes.c_void_p
+c_uint32 = ctypes.c_uint32
+c_fp = ctypes.POINTER(ctypes.c_float)
+
+
+class CDKLM16Params(ctypes.Structure):
+    _fields_ = [
+        ("nx", c_int32), ("ny", c_int32), ("rk_order", c_int32), ("one_dimensional", c_int32),
+        ("has_land", c_int32), ("f_is_array", c_int32), ("angle_is_zero", c_int32), ("chunk_rows", c_int32),
+        ("dx", c_float), ("dy", c_float), ("g", c_float), ("f", c_float), ("r", c_float),
+        ("theta", c_float), ("desing_eps", c_float), ("flux_slope_eps", c_float),
+        ("depth_cutoff", c_float), ("flux_balancer", c_float), ("land_value", c_float),
+        ("north_x", c_float), ("north_y", c_float), ("wind_stress_factor", c_float), ("rho_o", c_float),
+    ]
+
+
+class Forcing(ctypes.Structure):
+    _fields_ = [
+        ("wind_x_cur", c_void_p), ("wind_x_next", c_void_p), ("wind_y_cur", c_void_p), ("wind_y_next", c_void_p),
+        ("wind_x_nx", c_int32), ("wind_x_ny", c_int32), ("wind_y_nx", c_int32), ("wind_y_ny", c_int32),
+        ("wind_t", c_float),
+        ("pressure_cur", c_void_p), ("pressure_next", c_void_p),
+        ("pressure_nx", c_int32), ("pressure_ny", c_int32), ("pressure_t", c_float),
+    ]
+
+
+class GpoError(RuntimeError):
+    pass
+
+
+_lib = None
+
+# (name, restype, argtypes)
+_SIGNATURES = [
+    ("gpo_cdklm16_substep", c_int32,
+     [ctypes.POINTER(CDKLM16Params), c_float, c_int32,
+      c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int32,
+      c_void_p, c_int32, c_void_p, ctypes.POINTER(Forcing), c_uint32, c_int32, c_void_p]),
+    ("gpo_boundary_conditions", c_int32,
+     [c_int32, c_int32, c_uint32, ctypes.POINTER(c_int32), c_void_p, c_void_p, c_void_p, c_int32,
+      c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_int32, c_float, c_void_p]),
+    ("gpo_bathymetry_init", c_int32,
+     [c_int32, c_int32, c_uint32, c_float, c_void_p, c_int32, c_void_p, c_int32, c_void_p]),
+    ("gpo_coriolis_cells", c_int32,
+     [c_int32, c_int32, c_void_p, c_int32, c_int32, c_void_p, c_int32, c_void_p]),
+    ("gpo_max_dt", c_int32,
+     [c_int32, c_int32, c_float, c_float, c_float, c_float, c_void_p, c_void_p, c_void_p, c_int32,
+      c_void_p, c_int32, c_void_p, c_void_p]),
+    ("gpo_last_error", ctypes.c_char_p, []),
+    ("gpo_version", ctypes.c_char_p, []),
+]
+
+# Optional symbols bound when the library exports them.
+_OPTIONAL = []
+
+
+def declared_symbols():
+    """Names of every function declared in include/gpuocean_amd.h."""
+    import re
+    hdr = os.path.join(os.path.dirname(HERE), "include", "gpuocean_amd.h")
+    with open(hdr) as f:
+        text = f.read()
+    return sorted(set(re.findall(r"\b(gpo_[a-z0-9_]+)\s*\(", text)))
+
+
+def lib():
+    """The loaded library; raises if it is missing (no CPU fallback exists)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise GpoError(f"{LIB_PATH} is missing: build it with `python -m gpuocean_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    for name, res, args in _SIGNATURES + _OPTIONAL:
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def register(name, restype, argtypes):
+    _OPTIONAL.append((name, restype, argtypes))
+    if _lib is not None:
+        fn = getattr(_lib, name)
+        fn.restype = restype
+        fn.argtypes = argtypes
+
+
+def check(rc, what=""):
+    if rc != 0:
+        msg = lib().gpo_last_error().decode(errors="replace")
+        raise GpoError(f"{what} failed ({rc}): {msg}")
+
+
+def ptr(t):
+    """Device (or host) address of a torch tensor, or None."""
+    return None if t is None else ctypes.c_void_p(t.data_ptr())

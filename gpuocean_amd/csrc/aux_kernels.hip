@@ -1,0 +1,414 @@
+// Boundary-condition, bathymetry, Coriolis-texture and CFL kernels of the
+// CDKLM16 path, plus the C ABI entry points that dispatch them.  Built with
+// -ffp-contract=off so their float arithmetic matches the host oracle.
+
+#include <cstdio>
+#include <string>
+
+#include "cdklm_common.h"
+
+namespace gpo {
+
+static thread_local std::string g_last_error;
+void set_error(const char* msg) { g_last_error = msg ? msg : ""; }
+
+namespace {
+
+inline int ceil_div(int a, int b) { return (a + b - 1) / b; }
+
+__device__ __forceinline__ float* row_ptr(float* base, int pitch_bytes, int j) {
+  return (float*)((char*)base + (size_t)pitch_bytes * j);
+}
+__device__ __forceinline__ const float* row_ptr(const float* base, int pitch_bytes, int j) {
+  return (const float*)((const char*)base + (size_t)pitch_bytes * j);
+}
+
+// periodicBoundary_NS (boundary_kernels.cu:36-72): ghost rows copy the
+// opposite interior rows, every column including ghost columns.
+__global__ void periodic_ns(int nx, int ny, float* h, float* u, float* v, int pitch) {
+  const int ti = blockIdx.x * blockDim.x + threadIdx.x;
+  const int g = blockIdx.y;  // 0..3: ghost rows 0, 1, ny+2, ny+3
+  if (ti >= nx + 4) return;
+  const int tj = (g < 2) ? g : ny + g;
+  const int opp = (tj > ny + 1) ? tj - ny : tj + ny;
+  row_ptr(h, pitch, tj)[ti] = row_ptr(h, pitch, opp)[ti];
+  row_ptr(u, pitch, tj)[ti] = row_ptr(u, pitch, opp)[ti];
+  row_ptr(v, pitch, tj)[ti] = row_ptr(v, pitch, opp)[ti];
+}
+
+// periodicBoundary_EW (boundary_kernels.cu:77-110).
+__global__ void periodic_ew(int nx, int ny, float* h, float* u, float* v, int pitch) {
+  const int tj = blockIdx.x * blockDim.x + threadIdx.x;
+  const int g = blockIdx.y;  // ghost columns 0, 1, nx+2, nx+3
+  if (tj >= ny + 4) return;
+  const int ti = (g < 2) ? g : nx + g;
+  const int opp = (ti > nx + 1) ? ti - nx : ti + nx;
+  float* hr = row_ptr(h, pitch, tj);
+  float* ur = row_ptr(u, pitch, tj);
+  float* vr = row_ptr(v, pitch, tj);
+  hr[ti] = hr[opp];
+  ur[ti] = ur[opp];
+  vr[ti] = vr[opp];
+}
+
+__device__ float3 bilinear3(const float* data, int dnx, int dny, float nx_, float ny_) {
+  nx_ = fminf(fmaxf(nx_ - 0.5f / (float)dnx, 0.0f), 1.0f);
+  ny_ = fminf(fmaxf(ny_ - 0.5f / (float)dny, 0.0f), 1.0f);
+  const float x = nx_ * (float)dnx, y = ny_ * (float)dny;
+  int x0 = (int)floorf(x), y0 = (int)floorf(y);
+  const float fx = x - (float)x0, fy = y - (float)y0;
+  x0 = min(x0, dnx - 1);
+  y0 = min(y0, dny - 1);
+  const int x1 = min(x0 + 1, dnx - 1), y1 = min(y0 + 1, dny - 1);
+  float res[3];
+  for (int c = 0; c < 3; ++c) {
+    const float d00 = data[(y0 * dnx + x0) * 3 + c], d01 = data[(y0 * dnx + x1) * 3 + c];
+    const float d10 = data[(y1 * dnx + x0) * 3 + c], d11 = data[(y1 * dnx + x1) * 3 + c];
+    res[c] = d00 + fx * (d01 - d00) + fy * (d10 - d00) + fx * fy * (d00 + d11 - d01 - d10);
+  }
+  return make_float3(res[0], res[1], res[2]);
+}
+
+// flowRelaxationScheme_NS (boundary_kernels.cu:406-472)
+__global__ void frs_ns(int bc_n, int bc_s, int nx, int ny, int sn, int ss, float* h, float* u, float* v, int pitch,
+                       const float* cur, const float* next, int bc_nx, float t_) {
+  const int ti = blockIdx.x * blockDim.x + threadIdx.x;
+  const int tj = blockIdx.y * blockDim.y + threadIdx.y;
+  if (ti >= nx + 4 || tj >= ny + 4) return;
+  const bool in_s = (bc_s == 3) && (tj < ss);
+  const bool in_n = (bc_n == 3) && (tj > ny + 4 - 1 - sn);
+  if (!(in_s || in_n)) return;
+  const float j = (tj <= ss) ? (float)(tj - 1) : (float)((ny + 4 - 2) - tj);
+  const float alpha = 1.0f - tanhf(fmaxf(0.0f, j) / 2.0f);
+  const float s = (float)ti / (float)(nx + 4);
+  const float t = ((double)tj < ny / 2.0) ? 0.25f : 0.75f;
+  const float3 c = bilinear3(cur, bc_nx, 2, s, t);
+  const float3 n = bilinear3(next, bc_nx, 2, s, t);
+  const float eh = t_ * n.x + (1.0f - t_) * c.x;
+  const float eu = t_ * n.y + (1.0f - t_) * c.y;
+  const float ev = t_ * n.z + (1.0f - t_) * c.z;
+  float* hr = row_ptr(h, pitch, tj);
+  float* ur = row_ptr(u, pitch, tj);
+  float* vr = row_ptr(v, pitch, tj);
+  hr[ti] = (1.0f - alpha) * hr[ti] + alpha * eh;
+  ur[ti] = (1.0f - alpha) * ur[ti] + alpha * eu;
+  vr[ti] = (1.0f - alpha) * vr[ti] + alpha * ev;
+}
+
+// flowRelaxationScheme_EW (boundary_kernels.cu:477-540)
+__global__ void frs_ew(int bc_e, int bc_w, int nx, int ny, int se, int sw, float* h, float* u, float* v, int pitch,
+                       const float* cur, const float* next, int bc_ny, float t_) {
+  const int ti = blockIdx.x * blockDim.x + threadIdx.x;
+  const int tj = blockIdx.y * blockDim.y + threadIdx.y;
+  if (ti >= nx + 4 || tj >= ny + 4) return;
+  const bool in_w = (bc_w == 3) && (ti < sw);
+  const bool in_e = (bc_e == 3) && (ti > nx + 4 - 1 - se);
+  if (!(in_w || in_e)) return;
+  const float j = (ti <= sw) ? (float)(ti - 1) : (float)((nx + 4 - 2) - ti);
+  const float alpha = 1.0f - tanhf(fmaxf(0.0f, j) / 2.0f);
+  const float s = ((double)ti < nx / 2.0) ? 0.25f : 0.75f;
+  const float t = (float)tj / (float)(ny + 4);
+  const float3 c = bilinear3(cur, 2, bc_ny, s, t);
+  const float3 n = bilinear3(next, 2, bc_ny, s, t);
+  const float eh = t_ * n.x + (1.0f - t_) * c.x;
+  const float eu = t_ * n.y + (1.0f - t_) * c.y;
+  const float ev = t_ * n.z + (1.0f - t_) * c.z;
+  float* hr = row_ptr(h, pitch, tj);
+  float* ur = row_ptr(u, pitch, tj);
+  float* vr = row_ptr(v, pitch, tj);
+  hr[ti] = (1.0f - alpha) * hr[ti] + alpha * eh;
+  ur[ti] = (1.0f - alpha) * ur[ti] + alpha * eu;
+  vr[ti] = (1.0f - alpha) * vr[ti] + alpha * ev;
+}
+
+// periodic_boundary_intersections_NS / closed_boundary_intersections_NS
+// (boundary_kernels.cu:122-147, 188-212): one thread per corner column.
+__global__ void hi_ns(int nx, int ny, int periodic, float* Hi, int pitch) {
+  const int ti = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ti >= nx + 5) return;
+  if (periodic) {
+    // ghost rows 0,1 <- ny, ny+1 ; rows ny+2..ny+4 <- 2..4
+    for (int tj = 0; tj < ny + 5; ++tj) {
+      if (!(tj < 2 || tj > ny + 1)) continue;
+      const int opp = (tj > ny + 1) ? tj - ny : tj + ny;
+      row_ptr(Hi, pitch, tj)[ti] = row_ptr(Hi, pitch, opp)[ti];
+    }
+  } else {
+    for (int j = 0; j < 2; ++j) row_ptr(Hi, pitch, j)[ti] = row_ptr(Hi, pitch, 4 - j)[ti];
+    for (int j = 0; j < 2; ++j) row_ptr(Hi, pitch, ny + 4 - j)[ti] = row_ptr(Hi, pitch, ny + j)[ti];
+  }
+}
+
+// periodic_boundary_intersections_EW / closed_boundary_intersections_EW
+// (boundary_kernels.cu:152-179, 216-246): one thread per corner row.
+__global__ void hi_ew(int nx, int ny, int periodic, float* Hi, int pitch) {
+  const int tj = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tj >= ny + 5) return;
+  float* r = row_ptr(Hi, pitch, tj);
+  if (periodic) {
+    for (int ti = 0; ti < nx + 5; ++ti) {
+      if (!(ti < 2 || ti > nx + 1)) continue;
+      const int opp = (ti > nx + 1) ? ti - nx : ti + nx;
+      r[ti] = r[opp];
+    }
+  } else {
+    for (int i = 0; i < 2; ++i) r[i] = r[4 - i];
+    for (int i = 0; i < 2; ++i) r[nx + 4 - i] = r[nx + i];
+  }
+}
+
+// initBm (initHm_kernel.cu:38-106)
+__global__ void init_bm(int nx, int ny, float land, const float* Hi, int hpitch, float* Bm, int bpitch) {
+  const int ti = blockIdx.x * blockDim.x + threadIdx.x;
+  const int tj = blockIdx.y * blockDim.y + threadIdx.y;
+  if (ti >= nx + 4 || tj >= ny + 4) return;
+  auto fl = [land](float x) { return (fabsf(x - land) <= kDryEps) ? kDryFlag : x; };
+  const float a = fl(row_ptr(Hi, hpitch, tj)[ti]);
+  const float b = fl(row_ptr(Hi, hpitch, tj + 1)[ti]);
+  const float c = fl(row_ptr(Hi, hpitch, tj)[ti + 1]);
+  const float d = fl(row_ptr(Hi, hpitch, tj + 1)[ti + 1]);
+  const float wet = 4.0f - (float)(a == kDryFlag) - (float)(b == kDryFlag) - (float)(c == kDryFlag) -
+                    (float)(d == kDryFlag);
+  row_ptr(Bm, bpitch, tj)[ti] = (wet == 0.0f) ? land : (a + b + c + d) / wet;
+}
+
+__device__ float bilinear1(const float* data, int dnx, int dny, float nx_, float ny_) {
+  nx_ = fminf(fmaxf(nx_ - 0.5f / (float)dnx, 0.0f), 1.0f);
+  ny_ = fminf(fmaxf(ny_ - 0.5f / (float)dny, 0.0f), 1.0f);
+  const float x = nx_ * (float)dnx, y = ny_ * (float)dny;
+  int x0 = (int)floorf(x), y0 = (int)floorf(y);
+  const float fx = x - (float)x0, fy = y - (float)y0;
+  x0 = min(x0, dnx - 1);
+  y0 = min(y0, dny - 1);
+  const int x1 = min(x0 + 1, dnx - 1), y1 = min(y0 + 1, dny - 1);
+  const float d00 = data[y0 * dnx + x0], d01 = data[y0 * dnx + x1];
+  const float d10 = data[y1 * dnx + x0], d11 = data[y1 * dnx + x1];
+  return d00 + fx * (d01 - d00) + fy * (d10 - d00) + fx * fy * (d00 + d11 - d01 - d10);
+}
+
+// coriolisF at every cell (CDKLM16_kernel.cu:48-57)
+__global__ void coriolis_cells(int nx, int ny, const float* tex, int fnx, int fny, float* out, int pitch) {
+  const int ti = blockIdx.x * blockDim.x + threadIdx.x;
+  const int tj = blockIdx.y * blockDim.y + threadIdx.y;
+  if (ti >= nx + 4 || tj >= ny + 4) return;
+  const float s = ((float)ti + 0.5f) / ((float)nx + 4.0f);
+  const float t = ((float)tj + 0.5f) / ((float)ny + 4.0f);
+  row_ptr(out, pitch, tj)[ti] = bilinear1(tex, fnx, fny, s, t);
+}
+
+__global__ void fill_scalar(float* p, float v) { *p = v; }
+
+// per_block_max_dt + max_dt_reduction (max_dt.cu:42-195).  Positive floats
+// order like their bit patterns, so the global minimum is an integer atomic.
+__global__ void __launch_bounds__(256) max_dt_kernel(int nx, int ny, float dx, float dy, float g, float land,
+                                                     const float* eta, const float* hu, const float* hv, int pitch,
+                                                     const float* Bm, int bpitch, float* out) {
+  const int ti = blockIdx.x * 64 + (threadIdx.x & 63) + 2;
+  const int tj = blockIdx.y * 4 + (threadIdx.x >> 6) + 2;
+  float d = kMaxDtCap;
+  if (ti < nx + 2 && tj < ny + 2) {
+    const float e = row_ptr(eta, pitch, tj)[ti];
+    const float H = row_ptr(Bm, bpitch, tj)[ti];
+    const float h = e + H;
+    if (!(h <= 0.0f || fabsf(H - land) < kDryEps)) {
+      const float u = row_ptr(hu, pitch, tj)[ti] / h;
+      const float v = row_ptr(hv, pitch, tj)[ti] / h;
+      const float gw = sqrtf(g * h);
+      d = 0.25f * dx / fabsf(u + gw);
+      d = fminf(d, 0.25f * dx / fabsf(u - gw));
+      d = fminf(d, 0.25f * dy / fabsf(v + gw));
+      d = fminf(d, 0.25f * dy / fabsf(v - gw));
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) d = fminf(d, __shfl_xor(d, off, 64));
+  __shared__ float wmin[4];
+  if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = d;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const float m = fminf(fminf(wmin[0], wmin[1]), fminf(wmin[2], wmin[3]));
+    atomicMin((unsigned int*)out, __float_as_uint(m));
+  }
+}
+
+int check(hipError_t e, const char* what) {
+  if (e != hipSuccess) {
+    char buf[256];
+    snprintf(buf, sizeof buf, "%s: %s", what, hipGetErrorString(e));
+    set_error(buf);
+    return (int)e;
+  }
+  return 0;
+}
+
+int bad(const char* msg) {
+  set_error(msg);
+  return GPO_ERR_BAD_ARGUMENT;
+}
+
+}  // namespace
+}  // namespace gpo
+
+using namespace gpo;
+
+extern "C" {
+
+const char* gpo_last_error(void) { return g_last_error.c_str(); }
+
+const char* gpo_version(void) { return "gpuocean_amd 0.1 (gfx950)"; }
+
+int gpo_cdklm16_substep(const gpo_cdklm16_params* p, float dt, int32_t rk_step, const float* in_eta,
+                        const float* in_hu, const float* in_hv, const float* qn_eta, const float* qn_hu,
+                        const float* qn_hv, float* out_eta, float* out_hu, float* out_hv, int32_t pitch,
+                        const float* Hi, int32_t Hi_pitch, const float* f_cells, const gpo_forcing* forcing,
+                        uint32_t boundary_conditions, int32_t exact, void* stream) {
+  if (!p) return bad("gpo_cdklm16_substep: params is NULL");
+  if (p->nx < 1 || p->ny < 1) return bad("gpo_cdklm16_substep: nx and ny must be positive");
+  if (!in_eta || !in_hu || !in_hv || !out_eta || !out_hu || !out_hv || !Hi)
+    return bad("gpo_cdklm16_substep: NULL state or bathymetry pointer");
+  if (pitch % 4 || Hi_pitch % 4) return bad("gpo_cdklm16_substep: pitches must be multiples of 4 bytes");
+  if (pitch < 4 * (p->nx + 4) || Hi_pitch < 4 * (p->nx + 5))
+    return bad("gpo_cdklm16_substep: pitch smaller than the row");
+  if (p->rk_order < 1 || p->rk_order > 3) return bad("gpo_cdklm16_substep: rk_order must be 1, 2 or 3");
+  if (rk_step < 0 || rk_step >= p->rk_order) return bad("gpo_cdklm16_substep: rk_step out of range");
+  if (rk_step > 0 && (!qn_eta || !qn_hu || !qn_hv)) return bad("gpo_cdklm16_substep: Q^n pointers needed");
+  if (p->f_is_array && !f_cells) return bad("gpo_cdklm16_substep: f_is_array set but f_cells is NULL");
+  if (p->rk_order == 3 && p->r != 0.0f) return bad("gpo_cdklm16_substep: RK3 requires r == 0");
+
+  StepArgs a{};
+  a.nx = p->nx;
+  a.ny = p->ny;
+  a.pitch = pitch / 4;
+  a.hi_pitch = Hi_pitch / 4;
+  a.chunk_rows = p->chunk_rows > 0 ? p->chunk_rows : 16;
+  a.n_strips = (p->nx + kStripOut - 1) / kStripOut;
+  a.n_chunks = (p->ny + a.chunk_rows - 1) / a.chunk_rows;
+  a.bc_n = (boundary_conditions >> 24) & 0xFF;
+  a.bc_s = (boundary_conditions >> 16) & 0xFF;
+  a.bc_e = (boundary_conditions >> 8) & 0xFF;
+  a.bc_w = boundary_conditions & 0xFF;
+  a.rk_order = p->rk_order;
+  a.rk_step = rk_step;
+  a.one_dim = p->one_dimensional;
+  a.dt = dt;
+  a.dx = p->dx;
+  a.dy = p->dy;
+  a.g = p->g;
+  a.r = p->r;
+  a.theta = p->theta;
+  a.desing_eps = p->desing_eps;
+  a.flux_slope_eps = p->flux_slope_eps;
+  a.depth_cutoff = p->depth_cutoff;
+  a.flux_balancer = p->flux_balancer;
+  a.land_value = p->land_value;
+  a.f = p->f;
+  a.north_x = p->north_x;
+  a.north_y = p->north_y;
+  a.wind_stress_factor = p->wind_stress_factor;
+  a.rho_o = p->rho_o;
+  // Derived constants, formed in float exactly as the reference's constant
+  // expressions are (e.g. THETA*GRAV, DX/(2.0f*GRAV)).
+  a.g2 = 2.0f * p->g;
+  a.inv_g2 = 1.0f / a.g2;
+  a.half_g = 0.5f * p->g;
+  a.theta_g = p->theta * p->g;
+  a.Vc = p->dx / (2.0f * p->g);
+  a.Uc = p->dy / (2.0f * p->g);
+  a.inv_dx = 1.0f / p->dx;
+  a.inv_dy = 1.0f / p->dy;
+  a.one_m_fb = 1.0f - p->flux_balancer;
+  a.dxf = p->dx * p->f;
+  a.dyf = p->dy * p->f;
+  a.in_e = in_eta;
+  a.in_u = in_hu;
+  a.in_v = in_hv;
+  a.qn_e = qn_eta ? qn_eta : out_eta;
+  a.qn_u = qn_hu ? qn_hu : out_hu;
+  a.qn_v = qn_hv ? qn_hv : out_hv;
+  a.out_e = out_eta;
+  a.out_u = out_hu;
+  a.out_v = out_hv;
+  a.Hi = Hi;
+  a.fcell = f_cells;
+  if (forcing && (forcing->wind_x_cur || forcing->pressure_cur)) {
+    a.has_forcing = 1;
+    a.forcing = *forcing;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  const hipError_t e = exact ? launch_cdklm_exact(a, p->has_land, p->f_is_array, p->angle_is_zero, s)
+                             : launch_cdklm_fast(a, p->has_land, p->f_is_array, p->angle_is_zero, s);
+  return check(e, "gpo_cdklm16_substep launch");
+}
+
+int gpo_boundary_conditions(int32_t nx, int32_t ny, uint32_t bc, const int32_t sponge[4], float* eta, float* hu,
+                            float* hv, int32_t pitch, const float* ns_cur, const float* ns_next, int32_t ns_nx,
+                            const float* ew_cur, const float* ew_next, int32_t ew_ny, float frs_t, void* stream) {
+  if (nx < 1 || ny < 1 || !eta || !hu || !hv) return bad("gpo_boundary_conditions: bad arguments");
+  const int bn = (bc >> 24) & 0xFF, bs = (bc >> 16) & 0xFF, be = (bc >> 8) & 0xFF, bw = bc & 0xFF;
+  hipStream_t s = (hipStream_t)stream;
+  const int sn = sponge ? sponge[0] : 0, se = sponge ? sponge[1] : 0, ss = sponge ? sponge[2] : 0,
+            sw = sponge ? sponge[3] : 0;
+  if (bn == 2) {
+    hipLaunchKernelGGL(periodic_ns, dim3(ceil_div(nx + 4, 256), 4), dim3(256), 0, s, nx, ny, eta, hu, hv, pitch);
+  } else if (bn == 3 || bs == 3) {
+    if (!ns_cur || !ns_next || ns_nx < 1) return bad("gpo_boundary_conditions: FRS north/south data missing");
+    hipLaunchKernelGGL(frs_ns, dim3(ceil_div(nx + 4, 16), ceil_div(ny + 4, 16)), dim3(16, 16), 0, s, bn, bs, nx, ny,
+                       sn, ss, eta, hu, hv, pitch, ns_cur, ns_next, ns_nx, frs_t);
+  } else if (bn == 4 || bs == 4) {
+    set_error("gpo_boundary_conditions: open linear interpolation (code 4) is not implemented");
+    return GPO_ERR_UNSUPPORTED;
+  }
+  if (be == 2) {
+    hipLaunchKernelGGL(periodic_ew, dim3(ceil_div(ny + 4, 256), 4), dim3(256), 0, s, nx, ny, eta, hu, hv, pitch);
+  } else if (be == 3 || bw == 3) {
+    if (!ew_cur || !ew_next || ew_ny < 1) return bad("gpo_boundary_conditions: FRS east/west data missing");
+    hipLaunchKernelGGL(frs_ew, dim3(ceil_div(nx + 4, 16), ceil_div(ny + 4, 16)), dim3(16, 16), 0, s, be, bw, nx, ny,
+                       se, sw, eta, hu, hv, pitch, ew_cur, ew_next, ew_ny, frs_t);
+  } else if (be == 4 || bw == 4) {
+    set_error("gpo_boundary_conditions: open linear interpolation (code 4) is not implemented");
+    return GPO_ERR_UNSUPPORTED;
+  }
+  return check(hipGetLastError(), "gpo_boundary_conditions launch");
+}
+
+int gpo_bathymetry_init(int32_t nx, int32_t ny, uint32_t bc, float land_value, float* Hi, int32_t Hi_pitch, float* Bm,
+                        int32_t Bm_pitch, void* stream) {
+  if (nx < 1 || ny < 1 || !Hi || !Bm) return bad("gpo_bathymetry_init: bad arguments");
+  const int bn = (bc >> 24) & 0xFF, bs = (bc >> 16) & 0xFF, be = (bc >> 8) & 0xFF, bw = bc & 0xFF;
+  hipStream_t s = (hipStream_t)stream;
+  if (bn == 2 && bs == 2) {
+    hipLaunchKernelGGL(hi_ns, dim3(ceil_div(nx + 5, 256)), dim3(256), 0, s, nx, ny, 1, Hi, Hi_pitch);
+  } else if (bn == 1 && bs == 1) {
+    hipLaunchKernelGGL(hi_ns, dim3(ceil_div(nx + 5, 256)), dim3(256), 0, s, nx, ny, 0, Hi, Hi_pitch);
+  }
+  // The closed east-west branch is taken on the north/south codes, as in the
+  // reference (utils/Common.py:1557).
+  if (be == 2 && bw == 2) {
+    hipLaunchKernelGGL(hi_ew, dim3(ceil_div(ny + 5, 256)), dim3(256), 0, s, nx, ny, 1, Hi, Hi_pitch);
+  } else if (bn == 1 && bs == 1) {
+    hipLaunchKernelGGL(hi_ew, dim3(ceil_div(ny + 5, 256)), dim3(256), 0, s, nx, ny, 0, Hi, Hi_pitch);
+  }
+  hipLaunchKernelGGL(init_bm, dim3(ceil_div(nx + 4, 16), ceil_div(ny + 4, 16)), dim3(16, 16), 0, s, nx, ny,
+                     land_value, (const float*)Hi, Hi_pitch, Bm, Bm_pitch);
+  return check(hipGetLastError(), "gpo_bathymetry_init launch");
+}
+
+int gpo_coriolis_cells(int32_t nx, int32_t ny, const float* f_tex, int32_t f_nx, int32_t f_ny, float* f_cells,
+                       int32_t pitch, void* stream) {
+  if (nx < 1 || ny < 1 || !f_tex || !f_cells || f_nx < 1 || f_ny < 1) return bad("gpo_coriolis_cells: bad arguments");
+  hipLaunchKernelGGL(coriolis_cells, dim3(ceil_div(nx + 4, 16), ceil_div(ny + 4, 16)), dim3(16, 16), 0,
+                     (hipStream_t)stream, nx, ny, f_tex, f_nx, f_ny, f_cells, pitch);
+  return check(hipGetLastError(), "gpo_coriolis_cells launch");
+}
+
+int gpo_max_dt(int32_t nx, int32_t ny, float dx, float dy, float g, float land_value, const float* eta,
+               const float* hu, const float* hv, int32_t pitch, const float* Bm, int32_t Bm_pitch, float* dt_out,
+               void* stream) {
+  if (nx < 1 || ny < 1 || !eta || !hu || !hv || !Bm || !dt_out) return bad("gpo_max_dt: bad arguments");
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(fill_scalar, dim3(1), dim3(1), 0, s, dt_out, kMaxDtCap);
+  hipLaunchKernelGGL(max_dt_kernel, dim3(ceil_div(nx, 64), ceil_div(ny, 4)), dim3(256), 0, s, nx, ny, dx, dy, g,
+                     land_value, eta, hu, hv, pitch, Bm, Bm_pitch, dt_out);
+  return check(hipGetLastError(), "gpo_max_dt launch");
+}
+
+}  // extern "C"

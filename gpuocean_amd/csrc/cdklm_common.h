@@ -1,0 +1,53 @@
+// Shared declarations of the CDKLM16 HIP library (internal, not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/gpuocean_amd.h"
+
+namespace gpo {
+
+// Values that must match between the step, bathymetry and CFL kernels
+// (gpu_kernels/CDKLM16_kernel.cu:34-35, initHm_kernel.cu:27-28, max_dt.cu:30-35).
+constexpr float kDryFlag = 1.0e-30f;
+constexpr float kDryEps = 1.0e-3f;
+constexpr float kMaxDtCap = 100000.0f;
+
+// Output columns per 64-lane wavefront: lanes 0,1 and 62,63 are the halo.
+constexpr int kStripOut = 60;
+constexpr int kWave = 64;
+constexpr int kWavesPerBlock = 4;
+
+struct StepArgs {
+  // geometry
+  int nx, ny, pitch, hi_pitch;  // pitches in elements
+  int n_strips, n_chunks, chunk_rows;
+  int bc_n, bc_e, bc_s, bc_w;
+  int rk_order, rk_step, one_dim, has_forcing;
+  // constants as float32, derived on the host the same way the kernel would
+  float dt, dx, dy, g, r, theta, desing_eps, flux_slope_eps, depth_cutoff, flux_balancer, land_value;
+  float f, north_x, north_y, wind_stress_factor, rho_o;
+  float g2, inv_g2, half_g, theta_g, Vc, Uc, inv_dx, inv_dy, one_m_fb, dxf, dyf;
+  // buffers
+  const float* in_e;
+  const float* in_u;
+  const float* in_v;
+  const float* qn_e;
+  const float* qn_u;
+  const float* qn_v;
+  float* out_e;
+  float* out_u;
+  float* out_v;
+  const float* Hi;
+  const float* fcell;
+  gpo_forcing forcing;
+};
+
+// Launchers of the two builds of the step kernel (fast / exact).
+hipError_t launch_cdklm_fast(const StepArgs& a, int has_land, int f_array, int angle0, hipStream_t s);
+hipError_t launch_cdklm_exact(const StepArgs& a, int has_land, int f_array, int angle0, hipStream_t s);
+
+void set_error(const char* msg);
+
+}  // namespace gpo

@@ -1,0 +1,128 @@
+/*
+ * gpuocean_amd -- C ABI of the MI355X (gfx950) CDKLM16 step library.
+ *
+ * Plain C types only: device pointers come from the caller (torch tensors or
+ * hipMalloc), pitches are in BYTES as in the reference's PyCUDA calls, and
+ * `stream` is a hipStream_t passed as void*.  Every entry point returns 0 on
+ * success or a hipError_t / negative gpo error code; gpo_last_error() gives
+ * the message.  Nothing here allocates device memory unless stated.
+ *
+ * Reference interfaces replaced (paths relative to the reference root
+ * src/gpuocean/):
+ *   - the JIT-time #defines of CDKLM16_kernel.cu become gpo_cdklm16_params
+ *     fields (SWEsimulators/CDKLM16.py:242-269);
+ *   - each entry point names the PyCUDA prepared_async_call it replaces.
+ */
+#ifndef GPUOCEAN_AMD_H
+#define GPUOCEAN_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error codes beyond hipError_t. */
+#define GPO_ERR_BAD_ARGUMENT (-1)
+#define GPO_ERR_UNSUPPORTED (-2)
+
+/* Compile-time constants of the reference kernel, passed at run time.
+ * SWEsimulators/CDKLM16.py:242-269 (the `defines` dict). */
+typedef struct {
+  int32_t nx, ny;                 /* interior cells; arrays carry 2 ghost cells per side */
+  int32_t rk_order;               /* 1, 2 or 3 (RK_ORDER) */
+  int32_t one_dimensional;        /* ONE_DIMENSIONAL */
+  int32_t has_land;               /* Hi holds land_value corners (masked bathymetry) */
+  int32_t f_is_array;             /* coriolis f given per cell (see gpo_coriolis_cells) */
+  int32_t angle_is_zero;          /* north vector is exactly (0, 1) */
+  int32_t chunk_rows;             /* rows marched per wavefront (tuning; 0 = default) */
+  float dx, dy, g, f, r;          /* DX, DY, GRAV, constant f, FRIC */
+  float theta;                    /* THETA */
+  float desing_eps;               /* KPSIMULATOR_DESING_EPS */
+  float flux_slope_eps;           /* KPSIMULATOR_FLUX_SLOPE_EPS */
+  float depth_cutoff;             /* KPSIMULATOR_DEPTH_CUTOFF */
+  float flux_balancer;            /* FLUX_BALANCER */
+  float land_value;               /* Bathymetry.mask_value (1e20) */
+  float north_x, north_y;         /* sin(angle), cos(angle) for a constant angle */
+  float wind_stress_factor, rho_o;/* WIND_STRESS_FACTOR, RHO_O */
+} gpo_cdklm16_params;
+
+/* Optional wind stress / atmospheric pressure textures (time levels `cur`
+ * and `next`, dense row-major, blended with weight t).  NULL pointer = none.
+ * Replaces the wind_stress_* and atmospheric_pressure_* arguments of
+ * cdklm_swe_2D (gpu_kernels/CDKLM16_kernel.cu:502-511). */
+typedef struct {
+  const float *wind_x_cur, *wind_x_next, *wind_y_cur, *wind_y_next;
+  int32_t wind_x_nx, wind_x_ny, wind_y_nx, wind_y_ny;
+  float wind_t;
+  const float *pressure_cur, *pressure_next;
+  int32_t pressure_nx, pressure_ny;
+  float pressure_t;
+} gpo_forcing;
+
+/* One Runge-Kutta substep of CDKLM16 over the interior.
+ * Replaces cdklm_swe_2D (gpu_kernels/CDKLM16_kernel.cu:475-515) as launched by
+ * CDKLM16.callKernel (SWEsimulators/CDKLM16.py:688-723).
+ *   in_*   : stage input Q (the kernel's eta0/hu0/hv0), ghost cells included
+ *   qn_*   : Q^n, read at each interior cell when rk_step > 0 (the kernel
+ *            reads it from its output arrays); may alias out_*
+ *   out_*  : interior cells written
+ *   Hi     : bathymetry on corners, (ny+5) rows, pitch Hi_pitch bytes.  The
+ *            cell depth Hm/Bm is derived from Hi inside the kernel.
+ *   f_cells: per-cell coriolis f, same pitch as the state (f_is_array), else NULL
+ *   boundary_conditions: N<<24 | S<<16 | E<<8 | W (CDKLM16.py:695-699)
+ *   exact  : 0 = fast path (reciprocal-based division); 1 = IEEE division and
+ *            square root with no FMA contraction (bit-compatible with the
+ *            host oracle). */
+int gpo_cdklm16_substep(const gpo_cdklm16_params* p, float dt, int32_t rk_step,
+                        const float* in_eta, const float* in_hu, const float* in_hv,
+                        const float* qn_eta, const float* qn_hu, const float* qn_hv,
+                        float* out_eta, float* out_hu, float* out_hv, int32_t pitch,
+                        const float* Hi, int32_t Hi_pitch, const float* f_cells,
+                        const gpo_forcing* forcing, uint32_t boundary_conditions,
+                        int32_t exact, void* stream);
+
+/* Ghost-cell boundary conditions on a state triple.
+ * Replaces BoundaryConditionsArakawaA.boundaryCondition
+ * (utils/Common.py:1310-1330) with periodicBoundary_NS/_EW
+ * (gpu_kernels/boundary_kernels.cu:36-110) and flowRelaxationScheme_NS/_EW
+ * (boundary_kernels.cu:406-540).  Codes 1 (wall) write nothing, as in the
+ * reference.  frs_* may be NULL when no side is 3.
+ *   frs_ns_* : [2][frs_ns_nx][3] (south, north), frs_ew_* : [frs_ew_ny][2][3]. */
+int gpo_boundary_conditions(int32_t nx, int32_t ny, uint32_t boundary_conditions,
+                            const int32_t sponge_cells[4] /* n, e, s, w */,
+                            float* eta, float* hu, float* hv, int32_t pitch,
+                            const float* frs_ns_cur, const float* frs_ns_next, int32_t frs_ns_nx,
+                            const float* frs_ew_cur, const float* frs_ew_next, int32_t frs_ew_ny,
+                            float frs_t, void* stream);
+
+/* Bathymetry set-up: ghost corners of Hi (periodic or closed) then Bm.
+ * Replaces Bathymetry.__init__ (utils/Common.py:1414-1491) with
+ * periodic_/closed_boundary_intersections_* (boundary_kernels.cu:122-246) and
+ * initBm (gpu_kernels/initHm_kernel.cu:38-106).  Hi is modified in place. */
+int gpo_bathymetry_init(int32_t nx, int32_t ny, uint32_t boundary_conditions, float land_value,
+                        float* Hi, int32_t Hi_pitch, float* Bm, int32_t Bm_pitch, void* stream);
+
+/* Per-cell coriolis f from a texture, coriolisF evaluated at every cell
+ * (gpu_kernels/CDKLM16_kernel.cu:48-57, interpolation.cu:137-190). */
+int gpo_coriolis_cells(int32_t nx, int32_t ny, const float* f_tex, int32_t f_nx, int32_t f_ny,
+                       float* f_cells, int32_t pitch, void* stream);
+
+/* CFL time step: writes min over wet interior cells of
+ * 0.25*min(dx/|u+-c|, dy/|v+-c|) (1e5 if none) to *dt_out (device pointer).
+ * Replaces per_block_max_dt + max_dt_reduction (gpu_kernels/max_dt.cu:42-195)
+ * called from CDKLM16.updateDt (SWEsimulators/CDKLM16.py:814-841). */
+int gpo_max_dt(int32_t nx, int32_t ny, float dx, float dy, float g, float land_value,
+               const float* eta, const float* hu, const float* hv, int32_t pitch,
+               const float* Bm, int32_t Bm_pitch, float* dt_out, void* stream);
+
+/* Text of the last error on this thread. */
+const char* gpo_last_error(void);
+
+/* Library version string. */
+const char* gpo_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GPUOCEAN_AMD_H */

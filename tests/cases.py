@@ -1,0 +1,178 @@
+"""Initial conditions and configurations used by the tests.
+
+The 50x70 regression cases restate the reference's test set-up
+(test/schemes/CDKLM16_test.py:37-419, ICs from test/testUtils.py:99-142):
+dx=dy=200, H=60 (or the analytic bathymetry), dt=0.9, step(50.0).
+"""
+
+import numpy as np
+
+from gpuocean_amd import Common
+
+
+def _bump(eta, nx, ny, dx, dy, halo, x_center, y_center):
+    """exp(-(x^2+y^2)/size) inside radius size, size = 500*min(dx, dy)
+    (test/testUtils.py:111-142), added over ghost cells too."""
+    size = 500.0 * min(dx, dy)
+    for j in range(-halo[2], ny + halo[0]):
+        for i in range(-halo[3], nx + halo[1]):
+            x = dx * i - x_center
+            y = dy * j - y_center
+            if np.sqrt(x ** 2 + y ** 2) < size:
+                eta[j + halo[2], i + halo[3]] += np.exp(-(x ** 2 / size + y ** 2 / size))
+
+
+def add_central_bump(eta, nx, ny, dx, dy, halo):
+    _bump(eta, nx, ny, dx, dy, halo, dx * nx / 2.0, dy * ny / 2.0)
+
+
+def add_corner_bump(eta, nx, ny, dx, dy, halo):
+    _bump(eta, nx, ny, dx, dy, halo, 4 * dx, 4 * dy)
+
+
+def add_upper_corner_bump(eta, nx, ny, dx, dy, halo):
+    _bump(eta, nx, ny, dx, dy, halo, (nx - 4) * dx, (ny - 4) * dy)
+
+
+def make_bottom_topography(Hi, nx, ny, dx, dy, halo):
+    """Analytic corner bathymetry (test/testUtils.py:99-108)."""
+    for j in range(-halo[2], ny + halo[0] + 1):
+        for i in range(-halo[3], nx + halo[1] + 1):
+            Hi[j + halo[2], i + halo[3]] = 6 + 2.0 * np.cos(0.3 * (i + i / (np.sin(0.5 * j) + 2.5))) + \
+                2.0 * np.sin(2 * np.pi * (j + i) / (2.0 * ny))
+
+
+BC_SETTINGS = {
+    "wall": (1, 1, 1, 1),
+    "periodic": (2, 2, 2, 2),
+    "periodicNS": (2, 1, 2, 1),
+    "periodicEW": (1, 2, 1, 2),
+}
+
+# (test name, bc setting, bump, golden key prefix, f, beta, bathymetry)
+REGRESSION_CASES = [
+    ("wall_central", "wall", "central", "wallBC_central", 0.0, 0.0, False),
+    ("wall_corner", "wall", "corner", "wallBC_corner", 0.0, 0.0, False),
+    ("wall_upperCorner", "wall", "upperCorner", "wallBC_upperCorner", 0.0, 0.0, False),
+    ("periodic_central", "periodic", "central", "wallBC_central", 0.0, 0.0, False),
+    ("periodic_corner", "periodic", "corner", "periodic_corner", 0.0, 0.0, False),
+    ("periodic_upperCorner", "periodic", "upperCorner", "periodic_upperCorner", 0.0, 0.0, False),
+    ("periodicNS_central", "periodicNS", "central", "wallBC_central", 0.0, 0.0, False),
+    ("periodicNS_corner", "periodicNS", "corner", "periodicNS_corner", 0.0, 0.0, False),
+    ("periodicNS_upperCorner", "periodicNS", "upperCorner", "periodicNS_upperCorner", 0.0, 0.0, False),
+    ("periodicEW_central", "periodicEW", "central", "wallBC_central", 0.0, 0.0, False),
+    ("periodicEW_corner", "periodicEW", "corner", "periodicEW_corner", 0.0, 0.0, False),
+    ("periodicEW_upperCorner", "periodicEW", "upperCorner", "periodicEW_upperCorner", 0.0, 0.0, False),
+    ("coriolis_central", "wall", "central", "coriolis_central", 0.01, 0.0, False),
+    ("betamodel_central", "wall", "central", "betamodel_central", 0.01 + 2 * 1e-6 * 200.0, 1e-6, False),
+    ("bathymetry_central", "wall", "central", "wallBC_central_bathymetry", 0.0, 0.0, True),
+]
+
+
+def regression_setup(case):
+    """Constructor kwargs of one 50x70 regression case."""
+    name, bcs, bump, key, f, beta, bathy = case
+    nx, ny, dx, dy = 50, 70, 200.0, 200.0
+    ghosts = [2, 2, 2, 2]
+    shape = (ny + 4, nx + 4)
+    eta0 = np.zeros(shape, dtype=np.float32)
+    hu0 = np.zeros(shape, dtype=np.float32)
+    hv0 = np.zeros(shape, dtype=np.float32)
+    Hi = np.ones((shape[0] + 1, shape[1] + 1), dtype=np.float32) * 60
+    {"central": add_central_bump, "corner": add_corner_bump,
+     "upperCorner": add_upper_corner_bump}[bump](eta0, nx, ny, dx, dy, ghosts)
+    if bathy:
+        make_bottom_topography(Hi, nx, ny, dx, dy, ghosts)
+    kw = dict(eta0=eta0, hu0=hu0, hv0=hv0, H=Hi, nx=nx, ny=ny, dx=dx, dy=dy, dt=0.9, g=9.81, f=f, r=0.0,
+              boundary_conditions=Common.BoundaryConditions(*BC_SETTINGS[bcs]))
+    if beta != 0.0:
+        kw["coriolis_beta"] = beta
+    return kw, key
+
+
+def golden(key):
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "timestep50.npz")
+    with np.load(path) as z:
+        return z[key + "_eta"], z[key + "_u"], z[key + "_v"]
+
+
+def check_results_metric(sim_arrays, ref_arrays):
+    """The reference's checkResults metric (CDKLM16_test.py:112-143): signed
+    max(sim-ref)/max|ref| over the interior, and the L2 variant."""
+    out = []
+    for a, b in zip(sim_arrays, ref_arrays):
+        a, b = a[2:-2, 2:-2], b[2:-2, 2:-2]
+        scale = np.max(np.abs(b))
+        out.append((float(np.max(a - b) / scale), float(np.linalg.norm(a - b) / scale)))
+    return out
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    nb = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / nb) if nb > 0 else float(np.linalg.norm(a - b))
+
+
+# ------------------------------------------------------------------ benchmark configurations
+
+def config_c0(n=256, bump_amp=0.5, sigma=10.0):
+    """256x256, dx=dy=200, H=60 on corners, walls, g=9.81, f=1.2e-4, r=0;
+    Gaussian bump eta = 0.5 exp(-r^2/(2*10^2)) cell units, centred at the
+    domain centre (cell-centre convention: interior cell (i, j) at i+0.5)."""
+    nx = ny = n
+    shape = (ny + 4, nx + 4)
+    jj, ii = np.meshgrid(np.arange(ny + 4) - 2 + 0.5, np.arange(nx + 4) - 2 + 0.5, indexing="ij")
+    eta0 = (bump_amp * np.exp(-((ii - nx / 2) ** 2 + (jj - ny / 2) ** 2) / (2 * sigma ** 2))).astype(np.float32)
+    hu0 = np.zeros(shape, np.float32)
+    hv0 = np.zeros(shape, np.float32)
+    H = np.full((ny + 5, nx + 5), 60.0, np.float32)
+    return dict(eta0=eta0, hu0=hu0, hv0=hv0, H=H, nx=nx, ny=ny, dx=200.0, dy=200.0, dt=0.0, g=9.81,
+                f=1.2e-4, r=0.0, courant_number=0.7, boundary_conditions=Common.BoundaryConditions(1, 1, 1, 1))
+
+
+def smooth_random_field(shape, sigma, seed, wrap_x=True):
+    """Gaussian-filtered N(0,1) field (sigma in cells): periodic in x, reflecting in y."""
+    rng = np.random.default_rng(seed)
+    z = rng.standard_normal(shape)
+    ny, nx = shape
+
+    def kernel(n, s):
+        k = np.arange(-int(4 * s), int(4 * s) + 1)
+        w = np.exp(-0.5 * (k / s) ** 2)
+        return k, w / w.sum()
+
+    k, w = kernel(nx, sigma)
+    out = np.zeros_like(z)
+    for kk, ww in zip(k, w):
+        out += ww * np.roll(z, kk, axis=1)
+    z = out
+    k, w = kernel(ny, sigma)
+    pad = len(k) // 2
+    zp = np.pad(z, ((pad, pad), (0, 0)), mode="symmetric")
+    out = np.zeros_like(z)
+    for idx, ww in enumerate(w):
+        out += ww * zp[idx:idx + ny, :]
+    return out
+
+
+def config_c1(nx=2048, ny=1024, seed=1):
+    """Double jet: 2048x1024, dx=dy=1 km, periodic in x, walls in y, f=1e-4.
+    H = 250 + 30*p/max|p| on corners; eta = 0.3(tanh((y-y1)/50km) - tanh((y-y2)/50km)),
+    y1 = Ly/4, y2 = 3Ly/4; hu from geostrophic balance, hv = 0."""
+    dx = dy = 1000.0
+    g, f = 9.81, 1e-4
+    p = smooth_random_field((ny + 5, nx + 5), 20.0, seed)
+    H = (250.0 + 30.0 * p / np.max(np.abs(p))).astype(np.float32)
+    y = (np.arange(ny + 4) - 2 + 0.5) * dy
+    Ly = ny * dy
+    y1, y2 = Ly / 4, 3 * Ly / 4
+    eta_y = 0.3 * (np.tanh((y - y1) / 5e4) - np.tanh((y - y2) / 5e4))
+    eta0 = np.repeat(eta_y[:, None], nx + 4, axis=1).astype(np.float32)
+    Hm = 0.25 * (H[:-1, :-1] + H[1:, :-1] + H[:-1, 1:] + H[1:, 1:])
+    deta_dy = np.gradient(eta_y, dy)
+    hu0 = (-(g / f) * (Hm + eta0) * deta_dy[:, None]).astype(np.float32)
+    hv0 = np.zeros_like(eta0)
+    return dict(eta0=eta0, hu0=hu0, hv0=hv0, H=H, nx=nx, ny=ny, dx=dx, dy=dy, dt=0.0, g=g, f=f, r=0.0,
+                courant_number=0.7, boundary_conditions=Common.BoundaryConditions(1, 2, 1, 2))
