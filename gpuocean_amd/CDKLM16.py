@@ -376,6 +376,9 @@ class CDKLM16:
             self._substep(self.h0, self.h1, None, local_dt, 0, forcing)
             if explicit_bc:
                 self._apply_bc(self.h1)
+            with torch.cuda.stream(self.gpu_stream):
+                for a, b in zip(self.h_stage, self.h1):
+                    a.copy_from(b)  # ghost cells of the stage follow h1
             self._substep(self.h1, self.h_stage, self.h0, local_dt, 1, forcing)
             self.h1, self.h_stage = self.h_stage, self.h1
             if explicit_bc:
@@ -383,11 +386,54 @@ class CDKLM16:
             self._substep(self.h1, self.h0, self.h0, local_dt, 2, forcing)
 
     # ------------------------------------------------------------- public API
+    def _batched_eligible(self, update_dt):
+        bc = self.boundary_conditions
+        return (self.rk_order == 2 and not update_dt and self.model_error is None and not self.hasDrifters
+                and not (self._has_wind or self._has_pressure)
+                and all(int(c) in (1, 2) for c in (bc.north, bc.east, bc.south, bc.west)))
+
+    def _rk2_steps(self, n, dt):
+        p = self.params
+        h0, h1 = self.h0, self.h1
+        L.check(L.lib().gpo_cdklm16_rk2_steps(
+            ctypes.byref(p), int(n), float(dt),
+            ctypes.c_void_p(h0[0].data_ptr()), ctypes.c_void_p(h0[1].data_ptr()), ctypes.c_void_p(h0[2].data_ptr()),
+            ctypes.c_void_p(h1[0].data_ptr()), ctypes.c_void_p(h1[1].data_ptr()), ctypes.c_void_p(h1[2].data_ptr()),
+            h0[0].pitch, ctypes.c_void_p(self.Hi.data_ptr()), self.Hi.pitch,
+            None if self.f_cells is None else ctypes.c_void_p(self.f_cells.data_ptr()),
+            self.boundary_conditions.packed(), int(self.exact_arithmetic), self._stream_handle()),
+            "gpo_cdklm16_rk2_steps")
+
     def step(self, t_end=0.0, apply_stochastic_term=True, write_now=True, update_dt=False):
-        """Advance by t_end seconds of model time (CDKLM16.py:477-584)."""
+        """Advance by t_end seconds of model time (CDKLM16.py:477-584).
+
+        The host keeps the reference's time bookkeeping (local_dt =
+        f32(min(dt, f32(t_end - t_now))), t accumulated in float64).  Runs of
+        equal local_dt with nothing but wall/periodic boundaries between steps
+        are launched back to back by one library call."""
         if self.t == 0:
             self._update_bc_values()
             self._apply_bc(self.h0)
+        if self._batched_eligible(update_dt):
+            dts = []
+            t_now = 0.0
+            while t_now < t_end:
+                local_dt = np.float32(min(self.dt, np.float32(t_end - t_now)))
+                dts.append(local_dt)
+                t_now += np.float64(local_dt)
+            i = 0
+            while i < len(dts):
+                j = i
+                while j < len(dts) and dts[j] == dts[i]:
+                    j += 1
+                self._rk2_steps(j - i, dts[i])
+                i = j
+            for local_dt in dts:
+                self.t += np.float64(local_dt)
+                self.num_iterations += 1
+            self._apply_bc(self.h0)
+            return self.t
+
         t_now = 0.0
         ghosts_read_each_step = (self.model_error is not None) or self.hasDrifters
         while t_now < t_end:

@@ -1,0 +1,11 @@
+"""gpuocean_amd: MI355X-native CDKLM16 rotating shallow-water step.
+
+Drop-in for the reference's CDKLM16 path (SWEsimulators/CDKLM16.py,
+ensembles/OceanModelEnsemble.py) built on hand-written gfx950 HIP kernels.
+"""
+
+from . import Common  # noqa: F401
+from .CDKLM16 import CDKLM16  # noqa: F401
+from .forcing import AtmosphericPressure, WindStress  # noqa: F401
+
+__all__ = ["CDKLM16", "Common", "WindStress", "AtmosphericPressure"]

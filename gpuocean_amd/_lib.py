@@ -51,6 +51,9 @@ _SIGNATURES = [
      [ctypes.POINTER(CDKLM16Params), c_float, c_int32,
       c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int32,
       c_void_p, c_int32, c_void_p, ctypes.POINTER(Forcing), c_uint32, c_int32, c_void_p]),
+    ("gpo_cdklm16_rk2_steps", c_int32,
+     [ctypes.POINTER(CDKLM16Params), c_int32, c_float, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+      c_int32, c_void_p, c_int32, c_void_p, c_uint32, c_int32, c_void_p]),
     ("gpo_boundary_conditions", c_int32,
      [c_int32, c_int32, c_uint32, ctypes.POINTER(c_int32), c_void_p, c_void_p, c_void_p, c_int32,
       c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_int32, c_float, c_void_p]),

@@ -256,25 +256,26 @@ const char* gpo_last_error(void) { return g_last_error.c_str(); }
 
 const char* gpo_version(void) { return "gpuocean_amd 0.1 (gfx950)"; }
 
-int gpo_cdklm16_substep(const gpo_cdklm16_params* p, float dt, int32_t rk_step, const float* in_eta,
-                        const float* in_hu, const float* in_hv, const float* qn_eta, const float* qn_hu,
-                        const float* qn_hv, float* out_eta, float* out_hu, float* out_hv, int32_t pitch,
-                        const float* Hi, int32_t Hi_pitch, const float* f_cells, const gpo_forcing* forcing,
-                        uint32_t boundary_conditions, int32_t exact, void* stream) {
-  if (!p) return bad("gpo_cdklm16_substep: params is NULL");
-  if (p->nx < 1 || p->ny < 1) return bad("gpo_cdklm16_substep: nx and ny must be positive");
+static int make_step_args(const gpo_cdklm16_params* p, float dt, int32_t rk_step, const float* in_eta,
+                          const float* in_hu, const float* in_hv, const float* qn_eta, const float* qn_hu,
+                          const float* qn_hv, float* out_eta, float* out_hu, float* out_hv, int32_t pitch,
+                          const float* Hi, int32_t Hi_pitch, const float* f_cells, const gpo_forcing* forcing,
+                          uint32_t boundary_conditions, StepArgs* out) {
+  if (!p) return bad("gpo_cdklm16: params is NULL");
+  if (p->nx < 1 || p->ny < 1) return bad("gpo_cdklm16: nx and ny must be positive");
   if (!in_eta || !in_hu || !in_hv || !out_eta || !out_hu || !out_hv || !Hi)
-    return bad("gpo_cdklm16_substep: NULL state or bathymetry pointer");
-  if (pitch % 4 || Hi_pitch % 4) return bad("gpo_cdklm16_substep: pitches must be multiples of 4 bytes");
-  if (pitch < 4 * (p->nx + 4) || Hi_pitch < 4 * (p->nx + 5))
-    return bad("gpo_cdklm16_substep: pitch smaller than the row");
-  if (p->rk_order < 1 || p->rk_order > 3) return bad("gpo_cdklm16_substep: rk_order must be 1, 2 or 3");
-  if (rk_step < 0 || rk_step >= p->rk_order) return bad("gpo_cdklm16_substep: rk_step out of range");
-  if (rk_step > 0 && (!qn_eta || !qn_hu || !qn_hv)) return bad("gpo_cdklm16_substep: Q^n pointers needed");
-  if (p->f_is_array && !f_cells) return bad("gpo_cdklm16_substep: f_is_array set but f_cells is NULL");
-  if (p->rk_order == 3 && p->r != 0.0f) return bad("gpo_cdklm16_substep: RK3 requires r == 0");
+    return bad("gpo_cdklm16: NULL state or bathymetry pointer");
+  if (pitch % 4 || Hi_pitch % 4) return bad("gpo_cdklm16: pitches must be multiples of 4 bytes");
+  if (pitch < 4 * (p->nx + 4) || Hi_pitch < 4 * (p->nx + 5)) return bad("gpo_cdklm16: pitch smaller than the row");
+  if (p->rk_order < 1 || p->rk_order > 3) return bad("gpo_cdklm16: rk_order must be 1, 2 or 3");
+  if (rk_step < 0 || rk_step >= p->rk_order) return bad("gpo_cdklm16: rk_step out of range");
+  if (rk_step > 0 && (!qn_eta || !qn_hu || !qn_hv)) return bad("gpo_cdklm16: Q^n pointers needed");
+  if (p->f_is_array && !f_cells) return bad("gpo_cdklm16: f_is_array set but f_cells is NULL");
+  if (p->rk_order == 3 && p->r != 0.0f) return bad("gpo_cdklm16: RK3 requires r == 0");
+  if (p->chunk_rows < 0 || p->chunk_rows > 4096) return bad("gpo_cdklm16: chunk_rows out of range");
 
-  StepArgs a{};
+  StepArgs& a = *out;
+  a = StepArgs{};
   a.nx = p->nx;
   a.ny = p->ny;
   a.pitch = pitch / 4;
@@ -333,10 +334,50 @@ int gpo_cdklm16_substep(const gpo_cdklm16_params* p, float dt, int32_t rk_step, 
     a.has_forcing = 1;
     a.forcing = *forcing;
   }
+  return 0;
+}
+
+static hipError_t launch_step(const gpo_cdklm16_params* p, const StepArgs& a, int exact, hipStream_t s) {
+  return exact ? launch_cdklm_exact(a, p->has_land, p->f_is_array, p->angle_is_zero, s)
+               : launch_cdklm_fast(a, p->has_land, p->f_is_array, p->angle_is_zero, s);
+}
+
+int gpo_cdklm16_substep(const gpo_cdklm16_params* p, float dt, int32_t rk_step, const float* in_eta,
+                        const float* in_hu, const float* in_hv, const float* qn_eta, const float* qn_hu,
+                        const float* qn_hv, float* out_eta, float* out_hu, float* out_hv, int32_t pitch,
+                        const float* Hi, int32_t Hi_pitch, const float* f_cells, const gpo_forcing* forcing,
+                        uint32_t boundary_conditions, int32_t exact, void* stream) {
+  StepArgs a;
+  const int rc = make_step_args(p, dt, rk_step, in_eta, in_hu, in_hv, qn_eta, qn_hu, qn_hv, out_eta, out_hu,
+                                out_hv, pitch, Hi, Hi_pitch, f_cells, forcing, boundary_conditions, &a);
+  if (rc) return rc;
+  return check(launch_step(p, a, exact, (hipStream_t)stream), "gpo_cdklm16_substep launch");
+}
+
+int gpo_cdklm16_rk2_steps(const gpo_cdklm16_params* p, int32_t n_steps, float dt, float* h0_eta, float* h0_hu,
+                          float* h0_hv, float* h1_eta, float* h1_hu, float* h1_hv, int32_t pitch, const float* Hi,
+                          int32_t Hi_pitch, const float* f_cells, uint32_t boundary_conditions, int32_t exact,
+                          void* stream) {
+  if (!p || p->rk_order != 2) return bad("gpo_cdklm16_rk2_steps: rk_order must be 2");
+  if (n_steps < 0) return bad("gpo_cdklm16_rk2_steps: negative step count");
+  for (int side = 0; side < 4; ++side) {
+    const int code = (boundary_conditions >> (8 * side)) & 0xFF;
+    if (code != 1 && code != 2) return bad("gpo_cdklm16_rk2_steps: only wall (1) and periodic (2) boundaries");
+  }
+  StepArgs a0, a1;
+  int rc = make_step_args(p, dt, 0, h0_eta, h0_hu, h0_hv, nullptr, nullptr, nullptr, h1_eta, h1_hu, h1_hv, pitch,
+                          Hi, Hi_pitch, f_cells, nullptr, boundary_conditions, &a0);
+  if (rc) return rc;
+  rc = make_step_args(p, dt, 1, h1_eta, h1_hu, h1_hv, h0_eta, h0_hu, h0_hv, h0_eta, h0_hu, h0_hv, pitch, Hi,
+                      Hi_pitch, f_cells, nullptr, boundary_conditions, &a1);
+  if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
-  const hipError_t e = exact ? launch_cdklm_exact(a, p->has_land, p->f_is_array, p->angle_is_zero, s)
-                             : launch_cdklm_fast(a, p->has_land, p->f_is_array, p->angle_is_zero, s);
-  return check(e, "gpo_cdklm16_substep launch");
+  for (int i = 0; i < n_steps; ++i) {
+    hipError_t e = launch_step(p, a0, exact, s);
+    if (e == hipSuccess) e = launch_step(p, a1, exact, s);
+    if (e != hipSuccess) return check(e, "gpo_cdklm16_rk2_steps launch");
+  }
+  return 0;
 }
 
 int gpo_boundary_conditions(int32_t nx, int32_t ny, uint32_t bc, const int32_t sponge[4], float* eta, float* hu,

@@ -82,6 +82,18 @@ int gpo_cdklm16_substep(const gpo_cdklm16_params* p, float dt, int32_t rk_step,
                         const gpo_forcing* forcing, uint32_t boundary_conditions,
                         int32_t exact, void* stream);
 
+/* n_steps full RK2 steps h0 -> h1 -> h0 with a fixed dt, launched back to back
+ * (two substep launches per step) without host round trips.  Only wall and
+ * periodic boundaries, which the substep kernel resolves while loading; the
+ * global ghost cells are not written (call gpo_boundary_conditions after).
+ * Replaces the per-step host loop of CDKLM16.step (SWEsimulators/CDKLM16.py:493-579)
+ * for steps with no noise, drifters, forcing updates or FRS boundaries. */
+int gpo_cdklm16_rk2_steps(const gpo_cdklm16_params* p, int32_t n_steps, float dt,
+                          float* h0_eta, float* h0_hu, float* h0_hv,
+                          float* h1_eta, float* h1_hu, float* h1_hv, int32_t pitch,
+                          const float* Hi, int32_t Hi_pitch, const float* f_cells,
+                          uint32_t boundary_conditions, int32_t exact, void* stream);
+
 /* Ghost-cell boundary conditions on a state triple.
  * Replaces BoundaryConditionsArakawaA.boundaryCondition
  * (utils/Common.py:1310-1330) with periodicBoundary_NS/_EW

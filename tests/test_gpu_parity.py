@@ -1,0 +1,150 @@
+"""HIP CDKLM16 step against the CPU oracle (and the reference's golden vectors).
+
+Bars: the exact build (IEEE division/sqrt, no FMA contraction) equals the FP32
+oracle value for value; the fast build is within relative L2 1e-5 per field.
+"""
+import numpy as np
+import pytest
+
+import cases
+from oracle.oracle import OracleCDKLM16
+
+pytestmark = pytest.mark.gpu
+
+TOL_FAST = 1e-5  # relative L2 per field against the FP32 oracle
+
+
+def _run_pair(gpu_ctx, kw, t_end, exact):
+    from gpuocean_amd import CDKLM16
+    sim = CDKLM16(gpu_ctx, **kw, exact_arithmetic=exact)
+    ora = OracleCDKLM16(**kw)
+    sim.step(t_end)
+    ora.step(t_end)
+    return sim, ora
+
+
+@pytest.mark.parametrize("case", cases.REGRESSION_CASES, ids=[c[0] for c in cases.REGRESSION_CASES])
+def test_exact_build_equals_oracle(gpu_ctx, case):
+    kw, key = cases.regression_setup(case)
+    print(f"case {case[0]} exact", flush=True)
+    sim, ora = _run_pair(gpu_ctx, kw, 50.0, exact=True)
+    for name, a, b in zip(("eta", "hu", "hv"), sim.download(), ora.download()):
+        diff = np.argwhere(a != b)
+        assert diff.size == 0, f"{name}: {len(diff)} cells differ, first {diff[:5].tolist()}"
+
+
+@pytest.mark.parametrize("case", cases.REGRESSION_CASES, ids=[c[0] for c in cases.REGRESSION_CASES])
+def test_fast_build_close_to_oracle_and_golden(gpu_ctx, case):
+    kw, key = cases.regression_setup(case)
+    print(f"case {case[0]} fast", flush=True)
+    sim, ora = _run_pair(gpu_ctx, kw, 50.0, exact=False)
+    got = sim.download()
+    for a, b in zip(got, ora.download()):
+        assert cases.rel_l2(a[2:-2, 2:-2], b[2:-2, 2:-2]) <= TOL_FAST
+    if case[0] != "betamodel_central":
+        for max_rel, _ in cases.check_results_metric(got, cases.golden(key)):
+            assert round(max_rel, 3) == 0.0
+
+
+def test_golden_betamodel_full_texture(gpu_ctx):
+    case = [c for c in cases.REGRESSION_CASES if c[0] == "betamodel_central"][0]
+    kw, key = cases.regression_setup(case)
+    kw["subsample_f"] = 0
+    from gpuocean_amd import CDKLM16
+    sim = CDKLM16(gpu_ctx, **kw)
+    sim.step(50.0)
+    for max_rel, _ in cases.check_results_metric(sim.download(), cases.golden(key)):
+        assert round(max_rel, 3) == 0.0
+
+
+@pytest.mark.parametrize("exact", [True, False])
+def test_c0_1000_steps(gpu_ctx, exact):
+    """256x256 Gaussian bump, walls, f=1.2e-4, 1000 steps at dt = 0.7 CFL."""
+    from gpuocean_amd import CDKLM16
+    kw = cases.config_c0()
+    print(f"C0 1000 steps exact={exact}", flush=True)
+    sim = CDKLM16(gpu_ctx, **kw, exact_arithmetic=exact)
+    ora = OracleCDKLM16(**kw)
+    assert np.float32(sim.dt) == np.float32(ora.dt)
+    t_end = 1000 * np.float64(np.float32(ora.dt))
+    sim.step(t_end)
+    ora.step(t_end)
+    f64 = OracleCDKLM16(**kw, precision="f64")
+    f64.dt = np.float32(ora.dt)
+    f64.step(t_end)
+    for name, a, b, c in zip(("eta", "hu", "hv"), sim.download(True), ora.download(True), f64.download(True)):
+        e32 = cases.rel_l2(a, b)
+        e64 = cases.rel_l2(a, c)
+        print(f"C0 {name}: rel L2 vs FP32 oracle {e32:.3e}, vs FP64 oracle {e64:.3e}", flush=True)
+        if exact:
+            assert np.array_equal(a, b)
+        else:
+            assert e32 <= TOL_FAST
+
+
+@pytest.mark.parametrize("bathy", [False, True])
+def test_lake_at_rest(gpu_ctx, bathy):
+    from gpuocean_amd import CDKLM16
+    nx, ny = 130, 70
+    H = np.full((ny + 5, nx + 5), 60.0, np.float32)
+    if bathy:
+        cases.make_bottom_topography(H, nx, ny, 200.0, 200.0, [2, 2, 2, 2])
+        H += 20.0
+    z = np.zeros((ny + 4, nx + 4), np.float32)
+    sim = CDKLM16(gpu_ctx, z, z, z, H, nx, ny, 200.0, 200.0, 1.0, 9.81, 1.2e-4, 0.0, exact_arithmetic=True)
+    ora = OracleCDKLM16(z, z, z, H, nx, ny, 200.0, 200.0, 1.0, 9.81, 1.2e-4, 0.0)
+    sim.step(30.0)
+    ora.step(30.0)
+    for a, b in zip(sim.download(), ora.download()):
+        assert np.array_equal(a, b)
+        if not bathy:
+            assert np.all(a == 0.0)
+
+
+def test_max_dt_matches_oracle(gpu_ctx):
+    from gpuocean_amd import CDKLM16
+    kw = cases.config_c0(n=200)
+    rng = np.random.default_rng(5)
+    kw["hu0"] = (rng.standard_normal(kw["hu0"].shape) * 3).astype(np.float32)
+    kw["hv0"] = (rng.standard_normal(kw["hv0"].shape) * 3).astype(np.float32)
+    sim = CDKLM16(gpu_ctx, **kw)
+    ora = OracleCDKLM16(**kw)
+    assert np.float32(sim.dt) == np.float32(ora.dt)
+    sim.updateDt()
+    assert sim.dt == sim._getMaxTimestepHost(courant_number=sim.courant_number)
+
+
+@pytest.mark.parametrize("bcs", [(1, 2, 1, 2), (2, 1, 2, 1), (2, 2, 2, 2), (1, 1, 1, 1)])
+def test_nonsquare_sizes_and_bcs(gpu_ctx, bcs):
+    """Sizes that leave partial strips and chunks (nx not a multiple of 60,
+    ny not a multiple of the chunk height) against the oracle, exact build."""
+    from gpuocean_amd import CDKLM16, Common
+    nx, ny = 157, 83
+    rng = np.random.default_rng(11)
+    eta = np.zeros((ny + 4, nx + 4), np.float32)
+    cases.add_corner_bump(eta, nx, ny, 200.0, 200.0, [2, 2, 2, 2])
+    hu = (rng.standard_normal(eta.shape) * 0.5).astype(np.float32)
+    hv = (rng.standard_normal(eta.shape) * 0.5).astype(np.float32)
+    H = (50.0 + 10.0 * rng.random((ny + 5, nx + 5))).astype(np.float32)
+    kw = dict(eta0=eta, hu0=hu, hv0=hv, H=H, nx=nx, ny=ny, dx=200.0, dy=200.0, dt=0.8, g=9.81, f=1.2e-4,
+              r=0.0, boundary_conditions=Common.BoundaryConditions(*bcs))
+    for chunk in (7, 16, 40):
+        sim = CDKLM16(gpu_ctx, **kw, exact_arithmetic=True, chunk_rows=chunk)
+        ora = OracleCDKLM16(**kw)
+        sim.step(12.0)
+        ora.step(12.0)
+        for a, b in zip(sim.download(), ora.download()):
+            assert np.array_equal(a, b), f"chunk {chunk}"
+
+
+def test_friction_and_rk1_rk3(gpu_ctx):
+    from gpuocean_amd import CDKLM16
+    kw, _ = cases.regression_setup(cases.REGRESSION_CASES[12])
+    for rk, r in ((1, 0.0), (2, 2.4e-3), (3, 0.0)):
+        kw2 = dict(kw, rk_order=rk, r=r)
+        sim = CDKLM16(gpu_ctx, **kw2, exact_arithmetic=True)
+        ora = OracleCDKLM16(**kw2)
+        sim.step(20.0)
+        ora.step(20.0)
+        for a, b in zip(sim.download(), ora.download()):
+            assert np.array_equal(a, b), f"rk {rk} r {r}"
