@@ -96,7 +96,7 @@ class CDKLM16:
                  flux_balancer=0.8,
                  block_width=12, block_height=32, num_threads_dt=256,
                  use_direct_lookup=False,
-                 exact_arithmetic=False,
+                 exact_arithmetic=True,
                  chunk_rows=None):
         self.logger = logging.getLogger(__name__)
         assert 0 < rk_order < 4, "Only 1st, 2nd and 3rd order Runge Kutta supported"
@@ -240,6 +240,8 @@ class CDKLM16:
         p.north_x, p.north_y = float(north_x), float(north_y)
         p.wind_stress_factor = host.kernel_constant(wind_stress_factor)
         p.rho_o = host.kernel_constant(rho_o)
+        p.divc_ok = 0
+        L.check(L.lib().gpo_cdklm16_prepare(ctypes.byref(p), self._stream_handle()), "gpo_cdklm16_prepare")
 
         self._init_forcing()
         self._bc_times = Common.BCTimeline(self.bc_data)

@@ -7,5 +7,7 @@ ensembles/OceanModelEnsemble.py) built on hand-written gfx950 HIP kernels.
 from . import Common  # noqa: F401
 from .CDKLM16 import CDKLM16  # noqa: F401
 from .forcing import AtmosphericPressure, WindStress  # noqa: F401
+from .GPUDrifterCollection import GPUDrifterCollection  # noqa: F401
+from .OceanStateNoise import OceanStateNoise  # noqa: F401
 
-__all__ = ["CDKLM16", "Common", "WindStress", "AtmosphericPressure"]
+__all__ = ["CDKLM16", "Common", "WindStress", "AtmosphericPressure", "GPUDrifterCollection", "OceanStateNoise"]

@@ -26,6 +26,7 @@ class CDKLM16Params(ctypes.Structure):
         ("theta", c_float), ("desing_eps", c_float), ("flux_slope_eps", c_float),
         ("depth_cutoff", c_float), ("flux_balancer", c_float), ("land_value", c_float),
         ("north_x", c_float), ("north_y", c_float), ("wind_stress_factor", c_float), ("rho_o", c_float),
+        ("divc_ok", c_int32),
     ]
 
 
@@ -47,6 +48,7 @@ _lib = None
 
 # (name, restype, argtypes)
 _SIGNATURES = [
+    ("gpo_cdklm16_prepare", c_int32, [ctypes.POINTER(CDKLM16Params), c_void_p]),
     ("gpo_cdklm16_substep", c_int32,
      [ctypes.POINTER(CDKLM16Params), c_float, c_int32,
       c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int32,
@@ -64,6 +66,18 @@ _SIGNATURES = [
     ("gpo_max_dt", c_int32,
      [c_int32, c_int32, c_float, c_float, c_float, c_float, c_void_p, c_void_p, c_void_p, c_int32,
       c_void_p, c_int32, c_void_p, c_void_p]),
+    ("gpo_philox4x32_10", c_int32, [c_void_p, c_void_p, c_void_p, c_int32, c_void_p]),
+    ("gpo_random_normal", c_int32, [c_void_p, c_int32, c_int32, c_int32, ctypes.c_uint64, c_uint32, c_uint32,
+                                    c_void_p]),
+    ("gpo_soar", c_int32, [c_int32, c_int32, c_void_p, c_float, c_int32, c_int32, c_void_p, c_int32, c_void_p,
+                           c_int32, c_int32, c_void_p]),
+    ("gpo_geostrophic_balance", c_int32,
+     [c_int32, c_int32, c_float, c_float, c_float, c_float, c_float, c_float, c_float, c_void_p, c_int32,
+      c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_int32, c_float, c_void_p]),
+    ("gpo_drift", c_int32,
+     [c_int32, c_int32, c_float, c_float, c_float, c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_int32,
+      c_void_p, c_int32, c_int32, c_int32, c_int32, c_void_p, c_int32, c_float, c_void_p, c_void_p, c_void_p,
+      c_void_p, c_int32, c_int32, c_float, c_float, c_void_p]),
     ("gpo_last_error", ctypes.c_char_p, []),
     ("gpo_version", ctypes.c_char_p, []),
 ]

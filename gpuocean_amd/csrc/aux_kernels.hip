@@ -3,7 +3,9 @@
 // -ffp-contract=off so their float arithmetic matches the host oracle.
 
 #include <cstdio>
+#include <mutex>
 #include <string>
+#include <unordered_map>
 
 #include "cdklm_common.h"
 
@@ -198,6 +200,26 @@ __global__ void coriolis_cells(int nx, int ny, const float* tex, int fnx, int fn
 
 __global__ void fill_scalar(float* p, float v) { *p = v; }
 
+// Exhaustive check that one Markstein step q + (a - c q) y with y = RN(1/c)
+// equals IEEE a / c for every a with |a| in [2^-96, 2^100] (both signs).
+constexpr uint64_t kDivcDomain = 2ull * (0x62000000ull + 1ull);
+__global__ void __launch_bounds__(256) verify_divc(float c, float y, unsigned int* bad) {
+  const uint64_t base = ((uint64_t)blockIdx.x * 256u + threadIdx.x) * 64u;
+  unsigned int local = 0;
+  for (int k = 0; k < 64; ++k) {
+    const uint64_t i = base + (uint64_t)k;
+    if (i >= kDivcDomain) break;
+    const uint32_t bits = (0x0F800000u + (uint32_t)(i >> 1)) | ((uint32_t)(i & 1u) << 31);
+    const float a = __uint_as_float(bits);
+    const float ref = a / c;
+    const float q = a * y;
+    const float r = __builtin_fmaf(-c, q, a);
+    const float q1 = __builtin_fmaf(r, y, q);
+    local += (__float_as_uint(q1) != __float_as_uint(ref)) ? 1u : 0u;
+  }
+  if (local) atomicAdd(bad, local);
+}
+
 // per_block_max_dt + max_dt_reduction (max_dt.cu:42-195).  Positive floats
 // order like their bit patterns, so the global minimum is an integer atomic.
 __global__ void __launch_bounds__(256) max_dt_kernel(int nx, int ny, float dx, float dy, float g, float land,
@@ -228,6 +250,46 @@ __global__ void __launch_bounds__(256) max_dt_kernel(int nx, int ny, float dx, f
     const float m = fminf(fminf(wmin[0], wmin[1]), fminf(wmin[2], wmin[3]));
     atomicMin((unsigned int*)out, __float_as_uint(m));
   }
+}
+
+std::mutex g_divc_mutex;
+std::unordered_map<uint32_t, bool> g_divc_cache;
+
+int check(hipError_t e, const char* what);
+
+// Markstein division by c verified exhaustively (cached per value of c).
+int divc_verified(float c, hipStream_t s, bool* ok) {
+  const uint32_t key = __builtin_bit_cast(uint32_t, c);
+  {
+    std::lock_guard<std::mutex> lk(g_divc_mutex);
+    auto it = g_divc_cache.find(key);
+    if (it != g_divc_cache.end()) {
+      *ok = it->second;
+      return 0;
+    }
+  }
+  *ok = false;
+  if (!(c > 0.0f) || c > 1e30f || c < 1e-30f) return 0;
+  unsigned int* d_bad = nullptr;
+  int rc = check(hipMalloc(&d_bad, sizeof(unsigned int)), "gpo_cdklm16_prepare hipMalloc");
+  if (rc) return rc;
+  unsigned int h_bad = 0;
+  rc = check(hipMemcpyAsync(d_bad, &h_bad, sizeof h_bad, hipMemcpyHostToDevice, s), "gpo_cdklm16_prepare copy");
+  const float y = 1.0f / c;
+  const uint64_t threads = (kDivcDomain + 63) / 64;
+  const unsigned int blocks = (unsigned int)((threads + 255) / 256);
+  if (!rc) {
+    hipLaunchKernelGGL(verify_divc, dim3(blocks), dim3(256), 0, s, c, y, d_bad);
+    rc = check(hipGetLastError(), "gpo_cdklm16_prepare verify launch");
+  }
+  if (!rc) rc = check(hipMemcpyAsync(&h_bad, d_bad, sizeof h_bad, hipMemcpyDeviceToHost, s), "gpo_cdklm16_prepare copy");
+  if (!rc) rc = check(hipStreamSynchronize(s), "gpo_cdklm16_prepare sync");
+  (void)hipFree(d_bad);
+  if (rc) return rc;
+  *ok = (h_bad == 0);
+  std::lock_guard<std::mutex> lk(g_divc_mutex);
+  g_divc_cache[key] = *ok;
+  return 0;
 }
 
 int check(hipError_t e, const char* what) {
@@ -280,7 +342,7 @@ static int make_step_args(const gpo_cdklm16_params* p, float dt, int32_t rk_step
   a.ny = p->ny;
   a.pitch = pitch / 4;
   a.hi_pitch = Hi_pitch / 4;
-  a.chunk_rows = p->chunk_rows > 0 ? p->chunk_rows : 16;
+  a.chunk_rows = p->chunk_rows > 0 ? p->chunk_rows : kDefaultChunkRows;
   a.n_strips = (p->nx + kStripOut - 1) / kStripOut;
   a.n_chunks = (p->ny + a.chunk_rows - 1) / a.chunk_rows;
   a.bc_n = (boundary_conditions >> 24) & 0xFF;
@@ -319,6 +381,8 @@ static int make_step_args(const gpo_cdklm16_params* p, float dt, int32_t rk_step
   a.one_m_fb = 1.0f - p->flux_balancer;
   a.dxf = p->dx * p->f;
   a.dyf = p->dy * p->f;
+  a.divc_ok = p->divc_ok & 7;
+  a.sqrt_ok = (p->g * p->depth_cutoff) >= 0x1p-96f;
   a.in_e = in_eta;
   a.in_u = in_hu;
   a.in_v = in_hv;
@@ -340,6 +404,18 @@ static int make_step_args(const gpo_cdklm16_params* p, float dt, int32_t rk_step
 static hipError_t launch_step(const gpo_cdklm16_params* p, const StepArgs& a, int exact, hipStream_t s) {
   return exact ? launch_cdklm_exact(a, p->has_land, p->f_is_array, p->angle_is_zero, s)
                : launch_cdklm_fast(a, p->has_land, p->f_is_array, p->angle_is_zero, s);
+}
+
+int gpo_cdklm16_prepare(gpo_cdklm16_params* p, void* stream) {
+  if (!p) return bad("gpo_cdklm16_prepare: params is NULL");
+  hipStream_t s = (hipStream_t)stream;
+  bool okx = false, oky = false, okg = false;
+  int rc = divc_verified(p->dx, s, &okx);
+  if (!rc) rc = divc_verified(p->dy, s, &oky);
+  if (!rc) rc = divc_verified(2.0f * p->g, s, &okg);
+  if (rc) return rc;
+  p->divc_ok = (okx ? 1 : 0) | (oky ? 2 : 0) | (okg ? 4 : 0);
+  return 0;
 }
 
 int gpo_cdklm16_substep(const gpo_cdklm16_params* p, float dt, int32_t rk_step, const float* in_eta,

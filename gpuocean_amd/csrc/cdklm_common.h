@@ -18,6 +18,13 @@ constexpr float kMaxDtCap = 100000.0f;
 constexpr int kStripOut = 60;
 constexpr int kWave = 64;
 constexpr int kWavesPerBlock = 4;
+constexpr int kDefaultChunkRows = 8;
+
+// Lane exchange through DPP wave shifts (verified on gfx950: wave_shr:1 gives
+// lane i the value of lane i-1, wave_shl:1 that of lane i+1).
+#ifndef GPO_USE_DPP
+#define GPO_USE_DPP 1
+#endif
 
 struct StepArgs {
   // geometry
@@ -25,6 +32,8 @@ struct StepArgs {
   int n_strips, n_chunks, chunk_rows;
   int bc_n, bc_e, bc_s, bc_w;
   int rk_order, rk_step, one_dim, has_forcing;
+  int divc_ok;   // bit 0: Markstein division by dx verified, bit 1: by dy, bit 2: by 2g
+  bool sqrt_ok;  // g*depth_cutoff >= 2^-96: unscaled correctly rounded sqrt is valid
   // constants as float32, derived on the host the same way the kernel would
   float dt, dx, dy, g, r, theta, desing_eps, flux_slope_eps, depth_cutoff, flux_balancer, land_value;
   float f, north_x, north_y, wind_stress_factor, rho_o;

@@ -1,0 +1,67 @@
+// Philox4x32-10 counter-based generator (Salmon, Moraes, Dror & Shaw, "Parallel
+// random numbers: as easy as 1, 2, 3", SC'11), usable on host and device, and
+// the Box-Muller transform used for the stochastic model error.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gpo {
+
+struct U4 {
+  uint32_t x, y, z, w;
+};
+
+__host__ __device__ inline uint32_t mulhi32(uint32_t a, uint32_t b) {
+  return (uint32_t)(((uint64_t)a * (uint64_t)b) >> 32);
+}
+
+// One call: counter (c0..c3), key (k0, k1), ten rounds with key schedule.
+__host__ __device__ inline U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1) {
+  constexpr uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+  constexpr uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r > 0) {
+      k0 += W0;
+      k1 += W1;
+    }
+    const uint32_t hi0 = mulhi32(M0, c.x), lo0 = M0 * c.x;
+    const uint32_t hi1 = mulhi32(M1, c.z), lo1 = M1 * c.z;
+    U4 n;
+    n.x = hi1 ^ c.y ^ k0;
+    n.y = lo1;
+    n.z = hi0 ^ c.w ^ k1;
+    n.w = lo0;
+    c = n;
+  }
+  return c;
+}
+
+// Standard normal pair from two 32-bit words: uniforms at bin centres of a
+// 2^-24 grid (never 0 or 1), r = sqrt(-2 ln u1), angle 2 pi u2.
+__device__ inline void box_muller(uint32_t a, uint32_t b, float* z0, float* z1) {
+  const float u1 = (float)(a >> 8) * 0x1p-24f + 0x1p-25f;
+  const float u2 = (float)(b >> 8) * 0x1p-24f + 0x1p-25f;
+  const float r = sqrtf(-2.0f * logf(u1));
+  float s, c;
+  sincospif(2.0f * u2, &s, &c);
+  *z0 = r * c;
+  *z1 = r * s;
+}
+
+// The normal number of element `index` of draw `draw` in stream `stream`:
+// one Philox call per group of four elements.
+constexpr uint32_t kNoiseTag = 0x534F4152u;  // "SOAR"
+__device__ inline float normal_at(uint32_t index, uint32_t draw, uint32_t stream, uint32_t k0, uint32_t k1) {
+  const U4 r = philox4x32_10(U4{index >> 2, draw, stream, kNoiseTag}, k0, k1);
+  float a, b;
+  if ((index & 2u) == 0u) {
+    box_muller(r.x, r.y, &a, &b);
+  } else {
+    box_muller(r.z, r.w, &a, &b);
+  }
+  return (index & 1u) ? b : a;
+}
+
+}  // namespace gpo

@@ -45,6 +45,7 @@ typedef struct {
   float land_value;               /* Bathymetry.mask_value (1e20) */
   float north_x, north_y;         /* sin(angle), cos(angle) for a constant angle */
   float wind_stress_factor, rho_o;/* WIND_STRESS_FACTOR, RHO_O */
+  int32_t divc_ok;                /* set by gpo_cdklm16_prepare (0 = IEEE division only) */
 } gpo_cdklm16_params;
 
 /* Optional wind stress / atmospheric pressure textures (time levels `cur`
@@ -59,6 +60,15 @@ typedef struct {
   int32_t pressure_nx, pressure_ny;
   float pressure_t;
 } gpo_forcing;
+
+/* Set-up of a parameter block before its first substep: verifies, on the
+ * device and for every numerator in {0} U [2^-96, 2^100], that division by
+ * each of dx, dy and 2g equals one Markstein correction step with the
+ * correctly rounded reciprocal, and records the result in p->divc_ok (used by
+ * the exact build; results are cached per constant).  Synchronises `stream`
+ * and allocates a 4-byte counter.  No reference counterpart: the reference
+ * compiles its constants into the kernel (SWEsimulators/CDKLM16.py:242-292). */
+int gpo_cdklm16_prepare(gpo_cdklm16_params* p, void* stream);
 
 /* One Runge-Kutta substep of CDKLM16 over the interior.
  * Replaces cdklm_swe_2D (gpu_kernels/CDKLM16_kernel.cu:475-515) as launched by
@@ -127,6 +137,51 @@ int gpo_coriolis_cells(int32_t nx, int32_t ny, const float* f_tex, int32_t f_nx,
 int gpo_max_dt(int32_t nx, int32_t ny, float dx, float dy, float g, float land_value,
                const float* eta, const float* hu, const float* hv, int32_t pitch,
                const float* Bm, int32_t Bm_pitch, float* dt_out, void* stream);
+
+/* ---- stochastic model error (SOAR) ---------------------------------------- */
+
+/* Raw Philox4x32-10 blocks (Salmon et al., SC'11): out[4i..4i+3] for counter
+ * ctr[4i..4i+3] and key key[2i..2i+1], device pointers.  Exposed for the
+ * known-answer test of the generator that replaces the reference's curand
+ * XORWOW / LCG streams (utils/RandomNumbers.py:36-185). */
+int gpo_philox4x32_10(const uint32_t* ctr, const uint32_t* key, uint32_t* out, int32_t n, void* stream);
+
+/* Standard normal field: out(j, i) from Philox counter (idx/4, draw, stream_id,
+ * tag) with idx = j*cols + i, key = seed, Box-Muller on word pairs.  Replaces
+ * RandomNumbers.generateNormalDistribution (utils/RandomNumbers.py:168-185). */
+int gpo_random_normal(float* out, int32_t rows, int32_t cols, int32_t pitch, uint64_t seed, uint32_t stream_id,
+                      uint32_t draw, void* stream);
+
+/* SOAR-correlated field on the (ny+4) x (nx+4) coarse grid from xi with a
+ * 5x5 stencil of weights q0 (1 + d/L) exp(-d/L).  Replaces the SOAR kernel
+ * (gpu_kernels/ocean_noise.cu:141-241) launched by
+ * OceanStateNoise.perturbOceanState (SWEsimulators/OceanStateNoise.py:356-459).
+ * xi: (ny+8) x (nx+8) (or ny / nx rows/columns on a periodic axis). */
+int gpo_soar(int32_t nx, int32_t ny, const float* weights25, float scale, int32_t periodic_ns,
+             int32_t periodic_ew, const float* xi, int32_t xi_pitch, float* coarse, int32_t coarse_pitch,
+             int32_t additive, void* stream);
+
+/* Adds the coarse d_eta and its geostrophically balanced hu, hv to the
+ * interior state.  Replaces geostrophicBalance (gpu_kernels/ocean_noise.cu:256-390). */
+int gpo_geostrophic_balance(int32_t nx, int32_t ny, float dx, float dy, float g, float f, float beta,
+                            float north_x, float north_y, const float* f_cells, int32_t f_pitch,
+                            const float* coarse, int32_t coarse_pitch, float* eta, float* hu, float* hv,
+                            int32_t pitch, const float* Hi, int32_t Hi_pitch, float land_value, void* stream);
+
+/* ---- drifters ----------------------------------------------------------------- */
+
+/* One forward-Euler step of n_drifters + 1 passive drifters (the last one is the
+ * observation drifter) with bilinear cell-centre velocity.  Replaces
+ * passiveDrifterKernel (gpu_kernels/driftKernels.cu:74-183) launched by
+ * GPUDrifterCollection.drift (drifters/GPUDrifterCollection.py:210-232).
+ * positions: (n_drifters+1) rows of (x, y), pitch pos_pitch bytes; Hm is the
+ * cell-centre depth Bm. */
+int gpo_drift(int32_t nx, int32_t ny, float dx, float dy, float dt, int32_t x0ref, int32_t y0ref, const float* eta,
+              const float* hu, const float* hv, int32_t pitch, const float* Hm, int32_t Hm_pitch,
+              int32_t periodic_ns, int32_t periodic_ew, int32_t n_drifters, float* positions, int32_t pos_pitch,
+              float sensitivity, const float* wind_x_cur, const float* wind_x_next, const float* wind_y_cur,
+              const float* wind_y_next, int32_t wind_nx, int32_t wind_ny, float wind_t, float wind_drift_factor,
+              void* stream);
 
 /* Text of the last error on this thread. */
 const char* gpo_last_error(void);

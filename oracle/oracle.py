@@ -269,3 +269,57 @@ class OracleCDKLM16:
 
 def num_threads():
     return lib().gpo_oracle_num_threads()
+
+
+def _fp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def soar(nx, ny, W, scale, per_ns, per_ew, xi, coarse=None, additive=False):
+    """SOAR field on the (ny+4) x (nx+4) coarse grid from xi (noise_drift_oracle.cpp)."""
+    L = lib()
+    xi = np.ascontiguousarray(xi, dtype=np.float32)
+    W = np.ascontiguousarray(W, dtype=np.float32)
+    out = np.zeros((ny + 4, nx + 4), np.float32) if coarse is None else np.ascontiguousarray(coarse, np.float32)
+    L.gpo_oracle_soar(ctypes.c_int(nx), ctypes.c_int(ny), _fp(W), ctypes.c_float(scale), ctypes.c_int(int(per_ns)),
+                      ctypes.c_int(int(per_ew)), _fp(xi), ctypes.c_int(xi.shape[1]), _fp(out),
+                      ctypes.c_int(int(additive)))
+    return out
+
+
+def geostrophic_balance(nx, ny, dx, dy, g, f, beta, north_x, north_y, f_cells, coarse, eta, hu, hv, Hi,
+                        land_value=np.float32(1e20)):
+    """Adds the balanced perturbation to eta/hu/hv in place (float32 arrays)."""
+    L = lib()
+    args = [ctypes.c_int(nx), ctypes.c_int(ny)] + [ctypes.c_float(v) for v in (dx, dy, g, f, beta, north_x, north_y)]
+    fc = None if f_cells is None else _fp(np.ascontiguousarray(f_cells, np.float32))
+    L.gpo_oracle_geostrophic_balance(*args, fc, _fp(np.ascontiguousarray(coarse, np.float32)), _fp(eta), _fp(hu),
+                                     _fp(hv), _fp(np.ascontiguousarray(Hi, np.float32)), ctypes.c_float(land_value))
+
+
+def drift(nx, ny, dx, dy, dt, eta, hu, hv, Hm, per_ns, per_ew, pos, sensitivity=1.0, wind=(0.0, 0.0), factor=0.0):
+    """One drifter step in place on pos ((n+1) x 2 float32)."""
+    L = lib()
+    n = pos.shape[0] - 1
+    c = [np.ascontiguousarray(a, np.float32) for a in (eta, hu, hv, Hm)]
+    L.gpo_oracle_drift(ctypes.c_int(nx), ctypes.c_int(ny), ctypes.c_float(dx), ctypes.c_float(dy),
+                       ctypes.c_float(dt), ctypes.c_int(2), ctypes.c_int(2), *[_fp(a) for a in c],
+                       ctypes.c_int(int(per_ns)), ctypes.c_int(int(per_ew)), ctypes.c_int(n), _fp(pos),
+                       ctypes.c_float(sensitivity), ctypes.c_float(wind[0]), ctypes.c_float(wind[1]),
+                       ctypes.c_float(factor))
+
+
+def philox4x32_10(ctr, key):
+    """Reference Philox4x32-10 in Python integers (Salmon et al., SC'11)."""
+    M0, M1, W0, W1 = 0xD2511F53, 0xCD9E8D57, 0x9E3779B9, 0xBB67AE85
+    c = list(ctr)
+    k0, k1 = key
+    for r in range(10):
+        if r:
+            k0 = (k0 + W0) & 0xFFFFFFFF
+            k1 = (k1 + W1) & 0xFFFFFFFF
+        p0 = M0 * c[0]
+        p1 = M1 * c[2]
+        c = [((p1 >> 32) ^ c[1] ^ k0) & 0xFFFFFFFF, p1 & 0xFFFFFFFF, ((p0 >> 32) ^ c[3] ^ k1) & 0xFFFFFFFF,
+             p0 & 0xFFFFFFFF]
+    return c
