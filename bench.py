@@ -1,11 +1,18 @@
 """Benchmark of the CDKLM16 step on one MI355X.
 
-Default workload: the 2048x1024 double-jet member (dx=dy=1 km, periodic in x,
-walls in y, f=1e-4, dt = 0.7 x CFL), FP32, RK2.  A step is one RK2 time step
-(two launches of the substep kernel) over the whole domain; the state is
-resident in HBM.  Prints one JSON line.
+Default workload (c2): the batched stochastic ensemble of 256 members of
+seeded uneven sizes (nx in 128..512, ny in 128..384, multiples of 16), each the
+double jet scaled to its domain, SOAR model error q0=5e-3, L=0.75 dx, its own
+Philox seed and its own dt (0.7 x CFL).  A step is one RK2 time step of every
+member plus its model error (random field, SOAR, geostrophic balance), from a
+captured hipGraph; the states are resident in HBM.  value = member-cell-steps/s
+= sum of interior cells x steps / wall time.  The same members run one by one
+through the per-member path give the sequential figure the speedup is quoted
+against, and the 2048x1024 single member (c1) is measured alongside.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c1|c0]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c1|c0]
+
+Prints one JSON line.
 """
 
 import argparse
@@ -21,23 +28,37 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL_STEP = 68.0   # RK2: 28 B (substep 0) + 40 B (substep 1) per interior cell
 BYTES_SUBSTEP = (28.0, 40.0)
+BYTES_NOISE = 28.0           # model error: eta/hu/hv read-modify-write + bathymetry per interior cell
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def read_traffic(workload):
-    """HBM bytes per launch from the committed rocprofv3 counter summary, if any."""
-    path = os.path.join(ROOT, "profiles", f"cdklm_march_{workload}_pmc.json")
+def read_traffic(kernel, workload):
+    """HBM bytes per launch (FETCH + WRITE, calibrated on a copy kernel) from
+    the committed rocprofv3 counter summary profiles/<kernel>_<workload>_pmc.txt
+    (a line 'hbm_bytes_per_launch: N'), if present."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_pmc.txt")
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        d = json.load(f)
-    return d.get("hbm_bytes_per_launch")
+        for line in f:
+            if line.startswith("hbm_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    return None
 
 
-def cpu_baseline(kw, workload, budget_s=15.0):
+def event_pair(stream):
+    import torch
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    return a, b
+
+
+# ---------------------------------------------------------------------------- CPU baselines
+
+def cpu_baseline_single(kw, workload, budget_s=15.0):
     """FP32 oracle (OpenMP, all host threads) on a bounded number of steps."""
     import numpy as np
     from oracle.oracle import OracleCDKLM16, num_threads
@@ -56,64 +77,76 @@ def cpu_baseline(kw, workload, budget_s=15.0):
                       f"({el:.1f} s), wall and periodic ghosts in C"}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--workload", default="c1", choices=["c1", "c0"])
-    ap.add_argument("--fast", action="store_true",
-                    help="use the fast kernel build (not bit-compatible with the oracle)")
-    ap.add_argument("--chunk-rows", type=int, default=0)
-    ap.add_argument("--no-cpu", action="store_true")
-    args = ap.parse_args()
+def cpu_baseline_ensemble(members_kw, budget_s=15.0):
+    """FP32 oracle on the first members of the ensemble, one after another:
+    RK2 step + SOAR + geostrophic balance per step (the normal field is drawn
+    with numpy outside the timed calls)."""
+    import numpy as np
+    from oracle import oracle as O
+    from gpuocean_amd.OceanStateNoise import soar_weights
+    total_cells = 0
+    steps_done = 0
+    el = 0.0
+    used = 0
+    rng = np.random.default_rng(0)
+    for kw in members_kw:
+        ora = O.OracleCDKLM16(**kw)
+        nx, ny = kw["nx"], kw["ny"]
+        dt = np.float32(ora.dt)
+        W = soar_weights(np.float32(kw["dx"]), np.float32(kw["dy"]), np.float32(5e-3), np.float32(0.75 * kw["dx"]))
+        xi = rng.standard_normal((ny + 8, nx)).astype(np.float32)  # periodic in x, walls in y
+        for _ in range(4):
+            t0 = time.perf_counter()
+            ora.run_steps(1, dt)
+            coarse = O.soar(nx, ny, W, 1.0, False, True, xi)
+            e, u, v = ora.h0
+            O.geostrophic_balance(nx, ny, np.float32(kw["dx"]), np.float32(kw["dy"]), np.float32(kw["g"]),
+                                  np.float32(kw["f"]), 0.0, 0.0, 1.0, None, coarse, e, u, v, ora.Hi)
+            el += time.perf_counter() - t0
+            total_cells += nx * ny
+            steps_done += 1
+        used += 1
+        if el > budget_s:
+            break
+    return {"value": total_cells / el, "unit": "member-cell-steps/s", "cores": O.num_threads(), "kind": "port",
+            "sample": f"first {used} of the {len(members_kw)} members, 4 stochastic RK2 steps each "
+                      f"(step + SOAR + balance) of the FP32 oracle, one member after another ({el:.1f} s)"}
 
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(members or domains would shard across GPUs, see the design notes).", file=sys.stderr)
-        sys.exit(2)
 
-    import ctypes
+# ---------------------------------------------------------------------------- single member
+
+def run_single(kw, workload, steps, warmup, fast, chunk_rows):
     import numpy as np
     import torch
-    import cases
     from gpuocean_amd import CDKLM16, Common
-    from gpuocean_amd import _lib as L
 
-    kw = cases.config_c1() if args.workload == "c1" else cases.config_c0()
     ctx = Common.DeviceContext(0)
-    sim = CDKLM16(ctx, **kw, exact_arithmetic=not args.fast, chunk_rows=args.chunk_rows or None)
+    sim = CDKLM16(ctx, **kw, exact_arithmetic=not fast, chunk_rows=chunk_rows or None)
     dt = np.float32(sim.dt)
     nx, ny = kw["nx"], kw["ny"]
     cells = nx * ny
     stream = sim.gpu_stream
-    log(f"workload {args.workload} {nx}x{ny} dt={dt:.4f} s, warmup {args.warmup}, steps {args.steps}")
-
-    # warmup
-    sim._rk2_steps(args.warmup, dt)
+    log(f"[{workload}] {nx}x{ny} dt={dt:.4f} s, warmup {warmup}, steps {steps}")
+    sim._rk2_steps(warmup, dt)
     stream.synchronize()
 
-    # timed region: K RK2 steps, bracketed by synchronisation
     torch.cuda.synchronize()
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0, ev1 = event_pair(stream)
     t_wall0 = time.perf_counter()
     ev0.record(stream)
-    sim._rk2_steps(args.steps, dt)
+    sim._rk2_steps(steps, dt)
     ev1.record(stream)
     stream.synchronize()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_wall0
     gpu_ms = ev0.elapsed_time(ev1)
-    ms_per_step = gpu_ms / args.steps
-    value = cells * args.steps / (gpu_ms / 1e3)
-    log(f"timed: {gpu_ms:.2f} ms GPU ({wall * 1e3:.2f} ms wall) for {args.steps} steps -> "
-        f"{ms_per_step * 1e3:.2f} us/step, {value:.4g} cell-steps/s")
+    value = cells * steps / wall
+    log(f"[{workload}] timed: {gpu_ms:.2f} ms GPU, {wall * 1e3:.2f} ms wall for {steps} steps -> "
+        f"{1e3 * wall / steps:.2f} us/step, {value:.4g} cell-steps/s")
 
     # per-launch durations of the substep kernel, events on the kernel's stream
-    n_prof = min(args.steps, 200)
+    n_prof = min(steps, 200)
     evs = [torch.cuda.Event(enable_timing=True) for _ in range(2 * n_prof + 1)]
-    p = sim.params
     h0, h1 = sim.h0, sim.h1
     evs[0].record(stream)
     k = 1
@@ -124,46 +157,191 @@ def main():
         evs[k + 1].record(stream)
         k += 2
     stream.synchronize()
-    d0 = [evs[2 * i].elapsed_time(evs[2 * i + 1]) for i in range(n_prof)]
-    d1 = [evs[2 * i + 1].elapsed_time(evs[2 * i + 2]) for i in range(n_prof)]
-    t0_us = 1e3 * float(np.mean(d0))
-    t1_us = 1e3 * float(np.mean(d1))
+    t0_us = 1e3 * float(np.mean([evs[2 * i].elapsed_time(evs[2 * i + 1]) for i in range(n_prof)]))
+    t1_us = 1e3 * float(np.mean([evs[2 * i + 1].elapsed_time(evs[2 * i + 2]) for i in range(n_prof)]))
     avg_launch_us = 0.5 * (t0_us + t1_us)
-    bytes_per_launch = 0.5 * BYTES_PER_CELL_STEP * cells
-    achieved = bytes_per_launch / (avg_launch_us * 1e-6) / 1e9
-    log(f"substep 0: {t0_us:.2f} us ({BYTES_SUBSTEP[0] * cells / t0_us / 1e3:.0f} GB/s), "
+    achieved = 0.5 * BYTES_PER_CELL_STEP * cells / (avg_launch_us * 1e-6) / 1e9
+    log(f"[{workload}] substep 0: {t0_us:.2f} us ({BYTES_SUBSTEP[0] * cells / t0_us / 1e3:.0f} GB/s), "
         f"substep 1: {t1_us:.2f} us ({BYTES_SUBSTEP[1] * cells / t1_us / 1e3:.0f} GB/s)")
+    assert np.all(np.isfinite(sim.download()[0])), "non-finite state"
+    return {
+        "value": value, "unit": "cell-steps/s", "ms_per_step": 1e3 * wall / steps, "steps": steps,
+        "config": {"workload": f"CDKLM16 single member {nx}x{ny}, RK2, "
+                               f"{'periodic-x walls-y double jet' if workload == 'c1' else 'walls, Gaussian bump'}",
+                   "nx": nx, "ny": ny, "dt": float(dt), "members": 1,
+                   "kernel_build": "fast" if fast else "exact"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic("cdklm_march", workload),
+                     "kernel": "cdklm_march", "bytes_per_cell_per_launch": BYTES_PER_CELL_STEP / 2,
+                     "cells_per_launch": cells, "avg_launch_us": avg_launch_us},
+    }
 
-    # sanity: the state stays finite
-    eta = sim.download()[0]
-    assert np.all(np.isfinite(eta)), "non-finite state"
 
-    result = {
-        "metric": "member-cell-steps/s (cell-steps/s of one member); % HBM roofline of the fused step",
+# ---------------------------------------------------------------------------- batched ensemble
+
+def run_ensemble(args):
+    import ctypes
+    import numpy as np
+    import torch
+    import cases
+    from gpuocean_amd import BatchedEnsemble, CDKLM16, Common
+    from gpuocean_amd import _lib as L
+
+    ctx = Common.DeviceContext(0)
+    t0 = time.perf_counter()
+    members_kw = cases.config_c2(args.members)
+    sims = []
+    for i, kw in enumerate(members_kw):
+        sim = CDKLM16(ctx, **kw, local_particle_id=i, exact_arithmetic=not args.fast)
+        sim.setSOARModelError(5e-3, xorwow_seed=1_000_003 + i)
+        sims.append(sim)
+    torch.cuda.synchronize()
+    cells = sum(int(s.nx) * int(s.ny) for s in sims)
+    log(f"[c2] {len(sims)} members, {cells / 1e6:.2f} M interior cells, built in {time.perf_counter() - t0:.1f} s")
+
+    t0 = time.perf_counter()
+    ens = BatchedEnsemble(sims, autotune=not args.no_autotune, chunk_rows=args.chunk_rows or None)
+    n_work, n_blocks = ens.info()
+    log(f"[c2] chunk_rows={ens.chunk_rows} waves_per_simd={ens.waves_per_simd} work items={n_work} "
+        f"workgroups={n_blocks} (tuned in {time.perf_counter() - t0:.1f} s)")
+    G = args.graph_steps
+
+    ens.step(args.warmup, graph_steps=G)
+    torch.cuda.synchronize()
+    t_wall0 = time.perf_counter()
+    ens.step(args.steps, graph_steps=G)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_wall0
+    value = cells * args.steps / wall
+    log(f"[c2] batched: {args.steps} steps in {wall * 1e3:.1f} ms -> {1e3 * wall / args.steps:.3f} ms/step, "
+        f"{value:.4g} member-cell-steps/s")
+
+    # per-launch durations of each part of a step, events on the ensemble stream
+    st = ens.stream
+    sh = ctypes.c_void_p(st.cuda_stream)
+    n_prof = min(args.steps, 50)
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(3 * n_prof + 1)]
+    ens._sync_in()
+    evs[0].record(st)
+    for i in range(n_prof):
+        for part in range(3):
+            L.check(L.lib().gpo_ensemble_part(ens.handle, part, sh), "gpo_ensemble_part")
+            evs[3 * i + part + 1].record(st)
+    st.synchronize()
+    part_us = [1e3 * float(np.mean([evs[3 * i + p].elapsed_time(evs[3 * i + p + 1]) for i in range(n_prof)]))
+               for p in range(3)]
+    for s in sims:   # keep the members' host bookkeeping in step with the device
+        s._after_batched_steps(n_prof, np.float32(s.dt))
+    ens._draws = [d + n_prof for d in ens._draws]
+    avg_launch_us = 0.5 * (part_us[0] + part_us[1])
+    achieved = 0.5 * BYTES_PER_CELL_STEP * cells / (avg_launch_us * 1e-6) / 1e9
+    noise_gbs = BYTES_NOISE * cells / (part_us[2] * 1e-6) / 1e9
+    log(f"[c2] substep 0: {part_us[0]:.1f} us, substep 1: {part_us[1]:.1f} us "
+        f"({achieved:.0f} GB/s algorithmic), model error: {part_us[2]:.1f} us ({noise_gbs:.0f} GB/s algorithmic)")
+
+    # the same members one by one through the per-member path
+    torch.cuda.synchronize()
+    n_seq = args.seq_steps
+    t_wall0 = time.perf_counter()
+    for s in sims:
+        for _ in range(n_seq):
+            s.step(np.float32(s.dt))
+    torch.cuda.synchronize()
+    wall_seq = time.perf_counter() - t_wall0
+    value_seq = cells * n_seq / wall_seq
+    log(f"[c2] sequential per-member path: {n_seq} steps of every member in {wall_seq:.2f} s -> "
+        f"{value_seq:.4g} member-cell-steps/s; batched speedup {value / value_seq:.2f}x")
+    for s in sims[:4]:
+        assert np.all(np.isfinite(s.download()[0])), "non-finite state"
+    ens.destroy()
+
+    sizes = [(int(s.nx), int(s.ny)) for s in sims]
+    return members_kw, {
+        "metric": "member-cell-steps/s at 256 members; % HBM roofline; adjoint/forward time",
         "value": value,
         "unit": "member-cell-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": 1e3 * wall / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded double jet over Gaussian-filtered random bathymetry)",
-        "config": {"workload": f"CDKLM16 {args.workload.upper()} single member {nx}x{ny}, RK2, "
-                               f"{'periodic-x walls-y' if args.workload == 'c1' else 'walls'}",
-                   "nx": nx, "ny": ny, "dt": float(dt), "members": 1,
+        "data": "synthetic (seeded double jets over Gaussian-filtered random bathymetry, Philox model error)",
+        "config": {"workload": f"CDKLM16 batched stochastic ensemble, {len(sims)} uneven members "
+                               f"(nx 128..512, ny 128..384), RK2 + SOAR q0=5e-3 L=0.75dx, hipGraph of {G} steps",
+                   "members": len(sims), "interior_cells": cells,
+                   "nx_range": [min(a for a, _ in sizes), max(a for a, _ in sizes)],
+                   "ny_range": [min(b for _, b in sizes), max(b for _, b in sizes)],
+                   "chunk_rows": ens.chunk_rows, "waves_per_simd": ens.waves_per_simd,
                    "kernel_build": "fast" if args.fast else "exact"},
+        "sequential": {"value": value_seq, "unit": "member-cell-steps/s", "steps": n_seq,
+                       "speedup_batched": value / value_seq,
+                       "path": "each member's own CDKLM16.step, one member after another"},
+        "parts_us": {"substep0": part_us[0], "substep1": part_us[1], "model_error": part_us[2]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(args.workload),
-                     "kernel": "cdklm_march", "bytes_per_cell_per_launch": BYTES_PER_CELL_STEP / 2,
-                     "avg_launch_us": avg_launch_us},
-        "cpu_baseline": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic("cdklm_march_batched", "c2"),
+                     "kernel": "cdklm_march_batched", "bytes_per_cell_per_launch": BYTES_PER_CELL_STEP / 2,
+                     "cells_per_launch": cells, "avg_launch_us": avg_launch_us},
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=None)
+    ap.add_argument("--workload", default="c2", choices=["c2", "c1", "c0"])
+    ap.add_argument("--members", type=int, default=256)
+    ap.add_argument("--graph-steps", type=int, default=10)
+    ap.add_argument("--seq-steps", type=int, default=10)
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--fast", action="store_true",
+                    help="use the fast kernel build (not bit-compatible with the oracle)")
+    ap.add_argument("--chunk-rows", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-c1", action="store_true", help="skip the single-member line of the c2 run")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(ensemble members would be split across GPUs by member, with no exchange while stepping).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import cases
+
+    if args.workload == "c2":
+        args.steps = 500 if args.steps is None else args.steps
+        args.warmup = 20 if args.warmup is None else args.warmup
+        members_kw, result = run_ensemble(args)
+        if not args.no_c1:
+            c1 = run_single(cases.config_c1(), "c1", 2000, 50, args.fast, 0)
+            result["single_member_c1"] = c1
+        result["cpu_baseline"] = None
+        if not args.no_cpu:
+            log("cpu baseline (FP32 oracle, ensemble members one after another)...")
+            result["cpu_baseline"] = cpu_baseline_ensemble(members_kw)
+        print(json.dumps(result), flush=True)
+        return
+
+    args.steps = 2000 if args.steps is None else args.steps
+    args.warmup = 50 if args.warmup is None else args.warmup
+    kw = cases.config_c1() if args.workload == "c1" else cases.config_c0()
+    r = run_single(kw, args.workload, args.steps, args.warmup, args.fast, args.chunk_rows)
+    result = {
+        "metric": "member-cell-steps/s at 256 members; % HBM roofline; adjoint/forward time",
+        "value": r["value"], "unit": "member-cell-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded double jet over Gaussian-filtered random bathymetry)"
+        if args.workload == "c1" else "synthetic (Gaussian bump)",
+        "config": r["config"], "roofline": r["roofline"], "cpu_baseline": None,
     }
     if not args.no_cpu:
         log("cpu baseline (FP32 oracle)...")
-        result["cpu_baseline"] = cpu_baseline(kw, args.workload)
+        result["cpu_baseline"] = cpu_baseline_single(kw, args.workload)
     print(json.dumps(result), flush=True)
 
 

@@ -406,6 +406,17 @@ class CDKLM16:
             self.boundary_conditions.packed(), int(self.exact_arithmetic), self._stream_handle()),
             "gpo_cdklm16_rk2_steps")
 
+    def local_dts(self, t_end):
+        """The time steps step(t_end) takes: local_dt = f32(min(dt, f32(t_end -
+        t_now))) with t_now accumulated in float64 (CDKLM16.py:493-500)."""
+        dts = []
+        t_now = 0.0
+        while t_now < t_end:
+            local_dt = np.float32(min(self.dt, np.float32(t_end - t_now)))
+            dts.append(local_dt)
+            t_now += np.float64(local_dt)
+        return dts
+
     def step(self, t_end=0.0, apply_stochastic_term=True, write_now=True, update_dt=False):
         """Advance by t_end seconds of model time (CDKLM16.py:477-584).
 
@@ -417,12 +428,7 @@ class CDKLM16:
             self._update_bc_values()
             self._apply_bc(self.h0)
         if self._batched_eligible(update_dt):
-            dts = []
-            t_now = 0.0
-            while t_now < t_end:
-                local_dt = np.float32(min(self.dt, np.float32(t_end - t_now)))
-                dts.append(local_dt)
-                t_now += np.float64(local_dt)
+            dts = self.local_dts(t_end)
             i = 0
             while i < len(dts):
                 j = i
@@ -437,25 +443,94 @@ class CDKLM16:
             return self.t
 
         t_now = 0.0
-        ghosts_read_each_step = (self.model_error is not None) or self.hasDrifters
         while t_now < t_end:
             if update_dt:
                 self.updateDt()
             local_dt = np.float32(min(self.dt, np.float32(t_end - t_now)))
-            forcing = self._update_forcing()
-            self._update_bc_values()
-            self._rk_step(local_dt, forcing)
-            if self.model_error is not None and apply_stochastic_term:
-                self.perturbState()
-            if ghosts_read_each_step or self._needs_explicit_bc_each_substep():
-                self._apply_bc(self.h0)
-            self.drifterStep(local_dt)
-            self.t += np.float64(local_dt)
+            self._one_step(local_dt, apply_stochastic_term)
             t_now += np.float64(local_dt)
-            self.num_iterations += 1
-        if not ghosts_read_each_step:
+        if not self._ghosts_read_each_step():
             self._apply_bc(self.h0)
         return self.t
+
+    def step_dts(self, dts, apply_stochastic_term=True):
+        """Steps of the given sizes through the per-step path (the tail of an
+        ensemble step that the batched launch does not cover)."""
+        for local_dt in dts:
+            self._one_step(np.float32(local_dt), apply_stochastic_term)
+        if not self._ghosts_read_each_step():
+            self._apply_bc(self.h0)
+        return self.t
+
+    def _ghosts_read_each_step(self):
+        return (self.model_error is not None) or self.hasDrifters
+
+    def _one_step(self, local_dt, apply_stochastic_term):
+        forcing = self._update_forcing()
+        self._update_bc_values()
+        self._rk_step(local_dt, forcing)
+        if self.model_error is not None and apply_stochastic_term:
+            self.perturbState()
+        if self._ghosts_read_each_step() or self._needs_explicit_bc_each_substep():
+            self._apply_bc(self.h0)
+        self.drifterStep(local_dt)
+        self.t += np.float64(local_dt)
+        self.num_iterations += 1
+
+    def _after_batched_steps(self, n, dt):
+        """Host bookkeeping after n batched steps of size dt: time, iteration
+        count, the model error's draw index and the ghost cells."""
+        for _ in range(n):
+            self.t += np.float64(dt)
+        self.num_iterations += n
+        if self.model_error is not None:
+            self.model_error.draw += n
+        self._apply_bc(self.h0)
+
+    def dataAssimilationStep(self, observation_time, otherSim=None, model_error_final_step=True, write_now=True,
+                             courant_number=0.8):
+        """Steps to observation_time: the leftover step first, then full
+        model_time_steps, each followed by model error (CDKLM16.py:742-806)."""
+        full_model_time_steps = int(round(observation_time - self.t) / self.model_time_step)
+        leftover_step_size = observation_time - self.t - full_model_time_steps * self.model_time_step
+        if leftover_step_size / self.model_time_step < 0.1 and full_model_time_steps > 1:
+            leftover_step_size += self.model_time_step
+            full_model_time_steps -= 1
+        if leftover_step_size / self.model_time_step < 0.00001:
+            leftover_step_size = 0
+        assert full_model_time_steps > 0, "There is less than CDKLM16.model_time_step until the observation"
+        self.updateDt(courant_number=courant_number)
+        if otherSim is not None:
+            otherSim.updateDt(courant_number=courant_number)
+        for i in range(full_model_time_steps + 1):
+            if i == 0 and leftover_step_size == 0:
+                continue
+            elif i == 0:
+                scale = np.sqrt(leftover_step_size / self.model_time_step)
+                self.step(leftover_step_size, apply_stochastic_term=False, write_now=False)
+                self.perturbState(perturbation_scale=scale)
+                if otherSim is not None:
+                    otherSim.step(leftover_step_size, apply_stochastic_term=False, write_now=False)
+                    otherSim.perturbSimilarAs(self, perturbation_scale=scale)
+            else:
+                self.step(self.model_time_step, apply_stochastic_term=False, write_now=False)
+                if (i < full_model_time_steps) or model_error_final_step:
+                    self.perturbState()
+                if otherSim is not None:
+                    otherSim.step(self.model_time_step, apply_stochastic_term=False, write_now=False)
+                    if (i < full_model_time_steps) or model_error_final_step:
+                        otherSim.perturbSimilarAs(self)
+            self.total_time_steps += 1
+            if self.total_time_steps % 5 == 0:
+                self.updateDt(courant_number=courant_number)
+        assert round(observation_time) == round(self.t), \
+            "The simulation time is not the same as observation time after dataAssimilationStep! " \
+            f"(self.t, observation_time, diff): {(self.t, observation_time, self.t - observation_time)}"
+
+    def perturbSimilarAs(self, otherSim, perturbation_scale=1.0):
+        """Perturbation from otherSim's current random field (CDKLM16.py:735-736)."""
+        self.model_error.perturbSimSimilarAs(self, modelError=otherSim.model_error,
+                                             perturbation_scale=perturbation_scale)
 
     def drifterStep(self, dt):
         if self.hasDrifters:

@@ -1,0 +1,428 @@
+"""Ensembles of CDKLM16 members on one GPU.
+
+``OceanModelEnsemble`` is a drop-in for the reference's
+``ensembles/OceanModelEnsemble.py``: the same constructor, ``modelStep``,
+``stepToObservation``, ``updateDt``, ``observeParticles`` and
+``attachDrifters``.  ``data_args`` may also be a list with one dict per member,
+so members can differ in size.
+
+Where the reference steps its members one after another (a Python loop with a
+blocking dt read-back per member, OceanModelEnsemble.py:138-150),
+``modelStep`` hands every eligible run of full time steps to a
+``BatchedEnsemble``: one persistent launch per RK substep over all members
+(a table of member/strip/row-chunk work items), then one batched launch each
+of the random field, SOAR and geostrophic balance, optionally replayed from a
+captured hipGraph.  The batched launches produce bit for bit the values of the
+per-member path.
+"""
+
+import ctypes
+import logging
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from .CDKLM16 import CDKLM16
+from .GPUDrifterCollection import GPUDrifterCollection
+
+
+def _vp(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class BatchedEnsemble:
+    """Members stepped together by the library's batched ensemble entry points.
+
+    Every member must use RK2 with wall or periodic boundaries and no wind,
+    pressure or drifters, and all members must agree on exact arithmetic,
+    land, coriolis texture, north angle and whether they carry SOAR model
+    error.  Each member keeps its own dt, seed and draw index."""
+
+    def __init__(self, sims, chunk_rows=None, waves_per_simd=None, autotune=False):
+        ok, why = BatchedEnsemble.eligible(sims)
+        if not ok:
+            raise ValueError(f"members cannot be batched: {why}")
+        self.sims = list(sims)
+        self.n = len(self.sims)
+        self.device = self.sims[0].device
+        self.stream = torch.cuda.Stream(device=self.device)
+        self.exact = bool(self.sims[0].exact_arithmetic)
+        self.has_noise = self.sims[0].model_error is not None
+        self.handle = None
+        self._graph_steps = 0
+        self._dts = None
+        self._draws = None
+        if autotune:
+            chunk_rows, waves_per_simd = Autotuner.tune(self)
+        self.chunk_rows = int(chunk_rows or 0)
+        self.waves_per_simd = int(waves_per_simd or 0)
+        self._create()
+
+    # ----------------------------------------------------------------- set-up
+    @staticmethod
+    def eligible(sims):
+        if len(sims) == 0:
+            return False, "no members"
+        s0 = sims[0]
+        for s in sims:
+            bc = s.boundary_conditions
+            if int(s.rk_order) != 2:
+                return False, "RK order other than 2"
+            if any(int(c) not in (1, 2) for c in (bc.north, bc.east, bc.south, bc.west)):
+                return False, "boundaries other than wall and periodic"
+            if s._has_wind or s._has_pressure:
+                return False, "wind or pressure forcing"
+            if s.hasDrifters:
+                return False, "drifters attached"
+            if (s.model_error is None) != (s0.model_error is None):
+                return False, "some members carry model error and some do not"
+            if s.device != s0.device or bool(s.exact_arithmetic) != bool(s0.exact_arithmetic):
+                return False, "members on different devices or arithmetic modes"
+            p, q = s.params, s0.params
+            if (p.has_land, p.f_is_array, p.angle_is_zero) != (q.has_land, q.f_is_array, q.angle_is_zero):
+                return False, "members differ in land, coriolis texture or north angle"
+        return True, ""
+
+    def _member_table(self):
+        arr = (L.Member * self.n)()
+        for i, s in enumerate(self.sims):
+            m = arr[i]
+            m.params = s.params
+            m.dt = float(np.float32(s.dt))
+            m.h0_eta, m.h0_hu, m.h0_hv = (s.h0[k].data_ptr() for k in range(3))
+            m.h1_eta, m.h1_hu, m.h1_hv = (s.h1[k].data_ptr() for k in range(3))
+            m.pitch = s.h0[0].pitch
+            m.Hi, m.Hi_pitch = s.Hi.data_ptr(), s.Hi.pitch
+            m.f_cells = None if s.f_cells is None else s.f_cells.data_ptr()
+            m.boundary_conditions = s.boundary_conditions.packed()
+            me = s.model_error
+            m.noise = int(me is not None)
+            if me is not None:
+                rows, cols = me.random_numbers.shape
+                m.xi, m.xi_rows, m.xi_cols, m.xi_pitch = me.random_numbers.data_ptr(), rows, cols, 4 * cols
+                m.coarse, m.coarse_pitch = me.coarse_buffer.data_ptr(), 4 * (int(me.coarse_nx) + 4)
+                m.seed, m.stream_id, m.draw = me.seed, me.stream_id, me.draw
+                W = me.weights(1.0).reshape(-1)
+                for k in range(25):
+                    m.soar_weights[k] = float(W[k])
+                m.perturbation_scale = 1.0
+                m.g = float(np.float32(s.g))
+                m.f = float(np.float32(s.f)) if np.asarray(s.f).size == 1 else 0.0
+                m.beta = float(np.float32(s.coriolis_beta))
+        return arr
+
+    def _create(self):
+        self.destroy()
+        h = ctypes.c_void_p()
+        self._sync_in()
+        L.check(L.lib().gpo_ensemble_create(self._member_table(), self.n, self.chunk_rows, self.waves_per_simd,
+                                            int(self.exact), ctypes.byref(h), self._sh()), "gpo_ensemble_create")
+        self.handle = h
+        self._graph_steps = 0
+        self._dts = [np.float32(s.dt) for s in self.sims]
+        self._draws = [s.model_error.draw if self.has_noise else 0 for s in self.sims]
+
+    def _sh(self):
+        return ctypes.c_void_p(self.stream.cuda_stream)
+
+    def _sync_in(self):
+        for s in self.sims:
+            self.stream.wait_stream(s.gpu_stream)
+
+    def _sync_out(self):
+        for s in self.sims:
+            s.gpu_stream.wait_stream(self.stream)
+
+    def _refresh(self):
+        """Push members' dt and draw index to the device tables if they changed."""
+        dts = [np.float32(s.dt) for s in self.sims]
+        draws = [s.model_error.draw if self.has_noise else 0 for s in self.sims]
+        new_dt = dts != self._dts
+        new_draw = draws != self._draws
+        if new_dt or new_draw:
+            dt_arr = (ctypes.c_float * self.n)(*[float(d) for d in dts]) if new_dt else None
+            dr_arr = (ctypes.c_uint32 * self.n)(*draws) if new_draw else None
+            L.check(L.lib().gpo_ensemble_update(self.handle, dt_arr, dr_arr, self._sh()), "gpo_ensemble_update")
+            self._dts, self._draws = dts, draws
+
+    def info(self):
+        nw, nb = ctypes.c_int32(), ctypes.c_int32()
+        L.check(L.lib().gpo_ensemble_info(self.handle, ctypes.byref(nw), ctypes.byref(nb)), "gpo_ensemble_info")
+        return nw.value, nb.value
+
+    def device_draws(self):
+        out = (ctypes.c_uint32 * self.n)()
+        L.check(L.lib().gpo_ensemble_draws(self.handle, out, self._sh()), "gpo_ensemble_draws")
+        return list(out)
+
+    # --------------------------------------------------------------- stepping
+    def launch(self, n_steps, graph_steps=0):
+        """Enqueue n_steps steps on the ensemble stream, with no host
+        bookkeeping (members' t, draw and ghost cells are not updated)."""
+        if graph_steps and graph_steps > 0:
+            if self._graph_steps != graph_steps:
+                L.check(L.lib().gpo_ensemble_capture(self.handle, int(graph_steps), self._sh()),
+                        "gpo_ensemble_capture")
+                self._graph_steps = graph_steps
+            reps, rest = divmod(int(n_steps), int(graph_steps))
+            if reps:
+                L.check(L.lib().gpo_ensemble_replay(self.handle, reps, self._sh()), "gpo_ensemble_replay")
+        else:
+            rest = int(n_steps)
+        if rest:
+            L.check(L.lib().gpo_ensemble_step(self.handle, rest, self._sh()), "gpo_ensemble_step")
+
+    def step(self, n_steps, graph_steps=0):
+        """n_steps steps of every member, each with its own dt; then the host
+        bookkeeping of the per-member path (time, iteration count, draw index,
+        ghost cells)."""
+        if n_steps <= 0:
+            return
+        self._sync_in()
+        self._refresh()
+        self.launch(n_steps, graph_steps)
+        self._draws = [d + n_steps for d in self._draws] if self.has_noise else self._draws
+        self._sync_out()
+        for s, dt in zip(self.sims, self._dts):
+            s._after_batched_steps(int(n_steps), dt)
+
+    def destroy(self):
+        if self.handle is not None:
+            torch.cuda.synchronize(self.device)
+            L.lib().gpo_ensemble_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class Autotuner:
+    """Picks the work-item height (rows per wavefront) and the persistent grid
+    size (waves per SIMD) of a batched ensemble by timing a few steps of each
+    candidate on the ensemble's own members.  The members' states and draw
+    indices are restored afterwards.  Results are cached per set of member
+    shapes."""
+
+    CHUNK_ROWS = (4, 8, 12, 16, 24, 32)
+    WAVES_PER_SIMD = (2, 4, 8)
+    _cache = {}
+
+    @staticmethod
+    def key(ens):
+        shapes = tuple(sorted((int(s.nx), int(s.ny)) for s in ens.sims))
+        return (ens.exact, ens.has_noise, shapes)
+
+    @classmethod
+    def tune(cls, ens, steps=4, verbose=False):
+        k = cls.key(ens)
+        if k in cls._cache:
+            return cls._cache[k]
+        ens._sync_in()
+        with torch.cuda.stream(ens.stream):
+            saved = [[a.buf.clone() for a in s.h0 + s.h1] for s in ens.sims]
+        draws0 = [s.model_error.draw if ens.has_noise else 0 for s in ens.sims]
+        best, best_t = (0, 0), float("inf")
+        try:
+            for cr in cls.CHUNK_ROWS:
+                for w in cls.WAVES_PER_SIMD:
+                    ens.chunk_rows, ens.waves_per_simd = cr, w
+                    ens._create()
+                    ens.launch(1)
+                    t0 = torch.cuda.Event(enable_timing=True)
+                    t1 = torch.cuda.Event(enable_timing=True)
+                    t0.record(ens.stream)
+                    ens.launch(steps)
+                    t1.record(ens.stream)
+                    t1.synchronize()
+                    ms = t0.elapsed_time(t1) / steps
+                    if verbose:
+                        print(f"autotune chunk_rows={cr} waves_per_simd={w}: {ms:.3f} ms/step", flush=True)
+                    if ms < best_t:
+                        best, best_t = (cr, w), ms
+        finally:
+            with torch.cuda.stream(ens.stream):
+                for s, bufs in zip(ens.sims, saved):
+                    for a, b in zip(s.h0 + s.h1, bufs):
+                        a.buf.copy_(b)
+            ens.stream.synchronize()
+            if ens.has_noise:
+                for s, d in zip(ens.sims, draws0):
+                    s.model_error.draw = d
+            ens.destroy()
+        cls._cache[k] = best
+        return best
+
+
+class OceanModelEnsemble:
+    """numParticles CDKLM16 members on one GPU (OceanModelEnsemble.py:34-218)."""
+
+    def __init__(self, gpu_ctx, sim_args, data_args, model_error_args, numParticles,
+                 observation_variance=0.01 ** 2,
+                 initialization_variance_factor_ocean_field=0.0,
+                 super_dir_name=None, netcdf_filename=None,
+                 rank=0, batched=True, autotune=False, graph_steps=0):
+        self.logger = logging.getLogger(__name__)
+        self.gpu_ctx = gpu_ctx
+        self.sim_args = sim_args
+        self.data_args = data_args
+        self.numParticles = numParticles
+        self.observation_variance = observation_variance
+        self.initialization_variance_factor_ocean_field = initialization_variance_factor_ocean_field
+        if np.isscalar(self.observation_variance):
+            self.observation_cov = np.eye(2) * self.observation_variance
+            self.observation_cov_inverse = np.eye(2) * (1.0 / self.observation_variance)
+        else:
+            self.observation_cov = self.observation_variance
+            self.observation_cov_inverse = np.linalg.inv(self.observation_cov)
+
+        per_member = isinstance(data_args, (list, tuple))
+        if per_member:
+            assert len(data_args) == numParticles, "one data_args dict per member"
+        self.particles = [None] * numParticles
+        self.particleInfos = [None] * numParticles
+        self.drifterForecast = [None] * numParticles
+        for i in range(numParticles):
+            d = data_args[i] if per_member else data_args
+            self.particles[i] = CDKLM16(self.gpu_ctx, **self.sim_args, **d, local_particle_id=i,
+                                        super_dir_name=super_dir_name, netcdf_filename=netcdf_filename)
+            if 'small_scale_perturbation_amplitude' in model_error_args:
+                self.particles[i].setSOARModelError(**model_error_args)
+            elif 'kl_scaling' in model_error_args:
+                raise NotImplementedError("KL model error is not part of gpuocean_amd")
+            if self.initialization_variance_factor_ocean_field != 0.0:
+                self.particles[i].perturbState(q0_scale=self.initialization_variance_factor_ocean_field)
+
+        self.nx, self.ny = self.particles[0].nx, self.particles[0].ny
+        self.dx, self.dy = self.particles[0].dx, self.particles[0].dy
+        self.t = self.particles[0].t
+        self.particlesActive = [True] * numParticles
+
+        self.batched = batched
+        self.autotune = autotune
+        self.graph_steps = graph_steps
+        self._ensemble = None
+
+    def getNumParticles(self):
+        return self.numParticles
+
+    def _batched_ensemble(self):
+        if not self.batched:
+            return None
+        if self._ensemble is None:
+            ok, why = BatchedEnsemble.eligible(self.particles)
+            if not ok:
+                self.logger.debug("stepping members one by one: %s", why)
+                return None
+            self._ensemble = BatchedEnsemble(self.particles, autotune=self.autotune)
+        return self._ensemble
+
+    def attachDrifters(self, drifter_positions):
+        if self._ensemble is not None:
+            self._ensemble.destroy()
+            self._ensemble = None
+        for i in range(self.numParticles):
+            if len(drifter_positions) > 0:
+                p = self.particles[i]
+                drifters = GPUDrifterCollection(self.gpu_ctx, len(drifter_positions),
+                                                observation_variance=self.observation_variance,
+                                                boundaryConditions=p.boundary_conditions,
+                                                domain_size_x=p.nx * p.dx, domain_size_y=p.ny * p.dy,
+                                                gpu_stream=p.gpu_stream)
+                drifters.setDrifterPositions(drifter_positions)
+                p.attachDrifters(drifters)
+        self.batched = self.batched and len(drifter_positions) == 0
+
+    def cleanUp(self):
+        if self._ensemble is not None:
+            self._ensemble.destroy()
+            self._ensemble = None
+        for p in self.particles:
+            if p is not None:
+                p.cleanUp(do_gc=False)
+
+    def stepToObservation(self, observation_time, write_now=False):
+        """dataAssimilationStep on every active member (OceanModelEnsemble.py:121-134)."""
+        for p in range(self.numParticles):
+            if self.particlesActive[p]:
+                self.particles[p].dataAssimilationStep(observation_time, write_now=write_now)
+        self.t = observation_time
+
+    def modelStep(self, sub_t, rank="", update_dt=True):
+        """Every member steps sub_t seconds, then (update_dt) takes a new dt
+        (OceanModelEnsemble.py:138-150).  The leading full-dt steps common to
+        all members run batched; each member's remaining steps (a shorter last
+        step, or extra steps of a member with a smaller dt) run through its own
+        per-step path."""
+        ens = self._batched_ensemble()
+        if ens is None:
+            for p in self.particles:
+                self.t = p.step(sub_t)
+                if update_dt:
+                    p.updateDt()
+            return self.t
+        seqs = [p.local_dts(sub_t) for p in self.particles]
+        n_full = []
+        for p, seq in zip(self.particles, seqs):
+            full = np.float32(p.dt)
+            k = 0
+            while k < len(seq) and seq[k] == full:
+                k += 1
+            n_full.append(k)
+        k_common = min(n_full)
+        for p in self.particles:
+            if p.t == 0:
+                p._update_bc_values()
+                p._apply_bc(p.h0)
+        ens.step(k_common, graph_steps=self.graph_steps)
+        for p, seq in zip(self.particles, seqs):
+            if len(seq) > k_common:
+                p.step_dts(seq[k_common:])
+            self.t = p.t
+        if update_dt:
+            self.updateDt()
+        return self.t
+
+    def updateDt(self, courant_number=None):
+        """New dt for every member from one batched set of CFL reductions and a
+        single read-back (replaces the per-member blocking updateDt loop)."""
+        n = self.numParticles
+        dev = self.particles[0].device
+        out = torch.empty(n, dtype=torch.float32, device=dev)
+        stream = torch.cuda.Stream(device=dev)
+        sh = ctypes.c_void_p(stream.cuda_stream)
+        for p in self.particles:
+            stream.wait_stream(p.gpu_stream)
+        lib = L.lib()
+        with torch.cuda.stream(stream):
+            for i, p in enumerate(self.particles):
+                L.check(lib.gpo_max_dt(int(p.nx), int(p.ny), float(p.dx), float(p.dy), float(p.g),
+                                       float(p.mask_value), _vp(p.h0[0]), _vp(p.h0[1]), _vp(p.h0[2]), p.h0[0].pitch,
+                                       _vp(p.Bm), p.Bm.pitch, ctypes.c_void_p(out.data_ptr() + 4 * i), sh),
+                        "gpo_max_dt")
+            host = out.cpu().numpy()
+        for i, p in enumerate(self.particles):
+            c = p.courant_number if courant_number is None else courant_number
+            p.dt = c * host[i:i + 1][0]
+
+    def observeParticles(self, drifter_positions):
+        """hu, hv at the cells holding the given positions, per member:
+        (numParticles, num_drifters, 2) (OceanModelEnsemble.py:168-198)."""
+        drifter_positions = np.asarray(drifter_positions)
+        out = np.empty((self.numParticles, len(drifter_positions), 2))
+        for p in range(self.numParticles):
+            sim = self.particles[p]
+            eta, hu, hv = sim.download(interior_domain_only=True)
+            for d in range(len(drifter_positions)):
+                id_x = int(np.floor(drifter_positions[d, 0] / sim.dx))
+                id_y = int(np.floor(drifter_positions[d, 1] / sim.dy))
+                out[p, d, 0] = hu[id_y, id_x]
+                out[p, d, 1] = hv[id_y, id_x]
+        return out
+
+    def syncGPU(self):
+        for p in self.particles:
+            p.gpu_stream.synchronize()

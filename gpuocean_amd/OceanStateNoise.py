@@ -131,6 +131,15 @@ class OceanStateNoise:
                                perturbation_scale=perturbation_scale, perpendicular_scale=perpendicular_scale,
                                land_mask_value=sim.mask_value, north=sim.params)
 
+    def perturbSimSimilarAs(self, simToPerturb, modelError=None, perturbation_scale=1.0):
+        """Perturb simToPerturb with modelError's current random field (the
+        SOAR counterpart of ModelErrorKL.perturbSimSimilarAs, ModelErrorKL.py:450)."""
+        assert modelError.random_numbers.shape == self.random_numbers.shape
+        modelError.gpu_stream.synchronize()
+        with torch.cuda.stream(self.gpu_stream):
+            self.random_numbers.copy_(modelError.random_numbers)
+        self.perturbSim(simToPerturb, update_random_field=False, perturbation_scale=perturbation_scale)
+
     def perturbOceanState(self, eta, hu, hv, H, f, beta=0.0, g=9.81, y0_reference_cell=0, ghost_cells_x=0,
                           ghost_cells_y=0, q0_scale=1.0, update_random_field=True, perturbation_scale=1.0,
                           perpendicular_scale=0.0, align_with_cell_i=None, align_with_cell_j=None,

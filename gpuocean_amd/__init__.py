@@ -9,5 +9,7 @@ from .CDKLM16 import CDKLM16  # noqa: F401
 from .forcing import AtmosphericPressure, WindStress  # noqa: F401
 from .GPUDrifterCollection import GPUDrifterCollection  # noqa: F401
 from .OceanStateNoise import OceanStateNoise  # noqa: F401
+from .OceanModelEnsemble import Autotuner, BatchedEnsemble, OceanModelEnsemble  # noqa: F401
 
-__all__ = ["CDKLM16", "Common", "WindStress", "AtmosphericPressure", "GPUDrifterCollection", "OceanStateNoise"]
+__all__ = ["CDKLM16", "Common", "WindStress", "AtmosphericPressure", "GPUDrifterCollection", "OceanStateNoise",
+           "OceanModelEnsemble", "BatchedEnsemble", "Autotuner"]

@@ -40,6 +40,21 @@ class Forcing(ctypes.Structure):
     ]
 
 
+class Member(ctypes.Structure):
+    """gpo_member: one member of a batched ensemble."""
+    _fields_ = [
+        ("params", CDKLM16Params), ("dt", c_float),
+        ("h0_eta", c_void_p), ("h0_hu", c_void_p), ("h0_hv", c_void_p),
+        ("h1_eta", c_void_p), ("h1_hu", c_void_p), ("h1_hv", c_void_p),
+        ("pitch", c_int32), ("Hi", c_void_p), ("Hi_pitch", c_int32), ("f_cells", c_void_p),
+        ("boundary_conditions", c_uint32),
+        ("noise", c_int32), ("xi", c_void_p), ("xi_rows", c_int32), ("xi_cols", c_int32), ("xi_pitch", c_int32),
+        ("coarse", c_void_p), ("coarse_pitch", c_int32), ("seed", ctypes.c_uint64), ("stream_id", c_uint32), ("draw", c_uint32),
+        ("soar_weights", c_float * 25), ("perturbation_scale", c_float),
+        ("g", c_float), ("f", c_float), ("beta", c_float),
+    ]
+
+
 class GpoError(RuntimeError):
     pass
 
@@ -78,6 +93,16 @@ _SIGNATURES = [
      [c_int32, c_int32, c_float, c_float, c_float, c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_int32,
       c_void_p, c_int32, c_int32, c_int32, c_int32, c_void_p, c_int32, c_float, c_void_p, c_void_p, c_void_p,
       c_void_p, c_int32, c_int32, c_float, c_float, c_void_p]),
+    ("gpo_ensemble_create", c_int32,
+     [ctypes.POINTER(Member), c_int32, c_int32, c_int32, c_int32, ctypes.POINTER(c_void_p), c_void_p]),
+    ("gpo_ensemble_update", c_int32, [c_void_p, c_fp, ctypes.POINTER(c_uint32), c_void_p]),
+    ("gpo_ensemble_info", c_int32, [c_void_p, ctypes.POINTER(c_int32), ctypes.POINTER(c_int32)]),
+    ("gpo_ensemble_step", c_int32, [c_void_p, c_int32, c_void_p]),
+    ("gpo_ensemble_part", c_int32, [c_void_p, c_int32, c_void_p]),
+    ("gpo_ensemble_capture", c_int32, [c_void_p, c_int32, c_void_p]),
+    ("gpo_ensemble_replay", c_int32, [c_void_p, c_int32, c_void_p]),
+    ("gpo_ensemble_draws", c_int32, [c_void_p, ctypes.POINTER(c_uint32), c_void_p]),
+    ("gpo_ensemble_destroy", c_int32, [c_void_p]),
     ("gpo_last_error", ctypes.c_char_p, []),
     ("gpo_version", ctypes.c_char_p, []),
 ]

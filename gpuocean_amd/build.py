@@ -30,7 +30,7 @@ UNITS = [
     ("cdklm_step_exact.hip", ["-ffp-contract=off"]),
     ("aux_kernels.hip", ["-ffp-contract=off"]),
     ("noise_drift.hip", ["-ffp-contract=off"]),
-    ("ensemble.hip", ["-ffp-contract=fast"]),
+    ("ensemble.hip", ["-ffp-contract=off"]),
 ]
 
 

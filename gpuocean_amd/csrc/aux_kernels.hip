@@ -310,15 +310,9 @@ int bad(const char* msg) {
 }  // namespace
 }  // namespace gpo
 
-using namespace gpo;
+namespace gpo {
 
-extern "C" {
-
-const char* gpo_last_error(void) { return g_last_error.c_str(); }
-
-const char* gpo_version(void) { return "gpuocean_amd 0.1 (gfx950)"; }
-
-static int make_step_args(const gpo_cdklm16_params* p, float dt, int32_t rk_step, const float* in_eta,
+int make_step_args(const gpo_cdklm16_params* p, float dt, int32_t rk_step, const float* in_eta,
                           const float* in_hu, const float* in_hv, const float* qn_eta, const float* qn_hu,
                           const float* qn_hv, float* out_eta, float* out_hu, float* out_hv, int32_t pitch,
                           const float* Hi, int32_t Hi_pitch, const float* f_cells, const gpo_forcing* forcing,
@@ -400,6 +394,16 @@ static int make_step_args(const gpo_cdklm16_params* p, float dt, int32_t rk_step
   }
   return 0;
 }
+
+}  // namespace gpo
+
+using namespace gpo;
+
+extern "C" {
+
+const char* gpo_last_error(void) { return g_last_error.c_str(); }
+
+const char* gpo_version(void) { return "gpuocean_amd 0.1 (gfx950)"; }
 
 static hipError_t launch_step(const gpo_cdklm16_params* p, const StepArgs& a, int exact, hipStream_t s) {
   return exact ? launch_cdklm_exact(a, p->has_land, p->f_is_array, p->angle_is_zero, s)

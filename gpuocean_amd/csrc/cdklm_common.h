@@ -57,6 +57,17 @@ struct StepArgs {
 hipError_t launch_cdklm_fast(const StepArgs& a, int has_land, int f_array, int angle0, hipStream_t s);
 hipError_t launch_cdklm_exact(const StepArgs& a, int has_land, int f_array, int angle0, hipStream_t s);
 
+hipError_t launch_cdklm_batched_fast(const StepArgs* members, const int2* work, int n_work, int rk_step,
+                                     int n_blocks, int has_land, int f_array, int angle0, hipStream_t s);
+hipError_t launch_cdklm_batched_exact(const StepArgs* members, const int2* work, int n_work, int rk_step,
+                                      int n_blocks, int has_land, int f_array, int angle0, hipStream_t s);
+
+// Builds the argument block of one substep (aux_kernels.hip); returns 0 or an error code.
+int make_step_args(const gpo_cdklm16_params* p, float dt, int32_t rk_step, const float* in_eta, const float* in_hu,
+                   const float* in_hv, const float* qn_eta, const float* qn_hu, const float* qn_hv, float* out_eta,
+                   float* out_hu, float* out_hv, int32_t pitch, const float* Hi, int32_t Hi_pitch,
+                   const float* f_cells, const gpo_forcing* forcing, uint32_t boundary_conditions, StepArgs* out);
+
 void set_error(const char* msg);
 
 }  // namespace gpo

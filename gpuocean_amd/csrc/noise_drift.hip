@@ -6,6 +6,7 @@
 #include <cstdio>
 
 #include "cdklm_common.h"
+#include "noise_cells.h"
 #include "philox.h"
 
 namespace gpo {
@@ -54,17 +55,7 @@ __global__ void soar(int nx, int ny, SoarW W, float scale, int per_ns, int per_e
   const int ti = blockIdx.x * blockDim.x + threadIdx.x;
   const int tj = blockIdx.y * blockDim.y + threadIdx.y;
   if (ti >= nx + 4 || tj >= ny + 4) return;
-  float acc = 0.0f;
-#pragma unroll
-  for (int k = -2; k <= 2; ++k) {
-    const int yj = per_ns ? (tj - 2 + k + 2 * ny) % ny : tj + 2 + k;
-    const float* row = rowp(xi, xi_pitch, yj);
-#pragma unroll
-    for (int l = -2; l <= 2; ++l) {
-      const int xi_i = per_ew ? (ti - 2 + l + 2 * nx) % nx : ti + 2 + l;
-      acc = __builtin_fmaf(W.w[(k + 2) * 5 + (l + 2)], row[xi_i], acc);
-    }
-  }
+  const float acc = soar_cell(nx, ny, W.w, per_ns, per_ew, xi, xi_pitch, tj, ti);
   float* out = rowp(coarse, c_pitch, tj) + ti;
   if (additive)
     *out += scale * acc;
@@ -81,31 +72,8 @@ __global__ void geo_balance(int nx, int ny, float dx, float dy, float g, float f
   const int ti = blockIdx.x * blockDim.x + threadIdx.x + 2;
   const int tj = blockIdx.y * blockDim.y + threadIdx.y + 2;
   if (ti >= nx + 2 || tj >= ny + 2) return;
-  const float h00 = rowp(Hi, h_pitch, tj)[ti], h01 = rowp(Hi, h_pitch, tj)[ti + 1];
-  const float h10 = rowp(Hi, h_pitch, tj + 1)[ti], h11 = rowp(Hi, h_pitch, tj + 1)[ti + 1];
-  const bool dry = (h00 == land_value) || (h01 == land_value) || (h10 == land_value) || (h11 == land_value);
-  const float H_mid = 0.25f * (h00 + h01 + h10 + h11);
-  float coriolis = f + beta * (((float)ti + 0.5f) * dx * north_x + ((float)tj + 0.5f) * dy * north_y);
-  if (f == 0.0f) coriolis = f_cells ? rowp(f_cells, f_pitch, tj)[ti] : 0.0f;
-  const float d = rowp(coarse, c_pitch, tj)[ti];
-  float* er = rowp(eta, pitch, tj);
-  float* ur = rowp(hu, pitch, tj);
-  float* vr = rowp(hv, pitch, tj);
-  const float h_mid = d + H_mid + er[ti];
-  const float ddx = (rowp(coarse, c_pitch, tj)[ti + 1] - rowp(coarse, c_pitch, tj)[ti - 1]) / (2.0f * dx);
-  const float ddy = (rowp(coarse, c_pitch, tj + 1)[ti] - rowp(coarse, c_pitch, tj - 1)[ti]) / (2.0f * dy);
-  const float gc = g / coriolis;
-  const float d_hu = -gc * h_mid * ddy;
-  const float d_hv = gc * h_mid * ddx;
-  float e = er[ti] + d, u = ur[ti] + d_hu, v = vr[ti] + d_hv;
-  if (dry) {
-    e = 0.0f;
-    u = 0.0f;
-    v = 0.0f;
-  }
-  er[ti] = e;
-  ur[ti] = u;
-  vr[ti] = v;
+  balance_cell(dx, dy, g, f, beta, north_x, north_y, f_cells, f_pitch, coarse, c_pitch, eta, hu, hv, pitch, Hi,
+               h_pitch, land_value, tj, ti);
 }
 
 __device__ float bilin1(const float* data, int dnx, int dny, float sx, float sy) {

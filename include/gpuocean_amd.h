@@ -183,6 +183,83 @@ int gpo_drift(int32_t nx, int32_t ny, float dx, float dy, float dt, int32_t x0re
               const float* wind_y_next, int32_t wind_nx, int32_t wind_ny, float wind_t, float wind_drift_factor,
               void* stream);
 
+/* ---- batched ensemble ------------------------------------------------------- */
+
+/* One member of a batched ensemble: its parameter block (after
+ * gpo_cdklm16_prepare), time step, double-buffered state and, when the
+ * ensemble carries SOAR model error, the buffers and constants that
+ * OceanStateNoise.perturbSim would use for it.  Device pointers; pitches in
+ * bytes. */
+typedef struct {
+  gpo_cdklm16_params params;
+  float dt;
+  float *h0_eta, *h0_hu, *h0_hv;   /* Q^n in, Q^{n+1} out */
+  float *h1_eta, *h1_hu, *h1_hv;   /* RK stage */
+  int32_t pitch;
+  const float* Hi;
+  int32_t Hi_pitch;
+  const float* f_cells;            /* per-cell f (pitch = pitch) or NULL */
+  uint32_t boundary_conditions;    /* walls (1) and periodic (2) only */
+  /* model error (noise != 0) */
+  int32_t noise;
+  float* xi;                       /* standard normal field, xi_rows x xi_cols */
+  int32_t xi_rows, xi_cols, xi_pitch;
+  float* coarse;                   /* (ny+4) x (nx+4) SOAR field */
+  int32_t coarse_pitch;
+  uint64_t seed;
+  uint32_t stream_id;
+  uint32_t draw;                   /* index of the member's next random field */
+  float soar_weights[25];
+  float perturbation_scale;
+  float g, f, beta;                /* as passed to gpo_geostrophic_balance */
+} gpo_member;
+
+typedef void* gpo_ensemble;
+
+/* Builds a batched ensemble over n RK2 members of any sizes: one persistent
+ * launch per RK substep walks a table of (member, strip, row chunk) items,
+ * followed, with model error, by one batched launch each of the normal field,
+ * SOAR and geostrophic balance over all members.  Each member's draw index
+ * lives on the device and advances once per step, so replays of a captured
+ * graph draw fresh random numbers.  Results are bit for
+ * bit those of the per-member sequence gpo_cdklm16_substep x2, gpo_random_normal,
+ * gpo_soar, gpo_geostrophic_balance.  chunk_rows (0 = each member's own) and
+ * waves_per_simd (0 = 4) set the work-item height and the persistent grid.
+ * Allocates the device tables; synchronises `stream`.
+ * Replaces the member loop of OceanModelEnsemble.modelStep
+ * (ensembles/OceanModelEnsemble.py:138-150, 331-343). */
+int gpo_ensemble_create(const gpo_member* members, int32_t n, int32_t chunk_rows, int32_t waves_per_simd,
+                        int32_t exact, gpo_ensemble* out, void* stream);
+
+/* New time steps (dts: n host floats) and/or draw indices (draws: n host
+ * uint32) of every member; either may be NULL.  Synchronises `stream`.  A
+ * captured graph stays valid (it reads both tables at run time). */
+int gpo_ensemble_update(gpo_ensemble e, const float* dts, const uint32_t* draws, void* stream);
+
+/* Number of work items and persistent workgroups of an ensemble. */
+int gpo_ensemble_info(gpo_ensemble e, int32_t* n_work, int32_t* n_blocks);
+
+/* n_steps steps of every member, launched directly on `stream`.  Ghost cells
+ * are not written (apply gpo_boundary_conditions afterwards). */
+int gpo_ensemble_step(gpo_ensemble e, int32_t n_steps, void* stream);
+
+/* One part of a step, for timing and profiling: part 0 and 1 = RK substep 0
+ * and 1 of every member, part 2 = the model error of every member (random
+ * field, SOAR, balance; nothing without model error).  Parts 0, 1, 2 in this
+ * order make one gpo_ensemble_step. */
+int gpo_ensemble_part(gpo_ensemble e, int32_t part, void* stream);
+
+/* Captures steps_per_graph steps on `stream` into a hipGraph (replacing any
+ * earlier one); gpo_ensemble_replay launches it n_replays times. */
+int gpo_ensemble_capture(gpo_ensemble e, int32_t steps_per_graph, void* stream);
+int gpo_ensemble_replay(gpo_ensemble e, int32_t n_replays, void* stream);
+
+/* Draw index of every member into draws[n] (host; synchronises `stream`). */
+int gpo_ensemble_draws(gpo_ensemble e, uint32_t* draws, void* stream);
+
+/* Frees the device tables and the graph. */
+int gpo_ensemble_destroy(gpo_ensemble e);
+
 /* Text of the last error on this thread. */
 const char* gpo_last_error(void);
 

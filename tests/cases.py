@@ -176,3 +176,42 @@ def config_c1(nx=2048, ny=1024, seed=1):
     hv0 = np.zeros_like(eta0)
     return dict(eta0=eta0, hu0=hu0, hv0=hv0, H=H, nx=nx, ny=ny, dx=dx, dy=dy, dt=0.0, g=g, f=f, r=0.0,
                 courant_number=0.7, boundary_conditions=Common.BoundaryConditions(1, 2, 1, 2))
+
+
+def c2_member_sizes(n_members=256, seed=2024):
+    """Seeded uneven member sizes: nx = 16*round(U{128..512}/16),
+    ny = 16*round(U{128..384}/16)."""
+    rng = np.random.default_rng(seed)
+    nx = 16 * np.round(rng.integers(128, 513, n_members) / 16).astype(int)
+    ny = 16 * np.round(rng.integers(128, 385, n_members) / 16).astype(int)
+    return [(int(a), int(b)) for a, b in zip(nx, ny)]
+
+
+def config_c2_member(nx, ny, seed):
+    """The double jet of config_c1 on an nx x ny member, with every length
+    scaled by the member's Ly / 1024 km: jet width 50 km * s, bathymetry
+    smoothing 20 cells * s (at least 2 cells); dx=dy=1 km, periodic in x."""
+    dx = dy = 1000.0
+    g, f = 9.81, 1e-4
+    s = ny / 1024.0
+    p = smooth_random_field((ny + 5, nx + 5), max(2.0, 20.0 * s), seed)
+    H = (250.0 + 30.0 * p / np.max(np.abs(p))).astype(np.float32)
+    y = (np.arange(ny + 4) - 2 + 0.5) * dy
+    Ly = ny * dy
+    y1, y2 = Ly / 4, 3 * Ly / 4
+    w = 5e4 * s
+    eta_y = 0.3 * (np.tanh((y - y1) / w) - np.tanh((y - y2) / w))
+    eta0 = np.repeat(eta_y[:, None], nx + 4, axis=1).astype(np.float32)
+    Hm = 0.25 * (H[:-1, :-1] + H[1:, :-1] + H[:-1, 1:] + H[1:, 1:])
+    deta_dy = np.gradient(eta_y, dy)
+    hu0 = (-(g / f) * (Hm + eta0) * deta_dy[:, None]).astype(np.float32)
+    hv0 = np.zeros_like(eta0)
+    return dict(eta0=eta0, hu0=hu0, hv0=hv0, H=H, nx=nx, ny=ny, dx=dx, dy=dy, dt=0.0, g=g, f=f, r=0.0,
+                courant_number=0.7, boundary_conditions=Common.BoundaryConditions(1, 2, 1, 2))
+
+
+def config_c2(n_members=256, seed=2024, sizes=None):
+    """Per-member keyword arguments of the batched stochastic ensemble:
+    uneven members, SOAR q0 = 5e-3, L = 0.75 dx, seed per member."""
+    sizes = c2_member_sizes(n_members, seed) if sizes is None else sizes
+    return [config_c2_member(nx, ny, seed + 1 + i) for i, (nx, ny) in enumerate(sizes)]
