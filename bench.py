@@ -142,7 +142,7 @@ def run_single(kw, workload, steps, warmup, fast, chunk_rows):
     gpu_ms = ev0.elapsed_time(ev1)
     value = cells * steps / wall
     log(f"[{workload}] timed: {gpu_ms:.2f} ms GPU, {wall * 1e3:.2f} ms wall for {steps} steps -> "
-        f"{1e3 * wall / steps:.2f} us/step, {value:.4g} cell-steps/s")
+        f"{1e6 * wall / steps:.2f} us/step, {value:.4g} cell-steps/s")
 
     # per-launch durations of the substep kernel, events on the kernel's stream
     n_prof = min(steps, 200)
@@ -287,12 +287,32 @@ def run_ensemble(args):
     }
 
 
+def run_copy(n_bytes=256 << 20, reps=5):
+    """A plain device-to-device copy of n_bytes, reps times: the reference
+    kernel whose FETCH_SIZE / WRITE_SIZE counters calibrate the HBM traffic
+    figures of the step kernels."""
+    import torch
+    src = torch.ones(n_bytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    ev0, ev1 = event_pair(None)
+    ev0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    ev1.record()
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / reps
+    print(json.dumps({"copy_bytes": n_bytes, "reps": reps, "ms_per_copy": ms,
+                      "GB_per_s_read_plus_write": 2 * n_bytes / (ms * 1e-3) / 1e9}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c1", "c0"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "c1", "c0", "copy"],
+                    help="copy = HBM copy of 256 MiB, the calibration kernel for the traffic counters")
     ap.add_argument("--members", type=int, default=256)
     ap.add_argument("--graph-steps", type=int, default=10)
     ap.add_argument("--seq-steps", type=int, default=10)
@@ -309,6 +329,10 @@ def main():
               "(ensemble members would be split across GPUs by member, with no exchange while stepping).",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "copy":
+        run_copy()
+        return
 
     import cases
 
