@@ -144,24 +144,28 @@ def run_single(kw, workload, steps, warmup, fast, chunk_rows):
     log(f"[{workload}] timed: {gpu_ms:.2f} ms GPU, {wall * 1e3:.2f} ms wall for {steps} steps -> "
         f"{1e6 * wall / steps:.2f} us/step, {value:.4g} cell-steps/s")
 
-    # per-launch durations of the substep kernel, events on the kernel's stream
-    n_prof = min(steps, 200)
-    evs = [torch.cuda.Event(enable_timing=True) for _ in range(2 * n_prof + 1)]
-    h0, h1 = sim.h0, sim.h1
-    evs[0].record(stream)
-    k = 1
-    for _ in range(n_prof):
-        sim._substep(h0, h1, None, dt, 0, None)
-        evs[k].record(stream)
-        sim._substep(h1, h0, h0, dt, 1, None)
-        evs[k + 1].record(stream)
-        k += 2
-    stream.synchronize()
-    t0_us = 1e3 * float(np.mean([evs[2 * i].elapsed_time(evs[2 * i + 1]) for i in range(n_prof)]))
-    t1_us = 1e3 * float(np.mean([evs[2 * i + 1].elapsed_time(evs[2 * i + 2]) for i in range(n_prof)]))
-    avg_launch_us = 0.5 * (t0_us + t1_us)
+    # The timed region holds nothing but back-to-back launches of the substep
+    # kernel on one stream: its average launch duration is the GPU time / 2K.
+    avg_launch_us = 1e3 * gpu_ms / (2 * steps)
     achieved = 0.5 * BYTES_PER_CELL_STEP * cells / (avg_launch_us * 1e-6) / 1e9
-    log(f"[{workload}] substep 0: {t0_us:.2f} us ({BYTES_SUBSTEP[0] * cells / t0_us / 1e3:.0f} GB/s), "
+    # each substep alone: blocks of back-to-back launches between two events
+    n_prof = min(steps, 200)
+    h0, h1 = sim.h0, sim.h1
+    part_us = []
+    for rk in (0, 1):
+        ev0, ev1 = event_pair(stream)
+        ev0.record(stream)
+        for _ in range(n_prof):
+            if rk == 0:
+                sim._substep(h0, h1, None, dt, 0, None)
+            else:
+                sim._substep(h1, h0, h0, dt, 1, None)
+        ev1.record(stream)
+        stream.synchronize()
+        part_us.append(1e3 * ev0.elapsed_time(ev1) / n_prof)
+    t0_us, t1_us = part_us
+    log(f"[{workload}] average launch {avg_launch_us:.2f} us ({achieved:.0f} GB/s algorithmic); "
+        f"substep 0: {t0_us:.2f} us ({BYTES_SUBSTEP[0] * cells / t0_us / 1e3:.0f} GB/s), "
         f"substep 1: {t1_us:.2f} us ({BYTES_SUBSTEP[1] * cells / t1_us / 1e3:.0f} GB/s)")
     assert np.all(np.isfinite(sim.download()[0])), "non-finite state"
     return {
@@ -200,7 +204,8 @@ def run_ensemble(args):
     log(f"[c2] {len(sims)} members, {cells / 1e6:.2f} M interior cells, built in {time.perf_counter() - t0:.1f} s")
 
     t0 = time.perf_counter()
-    ens = BatchedEnsemble(sims, autotune=not args.no_autotune, chunk_rows=args.chunk_rows or None)
+    ens = BatchedEnsemble(sims, autotune=not args.no_autotune, chunk_rows=args.chunk_rows or None,
+                          waves_per_simd=args.waves_per_simd or None)
     n_work, n_blocks = ens.info()
     log(f"[c2] chunk_rows={ens.chunk_rows} waves_per_simd={ens.waves_per_simd} work items={n_work} "
         f"workgroups={n_blocks} (tuned in {time.perf_counter() - t0:.1f} s)")
@@ -216,22 +221,24 @@ def run_ensemble(args):
     log(f"[c2] batched: {args.steps} steps in {wall * 1e3:.1f} ms -> {1e3 * wall / args.steps:.3f} ms/step, "
         f"{value:.4g} member-cell-steps/s")
 
-    # per-launch durations of each part of a step, events on the ensemble stream
+    # per-launch durations of each part of a step: blocks of back-to-back
+    # launches of one part between two events on the ensemble stream
     st = ens.stream
     sh = ctypes.c_void_p(st.cuda_stream)
     n_prof = min(args.steps, 50)
-    evs = [torch.cuda.Event(enable_timing=True) for _ in range(3 * n_prof + 1)]
     ens._sync_in()
-    evs[0].record(st)
-    for i in range(n_prof):
-        for part in range(3):
+    part_us = []
+    for part in range(3):
+        ev0, ev1 = event_pair(st)
+        ev0.record(st)
+        for _ in range(n_prof):
             L.check(L.lib().gpo_ensemble_part(ens.handle, part, sh), "gpo_ensemble_part")
-            evs[3 * i + part + 1].record(st)
-    st.synchronize()
-    part_us = [1e3 * float(np.mean([evs[3 * i + p].elapsed_time(evs[3 * i + p + 1]) for i in range(n_prof)]))
-               for p in range(3)]
-    for s in sims:   # keep the members' host bookkeeping in step with the device
-        s._after_batched_steps(n_prof, np.float32(s.dt))
+        ev1.record(st)
+        st.synchronize()
+        part_us.append(1e3 * ev0.elapsed_time(ev1) / n_prof)
+    for s in sims:   # the model error advanced every member's draw index n_prof times
+        s.model_error.draw += n_prof
+        s._apply_bc(s.h0)
     ens._draws = [d + n_prof for d in ens._draws]
     avg_launch_us = 0.5 * (part_us[0] + part_us[1])
     achieved = 0.5 * BYTES_PER_CELL_STEP * cells / (avg_launch_us * 1e-6) / 1e9
@@ -320,6 +327,7 @@ def main():
     ap.add_argument("--fast", action="store_true",
                     help="use the fast kernel build (not bit-compatible with the oracle)")
     ap.add_argument("--chunk-rows", type=int, default=0)
+    ap.add_argument("--waves-per-simd", type=int, default=0, help="persistent grid of the batched launch")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c1", action="store_true", help="skip the single-member line of the c2 run")
     args = ap.parse_args()
