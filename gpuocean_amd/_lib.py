@@ -103,6 +103,29 @@ _SIGNATURES = [
     ("gpo_ensemble_replay", c_int32, [c_void_p, c_int32, c_void_p]),
     ("gpo_ensemble_draws", c_int32, [c_void_p, ctypes.POINTER(c_uint32), c_void_p]),
     ("gpo_ensemble_destroy", c_int32, [c_void_p]),
+    ("gpo_adjoint_workspace_bytes", ctypes.c_size_t, [c_int32, c_int32, c_int32]),
+    ("gpo_tlm_workspace_bytes", ctypes.c_size_t, [c_int32, c_int32]),
+    ("gpo_cdklm16_substep_f64", c_int32,
+     [ctypes.POINTER(CDKLM16Params), ctypes.c_double, c_int32] + [c_void_p] * 9 +
+     [c_int32, c_void_p, c_int32, c_void_p, c_uint32, c_void_p, c_void_p, c_void_p]),
+    ("gpo_cdklm16_substep_tlm_f64", c_int32,
+     [ctypes.POINTER(CDKLM16Params), ctypes.c_double, c_int32] + [c_void_p] * 18 +
+     [c_int32, c_void_p, c_int32, c_void_p, c_uint32, c_void_p, c_void_p, c_void_p]),
+    ("gpo_cdklm16_substep_adj_f32", c_int32,
+     [ctypes.POINTER(CDKLM16Params), c_float, c_int32] + [c_void_p] * 13 +
+     [c_int32, c_int32, c_void_p, c_int32, c_void_p, c_uint32, c_void_p, c_void_p, c_void_p]),
+    ("gpo_cdklm16_substep_adj_f64", c_int32,
+     [ctypes.POINTER(CDKLM16Params), ctypes.c_double, c_int32] + [c_void_p] * 13 +
+     [c_int32, c_int32, c_void_p, c_int32, c_void_p, c_uint32, c_void_p, c_void_p, c_void_p]),
+    ("gpo_balance_adj_f32", c_int32,
+     [ctypes.POINTER(CDKLM16Params), c_float, c_float, c_float, c_void_p, c_int32, c_void_p, c_int32,
+      c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_int32, c_void_p]),
+    ("gpo_point_misfit_f32", c_int32,
+     [c_int32, c_int32, c_float, c_float, c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_int32, c_int32,
+      c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("gpo_point_misfit_f64", c_int32,
+     [c_int32, c_int32, ctypes.c_double, ctypes.c_double, c_void_p, c_void_p, c_void_p, c_int32, c_void_p,
+      c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     ("gpo_last_error", ctypes.c_char_p, []),
     ("gpo_version", ctypes.c_char_p, []),
 ]

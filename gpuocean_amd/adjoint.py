@@ -1,0 +1,331 @@
+"""Gradient of a point-observation misfit through CDKLM16 time steps.
+
+No counterpart exists in the reference.  ``DifferentiableCDKLM16`` takes the
+configuration of a ``CDKLM16`` simulator (RK2, wet domain, wall/periodic
+boundaries, no friction, land or forcing) and gives
+
+  * the FP32 forward run: the simulator's own step kernel (bit-compatible with
+    the host oracle), keeping a checkpoint every ``checkpoint_every`` steps;
+  * J = sum over observation points of (u - u_obs)^2 + (v - v_obs)^2, with
+    (u, v) = (hu, hv)/h interpolated bilinearly between cell centres;
+  * dJ/d(eta0, hu0, hv0) by the hand-written adjoint kernels: each segment
+    between checkpoints is recomputed once, storing every stage state, and
+    then swept backwards;
+  * an FP64 build of the forward, tangent-linear and adjoint runs, used by the
+    finite-difference and dot-product checks.
+
+``CDKLM16Misfit`` wraps the FP32 path as a ``torch.autograd.Function``.
+"""
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib as L
+
+
+def _pitch_elems(width):
+    return ((width + 63) // 64) * 64
+
+
+def _vp(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class _State:
+    """(eta, hu, hv), each (ny+4) x pitch with a padded row pitch."""
+
+    def __init__(self, ny, nx, dtype, device):
+        self.P = _pitch_elems(nx + 4)
+        self.nx, self.ny = nx, ny
+        self.t = [torch.zeros((ny + 4, self.P), dtype=dtype, device=device) for _ in range(3)]
+        self.pitch = self.P * self.t[0].element_size()
+
+    def ptrs(self):
+        return [_vp(a) for a in self.t]
+
+    def views(self):
+        return [a[:, :self.nx + 4] for a in self.t]
+
+    def copy_from(self, other):
+        for a, b in zip(self.t, other.t):
+            a.copy_(b)
+
+    def load(self, eta, hu, hv):
+        for a, b in zip(self.views(), (eta, hu, hv)):
+            a.copy_(b)
+
+    def zero_(self):
+        for a in self.t:
+            a.zero_()
+
+
+class DifferentiableCDKLM16:
+    """Misfit and its gradient for n_steps RK2 steps of fixed dt of the given
+    simulator's configuration (its bathymetry, coriolis, boundaries, dt)."""
+
+    def __init__(self, sim, positions, observed, n_steps, dt=None, checkpoint_every=20):
+        p = sim.params
+        bc = sim.boundary_conditions
+        if int(sim.rk_order) != 2 or float(sim.r) != 0.0:
+            raise ValueError("the differentiable step needs RK2 and r = 0")
+        if sim.has_land or sim._has_wind or sim._has_pressure or sim.one_dimensional:
+            raise ValueError("the differentiable step supports no land, wind, pressure or one-dimensional mode")
+        if any(int(c) not in (1, 2) for c in (bc.north, bc.east, bc.south, bc.west)):
+            raise ValueError("the differentiable step supports wall and periodic boundaries only")
+        if not sim.exact_arithmetic:
+            raise ValueError("the differentiable step runs the exact forward build")
+        self.sim = sim
+        self.params = p
+        self.nx, self.ny = int(sim.nx), int(sim.ny)
+        self.dx, self.dy = float(sim.dx), float(sim.dy)
+        self.bc = bc.packed()
+        self.dev = sim.device
+        self.stream = sim.gpu_stream
+        self.n_steps = int(n_steps)
+        self.dt = np.float32(sim.dt if dt is None else dt)
+        self.k = max(1, int(checkpoint_every))
+        pos = np.asarray(positions, np.float32).reshape(-1, 2)
+        obs = np.asarray(observed, np.float32).reshape(-1, 2)
+        assert pos.shape == obs.shape
+        self.n_obs = pos.shape[0]
+        self.pos = torch.from_numpy(pos).to(self.dev)
+        self.obs = torch.from_numpy(obs).to(self.dev)
+        self.Hi = sim.Hi
+        self.fc = sim.f_cells
+        lib = L.lib()
+        with torch.cuda.stream(self.stream):
+            self.ws32 = torch.empty(lib.gpo_adjoint_workspace_bytes(self.nx, self.ny, 4), dtype=torch.uint8,
+                                    device=self.dev)
+            self.flag = torch.zeros(1, dtype=torch.int32, device=self.dev)
+            self.J = torch.zeros(1, dtype=torch.float64, device=self.dev)
+        self._f64 = None
+        self._seg = None
+
+    # ------------------------------------------------------------------ helpers
+    def _sh(self):
+        return ctypes.c_void_p(self.stream.cuda_stream)
+
+    def _new_state(self, dtype=torch.float32):
+        with torch.cuda.stream(self.stream):
+            return _State(self.ny, self.nx, dtype, self.dev)
+
+    def _fc_ptr(self):
+        return None if self.fc is None else ctypes.c_void_p(self.fc.data_ptr())
+
+    def _substep32(self, src, dst, qn, rk):
+        L.check(L.lib().gpo_cdklm16_substep(
+            ctypes.byref(self.params), float(self.dt), rk, *src.ptrs(),
+            *(qn.ptrs() if qn is not None else [None] * 3), *dst.ptrs(), src.pitch,
+            _vp(self.Hi), self.Hi.pitch, self._fc_ptr(), None, self.bc, 1, self._sh()), "gpo_cdklm16_substep")
+
+    def _step32(self, qn, q1, out):
+        """q1 = S0(qn); out = S1(q1; qn).  out may be qn."""
+        self._substep32(qn, q1, None, 0)
+        self._substep32(q1, out, qn, 1)
+
+    def _misfit32(self, st, lam=None):
+        L.check(L.lib().gpo_point_misfit_f32(
+            self.nx, self.ny, self.dx, self.dy, *st.ptrs(), st.pitch, _vp(self.Hi), self.Hi.pitch, self.n_obs,
+            _vp(self.pos), _vp(self.obs), _vp(self.J), *(lam.ptrs() if lam is not None else [None] * 3),
+            self._sh()), "gpo_point_misfit_f32")
+
+    def _adj32(self, rk, inp, out, lam_out, lam_in, lam_qn, accumulate):
+        L.check(L.lib().gpo_cdklm16_substep_adj_f32(
+            ctypes.byref(self.params), float(self.dt), rk, *inp.ptrs(), _vp(out.t[0]), *lam_out.ptrs(),
+            *lam_in.ptrs(), *(lam_qn.ptrs() if lam_qn is not None else [None] * 3), int(accumulate), inp.pitch,
+            _vp(self.Hi), self.Hi.pitch, self._fc_ptr(), self.bc, _vp(self.ws32), _vp(self.flag), self._sh()),
+            "gpo_cdklm16_substep_adj_f32")
+
+    def flag_value(self):
+        self.stream.synchronize()
+        return int(self.flag.item())
+
+    # -------------------------------------------------------------- FP32 path
+    def forward(self, eta0, hu0, hv0):
+        """Run n_steps from (eta0, hu0, hv0) ((ny+4) x (nx+4) device float32),
+        keeping checkpoints.  Returns J (python float) and the checkpoints."""
+        s = self.stream
+        s.wait_stream(torch.cuda.current_stream(self.dev))
+        A, B = self._new_state(), self._new_state()
+        with torch.cuda.stream(s):
+            A.load(eta0, hu0, hv0)
+            ckpts = []
+            for i in range(self.n_steps):
+                if i % self.k == 0:
+                    c = self._new_state()
+                    c.copy_from(A)
+                    ckpts.append(c)
+                self._step32(A, B, A)
+            self.J.zero_()
+        self._misfit32(A)
+        self._final = A
+        self._ckpts = ckpts
+        return float(self.J.item()), A
+
+    def _segment_buffers(self):
+        if self._seg is None:
+            self._seg = ([self._new_state() for _ in range(self.k + 1)], [self._new_state() for _ in range(self.k)])
+            self._lam = [self._new_state() for _ in range(3)]
+        return self._seg
+
+    def backward(self, grad_J=1.0):
+        """dJ/d(eta0, hu0, hv0) of the last forward(), scaled by grad_J, as
+        three (ny+4) x (nx+4) float32 tensors (ghost cells zero)."""
+        s = self.stream
+        Qs, Q1s = self._segment_buffers()
+        lam, L1, X = self._lam
+        with torch.cuda.stream(s):
+            lam.zero_()
+            self.J.zero_()
+        self._misfit32(self._final, lam)
+        with torch.cuda.stream(s):
+            if grad_J != 1.0:
+                for a in lam.t:
+                    a.mul_(float(grad_J))
+        n = self.n_steps
+        starts = list(range(0, n, self.k))
+        for si in reversed(range(len(starts))):
+            i0 = starts[si]
+            m = min(self.k, n - i0)
+            with torch.cuda.stream(s):
+                Qs[0].copy_from(self._ckpts[si])
+            for j in range(m):            # recompute the segment, keeping every stage
+                self._step32(Qs[j], Q1s[j], Qs[j + 1])
+            for j in reversed(range(m)):  # sweep it backwards
+                with torch.cuda.stream(s):
+                    X.zero_()
+                self._adj32(1, Q1s[j], Qs[j + 1], lam, L1, X, 0)
+                self._adj32(0, Qs[j], Q1s[j], L1, X, None, 1)
+                lam, X = X, lam
+        self._lam = [lam, L1, X]
+        s.synchronize()
+        return [a.clone() for a in lam.views()]
+
+    # -------------------------------------------------------------- FP64 path
+    def _init_f64(self):
+        if self._f64 is not None:
+            return
+        lib = L.lib()
+        with torch.cuda.stream(self.stream):
+            Hi64 = self.Hi.buf.double()
+            fc64 = None if self.fc is None else self.fc.buf.double()
+            ws64 = torch.empty(lib.gpo_adjoint_workspace_bytes(self.nx, self.ny, 8), dtype=torch.uint8,
+                               device=self.dev)
+            wst = torch.empty(lib.gpo_tlm_workspace_bytes(self.nx, self.ny), dtype=torch.uint8, device=self.dev)
+        self._f64 = dict(Hi=Hi64, Hi_pitch=8 * self.Hi.pitch_elems, fc=fc64, ws=ws64, wst=wst)
+
+    def _fc64(self):
+        f = self._f64["fc"]
+        return None if f is None else _vp(f)
+
+    def _substep64(self, src, dst, qn, rk):
+        d = self._f64
+        L.check(L.lib().gpo_cdklm16_substep_f64(
+            ctypes.byref(self.params), float(self.dt), rk, *src.ptrs(),
+            *(qn.ptrs() if qn is not None else [None] * 3), *dst.ptrs(), src.pitch, _vp(d["Hi"]), d["Hi_pitch"],
+            self._fc64(), self.bc, _vp(d["ws"]), _vp(self.flag), self._sh()), "gpo_cdklm16_substep_f64")
+
+    def _tlm64(self, src, dsrc, dst, ddst, qn, dqn, rk):
+        d = self._f64
+        L.check(L.lib().gpo_cdklm16_substep_tlm_f64(
+            ctypes.byref(self.params), float(self.dt), rk, *src.ptrs(), *dsrc.ptrs(),
+            *(qn.ptrs() if qn is not None else [None] * 3), *(dqn.ptrs() if dqn is not None else [None] * 3),
+            *dst.ptrs(), *ddst.ptrs(), src.pitch, _vp(d["Hi"]), d["Hi_pitch"], self._fc64(), self.bc,
+            _vp(d["wst"]), _vp(self.flag), self._sh()), "gpo_cdklm16_substep_tlm_f64")
+
+    def _adj64(self, rk, inp, out, lam_out, lam_in, lam_qn, accumulate):
+        d = self._f64
+        L.check(L.lib().gpo_cdklm16_substep_adj_f64(
+            ctypes.byref(self.params), float(self.dt), rk, *inp.ptrs(), _vp(out.t[0]), *lam_out.ptrs(),
+            *lam_in.ptrs(), *(lam_qn.ptrs() if lam_qn is not None else [None] * 3), int(accumulate), inp.pitch,
+            _vp(d["Hi"]), d["Hi_pitch"], self._fc64(), self.bc, _vp(d["ws"]), _vp(self.flag), self._sh()),
+            "gpo_cdklm16_substep_adj_f64")
+
+    def _misfit64(self, st, lam=None):
+        d = self._f64
+        L.check(L.lib().gpo_point_misfit_f64(
+            self.nx, self.ny, self.dx, self.dy, *st.ptrs(), st.pitch, _vp(d["Hi"]), d["Hi_pitch"], self.n_obs,
+            _vp(self.pos), _vp(self.obs), _vp(self.J), *(lam.ptrs() if lam is not None else [None] * 3),
+            self._sh()), "gpo_point_misfit_f64")
+
+    def trajectory_f64(self, eta0, hu0, hv0, n_steps=None):
+        """FP64 states Q^0 .. Q^n and stages Q1^0 .. Q1^(n-1) (kept whole)."""
+        self._init_f64()
+        n = self.n_steps if n_steps is None else n_steps
+        Qs = [self._new_state(torch.float64) for _ in range(n + 1)]
+        Q1s = [self._new_state(torch.float64) for _ in range(n)]
+        with torch.cuda.stream(self.stream):
+            Qs[0].load(eta0, hu0, hv0)
+        for j in range(n):
+            self._substep64(Qs[j], Q1s[j], None, 0)
+            self._substep64(Q1s[j], Qs[j + 1], Qs[j], 1)
+        return Qs, Q1s
+
+    def misfit_f64(self, eta0, hu0, hv0):
+        """J of the FP64 forward run from the given (float64) initial state."""
+        self._init_f64()
+        A, B = self._new_state(torch.float64), self._new_state(torch.float64)
+        with torch.cuda.stream(self.stream):
+            A.load(eta0, hu0, hv0)
+        for _ in range(self.n_steps):
+            self._substep64(A, B, None, 0)
+            self._substep64(B, A, A, 1)
+        with torch.cuda.stream(self.stream):
+            self.J.zero_()
+        self._misfit64(A)
+        return float(self.J.item())
+
+    def adjoint_f64(self, Qs, Q1s, lam_final):
+        """lambda_0 = (dQ^n/dQ^0)^T lambda_final along an FP64 trajectory."""
+        n = len(Q1s)
+        lam, L1, X = (self._new_state(torch.float64) for _ in range(3))
+        with torch.cuda.stream(self.stream):
+            lam.copy_from(lam_final)
+        for j in reversed(range(n)):
+            with torch.cuda.stream(self.stream):
+                X.zero_()
+            self._adj64(1, Q1s[j], Qs[j + 1], lam, L1, X, 0)
+            self._adj64(0, Qs[j], Q1s[j], L1, X, None, 1)
+            lam, X = X, lam
+        return lam
+
+    def tangent_f64(self, Qs, Q1s, d0):
+        """dQ^n = (dQ^n/dQ^0) d0 along an FP64 trajectory (tangent-linear model)."""
+        n = len(Q1s)
+        out = self._new_state(torch.float64)
+        d, d1, dn = d0, self._new_state(torch.float64), self._new_state(torch.float64)
+        for j in range(n):
+            self._tlm64(Qs[j], d, out, d1, None, None, 0)
+            self._tlm64(Q1s[j], d1, out, dn, Qs[j], d, 1)
+            d = dn
+            dn = self._new_state(torch.float64)
+        return d
+
+    def gradient_f64(self, eta0, hu0, hv0):
+        """J and dJ/dQ^0 of the FP64 run (trajectory kept whole)."""
+        Qs, Q1s = self.trajectory_f64(eta0, hu0, hv0)
+        lamN = self._new_state(torch.float64)
+        with torch.cuda.stream(self.stream):
+            self.J.zero_()
+        self._misfit64(Qs[-1], lamN)
+        J = float(self.J.item())
+        lam0 = self.adjoint_f64(Qs, Q1s, lamN)
+        return J, [a.clone() for a in lam0.views()]
+
+
+class CDKLM16Misfit(torch.autograd.Function):
+    """J(eta0, hu0, hv0) of a DifferentiableCDKLM16 run as an autograd node:
+    forward = FP32 step kernel with checkpoints, backward = adjoint sweep."""
+
+    @staticmethod
+    def forward(ctx, eta0, hu0, hv0, model):
+        J, _ = model.forward(eta0.detach(), hu0.detach(), hv0.detach())
+        ctx.model = model
+        return torch.tensor(J, dtype=torch.float64, device=eta0.device)
+
+    @staticmethod
+    def backward(ctx, grad_J):
+        g = ctx.model.backward(float(grad_J.item()))
+        return g[0], g[1], g[2], None

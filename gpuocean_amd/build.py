@@ -31,6 +31,7 @@ UNITS = [
     ("aux_kernels.hip", ["-ffp-contract=off"]),
     ("noise_drift.hip", ["-ffp-contract=off"]),
     ("ensemble.hip", ["-ffp-contract=off"]),
+    ("adjoint.hip", ["-ffp-contract=off"]),
 ]
 
 

@@ -16,6 +16,7 @@
 #ifndef GPUOCEAN_AMD_H
 #define GPUOCEAN_AMD_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -259,6 +260,83 @@ int gpo_ensemble_draws(gpo_ensemble e, uint32_t* draws, void* stream);
 
 /* Frees the device tables and the graph. */
 int gpo_ensemble_destroy(gpo_ensemble e);
+
+/* ---- differentiation ------------------------------------------------------------
+ * No counterpart in the reference, which has no derivative of the step.  The
+ * differentiable substep covers RK2 substeps of wet cells (every cell
+ * h >= desing_eps), wall and periodic boundaries, r = 0, no land, wind or
+ * pressure.  Unsupported parameters are rejected; state branches outside the
+ * scope met at run time set bits of *flag (1: dry or desingularised cell,
+ * 2: depth clamp, 8: face depth clamp), flag may be NULL.  Minmod, max and
+ * min differentiate through the first minimal/maximal argument.  The stage
+ * arithmetic restates gpu_kernels/CDKLM16_kernel.cu:475-1090. */
+
+/* Workspace bytes of one differentiable substep (elem_bytes 4 or 8). */
+size_t gpo_adjoint_workspace_bytes(int32_t nx, int32_t ny, int32_t elem_bytes);
+/* Workspace bytes of one tangent-linear substep. */
+size_t gpo_tlm_workspace_bytes(int32_t nx, int32_t ny);
+
+/* FP64 substep (the verification build): same arguments as
+ * gpo_cdklm16_substep with double arrays (pitches in bytes), Hi and f_cells
+ * in double. */
+int gpo_cdklm16_substep_f64(const gpo_cdklm16_params* p, double dt, int32_t rk_step, const double* in_eta,
+                            const double* in_hu, const double* in_hv, const double* qn_eta, const double* qn_hu,
+                            const double* qn_hv, double* out_eta, double* out_hu, double* out_hv, int32_t pitch,
+                            const double* Hi, int32_t Hi_pitch, const double* f_cells, uint32_t boundary_conditions,
+                            void* workspace, int32_t* flag, void* stream);
+
+/* Tangent-linear FP64 substep: out and d_out = (d out / d (in, qn)) . (d_in, d_qn). */
+int gpo_cdklm16_substep_tlm_f64(const gpo_cdklm16_params* p, double dt, int32_t rk_step, const double* in_eta,
+                                const double* in_hu, const double* in_hv, const double* d_in_eta,
+                                const double* d_in_hu, const double* d_in_hv, const double* qn_eta,
+                                const double* qn_hu, const double* qn_hv, const double* d_qn_eta,
+                                const double* d_qn_hu, const double* d_qn_hv, double* out_eta, double* out_hu,
+                                double* out_hv, double* d_out_eta, double* d_out_hu, double* d_out_hv,
+                                int32_t pitch, const double* Hi, int32_t Hi_pitch, const double* f_cells,
+                                uint32_t boundary_conditions, void* workspace, int32_t* flag, void* stream);
+
+/* Adjoint of one substep out = S(in [, qn]): given lambda_out (interior
+ * cells) and the forward's input and output, writes (accumulate = 0) or adds
+ * (accumulate = 1) lambda_in = (dS/din)^T lambda_out on the interior cells,
+ * ghost-cell sensitivities folded onto the cells they mirror or wrap; for
+ * rk_step 1 also adds (dS/dqn)^T lambda_out to lambda_qn (may be NULL).
+ * lambda_in may alias lambda_out.  FP32 (product) and FP64 (verification). */
+int gpo_cdklm16_substep_adj_f32(const gpo_cdklm16_params* p, float dt, int32_t rk_step, const float* in_eta,
+                                const float* in_hu, const float* in_hv, const float* out_eta,
+                                const float* lam_out_eta, const float* lam_out_hu, const float* lam_out_hv,
+                                float* lam_in_eta, float* lam_in_hu, float* lam_in_hv, float* lam_qn_eta,
+                                float* lam_qn_hu, float* lam_qn_hv, int32_t accumulate, int32_t pitch,
+                                const float* Hi, int32_t Hi_pitch, const float* f_cells,
+                                uint32_t boundary_conditions, void* workspace, int32_t* flag, void* stream);
+int gpo_cdklm16_substep_adj_f64(const gpo_cdklm16_params* p, double dt, int32_t rk_step, const double* in_eta,
+                                const double* in_hu, const double* in_hv, const double* out_eta,
+                                const double* lam_out_eta, const double* lam_out_hu, const double* lam_out_hv,
+                                double* lam_in_eta, double* lam_in_hu, double* lam_in_hv, double* lam_qn_eta,
+                                double* lam_qn_hu, double* lam_qn_hv, int32_t accumulate, int32_t pitch,
+                                const double* Hi, int32_t Hi_pitch, const double* f_cells,
+                                uint32_t boundary_conditions, void* workspace, int32_t* flag, void* stream);
+
+/* Adjoint of the state dependence of gpo_geostrophic_balance for a given
+ * coarse field: lambda_eta += -g/f d_y(coarse) lambda_hu + g/f d_x(coarse)
+ * lambda_hv, in place (the noise is additive otherwise). */
+int gpo_balance_adj_f32(const gpo_cdklm16_params* p, float g, float f, float beta, const float* f_cells,
+                        int32_t f_pitch, const float* coarse, int32_t coarse_pitch, float* lam_eta,
+                        float* lam_hu, float* lam_hv, int32_t pitch, const float* Hi, int32_t Hi_pitch,
+                        void* stream);
+
+/* Point-observation misfit J += sum_d (u - u_obs)^2 + (v - v_obs)^2 with
+ * (u, v) = (hu, hv)/(eta + Hm) interpolated bilinearly between the four cell
+ * centres around positions[2d], positions[2d+1] (metres); observed holds
+ * (u_obs, v_obs) pairs.  *J (device double) is accumulated; lam_* (may be
+ * NULL) accumulate dJ/d(eta, hu, hv). */
+int gpo_point_misfit_f32(int32_t nx, int32_t ny, float dx, float dy, const float* eta, const float* hu,
+                         const float* hv, int32_t pitch, const float* Hi, int32_t Hi_pitch, int32_t n,
+                         const float* positions, const float* observed, double* J, float* lam_eta, float* lam_hu,
+                         float* lam_hv, void* stream);
+int gpo_point_misfit_f64(int32_t nx, int32_t ny, double dx, double dy, const double* eta, const double* hu,
+                         const double* hv, int32_t pitch, const double* Hi, int32_t Hi_pitch, int32_t n,
+                         const float* positions, const float* observed, double* J, double* lam_eta, double* lam_hu,
+                         double* lam_hv, void* stream);
 
 /* Text of the last error on this thread. */
 const char* gpo_last_error(void);

@@ -215,3 +215,33 @@ def config_c2(n_members=256, seed=2024, sizes=None):
     uneven members, SOAR q0 = 5e-3, L = 0.75 dx, seed per member."""
     sizes = c2_member_sizes(n_members, seed) if sizes is None else sizes
     return [config_c2_member(nx, ny, seed + 1 + i) for i, (nx, ny) in enumerate(sizes)]
+
+
+def config_c3(n=512, seed=3):
+    """Configuration-0 physics (dx=dy=200 m, walls, g=9.81, f=1.2e-4, r=0,
+    CFL 0.7) over a seeded smooth random bathymetry of 50..150 m on corners,
+    Gaussian bump 0.5 m (sigma 10 cells) at the centre, at rest."""
+    kw = config_c0(n=n)
+    p = smooth_random_field((n + 5, n + 5), 20.0, seed)
+    kw["H"] = (100.0 + 50.0 * p / np.max(np.abs(p))).astype(np.float32)
+    return kw
+
+
+def c3_background_and_truth(kw, seed=3):
+    """The state the gradient is taken at (the bump plus small smooth random
+    perturbations of eta, hu and hv, so that no limiter sits exactly at a
+    kink) and the perturbed truth the observations come from."""
+    n_y, n_x = kw["eta0"].shape
+    r = [smooth_random_field((n_y, n_x), 8.0, seed + 10 + i) for i in range(6)]
+    r = [a / np.max(np.abs(a)) for a in r]
+    bg = (kw["eta0"] + 0.01 * r[0], 0.5 * r[1], 0.5 * r[2])
+    truth = (bg[0] + 0.05 * r[3], bg[1] + 2.0 * r[4], bg[2] + 2.0 * r[5])
+    return [a.astype(np.float32) for a in bg], [a.astype(np.float32) for a in truth]
+
+
+def c3_drifters(nx, ny, dx, dy, n=200, seed=3):
+    """Seeded observation positions inside the interior (one cell from the edges)."""
+    rng = np.random.default_rng(seed + 100)
+    x = rng.uniform(1.0 * dx, (nx - 1.0) * dx, n)
+    y = rng.uniform(1.0 * dy, (ny - 1.0) * dy, n)
+    return np.stack([x, y], axis=1).astype(np.float32)
