@@ -138,6 +138,14 @@ class DifferentiableCDKLM16:
             _vp(self.Hi), self.Hi.pitch, self._fc_ptr(), self.bc, _vp(self.ws32), _vp(self.flag), self._sh()),
             "gpo_cdklm16_substep_adj_f32")
 
+    def _enter(self):
+        """Order the work stream after the caller's current stream."""
+        self.stream.wait_stream(torch.cuda.current_stream(self.dev))
+
+    def _leave(self):
+        """Order the caller's current stream after the work stream."""
+        torch.cuda.current_stream(self.dev).wait_stream(self.stream)
+
     def flag_value(self):
         self.stream.synchronize()
         return int(self.flag.item())
@@ -147,7 +155,7 @@ class DifferentiableCDKLM16:
         """Run n_steps from (eta0, hu0, hv0) ((ny+4) x (nx+4) device float32),
         keeping checkpoints.  Returns J (python float) and the checkpoints."""
         s = self.stream
-        s.wait_stream(torch.cuda.current_stream(self.dev))
+        self._enter()
         A, B = self._new_state(), self._new_state()
         with torch.cuda.stream(s):
             A.load(eta0, hu0, hv0)
@@ -162,6 +170,7 @@ class DifferentiableCDKLM16:
         self._misfit32(A)
         self._final = A
         self._ckpts = ckpts
+        s.synchronize()
         return float(self.J.item()), A
 
     def _segment_buffers(self):
@@ -174,6 +183,7 @@ class DifferentiableCDKLM16:
         """dJ/d(eta0, hu0, hv0) of the last forward(), scaled by grad_J, as
         three (ny+4) x (nx+4) float32 tensors (ghost cells zero)."""
         s = self.stream
+        self._enter()
         Qs, Q1s = self._segment_buffers()
         lam, L1, X = self._lam
         with torch.cuda.stream(s):
@@ -253,6 +263,7 @@ class DifferentiableCDKLM16:
     def trajectory_f64(self, eta0, hu0, hv0, n_steps=None):
         """FP64 states Q^0 .. Q^n and stages Q1^0 .. Q1^(n-1) (kept whole)."""
         self._init_f64()
+        self._enter()
         n = self.n_steps if n_steps is None else n_steps
         Qs = [self._new_state(torch.float64) for _ in range(n + 1)]
         Q1s = [self._new_state(torch.float64) for _ in range(n)]
@@ -261,11 +272,13 @@ class DifferentiableCDKLM16:
         for j in range(n):
             self._substep64(Qs[j], Q1s[j], None, 0)
             self._substep64(Q1s[j], Qs[j + 1], Qs[j], 1)
+        self._leave()
         return Qs, Q1s
 
     def misfit_f64(self, eta0, hu0, hv0):
         """J of the FP64 forward run from the given (float64) initial state."""
         self._init_f64()
+        self._enter()
         A, B = self._new_state(torch.float64), self._new_state(torch.float64)
         with torch.cuda.stream(self.stream):
             A.load(eta0, hu0, hv0)
@@ -275,11 +288,13 @@ class DifferentiableCDKLM16:
         with torch.cuda.stream(self.stream):
             self.J.zero_()
         self._misfit64(A)
+        self.stream.synchronize()
         return float(self.J.item())
 
     def adjoint_f64(self, Qs, Q1s, lam_final):
         """lambda_0 = (dQ^n/dQ^0)^T lambda_final along an FP64 trajectory."""
         n = len(Q1s)
+        self._enter()
         lam, L1, X = (self._new_state(torch.float64) for _ in range(3))
         with torch.cuda.stream(self.stream):
             lam.copy_from(lam_final)
@@ -289,11 +304,13 @@ class DifferentiableCDKLM16:
             self._adj64(1, Q1s[j], Qs[j + 1], lam, L1, X, 0)
             self._adj64(0, Qs[j], Q1s[j], L1, X, None, 1)
             lam, X = X, lam
+        self._leave()
         return lam
 
     def tangent_f64(self, Qs, Q1s, d0):
         """dQ^n = (dQ^n/dQ^0) d0 along an FP64 trajectory (tangent-linear model)."""
         n = len(Q1s)
+        self._enter()
         out = self._new_state(torch.float64)
         d, d1, dn = d0, self._new_state(torch.float64), self._new_state(torch.float64)
         for j in range(n):
@@ -301,6 +318,7 @@ class DifferentiableCDKLM16:
             self._tlm64(Q1s[j], d1, out, dn, Qs[j], d, 1)
             d = dn
             dn = self._new_state(torch.float64)
+        self._leave()
         return d
 
     def gradient_f64(self, eta0, hu0, hv0):
@@ -310,8 +328,10 @@ class DifferentiableCDKLM16:
         with torch.cuda.stream(self.stream):
             self.J.zero_()
         self._misfit64(Qs[-1], lamN)
+        self.stream.synchronize()
         J = float(self.J.item())
         lam0 = self.adjoint_f64(Qs, Q1s, lamN)
+        self.stream.synchronize()
         return J, [a.clone() for a in lam0.views()]
 
 

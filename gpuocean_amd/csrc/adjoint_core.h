@@ -1,0 +1,1038 @@
+// Per-cell stages of the differentiable CDKLM16 substep, shared by the gfx950
+// kernels (adjoint.hip) and a host build of the same code used by the CPU
+// tests.  Include from translation units built with -ffp-contract=off.
+//
+//  * Forward stages templated on the arithmetic type: double (the FP64
+//    verification build), float (recomputation of intermediates for the FP32
+//    adjoint) and Dual (value + one tangent, the tangent-linear model).
+//    F1 conversion of every cell (eta, u, v) with wall mirror / periodic wrap,
+//    F2 x- and y-slopes (minmod, well-balanced potential, adjustSlopes),
+//    F3 face fluxes (central-upwind), F4 sources + RK update.
+//  * The hand-written reverse of the same stages: A1 face fluxes, A2 per-cell
+//    gather (update + sources, face parts, slope subgradients, conversion),
+//    A3 fold of ghost-cell sensitivities onto the interior cells they mirror
+//    or wrap.
+//
+// Scope of the derivative: RK2 substeps of wet cells (h >= desingularisation
+// eps everywhere), wall and periodic boundaries, no friction, wind or
+// pressure forcing and no land.  Branches outside that scope set bits in the
+// caller's flag word.  Minmod, max and min take the subgradient of the first
+// minimal/maximal argument, the same choice the tangent-linear model makes.
+//
+// The arithmetic follows gpu_kernels/CDKLM16_kernel.cu:475-1090 (forward).
+#pragma once
+
+#include <cmath>
+#include <cstddef>
+#include <cstdint>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define GPO_HD __host__ __device__
+#define GPO_FLAG(flag, bits) atomicOr((flag), (bits))
+#else
+#define GPO_HD
+#define GPO_FLAG(flag, bits) (*(flag) |= (bits))
+#endif
+
+namespace gpo {
+namespace adj {
+
+using std::copysign;
+using std::fabs;
+using std::floor;
+using std::sqrt;
+
+// ----------------------------------------------------------------------------- number types
+
+struct Dual {
+  double v, d;
+  Dual() = default;
+  GPO_HD constexpr Dual(double a) : v(a), d(0.0) {}
+  GPO_HD constexpr Dual(double a, double b) : v(a), d(b) {}
+};
+GPO_HD inline Dual operator+(Dual a, Dual b) { return {a.v + b.v, a.d + b.d}; }
+GPO_HD inline Dual operator-(Dual a, Dual b) { return {a.v - b.v, a.d - b.d}; }
+GPO_HD inline Dual operator-(Dual a) { return {-a.v, -a.d}; }
+GPO_HD inline Dual operator*(Dual a, Dual b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+GPO_HD inline Dual operator/(Dual a, Dual b) {
+  const double q = a.v / b.v;
+  return {q, (a.d - q * b.d) / b.v};
+}
+GPO_HD inline Dual tsqrt(Dual a) {
+  const double s = sqrt(a.v);
+  return {s, s > 0.0 ? a.d / (2.0 * s) : 0.0};
+}
+GPO_HD inline float tsqrt(float a) { return std::sqrt(a); }
+GPO_HD inline double tsqrt(double a) { return sqrt(a); }
+
+GPO_HD inline double val(Dual a) { return a.v; }
+GPO_HD inline float val(float a) { return a; }
+GPO_HD inline double val(double a) { return a; }
+
+// sign as copysign(1, x): a constant of the derivative
+template <class T> GPO_HD inline T sgn1(T x) { return T(copysign(1.0, (double)val(x))); }
+template <> GPO_HD inline float sgn1<float>(float x) { return std::copysign(1.0f, x); }
+template <class T> GPO_HD inline T tabs(T x) { return x * sgn1(x); }
+// std::min / std::max: the first argument on ties
+template <class T> GPO_HD inline T tmin(T a, T b) { return (val(b) < val(a)) ? b : a; }
+template <class T> GPO_HD inline T tmax(T a, T b) { return (val(a) < val(b)) ? b : a; }
+
+// ----------------------------------------------------------------------------- geometry
+
+struct Geo {
+  int nx, ny;
+  int W;                    // dense workspace row length (nx + 4)
+  int sp, hp, fp;           // state, Hi and f_cells pitches in elements
+  int bc_n, bc_e, bc_s, bc_w;
+  double dx, dy, g, f, theta, desing_eps, depth_cutoff, flux_slope_eps, flux_balancer, north_x, north_y;
+};
+
+// Cell of the input array that supplies stencil cell (r, c): wall ghosts
+// mirror (normal momentum negated), periodic ghosts wrap; only for ghost
+// columns of interior rows and ghost rows of interior columns
+// (CDKLM16_kernel.cu:419-470, boundary_kernels.cu:36-110).
+struct Src {
+  int r, c;
+  float shu, shv;
+};
+GPO_HD inline Src map_cell(const Geo& g, int r, int c) {
+  Src s{r, c, 1.0f, 1.0f};
+  const bool row_in = r >= 2 && r <= g.ny + 1, col_in = c >= 2 && c <= g.nx + 1;
+  if (row_in && c < 2) {
+    if (g.bc_w == 1) { s.c = 3 - c; s.shu = -1.0f; }
+    else if (g.bc_w == 2) s.c = c + g.nx;
+  } else if (row_in && c > g.nx + 1) {
+    if (g.bc_e == 1) { s.c = 2 * g.nx + 3 - c; s.shu = -1.0f; }
+    else if (g.bc_e == 2) s.c = c - g.nx;
+  } else if (col_in && r < 2) {
+    if (g.bc_s == 1) { s.r = 3 - r; s.shv = -1.0f; }
+    else if (g.bc_s == 2) s.r = r + g.ny;
+  } else if (col_in && r > g.ny + 1) {
+    if (g.bc_n == 1) { s.r = 2 * g.ny + 3 - r; s.shv = -1.0f; }
+    else if (g.bc_n == 2) s.r = r - g.ny;
+  }
+  return s;
+}
+
+// Dense workspace arrays ((ny+4) x (nx+4) each, element type S).
+template <class S> struct Fwd {
+  S *ce, *cu, *cv;      // converted cells
+  S *qx0, *qx1, *kx;    // x-slopes of u, v and the potential K
+  S *qy0, *qy1, *ly;    // y-slopes of u, v and the potential L
+  S *fx, *fy, *fz;      // x-face fluxes (face c between c and c+1)
+  S *gx, *gy, *gz;      // y-face fluxes (face r between r and r+1)
+};
+
+template <class S> inline Fwd<S> carve(void* ws, int nx, int ny) {
+  const size_t n = (size_t)(nx + 4) * (ny + 4);
+  S* p = (S*)ws;
+  Fwd<S> f;
+  S** slots[] = {&f.ce, &f.cu, &f.cv, &f.qx0, &f.qx1, &f.kx, &f.qy0, &f.qy1,
+                 &f.ly, &f.fx, &f.fy, &f.fz, &f.gx, &f.gy, &f.gz};
+  for (int i = 0; i < 15; ++i) *slots[i] = p + n * i;
+  return f;
+}
+
+template <class T> GPO_HD inline T cell_depth(const Geo& g, const T* Hi, int r, int c) {
+  const T a = Hi[(size_t)r * g.hp + c], b = Hi[(size_t)(r + 1) * g.hp + c];
+  const T c_ = Hi[(size_t)r * g.hp + c + 1], d = Hi[(size_t)(r + 1) * g.hp + c + 1];
+  return T(0.25) * (a + b + c_ + d);
+}
+template <class T> GPO_HD inline T corner(const Geo& g, const T* Hi, int r, int c) {
+  return Hi[(size_t)r * g.hp + c];
+}
+template <class T> GPO_HD inline T fcor(const Geo& g, const T* fc, int r, int c) {
+  return fc ? fc[(size_t)r * g.fp + c] : T(g.f);
+}
+
+// ----------------------------------------------------------------------------- forward stages
+
+// desingularize (common.cu:664-666), the double formula
+template <class S> GPO_HD inline S desing(S h, S hu, double eps) {
+  const S a = h * h / S(2.0 * eps) + S(0.5 * eps);
+  return hu / tmax(tmin(a, S(eps)), tabs(h));
+}
+
+// F1: conversion of the stage input (CDKLM16_kernel.cu:614-655).  `in_*` are
+// the caller's arrays (pitch sp) of type S.
+template <class S>
+GPO_HD void f1_cell(Geo g, const S* in_e, const S* in_u, const S* in_v, const S* Hi, Fwd<S> w, int* flag, int r, int c) {
+  if (c >= g.nx + 4 || r >= g.ny + 4) return;
+  const Src s = map_cell(g, r, c);
+  const size_t k = (size_t)s.r * g.sp + s.c;
+  S e = in_e[k];
+  S hu = in_u[k] * S(s.shu);
+  S hv = in_v[k] * S(s.shv);
+  const S Hm = cell_depth(g, Hi, r, c);
+  const S h = e + Hm;
+  S u, v;
+  if (val(h) < g.desing_eps) {
+    if (flag && r >= 1 && r <= g.ny + 2 && c >= 1 && c <= g.nx + 2) GPO_FLAG(flag, 1);
+    if (val(h) <= g.depth_cutoff) {
+      e = -Hm + S(g.depth_cutoff);
+      u = S(0.0);
+      v = S(0.0);
+    } else {
+      u = desing(h, hu, g.desing_eps);
+      v = desing(h, hv, g.desing_eps);
+    }
+  } else {
+    u = hu / h;
+    v = hv / h;
+  }
+  const size_t o = (size_t)r * g.W + c;
+  w.ce[o] = e;
+  w.cu[o] = u;
+  w.cv[o] = v;
+}
+
+// minmodSlope / minmodRaw (common.cu:532-551)
+template <class S> GPO_HD inline S minmod_raw(S b, S c, S f) {
+  const S sb = sgn1(b), sc = sgn1(c), sf = sgn1(f);
+  return S(0.25) * sb * (sb + sc) * (sc + sf) * tmin(tmin(tabs(b), tabs(c)), tabs(f));
+}
+template <class S> GPO_HD inline S minmod_slope(S l, S c, S r, S theta) {
+  return minmod_raw((c - l) * theta, (r - l) * S(0.5), (r - c) * theta);
+}
+
+// x-slopes at (r, s): CDKLM16_kernel.cu:673-737 with adjustSlopes_x (:159-203)
+template <class S>
+GPO_HD void xslope(const Geo& g, const Fwd<S>& w, const S* Hi, const S* fc, int r, int s, S& q0, S& q1, S& K) {
+  const size_t o = (size_t)r * g.W + s;
+  const S le = w.ce[o - 1], ce = w.ce[o], re = w.ce[o + 1];
+  const S lu = w.cu[o - 1], cu = w.cu[o], ru = w.cu[o + 1];
+  S lv = w.cv[o - 1], cv = w.cv[o], rv = w.cv[o + 1];
+  const S th = S(g.theta);
+  q0 = minmod_slope(lu, cu, ru, th);
+  q1 = minmod_slope(lv, cv, rv, th);
+  if (g.bc_w == 1) {
+    if (s < 3) lv = -lv;
+    if (s < 2) cv = -cv;
+  }
+  if (g.bc_e == 1) {
+    if (s > g.nx) rv = -rv;
+    if (s > g.nx + 1) cv = -cv;
+  }
+  const S nx_ = S(g.north_x), ny_ = S(g.north_y), G = S(g.g), Vc = S(g.dx) / (S(2.0) * G);
+  const S lfv = (nx_ * lu + ny_ * lv) * fcor(g, fc, r, s - 1);
+  const S cfv = (nx_ * cu + ny_ * cv) * fcor(g, fc, r, s);
+  const S rfv = (nx_ * ru + ny_ * rv) * fcor(g, fc, r, s + 1);
+  const S b = th * G * (ce - le - Vc * (cfv + lfv));
+  const S c = S(0.5) * G * (re - le - Vc * (rfv + S(2.0) * cfv + lfv));
+  const S f = th * G * (re - ce - Vc * (cfv + rfv));
+  S k = minmod_raw(b, c, f);
+  S v = w.cv[o];
+  if (g.bc_w == 1 && s < 2) v = -v;
+  if (g.bc_e == 1 && s > g.nx + 2) v = -v;
+  const S g2 = S(2.0) * G;
+  const S dxfv = S(g.dx) * fcor(g, fc, r, s) * v;
+  const S Hw = S(0.5) * (corner(g, Hi, r, s) + corner(g, Hi, r + 1, s));
+  const S He = S(0.5) * (corner(g, Hi, r, s + 1) + corner(g, Hi, r + 1, s + 1));
+  const S hw = ce + Hw - (k + dxfv) / g2;
+  const S he = ce + He + (k + dxfv) / g2;
+  k = (val(hw) > 0) ? k : -dxfv + g2 * (ce + Hw);
+  k = (val(he) > 0) ? k : -dxfv - g2 * (ce + He);
+  K = k;
+}
+
+// y-slopes at (s, c): CDKLM16_kernel.cu:779-842 with adjustSlopes_y (:210-254)
+template <class S>
+GPO_HD void yslope(const Geo& g, const Fwd<S>& w, const S* Hi, const S* fc, int s, int c, S& q0, S& q1, S& L) {
+  const size_t o = (size_t)s * g.W + c, up = g.W;
+  const S le = w.ce[o - up], ce = w.ce[o], ue = w.ce[o + up];
+  S lu = w.cu[o - up], cu = w.cu[o], uu = w.cu[o + up];
+  const S lv = w.cv[o - up], cv = w.cv[o], uv = w.cv[o + up];
+  const S th = S(g.theta);
+  q0 = minmod_slope(lu, cu, uu, th);
+  q1 = minmod_slope(lv, cv, uv, th);
+  if (g.bc_s == 1) {
+    if (s < 3) lu = -lu;
+    if (s < 2) cu = -cu;
+  }
+  if (g.bc_n == 1) {
+    if (s > g.ny) uu = -uu;
+    if (s > g.ny + 1) cu = -cu;
+  }
+  const S ex = S(g.north_y), ey = S(-g.north_x), G = S(g.g), Uc = S(g.dy) / (S(2.0) * G);
+  const S lfu = (ex * lu + ey * lv) * fcor(g, fc, s - 1, c);
+  const S cfu = (ex * cu + ey * cv) * fcor(g, fc, s, c);
+  const S ufu = (ex * uu + ey * uv) * fcor(g, fc, s + 1, c);
+  const S b = th * G * (ce - le + Uc * (cfu + lfu));
+  const S cc = S(0.5) * G * (ue - le + Uc * (ufu + S(2.0) * cfu + lfu));
+  const S f = th * G * (ue - ce + Uc * (cfu + ufu));
+  S l = minmod_raw(b, cc, f);
+  S u = w.cu[o];
+  if (g.bc_s == 1 && s < 2) u = -u;
+  if (g.bc_n == 1 && s > g.ny + 2) u = -u;
+  const S g2 = S(2.0) * G;
+  const S dyfu = S(g.dy) * fcor(g, fc, s, c) * u;
+  const S Hs = S(0.5) * (corner(g, Hi, s, c) + corner(g, Hi, s, c + 1));
+  const S Hn = S(0.5) * (corner(g, Hi, s + 1, c) + corner(g, Hi, s + 1, c + 1));
+  const S hs = ce + Hs - (l - dyfu) / g2;
+  const S hn = ce + Hn + (l - dyfu) / g2;
+  l = (val(hs) > 0) ? l : dyfu + g2 * (ce + Hs);
+  l = (val(hn) > 0) ? l : dyfu - g2 * (ce + Hn);
+  L = l;
+}
+
+template <class S>
+GPO_HD void f2_cell(Geo g, const S* Hi, const S* fc, Fwd<S> w, int r, int c) {
+  if (c >= g.nx + 4 || r >= g.ny + 4) return;
+  const size_t o = (size_t)r * g.W + c;
+  if (r >= 2 && r <= g.ny + 1 && c >= 1 && c <= g.nx + 2) xslope(g, w, Hi, fc, r, c, w.qx0[o], w.qx1[o], w.kx[o]);
+  if (r >= 1 && r <= g.ny + 2 && c >= 2 && c <= g.nx + 1) yslope(g, w, Hi, fc, r, c, w.qy0[o], w.qy1[o], w.ly[o]);
+}
+
+// CDKLM16_F_func (CDKLM16_kernel.cu:82-91) and CDKLM16_flux (:104-151), the
+// wet-wet case (the dry face cases of :318-338 do not arise for wet cells).
+template <class S>
+GPO_HD void flux_fwd(const Geo& g, const S* Qm, const S* Qp, S H, S* F) {
+  const S G = S(g.g);
+  const S hp = Qp[0] + H, hm = Qm[0] + H;
+  S Fp0 = S(0.0), Fp1 = S(0.0), Fp2 = S(0.0), Fm0 = S(0.0), Fm1 = S(0.0), Fm2 = S(0.0);
+  S up = S(0.0), cp = S(0.0), um = S(0.0), cm = S(0.0);
+  if (val(hp) > g.depth_cutoff) {
+    Fp0 = hp * Qp[1];
+    Fp1 = hp * Qp[1] * Qp[1] + S(0.5) * G * hp * hp;
+    Fp2 = hp * Qp[1] * Qp[2];
+    up = Qp[1];
+    cp = tsqrt(G * hp);
+  }
+  if (val(hm) > g.depth_cutoff) {
+    Fm0 = hm * Qm[1];
+    Fm1 = hm * Qm[1] * Qm[1] + S(0.5) * G * hm * hm;
+    Fm2 = hm * Qm[1] * Qm[2];
+    um = Qm[1];
+    cm = tsqrt(G * hm);
+  }
+  const S am = tmin(tmin(um - cm, up - cp), S(0.0));
+  const S ap = tmax(tmax(um + cm, up + cp), S(0.0));
+  if (fabs((double)val(ap - am)) < g.flux_slope_eps) {
+    F[0] = F[1] = F[2] = S(0.0);
+    return;
+  }
+  const S fb = S(g.flux_balancer);
+  F[0] = ((ap * Fm0 - am * Fp0) + ap * am * (Qp[0] - Qm[0])) / (ap - am);
+  F[1] = ((ap * Fm1 - am * Fp1) + ap * am * (Fp0 - Fm0)) / (ap - am);
+  S z = (val(Qm[1]) > -val(Qp[1])) ? fb * Fm2 : fb * Fp2;
+  F[2] = z + (S(1.0) - fb) * (((ap * Fm2 - am * Fp2) + ap * am * (hp * Qp[2] - hm * Qm[2])) / (ap - am));
+}
+
+// x-face between (r, s) and (r, s+1): CDKLM16_kernel.cu:259-340 (computeFFaceFlux)
+template <class S>
+GPO_HD void xface(const Geo& g, const Fwd<S>& w, const S* Hi, const S* fc, int r, int s, S* F, int* flag) {
+  const size_t m = (size_t)r * g.W + s, p = m + 1;
+  const S G = S(g.g), g2 = S(2.0) * G;
+  const S ebp = w.ce[p], ebm = w.ce[m], up = w.cu[p], um = w.cu[m];
+  S vp = w.cv[p], vm = w.cv[m];
+  const S Rpx = up - S(0.5) * w.qx0[p], Rpy = vp - S(0.5) * w.qx1[p];
+  const S Rmx = um + S(0.5) * w.qx0[m], Rmy = vm + S(0.5) * w.qx1[m];
+  const S H = S(0.5) * (corner(g, Hi, r, s + 1) + corner(g, Hi, r + 1, s + 1));
+  if (g.bc_w == 1 && s + 1 == 2) vm = -vm;
+  if (g.bc_e == 1 && s + 1 == g.nx + 2) vp = -vp;
+  const S nx_ = S(g.north_x), ny_ = S(g.north_y);
+  const S vpn = up * nx_ + vp * ny_, vmn = um * nx_ + vm * ny_;
+  const S ap = ebp - (w.kx[p] + S(g.dx) * fcor(g, fc, r, s + 1) * vpn) / g2;
+  const S am = ebm + (w.kx[m] + S(g.dx) * fcor(g, fc, r, s) * vmn) / g2;
+  if (flag && (!(val(ap) > val(-H)) || !(val(am) > val(-H)))) GPO_FLAG(flag, 8);
+  const S Qp[3] = {tmax(-H, ap), Rpx, Rpy}, Qm[3] = {tmax(-H, am), Rmx, Rmy};
+  flux_fwd(g, Qm, Qp, H, F);
+}
+
+// y-face between (s, c) and (s+1, c): CDKLM16_kernel.cu:345-416 (computeGFaceFlux)
+template <class S>
+GPO_HD void yface(const Geo& g, const Fwd<S>& w, const S* Hi, const S* fc, int s, int c, S* G3, int* flag) {
+  const size_t m = (size_t)s * g.W + c, p = m + g.W;
+  const S G = S(g.g), g2 = S(2.0) * G;
+  const S ebp = w.ce[p], ebm = w.ce[m], vp = w.cv[p], vm = w.cv[m];
+  S up = w.cu[p], um = w.cu[m];
+  const S Rpx = up - S(0.5) * w.qy0[p], Rpy = vp - S(0.5) * w.qy1[p];
+  const S Rmx = um + S(0.5) * w.qy0[m], Rmy = vm + S(0.5) * w.qy1[m];
+  const S H = S(0.5) * (corner(g, Hi, s + 1, c) + corner(g, Hi, s + 1, c + 1));
+  if (g.bc_s == 1 && s + 1 == 2) um = -um;
+  if (g.bc_n == 1 && s + 1 == g.ny + 2) up = -up;
+  const S ex = S(g.north_y), ey = S(-g.north_x);
+  const S upe = up * ex + vp * ey, ume = um * ex + vm * ey;
+  const S ap = ebp - (w.ly[p] - S(g.dy) * fcor(g, fc, s + 1, c) * upe) / g2;
+  const S am = ebm + (w.ly[m] - S(g.dy) * fcor(g, fc, s, c) * ume) / g2;
+  if (flag && (!(val(ap) > val(-H)) || !(val(am) > val(-H)))) GPO_FLAG(flag, 8);
+  const S Qp[3] = {tmax(-H, ap), Rpy, Rpx}, Qm[3] = {tmax(-H, am), Rmy, Rmx};
+  S fl[3];
+  flux_fwd(g, Qm, Qp, H, fl);
+  G3[0] = fl[0];
+  G3[1] = fl[2];
+  G3[2] = fl[1];
+}
+
+template <class S>
+GPO_HD void f3_cell(Geo g, const S* Hi, const S* fc, Fwd<S> w, int* flag, int r, int c) {
+  if (c >= g.nx + 4 || r >= g.ny + 4) return;
+  const size_t o = (size_t)r * g.W + c;
+  if (r >= 2 && r <= g.ny + 1 && c >= 1 && c <= g.nx + 1) {
+    S F[3];
+    xface(g, w, Hi, fc, r, c, F, flag);
+    w.fx[o] = F[0];
+    w.fy[o] = F[1];
+    w.fz[o] = F[2];
+  }
+  if (r >= 1 && r <= g.ny + 1 && c >= 2 && c <= g.nx + 1) {
+    S G3[3];
+    yface(g, w, Hi, fc, r, c, G3, flag);
+    w.gx[o] = G3[0];
+    w.gy[o] = G3[1];
+    w.gz[o] = G3[2];
+  }
+}
+
+// F4: sources and the RK2 update at interior cells (CDKLM16_kernel.cu:869-1089),
+// no friction and no forcing.  qn is read before out is written (may alias).
+template <class S>
+GPO_HD void f4_cell(Geo g, S dt, int rk_step, const S* Hi, const S* fc, Fwd<S> w, const S* qn_e,
+                          const S* qn_u, const S* qn_v, S* out_e, S* out_u, S* out_v, int* flag, int r, int c) {
+  if (c > g.nx + 1 || r > g.ny + 1) return;
+  const size_t o = (size_t)r * g.W + c;
+  const S G = S(g.g), g2 = S(2.0) * G, dx = S(g.dx), dy = S(g.dy);
+  const S hm = cell_depth(g, Hi, r, c);
+  const S R0 = w.ce[o], R1 = w.cu[o], R2 = w.cv[o];
+  S hu = S(0.0), hv = S(0.0);
+  if (val(R0 + hm) > g.depth_cutoff) {
+    hu = R1 * (R0 + hm);
+    hv = R2 * (R0 + hm);
+  }
+  const S f = fcor(g, fc, r, c);
+  S fdx = (w.fx[o] - w.fx[o - 1]) / dx;
+  S fdy = (w.fy[o] - w.fy[o - 1]) / dx;
+  S fdz = (w.fz[o] - w.fz[o - 1]) / dx;
+  const S eta_west = R0 - (w.kx[o] + dx * f * R2) / g2;
+  const S eta_east = R0 + (w.kx[o] + dx * f * R2) / g2;
+  fdx = fdx + (w.gx[o] - w.gx[o - g.W]) / dy;
+  fdy = fdy + (w.gy[o] - w.gy[o - g.W]) / dy;
+  fdz = fdz + (w.gz[o] - w.gz[o - g.W]) / dy;
+  const S eta_south = R0 - (w.ly[o] - dy * f * R1) / g2;
+  const S eta_north = R0 + (w.ly[o] - dy * f * R1) / g2;
+  S st1 = S(0.0), st2 = S(0.0);
+  const S h = R0 + hm;
+  if (val(h) >= g.depth_cutoff) {
+    const S RHxp = S(0.5) * (corner(g, Hi, r, c + 1) + corner(g, Hi, r + 1, c + 1));
+    const S RHxm = S(0.5) * (corner(g, Hi, r, c) + corner(g, Hi, r + 1, c));
+    const S RHyp = S(0.5) * (corner(g, Hi, r + 1, c) + corner(g, Hi, r + 1, c + 1));
+    const S RHym = S(0.5) * (corner(g, Hi, r, c) + corner(g, Hi, r, c + 1));
+    const S nx_ = S(g.north_x), ny_ = S(g.north_y), ex = ny_, ey = -nx_;
+    const S hu_east = hu * ex + hv * ey;
+    const S hv_north = hu * nx_ + hv * ny_;
+    const S hu_east_cor = f * hv_north;
+    const S hv_north_cor = -f * hu_east;
+    const S upx = -nx_, upy = ny_, rtx = upy, rty = -upx;
+    const S hu_cor = rtx * hu_east_cor + rty * hv_north_cor;
+    const S hv_cor = upx * hu_east_cor + upy * hv_north_cor;
+    const S eta_we = S(0.5) * (eta_west + eta_east);
+    st1 = hu_cor + (G * (eta_we + hm) * (RHxp - RHxm)) / dx;
+    const S eta_sn = S(0.5) * (eta_north + eta_south);
+    st2 = hv_cor + (G * (eta_sn + hm) * (RHyp - RHym)) / dy;
+  }
+  const S L1 = -fdx, L2 = -fdy + st1, L3 = -fdz + st2;
+  S ue, uhu, uhv;
+  const size_t k = (size_t)r * g.sp + c;
+  if (rk_step == 0) {
+    ue = R0 + dt * L1;
+    uhu = hu + dt * L2;
+    uhv = hv + dt * L3;
+  } else {
+    ue = S(0.5) * (qn_e[k] + (R0 + dt * L1));
+    uhu = S(0.5) * (qn_u[k] + (hu + dt * L2));
+    uhv = S(0.5) * (qn_v[k] + (hv + dt * L3));
+  }
+  if (val(ue + hm) <= g.depth_cutoff) {
+    if (flag) GPO_FLAG(flag, 2);
+    ue = -hm + S(g.depth_cutoff);
+    uhu = S(0.0);
+    uhv = S(0.0);
+  }
+  out_e[k] = tmax(-hm + S(g.depth_cutoff), ue);
+  out_u[k] = uhu;
+  out_v[k] = uhv;
+}
+
+// ----------------------------------------------------------------------------- reverse stages
+
+// Face sensitivities: 6 values for the cell on each side of a face
+// (eta, u, v, slope0, slope1, potential).
+template <class T> struct Rev {
+  T* xl[6];  // x-face s -> cell s
+  T* xr[6];  // x-face s -> cell s+1
+  T* yl[6];  // y-face s -> cell s
+  T* yr[6];  // y-face s -> cell s+1
+  T *le, *lu, *lv;  // gathered sensitivities of the loaded input, all cells
+};
+
+template <class T> inline Rev<T> carve_rev(void* ws, int nx, int ny) {
+  const size_t n = (size_t)(nx + 4) * (ny + 4);
+  T* p = (T*)ws + 9 * n;  // after conv (3) and slopes (6) of Fwd<T>
+  Rev<T> v;
+  for (int i = 0; i < 6; ++i) {
+    v.xl[i] = p + n * i;
+    v.xr[i] = p + n * (6 + i);
+    v.yl[i] = p + n * (12 + i);
+    v.yr[i] = p + n * (18 + i);
+  }
+  v.le = p + n * 24;
+  v.lu = p + n * 25;
+  v.lv = p + n * 26;
+  return v;
+}
+
+// Adjoint of flux_fwd: lQm, lQp += d(F . lF)/dQ.
+template <class T>
+GPO_HD void flux_rev(const Geo& g, const T* Qm, const T* Qp, T H, const T* lF, T* lQm, T* lQp) {
+  const T G = T(g.g);
+  const T hp = Qp[0] + H, hm = Qm[0] + H;
+  const bool wp = hp > T(g.depth_cutoff), wm = hm > T(g.depth_cutoff);
+  T Fp0 = 0, Fp1 = 0, Fp2 = 0, Fm0 = 0, Fm1 = 0, Fm2 = 0, up = 0, cp = 0, um = 0, cm = 0;
+  if (wp) {
+    Fp0 = hp * Qp[1];
+    Fp1 = hp * Qp[1] * Qp[1] + T(0.5) * G * hp * hp;
+    Fp2 = hp * Qp[1] * Qp[2];
+    up = Qp[1];
+    cp = tsqrt(G * hp);
+  }
+  if (wm) {
+    Fm0 = hm * Qm[1];
+    Fm1 = hm * Qm[1] * Qm[1] + T(0.5) * G * hm * hm;
+    Fm2 = hm * Qm[1] * Qm[2];
+    um = Qm[1];
+    cm = tsqrt(G * hm);
+  }
+  // am = min(min(um - cm, up - cp), 0), ap = max(max(um + cm, up + cp), 0): first on ties
+  const T am1 = um - cm, am2 = up - cp, ap1 = um + cm, ap2 = up + cp;
+  const int am_sel = (am2 < am1) ? ((T(0) < am2) ? 0 : 2) : ((T(0) < am1) ? 0 : 1);
+  const int ap_sel = (ap1 < ap2) ? ((ap2 < T(0)) ? 0 : 2) : ((ap1 < T(0)) ? 0 : 1);
+  const T am = am_sel == 1 ? am1 : (am_sel == 2 ? am2 : T(0));
+  const T ap = ap_sel == 1 ? ap1 : (ap_sel == 2 ? ap2 : T(0));
+  if (fabs((double)(ap - am)) < g.flux_slope_eps) return;
+  const T fb = T(g.flux_balancer), d = ap - am;
+  const T dq = Qp[0] - Qm[0], dfx = Fp0 - Fm0, dhv = hp * Qp[2] - hm * Qm[2];
+  const T Nx = (ap * Fm0 - am * Fp0) + ap * am * dq;
+  const T Ny = (ap * Fm1 - am * Fp1) + ap * am * dfx;
+  const T Z = (ap * Fm2 - am * Fp2) + ap * am * dhv;
+  const T lNx = lF[0] / d, lNy = lF[1] / d, lZ = (T(1) - fb) * lF[2] / d;
+  T ld = -(lNx * Nx + lNy * Ny + lZ * Z) / d;
+  T lap = lNx * (Fm0 + am * dq) + lNy * (Fm1 + am * dfx) + lZ * (Fm2 + am * dhv) + ld;
+  T lam = lNx * (-Fp0 + ap * dq) + lNy * (-Fp1 + ap * dfx) + lZ * (-Fp2 + ap * dhv) - ld;
+  T lFm0 = lNx * ap - lNy * ap * am, lFp0 = -lNx * am + lNy * ap * am;
+  T lFm1 = lNy * ap, lFp1 = -lNy * am;
+  T lFm2 = lZ * ap, lFp2 = -lZ * am;
+  if (Qm[1] > -Qp[1]) lFm2 += fb * lF[2];
+  else lFp2 += fb * lF[2];
+  T lhp = lZ * ap * am * Qp[2], lhm = -lZ * ap * am * Qm[2];
+  T lQp0 = lNx * ap * am, lQm0 = -lNx * ap * am;
+  T lvp = lZ * ap * am * hp, lvm = -lZ * ap * am * hm;
+  // wave speeds: um, cm (up, cp) are constants 0 on a dry side
+  T lup = 0, lum = 0, lcp = 0, lcm = 0;
+  if (ap_sel == 1) { lum += lap; lcm += lap; }
+  else if (ap_sel == 2) { lup += lap; lcp += lap; }
+  if (am_sel == 1) { lum += lam; lcm -= lam; }
+  else if (am_sel == 2) { lup += lam; lcp -= lam; }
+  if (!wp) lup = lcp = 0;
+  if (!wm) lum = lcm = 0;
+  if (wp) {
+    lhp += lFp0 * Qp[1] + lFp1 * (Qp[1] * Qp[1] + G * hp) + lFp2 * Qp[1] * Qp[2];
+    lup += lFp0 * hp + lFp1 * T(2) * hp * Qp[1] + lFp2 * hp * Qp[2];
+    lvp += lFp2 * hp * Qp[1];
+    if (cp > T(0)) lhp += lcp * G / (T(2) * cp);
+  }
+  if (wm) {
+    lhm += lFm0 * Qm[1] + lFm1 * (Qm[1] * Qm[1] + G * hm) + lFm2 * Qm[1] * Qm[2];
+    lum += lFm0 * hm + lFm1 * T(2) * hm * Qm[1] + lFm2 * hm * Qm[2];
+    lvm += lFm2 * hm * Qm[1];
+    if (cm > T(0)) lhm += lcm * G / (T(2) * cm);
+  }
+  lQp[0] += lQp0 + lhp;
+  lQp[1] += lup;
+  lQp[2] += lvp;
+  lQm[0] += lQm0 + lhm;
+  lQm[1] += lum;
+  lQm[2] += lvm;
+}
+
+// Sensitivity of the interior cell's update outputs, after the depth clamp.
+template <class T> struct CellAdj {
+  T e, u, v;  // lambda of (ue, uhu, uhv)
+};
+template <class T>
+GPO_HD inline CellAdj<T> out_adj(const Geo& g, const T* Hi, const T* out_e, const T* lo_e,
+                                              const T* lo_u, const T* lo_v, int r, int c) {
+  CellAdj<T> a{0, 0, 0};
+  if (r < 2 || r > g.ny + 1 || c < 2 || c > g.nx + 1) return a;
+  const size_t k = (size_t)r * g.sp + c;
+  const T hm = cell_depth(g, Hi, r, c);
+  if (out_e[k] == -hm + T(g.depth_cutoff)) return a;  // clamped to the dry state: no sensitivity
+  a.e = lo_e[k];
+  a.u = lo_u[k];
+  a.v = lo_v[k];
+  return a;
+}
+
+// A1: reverse of the face fluxes.
+template <class T>
+GPO_HD void a1_cell(Geo g, T dt, int rk_step, const T* Hi, const T* fc, Fwd<T> w, Rev<T> rv, const T* out_e,
+                         const T* lo_e, const T* lo_u, const T* lo_v, int r, int c) {
+  if (c >= g.nx + 4 || r >= g.ny + 4) return;
+  const size_t o = (size_t)r * g.W + c;
+  const T coef = (rk_step == 0 ? T(1) : T(0.5)) * dt;
+  const T G = T(g.g), g2 = T(2) * G;
+  if (r >= 2 && r <= g.ny + 1 && c >= 1 && c <= g.nx + 1) {
+    // lambda of the three x-fluxes at face c: L_k = -(F_c - F_{c-1})/dx - ...
+    const CellAdj<T> am = out_adj(g, Hi, out_e, lo_e, lo_u, lo_v, r, c);
+    const CellAdj<T> ap = out_adj(g, Hi, out_e, lo_e, lo_u, lo_v, r, c + 1);
+    const T dxx = T(g.dx);
+    const T lF[3] = {coef * ap.e / dxx - coef * am.e / dxx, coef * ap.u / dxx - coef * am.u / dxx,
+                     coef * ap.v / dxx - coef * am.v / dxx};
+    const size_t m = o, p = o + 1;
+    const T ebp = w.ce[p], ebm = w.ce[m], up = w.cu[p], um = w.cu[m];
+    const T vp0 = w.cv[p], vm0 = w.cv[m];
+    const T Rpx = up - T(0.5) * w.qx0[p], Rpy = vp0 - T(0.5) * w.qx1[p];
+    const T Rmx = um + T(0.5) * w.qx0[m], Rmy = vm0 + T(0.5) * w.qx1[m];
+    const T H = T(0.5) * (corner(g, Hi, r, c + 1) + corner(g, Hi, r + 1, c + 1));
+    const T sm = (g.bc_w == 1 && c + 1 == 2) ? T(-1) : T(1);
+    const T sp = (g.bc_e == 1 && c + 1 == g.nx + 2) ? T(-1) : T(1);
+    const T nx_ = T(g.north_x), ny_ = T(g.north_y);
+    const T vpn = up * nx_ + (sp * vp0) * ny_, vmn = um * nx_ + (sm * vm0) * ny_;
+    const T fp = fcor(g, fc, r, c + 1), fm = fcor(g, fc, r, c);
+    const T argp = ebp - (w.kx[p] + T(g.dx) * fp * vpn) / g2;
+    const T argm = ebm + (w.kx[m] + T(g.dx) * fm * vmn) / g2;
+    const T Qp[3] = {(-H < argp) ? argp : -H, Rpx, Rpy}, Qm[3] = {(-H < argm) ? argm : -H, Rmx, Rmy};
+    T lQp[3] = {0, 0, 0}, lQm[3] = {0, 0, 0};
+    flux_rev(g, Qm, Qp, H, lF, lQm, lQp);
+    T L[6] = {0, 0, 0, 0, 0, 0}, R[6] = {0, 0, 0, 0, 0, 0};  // e, u, v, q0, q1, K
+    if (-H < argp) {
+      R[0] += lQp[0];
+      R[5] -= lQp[0] / g2;
+      const T lvpn = -lQp[0] * T(g.dx) * fp / g2;
+      R[1] += nx_ * lvpn;
+      R[2] += sp * ny_ * lvpn;
+    }
+    if (-H < argm) {
+      L[0] += lQm[0];
+      L[5] += lQm[0] / g2;
+      const T lvmn = lQm[0] * T(g.dx) * fm / g2;
+      L[1] += nx_ * lvmn;
+      L[2] += sm * ny_ * lvmn;
+    }
+    R[1] += lQp[1];
+    R[3] -= T(0.5) * lQp[1];
+    R[2] += lQp[2];
+    R[4] -= T(0.5) * lQp[2];
+    L[1] += lQm[1];
+    L[3] += T(0.5) * lQm[1];
+    L[2] += lQm[2];
+    L[4] += T(0.5) * lQm[2];
+    for (int i = 0; i < 6; ++i) {
+      rv.xl[i][o] = L[i];
+      rv.xr[i][o] = R[i];
+    }
+  }
+  if (r >= 1 && r <= g.ny + 1 && c >= 2 && c <= g.nx + 1) {
+    const CellAdj<T> am = out_adj(g, Hi, out_e, lo_e, lo_u, lo_v, r, c);
+    const CellAdj<T> ap = out_adj(g, Hi, out_e, lo_e, lo_u, lo_v, r + 1, c);
+    const T dyy = T(g.dy);
+    const T lG[3] = {coef * ap.e / dyy - coef * am.e / dyy, coef * ap.u / dyy - coef * am.u / dyy,
+                     coef * ap.v / dyy - coef * am.v / dyy};
+    const T lfl[3] = {lG[0], lG[2], lG[1]};  // G = (fl.x, fl.z, fl.y)
+    const size_t m = o, p = o + g.W;
+    const T ebp = w.ce[p], ebm = w.ce[m], vp = w.cv[p], vm = w.cv[m];
+    const T up0 = w.cu[p], um0 = w.cu[m];
+    const T Rpx = up0 - T(0.5) * w.qy0[p], Rpy = vp - T(0.5) * w.qy1[p];
+    const T Rmx = um0 + T(0.5) * w.qy0[m], Rmy = vm + T(0.5) * w.qy1[m];
+    const T H = T(0.5) * (corner(g, Hi, r + 1, c) + corner(g, Hi, r + 1, c + 1));
+    const T sm = (g.bc_s == 1 && r + 1 == 2) ? T(-1) : T(1);
+    const T sp = (g.bc_n == 1 && r + 1 == g.ny + 2) ? T(-1) : T(1);
+    const T ex = T(g.north_y), ey = T(-g.north_x);
+    const T upe = (sp * up0) * ex + vp * ey, ume = (sm * um0) * ex + vm * ey;
+    const T fp = fcor(g, fc, r + 1, c), fm = fcor(g, fc, r, c);
+    const T argp = ebp - (w.ly[p] - T(g.dy) * fp * upe) / g2;
+    const T argm = ebm + (w.ly[m] - T(g.dy) * fm * ume) / g2;
+    const T Qp[3] = {(-H < argp) ? argp : -H, Rpy, Rpx}, Qm[3] = {(-H < argm) ? argm : -H, Rmy, Rmx};
+    T lQp[3] = {0, 0, 0}, lQm[3] = {0, 0, 0};
+    flux_rev(g, Qm, Qp, H, lfl, lQm, lQp);
+    T L[6] = {0, 0, 0, 0, 0, 0}, R[6] = {0, 0, 0, 0, 0, 0};
+    if (-H < argp) {
+      R[0] += lQp[0];
+      R[5] -= lQp[0] / g2;
+      const T lupe = lQp[0] * T(g.dy) * fp / g2;
+      R[1] += sp * ex * lupe;
+      R[2] += ey * lupe;
+    }
+    if (-H < argm) {
+      L[0] += lQm[0];
+      L[5] += lQm[0] / g2;
+      const T lume = -lQm[0] * T(g.dy) * fm / g2;
+      L[1] += sm * ex * lume;
+      L[2] += ey * lume;
+    }
+    // Qp = (eta, Rpy, Rpx): index 1 is v, index 2 is u
+    R[2] += lQp[1];
+    R[4] -= T(0.5) * lQp[1];
+    R[1] += lQp[2];
+    R[3] -= T(0.5) * lQp[2];
+    L[2] += lQm[1];
+    L[4] += T(0.5) * lQm[1];
+    L[1] += lQm[2];
+    L[3] += T(0.5) * lQm[2];
+    for (int i = 0; i < 6; ++i) {
+      rv.yl[i][o] = L[i];
+      rv.yr[i][o] = R[i];
+    }
+  }
+}
+
+// minmod_raw reverse: d out / d (b, c, f) for the selected argument
+template <class T> GPO_HD inline void minmod_rev(T b, T c, T f, T lout, T& lb, T& lc, T& lf) {
+  const T sb = sgn1(b), sc = sgn1(c), sf = sgn1(f);
+  const T s = T(0.25) * sb * (sb + sc) * (sc + sf);
+  if (s == T(0)) return;
+  const T ab = tabs(b), ac = tabs(c), af = tabs(f);
+  int sel = (ac < ab) ? 1 : 0;
+  const T m1 = sel ? ac : ab;
+  if (af < m1) sel = 2;
+  const T l = lout * s;
+  if (sel == 0) lb += l * sb;
+  else if (sel == 1) lc += l * sc;
+  else lf += l * sf;
+}
+
+// Reverse of the x-slope computation at (r, s) given the slope sensitivities;
+// adds the contributions to cells s-1, s, s+1 (index 0, 1, 2) into le/lu/lv.
+template <class T>
+GPO_HD void xslope_rev(const Geo& g, const Fwd<T>& w, const T* Hi, const T* fc, int r, int s, T lq0, T lq1,
+                           T lK, T* le, T* lu, T* lv) {
+  const size_t o = (size_t)r * g.W + s;
+  const T le_ = w.ce[o - 1], ce = w.ce[o], re = w.ce[o + 1];
+  const T lu_ = w.cu[o - 1], cu = w.cu[o], ru = w.cu[o + 1];
+  const T lv0 = w.cv[o - 1], cv0 = w.cv[o], rv0 = w.cv[o + 1];
+  const T th = T(g.theta);
+  {  // q0 = minmod_slope(lu, cu, ru), q1 = minmod_slope(lv, cv, rv)
+    T lb = 0, lc = 0, lf = 0;
+    minmod_rev((cu - lu_) * th, (ru - lu_) * T(0.5), (ru - cu) * th, lq0, lb, lc, lf);
+    lu[1] += th * lb - th * lf;
+    lu[0] += -th * lb - T(0.5) * lc;
+    lu[2] += T(0.5) * lc + th * lf;
+    lb = lc = lf = 0;
+    minmod_rev((cv0 - lv0) * th, (rv0 - lv0) * T(0.5), (rv0 - cv0) * th, lq1, lb, lc, lf);
+    lv[1] += th * lb - th * lf;
+    lv[0] += -th * lb - T(0.5) * lc;
+    lv[2] += T(0.5) * lc + th * lf;
+  }
+  T sl = 1, sc = 1, sr = 1;
+  if (g.bc_w == 1) {
+    if (s < 3) sl = -1;
+    if (s < 2) sc = -1;
+  }
+  if (g.bc_e == 1) {
+    if (s > g.nx) sr = -1;
+    if (s > g.nx + 1) sc = -1;
+  }
+  const T nx_ = T(g.north_x), ny_ = T(g.north_y), G = T(g.g), Vc = T(g.dx) / (T(2) * G), g2 = T(2) * G;
+  const T Fl = fcor(g, fc, r, s - 1), Fc = fcor(g, fc, r, s), Fr = fcor(g, fc, r, s + 1);
+  const T lfv = (nx_ * lu_ + ny_ * (sl * lv0)) * Fl;
+  const T cfv = (nx_ * cu + ny_ * (sc * cv0)) * Fc;
+  const T rfv = (nx_ * ru + ny_ * (sr * rv0)) * Fr;
+  const T b = th * G * (ce - le_ - Vc * (cfv + lfv));
+  const T c = T(0.5) * G * (re - le_ - Vc * (rfv + T(2) * cfv + lfv));
+  const T f = th * G * (re - ce - Vc * (cfv + rfv));
+  const T K0 = minmod_raw(b, c, f);
+  const T sa = ((g.bc_w == 1 && s < 2) || (g.bc_e == 1 && s > g.nx + 2)) ? T(-1) : T(1);
+  const T dxfv = T(g.dx) * Fc * (sa * cv0);
+  const T Hw = T(0.5) * (corner(g, Hi, r, s) + corner(g, Hi, r + 1, s));
+  const T He = T(0.5) * (corner(g, Hi, r, s + 1) + corner(g, Hi, r + 1, s + 1));
+  const T hw = ce + Hw - (K0 + dxfv) / g2;
+  const T he = ce + He + (K0 + dxfv) / g2;
+  T lK1 = lK, ldxfv = 0, lce = 0;
+  if (!(he > T(0))) {
+    ldxfv -= lK;
+    lce -= g2 * lK;
+    lK1 = 0;
+  }
+  T lK0 = lK1;
+  if (!(hw > T(0))) {
+    ldxfv -= lK1;
+    lce += g2 * lK1;
+    lK0 = 0;
+  }
+  lv[1] += sa * T(g.dx) * Fc * ldxfv;
+  T lb = 0, lc = 0, lf = 0;
+  minmod_rev(b, c, f, lK0, lb, lc, lf);
+  lce += th * G * lb - th * G * lf;
+  const T lle = -th * G * lb - T(0.5) * G * lc;
+  const T lre = T(0.5) * G * lc + th * G * lf;
+  const T lcfv = -th * G * Vc * lb - G * Vc * lc - th * G * Vc * lf;
+  const T llfv = -th * G * Vc * lb - T(0.5) * G * Vc * lc;
+  const T lrfv = -T(0.5) * G * Vc * lc - th * G * Vc * lf;
+  le[0] += lle;
+  le[1] += lce;
+  le[2] += lre;
+  lu[0] += nx_ * Fl * llfv;
+  lv[0] += sl * ny_ * Fl * llfv;
+  lu[1] += nx_ * Fc * lcfv;
+  lv[1] += sc * ny_ * Fc * lcfv;
+  lu[2] += nx_ * Fr * lrfv;
+  lv[2] += sr * ny_ * Fr * lrfv;
+}
+
+template <class T>
+GPO_HD void yslope_rev(const Geo& g, const Fwd<T>& w, const T* Hi, const T* fc, int s, int c, T lq0, T lq1,
+                           T lL, T* le, T* lu, T* lv) {
+  const size_t o = (size_t)s * g.W + c, up = g.W;
+  const T le_ = w.ce[o - up], ce = w.ce[o], ue = w.ce[o + up];
+  const T lu0 = w.cu[o - up], cu0 = w.cu[o], uu0 = w.cu[o + up];
+  const T lv_ = w.cv[o - up], cv = w.cv[o], uv = w.cv[o + up];
+  const T th = T(g.theta);
+  {
+    T lb = 0, lc = 0, lf = 0;
+    minmod_rev((cu0 - lu0) * th, (uu0 - lu0) * T(0.5), (uu0 - cu0) * th, lq0, lb, lc, lf);
+    lu[1] += th * lb - th * lf;
+    lu[0] += -th * lb - T(0.5) * lc;
+    lu[2] += T(0.5) * lc + th * lf;
+    lb = lc = lf = 0;
+    minmod_rev((cv - lv_) * th, (uv - lv_) * T(0.5), (uv - cv) * th, lq1, lb, lc, lf);
+    lv[1] += th * lb - th * lf;
+    lv[0] += -th * lb - T(0.5) * lc;
+    lv[2] += T(0.5) * lc + th * lf;
+  }
+  T sl = 1, sc = 1, su = 1;
+  if (g.bc_s == 1) {
+    if (s < 3) sl = -1;
+    if (s < 2) sc = -1;
+  }
+  if (g.bc_n == 1) {
+    if (s > g.ny) su = -1;
+    if (s > g.ny + 1) sc = -1;
+  }
+  const T ex = T(g.north_y), ey = T(-g.north_x), G = T(g.g), Uc = T(g.dy) / (T(2) * G), g2 = T(2) * G;
+  const T Fl = fcor(g, fc, s - 1, c), Fc = fcor(g, fc, s, c), Fu = fcor(g, fc, s + 1, c);
+  const T lfu = (ex * (sl * lu0) + ey * lv_) * Fl;
+  const T cfu = (ex * (sc * cu0) + ey * cv) * Fc;
+  const T ufu = (ex * (su * uu0) + ey * uv) * Fu;
+  const T b = th * G * (ce - le_ + Uc * (cfu + lfu));
+  const T cc = T(0.5) * G * (ue - le_ + Uc * (ufu + T(2) * cfu + lfu));
+  const T f = th * G * (ue - ce + Uc * (cfu + ufu));
+  const T L0 = minmod_raw(b, cc, f);
+  const T sa = ((g.bc_s == 1 && s < 2) || (g.bc_n == 1 && s > g.ny + 2)) ? T(-1) : T(1);
+  const T dyfu = T(g.dy) * Fc * (sa * cu0);
+  const T Hs = T(0.5) * (corner(g, Hi, s, c) + corner(g, Hi, s, c + 1));
+  const T Hn = T(0.5) * (corner(g, Hi, s + 1, c) + corner(g, Hi, s + 1, c + 1));
+  const T hs = ce + Hs - (L0 - dyfu) / g2;
+  const T hn = ce + Hn + (L0 - dyfu) / g2;
+  T lL1 = lL, ldyfu = 0, lce = 0;
+  if (!(hn > T(0))) {
+    ldyfu += lL;
+    lce -= g2 * lL;
+    lL1 = 0;
+  }
+  T lL0 = lL1;
+  if (!(hs > T(0))) {
+    ldyfu += lL1;
+    lce += g2 * lL1;
+    lL0 = 0;
+  }
+  lu[1] += sa * T(g.dy) * Fc * ldyfu;
+  T lb = 0, lc = 0, lf = 0;
+  minmod_rev(b, cc, f, lL0, lb, lc, lf);
+  lce += th * G * lb - th * G * lf;
+  const T lle = -th * G * lb - T(0.5) * G * lc;
+  const T lue = T(0.5) * G * lc + th * G * lf;
+  const T lcfu = th * G * Uc * lb + G * Uc * lc + th * G * Uc * lf;
+  const T llfu = th * G * Uc * lb + T(0.5) * G * Uc * lc;
+  const T lufu = T(0.5) * G * Uc * lc + th * G * Uc * lf;
+  le[0] += lle;
+  le[1] += lce;
+  le[2] += lue;
+  lu[0] += sl * ex * Fl * llfu;
+  lv[0] += ey * Fl * llfu;
+  lu[1] += sc * ex * Fc * lcfu;
+  lv[1] += ey * Fc * lcfu;
+  lu[2] += su * ex * Fu * lufu;
+  lv[2] += ey * Fu * lufu;
+}
+
+// A2: per-cell gather of every sensitivity of the converted cell (r, c), then
+// the conversion reverse; result in rv.le/lu/lv (all cells).
+template <class T>
+GPO_HD void a2_cell(Geo g, T dt, int rk_step, const T* Hi, const T* fc, Fwd<T> w, Rev<T> rv, const T* in_e,
+                         const T* in_u, const T* in_v, const T* out_e, const T* lo_e, const T* lo_u,
+                         const T* lo_v, int r, int c) {
+  if (c >= g.nx + 4 || r >= g.ny + 4) return;
+  const size_t o = (size_t)r * g.W + c;
+  const bool row_in = r >= 2 && r <= g.ny + 1, col_in = c >= 2 && c <= g.nx + 1;
+  T lR0 = 0, lR1 = 0, lR2 = 0;
+  const T G = T(g.g);
+  if (row_in && col_in) {
+    // update + sources (CDKLM16_kernel.cu:869-1089), reversed
+    const CellAdj<T> a = out_adj(g, Hi, out_e, lo_e, lo_u, lo_v, r, c);
+    const T coef = rk_step == 0 ? T(1) : T(0.5);
+    const T hm = cell_depth(g, Hi, r, c);
+    const T R0 = w.ce[o], R1 = w.cu[o], R2 = w.cv[o];
+    lR0 += coef * a.e;
+    T lhu = coef * a.u, lhv = coef * a.v;
+    const T lst1 = coef * dt * a.u, lst2 = coef * dt * a.v;
+    const T h = R0 + hm;
+    if (h >= T(g.depth_cutoff)) {
+      const T RHxp = T(0.5) * (corner(g, Hi, r, c + 1) + corner(g, Hi, r + 1, c + 1));
+      const T RHxm = T(0.5) * (corner(g, Hi, r, c) + corner(g, Hi, r + 1, c));
+      const T RHyp = T(0.5) * (corner(g, Hi, r + 1, c) + corner(g, Hi, r + 1, c + 1));
+      const T RHym = T(0.5) * (corner(g, Hi, r, c) + corner(g, Hi, r, c + 1));
+      // bathymetry terms: eta_we = (eta_west + eta_east)/2 depends on R0 only
+      lR0 += G * (RHxp - RHxm) * (lst1 / T(g.dx));
+      lR0 += G * (RHyp - RHym) * (lst2 / T(g.dy));
+      // Coriolis terms (linear in hu, hv)
+      const T f = fcor(g, fc, r, c);
+      const T nx_ = T(g.north_x), ny_ = T(g.north_y), ex = ny_, ey = -nx_;
+      const T upx = -nx_, upy = ny_, rtx = upy, rty = -upx;
+      const T lhu_east_cor = rtx * lst1 + upx * lst2;
+      const T lhv_north_cor = rty * lst1 + upy * lst2;
+      const T lhv_north = f * lhu_east_cor;
+      const T lhu_east = -f * lhv_north_cor;
+      lhu += ex * lhu_east + nx_ * lhv_north;
+      lhv += ey * lhu_east + ny_ * lhv_north;
+    }
+    if (h > T(g.depth_cutoff)) {  // hu = R1 h, hv = R2 h
+      lR1 += lhu * h;
+      lR2 += lhv * h;
+      lR0 += lhu * R1 + lhv * R2;
+    }
+  }
+  // face parts and slope subgradients
+  if (row_in) {
+    if (c >= 1 && c <= g.nx + 1) {
+      lR0 += rv.xl[0][o];
+      lR1 += rv.xl[1][o];
+      lR2 += rv.xl[2][o];
+    }
+    if (c - 1 >= 1 && c - 1 <= g.nx + 1) {
+      lR0 += rv.xr[0][o - 1];
+      lR1 += rv.xr[1][o - 1];
+      lR2 += rv.xr[2][o - 1];
+    }
+    for (int s = c - 1; s <= c + 1; ++s) {
+      if (s < 1 || s > g.nx + 2) continue;
+      const size_t os = (size_t)r * g.W + s;
+      T lq0 = 0, lq1 = 0, lK = 0;
+      if (s <= g.nx + 1) {
+        lq0 += rv.xl[3][os];
+        lq1 += rv.xl[4][os];
+        lK += rv.xl[5][os];
+      }
+      if (s - 1 >= 1) {
+        lq0 += rv.xr[3][os - 1];
+        lq1 += rv.xr[4][os - 1];
+        lK += rv.xr[5][os - 1];
+      }
+      if (lq0 == T(0) && lq1 == T(0) && lK == T(0)) continue;
+      T e3[3] = {0, 0, 0}, u3[3] = {0, 0, 0}, v3[3] = {0, 0, 0};
+      xslope_rev(g, w, Hi, fc, r, s, lq0, lq1, lK, e3, u3, v3);
+      const int j = c - s + 1;
+      lR0 += e3[j];
+      lR1 += u3[j];
+      lR2 += v3[j];
+    }
+  }
+  if (col_in) {
+    if (r >= 1 && r <= g.ny + 1) {
+      lR0 += rv.yl[0][o];
+      lR1 += rv.yl[1][o];
+      lR2 += rv.yl[2][o];
+    }
+    if (r - 1 >= 1 && r - 1 <= g.ny + 1) {
+      lR0 += rv.yr[0][o - g.W];
+      lR1 += rv.yr[1][o - g.W];
+      lR2 += rv.yr[2][o - g.W];
+    }
+    for (int s = r - 1; s <= r + 1; ++s) {
+      if (s < 1 || s > g.ny + 2) continue;
+      const size_t os = (size_t)s * g.W + c;
+      T lq0 = 0, lq1 = 0, lL = 0;
+      if (s <= g.ny + 1) {
+        lq0 += rv.yl[3][os];
+        lq1 += rv.yl[4][os];
+        lL += rv.yl[5][os];
+      }
+      if (s - 1 >= 1) {
+        lq0 += rv.yr[3][os - g.W];
+        lq1 += rv.yr[4][os - g.W];
+        lL += rv.yr[5][os - g.W];
+      }
+      if (lq0 == T(0) && lq1 == T(0) && lL == T(0)) continue;
+      T e3[3] = {0, 0, 0}, u3[3] = {0, 0, 0}, v3[3] = {0, 0, 0};
+      yslope_rev(g, w, Hi, fc, s, c, lq0, lq1, lL, e3, u3, v3);
+      const int j = r - s + 1;
+      lR0 += e3[j];
+      lR1 += u3[j];
+      lR2 += v3[j];
+    }
+  }
+  // conversion reverse: u = hu/h, v = hv/h, h = eta + Hm (wet branch)
+  const Src src = map_cell(g, r, c);
+  const size_t k = (size_t)src.r * g.sp + src.c;
+  const T e = in_e[k];
+  const T Hm = cell_depth(g, Hi, r, c);
+  const T h = e + Hm;
+  T oe = 0, ou = 0, ov = 0;
+  if (h >= T(g.desing_eps)) {
+    const T u = w.cu[o], v = w.cv[o];
+    ou = lR1 / h;
+    ov = lR2 / h;
+    oe = lR0 - (lR1 * u + lR2 * v) / h;
+  }
+  rv.le[o] = oe;
+  rv.lu[o] = ou * T(src.shu);
+  rv.lv[o] = ov * T(src.shv);
+}
+
+// A3: fold ghost-cell sensitivities onto the interior cells they came from;
+// write lambda_in and, for the second RK stage, add the direct Q^n term.
+template <class T>
+GPO_HD void a3_cell(Geo g, int rk_step, int accumulate, const T* Hi, Rev<T> rv, const T* out_e, const T* lo_e,
+                        const T* lo_u, const T* lo_v, T* li_e, T* li_u, T* li_v, T* lq_e, T* lq_u, T* lq_v, int r, int c) {
+  if (c > g.nx + 1 || r > g.ny + 1) return;
+  const size_t o = (size_t)r * g.W + c;
+  T e = rv.le[o], u = rv.lu[o], v = rv.lv[o];
+  const int gc[4] = {0, 1, g.nx + 2, g.nx + 3};
+  for (int i = 0; i < 4; ++i) {
+    const Src s = map_cell(g, r, gc[i]);
+    if (s.r == r && s.c == c && !(gc[i] == c)) {
+      const size_t q = (size_t)r * g.W + gc[i];
+      e += rv.le[q];
+      u += rv.lu[q];
+      v += rv.lv[q];
+    }
+  }
+  const int gr[4] = {0, 1, g.ny + 2, g.ny + 3};
+  for (int i = 0; i < 4; ++i) {
+    const Src s = map_cell(g, gr[i], c);
+    if (s.r == r && s.c == c && !(gr[i] == r)) {
+      const size_t q = (size_t)gr[i] * g.W + c;
+      e += rv.le[q];
+      u += rv.lu[q];
+      v += rv.lv[q];
+    }
+  }
+  const size_t k = (size_t)r * g.sp + c;
+  if (rk_step == 1 && lq_e) {  // read lambda_out before lambda_in may overwrite it (they may alias)
+    const CellAdj<T> a = out_adj(g, Hi, out_e, lo_e, lo_u, lo_v, r, c);
+    lq_e[k] += T(0.5) * a.e;
+    lq_u[k] += T(0.5) * a.u;
+    lq_v[k] += T(0.5) * a.v;
+  }
+  if (accumulate) {
+    li_e[k] += e;
+    li_u[k] += u;
+    li_v[k] += v;
+  } else {
+    li_e[k] = e;
+    li_u[k] = u;
+    li_v[k] = v;
+  }
+}
+
+}  // namespace adj
+}  // namespace gpo

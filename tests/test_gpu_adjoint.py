@@ -65,8 +65,10 @@ BCS = [Common.BoundaryConditions(1, 1, 1, 1), Common.BoundaryConditions(1, 2, 1,
 def test_f64_forward_matches_f64_oracle(gpu_ctx, bc):
     print(f"f64 forward vs oracle, bc={bc.packed():#x}", flush=True)
     kw, sim, model, bg = _model(gpu_ctx, n=48, bc=bc, n_steps=15)
+    # the FP64 GPU build widens the float32 parameter block, so the oracle gets
+    # the float32 value of the flux balancer (its f64 build would widen 0.8)
     ora = O.OracleCDKLM16(**{**kw, "eta0": bg[0], "hu0": bg[1], "hv0": bg[2], "dt": float(model.dt)},
-                          precision="f64")
+                          flux_balancer=float(np.float32(0.8)), precision="f64")
     ora.run_steps(15, float(model.dt))
     Qs, _ = model.trajectory_f64(*[torch.from_numpy(a).double().cuda() for a in bg], n_steps=15)
     for a, b in zip(Qs[-1].views(), ora.download()):
