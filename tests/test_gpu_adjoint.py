@@ -101,7 +101,20 @@ def test_dot_product_f64(gpu_ctx, bc):
     assert model.flag_value() == 0
 
 
-def _fd_check(model, bg, n_dirs, rng, tol):
+def _smooth_direction(shape, rng, scale):
+    """Seeded smooth random field (Gaussian-filtered, 4 cells) on the interior:
+    a perturbation whose own slopes are small, so that a step of 1e-6 along it
+    crosses few minmod branch switches."""
+    a = np.zeros(shape)
+    f = cases.smooth_random_field(shape, 4.0, int(rng.integers(1 << 30)))
+    a[2:-2, 2:-2] = scale * (f / np.max(np.abs(f)))[2:-2, 2:-2]
+    return a
+
+
+def _fd_check(model, bg, n_dirs, rng, tol64, tol32):
+    """Directional derivatives <g, d> of the FP32 product gradient and of the
+    FP64 adjoint build against FP64 central differences (eps 1e-6) along
+    smooth random directions; errors relative to the RMS of the FD values."""
     x0 = [torch.from_numpy(a).double().cuda() for a in bg]
     x32 = [torch.from_numpy(a).cuda() for a in bg]
     J32, _ = model.forward(*x32)
@@ -110,28 +123,32 @@ def _fd_check(model, bg, n_dirs, rng, tol):
     g64 = [a.cpu().numpy() for a in g64]
     assert abs(J32 - J64) <= 1e-3 * abs(J64)
     eps = 1e-6
-    worst32 = worst64 = 0.0
+    fds, a32, a64 = [], [], []
     for _ in range(n_dirs):
-        d = [_interior_noise(bg[0].shape, rng, s) for s in (1e-2, 1.0, 1.0)]
+        d = [_smooth_direction(bg[0].shape, rng, s) for s in (1e-2, 1.0, 1.0)]
         xp = [a + eps * torch.from_numpy(b).cuda() for a, b in zip(x0, d)]
         xm = [a - eps * torch.from_numpy(b).cuda() for a, b in zip(x0, d)]
-        fd = (model.misfit_f64(*xp) - model.misfit_f64(*xm)) / (2 * eps)
-        ad32 = sum(float((gg * dd).sum()) for gg, dd in zip(g32, d))
-        ad64 = sum(float((gg * dd).sum()) for gg, dd in zip(g64, d))
-        e32, e64 = abs(ad32 - fd) / abs(fd), abs(ad64 - fd) / abs(fd)
-        print(f"  FD {fd:.10e}  FP32 adjoint {ad32:.10e} ({e32:.2e})  FP64 adjoint {ad64:.10e} ({e64:.2e})",
-              flush=True)
-        worst32, worst64 = max(worst32, e32), max(worst64, e64)
-    assert worst32 <= tol, worst32
-    assert worst64 <= tol, worst64
-    return worst32, worst64
+        fds.append((model.misfit_f64(*xp) - model.misfit_f64(*xm)) / (2 * eps))
+        a32.append(sum(float((gg * dd).sum()) for gg, dd in zip(g32, d)))
+        a64.append(sum(float((gg * dd).sum()) for gg, dd in zip(g64, d)))
+    fds, a32, a64 = np.array(fds), np.array(a32), np.array(a64)
+    rms = np.sqrt(np.mean(fds ** 2))
+    e32, e64 = np.abs(a32 - fds) / rms, np.abs(a64 - fds) / rms
+    for i in range(n_dirs):
+        print(f"  FD {fds[i]: .10e}  FP32 adjoint {a32[i]: .10e} ({e32[i]:.2e})  "
+              f"FP64 adjoint {a64[i]: .10e} ({e64[i]:.2e})", flush=True)
+    print(f"  worst: FP32 {e32.max():.2e}, FP64 {e64.max():.2e} (relative to the RMS directional derivative)",
+          flush=True)
+    assert e64.max() <= tol64, e64.max()
+    assert e32.max() <= tol32, e32.max()
+    return g32, g64
 
 
 @pytest.mark.parametrize("bc", BCS[:2], ids=["walls", "periodic_x"])
 def test_gradient_against_finite_differences_small(gpu_ctx, bc):
     print(f"FD check 64x64, bc={bc.packed():#x}", flush=True)
     kw, sim, model, bg = _model(gpu_ctx, n=64, bc=bc, n_steps=40, k=7, n_obs=30)
-    _fd_check(model, bg, 4, np.random.default_rng(2), 1e-3)
+    _fd_check(model, bg, 4, np.random.default_rng(2), 1e-5, 1e-3)
     assert model.flag_value() == 0
 
 
@@ -156,9 +173,46 @@ def test_autograd_function_matches_direct_calls(gpu_ctx):
 
 def test_c3_gradient_512(gpu_ctx):
     """C3: 512x512, 400 steps, 200 observation points, checkpoints every 20
-    steps; 16 random directions against FP64 central differences (eps 1e-6)."""
+    steps; 16 smooth random directions against FP64 central differences
+    (eps 1e-6).  The FP64 build of the adjoint agrees with the differences to
+    1e-3.  The FP32 product gradient linearises the FP32 trajectory, which after
+    400 steps differs from the FP64 one by ~2e-4 (rounding growth); the minmod
+    limiter switches branches on that scale, so it is held to 5e-2 against the
+    differences, and the FP32 adjoint arithmetic itself is pinned separately
+    along the FP64 trajectory (1e-3)."""
     print("C3 512x512 FD check", flush=True)
     kw, sim, model, bg = _model(gpu_ctx, n=512, n_steps=400, k=20, n_obs=200, seed=3)
-    w32, w64 = _fd_check(model, bg, 16, np.random.default_rng(7), 1e-3)
-    print(f"  worst relative error: FP32 adjoint {w32:.2e}, FP64 adjoint {w64:.2e}", flush=True)
+    g32, g64 = _fd_check(model, bg, 16, np.random.default_rng(7), 1e-3, 5e-2)
+    for f in range(3):
+        rel = np.linalg.norm(g32[f] - g64[f]) / np.linalg.norm(g64[f])
+        print(f"  FP32 vs FP64 gradient, field {f}: relative L2 {rel:.2e}", flush=True)
+        assert rel <= 5e-2
+    # the FP32 adjoint kernels along the FP64 trajectory (cast to float)
+    Qs, Q1s = model.trajectory_f64(*[torch.from_numpy(a).double().cuda() for a in bg])
+    st = model.stream
+
+    def cast(S):
+        T = model._new_state()
+        with torch.cuda.stream(st):
+            for a, b in zip(T.t, S.t):
+                a.copy_(b.float())
+        return T
+
+    lam, L1, X = model._new_state(), model._new_state(), model._new_state()
+    with torch.cuda.stream(st):
+        lam.zero_()
+        model.J.zero_()
+    model._misfit32(cast(Qs[-1]), lam)
+    for j in reversed(range(model.n_steps)):
+        q1, qn, qn1 = cast(Q1s[j]), cast(Qs[j]), cast(Qs[j + 1])
+        with torch.cuda.stream(st):
+            X.zero_()
+        model._adj32(1, q1, qn1, lam, L1, X, 0)
+        model._adj32(0, qn, q1, L1, X, None, 1)
+        lam, X = X, lam
+    st.synchronize()
+    for f, a in enumerate(lam.views()):
+        rel = np.linalg.norm(a.double().cpu().numpy() - g64[f]) / np.linalg.norm(g64[f])
+        print(f"  FP32 adjoint on the FP64 trajectory, field {f}: relative L2 {rel:.2e}", flush=True)
+        assert rel <= 1e-3
     assert model.flag_value() == 0
