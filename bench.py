@@ -313,6 +313,44 @@ def run_copy(n_bytes=256 << 20, reps=5):
                       "GB_per_s_read_plus_write": 2 * n_bytes / (ms * 1e-3) / 1e9}), flush=True)
 
 
+def run_adjoint(reps=3):
+    """Adjoint/forward time on the differentiable 512x512 member: 400 RK2 steps,
+    200 observation points, checkpoints every 20 steps; forward = FP32 run with
+    checkpoints + misfit, gradient = recompute of every segment + adjoint sweep."""
+    import numpy as np
+    import torch
+    import cases
+    from gpuocean_amd import CDKLM16, Common
+    from gpuocean_amd.adjoint import DifferentiableCDKLM16
+
+    ctx = Common.DeviceContext(0)
+    kw = cases.config_c3(n=512, seed=3)
+    bg, truth = cases.c3_background_and_truth(kw, seed=3)
+    sim = CDKLM16(ctx, **kw)
+    pos = cases.c3_drifters(512, 512, 200.0, 200.0, n=200, seed=3)
+    probe = DifferentiableCDKLM16(sim, pos, np.zeros_like(pos), 400, checkpoint_every=20)
+    probe.forward(*[torch.from_numpy(a).cuda() for a in truth])
+    model = DifferentiableCDKLM16(sim, pos, probe.observe(), 400, checkpoint_every=20)
+    x = [torch.from_numpy(a).cuda() for a in bg]
+    fw, bw = [], []
+    for _ in range(reps + 1):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        J, _ = model.forward(*x)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        g = model.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        fw.append(t1 - t0)
+        bw.append(t2 - t1)
+    f_ms, b_ms = 1e3 * float(np.median(fw[1:])), 1e3 * float(np.median(bw[1:]))
+    assert all(np.isfinite(a.cpu().numpy()).all() for a in g) and model.flag_value() == 0
+    log(f"[c3] forward {f_ms:.1f} ms, gradient {b_ms:.1f} ms, gradient/forward {b_ms / f_ms:.2f}")
+    return {"forward_ms": f_ms, "gradient_ms": b_ms, "ratio": b_ms / f_ms, "J": J,
+            "config": "512x512, 400 RK2 steps, 200 observation points, checkpoints every 20 steps, FP32"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -330,6 +368,7 @@ def main():
     ap.add_argument("--waves-per-simd", type=int, default=0, help="persistent grid of the batched launch")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c1", action="store_true", help="skip the single-member line of the c2 run")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward timing of the c2 run")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -351,6 +390,8 @@ def main():
         if not args.no_c1:
             c1 = run_single(cases.config_c1(), "c1", 2000, 50, args.fast, 0)
             result["single_member_c1"] = c1
+        if not args.no_adjoint:
+            result["adjoint_c3"] = run_adjoint()
         result["cpu_baseline"] = None
         if not args.no_cpu:
             log("cpu baseline (FP32 oracle, ensemble members one after another)...")

@@ -173,6 +173,27 @@ class DifferentiableCDKLM16:
         s.synchronize()
         return float(self.J.item()), A
 
+    def observe(self, state=None):
+        """(u, v) at the observation positions of a state (default: the final
+        state of the last forward run), bilinear between cell centres, as an
+        (n, 2) float32 array: the observation operator of the misfit."""
+        st = self._final if state is None else state
+        self.stream.synchronize()
+        e, hu, hv = [a.double().cpu().numpy() for a in st.views()]
+        Hi = self.Hi.buf.double().cpu().numpy()
+        Hm = 0.25 * (Hi[:-1, :-1] + Hi[1:, :-1] + Hi[:-1, 1:] + Hi[1:, 1:])[:e.shape[0], :e.shape[1]]
+        out = np.empty((self.n_obs, 2), np.float32)
+        for d, (x, y) in enumerate(self.pos.cpu().numpy()):
+            sx, sy = x / self.dx - 0.5, y / self.dy - 0.5
+            i0 = min(max(int(np.floor(sx)), 0), self.nx - 2)
+            j0 = min(max(int(np.floor(sy)), 0), self.ny - 2)
+            ax, ay = sx - i0, sy - j0
+            wq = ((1 - ax) * (1 - ay), ax * (1 - ay), (1 - ax) * ay, ax * ay)
+            cells = ((j0 + 2, i0 + 2), (j0 + 2, i0 + 3), (j0 + 3, i0 + 2), (j0 + 3, i0 + 3))
+            out[d, 0] = sum(w * hu[c] / (e[c] + Hm[c]) for w, c in zip(wq, cells))
+            out[d, 1] = sum(w * hv[c] / (e[c] + Hm[c]) for w, c in zip(wq, cells))
+        return out
+
     def _segment_buffers(self):
         if self._seg is None:
             self._seg = ([self._new_state() for _ in range(self.k + 1)], [self._new_state() for _ in range(self.k)])
