@@ -590,6 +590,17 @@ class CDKLM16:
                             self.dy / np.max(np.abs(v) + gravity_waves))
         return courant_number * max_dt
 
+    def mass(self):
+        """Sum of the interior eta of the latest time step, accumulated in FP64
+        on the device (the quantity ConservationOfMass_test.py sums on the host)."""
+        with torch.cuda.stream(self.gpu_stream):
+            out = torch.empty(1, dtype=torch.float64, device=self.device)
+        L.check(L.lib().gpo_mass_sum(int(self.nx), int(self.ny), ctypes.c_void_p(self.h0[0].data_ptr()),
+                                     self.h0[0].pitch, ctypes.c_void_p(out.data_ptr()), self._stream_handle()),
+                "gpo_mass_sum")
+        self.gpu_stream.synchronize()
+        return float(out.item())
+
     def download(self, interior_domain_only=False):
         """(eta, hu, hv) of the latest time step (Simulator.py:348-361)."""
         self.gpu_stream.synchronize()

@@ -81,6 +81,7 @@ _SIGNATURES = [
     ("gpo_max_dt", c_int32,
      [c_int32, c_int32, c_float, c_float, c_float, c_float, c_void_p, c_void_p, c_void_p, c_int32,
       c_void_p, c_int32, c_void_p, c_void_p]),
+    ("gpo_mass_sum", c_int32, [c_int32, c_int32, c_void_p, c_int32, c_void_p, c_void_p]),
     ("gpo_philox4x32_10", c_int32, [c_void_p, c_void_p, c_void_p, c_int32, c_void_p]),
     ("gpo_random_normal", c_int32, [c_void_p, c_int32, c_int32, c_int32, ctypes.c_uint64, c_uint32, c_uint32,
                                     c_void_p]),

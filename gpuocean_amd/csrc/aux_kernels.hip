@@ -399,6 +399,22 @@ int make_step_args(const gpo_cdklm16_params* p, float dt, int32_t rk_step, const
 
 using namespace gpo;
 
+namespace {
+
+__global__ void fill_double(double* p, double v) { *p = v; }
+
+// FP64 sum of the interior eta of one row per workgroup (row j+2): wave
+// reduction, one atomic per wavefront.
+__global__ void __launch_bounds__(256) mass_sum_kernel(int nx, const float* eta, int pitch, double* out) {
+  const float* row = (const float*)((const char*)eta + (size_t)pitch * (blockIdx.x + 2));
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < nx; i += 256) acc += (double)row[i + 2];
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off);
+  if ((threadIdx.x & 63) == 0) atomicAdd(out, acc);
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* gpo_last_error(void) { return g_last_error.c_str(); }
@@ -530,6 +546,14 @@ int gpo_max_dt(int32_t nx, int32_t ny, float dx, float dy, float g, float land_v
   hipLaunchKernelGGL(max_dt_kernel, dim3(ceil_div(nx, 64), ceil_div(ny, 4)), dim3(256), 0, s, nx, ny, dx, dy, g,
                      land_value, eta, hu, hv, pitch, Bm, Bm_pitch, dt_out);
   return check(hipGetLastError(), "gpo_max_dt launch");
+}
+
+int gpo_mass_sum(int32_t nx, int32_t ny, const float* eta, int32_t pitch, double* sum_out, void* stream) {
+  if (nx < 1 || ny < 1 || !eta || !sum_out || pitch < 4 * (nx + 4)) return bad("gpo_mass_sum: bad arguments");
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(fill_double, dim3(1), dim3(1), 0, s, sum_out, 0.0);
+  hipLaunchKernelGGL(mass_sum_kernel, dim3(ny), dim3(256), 0, s, nx, eta, pitch, sum_out);
+  return check(hipGetLastError(), "gpo_mass_sum launch");
 }
 
 }  // extern "C"

@@ -139,6 +139,11 @@ int gpo_max_dt(int32_t nx, int32_t ny, float dx, float dy, float g, float land_v
                const float* eta, const float* hu, const float* hv, int32_t pitch,
                const float* Bm, int32_t Bm_pitch, float* dt_out, void* stream);
 
+/* Global mass: *sum_out (device double) = sum of the interior eta, summed in
+ * FP64.  The conserved quantity of wall/periodic runs
+ * (test/schemes/ConservationOfMass_test.py:126-134 sums eta on the host). */
+int gpo_mass_sum(int32_t nx, int32_t ny, const float* eta, int32_t pitch, double* sum_out, void* stream);
+
 /* ---- stochastic model error (SOAR) ---------------------------------------- */
 
 /* Raw Philox4x32-10 blocks (Salmon et al., SC'11): out[4i..4i+3] for counter
