@@ -449,8 +449,8 @@ class CDKLM16:
             local_dt = np.float32(min(self.dt, np.float32(t_end - t_now)))
             self._one_step(local_dt, apply_stochastic_term)
             t_now += np.float64(local_dt)
-        if not self._ghosts_read_each_step():
-            self._apply_bc(self.h0)
+        if not self._bc_each_step():
+            self._apply_bc(self.h0)   # idempotent wall/periodic ghosts, written once at the end
         return self.t
 
     def step_dts(self, dts, apply_stochastic_term=True):
@@ -458,12 +458,20 @@ class CDKLM16:
         ensemble step that the batched launch does not cover)."""
         for local_dt in dts:
             self._one_step(np.float32(local_dt), apply_stochastic_term)
-        if not self._ghosts_read_each_step():
+        if not self._bc_each_step():
             self._apply_bc(self.h0)
         return self.t
 
     def _ghosts_read_each_step(self):
         return (self.model_error is not None) or self.hasDrifters
+
+    def _bc_each_step(self):
+        """Whether the boundary conditions are applied after every step, as
+        the reference does (CDKLM16.py:571-573): needed when noise or
+        drifters read the ghost cells and for flow relaxation, which is not
+        idempotent.  Wall and periodic ghosts alone are written once at the
+        end of step()."""
+        return self._ghosts_read_each_step() or self._needs_explicit_bc_each_substep()
 
     def _one_step(self, local_dt, apply_stochastic_term):
         forcing = self._update_forcing()
@@ -471,7 +479,7 @@ class CDKLM16:
         self._rk_step(local_dt, forcing)
         if self.model_error is not None and apply_stochastic_term:
             self.perturbState()
-        if self._ghosts_read_each_step() or self._needs_explicit_bc_each_substep():
+        if self._bc_each_step():
             self._apply_bc(self.h0)
         self.drifterStep(local_dt)
         self.t += np.float64(local_dt)
