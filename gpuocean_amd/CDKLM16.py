@@ -493,6 +493,7 @@ class CDKLM16:
         self.num_iterations += n
         if self.model_error is not None:
             self.model_error.draw += n
+            self.model_error._last_scales = (1.0, 1.0)   # the batched model error: q0_scale 1, perturbation_scale 1
         self._apply_bc(self.h0)
 
     def dataAssimilationStep(self, observation_time, otherSim=None, model_error_final_step=True, write_now=True,

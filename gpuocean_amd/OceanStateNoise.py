@@ -73,6 +73,13 @@ class OceanStateNoise:
         self.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
         self.stream_id = int(stream_id)
         self.draw = 0
+        # Draw index whose normal field random_numbers holds, and the
+        # (draw, q0_scale, perturbation_scale) coarse_buffer was formed from.
+        # Batched ensemble steps apply the model error without filling these
+        # buffers; the getters regenerate them from the counter-based stream.
+        self._xi_draw = None
+        self._coarse_key = None
+        self._last_scales = (1.0, 1.0)
 
         self.soar_q0 = np.float32(self.dx / 100000) if soar_q0 is None else np.float32(soar_q0)
         self.soar_L = np.float32(0.75 * self.coarse_dx) if soar_L is None else np.float32(soar_L)
@@ -104,20 +111,46 @@ class OceanStateNoise:
     def resetSeed(self):
         self.draw = 0
 
+    def _refresh_random_numbers(self):
+        """random_numbers = the normal field of the latest draw."""
+        if self.draw > 0 and self._xi_draw != self.draw - 1:
+            self._fill(self.draw - 1)
+
+    def _refresh_coarse(self):
+        if self.draw > 0 and self._coarse_key != (self.draw - 1,) + tuple(self._last_scales):
+            self._refresh_random_numbers()
+            self._soar(*self._last_scales)
+
     def getRandomNumbers(self):
+        self._refresh_random_numbers()
         self.gpu_stream.synchronize()
         return self.random_numbers.cpu().numpy()
 
     def getCoarseBuffer(self):
+        self._refresh_coarse()
         self.gpu_stream.synchronize()
         return self.coarse_buffer.cpu().numpy()
 
-    def generateNormalDistribution(self):
-        """Next standard normal field into random_numbers (one draw)."""
+    def _fill(self, draw):
         rows, cols = self.random_numbers.shape
         L.check(L.lib().gpo_random_normal(ctypes.c_void_p(self.random_numbers.data_ptr()), rows, cols, 4 * cols,
-                                          self.seed, self.stream_id, self.draw, self._stream()),
+                                          self.seed, self.stream_id, int(draw), self._stream()),
                 "gpo_random_normal")
+        self._xi_draw = int(draw)
+
+    def _soar(self, q0_scale, perturbation_scale):
+        W = self.weights(q0_scale)
+        cols = int(self.rand_nx)
+        L.check(L.lib().gpo_soar(int(self.coarse_nx), int(self.coarse_ny), W.ctypes.data,
+                                 float(np.float32(perturbation_scale)), int(self.periodicNorthSouth),
+                                 int(self.periodicEastWest), ctypes.c_void_p(self.random_numbers.data_ptr()),
+                                 4 * cols, ctypes.c_void_p(self.coarse_buffer.data_ptr()),
+                                 4 * (int(self.coarse_nx) + 4), 0, self._stream()), "gpo_soar")
+        self._coarse_key = (self._xi_draw, q0_scale, perturbation_scale)
+
+    def generateNormalDistribution(self):
+        """Next standard normal field into random_numbers (one draw)."""
+        self._fill(self.draw)
         self.draw += 1
 
     def weights(self, q0_scale=1.0):
@@ -135,6 +168,7 @@ class OceanStateNoise:
         """Perturb simToPerturb with modelError's current random field (the
         SOAR counterpart of ModelErrorKL.perturbSimSimilarAs, ModelErrorKL.py:450)."""
         assert modelError.random_numbers.shape == self.random_numbers.shape
+        modelError._refresh_random_numbers()
         modelError.gpu_stream.synchronize()
         with torch.cuda.stream(self.gpu_stream):
             self.random_numbers.copy_(modelError.random_numbers)
@@ -152,14 +186,11 @@ class OceanStateNoise:
             raise NotImplementedError("coarse-grid alignment offsets are not implemented")
         if update_random_field:
             self.generateNormalDistribution()
-        W = self.weights(q0_scale)
-        cols = int(self.rand_nx)
+        else:
+            self._refresh_random_numbers()
+        self._last_scales = (q0_scale, perturbation_scale)
+        self._soar(q0_scale, perturbation_scale)
         lib = L.lib()
-        L.check(lib.gpo_soar(int(self.coarse_nx), int(self.coarse_ny), W.ctypes.data, float(np.float32(perturbation_scale)),
-                             int(self.periodicNorthSouth), int(self.periodicEastWest),
-                             ctypes.c_void_p(self.random_numbers.data_ptr()), 4 * cols,
-                             ctypes.c_void_p(self.coarse_buffer.data_ptr()), 4 * (int(self.coarse_nx) + 4), 0,
-                             self._stream()), "gpo_soar")
         nxv = ny_ = None
         if north is not None:
             nxv, ny_ = north.north_x, north.north_y

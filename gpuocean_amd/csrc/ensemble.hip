@@ -33,37 +33,101 @@ struct NoiseMember {
   const float* f_cells;
 };
 
-__global__ void __launch_bounds__(256) noise_fill(const NoiseMember* __restrict__ members,
-                                                  const uint32_t* __restrict__ draw) {
-  const NoiseMember& m = members[blockIdx.y];
-  const uint32_t d = draw[blockIdx.y];
-  const int n = m.xi_rows * m.xi_cols;
-  for (int idx = blockIdx.x * 256 + threadIdx.x; idx < n; idx += gridDim.x * 256) {
-    const int j = idx / m.xi_cols, i = idx - j * m.xi_cols;
-    mrow(m.xi, m.xi_pitch, j)[i] = normal_at((uint32_t)idx, d, m.stream_id, m.k0, m.k1);
+// Model error of one output tile (kTileH rows x kTileW columns of interior
+// cells of one member), fused: the normal field of the tile and its halo is
+// generated into LDS (one Philox call per four consecutive elements, as the
+// element index orders them), the SOAR field of the tile and a one-cell halo
+// is formed from it in LDS, and the geostrophic balance is added to the
+// state.  Every value equals the per-member path's (same element indices,
+// same Box-Muller, same fused multiply-add order, same balance arithmetic).
+constexpr int kTileW = 64, kTileH = 16;
+constexpr int kXiW = kTileW + 6, kXiH = kTileH + 6;  // xi: coarse halo 1 + SOAR radius 2 (+ column offset)
+constexpr int kCoW = kTileW + 2, kCoH = kTileH + 2;  // coarse: balance halo 1
+
+__global__ void __launch_bounds__(256) noise_fused(const NoiseMember* __restrict__ members,
+                                                   const int2* __restrict__ tiles,
+                                                   const uint32_t* __restrict__ draw) {
+  __shared__ float xs[kXiH][kXiW];
+  __shared__ float cs[kCoH][kCoW];
+  const int2 t = tiles[blockIdx.x];
+  const NoiseMember& m = members[t.x];
+  const uint32_t d = draw[t.x];
+  const int T0 = 2 + kTileH * (t.y >> 16), S0 = 2 + kTileW * (t.y & 0xFFFF);
+  // coarse cells needed: rows T0-1 .. cj_hi, columns S0-1 .. ci_hi
+  const int cj_hi = min(T0 + kTileH, m.ny + 2), ci_hi = min(S0 + kTileW, m.nx + 2);
+  // xi virtual rows V = T0-1+lr and columns U = S0-1+lc are needed up to cj_hi+4, ci_hi+4
+  const int nrow = cj_hi + 4 - (T0 - 1) + 1, ncol = ci_hi + 4 - (S0 - 1) + 1;
+  const int cols = m.xi_cols;
+
+  // 1. normal field into LDS
+  const int a0 = m.per_ew ? ((S0 - 1 - 4 + 2 * m.nx) % m.nx) : (S0 - 1);  // actual column of U = S0-1
+  if (!m.per_ew || ncol <= m.nx) {
+    // the needed actual columns form one run, or two when a periodic run wraps
+    const int seg_len0 = m.per_ew ? min(ncol, m.nx - a0) : ncol;
+    constexpr int kMaxQ = kXiW / 4 + 2;
+    for (int u = threadIdx.x; u < kXiH * 2 * kMaxQ; u += 256) {
+      const int lr = u / (2 * kMaxQ), rem = u - lr * (2 * kMaxQ), seg = rem / kMaxQ, j = rem - seg * kMaxQ;
+      if (lr >= nrow) continue;
+      const int V = T0 - 1 + lr;
+      const int r = m.per_ns ? (V - 4 + 2 * m.ny) % m.ny : V;
+      const int ca = seg == 0 ? a0 : 0;
+      const int len = seg == 0 ? seg_len0 : ncol - seg_len0;
+      if (len <= 0) continue;
+      const int cb = ca + len - 1;
+      const uint32_t base = (uint32_t)r * (uint32_t)cols;
+      const uint32_t q = ((base + (uint32_t)ca) >> 2) + (uint32_t)j;
+      if (q > ((base + (uint32_t)cb) >> 2)) continue;
+      float v4[4];
+      normals4(q, d, m.stream_id, m.k0, m.k1, v4);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int c = (int)(4u * q + (uint32_t)i - base);
+        if (c < ca || c > cb) continue;
+        const int lc = m.per_ew ? (c - a0 + m.nx) % m.nx : c - (S0 - 1);
+        xs[lr][lc] = v4[i];
+      }
+    }
+  } else {
+    // periodic member narrower than the tile window: element by element
+    for (int u = threadIdx.x; u < kXiH * kXiW; u += 256) {
+      const int lr = u / kXiW, lc = u - lr * kXiW;
+      if (lr >= nrow || lc >= ncol) continue;
+      const int V = T0 - 1 + lr, U = S0 - 1 + lc;
+      const int r = m.per_ns ? (V - 4 + 2 * m.ny) % m.ny : V;
+      const int c = (U - 4 + 2 * m.nx) % m.nx;
+      xs[lr][lc] = normal_at((uint32_t)r * (uint32_t)cols + (uint32_t)c, d, m.stream_id, m.k0, m.k1);
+    }
+  }
+  __syncthreads();
+
+  // 2. SOAR field on the tile and its one-cell halo (soar_cell's order)
+  for (int u = threadIdx.x; u < kCoH * kCoW; u += 256) {
+    const int lr = u / kCoW, lc = u - lr * kCoW;
+    if (T0 - 1 + lr > cj_hi || S0 - 1 + lc > ci_hi) continue;
+    float acc = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 5; ++k)
+#pragma unroll
+      for (int l = 0; l < 5; ++l) acc = __builtin_fmaf(m.W[k * 5 + l], xs[lr + k][lc + l], acc);
+    cs[lr][lc] = m.scale * acc;
+  }
+  __syncthreads();
+
+  // 3. geostrophic balance on the tile
+  const int lx = threadIdx.x & 63;
+  for (int ly = threadIdx.x >> 6; ly < kTileH; ly += 4) {
+    const int tj = T0 + ly, ti = S0 + lx;
+    if (tj > m.ny + 1 || ti > m.nx + 1) continue;
+    balance_apply(m.dx, m.dy, m.g, m.f, m.beta, m.north_x, m.north_y, m.f_cells, m.f_pitch, cs[ly + 1][lx + 1],
+                  cs[ly + 1][lx], cs[ly + 1][lx + 2], cs[ly][lx + 1], cs[ly + 2][lx + 1], m.eta, m.hu, m.hv, m.pitch,
+                  m.Hi, m.hi_pitch, m.land, tj, ti);
   }
 }
 
-__global__ void __launch_bounds__(256) noise_soar(const NoiseMember* __restrict__ members) {
-  const NoiseMember& m = members[blockIdx.y];
-  const int w = m.nx + 4, n = (m.ny + 4) * w;
-  for (int idx = blockIdx.x * 256 + threadIdx.x; idx < n; idx += gridDim.x * 256) {
-    const int tj = idx / w, ti = idx - tj * w;
-    const float acc = soar_cell(m.nx, m.ny, m.W, m.per_ns, m.per_ew, m.xi, m.xi_pitch, tj, ti);
-    mrow(m.coarse, m.c_pitch, tj)[ti] = m.scale * acc;
-  }
-}
-
-__global__ void __launch_bounds__(256) noise_balance(const NoiseMember* __restrict__ members, uint32_t* draw) {
-  const NoiseMember& m = members[blockIdx.y];
-  const int n = m.nx * m.ny;
-  for (int idx = blockIdx.x * 256 + threadIdx.x; idx < n; idx += gridDim.x * 256) {
-    const int j = idx / m.nx, i = idx - j * m.nx;
-    balance_cell(m.dx, m.dy, m.g, m.f, m.beta, m.north_x, m.north_y, m.f_cells, m.f_pitch, m.coarse, m.c_pitch,
-                 m.eta, m.hu, m.hv, m.pitch, m.Hi, m.hi_pitch, m.land, j + 2, i + 2);
-  }
-  // The fill kernel of this step has finished reading the member's draw index.
-  if (blockIdx.x == 0 && threadIdx.x == 0) draw[blockIdx.y] = draw[blockIdx.y] + 1u;
+// Every member's draw index advances once the step's model error is applied.
+__global__ void advance_draws(uint32_t* draw, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) draw[i] = draw[i] + 1u;
 }
 
 int check(hipError_t e, const char* what) {
@@ -86,8 +150,9 @@ int bad(const char* msg) {
 struct Ensemble {
   int n = 0, n_work = 0, n_blocks = 0, exact = 1, has_noise = 0;
   int has_land = 0, f_array = 0, angle0 = 1;
-  int noise_blocks = 64;
+  int n_tiles = 0;  // model-error tiles over all members
   StepArgs* d_args = nullptr;
+  int2* d_tiles = nullptr;
   int2* d_work = nullptr;
   NoiseMember* d_noise = nullptr;
   uint32_t* d_draw = nullptr;
@@ -108,10 +173,8 @@ static int ensemble_launch_substep(Ensemble* e, int rk, hipStream_t s) {
 
 static int ensemble_launch_noise(Ensemble* e, hipStream_t s) {
   if (!e->has_noise) return 0;
-  const dim3 grid(e->noise_blocks, e->n);
-  hipLaunchKernelGGL(noise_fill, grid, dim3(256), 0, s, e->d_noise, e->d_draw);
-  hipLaunchKernelGGL(noise_soar, grid, dim3(256), 0, s, e->d_noise);
-  hipLaunchKernelGGL(noise_balance, grid, dim3(256), 0, s, e->d_noise, e->d_draw);
+  hipLaunchKernelGGL(noise_fused, dim3(e->n_tiles), dim3(256), 0, s, e->d_noise, e->d_tiles, e->d_draw);
+  hipLaunchKernelGGL(advance_draws, dim3((e->n + 255) / 256), dim3(256), 0, s, e->d_draw, e->n);
   return check(hipGetLastError(), "ensemble noise launch");
 }
 
@@ -128,6 +191,7 @@ static void ensemble_free(Ensemble* e) {
   if (e->d_args) (void)hipFree(e->d_args);
   if (e->d_work) (void)hipFree(e->d_work);
   if (e->d_noise) (void)hipFree(e->d_noise);
+  if (e->d_tiles) (void)hipFree(e->d_tiles);
   if (e->d_draw) (void)hipFree(e->d_draw);
   delete e;
 }
@@ -156,6 +220,7 @@ int gpo_ensemble_create(const gpo_member* members, int32_t n, int32_t chunk_rows
   std::vector<StepArgs>& args = e->h_args;
   args.resize(2 * (size_t)n);
   std::vector<int2> work;
+  std::vector<int2> tiles;
   std::vector<NoiseMember> noise;
   std::vector<uint32_t> draws((size_t)n);
   for (int m = 0; m < n; ++m) {
@@ -236,6 +301,13 @@ int gpo_ensemble_create(const gpo_member* members, int32_t n, int32_t chunk_rows
         return bad("gpo_ensemble_create: model error buffers missing or mis-sized");
       }
       noise.push_back(nm);
+      const int tx_n = (p.nx + kTileW - 1) / kTileW, ty_n = (p.ny + kTileH - 1) / kTileH;
+      if (tx_n > 0xFFFF || ty_n > 0x7FFF) {
+        ensemble_free(e);
+        return bad("gpo_ensemble_create: member too large for the model-error tile table");
+      }
+      for (int ty = 0; ty < ty_n; ++ty)
+        for (int tx = 0; tx < tx_n; ++tx) tiles.push_back(make_int2((int)noise.size() - 1, tx | (ty << 16)));
     }
   }
   e->n_work = (int)work.size();
@@ -252,6 +324,8 @@ int gpo_ensemble_create(const gpo_member* members, int32_t n, int32_t chunk_rows
   if (!rc) rc = check(hipMalloc(&e->d_draw, sizeof(uint32_t) * n), "gpo_ensemble_create hipMalloc");
   if (!rc && e->has_noise)
     rc = check(hipMalloc(&e->d_noise, sizeof(NoiseMember) * noise.size()), "gpo_ensemble_create hipMalloc");
+  if (!rc && e->has_noise)
+    rc = check(hipMalloc(&e->d_tiles, sizeof(int2) * tiles.size()), "gpo_ensemble_create hipMalloc");
   if (!rc)
     rc = check(hipMemcpyAsync(e->d_args, args.data(), sizeof(StepArgs) * args.size(), hipMemcpyHostToDevice, s),
                "gpo_ensemble_create copy");
@@ -264,6 +338,10 @@ int gpo_ensemble_create(const gpo_member* members, int32_t n, int32_t chunk_rows
   if (!rc && e->has_noise)
     rc = check(hipMemcpyAsync(e->d_noise, noise.data(), sizeof(NoiseMember) * noise.size(), hipMemcpyHostToDevice, s),
                "gpo_ensemble_create copy");
+  if (!rc && e->has_noise)
+    rc = check(hipMemcpyAsync(e->d_tiles, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice, s),
+               "gpo_ensemble_create copy");
+  e->n_tiles = (int)tiles.size();
   if (!rc) rc = check(hipStreamSynchronize(s), "gpo_ensemble_create sync");
   if (rc) {
     ensemble_free(e);

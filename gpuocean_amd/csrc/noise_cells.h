@@ -35,24 +35,25 @@ __device__ __forceinline__ float soar_cell(int nx, int ny, const float* W, int p
 }
 
 // geostrophicBalance (gpu_kernels/ocean_noise.cu:256-390) at interior cell
-// (tj, ti), coarse grid equal to the computational grid.
-__device__ __forceinline__ void balance_cell(float dx, float dy, float g, float f, float beta, float north_x,
-                                             float north_y, const float* f_cells, int f_pitch, const float* coarse,
-                                             int c_pitch, float* eta, float* hu, float* hv, int pitch,
-                                             const float* Hi, int h_pitch, float land_value, int tj, int ti) {
+// (tj, ti), coarse grid equal to the computational grid, given the coarse
+// value d at the cell and at its four neighbours.
+__device__ __forceinline__ void balance_apply(float dx, float dy, float g, float f, float beta, float north_x,
+                                              float north_y, const float* f_cells, int f_pitch, float d, float d_xm,
+                                              float d_xp, float d_ym, float d_yp, float* eta, float* hu, float* hv,
+                                              int pitch, const float* Hi, int h_pitch, float land_value, int tj,
+                                              int ti) {
   const float h00 = crow(Hi, h_pitch, tj)[ti], h01 = crow(Hi, h_pitch, tj)[ti + 1];
   const float h10 = crow(Hi, h_pitch, tj + 1)[ti], h11 = crow(Hi, h_pitch, tj + 1)[ti + 1];
   const bool dry = (h00 == land_value) || (h01 == land_value) || (h10 == land_value) || (h11 == land_value);
   const float H_mid = 0.25f * (h00 + h01 + h10 + h11);
   float coriolis = f + beta * (((float)ti + 0.5f) * dx * north_x + ((float)tj + 0.5f) * dy * north_y);
   if (f == 0.0f) coriolis = f_cells ? crow(f_cells, f_pitch, tj)[ti] : 0.0f;
-  const float d = crow(coarse, c_pitch, tj)[ti];
   float* er = mrow(eta, pitch, tj);
   float* ur = mrow(hu, pitch, tj);
   float* vr = mrow(hv, pitch, tj);
   const float h_mid = d + H_mid + er[ti];
-  const float ddx = (crow(coarse, c_pitch, tj)[ti + 1] - crow(coarse, c_pitch, tj)[ti - 1]) / (2.0f * dx);
-  const float ddy = (crow(coarse, c_pitch, tj + 1)[ti] - crow(coarse, c_pitch, tj - 1)[ti]) / (2.0f * dy);
+  const float ddx = (d_xp - d_xm) / (2.0f * dx);
+  const float ddy = (d_yp - d_ym) / (2.0f * dy);
   const float gc = g / coriolis;
   const float d_hu = -gc * h_mid * ddy;
   const float d_hv = gc * h_mid * ddx;
@@ -65,6 +66,17 @@ __device__ __forceinline__ void balance_cell(float dx, float dy, float g, float 
   er[ti] = e;
   ur[ti] = u;
   vr[ti] = v;
+}
+
+// The same, reading the coarse field from a pitched array.
+__device__ __forceinline__ void balance_cell(float dx, float dy, float g, float f, float beta, float north_x,
+                                             float north_y, const float* f_cells, int f_pitch, const float* coarse,
+                                             int c_pitch, float* eta, float* hu, float* hv, int pitch,
+                                             const float* Hi, int h_pitch, float land_value, int tj, int ti) {
+  const float* c0 = crow(coarse, c_pitch, tj);
+  balance_apply(dx, dy, g, f, beta, north_x, north_y, f_cells, f_pitch, c0[ti], c0[ti - 1], c0[ti + 1],
+                crow(coarse, c_pitch, tj - 1)[ti], crow(coarse, c_pitch, tj + 1)[ti], eta, hu, hv, pitch, Hi, h_pitch,
+                land_value, tj, ti);
 }
 
 }  // namespace gpo

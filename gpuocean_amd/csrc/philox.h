@@ -64,4 +64,12 @@ __device__ inline float normal_at(uint32_t index, uint32_t draw, uint32_t stream
   return (index & 1u) ? b : a;
 }
 
+// The four normal numbers of elements 4q .. 4q+3 (one Philox call, two
+// Box-Muller pairs): out[i] == normal_at(4q + i, ...).
+__device__ inline void normals4(uint32_t q, uint32_t draw, uint32_t stream, uint32_t k0, uint32_t k1, float* out) {
+  const U4 r = philox4x32_10(U4{q, draw, stream, kNoiseTag}, k0, k1);
+  box_muller(r.x, r.y, &out[0], &out[1]);
+  box_muller(r.z, r.w, &out[2], &out[3]);
+}
+
 }  // namespace gpo

@@ -50,6 +50,10 @@ def test_batched_matches_per_member(gpu_ctx, noise):
     _assert_same(batched, plain)
     if noise:
         assert ens.device_draws() == [7] * len(SIZES)
+        # the batched path keeps no normal/SOAR fields; the getters regenerate them
+        for a, b in zip(batched, plain):
+            assert np.array_equal(a.model_error.getRandomNumbers(), b.model_error.getRandomNumbers())
+            assert np.array_equal(a.model_error.getCoarseBuffer(), b.model_error.getCoarseBuffer())
 
 
 def test_batched_all_walls_and_all_periodic(gpu_ctx):
@@ -62,6 +66,21 @@ def test_batched_all_walls_and_all_periodic(gpu_ctx):
                 s.step(np.float32(s.dt))
         BatchedEnsemble(batched, chunk_rows=5, waves_per_simd=1).step(5)
         _assert_same(batched, plain)
+
+
+def test_small_periodic_members_model_error(gpu_ctx):
+    """Members narrower than the model-error tile window (periodic wrap inside
+    one tile) and partial tiles."""
+    print("small periodic members", flush=True)
+    sizes = [(24, 20), (40, 18), (66, 35), (70, 17), (130, 33)]
+    bc = Common.BoundaryConditions(2, 2, 2, 2)
+    plain = _members(gpu_ctx, sizes, True, bc=bc)
+    batched = _members(gpu_ctx, sizes, True, bc=bc)
+    for s in plain:
+        for _ in range(3):
+            s.step(np.float32(s.dt))
+    BatchedEnsemble(batched).step(3)
+    _assert_same(batched, plain)
 
 
 def test_graph_replay_matches_direct(gpu_ctx):
