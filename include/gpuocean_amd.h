@@ -63,7 +63,7 @@ typedef struct {
 } gpo_forcing;
 
 /* Set-up of a parameter block before its first substep: verifies, on the
- * device and for every numerator in {0} U [2^-96, 2^100], that division by
+ * device and for every numerator in {+0} U +-[2^-96, 2^100], that division by
  * each of dx, dy and 2g equals one Markstein correction step with the
  * correctly rounded reciprocal, and records the result in p->divc_ok (used by
  * the exact build; results are cached per constant).  Synchronises `stream`
