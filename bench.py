@@ -115,13 +115,14 @@ def cpu_baseline_ensemble(members_kw, budget_s=15.0):
 
 # ---------------------------------------------------------------------------- single member
 
-def run_single(kw, workload, steps, warmup, fast, chunk_rows):
+def run_single(kw, workload, steps, warmup, fast, chunk_rows, waves_per_simd=0):
     import numpy as np
     import torch
     from gpuocean_amd import CDKLM16, Common
 
     ctx = Common.DeviceContext(0)
-    sim = CDKLM16(ctx, **kw, exact_arithmetic=not fast, chunk_rows=chunk_rows or None)
+    sim = CDKLM16(ctx, **kw, exact_arithmetic=not fast, chunk_rows=chunk_rows or None,
+                  waves_per_simd=waves_per_simd or None)
     dt = np.float32(sim.dt)
     nx, ny = kw["nx"], kw["ny"]
     cells = nx * ny
@@ -402,7 +403,7 @@ def main():
     args.steps = 2000 if args.steps is None else args.steps
     args.warmup = 50 if args.warmup is None else args.warmup
     kw = cases.config_c1() if args.workload == "c1" else cases.config_c0()
-    r = run_single(kw, args.workload, args.steps, args.warmup, args.fast, args.chunk_rows)
+    r = run_single(kw, args.workload, args.steps, args.warmup, args.fast, args.chunk_rows, args.waves_per_simd)
     result = {
         "metric": "member-cell-steps/s at 256 members; % HBM roofline; adjoint/forward time",
         "value": r["value"], "unit": "member-cell-steps/s", "n_gpus": 1, "steps": args.steps,

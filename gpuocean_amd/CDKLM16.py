@@ -97,7 +97,8 @@ class CDKLM16:
                  block_width=12, block_height=32, num_threads_dt=256,
                  use_direct_lookup=False,
                  exact_arithmetic=True,
-                 chunk_rows=None):
+                 chunk_rows=None,
+                 waves_per_simd=None):
         self.logger = logging.getLogger(__name__)
         assert 0 < rk_order < 4, "Only 1st, 2nd and 3rd order Runge Kutta supported"
         if rk_order == 3:
@@ -226,6 +227,7 @@ class CDKLM16:
         p.f_is_array = int(self.f_cells is not None)
         p.angle_is_zero = int(angle_zero)
         p.chunk_rows = int(chunk_rows) if chunk_rows else 0
+        p.waves_per_simd = int(waves_per_simd) if waves_per_simd else 0
         p.dx = host.kernel_constant(self.dx)
         p.dy = host.kernel_constant(self.dy)
         p.g = host.kernel_constant(self.g)

@@ -26,7 +26,7 @@ class CDKLM16Params(ctypes.Structure):
         ("theta", c_float), ("desing_eps", c_float), ("flux_slope_eps", c_float),
         ("depth_cutoff", c_float), ("flux_balancer", c_float), ("land_value", c_float),
         ("north_x", c_float), ("north_y", c_float), ("wind_stress_factor", c_float), ("rho_o", c_float),
-        ("divc_ok", c_int32),
+        ("divc_ok", c_int32), ("waves_per_simd", c_int32),
     ]
 
 

@@ -313,6 +313,20 @@ int bad(const char* msg) {
 
 namespace gpo {
 
+// Wavefronts of a persistent grid with `wps` waves on each SIMD of the
+// current device (4 SIMDs per CU); the CU count is cached per device.
+int persistent_waves(int wps) {
+  static int cus[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256 * 4 * wps;
+  if (cus[dev] == 0) {
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+    cus[dev] = n;
+  }
+  return cus[dev] * 4 * wps;
+}
+
 int make_step_args(const gpo_cdklm16_params* p, float dt, int32_t rk_step, const float* in_eta,
                           const float* in_hu, const float* in_hv, const float* qn_eta, const float* qn_hu,
                           const float* qn_hv, float* out_eta, float* out_hu, float* out_hv, int32_t pitch,
@@ -330,6 +344,7 @@ int make_step_args(const gpo_cdklm16_params* p, float dt, int32_t rk_step, const
   if (p->f_is_array && !f_cells) return bad("gpo_cdklm16: f_is_array set but f_cells is NULL");
   if (p->rk_order == 3 && p->r != 0.0f) return bad("gpo_cdklm16: RK3 requires r == 0");
   if (p->chunk_rows < 0 || p->chunk_rows > 4096) return bad("gpo_cdklm16: chunk_rows out of range");
+  if (p->waves_per_simd < 0 || p->waves_per_simd > 32) return bad("gpo_cdklm16: waves_per_simd out of range");
 
   StepArgs& a = *out;
   a = StepArgs{};
@@ -337,8 +352,17 @@ int make_step_args(const gpo_cdklm16_params* p, float dt, int32_t rk_step, const
   a.ny = p->ny;
   a.pitch = pitch / 4;
   a.hi_pitch = Hi_pitch / 4;
-  a.chunk_rows = p->chunk_rows > 0 ? p->chunk_rows : kDefaultChunkRows;
+  a.grid_waves = persistent_waves(p->waves_per_simd > 0 ? p->waves_per_simd : kDefaultWavesPerSimd);
   a.n_strips = (p->nx + kStripOut - 1) / kStripOut;
+  if (p->chunk_rows > 0) {
+    a.chunk_rows = p->chunk_rows;
+  } else {
+    // Default row chunk: enough (strip, chunk) items to give every wave of the
+    // persistent grid one, between 2 and kDefaultChunkRows rows.
+    const long long rows = (long long)a.n_strips * p->ny;
+    const long long c = (rows + a.grid_waves - 1) / a.grid_waves;
+    a.chunk_rows = (int)(c < 2 ? 2 : (c > kDefaultChunkRows ? kDefaultChunkRows : c));
+  }
   a.n_chunks = (p->ny + a.chunk_rows - 1) / a.chunk_rows;
   a.bc_n = (boundary_conditions >> 24) & 0xFF;
   a.bc_s = (boundary_conditions >> 16) & 0xFF;

@@ -18,7 +18,8 @@ constexpr float kMaxDtCap = 100000.0f;
 constexpr int kStripOut = 60;
 constexpr int kWave = 64;
 constexpr int kWavesPerBlock = 4;
-constexpr int kDefaultChunkRows = 8;
+constexpr int kDefaultChunkRows = 6;
+constexpr int kDefaultWavesPerSimd = 5;
 
 // Lane exchange through DPP wave shifts (verified on gfx950: wave_shr:1 gives
 // lane i the value of lane i-1, wave_shl:1 that of lane i+1).
@@ -30,6 +31,7 @@ struct StepArgs {
   // geometry
   int nx, ny, pitch, hi_pitch;  // pitches in elements
   int n_strips, n_chunks, chunk_rows;
+  int grid_waves;  // wavefronts of the persistent substep grid (0: one per item)
   int bc_n, bc_e, bc_s, bc_w;
   int rk_order, rk_step, one_dim, has_forcing;
   int divc_ok;   // bit 0: Markstein division by dx verified, bit 1: by dy, bit 2: by 2g
@@ -69,5 +71,8 @@ int make_step_args(const gpo_cdklm16_params* p, float dt, int32_t rk_step, const
                    const float* f_cells, const gpo_forcing* forcing, uint32_t boundary_conditions, StepArgs* out);
 
 void set_error(const char* msg);
+
+// Wavefronts of a persistent grid with `wps` waves per SIMD on the current device.
+int persistent_waves(int wps);
 
 }  // namespace gpo

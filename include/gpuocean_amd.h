@@ -47,6 +47,7 @@ typedef struct {
   float north_x, north_y;         /* sin(angle), cos(angle) for a constant angle */
   float wind_stress_factor, rho_o;/* WIND_STRESS_FACTOR, RHO_O */
   int32_t divc_ok;                /* set by gpo_cdklm16_prepare (0 = IEEE division only) */
+  int32_t waves_per_simd;         /* persistent substep grid, waves per SIMD (tuning; 0 = default) */
 } gpo_cdklm16_params;
 
 /* Optional wind stress / atmospheric pressure textures (time levels `cur`
