@@ -118,6 +118,13 @@ _SIGNATURES = [
     ("gpo_cdklm16_substep_adj_f64", c_int32,
      [ctypes.POINTER(CDKLM16Params), ctypes.c_double, c_int32] + [c_void_p] * 13 +
      [c_int32, c_int32, c_void_p, c_int32, c_void_p, c_uint32, c_void_p, c_void_p, c_void_p]),
+    ("gpo_cdklm16_trajectory_f32", c_int32,
+     [ctypes.POINTER(CDKLM16Params), c_int32, c_float, ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p), c_int32,
+      c_void_p, c_int32, c_void_p, c_uint32, c_void_p]),
+    ("gpo_cdklm16_adjoint_steps_f32", c_int32,
+     [ctypes.POINTER(CDKLM16Params), c_int32, c_float, ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p),
+      ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p), c_int32, c_void_p, c_int32, c_void_p, c_uint32, c_void_p,
+      c_void_p, c_void_p]),
     ("gpo_balance_adj_f32", c_int32,
      [ctypes.POINTER(CDKLM16Params), c_float, c_float, c_float, c_void_p, c_int32, c_void_p, c_int32,
       c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_int32, c_void_p]),

@@ -153,19 +153,24 @@ class DifferentiableCDKLM16:
     # -------------------------------------------------------------- FP32 path
     def forward(self, eta0, hu0, hv0):
         """Run n_steps from (eta0, hu0, hv0) ((ny+4) x (nx+4) device float32),
-        keeping checkpoints.  Returns J (python float) and the checkpoints."""
+        keeping a checkpoint every checkpoint_every steps.  Returns J (python
+        float) and the final state."""
         s = self.stream
         self._enter()
         A, B = self._new_state(), self._new_state()
+        ckpts = []
         with torch.cuda.stream(s):
             A.load(eta0, hu0, hv0)
-            ckpts = []
-            for i in range(self.n_steps):
-                if i % self.k == 0:
-                    c = self._new_state()
-                    c.copy_from(A)
-                    ckpts.append(c)
-                self._step32(A, B, A)
+        for i0 in range(0, self.n_steps, self.k):
+            c = self._new_state()
+            with torch.cuda.stream(s):
+                c.copy_from(A)
+            ckpts.append(c)
+            m = min(self.k, self.n_steps - i0)
+            L.check(L.lib().gpo_cdklm16_rk2_steps(
+                ctypes.byref(self.params), m, float(self.dt), *A.ptrs(), *B.ptrs(), A.pitch, _vp(self.Hi),
+                self.Hi.pitch, self._fc_ptr(), self.bc, 1, self._sh()), "gpo_cdklm16_rk2_steps")
+        with torch.cuda.stream(s):
             self.J.zero_()
         self._misfit32(A)
         self._final = A
@@ -196,17 +201,28 @@ class DifferentiableCDKLM16:
 
     def _segment_buffers(self):
         if self._seg is None:
-            self._seg = ([self._new_state() for _ in range(self.k + 1)], [self._new_state() for _ in range(self.k)])
+            Qs = [self._new_state() for _ in range(self.k + 1)]
+            Q1s = [self._new_state() for _ in range(self.k)]
             self._lam = [self._new_state() for _ in range(3)]
+            PP = ctypes.c_void_p
+            self._seg = dict(
+                Qs=Qs, Q1s=Q1s,
+                qs_p=(PP * (3 * (self.k + 1)))(*[a.data_ptr() for st in Qs for a in st.t]),
+                q1s_p=(PP * (3 * self.k))(*[a.data_ptr() for st in Q1s for a in st.t]),
+                lam_p=(PP * 3)(*[a.data_ptr() for a in self._lam[0].t]),
+                work_p=(PP * 6)(*[a.data_ptr() for st in self._lam[1:] for a in st.t]))
         return self._seg
 
     def backward(self, grad_J=1.0):
         """dJ/d(eta0, hu0, hv0) of the last forward(), scaled by grad_J, as
-        three (ny+4) x (nx+4) float32 tensors (ghost cells zero)."""
+        three (ny+4) x (nx+4) float32 tensors (ghost cells zero).  Each
+        segment between checkpoints is recomputed once, keeping every state
+        and stage, then swept backwards, both by single library calls."""
         s = self.stream
         self._enter()
-        Qs, Q1s = self._segment_buffers()
-        lam, L1, X = self._lam
+        seg = self._segment_buffers()
+        lam = self._lam[0]
+        lib = L.lib()
         with torch.cuda.stream(s):
             lam.zero_()
             self.J.zero_()
@@ -218,19 +234,16 @@ class DifferentiableCDKLM16:
         n = self.n_steps
         starts = list(range(0, n, self.k))
         for si in reversed(range(len(starts))):
-            i0 = starts[si]
-            m = min(self.k, n - i0)
+            m = min(self.k, n - starts[si])
             with torch.cuda.stream(s):
-                Qs[0].copy_from(self._ckpts[si])
-            for j in range(m):            # recompute the segment, keeping every stage
-                self._step32(Qs[j], Q1s[j], Qs[j + 1])
-            for j in reversed(range(m)):  # sweep it backwards
-                with torch.cuda.stream(s):
-                    X.zero_()
-                self._adj32(1, Q1s[j], Qs[j + 1], lam, L1, X, 0)
-                self._adj32(0, Qs[j], Q1s[j], L1, X, None, 1)
-                lam, X = X, lam
-        self._lam = [lam, L1, X]
+                seg["Qs"][0].copy_from(self._ckpts[si])
+            L.check(lib.gpo_cdklm16_trajectory_f32(
+                ctypes.byref(self.params), m, float(self.dt), seg["qs_p"], seg["q1s_p"], lam.pitch, _vp(self.Hi),
+                self.Hi.pitch, self._fc_ptr(), self.bc, self._sh()), "gpo_cdklm16_trajectory_f32")
+            L.check(lib.gpo_cdklm16_adjoint_steps_f32(
+                ctypes.byref(self.params), m, float(self.dt), seg["qs_p"], seg["q1s_p"], seg["lam_p"],
+                seg["work_p"], lam.pitch, _vp(self.Hi), self.Hi.pitch, self._fc_ptr(), self.bc, _vp(self.ws32),
+                _vp(self.flag), self._sh()), "gpo_cdklm16_adjoint_steps_f32")
         s.synchronize()
         return [a.clone() for a in lam.views()]
 

@@ -140,6 +140,26 @@ __global__ void misfit(Geo g, int n, const float* pos, const float* obs, const T
   if ((threadIdx.x & 63) == 0) atomicAdd(J, jd);
 }
 
+// Fill / copy of the interior cells of three state arrays (pitch sp).
+template <class T>
+__global__ void state_zero(Geo g, T* a, T* b, T* c) {
+  const int col = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y * blockDim.y + threadIdx.y;
+  if (col >= g.nx + 4 || r >= g.ny + 4) return;
+  const size_t k = (size_t)r * g.sp + col;
+  a[k] = T(0);
+  b[k] = T(0);
+  c[k] = T(0);
+}
+template <class T>
+__global__ void state_copy(Geo g, const T* a, const T* b, const T* c, T* x, T* y, T* z) {
+  const int col = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y * blockDim.y + threadIdx.y;
+  if (col >= g.nx + 4 || r >= g.ny + 4) return;
+  const size_t k = (size_t)r * g.sp + col;
+  x[k] = a[k];
+  y[k] = b[k];
+  z[k] = c[k];
+}
+
 }  // namespace adj
 }  // namespace gpo
 
@@ -313,6 +333,66 @@ size_t gpo_tlm_workspace_bytes(int32_t nx, int32_t ny) {
   if (nx < 1 || ny < 1) return 0;
   const size_t n = (size_t)(nx + 4) * (ny + 4), nh = (size_t)(ny + 5) * (nx + 5);
   return (25 * n + nh) * sizeof(Dual);
+}
+
+int gpo_cdklm16_trajectory_f32(const gpo_cdklm16_params* p, int32_t n_steps, float dt, float* const* qs,
+                               float* const* q1s, int32_t pitch, const float* Hi, int32_t Hi_pitch,
+                               const float* f_cells, uint32_t boundary_conditions, void* stream) {
+  if (!p || n_steps < 0 || !qs || !q1s || !Hi) return bad_arg("gpo_cdklm16_trajectory_f32: bad arguments");
+  if (p->rk_order != 2) return bad_arg("gpo_cdklm16_trajectory_f32: RK2 only");
+  hipStream_t s = (hipStream_t)stream;
+  for (int j = 0; j < n_steps; ++j) {
+    float* const* a = qs + 3 * j;
+    float* const* b = q1s + 3 * j;
+    float* const* c = qs + 3 * (j + 1);
+    StepArgs a0, a1;
+    int rc = make_step_args(p, dt, 0, a[0], a[1], a[2], nullptr, nullptr, nullptr, b[0], b[1], b[2], pitch, Hi,
+                            Hi_pitch, f_cells, nullptr, boundary_conditions, &a0);
+    if (!rc)
+      rc = make_step_args(p, dt, 1, b[0], b[1], b[2], a[0], a[1], a[2], c[0], c[1], c[2], pitch, Hi, Hi_pitch,
+                          f_cells, nullptr, boundary_conditions, &a1);
+    if (rc) return rc;
+    hipError_t e = launch_cdklm_exact(a0, p->has_land, p->f_is_array, p->angle_is_zero, s);
+    if (e == hipSuccess) e = launch_cdklm_exact(a1, p->has_land, p->f_is_array, p->angle_is_zero, s);
+    if (e != hipSuccess) return hip_check(e, "gpo_cdklm16_trajectory_f32 launch");
+  }
+  return 0;
+}
+
+int gpo_cdklm16_adjoint_steps_f32(const gpo_cdklm16_params* p, int32_t n_steps, float dt, const float* const* qs,
+                                  const float* const* q1s, float* const* lam, float* const* work, int32_t pitch,
+                                  const float* Hi, int32_t Hi_pitch, const float* f_cells,
+                                  uint32_t boundary_conditions, void* workspace, int32_t* flag, void* stream) {
+  Geo g;
+  int rc = make_geo(p, boundary_conditions, pitch, Hi_pitch, f_cells ? pitch : 4, 4, &g);
+  if (rc) return rc;
+  if (n_steps < 0 || !qs || !q1s || !lam || !work || !workspace)
+    return bad_arg("gpo_cdklm16_adjoint_steps_f32: bad arguments");
+  hipStream_t s = (hipStream_t)stream;
+  float* cur[3] = {lam[0], lam[1], lam[2]};     // lambda at Q^{j+1}, then at Q^j
+  float* nxt[3] = {work[0], work[1], work[2]};  // lambda being formed at Q^j
+  float* l1[3] = {work[3], work[4], work[5]};   // lambda at the RK stage Q^(1)
+  for (int j = n_steps - 1; j >= 0; --j) {
+    const float* const* a = qs + 3 * j;
+    const float* const* b = q1s + 3 * j;
+    const float* const* c = qs + 3 * (j + 1);
+    hipLaunchKernelGGL(state_zero<float>, cell_grid(g), kBlock, 0, s, g, nxt[0], nxt[1], nxt[2]);
+    rc = adjoint_stages<float>(g, dt, 1, b[0], b[1], b[2], c[0], cur[0], cur[1], cur[2], l1[0], l1[1], l1[2], nxt[0],
+                               nxt[1], nxt[2], 0, Hi, f_cells, workspace, flag, s);
+    if (!rc)
+      rc = adjoint_stages<float>(g, dt, 0, a[0], a[1], a[2], b[0], l1[0], l1[1], l1[2], nxt[0], nxt[1], nxt[2],
+                                 nullptr, nullptr, nullptr, 1, Hi, f_cells, workspace, flag, s);
+    if (rc) return rc;
+    for (int k = 0; k < 3; ++k) {
+      float* t = cur[k];
+      cur[k] = nxt[k];
+      nxt[k] = t;
+    }
+  }
+  if (cur[0] != lam[0])
+    hipLaunchKernelGGL(state_copy<float>, cell_grid(g), kBlock, 0, s, g, cur[0], cur[1], cur[2], lam[0], lam[1],
+                       lam[2]);
+  return hip_check(hipGetLastError(), "gpo_cdklm16_adjoint_steps_f32 launch");
 }
 
 int gpo_cdklm16_substep_adj_f32(const gpo_cdklm16_params* p, float dt, int32_t rk_step, const float* in_eta,

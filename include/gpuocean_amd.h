@@ -322,6 +322,22 @@ int gpo_cdklm16_substep_adj_f64(const gpo_cdklm16_params* p, double dt, int32_t 
                                 const double* Hi, int32_t Hi_pitch, const double* f_cells,
                                 uint32_t boundary_conditions, void* workspace, int32_t* flag, void* stream);
 
+/* n_steps RK2 steps keeping every state: Q^{j+1} = S1(S0(Q^j); Q^j) with the
+ * stage Q^(1)_j stored.  qs holds 3(n_steps+1) device pointers (eta, hu, hv of
+ * Q^0 .. Q^n; Q^0 is read), q1s 3 n_steps (the stages).  The recomputation a
+ * checkpointed gradient makes of one segment; exact build. */
+int gpo_cdklm16_trajectory_f32(const gpo_cdklm16_params* p, int32_t n_steps, float dt, float* const* qs,
+                               float* const* q1s, int32_t pitch, const float* Hi, int32_t Hi_pitch,
+                               const float* f_cells, uint32_t boundary_conditions, void* stream);
+
+/* Reverse sweep over such a trajectory: lam (3 pointers) holds lambda at
+ * Q^n on entry and (dQ^n/dQ^0)^T lambda on return; work holds 6 arrays of
+ * the state's shape used as scratch. */
+int gpo_cdklm16_adjoint_steps_f32(const gpo_cdklm16_params* p, int32_t n_steps, float dt, const float* const* qs,
+                                  const float* const* q1s, float* const* lam, float* const* work, int32_t pitch,
+                                  const float* Hi, int32_t Hi_pitch, const float* f_cells,
+                                  uint32_t boundary_conditions, void* workspace, int32_t* flag, void* stream);
+
 /* Adjoint of the state dependence of gpo_geostrophic_balance for a given
  * coarse field: lambda_eta += -g/f d_y(coarse) lambda_hu + g/f d_x(coarse)
  * lambda_hv, in place (the noise is additive otherwise). */
