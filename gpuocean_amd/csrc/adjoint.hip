@@ -41,17 +41,17 @@ __global__ void a1_faces(Geo g, T dt, int rk_step, const T* Hi, const T* fc, Fwd
           blockIdx.x * blockDim.x + threadIdx.x);
 }
 template <class T>
-__global__ void a2_cells(Geo g, T dt, int rk_step, const T* Hi, const T* fc, Fwd<T> w, Rev<T> rv, const T* in_e,
-                         const T* in_u, const T* in_v, const T* out_e, const T* lo_e, const T* lo_u,
-                         const T* lo_v) {
-  a2_cell(g, dt, rk_step, Hi, fc, w, rv, in_e, in_u, in_v, out_e, lo_e, lo_u, lo_v,
-          blockIdx.y * blockDim.y + threadIdx.y, blockIdx.x * blockDim.x + threadIdx.x);
+__global__ void a2_cells(Geo g, T dt, int rk_step, const T* Hi, const T* fc, Fwd<T> w, Rev<T> rv, const T* out_e,
+                         const T* lo_e, const T* lo_u, const T* lo_v) {
+  a2_cell(g, dt, rk_step, Hi, fc, w, rv, out_e, lo_e, lo_u, lo_v, blockIdx.y * blockDim.y + threadIdx.y,
+          blockIdx.x * blockDim.x + threadIdx.x);
 }
 template <class T>
-__global__ void a3_fold(Geo g, int rk_step, int accumulate, const T* Hi, Rev<T> rv, const T* out_e, const T* lo_e,
-                        const T* lo_u, const T* lo_v, T* li_e, T* li_u, T* li_v, T* lq_e, T* lq_u, T* lq_v) {
-  a3_cell(g, rk_step, accumulate, Hi, rv, out_e, lo_e, lo_u, lo_v, li_e, li_u, li_v, lq_e, lq_u, lq_v,
-          blockIdx.y * blockDim.y + threadIdx.y + 2, blockIdx.x * blockDim.x + threadIdx.x + 2);
+__global__ void a3_fold(Geo g, int rk_step, int accumulate, int lq_mode, const T* Hi, Fwd<T> w, Rev<T> rv,
+                        const T* in_e, const T* out_e, const T* lo_e, const T* lo_u, const T* lo_v, T* li_e, T* li_u,
+                        T* li_v, T* lq_e, T* lq_u, T* lq_v) {
+  a3_cell(g, rk_step, accumulate, lq_mode, Hi, w, rv, in_e, out_e, lo_e, lo_u, lo_v, li_e, li_u, li_v, lq_e, lq_u,
+          lq_v, blockIdx.y * blockDim.y + threadIdx.y + 2, blockIdx.x * blockDim.x + threadIdx.x + 2);
 }
 
 // Dual <-> split value/tangent arrays for the tangent-linear entry point.
@@ -140,20 +140,11 @@ __global__ void misfit(Geo g, int n, const float* pos, const float* obs, const T
   if ((threadIdx.x & 63) == 0) atomicAdd(J, jd);
 }
 
-// Fill / copy of the interior cells of three state arrays (pitch sp).
-template <class T>
-__global__ void state_zero(Geo g, T* a, T* b, T* c) {
-  const int col = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y * blockDim.y + threadIdx.y;
-  if (col >= g.nx + 4 || r >= g.ny + 4) return;
-  const size_t k = (size_t)r * g.sp + col;
-  a[k] = T(0);
-  b[k] = T(0);
-  c[k] = T(0);
-}
+// Copy of the interior cells of three state arrays (pitch sp).
 template <class T>
 __global__ void state_copy(Geo g, const T* a, const T* b, const T* c, T* x, T* y, T* z) {
-  const int col = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y * blockDim.y + threadIdx.y;
-  if (col >= g.nx + 4 || r >= g.ny + 4) return;
+  const int col = blockIdx.x * blockDim.x + threadIdx.x + 2, r = blockIdx.y * blockDim.y + threadIdx.y + 2;
+  if (col > g.nx + 1 || r > g.ny + 1) return;
   const size_t k = (size_t)r * g.sp + col;
   x[k] = a[k];
   y[k] = b[k];
@@ -235,19 +226,21 @@ int forward_stages(const Geo& g, S dt, int rk, const S* ie, const S* iu, const S
   return hip_check(hipGetLastError(), "differentiable forward launch");
 }
 
+// Reverse of one substep.  lambda_in = (or +=, with accumulate) the
+// sensitivity of the stage input; for rk == 1 the direct Q^n term is added to
+// lambda_qn, or written to it with lq_mode 1.
 template <class T>
 int adjoint_stages(const Geo& g, T dt, int rk, const T* ie, const T* iu, const T* iv, const T* out_e, const T* le,
                    const T* lu, const T* lv, T* lie, T* liu, T* liv, T* lqe, T* lqu, T* lqv, int accumulate,
-                   const T* Hi, const T* fc, void* ws, int* flag, hipStream_t s) {
+                   int lq_mode, const T* Hi, const T* fc, void* ws, int* flag, hipStream_t s) {
   Fwd<T> w = carve<T>(ws, g.nx, g.ny);
   Rev<T> rv = carve_rev<T>(ws, g.nx, g.ny);
   hipLaunchKernelGGL(f1_convert<T>, cell_grid(g), kBlock, 0, s, g, ie, iu, iv, Hi, w, flag);
   hipLaunchKernelGGL(f2_slopes<T>, cell_grid(g), kBlock, 0, s, g, Hi, fc, w);
   hipLaunchKernelGGL(a1_faces<T>, cell_grid(g), kBlock, 0, s, g, dt, rk, Hi, fc, w, rv, out_e, le, lu, lv);
-  hipLaunchKernelGGL(a2_cells<T>, cell_grid(g), kBlock, 0, s, g, dt, rk, Hi, fc, w, rv, ie, iu, iv, out_e, le, lu,
-                     lv);
-  hipLaunchKernelGGL(a3_fold<T>, inner_grid(g), kBlock, 0, s, g, rk, accumulate, Hi, rv, out_e, le, lu, lv, lie, liu,
-                     liv, lqe, lqu, lqv);
+  hipLaunchKernelGGL(a2_cells<T>, cell_grid(g), kBlock, 0, s, g, dt, rk, Hi, fc, w, rv, out_e, le, lu, lv);
+  hipLaunchKernelGGL(a3_fold<T>, inner_grid(g), kBlock, 0, s, g, rk, accumulate, lq_mode, Hi, w, rv, ie, out_e, le,
+                     lu, lv, lie, liu, liv, lqe, lqu, lqv);
   return hip_check(hipGetLastError(), "adjoint launch");
 }
 
@@ -257,7 +250,7 @@ extern "C" {
 
 size_t gpo_adjoint_workspace_bytes(int32_t nx, int32_t ny, int32_t elem_bytes) {
   if (nx < 1 || ny < 1 || (elem_bytes != 4 && elem_bytes != 8)) return 0;
-  return (size_t)36 * (nx + 4) * (ny + 4) * elem_bytes;
+  return (size_t)kAdjWorkArrays * (nx + 4) * (ny + 4) * elem_bytes;
 }
 
 int gpo_cdklm16_substep_f64(const gpo_cdklm16_params* p, double dt, int32_t rk_step, const double* in_eta,
@@ -376,12 +369,13 @@ int gpo_cdklm16_adjoint_steps_f32(const gpo_cdklm16_params* p, int32_t n_steps, 
     const float* const* a = qs + 3 * j;
     const float* const* b = q1s + 3 * j;
     const float* const* c = qs + 3 * (j + 1);
-    hipLaunchKernelGGL(state_zero<float>, cell_grid(g), kBlock, 0, s, g, nxt[0], nxt[1], nxt[2]);
+    // reverse of the second RK stage (writes its direct Q^n term to nxt), then of
+    // the first (adds its sensitivity of Q^n to nxt)
     rc = adjoint_stages<float>(g, dt, 1, b[0], b[1], b[2], c[0], cur[0], cur[1], cur[2], l1[0], l1[1], l1[2], nxt[0],
-                               nxt[1], nxt[2], 0, Hi, f_cells, workspace, flag, s);
+                               nxt[1], nxt[2], 0, 1, Hi, f_cells, workspace, flag, s);
     if (!rc)
       rc = adjoint_stages<float>(g, dt, 0, a[0], a[1], a[2], b[0], l1[0], l1[1], l1[2], nxt[0], nxt[1], nxt[2],
-                                 nullptr, nullptr, nullptr, 1, Hi, f_cells, workspace, flag, s);
+                                 nullptr, nullptr, nullptr, 1, 0, Hi, f_cells, workspace, flag, s);
     if (rc) return rc;
     for (int k = 0; k < 3; ++k) {
       float* t = cur[k];
@@ -390,7 +384,7 @@ int gpo_cdklm16_adjoint_steps_f32(const gpo_cdklm16_params* p, int32_t n_steps, 
     }
   }
   if (cur[0] != lam[0])
-    hipLaunchKernelGGL(state_copy<float>, cell_grid(g), kBlock, 0, s, g, cur[0], cur[1], cur[2], lam[0], lam[1],
+    hipLaunchKernelGGL(state_copy<float>, inner_grid(g), kBlock, 0, s, g, cur[0], cur[1], cur[2], lam[0], lam[1],
                        lam[2]);
   return hip_check(hipGetLastError(), "gpo_cdklm16_adjoint_steps_f32 launch");
 }
@@ -408,7 +402,7 @@ int gpo_cdklm16_substep_adj_f32(const gpo_cdklm16_params* p, float dt, int32_t r
   if (!in_eta || !out_eta || !lam_out_eta || !lam_in_eta || !Hi || !workspace || rk_step < 0 || rk_step > 1)
     return bad_arg("gpo_cdklm16_substep_adj_f32: bad arguments");
   return adjoint_stages<float>(g, dt, rk_step, in_eta, in_hu, in_hv, out_eta, lam_out_eta, lam_out_hu, lam_out_hv,
-                               lam_in_eta, lam_in_hu, lam_in_hv, lam_qn_eta, lam_qn_hu, lam_qn_hv, accumulate, Hi,
+                               lam_in_eta, lam_in_hu, lam_in_hv, lam_qn_eta, lam_qn_hu, lam_qn_hv, accumulate, 0, Hi,
                                f_cells, workspace, flag, (hipStream_t)stream);
 }
 
@@ -425,7 +419,7 @@ int gpo_cdklm16_substep_adj_f64(const gpo_cdklm16_params* p, double dt, int32_t 
   if (!in_eta || !out_eta || !lam_out_eta || !lam_in_eta || !Hi || !workspace || rk_step < 0 || rk_step > 1)
     return bad_arg("gpo_cdklm16_substep_adj_f64: bad arguments");
   return adjoint_stages<double>(g, dt, rk_step, in_eta, in_hu, in_hv, out_eta, lam_out_eta, lam_out_hu, lam_out_hv,
-                                lam_in_eta, lam_in_hu, lam_in_hv, lam_qn_eta, lam_qn_hu, lam_qn_hv, accumulate, Hi,
+                                lam_in_eta, lam_in_hu, lam_in_hv, lam_qn_eta, lam_qn_hu, lam_qn_hv, accumulate, 0, Hi,
                                 f_cells, workspace, flag, (hipStream_t)stream);
 }
 

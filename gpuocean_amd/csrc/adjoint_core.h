@@ -457,14 +457,25 @@ GPO_HD void f4_cell(Geo g, S dt, int rk_step, const S* Hi, const S* fc, Fwd<S> w
 // ----------------------------------------------------------------------------- reverse stages
 
 // Face sensitivities: 6 values for the cell on each side of a face
-// (eta, u, v, slope0, slope1, potential).
+// (eta, u, v, slope0, slope1, potential), then the per-cell sums formed from
+// them: every sensitivity of a converted cell's (eta, u, v) is own[] of that
+// cell plus xp[] of its west neighbour, xm[] of its east neighbour, yp[] of
+// its south neighbour and ym[] of its north neighbour.
 template <class T> struct Rev {
   T* xl[6];  // x-face s -> cell s
   T* xr[6];  // x-face s -> cell s+1
   T* yl[6];  // y-face s -> cell s
   T* yr[6];  // y-face s -> cell s+1
-  T *le, *lu, *lv;  // gathered sensitivities of the loaded input, all cells
+  T* own[3];  // update/source reverse, the cell's face parts and the centre term of its own slopes
+  T* xm[3];   // x-slopes of cell s -> cell s-1
+  T* xp[3];   // x-slopes of cell s -> cell s+1
+  T* ym[3];   // y-slopes of row s -> row s-1
+  T* yp[3];   // y-slopes of row s -> row s+1
 };
+
+// Workspace arrays of (ny+4) x (nx+4) elements: the 9 converted/slope arrays
+// of Fwd, then the 24 face and 15 per-cell arrays of Rev.
+constexpr int kAdjWorkArrays = 9 + 24 + 15;
 
 template <class T> inline Rev<T> carve_rev(void* ws, int nx, int ny) {
   const size_t n = (size_t)(nx + 4) * (ny + 4);
@@ -476,9 +487,13 @@ template <class T> inline Rev<T> carve_rev(void* ws, int nx, int ny) {
     v.yl[i] = p + n * (12 + i);
     v.yr[i] = p + n * (18 + i);
   }
-  v.le = p + n * 24;
-  v.lu = p + n * 25;
-  v.lv = p + n * 26;
+  for (int i = 0; i < 3; ++i) {
+    v.own[i] = p + n * (24 + i);
+    v.xm[i] = p + n * (27 + i);
+    v.xp[i] = p + n * (30 + i);
+    v.ym[i] = p + n * (33 + i);
+    v.yp[i] = p + n * (36 + i);
+  }
   return v;
 }
 
@@ -855,16 +870,19 @@ GPO_HD void yslope_rev(const Geo& g, const Fwd<T>& w, const T* Hi, const T* fc, 
   lv[2] += ey * Fu * lufu;
 }
 
-// A2: per-cell gather of every sensitivity of the converted cell (r, c), then
-// the conversion reverse; result in rv.le/lu/lv (all cells).
+// A2: per converted cell (r, c): the reverse of its update and sources, its
+// face parts, and the reverse of its own x- and y-slopes, split into the part
+// for the cell itself and the parts for its four neighbours.  Written for
+// every cell of the (ny+4) x (nx+4) grid (zeros where nothing applies), so
+// that A3 gathers without bounds tests on the contributions.
 template <class T>
-GPO_HD void a2_cell(Geo g, T dt, int rk_step, const T* Hi, const T* fc, Fwd<T> w, Rev<T> rv, const T* in_e,
-                         const T* in_u, const T* in_v, const T* out_e, const T* lo_e, const T* lo_u,
-                         const T* lo_v, int r, int c) {
+GPO_HD void a2_cell(Geo g, T dt, int rk_step, const T* Hi, const T* fc, Fwd<T> w, Rev<T> rv, const T* out_e,
+                    const T* lo_e, const T* lo_u, const T* lo_v, int r, int c) {
   if (c >= g.nx + 4 || r >= g.ny + 4) return;
   const size_t o = (size_t)r * g.W + c;
   const bool row_in = r >= 2 && r <= g.ny + 1, col_in = c >= 2 && c <= g.nx + 1;
   T lR0 = 0, lR1 = 0, lR2 = 0;
+  T xm[3] = {0, 0, 0}, xp[3] = {0, 0, 0}, ym[3] = {0, 0, 0}, yp[3] = {0, 0, 0};
   const T G = T(g.g);
   if (row_in && col_in) {
     // update + sources (CDKLM16_kernel.cu:869-1089), reversed
@@ -901,127 +919,158 @@ GPO_HD void a2_cell(Geo g, T dt, int rk_step, const T* Hi, const T* fc, Fwd<T> w
       lR0 += lhu * R1 + lhv * R2;
     }
   }
-  // face parts and slope subgradients
   if (row_in) {
-    if (c >= 1 && c <= g.nx + 1) {
+    // face parts: the cell is the left side of x-face c and the right side of x-face c-1
+    const bool left = c >= 1 && c <= g.nx + 1, right = c - 1 >= 1 && c - 1 <= g.nx + 1;
+    T lq0 = 0, lq1 = 0, lK = 0;
+    if (left) {
       lR0 += rv.xl[0][o];
       lR1 += rv.xl[1][o];
       lR2 += rv.xl[2][o];
+      lq0 += rv.xl[3][o];
+      lq1 += rv.xl[4][o];
+      lK += rv.xl[5][o];
     }
-    if (c - 1 >= 1 && c - 1 <= g.nx + 1) {
+    if (right) {
       lR0 += rv.xr[0][o - 1];
       lR1 += rv.xr[1][o - 1];
       lR2 += rv.xr[2][o - 1];
+      lq0 += rv.xr[3][o - 1];
+      lq1 += rv.xr[4][o - 1];
+      lK += rv.xr[5][o - 1];
     }
-    for (int s = c - 1; s <= c + 1; ++s) {
-      if (s < 1 || s > g.nx + 2) continue;
-      const size_t os = (size_t)r * g.W + s;
-      T lq0 = 0, lq1 = 0, lK = 0;
-      if (s <= g.nx + 1) {
-        lq0 += rv.xl[3][os];
-        lq1 += rv.xl[4][os];
-        lK += rv.xl[5][os];
-      }
-      if (s - 1 >= 1) {
-        lq0 += rv.xr[3][os - 1];
-        lq1 += rv.xr[4][os - 1];
-        lK += rv.xr[5][os - 1];
-      }
-      if (lq0 == T(0) && lq1 == T(0) && lK == T(0)) continue;
+    if ((left || right) && !(lq0 == T(0) && lq1 == T(0) && lK == T(0))) {
       T e3[3] = {0, 0, 0}, u3[3] = {0, 0, 0}, v3[3] = {0, 0, 0};
-      xslope_rev(g, w, Hi, fc, r, s, lq0, lq1, lK, e3, u3, v3);
-      const int j = c - s + 1;
-      lR0 += e3[j];
-      lR1 += u3[j];
-      lR2 += v3[j];
+      xslope_rev(g, w, Hi, fc, r, c, lq0, lq1, lK, e3, u3, v3);
+      xm[0] = e3[0], xm[1] = u3[0], xm[2] = v3[0];
+      lR0 += e3[1], lR1 += u3[1], lR2 += v3[1];
+      xp[0] = e3[2], xp[1] = u3[2], xp[2] = v3[2];
     }
   }
   if (col_in) {
-    if (r >= 1 && r <= g.ny + 1) {
+    const bool low = r >= 1 && r <= g.ny + 1, high = r - 1 >= 1 && r - 1 <= g.ny + 1;
+    T lq0 = 0, lq1 = 0, lL = 0;
+    if (low) {
       lR0 += rv.yl[0][o];
       lR1 += rv.yl[1][o];
       lR2 += rv.yl[2][o];
+      lq0 += rv.yl[3][o];
+      lq1 += rv.yl[4][o];
+      lL += rv.yl[5][o];
     }
-    if (r - 1 >= 1 && r - 1 <= g.ny + 1) {
+    if (high) {
       lR0 += rv.yr[0][o - g.W];
       lR1 += rv.yr[1][o - g.W];
       lR2 += rv.yr[2][o - g.W];
+      lq0 += rv.yr[3][o - g.W];
+      lq1 += rv.yr[4][o - g.W];
+      lL += rv.yr[5][o - g.W];
     }
-    for (int s = r - 1; s <= r + 1; ++s) {
-      if (s < 1 || s > g.ny + 2) continue;
-      const size_t os = (size_t)s * g.W + c;
-      T lq0 = 0, lq1 = 0, lL = 0;
-      if (s <= g.ny + 1) {
-        lq0 += rv.yl[3][os];
-        lq1 += rv.yl[4][os];
-        lL += rv.yl[5][os];
-      }
-      if (s - 1 >= 1) {
-        lq0 += rv.yr[3][os - g.W];
-        lq1 += rv.yr[4][os - g.W];
-        lL += rv.yr[5][os - g.W];
-      }
-      if (lq0 == T(0) && lq1 == T(0) && lL == T(0)) continue;
+    if ((low || high) && !(lq0 == T(0) && lq1 == T(0) && lL == T(0))) {
       T e3[3] = {0, 0, 0}, u3[3] = {0, 0, 0}, v3[3] = {0, 0, 0};
-      yslope_rev(g, w, Hi, fc, s, c, lq0, lq1, lL, e3, u3, v3);
-      const int j = r - s + 1;
-      lR0 += e3[j];
-      lR1 += u3[j];
-      lR2 += v3[j];
+      yslope_rev(g, w, Hi, fc, r, c, lq0, lq1, lL, e3, u3, v3);
+      ym[0] = e3[0], ym[1] = u3[0], ym[2] = v3[0];
+      lR0 += e3[1], lR1 += u3[1], lR2 += v3[1];
+      yp[0] = e3[2], yp[1] = u3[2], yp[2] = v3[2];
     }
   }
-  // conversion reverse: u = hu/h, v = hv/h, h = eta + Hm (wet branch)
-  const Src src = map_cell(g, r, c);
-  const size_t k = (size_t)src.r * g.sp + src.c;
-  const T e = in_e[k];
-  const T Hm = cell_depth(g, Hi, r, c);
-  const T h = e + Hm;
-  T oe = 0, ou = 0, ov = 0;
-  if (h >= T(g.desing_eps)) {
-    const T u = w.cu[o], v = w.cv[o];
-    ou = lR1 / h;
-    ov = lR2 / h;
-    oe = lR0 - (lR1 * u + lR2 * v) / h;
+  rv.own[0][o] = lR0;
+  rv.own[1][o] = lR1;
+  rv.own[2][o] = lR2;
+  for (int i = 0; i < 3; ++i) {
+    rv.xm[i][o] = xm[i];
+    rv.xp[i][o] = xp[i];
+    rv.ym[i][o] = ym[i];
+    rv.yp[i][o] = yp[i];
   }
-  rv.le[o] = oe;
-  rv.lu[o] = ou * T(src.shu);
-  rv.lv[o] = ov * T(src.shv);
 }
 
-// A3: fold ghost-cell sensitivities onto the interior cells they came from;
-// write lambda_in and, for the second RK stage, add the direct Q^n term.
+// Sensitivity of the loaded stage input of converted cell (r, c) (any cell of
+// the grid): the gathered sensitivity of its (eta, u, v) through the
+// conversion u = hu/h, v = hv/h, h = eta + Hm (wet branch), with the sign of a
+// wall-mirrored momentum.
 template <class T>
-GPO_HD void a3_cell(Geo g, int rk_step, int accumulate, const T* Hi, Rev<T> rv, const T* out_e, const T* lo_e,
-                        const T* lo_u, const T* lo_v, T* li_e, T* li_u, T* li_v, T* lq_e, T* lq_u, T* lq_v, int r, int c) {
-  if (c > g.nx + 1 || r > g.ny + 1) return;
+GPO_HD void conv_rev(const Geo& g, const T* Hi, const Fwd<T>& w, const Rev<T>& rv, const T* in_e, int r, int c, T& oe,
+                     T& ou, T& ov) {
   const size_t o = (size_t)r * g.W + c;
-  T e = rv.le[o], u = rv.lu[o], v = rv.lv[o];
+  T l0 = rv.own[0][o], l1 = rv.own[1][o], l2 = rv.own[2][o];
+  if (c >= 1) {
+    l0 += rv.xp[0][o - 1];
+    l1 += rv.xp[1][o - 1];
+    l2 += rv.xp[2][o - 1];
+  }
+  if (c <= g.nx + 2) {
+    l0 += rv.xm[0][o + 1];
+    l1 += rv.xm[1][o + 1];
+    l2 += rv.xm[2][o + 1];
+  }
+  if (r >= 1) {
+    l0 += rv.yp[0][o - g.W];
+    l1 += rv.yp[1][o - g.W];
+    l2 += rv.yp[2][o - g.W];
+  }
+  if (r <= g.ny + 2) {
+    l0 += rv.ym[0][o + g.W];
+    l1 += rv.ym[1][o + g.W];
+    l2 += rv.ym[2][o + g.W];
+  }
+  const Src src = map_cell(g, r, c);
+  const T e = in_e[(size_t)src.r * g.sp + src.c];
+  const T h = e + cell_depth(g, Hi, r, c);
+  oe = ou = ov = T(0);
+  if (h >= T(g.desing_eps)) {
+    const T u = w.cu[o], v = w.cv[o];
+    ou = l1 / h * T(src.shu);
+    ov = l2 / h * T(src.shv);
+    oe = l0 - (l1 * u + l2 * v) / h;
+  }
+}
+
+// A3: the sensitivity of interior input cell (r, c): its own converted cell
+// plus the ghost cells loaded from it; written to lambda_in, and for the
+// second RK stage the direct Q^n term to lambda_qn.  lq_mode 0 adds to
+// lambda_qn, 1 overwrites it.
+template <class T>
+GPO_HD void a3_cell(Geo g, int rk_step, int accumulate, int lq_mode, const T* Hi, const Fwd<T>& w, const Rev<T>& rv,
+                    const T* in_e, const T* out_e, const T* lo_e, const T* lo_u, const T* lo_v, T* li_e, T* li_u,
+                    T* li_v, T* lq_e, T* lq_u, T* lq_v, int r, int c) {
+  if (c > g.nx + 1 || r > g.ny + 1) return;
+  T e, u, v;
+  conv_rev(g, Hi, w, rv, in_e, r, c, e, u, v);
   const int gc[4] = {0, 1, g.nx + 2, g.nx + 3};
   for (int i = 0; i < 4; ++i) {
     const Src s = map_cell(g, r, gc[i]);
     if (s.r == r && s.c == c && !(gc[i] == c)) {
-      const size_t q = (size_t)r * g.W + gc[i];
-      e += rv.le[q];
-      u += rv.lu[q];
-      v += rv.lv[q];
+      T ge, gu, gv;
+      conv_rev(g, Hi, w, rv, in_e, r, gc[i], ge, gu, gv);
+      e += ge;
+      u += gu;
+      v += gv;
     }
   }
   const int gr[4] = {0, 1, g.ny + 2, g.ny + 3};
   for (int i = 0; i < 4; ++i) {
     const Src s = map_cell(g, gr[i], c);
     if (s.r == r && s.c == c && !(gr[i] == r)) {
-      const size_t q = (size_t)gr[i] * g.W + c;
-      e += rv.le[q];
-      u += rv.lu[q];
-      v += rv.lv[q];
+      T ge, gu, gv;
+      conv_rev(g, Hi, w, rv, in_e, gr[i], c, ge, gu, gv);
+      e += ge;
+      u += gu;
+      v += gv;
     }
   }
   const size_t k = (size_t)r * g.sp + c;
   if (rk_step == 1 && lq_e) {  // read lambda_out before lambda_in may overwrite it (they may alias)
     const CellAdj<T> a = out_adj(g, Hi, out_e, lo_e, lo_u, lo_v, r, c);
-    lq_e[k] += T(0.5) * a.e;
-    lq_u[k] += T(0.5) * a.u;
-    lq_v[k] += T(0.5) * a.v;
+    if (lq_mode) {
+      lq_e[k] = T(0.5) * a.e;
+      lq_u[k] = T(0.5) * a.u;
+      lq_v[k] = T(0.5) * a.v;
+    } else {
+      lq_e[k] += T(0.5) * a.e;
+      lq_u[k] += T(0.5) * a.u;
+      lq_v[k] += T(0.5) * a.v;
+    }
   }
   if (accumulate) {
     li_e[k] += e;

@@ -331,8 +331,9 @@ int gpo_cdklm16_trajectory_f32(const gpo_cdklm16_params* p, int32_t n_steps, flo
                                const float* f_cells, uint32_t boundary_conditions, void* stream);
 
 /* Reverse sweep over such a trajectory: lam (3 pointers) holds lambda at
- * Q^n on entry and (dQ^n/dQ^0)^T lambda on return; work holds 6 arrays of
- * the state's shape used as scratch. */
+ * Q^n on entry and (dQ^n/dQ^0)^T lambda on return (interior cells; the ghost
+ * cells of lam are not written); work holds 6 arrays of the state's shape
+ * used as scratch. */
 int gpo_cdklm16_adjoint_steps_f32(const gpo_cdklm16_params* p, int32_t n_steps, float dt, const float* const* qs,
                                   const float* const* q1s, float* const* lam, float* const* work, int32_t pitch,
                                   const float* Hi, int32_t Hi_pitch, const float* f_cells,

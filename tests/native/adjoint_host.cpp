@@ -40,7 +40,7 @@ int make_geo(const gpo_cdklm16_params* p, uint32_t bc, Geo* g) {
 }
 
 template <class S> Fwd<S> carve_vec(std::vector<S>& ws, const Geo& g) {
-  ws.assign((size_t)36 * (g.nx + 4) * (g.ny + 4), S(0.0));
+  ws.assign((size_t)kAdjWorkArrays * (g.nx + 4) * (g.ny + 4), S(0.0));
   return carve<S>(ws.data(), g.nx, g.ny);
 }
 
@@ -128,10 +128,10 @@ int gpo_host_substep_adj_f64(const gpo_cdklm16_params* p, double dt, int rk, con
   for (int r = 0; r < H; ++r)
     for (int c = 0; c < W; ++c) a1_cell(g, dt, rk, Hi, fc, w, rv, out_e, le, lu, lv, r, c);
   for (int r = 0; r < H; ++r)
-    for (int c = 0; c < W; ++c) a2_cell(g, dt, rk, Hi, fc, w, rv, ie, iu, iv, out_e, le, lu, lv, r, c);
+    for (int c = 0; c < W; ++c) a2_cell(g, dt, rk, Hi, fc, w, rv, out_e, le, lu, lv, r, c);
   for (int r = 2; r <= g.ny + 1; ++r)
     for (int c = 2; c <= g.nx + 1; ++c)
-      a3_cell(g, rk, accumulate, Hi, rv, out_e, le, lu, lv, lie, liu, liv, lqe, lqu, lqv, r, c);
+      a3_cell(g, rk, accumulate, 0, Hi, w, rv, ie, out_e, le, lu, lv, lie, liu, liv, lqe, lqu, lqv, r, c);
   return 0;
 }
 
