@@ -34,11 +34,42 @@ __global__ void f4_update(Geo g, S dt, int rk_step, const S* Hi, const S* fc, Fw
   f4_cell(g, dt, rk_step, Hi, fc, w, qn_e, qn_u, qn_v, out_e, out_u, out_v, flag,
           blockIdx.y * blockDim.y + threadIdx.y + 2, blockIdx.x * blockDim.x + threadIdx.x + 2);
 }
+// A1 with the slopes of F2 formed in the same launch: a block of 64 x kSlopeRows
+// cells computes the x- and y-slopes of its cells (plus the column east of
+// it and the row north of it) into LDS, then the reverse of its cells'
+// east and north faces.
+constexpr int kSlopeRows = 8;
 template <class T>
-__global__ void a1_faces(Geo g, T dt, int rk_step, const T* Hi, const T* fc, Fwd<T> w, Rev<T> rv, const T* out_e,
-                         const T* lo_e, const T* lo_u, const T* lo_v) {
-  a1_cell(g, dt, rk_step, Hi, fc, w, rv, out_e, lo_e, lo_u, lo_v, blockIdx.y * blockDim.y + threadIdx.y,
-          blockIdx.x * blockDim.x + threadIdx.x);
+__global__ __launch_bounds__(64 * kSlopeRows) void a1_faces(Geo g, T dt, int rk_step, const T* Hi, const T* fc,
+                                                            Fwd<T> w, Rev<T> rv, const T* out_e, const T* lo_e,
+                                                            const T* lo_u, const T* lo_v) {
+  __shared__ T sx[3][kSlopeRows][65];
+  __shared__ T sy[3][kSlopeRows + 1][64];
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int c0 = blockIdx.x * 64, r0 = blockIdx.y * kSlopeRows;
+  const int c = c0 + tx, r = r0 + ty;
+  T a = 0, b = 0, k = 0;
+  if (has_xslope(g, r, c)) xslope(g, w, Hi, fc, r, c, a, b, k);
+  sx[0][ty][tx] = a, sx[1][ty][tx] = b, sx[2][ty][tx] = k;
+  a = b = k = 0;
+  if (has_yslope(g, r, c)) yslope(g, w, Hi, fc, r, c, a, b, k);
+  sy[0][ty][tx] = a, sy[1][ty][tx] = b, sy[2][ty][tx] = k;
+  if (ty == 0) {  // the row north of the block
+    a = b = k = 0;
+    if (has_yslope(g, r0 + kSlopeRows, c)) yslope(g, w, Hi, fc, r0 + kSlopeRows, c, a, b, k);
+    sy[0][kSlopeRows][tx] = a, sy[1][kSlopeRows][tx] = b, sy[2][kSlopeRows][tx] = k;
+  } else if (ty == 1 && tx < kSlopeRows) {  // the column east of the block
+    a = b = k = 0;
+    if (has_xslope(g, r0 + tx, c0 + 64)) xslope(g, w, Hi, fc, r0 + tx, c0 + 64, a, b, k);
+    sx[0][tx][64] = a, sx[1][tx][64] = b, sx[2][tx][64] = k;
+  }
+  __syncthreads();
+  if (has_xface(g, r, c))
+    a1_xface(g, dt, rk_step, Hi, fc, w, rv, out_e, lo_e, lo_u, lo_v, r, c, sx[0][ty][tx], sx[1][ty][tx],
+             sx[2][ty][tx], sx[0][ty][tx + 1], sx[1][ty][tx + 1], sx[2][ty][tx + 1]);
+  if (has_yface(g, r, c))
+    a1_yface(g, dt, rk_step, Hi, fc, w, rv, out_e, lo_e, lo_u, lo_v, r, c, sy[0][ty][tx], sy[1][ty][tx],
+             sy[2][ty][tx], sy[0][ty + 1][tx], sy[1][ty + 1][tx], sy[2][ty + 1][tx]);
 }
 template <class T>
 __global__ void a2_cells(Geo g, T dt, int rk_step, const T* Hi, const T* fc, Fwd<T> w, Rev<T> rv, const T* out_e,
@@ -236,8 +267,8 @@ int adjoint_stages(const Geo& g, T dt, int rk, const T* ie, const T* iu, const T
   Fwd<T> w = carve<T>(ws, g.nx, g.ny);
   Rev<T> rv = carve_rev<T>(ws, g.nx, g.ny);
   hipLaunchKernelGGL(f1_convert<T>, cell_grid(g), kBlock, 0, s, g, ie, iu, iv, Hi, w, flag);
-  hipLaunchKernelGGL(f2_slopes<T>, cell_grid(g), kBlock, 0, s, g, Hi, fc, w);
-  hipLaunchKernelGGL(a1_faces<T>, cell_grid(g), kBlock, 0, s, g, dt, rk, Hi, fc, w, rv, out_e, le, lu, lv);
+  const dim3 face_grid((g.nx + 4 + 63) / 64, (g.ny + 4 + kSlopeRows - 1) / kSlopeRows);
+  hipLaunchKernelGGL(a1_faces<T>, face_grid, dim3(64, kSlopeRows), 0, s, g, dt, rk, Hi, fc, w, rv, out_e, le, lu, lv);
   hipLaunchKernelGGL(a2_cells<T>, cell_grid(g), kBlock, 0, s, g, dt, rk, Hi, fc, w, rv, out_e, le, lu, lv);
   hipLaunchKernelGGL(a3_fold<T>, inner_grid(g), kBlock, 0, s, g, rk, accumulate, lq_mode, Hi, w, rv, ie, out_e, le,
                      lu, lv, lie, liu, liv, lqe, lqu, lqv);

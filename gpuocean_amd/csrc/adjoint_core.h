@@ -456,16 +456,16 @@ GPO_HD void f4_cell(Geo g, S dt, int rk_step, const S* Hi, const S* fc, Fwd<S> w
 
 // ----------------------------------------------------------------------------- reverse stages
 
-// Face sensitivities: 6 values for the cell on each side of a face
-// (eta, u, v, slope0, slope1, potential), then the per-cell sums formed from
-// them: every sensitivity of a converted cell's (eta, u, v) is own[] of that
+// Face sensitivities: for the reconstructed state on each side of a face,
+// those of (eta, first, second flux component), the eta one already zeroed
+// where the -H clamp was active; then the per-cell sums formed from them: every sensitivity of a converted cell's (eta, u, v) is own[] of that
 // cell plus xp[] of its west neighbour, xm[] of its east neighbour, yp[] of
 // its south neighbour and ym[] of its north neighbour.
 template <class T> struct Rev {
-  T* xl[6];  // x-face s -> cell s
-  T* xr[6];  // x-face s -> cell s+1
-  T* yl[6];  // y-face s -> cell s
-  T* yr[6];  // y-face s -> cell s+1
+  T* xl[3];  // x-face s, side of cell s: (eta, u, v)
+  T* xr[3];  // x-face s, side of cell s+1
+  T* yl[3];  // y-face s, side of cell s: (eta, v, u)
+  T* yr[3];  // y-face s, side of cell s+1
   T* own[3];  // update/source reverse, the cell's face parts and the centre term of its own slopes
   T* xm[3];   // x-slopes of cell s -> cell s-1
   T* xp[3];   // x-slopes of cell s -> cell s+1
@@ -474,25 +474,23 @@ template <class T> struct Rev {
 };
 
 // Workspace arrays of (ny+4) x (nx+4) elements: the 9 converted/slope arrays
-// of Fwd, then the 24 face and 15 per-cell arrays of Rev.
-constexpr int kAdjWorkArrays = 9 + 24 + 15;
+// of Fwd, then the 12 face and 15 per-cell arrays of Rev.
+constexpr int kAdjWorkArrays = 9 + 12 + 15;
 
 template <class T> inline Rev<T> carve_rev(void* ws, int nx, int ny) {
   const size_t n = (size_t)(nx + 4) * (ny + 4);
   T* p = (T*)ws + 9 * n;  // after conv (3) and slopes (6) of Fwd<T>
   Rev<T> v;
-  for (int i = 0; i < 6; ++i) {
-    v.xl[i] = p + n * i;
-    v.xr[i] = p + n * (6 + i);
-    v.yl[i] = p + n * (12 + i);
-    v.yr[i] = p + n * (18 + i);
-  }
   for (int i = 0; i < 3; ++i) {
-    v.own[i] = p + n * (24 + i);
-    v.xm[i] = p + n * (27 + i);
-    v.xp[i] = p + n * (30 + i);
-    v.ym[i] = p + n * (33 + i);
-    v.yp[i] = p + n * (36 + i);
+    v.xl[i] = p + n * i;
+    v.xr[i] = p + n * (3 + i);
+    v.yl[i] = p + n * (6 + i);
+    v.yr[i] = p + n * (9 + i);
+    v.own[i] = p + n * (12 + i);
+    v.xm[i] = p + n * (15 + i);
+    v.xp[i] = p + n * (18 + i);
+    v.ym[i] = p + n * (21 + i);
+    v.yp[i] = p + n * (24 + i);
   }
   return v;
 }
@@ -588,117 +586,105 @@ GPO_HD inline CellAdj<T> out_adj(const Geo& g, const T* Hi, const T* out_e, cons
   return a;
 }
 
-// A1: reverse of the face fluxes.
+// A1: reverse of the face fluxes.  x-face c of row r (between cells c and
+// c+1), given the x-slopes (q0, q1, K) of both cells.
 template <class T>
-GPO_HD void a1_cell(Geo g, T dt, int rk_step, const T* Hi, const T* fc, Fwd<T> w, Rev<T> rv, const T* out_e,
-                         const T* lo_e, const T* lo_u, const T* lo_v, int r, int c) {
-  if (c >= g.nx + 4 || r >= g.ny + 4) return;
+GPO_HD void a1_xface(const Geo& g, T dt, int rk_step, const T* Hi, const T* fc, const Fwd<T>& w, const Rev<T>& rv,
+                     const T* out_e, const T* lo_e, const T* lo_u, const T* lo_v, int r, int c, T qm0, T qm1, T km,
+                     T qp0, T qp1, T kp) {
   const size_t o = (size_t)r * g.W + c;
   const T coef = (rk_step == 0 ? T(1) : T(0.5)) * dt;
   const T G = T(g.g), g2 = T(2) * G;
-  if (r >= 2 && r <= g.ny + 1 && c >= 1 && c <= g.nx + 1) {
-    // lambda of the three x-fluxes at face c: L_k = -(F_c - F_{c-1})/dx - ...
-    const CellAdj<T> am = out_adj(g, Hi, out_e, lo_e, lo_u, lo_v, r, c);
-    const CellAdj<T> ap = out_adj(g, Hi, out_e, lo_e, lo_u, lo_v, r, c + 1);
-    const T dxx = T(g.dx);
-    const T lF[3] = {coef * ap.e / dxx - coef * am.e / dxx, coef * ap.u / dxx - coef * am.u / dxx,
-                     coef * ap.v / dxx - coef * am.v / dxx};
-    const size_t m = o, p = o + 1;
-    const T ebp = w.ce[p], ebm = w.ce[m], up = w.cu[p], um = w.cu[m];
-    const T vp0 = w.cv[p], vm0 = w.cv[m];
-    const T Rpx = up - T(0.5) * w.qx0[p], Rpy = vp0 - T(0.5) * w.qx1[p];
-    const T Rmx = um + T(0.5) * w.qx0[m], Rmy = vm0 + T(0.5) * w.qx1[m];
-    const T H = T(0.5) * (corner(g, Hi, r, c + 1) + corner(g, Hi, r + 1, c + 1));
-    const T sm = (g.bc_w == 1 && c + 1 == 2) ? T(-1) : T(1);
-    const T sp = (g.bc_e == 1 && c + 1 == g.nx + 2) ? T(-1) : T(1);
-    const T nx_ = T(g.north_x), ny_ = T(g.north_y);
-    const T vpn = up * nx_ + (sp * vp0) * ny_, vmn = um * nx_ + (sm * vm0) * ny_;
-    const T fp = fcor(g, fc, r, c + 1), fm = fcor(g, fc, r, c);
-    const T argp = ebp - (w.kx[p] + T(g.dx) * fp * vpn) / g2;
-    const T argm = ebm + (w.kx[m] + T(g.dx) * fm * vmn) / g2;
-    const T Qp[3] = {(-H < argp) ? argp : -H, Rpx, Rpy}, Qm[3] = {(-H < argm) ? argm : -H, Rmx, Rmy};
-    T lQp[3] = {0, 0, 0}, lQm[3] = {0, 0, 0};
-    flux_rev(g, Qm, Qp, H, lF, lQm, lQp);
-    T L[6] = {0, 0, 0, 0, 0, 0}, R[6] = {0, 0, 0, 0, 0, 0};  // e, u, v, q0, q1, K
-    if (-H < argp) {
-      R[0] += lQp[0];
-      R[5] -= lQp[0] / g2;
-      const T lvpn = -lQp[0] * T(g.dx) * fp / g2;
-      R[1] += nx_ * lvpn;
-      R[2] += sp * ny_ * lvpn;
-    }
-    if (-H < argm) {
-      L[0] += lQm[0];
-      L[5] += lQm[0] / g2;
-      const T lvmn = lQm[0] * T(g.dx) * fm / g2;
-      L[1] += nx_ * lvmn;
-      L[2] += sm * ny_ * lvmn;
-    }
-    R[1] += lQp[1];
-    R[3] -= T(0.5) * lQp[1];
-    R[2] += lQp[2];
-    R[4] -= T(0.5) * lQp[2];
-    L[1] += lQm[1];
-    L[3] += T(0.5) * lQm[1];
-    L[2] += lQm[2];
-    L[4] += T(0.5) * lQm[2];
-    for (int i = 0; i < 6; ++i) {
-      rv.xl[i][o] = L[i];
-      rv.xr[i][o] = R[i];
-    }
-  }
-  if (r >= 1 && r <= g.ny + 1 && c >= 2 && c <= g.nx + 1) {
-    const CellAdj<T> am = out_adj(g, Hi, out_e, lo_e, lo_u, lo_v, r, c);
-    const CellAdj<T> ap = out_adj(g, Hi, out_e, lo_e, lo_u, lo_v, r + 1, c);
-    const T dyy = T(g.dy);
-    const T lG[3] = {coef * ap.e / dyy - coef * am.e / dyy, coef * ap.u / dyy - coef * am.u / dyy,
-                     coef * ap.v / dyy - coef * am.v / dyy};
-    const T lfl[3] = {lG[0], lG[2], lG[1]};  // G = (fl.x, fl.z, fl.y)
-    const size_t m = o, p = o + g.W;
-    const T ebp = w.ce[p], ebm = w.ce[m], vp = w.cv[p], vm = w.cv[m];
-    const T up0 = w.cu[p], um0 = w.cu[m];
-    const T Rpx = up0 - T(0.5) * w.qy0[p], Rpy = vp - T(0.5) * w.qy1[p];
-    const T Rmx = um0 + T(0.5) * w.qy0[m], Rmy = vm + T(0.5) * w.qy1[m];
-    const T H = T(0.5) * (corner(g, Hi, r + 1, c) + corner(g, Hi, r + 1, c + 1));
-    const T sm = (g.bc_s == 1 && r + 1 == 2) ? T(-1) : T(1);
-    const T sp = (g.bc_n == 1 && r + 1 == g.ny + 2) ? T(-1) : T(1);
-    const T ex = T(g.north_y), ey = T(-g.north_x);
-    const T upe = (sp * up0) * ex + vp * ey, ume = (sm * um0) * ex + vm * ey;
-    const T fp = fcor(g, fc, r + 1, c), fm = fcor(g, fc, r, c);
-    const T argp = ebp - (w.ly[p] - T(g.dy) * fp * upe) / g2;
-    const T argm = ebm + (w.ly[m] - T(g.dy) * fm * ume) / g2;
-    const T Qp[3] = {(-H < argp) ? argp : -H, Rpy, Rpx}, Qm[3] = {(-H < argm) ? argm : -H, Rmy, Rmx};
-    T lQp[3] = {0, 0, 0}, lQm[3] = {0, 0, 0};
-    flux_rev(g, Qm, Qp, H, lfl, lQm, lQp);
-    T L[6] = {0, 0, 0, 0, 0, 0}, R[6] = {0, 0, 0, 0, 0, 0};
-    if (-H < argp) {
-      R[0] += lQp[0];
-      R[5] -= lQp[0] / g2;
-      const T lupe = lQp[0] * T(g.dy) * fp / g2;
-      R[1] += sp * ex * lupe;
-      R[2] += ey * lupe;
-    }
-    if (-H < argm) {
-      L[0] += lQm[0];
-      L[5] += lQm[0] / g2;
-      const T lume = -lQm[0] * T(g.dy) * fm / g2;
-      L[1] += sm * ex * lume;
-      L[2] += ey * lume;
-    }
-    // Qp = (eta, Rpy, Rpx): index 1 is v, index 2 is u
-    R[2] += lQp[1];
-    R[4] -= T(0.5) * lQp[1];
-    R[1] += lQp[2];
-    R[3] -= T(0.5) * lQp[2];
-    L[2] += lQm[1];
-    L[4] += T(0.5) * lQm[1];
-    L[1] += lQm[2];
-    L[3] += T(0.5) * lQm[2];
-    for (int i = 0; i < 6; ++i) {
-      rv.yl[i][o] = L[i];
-      rv.yr[i][o] = R[i];
-    }
-  }
+  // lambda of the three x-fluxes at face c: L_k = -(F_c - F_{c-1})/dx - ...
+  const CellAdj<T> am = out_adj(g, Hi, out_e, lo_e, lo_u, lo_v, r, c);
+  const CellAdj<T> ap = out_adj(g, Hi, out_e, lo_e, lo_u, lo_v, r, c + 1);
+  const T dxx = T(g.dx);
+  const T lF[3] = {coef * ap.e / dxx - coef * am.e / dxx, coef * ap.u / dxx - coef * am.u / dxx,
+                   coef * ap.v / dxx - coef * am.v / dxx};
+  const size_t m = o, p = o + 1;
+  const T ebp = w.ce[p], ebm = w.ce[m], up = w.cu[p], um = w.cu[m];
+  const T vp0 = w.cv[p], vm0 = w.cv[m];
+  const T Rpx = up - T(0.5) * qp0, Rpy = vp0 - T(0.5) * qp1;
+  const T Rmx = um + T(0.5) * qm0, Rmy = vm0 + T(0.5) * qm1;
+  const T H = T(0.5) * (corner(g, Hi, r, c + 1) + corner(g, Hi, r + 1, c + 1));
+  const T sm = (g.bc_w == 1 && c + 1 == 2) ? T(-1) : T(1);
+  const T sp = (g.bc_e == 1 && c + 1 == g.nx + 2) ? T(-1) : T(1);
+  const T nx_ = T(g.north_x), ny_ = T(g.north_y);
+  const T vpn = up * nx_ + (sp * vp0) * ny_, vmn = um * nx_ + (sm * vm0) * ny_;
+  const T fp = fcor(g, fc, r, c + 1), fm = fcor(g, fc, r, c);
+  const T argp = ebp - (kp + T(g.dx) * fp * vpn) / g2;
+  const T argm = ebm + (km + T(g.dx) * fm * vmn) / g2;
+  const T Qp[3] = {(-H < argp) ? argp : -H, Rpx, Rpy}, Qm[3] = {(-H < argm) ? argm : -H, Rmx, Rmy};
+  T lQp[3] = {0, 0, 0}, lQm[3] = {0, 0, 0};
+  flux_rev(g, Qm, Qp, H, lF, lQm, lQp);
+  // the reconstructed-eta sensitivity is zero where the -H clamp was active
+  rv.xl[0][o] = (-H < argm) ? lQm[0] : T(0);
+  rv.xl[1][o] = lQm[1];
+  rv.xl[2][o] = lQm[2];
+  rv.xr[0][o] = (-H < argp) ? lQp[0] : T(0);
+  rv.xr[1][o] = lQp[1];
+  rv.xr[2][o] = lQp[2];
+}
+
+// y-face r of column c (between rows r and r+1), given the y-slopes
+// (q0, q1, L) of both cells.
+template <class T>
+GPO_HD void a1_yface(const Geo& g, T dt, int rk_step, const T* Hi, const T* fc, const Fwd<T>& w, const Rev<T>& rv,
+                     const T* out_e, const T* lo_e, const T* lo_u, const T* lo_v, int r, int c, T qm0, T qm1, T lm,
+                     T qp0, T qp1, T lp) {
+  const size_t o = (size_t)r * g.W + c;
+  const T coef = (rk_step == 0 ? T(1) : T(0.5)) * dt;
+  const T G = T(g.g), g2 = T(2) * G;
+  const CellAdj<T> am = out_adj(g, Hi, out_e, lo_e, lo_u, lo_v, r, c);
+  const CellAdj<T> ap = out_adj(g, Hi, out_e, lo_e, lo_u, lo_v, r + 1, c);
+  const T dyy = T(g.dy);
+  const T lG[3] = {coef * ap.e / dyy - coef * am.e / dyy, coef * ap.u / dyy - coef * am.u / dyy,
+                   coef * ap.v / dyy - coef * am.v / dyy};
+  const T lfl[3] = {lG[0], lG[2], lG[1]};  // G = (fl.x, fl.z, fl.y)
+  const size_t m = o, p = o + g.W;
+  const T ebp = w.ce[p], ebm = w.ce[m], vp = w.cv[p], vm = w.cv[m];
+  const T up0 = w.cu[p], um0 = w.cu[m];
+  const T Rpx = up0 - T(0.5) * qp0, Rpy = vp - T(0.5) * qp1;
+  const T Rmx = um0 + T(0.5) * qm0, Rmy = vm + T(0.5) * qm1;
+  const T H = T(0.5) * (corner(g, Hi, r + 1, c) + corner(g, Hi, r + 1, c + 1));
+  const T sm = (g.bc_s == 1 && r + 1 == 2) ? T(-1) : T(1);
+  const T sp = (g.bc_n == 1 && r + 1 == g.ny + 2) ? T(-1) : T(1);
+  const T ex = T(g.north_y), ey = T(-g.north_x);
+  const T upe = (sp * up0) * ex + vp * ey, ume = (sm * um0) * ex + vm * ey;
+  const T fp = fcor(g, fc, r + 1, c), fm = fcor(g, fc, r, c);
+  const T argp = ebp - (lp - T(g.dy) * fp * upe) / g2;
+  const T argm = ebm + (lm - T(g.dy) * fm * ume) / g2;
+  const T Qp[3] = {(-H < argp) ? argp : -H, Rpy, Rpx}, Qm[3] = {(-H < argm) ? argm : -H, Rmy, Rmx};
+  T lQp[3] = {0, 0, 0}, lQm[3] = {0, 0, 0};
+  flux_rev(g, Qm, Qp, H, lfl, lQm, lQp);
+  rv.yl[0][o] = (-H < argm) ? lQm[0] : T(0);
+  rv.yl[1][o] = lQm[1];
+  rv.yl[2][o] = lQm[2];
+  rv.yr[0][o] = (-H < argp) ? lQp[0] : T(0);
+  rv.yr[1][o] = lQp[1];
+  rv.yr[2][o] = lQp[2];
+}
+
+// Which faces exist: x-faces c in [1, nx+1] of interior rows, y-faces r in
+// [1, ny+1] of interior columns; x-slopes at c in [1, nx+2] of interior rows,
+// y-slopes at r in [1, ny+2] of interior columns.
+GPO_HD inline bool has_xface(const Geo& g, int r, int c) { return r >= 2 && r <= g.ny + 1 && c >= 1 && c <= g.nx + 1; }
+GPO_HD inline bool has_yface(const Geo& g, int r, int c) { return r >= 1 && r <= g.ny + 1 && c >= 2 && c <= g.nx + 1; }
+GPO_HD inline bool has_xslope(const Geo& g, int r, int c) { return r >= 2 && r <= g.ny + 1 && c >= 1 && c <= g.nx + 2; }
+GPO_HD inline bool has_yslope(const Geo& g, int r, int c) { return r >= 1 && r <= g.ny + 2 && c >= 2 && c <= g.nx + 1; }
+
+// Both faces of cell (r, c) with the slopes stored by F2.
+template <class T>
+GPO_HD void a1_cell(Geo g, T dt, int rk_step, const T* Hi, const T* fc, Fwd<T> w, Rev<T> rv, const T* out_e,
+                    const T* lo_e, const T* lo_u, const T* lo_v, int r, int c) {
+  if (c >= g.nx + 4 || r >= g.ny + 4) return;
+  const size_t o = (size_t)r * g.W + c;
+  if (has_xface(g, r, c))
+    a1_xface(g, dt, rk_step, Hi, fc, w, rv, out_e, lo_e, lo_u, lo_v, r, c, w.qx0[o], w.qx1[o], w.kx[o], w.qx0[o + 1],
+             w.qx1[o + 1], w.kx[o + 1]);
+  if (has_yface(g, r, c))
+    a1_yface(g, dt, rk_step, Hi, fc, w, rv, out_e, lo_e, lo_u, lo_v, r, c, w.qy0[o], w.qy1[o], w.ly[o],
+             w.qy0[o + g.W], w.qy1[o + g.W], w.ly[o + g.W]);
 }
 
 // minmod_raw reverse: d out / d (b, c, f) for the selected argument
@@ -710,10 +696,11 @@ template <class T> GPO_HD inline void minmod_rev(T b, T c, T f, T lout, T& lb, T
   int sel = (ac < ab) ? 1 : 0;
   const T m1 = sel ? ac : ab;
   if (af < m1) sel = 2;
+  // selects rather than a branch on sel: the compiler keeps lb, lc, lf in registers
   const T l = lout * s;
-  if (sel == 0) lb += l * sb;
-  else if (sel == 1) lc += l * sc;
-  else lf += l * sf;
+  lb += (sel == 0) ? l * sb : T(0);
+  lc += (sel == 1) ? l * sc : T(0);
+  lf += (sel == 2) ? l * sf : T(0);
 }
 
 // Reverse of the x-slope computation at (r, s) given the slope sensitivities;
@@ -920,24 +907,35 @@ GPO_HD void a2_cell(Geo g, T dt, int rk_step, const T* Hi, const T* fc, Fwd<T> w
     }
   }
   if (row_in) {
-    // face parts: the cell is the left side of x-face c and the right side of x-face c-1
+    // face parts: the cell is the left side of x-face c and the right side of
+    // x-face c-1; the reconstruction there is eta -/+ (K + dx f v_n)/(2g),
+    // u -/+ q0/2, v -/+ q1/2, with v mirrored at a wall's first ghost cell
     const bool left = c >= 1 && c <= g.nx + 1, right = c - 1 >= 1 && c - 1 <= g.nx + 1;
     T lq0 = 0, lq1 = 0, lK = 0;
-    if (left) {
-      lR0 += rv.xl[0][o];
-      lR1 += rv.xl[1][o];
-      lR2 += rv.xl[2][o];
-      lq0 += rv.xl[3][o];
-      lq1 += rv.xl[4][o];
-      lK += rv.xl[5][o];
-    }
-    if (right) {
-      lR0 += rv.xr[0][o - 1];
-      lR1 += rv.xr[1][o - 1];
-      lR2 += rv.xr[2][o - 1];
-      lq0 += rv.xr[3][o - 1];
-      lq1 += rv.xr[4][o - 1];
-      lK += rv.xr[5][o - 1];
+    if (left || right) {
+      const T g2 = T(2) * G, f = fcor(g, fc, r, c), nx_ = T(g.north_x), ny_ = T(g.north_y);
+      if (left) {
+        const T a = rv.xl[0][o], b = rv.xl[1][o], d = rv.xl[2][o];
+        const T sm = (g.bc_w == 1 && c == 1) ? T(-1) : T(1);
+        const T lvn = a * T(g.dx) * f / g2;
+        lR0 += a;
+        lR1 += nx_ * lvn + b;
+        lR2 += sm * ny_ * lvn + d;
+        lq0 += T(0.5) * b;
+        lq1 += T(0.5) * d;
+        lK += a / g2;
+      }
+      if (right) {
+        const T a = rv.xr[0][o - 1], b = rv.xr[1][o - 1], d = rv.xr[2][o - 1];
+        const T sp = (g.bc_e == 1 && c == g.nx + 2) ? T(-1) : T(1);
+        const T lvn = -a * T(g.dx) * f / g2;
+        lR0 += a;
+        lR1 += nx_ * lvn + b;
+        lR2 += sp * ny_ * lvn + d;
+        lq0 -= T(0.5) * b;
+        lq1 -= T(0.5) * d;
+        lK -= a / g2;
+      }
     }
     if ((left || right) && !(lq0 == T(0) && lq1 == T(0) && lK == T(0))) {
       T e3[3] = {0, 0, 0}, u3[3] = {0, 0, 0}, v3[3] = {0, 0, 0};
@@ -948,23 +946,34 @@ GPO_HD void a2_cell(Geo g, T dt, int rk_step, const T* Hi, const T* fc, Fwd<T> w
     }
   }
   if (col_in) {
+    // y-faces r (cell below) and r-1 (cell above): eta -/+ (L - dy f u_e)/(2g),
+    // v -/+ q1/2, u -/+ q0/2; stored as (eta, v, u)
     const bool low = r >= 1 && r <= g.ny + 1, high = r - 1 >= 1 && r - 1 <= g.ny + 1;
     T lq0 = 0, lq1 = 0, lL = 0;
-    if (low) {
-      lR0 += rv.yl[0][o];
-      lR1 += rv.yl[1][o];
-      lR2 += rv.yl[2][o];
-      lq0 += rv.yl[3][o];
-      lq1 += rv.yl[4][o];
-      lL += rv.yl[5][o];
-    }
-    if (high) {
-      lR0 += rv.yr[0][o - g.W];
-      lR1 += rv.yr[1][o - g.W];
-      lR2 += rv.yr[2][o - g.W];
-      lq0 += rv.yr[3][o - g.W];
-      lq1 += rv.yr[4][o - g.W];
-      lL += rv.yr[5][o - g.W];
+    if (low || high) {
+      const T g2 = T(2) * G, f = fcor(g, fc, r, c), ex = T(g.north_y), ey = T(-g.north_x);
+      if (low) {
+        const T a = rv.yl[0][o], bv = rv.yl[1][o], bu = rv.yl[2][o];
+        const T sm = (g.bc_s == 1 && r == 1) ? T(-1) : T(1);
+        const T lue = -a * T(g.dy) * f / g2;
+        lR0 += a;
+        lR1 += sm * ex * lue + bu;
+        lR2 += ey * lue + bv;
+        lq0 += T(0.5) * bu;
+        lq1 += T(0.5) * bv;
+        lL += a / g2;
+      }
+      if (high) {
+        const T a = rv.yr[0][o - g.W], bv = rv.yr[1][o - g.W], bu = rv.yr[2][o - g.W];
+        const T sp = (g.bc_n == 1 && r == g.ny + 2) ? T(-1) : T(1);
+        const T lue = a * T(g.dy) * f / g2;
+        lR0 += a;
+        lR1 += sp * ex * lue + bu;
+        lR2 += ey * lue + bv;
+        lq0 -= T(0.5) * bu;
+        lq1 -= T(0.5) * bv;
+        lL -= a / g2;
+      }
     }
     if ((low || high) && !(lq0 == T(0) && lq1 == T(0) && lL == T(0))) {
       T e3[3] = {0, 0, 0}, u3[3] = {0, 0, 0}, v3[3] = {0, 0, 0};
