@@ -71,11 +71,22 @@ __global__ __launch_bounds__(64 * kSlopeRows) void a1_faces(Geo g, T dt, int rk_
     a1_yface(g, dt, rk_step, Hi, fc, w, rv, out_e, lo_e, lo_u, lo_v, r, c, sy[0][ty][tx], sy[1][ty][tx],
              sy[2][ty][tx], sy[0][ty + 1][tx], sy[1][ty + 1][tx], sy[2][ty + 1][tx]);
 }
+// A2: one wave per row segment of 64 cells, of which the inner 62 are stored;
+// the x-slope terms a cell sends to its west and east neighbours cross lanes.
 template <class T>
-__global__ void a2_cells(Geo g, T dt, int rk_step, const T* Hi, const T* fc, Fwd<T> w, Rev<T> rv, const T* out_e,
-                         const T* lo_e, const T* lo_u, const T* lo_v) {
-  a2_cell(g, dt, rk_step, Hi, fc, w, rv, out_e, lo_e, lo_u, lo_v, blockIdx.y * blockDim.y + threadIdx.y,
-          blockIdx.x * blockDim.x + threadIdx.x);
+__global__ __launch_bounds__(256) void a2_cells(Geo g, T dt, int rk_step, const T* Hi, const T* fc, Fwd<T> w,
+                                                Rev<T> rv, const T* out_e, const T* lo_e, const T* lo_u,
+                                                const T* lo_v) {
+  const int lane = threadIdx.x;
+  const int r = blockIdx.y * blockDim.y + threadIdx.y;
+  const int c = blockIdx.x * 62 - 1 + lane;
+  const CellParts<T> p = a2_parts(g, dt, rk_step, Hi, fc, w, rv, out_e, lo_e, lo_u, lo_v, r, c);
+  T xp_west[3], xm_east[3];
+  for (int i = 0; i < 3; ++i) {
+    xp_west[i] = __shfl_up(p.xp[i], 1, 64);
+    xm_east[i] = __shfl_down(p.xm[i], 1, 64);
+  }
+  if (lane >= 1 && lane <= 62 && c < g.nx + 4 && r < g.ny + 4) store_parts(g, rv, r, c, xp_west, p, xm_east);
 }
 template <class T>
 __global__ void a3_fold(Geo g, int rk_step, int accumulate, int lq_mode, const T* Hi, Fwd<T> w, Rev<T> rv,
@@ -269,7 +280,8 @@ int adjoint_stages(const Geo& g, T dt, int rk, const T* ie, const T* iu, const T
   hipLaunchKernelGGL(f1_convert<T>, cell_grid(g), kBlock, 0, s, g, ie, iu, iv, Hi, w, flag);
   const dim3 face_grid((g.nx + 4 + 63) / 64, (g.ny + 4 + kSlopeRows - 1) / kSlopeRows);
   hipLaunchKernelGGL(a1_faces<T>, face_grid, dim3(64, kSlopeRows), 0, s, g, dt, rk, Hi, fc, w, rv, out_e, le, lu, lv);
-  hipLaunchKernelGGL(a2_cells<T>, cell_grid(g), kBlock, 0, s, g, dt, rk, Hi, fc, w, rv, out_e, le, lu, lv);
+  const dim3 part_grid((g.nx + 4 + 61) / 62, (g.ny + 4 + 3) / 4);
+  hipLaunchKernelGGL(a2_cells<T>, part_grid, kBlock, 0, s, g, dt, rk, Hi, fc, w, rv, out_e, le, lu, lv);
   hipLaunchKernelGGL(a3_fold<T>, inner_grid(g), kBlock, 0, s, g, rk, accumulate, lq_mode, Hi, w, rv, ie, out_e, le,
                      lu, lv, lie, liu, liv, lqe, lqu, lqv);
   return hip_check(hipGetLastError(), "adjoint launch");

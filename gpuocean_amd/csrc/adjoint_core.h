@@ -458,24 +458,23 @@ GPO_HD void f4_cell(Geo g, S dt, int rk_step, const S* Hi, const S* fc, Fwd<S> w
 
 // Face sensitivities: for the reconstructed state on each side of a face,
 // those of (eta, first, second flux component), the eta one already zeroed
-// where the -H clamp was active; then the per-cell sums formed from them: every sensitivity of a converted cell's (eta, u, v) is own[] of that
-// cell plus xp[] of its west neighbour, xm[] of its east neighbour, yp[] of
-// its south neighbour and ym[] of its north neighbour.
+// where the -H clamp was active; then the per-cell sums formed from them:
+// every sensitivity of a converted cell's (eta, u, v) is px[] of that cell
+// (its own terms plus the x-slope terms of its west and east neighbours) plus
+// yp[] of its south neighbour and ym[] of its north neighbour.
 template <class T> struct Rev {
   T* xl[3];  // x-face s, side of cell s: (eta, u, v)
   T* xr[3];  // x-face s, side of cell s+1
   T* yl[3];  // y-face s, side of cell s: (eta, v, u)
   T* yr[3];  // y-face s, side of cell s+1
-  T* own[3];  // update/source reverse, the cell's face parts and the centre term of its own slopes
-  T* xm[3];   // x-slopes of cell s -> cell s-1
-  T* xp[3];   // x-slopes of cell s -> cell s+1
-  T* ym[3];   // y-slopes of row s -> row s-1
-  T* yp[3];   // y-slopes of row s -> row s+1
+  T* px[3];  // own terms of the cell plus its x-neighbours' x-slope terms
+  T* ym[3];  // y-slopes of row s -> row s-1
+  T* yp[3];  // y-slopes of row s -> row s+1
 };
 
 // Workspace arrays of (ny+4) x (nx+4) elements: the 9 converted/slope arrays
-// of Fwd, then the 12 face and 15 per-cell arrays of Rev.
-constexpr int kAdjWorkArrays = 9 + 12 + 15;
+// of Fwd, then the 12 face and 9 per-cell arrays of Rev.
+constexpr int kAdjWorkArrays = 9 + 12 + 9;
 
 template <class T> inline Rev<T> carve_rev(void* ws, int nx, int ny) {
   const size_t n = (size_t)(nx + 4) * (ny + 4);
@@ -486,11 +485,9 @@ template <class T> inline Rev<T> carve_rev(void* ws, int nx, int ny) {
     v.xr[i] = p + n * (3 + i);
     v.yl[i] = p + n * (6 + i);
     v.yr[i] = p + n * (9 + i);
-    v.own[i] = p + n * (12 + i);
-    v.xm[i] = p + n * (15 + i);
-    v.xp[i] = p + n * (18 + i);
-    v.ym[i] = p + n * (21 + i);
-    v.yp[i] = p + n * (24 + i);
+    v.px[i] = p + n * (12 + i);
+    v.ym[i] = p + n * (15 + i);
+    v.yp[i] = p + n * (18 + i);
   }
   return v;
 }
@@ -528,8 +525,9 @@ GPO_HD void flux_rev(const Geo& g, const T* Qm, const T* Qp, T H, const T* lF, T
   const T Nx = (ap * Fm0 - am * Fp0) + ap * am * dq;
   const T Ny = (ap * Fm1 - am * Fp1) + ap * am * dfx;
   const T Z = (ap * Fm2 - am * Fp2) + ap * am * dhv;
-  const T lNx = lF[0] / d, lNy = lF[1] / d, lZ = (T(1) - fb) * lF[2] / d;
-  T ld = -(lNx * Nx + lNy * Ny + lZ * Z) / d;
+  const T rd = T(1) / d;
+  const T lNx = lF[0] * rd, lNy = lF[1] * rd, lZ = (T(1) - fb) * lF[2] * rd;
+  T ld = -(lNx * Nx + lNy * Ny + lZ * Z) * rd;
   T lap = lNx * (Fm0 + am * dq) + lNy * (Fm1 + am * dfx) + lZ * (Fm2 + am * dhv) + ld;
   T lam = lNx * (-Fp0 + ap * dq) + lNy * (-Fp1 + ap * dfx) + lZ * (-Fp2 + ap * dhv) - ld;
   T lFm0 = lNx * ap - lNy * ap * am, lFp0 = -lNx * am + lNy * ap * am;
@@ -598,9 +596,8 @@ GPO_HD void a1_xface(const Geo& g, T dt, int rk_step, const T* Hi, const T* fc, 
   // lambda of the three x-fluxes at face c: L_k = -(F_c - F_{c-1})/dx - ...
   const CellAdj<T> am = out_adj(g, Hi, out_e, lo_e, lo_u, lo_v, r, c);
   const CellAdj<T> ap = out_adj(g, Hi, out_e, lo_e, lo_u, lo_v, r, c + 1);
-  const T dxx = T(g.dx);
-  const T lF[3] = {coef * ap.e / dxx - coef * am.e / dxx, coef * ap.u / dxx - coef * am.u / dxx,
-                   coef * ap.v / dxx - coef * am.v / dxx};
+  const T cd = coef / T(g.dx);
+  const T lF[3] = {cd * (ap.e - am.e), cd * (ap.u - am.u), cd * (ap.v - am.v)};
   const size_t m = o, p = o + 1;
   const T ebp = w.ce[p], ebm = w.ce[m], up = w.cu[p], um = w.cu[m];
   const T vp0 = w.cv[p], vm0 = w.cv[m];
@@ -637,9 +634,8 @@ GPO_HD void a1_yface(const Geo& g, T dt, int rk_step, const T* Hi, const T* fc, 
   const T G = T(g.g), g2 = T(2) * G;
   const CellAdj<T> am = out_adj(g, Hi, out_e, lo_e, lo_u, lo_v, r, c);
   const CellAdj<T> ap = out_adj(g, Hi, out_e, lo_e, lo_u, lo_v, r + 1, c);
-  const T dyy = T(g.dy);
-  const T lG[3] = {coef * ap.e / dyy - coef * am.e / dyy, coef * ap.u / dyy - coef * am.u / dyy,
-                   coef * ap.v / dyy - coef * am.v / dyy};
+  const T cd = coef / T(g.dy);
+  const T lG[3] = {cd * (ap.e - am.e), cd * (ap.u - am.u), cd * (ap.v - am.v)};
   const T lfl[3] = {lG[0], lG[2], lG[1]};  // G = (fl.x, fl.z, fl.y)
   const size_t m = o, p = o + g.W;
   const T ebp = w.ce[p], ebm = w.ce[m], vp = w.cv[p], vm = w.cv[m];
@@ -859,17 +855,25 @@ GPO_HD void yslope_rev(const Geo& g, const Fwd<T>& w, const T* Hi, const T* fc, 
 
 // A2: per converted cell (r, c): the reverse of its update and sources, its
 // face parts, and the reverse of its own x- and y-slopes, split into the part
-// for the cell itself and the parts for its four neighbours.  Written for
-// every cell of the (ny+4) x (nx+4) grid (zeros where nothing applies), so
-// that A3 gathers without bounds tests on the contributions.
+// for the cell itself and the parts for its four neighbours.  Zero outside
+// the (ny+4) x (nx+4) grid and wherever nothing applies.
+template <class T> struct CellParts {
+  T own[3], xm[3], xp[3], ym[3], yp[3];
+};
 template <class T>
-GPO_HD void a2_cell(Geo g, T dt, int rk_step, const T* Hi, const T* fc, Fwd<T> w, Rev<T> rv, const T* out_e,
-                    const T* lo_e, const T* lo_u, const T* lo_v, int r, int c) {
-  if (c >= g.nx + 4 || r >= g.ny + 4) return;
+GPO_HD CellParts<T> a2_parts(const Geo& g, T dt, int rk_step, const T* Hi, const T* fc, const Fwd<T>& w,
+                             const Rev<T>& rv, const T* out_e, const T* lo_e, const T* lo_u, const T* lo_v, int r,
+                             int c) {
+  CellParts<T> out;
+  for (int i = 0; i < 3; ++i) out.own[i] = out.xm[i] = out.xp[i] = out.ym[i] = out.yp[i] = T(0);
+  if (c < 0 || r < 0 || c >= g.nx + 4 || r >= g.ny + 4) return out;
   const size_t o = (size_t)r * g.W + c;
   const bool row_in = r >= 2 && r <= g.ny + 1, col_in = c >= 2 && c <= g.nx + 1;
   T lR0 = 0, lR1 = 0, lR2 = 0;
-  T xm[3] = {0, 0, 0}, xp[3] = {0, 0, 0}, ym[3] = {0, 0, 0}, yp[3] = {0, 0, 0};
+  T* xm = out.xm;
+  T* xp = out.xp;
+  T* ym = out.ym;
+  T* yp = out.yp;
   const T G = T(g.g);
   if (row_in && col_in) {
     // update + sources (CDKLM16_kernel.cu:869-1089), reversed
@@ -887,8 +891,8 @@ GPO_HD void a2_cell(Geo g, T dt, int rk_step, const T* Hi, const T* fc, Fwd<T> w
       const T RHyp = T(0.5) * (corner(g, Hi, r + 1, c) + corner(g, Hi, r + 1, c + 1));
       const T RHym = T(0.5) * (corner(g, Hi, r, c) + corner(g, Hi, r, c + 1));
       // bathymetry terms: eta_we = (eta_west + eta_east)/2 depends on R0 only
-      lR0 += G * (RHxp - RHxm) * (lst1 / T(g.dx));
-      lR0 += G * (RHyp - RHym) * (lst2 / T(g.dy));
+      lR0 += G * (RHxp - RHxm) * (lst1 * T(1.0 / g.dx));
+      lR0 += G * (RHyp - RHym) * (lst2 * T(1.0 / g.dy));
       // Coriolis terms (linear in hu, hv)
       const T f = fcor(g, fc, r, c);
       const T nx_ = T(g.north_x), ny_ = T(g.north_y), ex = ny_, ey = -nx_;
@@ -913,28 +917,29 @@ GPO_HD void a2_cell(Geo g, T dt, int rk_step, const T* Hi, const T* fc, Fwd<T> w
     const bool left = c >= 1 && c <= g.nx + 1, right = c - 1 >= 1 && c - 1 <= g.nx + 1;
     T lq0 = 0, lq1 = 0, lK = 0;
     if (left || right) {
-      const T g2 = T(2) * G, f = fcor(g, fc, r, c), nx_ = T(g.north_x), ny_ = T(g.north_y);
+      const T rg2 = T(0.5 / g.g), f = fcor(g, fc, r, c), nx_ = T(g.north_x), ny_ = T(g.north_y);
+      const T fdx = T(g.dx) * f * rg2;
       if (left) {
         const T a = rv.xl[0][o], b = rv.xl[1][o], d = rv.xl[2][o];
         const T sm = (g.bc_w == 1 && c == 1) ? T(-1) : T(1);
-        const T lvn = a * T(g.dx) * f / g2;
+        const T lvn = a * fdx;
         lR0 += a;
         lR1 += nx_ * lvn + b;
         lR2 += sm * ny_ * lvn + d;
         lq0 += T(0.5) * b;
         lq1 += T(0.5) * d;
-        lK += a / g2;
+        lK += a * rg2;
       }
       if (right) {
         const T a = rv.xr[0][o - 1], b = rv.xr[1][o - 1], d = rv.xr[2][o - 1];
         const T sp = (g.bc_e == 1 && c == g.nx + 2) ? T(-1) : T(1);
-        const T lvn = -a * T(g.dx) * f / g2;
+        const T lvn = -a * fdx;
         lR0 += a;
         lR1 += nx_ * lvn + b;
         lR2 += sp * ny_ * lvn + d;
         lq0 -= T(0.5) * b;
         lq1 -= T(0.5) * d;
-        lK -= a / g2;
+        lK -= a * rg2;
       }
     }
     if ((left || right) && !(lq0 == T(0) && lq1 == T(0) && lK == T(0))) {
@@ -951,28 +956,29 @@ GPO_HD void a2_cell(Geo g, T dt, int rk_step, const T* Hi, const T* fc, Fwd<T> w
     const bool low = r >= 1 && r <= g.ny + 1, high = r - 1 >= 1 && r - 1 <= g.ny + 1;
     T lq0 = 0, lq1 = 0, lL = 0;
     if (low || high) {
-      const T g2 = T(2) * G, f = fcor(g, fc, r, c), ex = T(g.north_y), ey = T(-g.north_x);
+      const T rg2 = T(0.5 / g.g), f = fcor(g, fc, r, c), ex = T(g.north_y), ey = T(-g.north_x);
+      const T fdy = T(g.dy) * f * rg2;
       if (low) {
         const T a = rv.yl[0][o], bv = rv.yl[1][o], bu = rv.yl[2][o];
         const T sm = (g.bc_s == 1 && r == 1) ? T(-1) : T(1);
-        const T lue = -a * T(g.dy) * f / g2;
+        const T lue = -a * fdy;
         lR0 += a;
         lR1 += sm * ex * lue + bu;
         lR2 += ey * lue + bv;
         lq0 += T(0.5) * bu;
         lq1 += T(0.5) * bv;
-        lL += a / g2;
+        lL += a * rg2;
       }
       if (high) {
         const T a = rv.yr[0][o - g.W], bv = rv.yr[1][o - g.W], bu = rv.yr[2][o - g.W];
         const T sp = (g.bc_n == 1 && r == g.ny + 2) ? T(-1) : T(1);
-        const T lue = a * T(g.dy) * f / g2;
+        const T lue = a * fdy;
         lR0 += a;
         lR1 += sp * ex * lue + bu;
         lR2 += ey * lue + bv;
         lq0 -= T(0.5) * bu;
         lq1 -= T(0.5) * bv;
-        lL -= a / g2;
+        lL -= a * rg2;
       }
     }
     if ((low || high) && !(lq0 == T(0) && lq1 == T(0) && lL == T(0))) {
@@ -983,14 +989,21 @@ GPO_HD void a2_cell(Geo g, T dt, int rk_step, const T* Hi, const T* fc, Fwd<T> w
       yp[0] = e3[2], yp[1] = u3[2], yp[2] = v3[2];
     }
   }
-  rv.own[0][o] = lR0;
-  rv.own[1][o] = lR1;
-  rv.own[2][o] = lR2;
+  out.own[0] = lR0;
+  out.own[1] = lR1;
+  out.own[2] = lR2;
+  return out;
+}
+
+// px of a cell from the parts of its west neighbour, itself and its east neighbour.
+template <class T>
+GPO_HD inline void store_parts(const Geo& g, const Rev<T>& rv, int r, int c, const T* xp_west, const CellParts<T>& self,
+                               const T* xm_east) {
+  const size_t o = (size_t)r * g.W + c;
   for (int i = 0; i < 3; ++i) {
-    rv.xm[i][o] = xm[i];
-    rv.xp[i][o] = xp[i];
-    rv.ym[i][o] = ym[i];
-    rv.yp[i][o] = yp[i];
+    rv.px[i][o] = self.own[i] + xp_west[i] + xm_east[i];
+    rv.ym[i][o] = self.ym[i];
+    rv.yp[i][o] = self.yp[i];
   }
 }
 
@@ -1002,17 +1015,7 @@ template <class T>
 GPO_HD void conv_rev(const Geo& g, const T* Hi, const Fwd<T>& w, const Rev<T>& rv, const T* in_e, int r, int c, T& oe,
                      T& ou, T& ov) {
   const size_t o = (size_t)r * g.W + c;
-  T l0 = rv.own[0][o], l1 = rv.own[1][o], l2 = rv.own[2][o];
-  if (c >= 1) {
-    l0 += rv.xp[0][o - 1];
-    l1 += rv.xp[1][o - 1];
-    l2 += rv.xp[2][o - 1];
-  }
-  if (c <= g.nx + 2) {
-    l0 += rv.xm[0][o + 1];
-    l1 += rv.xm[1][o + 1];
-    l2 += rv.xm[2][o + 1];
-  }
+  T l0 = rv.px[0][o], l1 = rv.px[1][o], l2 = rv.px[2][o];
   if (r >= 1) {
     l0 += rv.yp[0][o - g.W];
     l1 += rv.yp[1][o - g.W];

@@ -127,8 +127,11 @@ int gpo_host_substep_adj_f64(const gpo_cdklm16_params* p, double dt, int rk, con
     for (int c = 0; c < W; ++c) f2_cell(g, Hi, fc, w, r, c);
   for (int r = 0; r < H; ++r)
     for (int c = 0; c < W; ++c) a1_cell(g, dt, rk, Hi, fc, w, rv, out_e, le, lu, lv, r, c);
-  for (int r = 0; r < H; ++r)
-    for (int c = 0; c < W; ++c) a2_cell(g, dt, rk, Hi, fc, w, rv, out_e, le, lu, lv, r, c);
+  for (int r = 0; r < H; ++r) {
+    std::vector<CellParts<double>> row(W + 2);  // cells -1 .. W
+    for (int c = -1; c <= W; ++c) row[c + 1] = a2_parts(g, dt, rk, Hi, fc, w, rv, out_e, le, lu, lv, r, c);
+    for (int c = 0; c < W; ++c) store_parts(g, rv, r, c, row[c].xp, row[c + 1], row[c + 2].xm);
+  }
   for (int r = 2; r <= g.ny + 1; ++r)
     for (int c = 2; c <= g.nx + 1; ++c)
       a3_cell(g, rk, accumulate, 0, Hi, w, rv, ie, out_e, le, lu, lv, lie, liu, liv, lqe, lqu, lqv, r, c);
