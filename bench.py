@@ -77,7 +77,7 @@ def cpu_baseline_single(kw, workload, budget_s=15.0):
                       f"({el:.1f} s), wall and periodic ghosts in C"}
 
 
-def cpu_baseline_ensemble(members_kw, budget_s=15.0):
+def cpu_baseline_ensemble(members_kw, budget_s=15.0, steps_per_member=8):
     """FP32 oracle on the first members of the ensemble, one after another:
     RK2 step + SOAR + geostrophic balance per step (the normal field is drawn
     with numpy outside the timed calls)."""
@@ -95,7 +95,7 @@ def cpu_baseline_ensemble(members_kw, budget_s=15.0):
         dt = np.float32(ora.dt)
         W = soar_weights(np.float32(kw["dx"]), np.float32(kw["dy"]), np.float32(5e-3), np.float32(0.75 * kw["dx"]))
         xi = rng.standard_normal((ny + 8, nx)).astype(np.float32)  # periodic in x, walls in y
-        for _ in range(4):
+        for _ in range(steps_per_member):
             t0 = time.perf_counter()
             ora.run_steps(1, dt)
             coarse = O.soar(nx, ny, W, 1.0, False, True, xi)
@@ -109,7 +109,7 @@ def cpu_baseline_ensemble(members_kw, budget_s=15.0):
         if el > budget_s:
             break
     return {"value": total_cells / el, "unit": "member-cell-steps/s", "cores": O.num_threads(), "kind": "port",
-            "sample": f"first {used} of the {len(members_kw)} members, 4 stochastic RK2 steps each "
+            "sample": f"first {used} of the {len(members_kw)} members, {steps_per_member} stochastic RK2 steps each "
                       f"(step + SOAR + balance) of the FP32 oracle, one member after another ({el:.1f} s)"}
 
 
@@ -357,8 +357,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c1", "c0", "copy"],
-                    help="copy = HBM copy of 256 MiB, the calibration kernel for the traffic counters")
+    ap.add_argument("--workload", default="c2", choices=["c2", "c1", "c0", "c3", "copy"],
+                    help="c3 = the adjoint/forward timing alone; copy = HBM copy of 256 MiB, the calibration "
+                         "kernel for the traffic counters")
     ap.add_argument("--members", type=int, default=256)
     ap.add_argument("--graph-steps", type=int, default=10)
     ap.add_argument("--seq-steps", type=int, default=10)
@@ -383,6 +384,11 @@ def main():
         return
 
     import cases
+
+    if args.workload == "c3":
+        print(json.dumps({"adjoint_c3": run_adjoint(reps=3 if args.steps is None else max(1, args.steps))}),
+              flush=True)
+        return
 
     if args.workload == "c2":
         args.steps = 500 if args.steps is None else args.steps
