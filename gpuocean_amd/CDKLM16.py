@@ -487,16 +487,20 @@ class CDKLM16:
         self.t += np.float64(local_dt)
         self.num_iterations += 1
 
-    def _after_batched_steps(self, n, dt):
+    def _after_batched_steps(self, n, dt, ghosts_written=False):
         """Host bookkeeping after n batched steps of size dt: time, iteration
-        count, the model error's draw index and the ghost cells."""
+        count, the model error's draw index, the drifters' time and the ghost
+        cells (unless the batched steps wrote them after every step)."""
         for _ in range(n):
             self.t += np.float64(dt)
+            if self.drifters is not None:
+                self.drifter_t += dt   # as drifterStep accumulates it
         self.num_iterations += n
         if self.model_error is not None:
             self.model_error.draw += n
             self.model_error._last_scales = (1.0, 1.0)   # the batched model error: q0_scale 1, perturbation_scale 1
-        self._apply_bc(self.h0)
+        if not ghosts_written:
+            self._apply_bc(self.h0)
 
     def dataAssimilationStep(self, observation_time, otherSim=None, model_error_final_step=True, write_now=True,
                              courant_number=0.8):

@@ -34,10 +34,12 @@ def _vp(t):
 class BatchedEnsemble:
     """Members stepped together by the library's batched ensemble entry points.
 
-    Every member must use RK2 with wall or periodic boundaries and no wind,
-    pressure or drifters, and all members must agree on exact arithmetic,
-    land, coriolis texture, north angle and whether they carry SOAR model
-    error.  Each member keeps its own dt, seed and draw index."""
+    Every member must use RK2 with wall, periodic or flow-relaxation
+    boundaries (flow relaxation toward boundary data of a single time) and no
+    wind or pressure forcing; drifters may be attached if they carry no wind
+    drag and share the member's periodicity.  All members must agree on exact
+    arithmetic, land, coriolis texture, north angle and whether they carry
+    SOAR model error.  Each member keeps its own dt, seed and draw index."""
 
     def __init__(self, sims, chunk_rows=None, waves_per_simd=None, autotune=False):
         ok, why = BatchedEnsemble.eligible(sims)
@@ -69,12 +71,20 @@ class BatchedEnsemble:
             bc = s.boundary_conditions
             if int(s.rk_order) != 2:
                 return False, "RK order other than 2"
-            if any(int(c) not in (1, 2) for c in (bc.north, bc.east, bc.south, bc.west)):
-                return False, "boundaries other than wall and periodic"
+            if any(int(c) not in (1, 2, 3) for c in (bc.north, bc.east, bc.south, bc.west)):
+                return False, "boundaries other than wall, periodic and flow relaxation"
+            if s._needs_explicit_bc_each_substep() and len(s.bc_data.t) != 1:
+                return False, "flow-relaxation data that changes in time"
             if s._has_wind or s._has_pressure:
                 return False, "wind or pressure forcing"
             if s.hasDrifters:
-                return False, "drifters attached"
+                d = s.drifters
+                if not hasattr(d, "positions") or float(getattr(d, "wind_drift_factor", 0.0)) != 0.0:
+                    return False, "drifters without device positions, or with wind drag"
+                db = d.boundaryConditions
+                if (db.isPeriodicNorthSouth(), db.isPeriodicEastWest()) != \
+                        (bc.isPeriodicNorthSouth(), bc.isPeriodicEastWest()):
+                    return False, "drifters whose periodicity differs from their member's"
             if (s.model_error is None) != (s0.model_error is None):
                 return False, "some members carry model error and some do not"
             if s.device != s0.device or bool(s.exact_arithmetic) != bool(s0.exact_arithmetic):
@@ -110,7 +120,43 @@ class BatchedEnsemble:
                 m.g = float(np.float32(s.g))
                 m.f = float(np.float32(s.f)) if np.asarray(s.f).size == 1 else 0.0
                 m.beta = float(np.float32(s.coriolis_beta))
+            m.bc_each_step = int(self._bc_in_batch(s))
+            for k in range(4):
+                m.sponge[k] = int(s.sponge[k])
+            if s._needs_explicit_bc_each_substep():
+                s._update_bc_values()
+                f = s._frs
+                addr = lambda t: None if t is None else t.data_ptr()
+                m.frs_ns_cur, m.frs_ns_next = addr(f.get('ns_cur')), addr(f.get('ns_next'))
+                m.frs_ew_cur, m.frs_ew_next = addr(f.get('ew_cur')), addr(f.get('ew_next'))
+                m.frs_ns_nx = int(s.bc_data.north.h[0].shape[0])
+                m.frs_ew_ny = int(s.bc_data.east.h[0].shape[0])
+                # boundary data of one time: the weight is 1 after the first step (which the per-member path takes)
+                m.frs_t = 1.0
+            if s.hasDrifters:
+                d = s.drifters
+                m.drifters = 1
+                m.n_drifters = int(d.numDrifters)
+                m.positions, m.pos_pitch = d.positions.data_ptr(), 8
+                m.sensitivity = float(np.float32(d.sensitivity))
+                m.Bm, m.Bm_pitch = s.Bm.data_ptr(), s.Bm.pitch
         return arr
+
+    @staticmethod
+    def _bc_in_batch(s):
+        """Whether the batched step writes this member's ghost cells every
+        step (flow relaxation, or drifters that read them)."""
+        return s._needs_explicit_bc_each_substep() or s.hasDrifters
+
+    def _frs_first_step_pending(self):
+        """Whether a flow-relaxation member still has to take the step whose
+        boundary-data weight is not 1 (the first step of the data's time)."""
+        for s in self.sims:
+            if s._needs_explicit_bc_each_substep():
+                s._update_bc_values()
+                if s._bc_times.bc_t != np.float32(1.0):
+                    return True
+        return False
 
     def _create(self):
         self.destroy()
@@ -173,19 +219,33 @@ class BatchedEnsemble:
         if rest:
             L.check(L.lib().gpo_ensemble_step(self.handle, rest, self._sh()), "gpo_ensemble_step")
 
-    def step(self, n_steps, graph_steps=0):
+    def step(self, n_steps, graph_steps=0, initial_bc=True):
         """n_steps steps of every member, each with its own dt; then the host
         bookkeeping of the per-member path (time, iteration count, draw index,
-        ghost cells)."""
+        drifter time, ghost cells).  initial_bc: write the ghost cells of
+        members at t = 0 first, as CDKLM16.step does."""
         if n_steps <= 0:
             return
+        if initial_bc:
+            for s in self.sims:
+                if s.t == 0:
+                    s._update_bc_values()
+                    s._apply_bc(s.h0)
+        if self._frs_first_step_pending():
+            # the step with boundary-data weight below 1 goes through each member's own path
+            for s in self.sims:
+                s.step_dts([np.float32(s.dt)])
+            n_steps -= 1
+            self._draws = None   # the members' draw indices moved on the host
+            if n_steps <= 0:
+                return
         self._sync_in()
         self._refresh()
         self.launch(n_steps, graph_steps)
         self._draws = [d + n_steps for d in self._draws] if self.has_noise else self._draws
         self._sync_out()
         for s, dt in zip(self.sims, self._dts):
-            s._after_batched_steps(int(n_steps), dt)
+            s._after_batched_steps(int(n_steps), dt, ghosts_written=self._bc_in_batch(s))
 
     def destroy(self):
         if self.handle is not None:
@@ -334,7 +394,6 @@ class OceanModelEnsemble:
                                                 gpu_stream=p.gpu_stream)
                 drifters.setDrifterPositions(drifter_positions)
                 p.attachDrifters(drifters)
-        self.batched = self.batched and len(drifter_positions) == 0
 
     def cleanUp(self):
         if self._ensemble is not None:
@@ -377,7 +436,7 @@ class OceanModelEnsemble:
             if p.t == 0:
                 p._update_bc_values()
                 p._apply_bc(p.h0)
-        ens.step(k_common, graph_steps=self.graph_steps)
+        ens.step(k_common, graph_steps=self.graph_steps, initial_bc=False)
         for p, seq in zip(self.particles, seqs):
             if len(seq) > k_common:
                 p.step_dts(seq[k_common:])

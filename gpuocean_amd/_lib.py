@@ -52,6 +52,11 @@ class Member(ctypes.Structure):
         ("coarse", c_void_p), ("coarse_pitch", c_int32), ("seed", ctypes.c_uint64), ("stream_id", c_uint32), ("draw", c_uint32),
         ("soar_weights", c_float * 25), ("perturbation_scale", c_float),
         ("g", c_float), ("f", c_float), ("beta", c_float),
+        ("bc_each_step", c_int32), ("sponge", c_int32 * 4),
+        ("frs_ns_cur", c_void_p), ("frs_ns_next", c_void_p), ("frs_ns_nx", c_int32),
+        ("frs_ew_cur", c_void_p), ("frs_ew_next", c_void_p), ("frs_ew_ny", c_int32), ("frs_t", c_float),
+        ("drifters", c_int32), ("n_drifters", c_int32), ("positions", c_void_p), ("pos_pitch", c_int32),
+        ("sensitivity", c_float), ("Bm", c_void_p), ("Bm_pitch", c_int32),
     ]
 
 

@@ -7,11 +7,14 @@
 // (ensembles/OceanModelEnsemble.py:138-150), where each member is a separate
 // CDKLM16 object with its own JIT module, stream and blocking dt read-back.
 
+#include <algorithm>
 #include <cstdio>
 #include <new>
 #include <vector>
 
+#include "boundary_cells.h"
 #include "cdklm_common.h"
+#include "drift_cells.h"
 #include "noise_cells.h"
 #include "philox.h"
 
@@ -130,6 +133,72 @@ __global__ void advance_draws(uint32_t* draw, int n) {
   if (i < n) draw[i] = draw[i] + 1u;
 }
 
+// Ghost cells of one member, as gpo_boundary_conditions writes them: the
+// north/south pass (periodic rows, or the flow-relaxation sponge rows) and
+// the east/west pass, each over a list of cells of its own.
+struct BcMember {
+  int nx, ny, bn, be, bs, bw;
+  int sn, se, ss, sw;
+  int frs;            // some side is flow relaxation: ghost cells after substep 0 too
+  int ns_rows_lo, ns_rows_hi;  // north/south pass: the first lo and the last hi rows (all columns)
+  int ew_cols_lo, ew_cols_hi;  // east/west pass: the first lo and the last hi columns (all rows)
+  float *h0e, *h0u, *h0v, *h1e, *h1u, *h1v;
+  int pitch;
+  const float *ns_cur, *ns_next, *ew_cur, *ew_next;
+  int ns_nx, ew_ny;
+  float frs_t;
+};
+
+// pass 0: north/south, pass 1: east/west; target 0: h0, 1: h1 (members with
+// flow relaxation only).  Block row y is member y.
+__global__ void __launch_bounds__(256) bc_batched(const BcMember* __restrict__ members, int pass, int target) {
+  const BcMember& b = members[blockIdx.y];
+  if (target == 1 && !b.frs) return;
+  float* h = target ? b.h1e : b.h0e;
+  float* u = target ? b.h1u : b.h0u;
+  float* v = target ? b.h1v : b.h0v;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pass == 0) {
+    const int w = b.nx + 4, rows = b.ns_rows_lo + b.ns_rows_hi;
+    if (k >= rows * w) return;
+    const int r = k / w, ti = k - r * w;
+    const int tj = r < b.ns_rows_lo ? r : (b.ny + 4 - b.ns_rows_hi) + (r - b.ns_rows_lo);
+    if (b.bn == 2)
+      cells::periodic_ns_cell(b.nx, b.ny, h, u, v, b.pitch, ti, tj < 2 ? tj : tj - b.ny);
+    else
+      cells::frs_ns_cell(b.bn, b.bs, b.nx, b.ny, b.sn, b.ss, h, u, v, b.pitch, b.ns_cur, b.ns_next, b.ns_nx, b.frs_t,
+                         ti, tj);
+  } else {
+    const int hgt = b.ny + 4, cols = b.ew_cols_lo + b.ew_cols_hi;
+    if (k >= cols * hgt) return;
+    const int c = k / hgt, tj = k - c * hgt;
+    const int ti = c < b.ew_cols_lo ? c : (b.nx + 4 - b.ew_cols_hi) + (c - b.ew_cols_lo);
+    if (b.be == 2)
+      cells::periodic_ew_cell(b.nx, b.ny, h, u, v, b.pitch, tj, ti < 2 ? ti : ti - b.nx);
+    else
+      cells::frs_ew_cell(b.be, b.bw, b.nx, b.ny, b.se, b.sw, h, u, v, b.pitch, b.ew_cur, b.ew_next, b.ew_ny,
+                         b.frs_t, ti, tj);
+  }
+}
+
+// Drifters of one member; the time step is the member's current one.
+struct DriftMember {
+  int nx, ny, per_ns, per_ew, rows, pos_pitch, pitch, bm_pitch, arg;
+  float dx, dy, sensitivity;
+  const float *eta, *hu, *hv, *Bm;
+  float* pos;
+};
+
+__global__ void __launch_bounds__(64) drift_batched(const DriftMember* __restrict__ members,
+                                                    const StepArgs* __restrict__ args) {
+  const DriftMember& d = members[blockIdx.y];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.rows) return;
+  cells::drift_one(d.nx, d.ny, d.dx, d.dy, args[d.arg].dt, 2, 2, d.eta, d.hu, d.hv, d.pitch, d.Bm, d.bm_pitch,
+                   d.per_ns, d.per_ew, cells::drift_row(d.pos, d.pos_pitch, i), d.sensitivity, nullptr, nullptr,
+                   nullptr, nullptr, 0, 0, 0.0f, 0.0f);
+}
+
 int check(hipError_t e, const char* what) {
   if (e != hipSuccess) {
     char buf[256];
@@ -156,6 +225,10 @@ struct Ensemble {
   int2* d_work = nullptr;
   NoiseMember* d_noise = nullptr;
   uint32_t* d_draw = nullptr;
+  BcMember* d_bc = nullptr;   // members whose ghost cells are written every step
+  int n_bc = 0, bc_blocks = 0, any_frs = 0;
+  DriftMember* d_drift = nullptr;
+  int n_drift = 0, drift_blocks = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int graph_steps = 0;
@@ -178,10 +251,39 @@ static int ensemble_launch_noise(Ensemble* e, hipStream_t s) {
   return check(hipGetLastError(), "ensemble noise launch");
 }
 
+static int ensemble_launch_bc(Ensemble* e, int target, hipStream_t s) {
+  if (e->n_bc == 0 || (target == 1 && !e->any_frs)) return 0;
+  for (int pass = 0; pass < 2; ++pass)
+    hipLaunchKernelGGL(bc_batched, dim3(e->bc_blocks, e->n_bc), dim3(256), 0, s, e->d_bc, pass, target);
+  return check(hipGetLastError(), "ensemble boundary launch");
+}
+
+static int ensemble_launch_drift(Ensemble* e, hipStream_t s) {
+  if (e->n_drift == 0) return 0;
+  hipLaunchKernelGGL(drift_batched, dim3(e->drift_blocks, e->n_drift), dim3(64), 0, s, e->d_drift, e->d_args);
+  return check(hipGetLastError(), "ensemble drift launch");
+}
+
+// Part 0: substep 0 (+ flow-relaxation ghost cells of the stage); part 1:
+// substep 1; part 2: model error, ghost cells of the new state, drifters.
+static int ensemble_launch_part(Ensemble* e, int part, hipStream_t s) {
+  int rc = 0;
+  if (part == 0) {
+    rc = ensemble_launch_substep(e, 0, s);
+    if (!rc) rc = ensemble_launch_bc(e, 1, s);
+  } else if (part == 1) {
+    rc = ensemble_launch_substep(e, 1, s);
+  } else {
+    rc = ensemble_launch_noise(e, s);
+    if (!rc) rc = ensemble_launch_bc(e, 0, s);
+    if (!rc) rc = ensemble_launch_drift(e, s);
+  }
+  return rc;
+}
+
 static int ensemble_launch_step(Ensemble* e, hipStream_t s) {
-  int rc = ensemble_launch_substep(e, 0, s);
-  if (!rc) rc = ensemble_launch_substep(e, 1, s);
-  if (!rc) rc = ensemble_launch_noise(e, s);
+  int rc = 0;
+  for (int part = 0; part < 3 && !rc; ++part) rc = ensemble_launch_part(e, part, s);
   return rc;
 }
 
@@ -193,6 +295,8 @@ static void ensemble_free(Ensemble* e) {
   if (e->d_noise) (void)hipFree(e->d_noise);
   if (e->d_tiles) (void)hipFree(e->d_tiles);
   if (e->d_draw) (void)hipFree(e->d_draw);
+  if (e->d_bc) (void)hipFree(e->d_bc);
+  if (e->d_drift) (void)hipFree(e->d_drift);
   delete e;
 }
 
@@ -222,6 +326,10 @@ int gpo_ensemble_create(const gpo_member* members, int32_t n, int32_t chunk_rows
   std::vector<int2> work;
   std::vector<int2> tiles;
   std::vector<NoiseMember> noise;
+  std::vector<BcMember> bcs;
+  std::vector<DriftMember> drifts;
+  long long max_bc_cells = 0;
+  int max_drift_rows = 0;
   std::vector<uint32_t> draws((size_t)n);
   for (int m = 0; m < n; ++m) {
     const gpo_member& M = members[m];
@@ -235,12 +343,18 @@ int gpo_ensemble_create(const gpo_member* members, int32_t n, int32_t chunk_rows
       ensemble_free(e);
       return bad("gpo_ensemble_create: members differ in land, coriolis texture, angle or model error");
     }
+    int frs = 0;
     for (int side = 0; side < 4; ++side) {
       const int code = (M.boundary_conditions >> (8 * side)) & 0xFF;
-      if (code != 1 && code != 2) {
+      if (code < 1 || code > 3) {
         ensemble_free(e);
-        return bad("gpo_ensemble_create: only wall (1) and periodic (2) boundaries");
+        return bad("gpo_ensemble_create: only wall (1), periodic (2) and flow-relaxation (3) boundaries");
       }
+      frs |= code == 3;
+    }
+    if (frs && !M.bc_each_step) {
+      ensemble_free(e);
+      return bad("gpo_ensemble_create: flow-relaxation members need bc_each_step");
     }
     if (chunk_rows > 0) p.chunk_rows = chunk_rows;
     int rc = make_step_args(&p, M.dt, 0, M.h0_eta, M.h0_hu, M.h0_hv, nullptr, nullptr, nullptr, M.h1_eta, M.h1_hu,
@@ -260,6 +374,79 @@ int gpo_ensemble_create(const gpo_member* members, int32_t n, int32_t chunk_rows
       return bad("gpo_ensemble_create: member too large for the work table");
     }
     draws[m] = M.draw;
+    if (M.bc_each_step) {
+      BcMember b{};
+      b.nx = p.nx;
+      b.ny = p.ny;
+      b.bn = (M.boundary_conditions >> 24) & 0xFF;
+      b.bs = (M.boundary_conditions >> 16) & 0xFF;
+      b.be = (M.boundary_conditions >> 8) & 0xFF;
+      b.bw = M.boundary_conditions & 0xFF;
+      b.sn = M.sponge[0];
+      b.se = M.sponge[1];
+      b.ss = M.sponge[2];
+      b.sw = M.sponge[3];
+      b.frs = frs;
+      // rows and columns each pass writes (the pass of gpo_boundary_conditions
+      // runs periodic when N (E) is 2, else flow relaxation when N or S (E or W) is 3)
+      if (b.bn == 2) {
+        b.ns_rows_lo = 2, b.ns_rows_hi = 2;
+      } else if (b.bn == 3 || b.bs == 3) {
+        const int lo = b.bs == 3 ? std::max(0, b.ss) : 0, hi = b.bn == 3 ? std::max(0, b.sn) : 0;
+        if (lo + hi >= p.ny + 4) b.ns_rows_lo = p.ny + 4, b.ns_rows_hi = 0;
+        else b.ns_rows_lo = lo, b.ns_rows_hi = hi;
+        if ((b.bn == 3 || b.bs == 3) && (!M.frs_ns_cur || !M.frs_ns_next || M.frs_ns_nx < 1)) {
+          ensemble_free(e);
+          return bad("gpo_ensemble_create: flow-relaxation north/south data missing");
+        }
+      }
+      if (b.be == 2) {
+        b.ew_cols_lo = 2, b.ew_cols_hi = 2;
+      } else if (b.be == 3 || b.bw == 3) {
+        const int lo = b.bw == 3 ? std::max(0, b.sw) : 0, hi = b.be == 3 ? std::max(0, b.se) : 0;
+        if (lo + hi >= p.nx + 4) b.ew_cols_lo = p.nx + 4, b.ew_cols_hi = 0;
+        else b.ew_cols_lo = lo, b.ew_cols_hi = hi;
+        if (!M.frs_ew_cur || !M.frs_ew_next || M.frs_ew_ny < 1) {
+          ensemble_free(e);
+          return bad("gpo_ensemble_create: flow-relaxation east/west data missing");
+        }
+      }
+      b.h0e = M.h0_eta, b.h0u = M.h0_hu, b.h0v = M.h0_hv;
+      b.h1e = M.h1_eta, b.h1u = M.h1_hu, b.h1v = M.h1_hv;
+      b.pitch = M.pitch;
+      b.ns_cur = M.frs_ns_cur, b.ns_next = M.frs_ns_next, b.ns_nx = M.frs_ns_nx;
+      b.ew_cur = M.frs_ew_cur, b.ew_next = M.frs_ew_next, b.ew_ny = M.frs_ew_ny;
+      b.frs_t = M.frs_t;
+      bcs.push_back(b);
+      const long long cells = std::max((long long)(b.ns_rows_lo + b.ns_rows_hi) * (p.nx + 4),
+                                       (long long)(b.ew_cols_lo + b.ew_cols_hi) * (p.ny + 4));
+      max_bc_cells = std::max(max_bc_cells, cells);
+      e->any_frs |= frs;
+    }
+    if (M.drifters) {
+      if (M.n_drifters < 0 || !M.positions || !M.Bm || M.pos_pitch < 8) {
+        ensemble_free(e);
+        return bad("gpo_ensemble_create: drifter buffers missing");
+      }
+      DriftMember d{};
+      d.nx = p.nx;
+      d.ny = p.ny;
+      d.per_ns = ((M.boundary_conditions >> 24) & 0xFF) == 2 && ((M.boundary_conditions >> 16) & 0xFF) == 2;
+      d.per_ew = ((M.boundary_conditions >> 8) & 0xFF) == 2 && (M.boundary_conditions & 0xFF) == 2;
+      d.rows = M.n_drifters + 1;
+      d.pos_pitch = M.pos_pitch;
+      d.pitch = M.pitch;
+      d.bm_pitch = M.Bm_pitch;
+      d.arg = 2 * m;
+      d.dx = p.dx;
+      d.dy = p.dy;
+      d.sensitivity = M.sensitivity;
+      d.eta = M.h0_eta, d.hu = M.h0_hu, d.hv = M.h0_hv;
+      d.Bm = M.Bm;
+      d.pos = M.positions;
+      drifts.push_back(d);
+      max_drift_rows = std::max(max_drift_rows, d.rows);
+    }
     for (int sy = 0; sy < a.n_chunks; ++sy)
       for (int sx = 0; sx < a.n_strips; ++sx) work.push_back(make_int2(m, sx | (sy << 16)));
     if (e->has_noise) {
@@ -342,6 +529,25 @@ int gpo_ensemble_create(const gpo_member* members, int32_t n, int32_t chunk_rows
     rc = check(hipMemcpyAsync(e->d_tiles, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice, s),
                "gpo_ensemble_create copy");
   e->n_tiles = (int)tiles.size();
+  e->n_bc = (int)bcs.size();
+  e->bc_blocks = (int)((max_bc_cells + 255) / 256);
+  e->n_drift = (int)drifts.size();
+  e->drift_blocks = (max_drift_rows + 63) / 64;
+  if (!rc && e->n_bc > 0 && e->bc_blocks > 0) {
+    rc = check(hipMalloc(&e->d_bc, sizeof(BcMember) * bcs.size()), "gpo_ensemble_create hipMalloc");
+    if (!rc)
+      rc = check(hipMemcpyAsync(e->d_bc, bcs.data(), sizeof(BcMember) * bcs.size(), hipMemcpyHostToDevice, s),
+                 "gpo_ensemble_create copy");
+  } else {
+    e->n_bc = 0;  // nothing to write (walls only)
+  }
+  if (!rc && e->n_drift > 0) {
+    rc = check(hipMalloc(&e->d_drift, sizeof(DriftMember) * drifts.size()), "gpo_ensemble_create hipMalloc");
+    if (!rc)
+      rc = check(hipMemcpyAsync(e->d_drift, drifts.data(), sizeof(DriftMember) * drifts.size(),
+                                hipMemcpyHostToDevice, s),
+                 "gpo_ensemble_create copy");
+  }
   if (!rc) rc = check(hipStreamSynchronize(s), "gpo_ensemble_create sync");
   if (rc) {
     ensemble_free(e);
@@ -372,7 +578,7 @@ int gpo_ensemble_step(gpo_ensemble h, int32_t n_steps, void* stream) {
 int gpo_ensemble_part(gpo_ensemble h, int32_t part, void* stream) {
   Ensemble* e = (Ensemble*)h;
   if (!e || part < 0 || part > 2) return bad("gpo_ensemble_part: bad arguments");
-  return part < 2 ? ensemble_launch_substep(e, part, (hipStream_t)stream) : ensemble_launch_noise(e, (hipStream_t)stream);
+  return ensemble_launch_part(e, part, (hipStream_t)stream);
 }
 
 int gpo_ensemble_capture(gpo_ensemble h, int32_t steps_per_graph, void* stream) {

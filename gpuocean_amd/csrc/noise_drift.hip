@@ -6,6 +6,7 @@
 #include <cstdio>
 
 #include "cdklm_common.h"
+#include "drift_cells.h"
 #include "noise_cells.h"
 #include "philox.h"
 
@@ -76,30 +77,6 @@ __global__ void geo_balance(int nx, int ny, float dx, float dy, float g, float f
                h_pitch, land_value, tj, ti);
 }
 
-__device__ float bilin1(const float* data, int dnx, int dny, float sx, float sy) {
-  sx = fminf(fmaxf(sx - 0.5f / (float)dnx, 0.0f), 1.0f);
-  sy = fminf(fmaxf(sy - 0.5f / (float)dny, 0.0f), 1.0f);
-  const float x = sx * (float)dnx, y = sy * (float)dny;
-  int x0 = (int)floorf(x), y0 = (int)floorf(y);
-  const float fx = x - (float)x0, fy = y - (float)y0;
-  x0 = min(x0, dnx - 1);
-  y0 = min(y0, dny - 1);
-  const int x1 = min(x0 + 1, dnx - 1), y1 = min(y0 + 1, dny - 1);
-  const float d00 = data[y0 * dnx + x0], d01 = data[y0 * dnx + x1];
-  const float d10 = data[y1 * dnx + x0], d11 = data[y1 * dnx + x1];
-  return d00 + fx * (d01 - d00) + fy * (d10 - d00) + fx * fy * (d00 + d11 - d01 - d10);
-}
-
-__device__ __forceinline__ float water_velocity(const float* eta, const float* mom, const float* Hm, int pitch,
-                                                int hm_pitch, int cx, int cy) {
-  const float h = rowp(Hm, hm_pitch, cy)[cx] + rowp(eta, pitch, cy)[cx];
-  return rowp(mom, pitch, cy)[cx] / h;
-}
-
-// passiveDrifterKernel (gpu_kernels/driftKernels.cu:74-183): bilinear velocity
-// between cell centres, optional wind drag, forward Euler, periodic wrap.
-// Positions are (n_drifters + 1) x 2 floats with row pitch `dpitch` bytes
-// (the last row is the observation drifter).
 __global__ void drift(int nx, int ny, float dx, float dy, float dt, int x0ref, int y0ref, const float* eta,
                       const float* hu, const float* hv, int pitch, const float* Hm, int hm_pitch, int per_ns,
                       int per_ew, int n_drifters, float* pos, int dpitch, float sensitivity, const float* wx_cur,
@@ -107,54 +84,9 @@ __global__ void drift(int nx, int ny, float dx, float dy, float dt, int x0ref, i
                       float wind_t, float wind_drift_factor) {
   const int ti = blockIdx.x * blockDim.x + threadIdx.x;
   if (ti >= n_drifters + 1) return;
-  float* d = rowp(pos, dpitch, ti);
-  float px = d[0], py = d[1];
-  const float qx = px / dx, qy = py / dy;
-  const int cx = (int)(floorf(qx) + (float)x0ref);
-  const int cy = (int)(floorf(qy) + (float)y0ref);
-  const float frx = qx - floorf(qx), fry = qy - floorf(qy);
-  const int cx0 = frx < 0.5f ? cx - 1 : cx;
-  const float xf = frx < 0.5f ? frx + 0.5f : frx - 0.5f;
-  const int cy0 = fry < 0.5f ? cy - 1 : cy;
-  const float yf = fry < 0.5f ? fry + 0.5f : fry - 0.5f;
-  const int cx1 = cx0 + 1, cy1 = cy0 + 1;
-  const float u00 = water_velocity(eta, hu, Hm, pitch, hm_pitch, cx0, cy0);
-  const float u10 = water_velocity(eta, hu, Hm, pitch, hm_pitch, cx1, cy0);
-  const float u01 = water_velocity(eta, hu, Hm, pitch, hm_pitch, cx0, cy1);
-  const float u11 = water_velocity(eta, hu, Hm, pitch, hm_pitch, cx1, cy1);
-  const float v00 = water_velocity(eta, hv, Hm, pitch, hm_pitch, cx0, cy0);
-  const float v10 = water_velocity(eta, hv, Hm, pitch, hm_pitch, cx1, cy0);
-  const float v01 = water_velocity(eta, hv, Hm, pitch, hm_pitch, cx0, cy1);
-  const float v11 = water_velocity(eta, hv, Hm, pitch, hm_pitch, cx1, cy1);
-  const float uy0 = (1.0f - xf) * u00 + xf * u10;
-  const float uy1 = (1.0f - xf) * u01 + xf * u11;
-  const float vy0 = (1.0f - xf) * v00 + xf * v10;
-  const float vy1 = (1.0f - xf) * v01 + xf * v11;
-  float u = (1.0f - yf) * uy0 + yf * uy1;
-  float v = (1.0f - yf) * vy0 + yf * vy1;
-  if (wind_drift_factor != 0.0f && wx_cur) {
-    const float Hmc = rowp(Hm, hm_pitch, cy)[cx];
-    if (Hmc < 1e20f) {
-      const float sx = px / ((float)nx * dx), sy = py / ((float)ny * dy);
-      // The reference samples the y wind with the x wind's texture size
-      // (driftKernels.cu:156); both textures share one shape here.
-      const float wxc = bilin1(wx_cur, w_nx, w_ny, sx, sy), wxn = bilin1(wx_next, w_nx, w_ny, sx, sy);
-      const float wyc = bilin1(wy_cur, w_nx, w_ny, sx, sy), wyn = bilin1(wy_next, w_nx, w_ny, sx, sy);
-      const float wx = wind_t * wxn + (1.0f - wind_t) * wxc;
-      const float wy = wind_t * wyn + (1.0f - wind_t) * wyc;
-      u = u + (wx - u) * wind_drift_factor;
-      v = v + (wy - v) * wind_drift_factor;
-    }
-  }
-  px += sensitivity * u * dt;
-  py += sensitivity * v * dt;
-  const float Lx = (float)nx * dx, Ly = (float)ny * dy;
-  if (per_ew && px < 0.0f) px += Lx;
-  if (per_ew && px > Lx) px -= Lx;
-  if (per_ns && py < 0.0f) py += Ly;
-  if (per_ns && py > Ly) py -= Ly;
-  d[0] = px;
-  d[1] = py;
+  cells::drift_one(nx, ny, dx, dy, dt, x0ref, y0ref, eta, hu, hv, pitch, Hm, hm_pitch, per_ns, per_ew,
+                   cells::drift_row(pos, dpitch, ti), sensitivity, wx_cur, wx_next, wy_cur, wy_next, w_nx, w_ny,
+                   wind_t, wind_drift_factor);
 }
 
 int check(hipError_t e, const char* what) {

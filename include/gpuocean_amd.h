@@ -206,7 +206,7 @@ typedef struct {
   const float* Hi;
   int32_t Hi_pitch;
   const float* f_cells;            /* per-cell f (pitch = pitch) or NULL */
-  uint32_t boundary_conditions;    /* walls (1) and periodic (2) only */
+  uint32_t boundary_conditions;    /* wall (1), periodic (2), flow relaxation (3) */
   /* model error (noise != 0) */
   int32_t noise;
   float* xi;                       /* standard normal field, xi_rows x xi_cols */
@@ -219,19 +219,42 @@ typedef struct {
   float soar_weights[25];
   float perturbation_scale;
   float g, f, beta;                /* as passed to gpo_geostrophic_balance */
+  /* ghost cells (bc_each_step != 0): written after RK substep 0 (on h1, when
+   * a side is flow relaxation) and after each step and its model error (on
+   * h0), as gpo_boundary_conditions writes them; otherwise never */
+  int32_t bc_each_step;
+  int32_t sponge[4];               /* N, E, S, W sponge widths */
+  const float *frs_ns_cur, *frs_ns_next;  /* [2][frs_ns_nx][3] when N or S is 3 */
+  int32_t frs_ns_nx;
+  const float *frs_ew_cur, *frs_ew_next;  /* [frs_ew_ny][2][3] when E or W is 3 */
+  int32_t frs_ew_ny;
+  float frs_t;                     /* weight of next against cur */
+  /* passive drifters (drifters != 0): n_drifters + 1 rows of (x, y), moved
+   * after each step (after the ghost cells) with the member's dt, no wind */
+  int32_t drifters;
+  int32_t n_drifters;
+  float* positions;
+  int32_t pos_pitch;
+  float sensitivity;
+  const float* Bm;                 /* cell-centre depth, (ny+4) rows */
+  int32_t Bm_pitch;
 } gpo_member;
 
 typedef void* gpo_ensemble;
 
 /* Builds a batched ensemble over n RK2 members of any sizes: one persistent
- * launch per RK substep walks a table of (member, strip, row chunk) items,
- * followed, with model error, by one batched launch each of the normal field,
- * SOAR and geostrophic balance over all members.  Each member's draw index
- * lives on the device and advances once per step, so replays of a captured
- * graph draw fresh random numbers.  Results are bit for
- * bit those of the per-member sequence gpo_cdklm16_substep x2, gpo_random_normal,
- * gpo_soar, gpo_geostrophic_balance.  chunk_rows (0 = each member's own) and
- * waves_per_simd (0 = 4) set the work-item height and the persistent grid.
+ * launch per RK substep walks a table of (member, strip, row chunk) items;
+ * with model error, one fused launch (normal field, SOAR, geostrophic
+ * balance) covers every member; members with bc_each_step get their ghost
+ * cells from two batched launches (north/south, then east/west) after
+ * substep 0 (flow relaxation only) and after the model error; members with
+ * drifters are moved by one batched launch at the end of the step.  Each
+ * member's draw index lives on the device and advances once per step, so
+ * replays of a captured graph draw fresh random numbers.  Results are bit for
+ * bit those of the per-member sequence gpo_cdklm16_substep, (gpo_boundary_conditions),
+ * gpo_cdklm16_substep, gpo_random_normal + gpo_soar + gpo_geostrophic_balance,
+ * (gpo_boundary_conditions), (gpo_drift).  chunk_rows (0 = each member's own)
+ * and waves_per_simd (0 = 4) set the work-item height and the persistent grid.
  * Allocates the device tables; synchronises `stream`.
  * Replaces the member loop of OceanModelEnsemble.modelStep
  * (ensembles/OceanModelEnsemble.py:138-150, 331-343). */

@@ -245,3 +245,53 @@ def c3_drifters(nx, ny, dx, dy, n=200, seed=3):
     x = rng.uniform(1.0 * dx, (nx - 1.0) * dx, n)
     y = rng.uniform(1.0 * dy, (ny - 1.0) * dy, n)
     return np.stack([x, y], axis=1).astype(np.float32)
+
+
+def config_c4_member(nx, ny, seed, codes=(3, 3, 3, 1), sponge=10):
+    """Coastal member: dx=dy=1 km, f=1e-4; depth ramps linearly from 0 m at
+    x = 0.1 Lx to 300 m at the east edge, so the western 10 % (depth <= 0) is
+    land (masked corner depths); a seeded zonal jet (width and position
+    jittered by the seed) in geostrophic balance over the ramp; flow
+    relaxation toward that jet on the sides coded 3 (sponge cells wide),
+    boundary data of one time."""
+    rng = np.random.default_rng(seed)
+    dx = dy = 1000.0
+    g, f = 9.81, 1e-4
+    Lx, Ly = nx * dx, ny * dy
+    xc = (np.arange(nx + 5) - 2) * dx
+    Hrow = 300.0 * (xc - 0.1 * Lx) / (0.9 * Lx)
+    H = np.repeat(Hrow[None, :], ny + 5, axis=0).astype(np.float32)
+    H = np.ma.masked_where(H <= 0.0, H)
+    y = (np.arange(ny + 4) - 2 + 0.5) * dy
+    yc = Ly * (0.5 + 0.1 * rng.uniform(-1, 1))
+    w = Ly * (0.08 + 0.04 * rng.uniform())
+    eta_y = 0.2 * np.exp(-((y - yc) / w) ** 2)
+    eta0 = np.repeat(eta_y[:, None], nx + 4, axis=1).astype(np.float32)
+    Hf = np.ma.filled(H, 0.0)
+    Hm = np.maximum(0.25 * (Hf[:-1, :-1] + Hf[1:, :-1] + Hf[:-1, 1:] + Hf[1:, 1:]), 0.0)
+    wet = Hm > 1.0
+    eta0 = np.where(wet, eta0, 0.0).astype(np.float32)
+    deta_dy = np.gradient(eta_y, dy)
+    hu0 = np.where(wet, -(g / f) * (Hm + eta0) * deta_dy[:, None], 0.0).astype(np.float32)
+    hv0 = np.zeros_like(eta0)
+
+    def ns(row):
+        return Common.SingleBoundaryConditionData(h=[eta0[row].copy()], hu=[hu0[row].copy()], hv=[hv0[row].copy()])
+
+    def ew(col):
+        return Common.SingleBoundaryConditionData(h=[eta0[:, col].copy()], hu=[hu0[:, col].copy()],
+                                                  hv=[hv0[:, col].copy()])
+
+    data = Common.BoundaryConditionsData(north=ns(ny + 1), south=ns(2), east=ew(nx + 1), west=ew(2))
+    sp = {'north': sponge, 'east': sponge, 'south': sponge, 'west': sponge}
+    return dict(eta0=eta0, hu0=hu0, hv0=hv0, H=H, nx=nx, ny=ny, dx=dx, dy=dy, dt=0.0, g=g, f=f, r=0.0,
+                courant_number=0.7, boundary_conditions=Common.BoundaryConditions(*codes, spongeCells=sp),
+                boundary_conditions_data=data)
+
+
+def c4_drifter_positions(nx, ny, dx, dy, n, seed):
+    """n seeded drifters in the wet interior, away from the sponges."""
+    rng = np.random.default_rng(seed + 500)
+    x = rng.uniform(0.3 * nx * dx, 0.8 * nx * dx, n)
+    y = rng.uniform(0.25 * ny * dy, 0.75 * ny * dy, n)
+    return np.stack([x, y], axis=1).astype(np.float32)
