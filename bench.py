@@ -314,6 +314,76 @@ def run_copy(n_bytes=256 << 20, reps=5):
                       "GB_per_s_read_plus_write": 2 * n_bytes / (ms * 1e-3) / 1e9}), flush=True)
 
 
+def c4_member_sizes(n_members=128, seed=4048):
+    """Seeded coastal member sizes: nx = 16*round(U{256..768}/16), ny = 16*round(U{256..512}/16)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    nx = 16 * np.round(rng.integers(256, 769, n_members) / 16).astype(int)
+    ny = 16 * np.round(rng.integers(256, 513, n_members) / 16).astype(int)
+    return [(int(a), int(b)) for a, b in zip(nx, ny)]
+
+
+def run_coastal(args):
+    """Batched coastal ensemble: 128 members (nx 256..768, ny 256..512), land
+    in the western 10 %, flow relaxation (10 cells) on the other sides, SOAR
+    q0 = 5e-3, 1000 drifters per member; batched steps against the same
+    members stepped one after another."""
+    import numpy as np
+    import torch
+    import cases
+    from gpuocean_amd import BatchedEnsemble, CDKLM16, Common, GPUDrifterCollection
+
+    ctx = Common.DeviceContext(0)
+    t0 = time.perf_counter()
+    sims = []
+    for i, (nx, ny) in enumerate(c4_member_sizes(args.members)):
+        kw = cases.config_c4_member(nx, ny, 7000 + i)
+        sim = CDKLM16(ctx, **kw, local_particle_id=i)
+        sim.setSOARModelError(5e-3, xorwow_seed=2_000_003 + i)
+        d = GPUDrifterCollection(ctx, 1000, boundaryConditions=sim.boundary_conditions,
+                                 domain_size_x=nx * sim.dx, domain_size_y=ny * sim.dy, gpu_stream=sim.gpu_stream)
+        d.setDrifterPositions(cases.c4_drifter_positions(nx, ny, sim.dx, sim.dy, 1000, 7000 + i))
+        sim.attachDrifters(d)
+        sims.append(sim)
+    torch.cuda.synchronize()
+    cells = sum(int(s.nx) * int(s.ny) for s in sims)
+    log(f"[c4] {len(sims)} members, {cells / 1e6:.2f} M interior cells, built in {time.perf_counter() - t0:.1f} s")
+    ens = BatchedEnsemble(sims, autotune=not args.no_autotune, chunk_rows=args.chunk_rows or None,
+                          waves_per_simd=args.waves_per_simd or None)
+    G = args.graph_steps
+    ens.step(args.warmup, graph_steps=G)   # includes the first step, taken member by member
+    torch.cuda.synchronize()
+    t_wall0 = time.perf_counter()
+    ens.step(args.steps, graph_steps=G)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_wall0
+    value = cells * args.steps / wall
+    log(f"[c4] batched: {args.steps} steps in {wall * 1e3:.1f} ms -> {1e3 * wall / args.steps:.3f} ms/step, "
+        f"{value:.4g} member-cell-steps/s (chunk_rows={ens.chunk_rows}, waves_per_simd={ens.waves_per_simd})")
+    n_seq = args.seq_steps
+    torch.cuda.synchronize()
+    t_wall0 = time.perf_counter()
+    for s in sims:
+        for _ in range(n_seq):
+            s.step(np.float32(s.dt))
+    torch.cuda.synchronize()
+    wall_seq = time.perf_counter() - t_wall0
+    value_seq = cells * n_seq / wall_seq
+    log(f"[c4] sequential per-member path: {n_seq} steps of every member in {wall_seq:.2f} s -> "
+        f"{value_seq:.4g} member-cell-steps/s; batched speedup {value / value_seq:.2f}x")
+    for s in sims[:4]:
+        assert np.all(np.isfinite(s.download()[0])), "non-finite state"
+        assert np.all(np.isfinite(s.drifters.getDrifterPositions())), "non-finite drifter"
+    ens.destroy()
+    return {"value": value, "unit": "member-cell-steps/s", "steps": args.steps, "ms_per_step": 1e3 * wall / args.steps,
+            "sequential": {"value": value_seq, "steps": n_seq, "speedup_batched": value / value_seq},
+            "config": {"workload": "CDKLM16 batched coastal ensemble: land (western 10 %), flow relaxation 10 cells "
+                                   "N/E/S, wall W, SOAR q0=5e-3, 1000 drifters per member",
+                       "members": len(sims), "interior_cells": cells, "nx_range": [256, 768],
+                       "ny_range": [256, 512], "chunk_rows": ens.chunk_rows,
+                       "waves_per_simd": ens.waves_per_simd}}
+
+
 def run_adjoint(reps=3):
     """Adjoint/forward time on the differentiable 512x512 member: 400 RK2 steps,
     200 observation points, checkpoints every 20 steps; forward = FP32 run with
@@ -357,9 +427,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c1", "c0", "c3", "copy"],
-                    help="c3 = the adjoint/forward timing alone; copy = HBM copy of 256 MiB, the calibration "
-                         "kernel for the traffic counters")
+    ap.add_argument("--workload", default="c2", choices=["c2", "c1", "c0", "c3", "c4", "copy"],
+                    help="c3 = the adjoint/forward timing alone; c4 = the batched coastal ensemble (land, flow "
+                         "relaxation, drifters); copy = HBM copy of 256 MiB, the calibration kernel for the "
+                         "traffic counters")
     ap.add_argument("--members", type=int, default=256)
     ap.add_argument("--graph-steps", type=int, default=10)
     ap.add_argument("--seq-steps", type=int, default=10)
@@ -384,6 +455,14 @@ def main():
         return
 
     import cases
+
+    if args.workload == "c4":
+        args.members = 128 if args.members == 256 else args.members
+        args.steps = 300 if args.steps is None else args.steps
+        args.warmup = 10 if args.warmup is None else args.warmup
+        args.seq_steps = 3 if args.seq_steps == 10 else args.seq_steps
+        print(json.dumps({"coastal_c4": run_coastal(args)}), flush=True)
+        return
 
     if args.workload == "c3":
         print(json.dumps({"adjoint_c3": run_adjoint(reps=3 if args.steps is None else max(1, args.steps))}),

@@ -263,9 +263,9 @@ class BatchedEnsemble:
 class Autotuner:
     """Picks the work-item height (rows per wavefront) and the persistent grid
     size (waves per SIMD) of a batched ensemble by timing a few steps of each
-    candidate on the ensemble's own members.  The members' states and draw
-    indices are restored afterwards.  Results are cached per set of member
-    shapes."""
+    candidate on the ensemble's own members.  The members' states, drifter
+    positions and draw indices are restored afterwards.  Results are cached
+    per set of member shapes (and ghost-cell / drifter work)."""
 
     CHUNK_ROWS = (4, 8, 12, 16, 24, 32)
     WAVES_PER_SIMD = (2, 4, 8)
@@ -273,7 +273,8 @@ class Autotuner:
 
     @staticmethod
     def key(ens):
-        shapes = tuple(sorted((int(s.nx), int(s.ny)) for s in ens.sims))
+        shapes = tuple(sorted((int(s.nx), int(s.ny), bool(BatchedEnsemble._bc_in_batch(s)), bool(s.hasDrifters))
+                              for s in ens.sims))
         return (ens.exact, ens.has_noise, shapes)
 
     @classmethod
@@ -284,6 +285,7 @@ class Autotuner:
         ens._sync_in()
         with torch.cuda.stream(ens.stream):
             saved = [[a.buf.clone() for a in s.h0 + s.h1] for s in ens.sims]
+            saved_pos = [s.drifters.positions.clone() if s.hasDrifters else None for s in ens.sims]
         draws0 = [s.model_error.draw if ens.has_noise else 0 for s in ens.sims]
         best, best_t = (0, 0), float("inf")
         try:
@@ -305,9 +307,11 @@ class Autotuner:
                         best, best_t = (cr, w), ms
         finally:
             with torch.cuda.stream(ens.stream):
-                for s, bufs in zip(ens.sims, saved):
+                for s, bufs, pos in zip(ens.sims, saved, saved_pos):
                     for a, b in zip(s.h0 + s.h1, bufs):
                         a.buf.copy_(b)
+                    if pos is not None:
+                        s.drifters.positions.copy_(pos)
             ens.stream.synchronize()
             if ens.has_noise:
                 for s, d in zip(ens.sims, draws0):

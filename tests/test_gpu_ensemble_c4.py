@@ -94,3 +94,16 @@ def test_ocean_model_ensemble_coastal_batched_equals_sequential(gpu_ctx):
     assert ea._ensemble is not None
     _assert_same(ea.particles, eb.particles)
     assert np.array_equal(ea.observeParticles(pos[:5]), eb.observeParticles(pos[:5]))
+
+
+def test_autotuner_restores_coastal_state(gpu_ctx):
+    print("autotuner on coastal members with drifters", flush=True)
+    a = _members(gpu_ctx, (3, 3, 3, 1), True, 50)
+    b = _members(gpu_ctx, (3, 3, 3, 1), True, 50)
+    ens = BatchedEnsemble(a, autotune=True)
+    _assert_same(a, b)   # tuning left no trace, drifters included
+    ens.step(3)
+    for s in b:
+        for _ in range(3):
+            s.step(np.float32(s.dt))
+    _assert_same(a, b)
