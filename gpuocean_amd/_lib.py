@@ -60,6 +60,16 @@ class Member(ctypes.Structure):
     ]
 
 
+class AdjMember(ctypes.Structure):
+    """gpo_adj_member: one member of a batched differentiable ensemble."""
+    _fields_ = [
+        ("params", CDKLM16Params), ("dt", c_float), ("boundary_conditions", c_uint32),
+        ("Hi", c_void_p), ("Hi_pitch", c_int32), ("f_cells", c_void_p), ("pitch", c_int32),
+        ("plane", ctypes.c_int64), ("state", c_void_p), ("stage", c_void_p), ("ckpt", c_void_p),
+        ("qs", c_void_p), ("q1s", c_void_p), ("lam", c_void_p), ("workspace", c_void_p),
+    ]
+
+
 class GpoError(RuntimeError):
     pass
 
@@ -130,6 +140,13 @@ _SIGNATURES = [
      [ctypes.POINTER(CDKLM16Params), c_int32, c_float, ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p),
       ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p), c_int32, c_void_p, c_int32, c_void_p, c_uint32, c_void_p,
       c_void_p, c_void_p]),
+    ("gpo_adj_batch_create", c_int32,
+     [ctypes.POINTER(AdjMember), c_int32, c_int32, c_int32, ctypes.POINTER(c_void_p), c_void_p]),
+    ("gpo_adj_batch_forward", c_int32, [c_void_p, c_int32, c_void_p]),
+    ("gpo_adj_batch_checkpoint", c_int32, [c_void_p, c_int32, c_int32, c_void_p]),
+    ("gpo_adj_batch_trajectory", c_int32, [c_void_p, c_int32, c_void_p]),
+    ("gpo_adj_batch_sweep", c_int32, [c_void_p, c_int32, c_void_p, c_void_p]),
+    ("gpo_adj_batch_destroy", c_int32, [c_void_p]),
     ("gpo_balance_adj_f32", c_int32,
      [ctypes.POINTER(CDKLM16Params), c_float, c_float, c_float, c_void_p, c_int32, c_void_p, c_int32,
       c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_int32, c_void_p]),

@@ -14,7 +14,8 @@ boundaries, no friction, land or forcing) and gives
   * an FP64 build of the forward, tangent-linear and adjoint runs, used by the
     finite-difference and dot-product checks.
 
-``CDKLM16Misfit`` wraps the FP32 path as a ``torch.autograd.Function``.
+``CDKLM16Misfit`` wraps the FP32 path as a ``torch.autograd.Function``;
+``DifferentiableEnsemble`` runs the FP32 path for many members at once.
 """
 
 import ctypes
@@ -65,7 +66,7 @@ class DifferentiableCDKLM16:
     """Misfit and its gradient for n_steps RK2 steps of fixed dt of the given
     simulator's configuration (its bathymetry, coriolis, boundaries, dt)."""
 
-    def __init__(self, sim, positions, observed, n_steps, dt=None, checkpoint_every=20):
+    def __init__(self, sim, positions, observed, n_steps, dt=None, checkpoint_every=20, _workspace=True):
         p = sim.params
         bc = sim.boundary_conditions
         if int(sim.rk_order) != 2 or float(sim.r) != 0.0:
@@ -96,8 +97,8 @@ class DifferentiableCDKLM16:
         self.fc = sim.f_cells
         lib = L.lib()
         with torch.cuda.stream(self.stream):
-            self.ws32 = torch.empty(lib.gpo_adjoint_workspace_bytes(self.nx, self.ny, 4), dtype=torch.uint8,
-                                    device=self.dev)
+            self.ws32 = torch.empty(lib.gpo_adjoint_workspace_bytes(self.nx, self.ny, 4) if _workspace else 0,
+                                    dtype=torch.uint8, device=self.dev)
             self.flag = torch.zeros(1, dtype=torch.int32, device=self.dev)
             self.J = torch.zeros(1, dtype=torch.float64, device=self.dev)
         self._f64 = None
@@ -383,3 +384,135 @@ class CDKLM16Misfit(torch.autograd.Function):
     def backward(ctx, grad_J):
         g = ctx.model.backward(float(grad_J.item()))
         return g[0], g[1], g[2], None
+
+
+class DifferentiableEnsemble:
+    """The misfit and gradient of DifferentiableCDKLM16 for many members at
+    once: every member's forward steps, trajectory recomputation and reverse
+    sweep run as single launches over all members (gpo_adj_batch_*), each
+    member with its own grid size, dt, observations and checkpoints.  The
+    members share the number of steps and the checkpoint spacing."""
+
+    def __init__(self, sims, positions, observed, n_steps, checkpoint_every=20, dts=None):
+        self.models = [DifferentiableCDKLM16(s, p, o, n_steps, dt=None if dts is None else dts[i],
+                                             checkpoint_every=checkpoint_every, _workspace=False)
+                       for i, (s, p, o) in enumerate(zip(sims, positions, observed))]
+        self.n = len(self.models)
+        self.n_steps = int(n_steps)
+        self.k = max(1, int(checkpoint_every))
+        self.n_seg = max(1, (self.n_steps + self.k - 1) // self.k)
+        self.dev = sims[0].device
+        self.stream = torch.cuda.Stream(device=self.dev)
+        lib = L.lib()
+        table = (L.AdjMember * self.n)()
+        self._bufs = []
+        for i, m in enumerate(self.models):
+            P = _pitch_elems(m.nx + 4)
+            rows = m.ny + 4
+            # one allocation per member: state, stage, checkpoints, k+1 states, k stages, 3 lambda states
+            n_states = 2 + self.n_seg + (self.k + 1) + self.k + 3
+            with torch.cuda.stream(self.stream):
+                buf = torch.zeros((n_states, 3, rows, P), dtype=torch.float32, device=self.dev)
+                ws = torch.empty(lib.gpo_adjoint_workspace_bytes(m.nx, m.ny, 4), dtype=torch.uint8, device=self.dev)
+            self._bufs.append((buf, ws))
+            base = buf.data_ptr()
+            st = 3 * rows * P * 4
+            e = table[i]
+            e.params = m.params
+            e.dt = float(m.dt)
+            e.boundary_conditions = m.bc
+            e.Hi, e.Hi_pitch = m.Hi.data_ptr(), m.Hi.pitch
+            e.f_cells = None if m.fc is None else m.fc.data_ptr()
+            e.pitch = P * 4
+            e.plane = rows * P * 4
+            e.state = base
+            e.stage = base + st
+            e.ckpt = base + 2 * st
+            e.qs = base + (2 + self.n_seg) * st
+            e.q1s = base + (2 + self.n_seg + self.k + 1) * st
+            e.lam = base + (2 + self.n_seg + 2 * self.k + 1) * st
+            e.workspace = ws.data_ptr()
+        self._pitch = [_pitch_elems(m.nx + 4) for m in self.models]
+        h = ctypes.c_void_p()
+        self.stream.synchronize()
+        L.check(lib.gpo_adj_batch_create(table, self.n, self.k, self.n_seg, ctypes.byref(h), self._sh()),
+                "gpo_adj_batch_create")
+        self.handle = h
+        with torch.cuda.stream(self.stream):
+            self.J = torch.zeros(self.n, dtype=torch.float64, device=self.dev)
+            self.flag = torch.zeros(1, dtype=torch.int32, device=self.dev)
+
+    def _sh(self):
+        return ctypes.c_void_p(self.stream.cuda_stream)
+
+    def _state(self, i, idx):
+        """(eta, hu, hv) views of state idx of member i's buffer."""
+        buf = self._bufs[i][0]
+        m = self.models[i]
+        return [buf[idx, q, :, :m.nx + 4] for q in range(3)]
+
+    def _misfit(self, i, idx, lam_idx=None):
+        m = self.models[i]
+        buf = self._bufs[i][0]
+        P = self._pitch[i]
+        st = [buf[idx, q] for q in range(3)]
+        lam = [None] * 3 if lam_idx is None else [_vp(buf[lam_idx, q]) for q in range(3)]
+        L.check(L.lib().gpo_point_misfit_f32(
+            m.nx, m.ny, m.dx, m.dy, *[_vp(a) for a in st], P * 4, _vp(m.Hi), m.Hi.pitch, m.n_obs, _vp(m.pos),
+            _vp(m.obs), ctypes.c_void_p(self.J.data_ptr() + 8 * i), *lam, self._sh()), "gpo_point_misfit_f32")
+
+    def forward(self, states):
+        """states: one (eta0, hu0, hv0) triple of (ny+4) x (nx+4) device float32
+        tensors per member.  Returns the misfits J (numpy float64, one per
+        member), keeping a checkpoint every checkpoint_every steps."""
+        self.stream.wait_stream(torch.cuda.current_stream(self.dev))
+        lib = L.lib()
+        with torch.cuda.stream(self.stream):
+            for i, st in enumerate(states):
+                for a, b in zip(self._state(i, 0), st):
+                    a.copy_(b)
+        for sg in range(self.n_seg):
+            m = min(self.k, self.n_steps - sg * self.k)
+            L.check(lib.gpo_adj_batch_checkpoint(self.handle, sg, 0, self._sh()), "gpo_adj_batch_checkpoint")
+            L.check(lib.gpo_adj_batch_forward(self.handle, m, self._sh()), "gpo_adj_batch_forward")
+        with torch.cuda.stream(self.stream):
+            self.J.zero_()
+        for i in range(self.n):
+            self._misfit(i, 0)
+        self.stream.synchronize()
+        return self.J.cpu().numpy().copy()
+
+    def backward(self):
+        """dJ_i/d(eta0, hu0, hv0) of every member i of the last forward(), as
+        a list of three (ny+4) x (nx+4) float32 tensors per member."""
+        lib = L.lib()
+        lam_idx = 2 + self.n_seg + 2 * self.k + 1
+        with torch.cuda.stream(self.stream):
+            for buf, _ in self._bufs:
+                buf[lam_idx:lam_idx + 3].zero_()
+            self.J.zero_()
+        for i in range(self.n):
+            self._misfit(i, 0, lam_idx)
+        for sg in reversed(range(self.n_seg)):
+            m = min(self.k, self.n_steps - sg * self.k)
+            L.check(lib.gpo_adj_batch_checkpoint(self.handle, sg, 1, self._sh()), "gpo_adj_batch_checkpoint")
+            L.check(lib.gpo_adj_batch_trajectory(self.handle, m, self._sh()), "gpo_adj_batch_trajectory")
+            L.check(lib.gpo_adj_batch_sweep(self.handle, m, _vp(self.flag), self._sh()), "gpo_adj_batch_sweep")
+        self.stream.synchronize()
+        return [[a.clone() for a in self._state(i, lam_idx)] for i in range(self.n)]
+
+    def flag_value(self):
+        self.stream.synchronize()
+        return int(self.flag.item())
+
+    def destroy(self):
+        if getattr(self, "handle", None) is not None:
+            self.stream.synchronize()
+            L.lib().gpo_adj_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass

@@ -5,7 +5,10 @@
 
 #include <cmath>
 #include <cstdint>
+#include <algorithm>
 #include <cstdio>
+#include <new>
+#include <vector>
 
 #include "adjoint_core.h"
 #include "cdklm_common.h"
@@ -40,13 +43,13 @@ __global__ void f4_update(Geo g, S dt, int rk_step, const S* Hi, const S* fc, Fw
 // east and north faces.
 constexpr int kSlopeRows = 8;
 template <class T>
-__global__ __launch_bounds__(64 * kSlopeRows) void a1_faces(Geo g, T dt, int rk_step, const T* Hi, const T* fc,
-                                                            Fwd<T> w, Rev<T> rv, const T* out_e, const T* lo_e,
-                                                            const T* lo_u, const T* lo_v) {
+__device__ __forceinline__ void a1_faces_body(const Geo& g, T dt, int rk_step, const T* Hi, const T* fc,
+                                              const Fwd<T>& w, const Rev<T>& rv, const T* out_e, const T* lo_e,
+                                              const T* lo_u, const T* lo_v, int bx, int by) {
   __shared__ T sx[3][kSlopeRows][65];
   __shared__ T sy[3][kSlopeRows + 1][64];
   const int tx = threadIdx.x, ty = threadIdx.y;
-  const int c0 = blockIdx.x * 64, r0 = blockIdx.y * kSlopeRows;
+  const int c0 = bx * 64, r0 = by * kSlopeRows;
   const int c = c0 + tx, r = r0 + ty;
   T a = 0, b = 0, k = 0;
   if (has_xslope(g, r, c)) xslope(g, w, Hi, fc, r, c, a, b, k);
@@ -71,15 +74,21 @@ __global__ __launch_bounds__(64 * kSlopeRows) void a1_faces(Geo g, T dt, int rk_
     a1_yface(g, dt, rk_step, Hi, fc, w, rv, out_e, lo_e, lo_u, lo_v, r, c, sy[0][ty][tx], sy[1][ty][tx],
              sy[2][ty][tx], sy[0][ty + 1][tx], sy[1][ty + 1][tx], sy[2][ty + 1][tx]);
 }
+template <class T>
+__global__ __launch_bounds__(64 * kSlopeRows) void a1_faces(Geo g, T dt, int rk_step, const T* Hi, const T* fc,
+                                                            Fwd<T> w, Rev<T> rv, const T* out_e, const T* lo_e,
+                                                            const T* lo_u, const T* lo_v) {
+  a1_faces_body(g, dt, rk_step, Hi, fc, w, rv, out_e, lo_e, lo_u, lo_v, blockIdx.x, blockIdx.y);
+}
 // A2: one wave per row segment of 64 cells, of which the inner 62 are stored;
 // the x-slope terms a cell sends to its west and east neighbours cross lanes.
 template <class T>
-__global__ __launch_bounds__(256) void a2_cells(Geo g, T dt, int rk_step, const T* Hi, const T* fc, Fwd<T> w,
-                                                Rev<T> rv, const T* out_e, const T* lo_e, const T* lo_u,
-                                                const T* lo_v) {
+__device__ __forceinline__ void a2_cells_body(const Geo& g, T dt, int rk_step, const T* Hi, const T* fc,
+                                              const Fwd<T>& w, const Rev<T>& rv, const T* out_e, const T* lo_e,
+                                              const T* lo_u, const T* lo_v, int bx, int by) {
   const int lane = threadIdx.x;
-  const int r = blockIdx.y * blockDim.y + threadIdx.y;
-  const int c = blockIdx.x * 62 - 1 + lane;
+  const int r = by * blockDim.y + threadIdx.y;
+  const int c = bx * 62 - 1 + lane;
   const CellParts<T> p = a2_parts(g, dt, rk_step, Hi, fc, w, rv, out_e, lo_e, lo_u, lo_v, r, c);
   T xp_west[3], xm_east[3];
   for (int i = 0; i < 3; ++i) {
@@ -89,11 +98,75 @@ __global__ __launch_bounds__(256) void a2_cells(Geo g, T dt, int rk_step, const 
   if (lane >= 1 && lane <= 62 && c < g.nx + 4 && r < g.ny + 4) store_parts(g, rv, r, c, xp_west, p, xm_east);
 }
 template <class T>
+__global__ __launch_bounds__(256) void a2_cells(Geo g, T dt, int rk_step, const T* Hi, const T* fc, Fwd<T> w,
+                                                Rev<T> rv, const T* out_e, const T* lo_e, const T* lo_u,
+                                                const T* lo_v) {
+  a2_cells_body(g, dt, rk_step, Hi, fc, w, rv, out_e, lo_e, lo_u, lo_v, blockIdx.x, blockIdx.y);
+}
+template <class T>
 __global__ void a3_fold(Geo g, int rk_step, int accumulate, int lq_mode, const T* Hi, Fwd<T> w, Rev<T> rv,
                         const T* in_e, const T* out_e, const T* lo_e, const T* lo_u, const T* lo_v, T* li_e, T* li_u,
                         T* li_v, T* lq_e, T* lq_u, T* lq_v) {
   a3_cell(g, rk_step, accumulate, lq_mode, Hi, w, rv, in_e, out_e, lo_e, lo_u, lo_v, li_e, li_u, li_v, lq_e, lq_u,
           lq_v, blockIdx.y * blockDim.y + threadIdx.y + 2, blockIdx.x * blockDim.x + threadIdx.x + 2);
+}
+
+// ---- batched reverse substep: block row z is member z of a table
+// One member's reverse substep: its geometry, the stage input and output, the
+// sensitivities in and out, and its workspace.
+template <class T>
+struct AdjItem {
+  Geo g;
+  T dt;
+  int rk, accumulate, lq_mode;
+  const T *ie, *iu, *iv, *out_e, *le, *lu, *lv;
+  T *lie, *liu, *liv, *lqe, *lqu, *lqv;
+  const T *Hi, *fc;
+  Fwd<T> w;
+  Rev<T> rv;
+};
+
+template <class T>
+__global__ void f1_convert_b(const AdjItem<T>* __restrict__ tab, int* flag) {
+  const AdjItem<T>& it = tab[blockIdx.z];
+  f1_cell(it.g, it.ie, it.iu, it.iv, it.Hi, it.w, flag, blockIdx.y * blockDim.y + threadIdx.y,
+          blockIdx.x * blockDim.x + threadIdx.x);
+}
+template <class T>
+__global__ __launch_bounds__(64 * kSlopeRows) void a1_faces_b(const AdjItem<T>* __restrict__ tab) {
+  const AdjItem<T>& it = tab[blockIdx.z];
+  if ((int)blockIdx.x * 64 >= it.g.nx + 4 || (int)blockIdx.y * kSlopeRows >= it.g.ny + 4) return;
+  a1_faces_body(it.g, it.dt, it.rk, it.Hi, it.fc, it.w, it.rv, it.out_e, it.le, it.lu, it.lv, blockIdx.x,
+                blockIdx.y);
+}
+template <class T>
+__global__ __launch_bounds__(256) void a2_cells_b(const AdjItem<T>* __restrict__ tab) {
+  const AdjItem<T>& it = tab[blockIdx.z];
+  if ((int)blockIdx.x * 62 - 1 >= it.g.nx + 4 || (int)(blockIdx.y * blockDim.y) >= it.g.ny + 4) return;
+  a2_cells_body(it.g, it.dt, it.rk, it.Hi, it.fc, it.w, it.rv, it.out_e, it.le, it.lu, it.lv, blockIdx.x,
+                blockIdx.y);
+}
+template <class T>
+__global__ void a3_fold_b(const AdjItem<T>* __restrict__ tab) {
+  const AdjItem<T>& it = tab[blockIdx.z];
+  a3_cell(it.g, it.rk, it.accumulate, it.lq_mode, it.Hi, it.w, it.rv, it.ie, it.out_e, it.le, it.lu, it.lv, it.lie,
+          it.liu, it.liv, it.lqe, it.lqu, it.lqv, blockIdx.y * blockDim.y + threadIdx.y + 2,
+          blockIdx.x * blockDim.x + threadIdx.x + 2);
+}
+
+// Interior copy of one state per member: dst[m] = src[m] (3 planes each).
+struct CopyItem {
+  const float* src;
+  float* dst;
+  int nx, ny, sp;  // sp: row pitch in elements
+  size_t plane;    // elements between the eta, hu, hv planes
+};
+__global__ void copy_states_b(const CopyItem* __restrict__ tab) {
+  const CopyItem& it = tab[blockIdx.z];
+  const int c = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y * blockDim.y + threadIdx.y;
+  if (c >= it.nx + 4 || r >= it.ny + 4) return;
+  const size_t k = (size_t)r * it.sp + c;
+  for (int q = 0; q < 3; ++q) it.dst[q * it.plane + k] = it.src[q * it.plane + k];
 }
 
 // Dual <-> split value/tangent arrays for the tangent-linear entry point.
@@ -285,6 +358,48 @@ int adjoint_stages(const Geo& g, T dt, int rk, const T* ie, const T* iu, const T
   hipLaunchKernelGGL(a3_fold<T>, inner_grid(g), kBlock, 0, s, g, rk, accumulate, lq_mode, Hi, w, rv, ie, out_e, le,
                      lu, lv, lie, liu, liv, lqe, lqu, lqv);
   return hip_check(hipGetLastError(), "adjoint launch");
+}
+
+// ---- batched differentiable ensemble
+
+struct AdjBatch {
+  int n = 0, k = 0, n_seg = 0;
+  int max_w = 0, max_h = 0;  // largest (nx+4), (ny+4) over the members
+  int n_work = 0, n_blocks = 0;
+  int f_array = 0, angle0 = 1;      // shared by all members (kernel variant)
+  StepArgs* d_fwd = nullptr;         // 2n: forward state -> stage -> state
+  StepArgs* d_traj = nullptr;        // k x 2n: qs[j] -> q1s[j] -> qs[j+1]
+  int2* d_work = nullptr;            // (member, strip | chunk << 16)
+  AdjItem<float>* d_items = nullptr;  // (2j + rk) x n
+  CopyItem* d_copy = nullptr;        // save (n_seg x n), restore (n_seg x n), lambda parity (n)
+};
+
+void adj_batch_free(AdjBatch* b) {
+  for (void* p : {(void*)b->d_fwd, (void*)b->d_traj, (void*)b->d_work, (void*)b->d_items, (void*)b->d_copy})
+    if (p) (void)hipFree(p);
+  delete b;
+}
+
+int adj_batch_copy(AdjBatch* b, const CopyItem* tab, hipStream_t s) {
+  hipLaunchKernelGGL(copy_states_b, dim3((b->max_w + 63) / 64, (b->max_h + 3) / 4, b->n), kBlock, 0, s, tab);
+  return hip_check(hipGetLastError(), "batched state copy");
+}
+
+int adj_batch_steps(AdjBatch* b, const StepArgs* tab, int rk, hipStream_t s) {
+  const hipError_t e =
+      launch_cdklm_batched_exact(tab, b->d_work, b->n_work, rk, b->n_blocks, 0, b->f_array, b->angle0, s);
+  return e != hipSuccess ? hip_check(e, "batched substep launch") : 0;
+}
+
+int adj_batch_stage(AdjBatch* b, const AdjItem<float>* tab, int* flag, hipStream_t s) {
+  const int n = b->n;
+  hipLaunchKernelGGL(f1_convert_b<float>, dim3((b->max_w + 63) / 64, (b->max_h + 3) / 4, n), kBlock, 0, s, tab,
+                     flag);
+  hipLaunchKernelGGL(a1_faces_b<float>, dim3((b->max_w + 63) / 64, (b->max_h + kSlopeRows - 1) / kSlopeRows, n),
+                     dim3(64, kSlopeRows), 0, s, tab);
+  hipLaunchKernelGGL(a2_cells_b<float>, dim3((b->max_w + 61) / 62, (b->max_h + 3) / 4, n), kBlock, 0, s, tab);
+  hipLaunchKernelGGL(a3_fold_b<float>, dim3((b->max_w - 4 + 63) / 64, (b->max_h - 4 + 3) / 4, n), kBlock, 0, s, tab);
+  return hip_check(hipGetLastError(), "batched adjoint launch");
 }
 
 }  // namespace
@@ -524,6 +639,193 @@ int gpo_point_misfit_f64(int32_t nx, int32_t ny, double dx, double dy, const dou
   hipLaunchKernelGGL(misfit<double>, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, g, n, positions,
                      observed, eta, hu, hv, Hi, J, lam_eta, lam_hu, lam_hv);
   return hip_check(hipGetLastError(), "gpo_point_misfit_f64 launch");
+}
+
+// ---- batched differentiable ensemble
+
+int gpo_adj_batch_create(const gpo_adj_member* members, int32_t n, int32_t k, int32_t n_segments,
+                         gpo_adj_batch* out, void* stream) {
+  if (!members || n < 1 || n > 65535 || k < 1 || n_segments < 1 || !out)
+    return bad_arg("gpo_adj_batch_create: bad arguments");
+  hipStream_t s = (hipStream_t)stream;
+  AdjBatch* b = new (std::nothrow) AdjBatch();
+  if (!b) return bad_arg("gpo_adj_batch_create: out of host memory");
+  b->n = n, b->k = k, b->n_seg = n_segments;
+  std::vector<StepArgs> fwd(2 * (size_t)n), traj(2 * (size_t)n * k);
+  std::vector<AdjItem<float>> items(2 * (size_t)n * k);
+  std::vector<CopyItem> copies((2 * (size_t)n_segments + 1) * n);
+  std::vector<int2> work;
+  for (int m = 0; m < n; ++m) {
+    const gpo_adj_member& M = members[m];
+    gpo_cdklm16_params p = M.params;
+    Geo g;
+    int rc = make_geo(&p, M.boundary_conditions, M.pitch, M.Hi_pitch, M.f_cells ? M.pitch : 4, 4, &g);
+    if (!rc && (!M.state || !M.stage || !M.ckpt || !M.qs || !M.q1s || !M.lam || !M.workspace || !M.Hi ||
+                M.plane < (int64_t)M.pitch * (p.ny + 4) || !(M.dt > 0.0f)))
+      rc = bad_arg("gpo_adj_batch_create: member buffers, plane or dt");
+    if (m == 0) b->f_array = p.f_is_array, b->angle0 = p.angle_is_zero;
+    if (!rc && (p.f_is_array != b->f_array || p.angle_is_zero != b->angle0 || (p.f_is_array && !M.f_cells)))
+      rc = bad_arg("gpo_adj_batch_create: members differ in coriolis texture or north angle");
+    if (rc) {
+      adj_batch_free(b);
+      return rc;
+    }
+    if (p.chunk_rows == 0) p.chunk_rows = 16;
+    const size_t pl = (size_t)M.plane / 4, st = 3 * pl;  // elements per plane / per state
+    auto planes = [&](float* base, size_t i, float** e, float** u, float** v) {
+      *e = base + i * st, *u = base + i * st + pl, *v = base + i * st + 2 * pl;
+    };
+    float *se, *su, *sv, *ge, *gu, *gv;
+    planes(M.state, 0, &se, &su, &sv);
+    planes(M.stage, 0, &ge, &gu, &gv);
+    rc = make_step_args(&p, M.dt, 0, se, su, sv, nullptr, nullptr, nullptr, ge, gu, gv, M.pitch, M.Hi, M.Hi_pitch,
+                        M.f_cells, nullptr, M.boundary_conditions, &fwd[2 * m]);
+    if (!rc)
+      rc = make_step_args(&p, M.dt, 1, ge, gu, gv, se, su, sv, se, su, sv, M.pitch, M.Hi, M.Hi_pitch, M.f_cells,
+                          nullptr, M.boundary_conditions, &fwd[2 * m + 1]);
+    for (int j = 0; j < k && !rc; ++j) {
+      float *ae, *au, *av, *be, *bu, *bv, *ce, *cu, *cv;
+      planes(M.qs, j, &ae, &au, &av);
+      planes(M.q1s, j, &be, &bu, &bv);
+      planes(M.qs, j + 1, &ce, &cu, &cv);
+      StepArgs* t = &traj[(size_t)j * 2 * n];
+      rc = make_step_args(&p, M.dt, 0, ae, au, av, nullptr, nullptr, nullptr, be, bu, bv, M.pitch, M.Hi, M.Hi_pitch,
+                          M.f_cells, nullptr, M.boundary_conditions, &t[2 * m]);
+      if (!rc)
+        rc = make_step_args(&p, M.dt, 1, be, bu, bv, ae, au, av, ce, cu, cv, M.pitch, M.Hi, M.Hi_pitch, M.f_cells,
+                            nullptr, M.boundary_conditions, &t[2 * m + 1]);
+      // lambda buffers: buf[0] = lam, buf[1] = lam + 1 state, stage lambda = lam + 2 states
+      float* buf[2] = {M.lam, M.lam + st};
+      float* l1 = M.lam + 2 * st;
+      const Fwd<float> w = carve<float>(M.workspace, p.nx, p.ny);
+      const Rev<float> rv = carve_rev<float>(M.workspace, p.nx, p.ny);
+      for (int rk = 0; rk < 2; ++rk) {
+        AdjItem<float>& it = items[((size_t)2 * j + rk) * n + m];
+        it.g = g;
+        it.dt = M.dt;
+        it.rk = rk;
+        it.Hi = M.Hi;
+        it.fc = M.f_cells;
+        it.w = w;
+        it.rv = rv;
+        float* lo = rk ? buf[(j + 1) & 1] : l1;
+        float* li = rk ? l1 : buf[j & 1];
+        it.le = lo, it.lu = lo + pl, it.lv = lo + 2 * pl;
+        it.lie = li, it.liu = li + pl, it.liv = li + 2 * pl;
+        if (rk) {
+          it.ie = be, it.iu = bu, it.iv = bv, it.out_e = ce;
+          float* lq = buf[j & 1];
+          it.lqe = lq, it.lqu = lq + pl, it.lqv = lq + 2 * pl;
+          it.accumulate = 0, it.lq_mode = 1;
+        } else {
+          it.ie = ae, it.iu = au, it.iv = av, it.out_e = be;
+          it.lqe = it.lqu = it.lqv = nullptr;
+          it.accumulate = 1, it.lq_mode = 0;
+        }
+      }
+    }
+    if (rc) {
+      adj_batch_free(b);
+      return rc;
+    }
+    for (int sg = 0; sg < n_segments; ++sg) {
+      copies[(size_t)sg * n + m] = CopyItem{M.state, M.ckpt + sg * st, p.nx, p.ny, g.sp, pl};
+      copies[((size_t)n_segments + sg) * n + m] = CopyItem{M.ckpt + sg * st, M.qs, p.nx, p.ny, g.sp, pl};
+    }
+    copies[(size_t)2 * n_segments * n + m] = CopyItem{M.lam, M.lam + st, p.nx, p.ny, g.sp, pl};
+    const StepArgs& a = fwd[2 * m];
+    if (a.n_strips > 0xFFFF || a.n_chunks > 0x7FFF) {
+      adj_batch_free(b);
+      return bad_arg("gpo_adj_batch_create: member too large for the work table");
+    }
+    for (int sy = 0; sy < a.n_chunks; ++sy)
+      for (int sx = 0; sx < a.n_strips; ++sx) work.push_back(make_int2(m, sx | (sy << 16)));
+    b->max_w = std::max(b->max_w, p.nx + 4);
+    b->max_h = std::max(b->max_h, p.ny + 4);
+  }
+  b->n_work = (int)work.size();
+  const long long waves = std::min<long long>((long long)persistent_waves(8), b->n_work);
+  b->n_blocks = (int)((waves + kWavesPerBlock - 1) / kWavesPerBlock);
+  int rc = hip_check(hipMalloc(&b->d_fwd, sizeof(StepArgs) * fwd.size()), "gpo_adj_batch_create hipMalloc");
+  if (!rc) rc = hip_check(hipMalloc(&b->d_traj, sizeof(StepArgs) * traj.size()), "gpo_adj_batch_create hipMalloc");
+  if (!rc) rc = hip_check(hipMalloc(&b->d_work, sizeof(int2) * work.size()), "gpo_adj_batch_create hipMalloc");
+  if (!rc)
+    rc = hip_check(hipMalloc(&b->d_items, sizeof(AdjItem<float>) * items.size()), "gpo_adj_batch_create hipMalloc");
+  if (!rc) rc = hip_check(hipMalloc(&b->d_copy, sizeof(CopyItem) * copies.size()), "gpo_adj_batch_create hipMalloc");
+  if (!rc)
+    rc = hip_check(hipMemcpyAsync(b->d_fwd, fwd.data(), sizeof(StepArgs) * fwd.size(), hipMemcpyHostToDevice, s),
+                   "gpo_adj_batch_create copy");
+  if (!rc)
+    rc = hip_check(hipMemcpyAsync(b->d_traj, traj.data(), sizeof(StepArgs) * traj.size(), hipMemcpyHostToDevice, s),
+                   "gpo_adj_batch_create copy");
+  if (!rc)
+    rc = hip_check(hipMemcpyAsync(b->d_work, work.data(), sizeof(int2) * work.size(), hipMemcpyHostToDevice, s),
+                   "gpo_adj_batch_create copy");
+  if (!rc)
+    rc = hip_check(hipMemcpyAsync(b->d_items, items.data(), sizeof(AdjItem<float>) * items.size(),
+                                  hipMemcpyHostToDevice, s),
+                   "gpo_adj_batch_create copy");
+  if (!rc)
+    rc = hip_check(hipMemcpyAsync(b->d_copy, copies.data(), sizeof(CopyItem) * copies.size(), hipMemcpyHostToDevice,
+                                  s),
+                   "gpo_adj_batch_create copy");
+  if (!rc) rc = hip_check(hipStreamSynchronize(s), "gpo_adj_batch_create sync");
+  if (rc) {
+    adj_batch_free(b);
+    return rc;
+  }
+  *out = (gpo_adj_batch)b;
+  return 0;
+}
+
+int gpo_adj_batch_forward(gpo_adj_batch h, int32_t n_steps, void* stream) {
+  AdjBatch* b = (AdjBatch*)h;
+  if (!b || n_steps < 0) return bad_arg("gpo_adj_batch_forward: bad arguments");
+  hipStream_t s = (hipStream_t)stream;
+  for (int i = 0; i < n_steps; ++i) {
+    int rc = adj_batch_steps(b, b->d_fwd, 0, s);
+    if (!rc) rc = adj_batch_steps(b, b->d_fwd, 1, s);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+int gpo_adj_batch_checkpoint(gpo_adj_batch h, int32_t segment, int32_t restore, void* stream) {
+  AdjBatch* b = (AdjBatch*)h;
+  if (!b || segment < 0 || segment >= b->n_seg) return bad_arg("gpo_adj_batch_checkpoint: bad segment");
+  const size_t base = (restore ? (size_t)b->n_seg + segment : (size_t)segment) * b->n;
+  return adj_batch_copy(b, b->d_copy + base, (hipStream_t)stream);
+}
+
+int gpo_adj_batch_trajectory(gpo_adj_batch h, int32_t m_steps, void* stream) {
+  AdjBatch* b = (AdjBatch*)h;
+  if (!b || m_steps < 0 || m_steps > b->k) return bad_arg("gpo_adj_batch_trajectory: 0 <= steps <= k");
+  hipStream_t s = (hipStream_t)stream;
+  for (int j = 0; j < m_steps; ++j) {
+    const StepArgs* t = b->d_traj + (size_t)j * 2 * b->n;
+    int rc = adj_batch_steps(b, t, 0, s);
+    if (!rc) rc = adj_batch_steps(b, t, 1, s);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+int gpo_adj_batch_sweep(gpo_adj_batch h, int32_t m_steps, int32_t* flag, void* stream) {
+  AdjBatch* b = (AdjBatch*)h;
+  if (!b || m_steps < 0 || m_steps > b->k || !flag) return bad_arg("gpo_adj_batch_sweep: 0 <= steps <= k, flag");
+  hipStream_t s = (hipStream_t)stream;
+  // lambda at Q^{j+1} lives in buffer (j+1) & 1: start from buffer m & 1
+  int rc = (m_steps & 1) ? adj_batch_copy(b, b->d_copy + (size_t)2 * b->n_seg * b->n, s) : 0;
+  for (int j = m_steps - 1; j >= 0 && !rc; --j) {
+    rc = adj_batch_stage(b, b->d_items + ((size_t)2 * j + 1) * b->n, flag, s);
+    if (!rc) rc = adj_batch_stage(b, b->d_items + ((size_t)2 * j) * b->n, flag, s);
+  }
+  return rc;
+}
+
+int gpo_adj_batch_destroy(gpo_adj_batch h) {
+  if (h) adj_batch_free((AdjBatch*)h);
+  return 0;
 }
 
 }  // extern "C"

@@ -362,6 +362,47 @@ int gpo_cdklm16_adjoint_steps_f32(const gpo_cdklm16_params* p, int32_t n_steps, 
                                   const float* Hi, int32_t Hi_pitch, const float* f_cells,
                                   uint32_t boundary_conditions, void* workspace, int32_t* flag, void* stream);
 
+/* ---- batched differentiable ensemble: many members' checkpointed forward,
+ * trajectory recomputation and reverse sweep, one launch per stage for all
+ * members (no counterpart in the reference).  Members: RK2, wall and periodic
+ * boundaries, no land or friction; exact build.  Every state is three planes
+ * (eta, hu, hv) of (ny+4) rows x pitch bytes, `plane` bytes apart; a
+ * buffer of several states holds them one after another (3 planes each). */
+typedef struct {
+  gpo_cdklm16_params params;
+  float dt;
+  uint32_t boundary_conditions;
+  const float* Hi;
+  int32_t Hi_pitch;
+  const float* f_cells;            /* per-cell f (pitch = pitch) or NULL */
+  int32_t pitch;
+  int64_t plane;
+  float* state;                    /* forward state, stepped in place */
+  float* stage;                    /* forward RK stage */
+  float* ckpt;                     /* n_segments states: segment starts */
+  float* qs;                       /* k+1 states of the segment being swept */
+  float* q1s;                      /* k RK stages of that segment */
+  float* lam;                      /* 3 states: lambda, then scratch */
+  void* workspace;                 /* gpo_adjoint_workspace_bytes(nx, ny, 4) */
+} gpo_adj_member;
+
+typedef void* gpo_adj_batch;
+
+/* Builds the argument tables of n members with segments of at most k steps
+ * and n_segments checkpoints.  Allocates device tables; synchronises. */
+int gpo_adj_batch_create(const gpo_adj_member* members, int32_t n, int32_t k, int32_t n_segments,
+                         gpo_adj_batch* out, void* stream);
+/* n_steps RK2 steps of every member's state (each with its own dt). */
+int gpo_adj_batch_forward(gpo_adj_batch b, int32_t n_steps, void* stream);
+/* restore = 0: state -> ckpt[segment]; restore = 1: ckpt[segment] -> qs[0]. */
+int gpo_adj_batch_checkpoint(gpo_adj_batch b, int32_t segment, int32_t restore, void* stream);
+/* qs[0] -> qs[1..m], with the stages in q1s[0..m-1] (m <= k). */
+int gpo_adj_batch_trajectory(gpo_adj_batch b, int32_t m, void* stream);
+/* lam: lambda at qs[m] in, (dqs[m]/dqs[0])^T lambda out (interior cells),
+ * over the trajectory of the last gpo_adj_batch_trajectory(b, m). */
+int gpo_adj_batch_sweep(gpo_adj_batch b, int32_t m, int32_t* flag, void* stream);
+int gpo_adj_batch_destroy(gpo_adj_batch b);
+
 /* Adjoint of the state dependence of gpo_geostrophic_balance for a given
  * coarse field: lambda_eta += -g/f d_y(coarse) lambda_hu + g/f d_x(coarse)
  * lambda_hv, in place (the noise is additive otherwise). */
