@@ -119,9 +119,9 @@ struct AdjItem {
   Geo g;
   T dt;
   int rk, accumulate, lq_mode;
-  const T *ie, *iu, *iv, *out_e, *le, *lu, *lv;
-  T *lie, *liu, *liv, *lqe, *lqu, *lqv;
-  const T *Hi, *fc;
+  const GPO_GLOBAL T *ie, *iu, *iv, *out_e, *le, *lu, *lv;
+  GPO_GLOBAL T *lie, *liu, *liv, *lqe, *lqu, *lqv;
+  const GPO_GLOBAL T *Hi, *fc;
   Fwd<T> w;
   Rev<T> rv;
 };
@@ -129,35 +129,37 @@ struct AdjItem {
 template <class T>
 __global__ void f1_convert_b(const AdjItem<T>* __restrict__ tab, int* flag) {
   const AdjItem<T>& it = tab[blockIdx.z];
-  f1_cell(it.g, it.ie, it.iu, it.iv, it.Hi, it.w, flag, blockIdx.y * blockDim.y + threadIdx.y,
+  f1_cell<T>(it.g, (const T*)it.ie, (const T*)it.iu, (const T*)it.iv, (const T*)it.Hi, it.w, flag,
+             blockIdx.y * blockDim.y + threadIdx.y,
           blockIdx.x * blockDim.x + threadIdx.x);
 }
 template <class T>
 __global__ __launch_bounds__(64 * kSlopeRows) void a1_faces_b(const AdjItem<T>* __restrict__ tab) {
   const AdjItem<T>& it = tab[blockIdx.z];
   if ((int)blockIdx.x * 64 >= it.g.nx + 4 || (int)blockIdx.y * kSlopeRows >= it.g.ny + 4) return;
-  a1_faces_body(it.g, it.dt, it.rk, it.Hi, it.fc, it.w, it.rv, it.out_e, it.le, it.lu, it.lv, blockIdx.x,
-                blockIdx.y);
+  a1_faces_body<T>(it.g, it.dt, it.rk, (const T*)it.Hi, (const T*)it.fc, it.w, it.rv, (const T*)it.out_e,
+                   (const T*)it.le, (const T*)it.lu, (const T*)it.lv, blockIdx.x, blockIdx.y);
 }
 template <class T>
 __global__ __launch_bounds__(256) void a2_cells_b(const AdjItem<T>* __restrict__ tab) {
   const AdjItem<T>& it = tab[blockIdx.z];
   if ((int)blockIdx.x * 62 - 1 >= it.g.nx + 4 || (int)(blockIdx.y * blockDim.y) >= it.g.ny + 4) return;
-  a2_cells_body(it.g, it.dt, it.rk, it.Hi, it.fc, it.w, it.rv, it.out_e, it.le, it.lu, it.lv, blockIdx.x,
-                blockIdx.y);
+  a2_cells_body<T>(it.g, it.dt, it.rk, (const T*)it.Hi, (const T*)it.fc, it.w, it.rv, (const T*)it.out_e,
+                   (const T*)it.le, (const T*)it.lu, (const T*)it.lv, blockIdx.x, blockIdx.y);
 }
 template <class T>
 __global__ void a3_fold_b(const AdjItem<T>* __restrict__ tab) {
   const AdjItem<T>& it = tab[blockIdx.z];
-  a3_cell(it.g, it.rk, it.accumulate, it.lq_mode, it.Hi, it.w, it.rv, it.ie, it.out_e, it.le, it.lu, it.lv, it.lie,
-          it.liu, it.liv, it.lqe, it.lqu, it.lqv, blockIdx.y * blockDim.y + threadIdx.y + 2,
-          blockIdx.x * blockDim.x + threadIdx.x + 2);
+  a3_cell<T>(it.g, it.rk, it.accumulate, it.lq_mode, (const T*)it.Hi, it.w, it.rv, (const T*)it.ie,
+             (const T*)it.out_e, (const T*)it.le, (const T*)it.lu, (const T*)it.lv, (T*)it.lie, (T*)it.liu,
+             (T*)it.liv, (T*)it.lqe, (T*)it.lqu, (T*)it.lqv, blockIdx.y * blockDim.y + threadIdx.y + 2,
+             blockIdx.x * blockDim.x + threadIdx.x + 2);
 }
 
 // Interior copy of one state per member: dst[m] = src[m] (3 planes each).
 struct CopyItem {
-  const float* src;
-  float* dst;
+  const GPO_GLOBAL float* src;
+  GPO_GLOBAL float* dst;
   int nx, ny, sp;  // sp: row pitch in elements
   size_t plane;    // elements between the eta, hu, hv planes
 };
@@ -704,21 +706,21 @@ int gpo_adj_batch_create(const gpo_adj_member* members, int32_t n, int32_t k, in
         it.g = g;
         it.dt = M.dt;
         it.rk = rk;
-        it.Hi = M.Hi;
-        it.fc = M.f_cells;
+        it.Hi = gptr(M.Hi);
+        it.fc = gptr(M.f_cells);
         it.w = w;
         it.rv = rv;
         float* lo = rk ? buf[(j + 1) & 1] : l1;
         float* li = rk ? l1 : buf[j & 1];
-        it.le = lo, it.lu = lo + pl, it.lv = lo + 2 * pl;
-        it.lie = li, it.liu = li + pl, it.liv = li + 2 * pl;
+        it.le = gptr(lo), it.lu = gptr(lo + pl), it.lv = gptr(lo + 2 * pl);
+        it.lie = gptr(li), it.liu = gptr(li + pl), it.liv = gptr(li + 2 * pl);
         if (rk) {
-          it.ie = be, it.iu = bu, it.iv = bv, it.out_e = ce;
+          it.ie = gptr(be), it.iu = gptr(bu), it.iv = gptr(bv), it.out_e = gptr(ce);
           float* lq = buf[j & 1];
-          it.lqe = lq, it.lqu = lq + pl, it.lqv = lq + 2 * pl;
+          it.lqe = gptr(lq), it.lqu = gptr(lq + pl), it.lqv = gptr(lq + 2 * pl);
           it.accumulate = 0, it.lq_mode = 1;
         } else {
-          it.ie = ae, it.iu = au, it.iv = av, it.out_e = be;
+          it.ie = gptr(ae), it.iu = gptr(au), it.iv = gptr(av), it.out_e = gptr(be);
           it.lqe = it.lqu = it.lqv = nullptr;
           it.accumulate = 1, it.lq_mode = 0;
         }
@@ -729,10 +731,10 @@ int gpo_adj_batch_create(const gpo_adj_member* members, int32_t n, int32_t k, in
       return rc;
     }
     for (int sg = 0; sg < n_segments; ++sg) {
-      copies[(size_t)sg * n + m] = CopyItem{M.state, M.ckpt + sg * st, p.nx, p.ny, g.sp, pl};
-      copies[((size_t)n_segments + sg) * n + m] = CopyItem{M.ckpt + sg * st, M.qs, p.nx, p.ny, g.sp, pl};
+      copies[(size_t)sg * n + m] = CopyItem{gptr(M.state), gptr(M.ckpt + sg * st), p.nx, p.ny, g.sp, pl};
+      copies[((size_t)n_segments + sg) * n + m] = CopyItem{gptr(M.ckpt + sg * st), gptr(M.qs), p.nx, p.ny, g.sp, pl};
     }
-    copies[(size_t)2 * n_segments * n + m] = CopyItem{M.lam, M.lam + st, p.nx, p.ny, g.sp, pl};
+    copies[(size_t)2 * n_segments * n + m] = CopyItem{gptr(M.lam), gptr(M.lam + st), p.nx, p.ny, g.sp, pl};
     const StepArgs& a = fwd[2 * m];
     if (a.n_strips > 0xFFFF || a.n_chunks > 0x7FFF) {
       adj_batch_free(b);

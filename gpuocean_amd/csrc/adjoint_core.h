@@ -29,6 +29,22 @@
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
 #define GPO_HD __host__ __device__
+#endif
+
+// Workspace pointers are global memory: in the device pass they carry the
+// global address space, so every access through them is a global (not flat)
+// instruction.  The host pass and the host build see plain pointers of the
+// same size.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define GPO_GLOBAL __attribute__((address_space(1)))
+#else
+#define GPO_GLOBAL
+#endif
+
+// A pointer converted to the global address space of the device pass.
+template <class T> inline GPO_GLOBAL T* gptr(T* p) { return (GPO_GLOBAL T*)p; }
+
+#if defined(__HIPCC__)
 #define GPO_FLAG(flag, bits) atomicOr((flag), (bits))
 #else
 #define GPO_HD
@@ -117,19 +133,19 @@ GPO_HD inline Src map_cell(const Geo& g, int r, int c) {
 
 // Dense workspace arrays ((ny+4) x (nx+4) each, element type S).
 template <class S> struct Fwd {
-  S *ce, *cu, *cv;      // converted cells
-  S *qx0, *qx1, *kx;    // x-slopes of u, v and the potential K
-  S *qy0, *qy1, *ly;    // y-slopes of u, v and the potential L
-  S *fx, *fy, *fz;      // x-face fluxes (face c between c and c+1)
-  S *gx, *gy, *gz;      // y-face fluxes (face r between r and r+1)
+  GPO_GLOBAL S *ce, *cu, *cv;    // converted cells
+  GPO_GLOBAL S *qx0, *qx1, *kx;  // x-slopes of u, v and the potential K
+  GPO_GLOBAL S *qy0, *qy1, *ly;  // y-slopes of u, v and the potential L
+  GPO_GLOBAL S *fx, *fy, *fz;    // x-face fluxes (face c between c and c+1)
+  GPO_GLOBAL S *gx, *gy, *gz;    // y-face fluxes (face r between r and r+1)
 };
 
 template <class S> inline Fwd<S> carve(void* ws, int nx, int ny) {
   const size_t n = (size_t)(nx + 4) * (ny + 4);
-  S* p = (S*)ws;
+  GPO_GLOBAL S* p = (GPO_GLOBAL S*)ws;
   Fwd<S> f;
-  S** slots[] = {&f.ce, &f.cu, &f.cv, &f.qx0, &f.qx1, &f.kx, &f.qy0, &f.qy1,
-                 &f.ly, &f.fx, &f.fy, &f.fz, &f.gx, &f.gy, &f.gz};
+  GPO_GLOBAL S** slots[] = {&f.ce, &f.cu, &f.cv, &f.qx0, &f.qx1, &f.kx, &f.qy0, &f.qy1,
+                            &f.ly, &f.fx, &f.fy, &f.fz, &f.gx, &f.gy, &f.gz};
   for (int i = 0; i < 15; ++i) *slots[i] = p + n * i;
   return f;
 }
@@ -280,8 +296,15 @@ template <class S>
 GPO_HD void f2_cell(Geo g, const S* Hi, const S* fc, Fwd<S> w, int r, int c) {
   if (c >= g.nx + 4 || r >= g.ny + 4) return;
   const size_t o = (size_t)r * g.W + c;
-  if (r >= 2 && r <= g.ny + 1 && c >= 1 && c <= g.nx + 2) xslope(g, w, Hi, fc, r, c, w.qx0[o], w.qx1[o], w.kx[o]);
-  if (r >= 1 && r <= g.ny + 2 && c >= 2 && c <= g.nx + 1) yslope(g, w, Hi, fc, r, c, w.qy0[o], w.qy1[o], w.ly[o]);
+  S a, b, k;
+  if (r >= 2 && r <= g.ny + 1 && c >= 1 && c <= g.nx + 2) {
+    xslope(g, w, Hi, fc, r, c, a, b, k);
+    w.qx0[o] = a, w.qx1[o] = b, w.kx[o] = k;
+  }
+  if (r >= 1 && r <= g.ny + 2 && c >= 2 && c <= g.nx + 1) {
+    yslope(g, w, Hi, fc, r, c, a, b, k);
+    w.qy0[o] = a, w.qy1[o] = b, w.ly[o] = k;
+  }
 }
 
 // CDKLM16_F_func (CDKLM16_kernel.cu:82-91) and CDKLM16_flux (:104-151), the
@@ -463,13 +486,13 @@ GPO_HD void f4_cell(Geo g, S dt, int rk_step, const S* Hi, const S* fc, Fwd<S> w
 // (its own terms plus the x-slope terms of its west and east neighbours) plus
 // yp[] of its south neighbour and ym[] of its north neighbour.
 template <class T> struct Rev {
-  T* xl[3];  // x-face s, side of cell s: (eta, u, v)
-  T* xr[3];  // x-face s, side of cell s+1
-  T* yl[3];  // y-face s, side of cell s: (eta, v, u)
-  T* yr[3];  // y-face s, side of cell s+1
-  T* px[3];  // own terms of the cell plus its x-neighbours' x-slope terms
-  T* ym[3];  // y-slopes of row s -> row s-1
-  T* yp[3];  // y-slopes of row s -> row s+1
+  GPO_GLOBAL T* xl[3];  // x-face s, side of cell s: (eta, u, v)
+  GPO_GLOBAL T* xr[3];  // x-face s, side of cell s+1
+  GPO_GLOBAL T* yl[3];  // y-face s, side of cell s: (eta, v, u)
+  GPO_GLOBAL T* yr[3];  // y-face s, side of cell s+1
+  GPO_GLOBAL T* px[3];  // own terms of the cell plus its x-neighbours' x-slope terms
+  GPO_GLOBAL T* ym[3];  // y-slopes of row s -> row s-1
+  GPO_GLOBAL T* yp[3];  // y-slopes of row s -> row s+1
 };
 
 // Workspace arrays of (ny+4) x (nx+4) elements: the 9 converted/slope arrays
@@ -478,7 +501,7 @@ constexpr int kAdjWorkArrays = 9 + 12 + 9;
 
 template <class T> inline Rev<T> carve_rev(void* ws, int nx, int ny) {
   const size_t n = (size_t)(nx + 4) * (ny + 4);
-  T* p = (T*)ws + 9 * n;  // after conv (3) and slopes (6) of Fwd<T>
+  GPO_GLOBAL T* p = (GPO_GLOBAL T*)ws + 9 * n;  // after conv (3) and slopes (6) of Fwd<T>
   Rev<T> v;
   for (int i = 0; i < 3; ++i) {
     v.xl[i] = p + n * i;
