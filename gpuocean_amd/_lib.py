@@ -60,6 +60,14 @@ class Member(ctypes.Structure):
     ]
 
 
+class Frs(ctypes.Structure):
+    """gpo_frs: flow-relaxation data of a differentiable run."""
+    _fields_ = [
+        ("sponge", c_int32 * 4), ("ns_cur", c_void_p), ("ns_next", c_void_p), ("ns_nx", c_int32),
+        ("ew_cur", c_void_p), ("ew_next", c_void_p), ("ew_ny", c_int32), ("t", c_float),
+    ]
+
+
 class AdjMember(ctypes.Structure):
     """gpo_adj_member: one member of a batched differentiable ensemble."""
     _fields_ = [
@@ -135,11 +143,20 @@ _SIGNATURES = [
      [c_int32, c_int32, c_void_p, c_int32, c_void_p, c_uint32, c_void_p, c_void_p, c_void_p]),
     ("gpo_cdklm16_trajectory_f32", c_int32,
      [ctypes.POINTER(CDKLM16Params), c_int32, c_float, ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p), c_int32,
-      c_void_p, c_int32, c_void_p, c_uint32, c_void_p]),
+      c_void_p, c_int32, c_void_p, c_uint32, ctypes.POINTER(Frs), c_void_p]),
     ("gpo_cdklm16_adjoint_steps_f32", c_int32,
      [ctypes.POINTER(CDKLM16Params), c_int32, c_float, ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p),
-      ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p), c_int32, c_void_p, c_int32, c_void_p, c_uint32, c_void_p,
-      c_void_p, c_void_p]),
+      ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p), c_int32, c_void_p, c_int32, c_void_p, c_uint32,
+      ctypes.POINTER(Frs), c_void_p, c_void_p, c_void_p]),
+    ("gpo_frs_adj_f32", c_int32,
+     [ctypes.POINTER(CDKLM16Params), c_uint32, ctypes.POINTER(c_int32), c_void_p, c_void_p, c_void_p, c_int32,
+      c_void_p]),
+    ("gpo_frs_adj_f64", c_int32,
+     [ctypes.POINTER(CDKLM16Params), c_uint32, ctypes.POINTER(c_int32), c_void_p, c_void_p, c_void_p, c_int32,
+      c_void_p]),
+    ("gpo_frs_apply_f64", c_int32,
+     [ctypes.POINTER(CDKLM16Params), c_uint32, ctypes.POINTER(Frs), c_void_p, c_void_p, c_void_p, c_int32,
+      c_void_p]),
     ("gpo_adj_batch_create", c_int32,
      [ctypes.POINTER(AdjMember), c_int32, c_int32, c_int32, ctypes.POINTER(c_void_p), c_void_p]),
     ("gpo_adj_batch_forward", c_int32, [c_void_p, c_int32, c_void_p]),

@@ -1,8 +1,9 @@
 """Gradient of a point-observation misfit through CDKLM16 time steps.
 
 No counterpart exists in the reference.  ``DifferentiableCDKLM16`` takes the
-configuration of a ``CDKLM16`` simulator (RK2, wet domain, wall/periodic
-boundaries, no friction, land or forcing) and gives
+configuration of a ``CDKLM16`` simulator (RK2, wet domain, wall, periodic or
+flow-relaxation boundaries with boundary data of one time, no friction, land
+or forcing) and gives
 
   * the FP32 forward run: the simulator's own step kernel (bit-compatible with
     the host oracle), keeping a checkpoint every ``checkpoint_every`` steps;
@@ -73,8 +74,24 @@ class DifferentiableCDKLM16:
             raise ValueError("the differentiable step needs RK2 and r = 0")
         if sim.has_land or sim._has_wind or sim._has_pressure or sim.one_dimensional:
             raise ValueError("the differentiable step supports no land, wind, pressure or one-dimensional mode")
-        if any(int(c) not in (1, 2) for c in (bc.north, bc.east, bc.south, bc.west)):
-            raise ValueError("the differentiable step supports wall and periodic boundaries only")
+        if any(int(c) not in (1, 2, 3) for c in (bc.north, bc.east, bc.south, bc.west)):
+            raise ValueError("the differentiable step supports wall, periodic and flow-relaxation boundaries only")
+        self.frs = None
+        if sim._needs_explicit_bc_each_substep():
+            if len(sim.bc_data.t) != 1:
+                raise ValueError("the differentiable step needs flow-relaxation data of a single time")
+            sim._update_bc_values()
+            f = sim._frs
+            self.frs = L.Frs()
+            for i in range(4):
+                self.frs.sponge[i] = int(sim.sponge[i])
+            addr = lambda t: None if t is None else t.data_ptr()
+            self.frs.ns_cur, self.frs.ns_next = addr(f.get('ns_cur')), addr(f.get('ns_next'))
+            self.frs.ns_nx = int(sim.bc_data.north.h[0].shape[0])
+            self.frs.ew_cur, self.frs.ew_next = addr(f.get('ew_cur')), addr(f.get('ew_next'))
+            self.frs.ew_ny = int(sim.bc_data.east.h[0].shape[0])
+            self.frs.t = 1.0
+            self._frs_keep = f   # the data tensors stay alive with the model
         if not sim.exact_arithmetic:
             raise ValueError("the differentiable step runs the exact forward build")
         self.sim = sim
@@ -139,6 +156,22 @@ class DifferentiableCDKLM16:
             _vp(self.Hi), self.Hi.pitch, self._fc_ptr(), self.bc, _vp(self.ws32), _vp(self.flag), self._sh()),
             "gpo_cdklm16_substep_adj_f32")
 
+    def _frs_ptr(self):
+        return None if self.frs is None else ctypes.byref(self.frs)
+
+    def _frs_scale(self, st, f64=False):
+        """Transpose (and tangent) of the flow-relaxation blend on st."""
+        if self.frs is None:
+            return
+        fn = L.lib().gpo_frs_adj_f64 if f64 else L.lib().gpo_frs_adj_f32
+        L.check(fn(ctypes.byref(self.params), self.bc, self.frs.sponge, *st.ptrs(), st.pitch, self._sh()),
+                "gpo_frs_adj")
+
+    def _frs_apply64(self, st):
+        if self.frs is not None:
+            L.check(L.lib().gpo_frs_apply_f64(ctypes.byref(self.params), self.bc, ctypes.byref(self.frs), *st.ptrs(),
+                                              st.pitch, self._sh()), "gpo_frs_apply_f64")
+
     def _enter(self):
         """Order the work stream after the caller's current stream."""
         self.stream.wait_stream(torch.cuda.current_stream(self.dev))
@@ -168,9 +201,17 @@ class DifferentiableCDKLM16:
                 c.copy_from(A)
             ckpts.append(c)
             m = min(self.k, self.n_steps - i0)
-            L.check(L.lib().gpo_cdklm16_rk2_steps(
-                ctypes.byref(self.params), m, float(self.dt), *A.ptrs(), *B.ptrs(), A.pitch, _vp(self.Hi),
-                self.Hi.pitch, self._fc_ptr(), self.bc, 1, self._sh()), "gpo_cdklm16_rk2_steps")
+            if self.frs is None:
+                L.check(L.lib().gpo_cdklm16_rk2_steps(
+                    ctypes.byref(self.params), m, float(self.dt), *A.ptrs(), *B.ptrs(), A.pitch, _vp(self.Hi),
+                    self.Hi.pitch, self._fc_ptr(), self.bc, 1, self._sh()), "gpo_cdklm16_rk2_steps")
+            else:   # boundary blend after each substep: steps A -> (B) -> A through the trajectory entry
+                PP = ctypes.c_void_p
+                qs = (PP * (3 * (m + 1)))(*[a.data_ptr() for _ in range(m + 1) for a in A.t])
+                q1s = (PP * (3 * m))(*[a.data_ptr() for _ in range(m) for a in B.t])
+                L.check(L.lib().gpo_cdklm16_trajectory_f32(
+                    ctypes.byref(self.params), m, float(self.dt), qs, q1s, A.pitch, _vp(self.Hi), self.Hi.pitch,
+                    self._fc_ptr(), self.bc, self._frs_ptr(), self._sh()), "gpo_cdklm16_trajectory_f32")
         with torch.cuda.stream(s):
             self.J.zero_()
         self._misfit32(A)
@@ -240,11 +281,11 @@ class DifferentiableCDKLM16:
                 seg["Qs"][0].copy_from(self._ckpts[si])
             L.check(lib.gpo_cdklm16_trajectory_f32(
                 ctypes.byref(self.params), m, float(self.dt), seg["qs_p"], seg["q1s_p"], lam.pitch, _vp(self.Hi),
-                self.Hi.pitch, self._fc_ptr(), self.bc, self._sh()), "gpo_cdklm16_trajectory_f32")
+                self.Hi.pitch, self._fc_ptr(), self.bc, self._frs_ptr(), self._sh()), "gpo_cdklm16_trajectory_f32")
             L.check(lib.gpo_cdklm16_adjoint_steps_f32(
                 ctypes.byref(self.params), m, float(self.dt), seg["qs_p"], seg["q1s_p"], seg["lam_p"],
-                seg["work_p"], lam.pitch, _vp(self.Hi), self.Hi.pitch, self._fc_ptr(), self.bc, _vp(self.ws32),
-                _vp(self.flag), self._sh()), "gpo_cdklm16_adjoint_steps_f32")
+                seg["work_p"], lam.pitch, _vp(self.Hi), self.Hi.pitch, self._fc_ptr(), self.bc, self._frs_ptr(),
+                _vp(self.ws32), _vp(self.flag), self._sh()), "gpo_cdklm16_adjoint_steps_f32")
         s.synchronize()
         return [a.clone() for a in lam.views()]
 
@@ -306,7 +347,9 @@ class DifferentiableCDKLM16:
             Qs[0].load(eta0, hu0, hv0)
         for j in range(n):
             self._substep64(Qs[j], Q1s[j], None, 0)
+            self._frs_apply64(Q1s[j])
             self._substep64(Q1s[j], Qs[j + 1], Qs[j], 1)
+            self._frs_apply64(Qs[j + 1])
         self._leave()
         return Qs, Q1s
 
@@ -319,7 +362,9 @@ class DifferentiableCDKLM16:
             A.load(eta0, hu0, hv0)
         for _ in range(self.n_steps):
             self._substep64(A, B, None, 0)
+            self._frs_apply64(B)
             self._substep64(B, A, A, 1)
+            self._frs_apply64(A)
         with torch.cuda.stream(self.stream):
             self.J.zero_()
         self._misfit64(A)
@@ -336,7 +381,9 @@ class DifferentiableCDKLM16:
         for j in reversed(range(n)):
             with torch.cuda.stream(self.stream):
                 X.zero_()
+            self._frs_scale(lam, f64=True)
             self._adj64(1, Q1s[j], Qs[j + 1], lam, L1, X, 0)
+            self._frs_scale(L1, f64=True)
             self._adj64(0, Qs[j], Q1s[j], L1, X, None, 1)
             lam, X = X, lam
         self._leave()
@@ -350,7 +397,9 @@ class DifferentiableCDKLM16:
         d, d1, dn = d0, self._new_state(torch.float64), self._new_state(torch.float64)
         for j in range(n):
             self._tlm64(Qs[j], d, out, d1, None, None, 0)
+            self._frs_scale(d1, f64=True)
             self._tlm64(Q1s[j], d1, out, dn, Qs[j], d, 1)
+            self._frs_scale(dn, f64=True)
             d = dn
             dn = self._new_state(torch.float64)
         self._leave()

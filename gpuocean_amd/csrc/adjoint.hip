@@ -111,6 +111,88 @@ __global__ void a3_fold(Geo g, int rk_step, int accumulate, int lq_mode, const T
           lq_v, blockIdx.y * blockDim.y + threadIdx.y + 2, blockIdx.x * blockDim.x + threadIdx.x + 2);
 }
 
+// ---- flow relaxation (boundary_kernels.cu:406-540): q <- (1 - alpha) q + alpha q_ext
+// on the sponge rows (north/south pass) and then columns (east/west pass),
+// alpha = 1 - tanh(max(0, j) / 2) in float as the forward computes it.
+struct Sponge {
+  int sn, se, ss, sw;
+};
+__device__ inline float frs_alpha_ns(const Geo& g, const Sponge& sp, int tj, bool* in) {
+  *in = (g.bc_s == 3 && tj < sp.ss) || (g.bc_n == 3 && tj > g.ny + 4 - 1 - sp.sn);
+  const float j = (tj <= sp.ss) ? (float)(tj - 1) : (float)((g.ny + 4 - 2) - tj);
+  return 1.0f - tanhf(fmaxf(0.0f, j) / 2.0f);
+}
+__device__ inline float frs_alpha_ew(const Geo& g, const Sponge& sp, int ti, bool* in) {
+  *in = (g.bc_w == 3 && ti < sp.sw) || (g.bc_e == 3 && ti > g.nx + 4 - 1 - sp.se);
+  const float j = (ti <= sp.sw) ? (float)(ti - 1) : (float)((g.nx + 4 - 2) - ti);
+  return 1.0f - tanhf(fmaxf(0.0f, j) / 2.0f);
+}
+
+// Transpose (and tangent) of the relaxation: a <- (1 - alpha) a per pass, the
+// east/west pass first (it was applied last).
+template <class T>
+__global__ void frs_scale(Geo g, Sponge sp, T* a, T* b, T* c) {
+  const int ti = blockIdx.x * blockDim.x + threadIdx.x, tj = blockIdx.y * blockDim.y + threadIdx.y;
+  if (ti >= g.nx + 4 || tj >= g.ny + 4) return;
+  const size_t k = (size_t)tj * g.sp + ti;
+  bool in_ew, in_ns;
+  const float aew = frs_alpha_ew(g, sp, ti, &in_ew), ans = frs_alpha_ns(g, sp, tj, &in_ns);
+  if (in_ew) {
+    const T f = T(1.0f - aew);
+    a[k] *= f, b[k] *= f, c[k] *= f;
+  }
+  if (in_ns) {
+    const T f = T(1.0f - ans);
+    a[k] *= f, b[k] *= f, c[k] *= f;
+  }
+}
+
+__device__ inline float3 frs_bilinear3(const float* data, int dnx, int dny, float nx_, float ny_) {
+  nx_ = fminf(fmaxf(nx_ - 0.5f / (float)dnx, 0.0f), 1.0f);
+  ny_ = fminf(fmaxf(ny_ - 0.5f / (float)dny, 0.0f), 1.0f);
+  const float x = nx_ * (float)dnx, y = ny_ * (float)dny;
+  int x0 = (int)floorf(x), y0 = (int)floorf(y);
+  const float fx = x - (float)x0, fy = y - (float)y0;
+  x0 = min(x0, dnx - 1);
+  y0 = min(y0, dny - 1);
+  const int x1 = min(x0 + 1, dnx - 1), y1 = min(y0 + 1, dny - 1);
+  float r[3];
+  for (int q = 0; q < 3; ++q) {
+    const float d00 = data[(y0 * dnx + x0) * 3 + q], d01 = data[(y0 * dnx + x1) * 3 + q];
+    const float d10 = data[(y1 * dnx + x0) * 3 + q], d11 = data[(y1 * dnx + x1) * 3 + q];
+    r[q] = d00 + fx * (d01 - d00) + fy * (d10 - d00) + fx * fy * (d00 + d11 - d01 - d10);
+  }
+  return make_float3(r[0], r[1], r[2]);
+}
+
+// The relaxation itself on an FP64 state (exterior values from the float
+// boundary data, as the FP32 kernels form them).
+__global__ void frs_blend_f64(Geo g, Sponge sp, const float* ns_cur, const float* ns_next, int ns_nx,
+                              const float* ew_cur, const float* ew_next, int ew_ny, float t_, double* e, double* u,
+                              double* v) {
+  const int ti = blockIdx.x * blockDim.x + threadIdx.x, tj = blockIdx.y * blockDim.y + threadIdx.y;
+  if (ti >= g.nx + 4 || tj >= g.ny + 4) return;
+  const size_t k = (size_t)tj * g.sp + ti;
+  bool in_ns, in_ew;
+  const float ans = frs_alpha_ns(g, sp, tj, &in_ns), aew = frs_alpha_ew(g, sp, ti, &in_ew);
+  if (in_ns) {
+    const float s = (float)ti / (float)(g.nx + 4), t = ((double)tj < g.ny / 2.0) ? 0.25f : 0.75f;
+    const float3 c = frs_bilinear3(ns_cur, ns_nx, 2, s, t), n = frs_bilinear3(ns_next, ns_nx, 2, s, t);
+    const double a = ans;
+    e[k] = (1.0 - a) * e[k] + a * (double)(t_ * n.x + (1.0f - t_) * c.x);
+    u[k] = (1.0 - a) * u[k] + a * (double)(t_ * n.y + (1.0f - t_) * c.y);
+    v[k] = (1.0 - a) * v[k] + a * (double)(t_ * n.z + (1.0f - t_) * c.z);
+  }
+  if (in_ew) {
+    const float s = ((double)ti < g.nx / 2.0) ? 0.25f : 0.75f, t = (float)tj / (float)(g.ny + 4);
+    const float3 c = frs_bilinear3(ew_cur, 2, ew_ny, s, t), n = frs_bilinear3(ew_next, 2, ew_ny, s, t);
+    const double a = aew;
+    e[k] = (1.0 - a) * e[k] + a * (double)(t_ * n.x + (1.0f - t_) * c.x);
+    u[k] = (1.0 - a) * u[k] + a * (double)(t_ * n.y + (1.0f - t_) * c.y);
+    v[k] = (1.0 - a) * v[k] + a * (double)(t_ * n.z + (1.0f - t_) * c.z);
+  }
+}
+
 // ---- batched reverse substep: block row z is member z of a table
 // One member's reverse substep: its geometry, the stage input and output, the
 // sensitivities in and out, and its workspace.
@@ -307,7 +389,8 @@ int make_geo(const gpo_cdklm16_params* p, uint32_t bc, int32_t pitch_bytes, int3
   g->bc_e = (bc >> 8) & 0xFF;
   g->bc_w = bc & 0xFF;
   for (int code : {g->bc_n, g->bc_s, g->bc_e, g->bc_w})
-    if (code != 1 && code != 2) return bad_arg("differentiable substep: wall (1) and periodic (2) boundaries only");
+    if (code < 1 || code > 3)
+      return bad_arg("differentiable substep: wall (1), periodic (2) and flow-relaxation (3) boundaries only");
   if (pitch_bytes % elem || hi_pitch_bytes % elem || f_pitch_bytes % elem || pitch_bytes < elem * (p->nx + 4) ||
       hi_pitch_bytes < elem * (p->nx + 5))
     return bad_arg("differentiable substep: pitches must be multiples of the element size and cover a row");
@@ -490,7 +573,8 @@ size_t gpo_tlm_workspace_bytes(int32_t nx, int32_t ny) {
 
 int gpo_cdklm16_trajectory_f32(const gpo_cdklm16_params* p, int32_t n_steps, float dt, float* const* qs,
                                float* const* q1s, int32_t pitch, const float* Hi, int32_t Hi_pitch,
-                               const float* f_cells, uint32_t boundary_conditions, void* stream) {
+                               const float* f_cells, uint32_t boundary_conditions, const gpo_frs* frs,
+                               void* stream) {
   if (!p || n_steps < 0 || !qs || !q1s || !Hi) return bad_arg("gpo_cdklm16_trajectory_f32: bad arguments");
   if (p->rk_order != 2) return bad_arg("gpo_cdklm16_trajectory_f32: RK2 only");
   hipStream_t s = (hipStream_t)stream;
@@ -506,8 +590,21 @@ int gpo_cdklm16_trajectory_f32(const gpo_cdklm16_params* p, int32_t n_steps, flo
                           f_cells, nullptr, boundary_conditions, &a1);
     if (rc) return rc;
     hipError_t e = launch_cdklm_exact(a0, p->has_land, p->f_is_array, p->angle_is_zero, s);
-    if (e == hipSuccess) e = launch_cdklm_exact(a1, p->has_land, p->f_is_array, p->angle_is_zero, s);
     if (e != hipSuccess) return hip_check(e, "gpo_cdklm16_trajectory_f32 launch");
+    if (frs) {  // ghost cells and sponge of the stage, as the step does between substeps
+      rc = gpo_boundary_conditions(p->nx, p->ny, boundary_conditions, frs->sponge, b[0], b[1], b[2], pitch,
+                                   frs->ns_cur, frs->ns_next, frs->ns_nx, frs->ew_cur, frs->ew_next, frs->ew_ny,
+                                   frs->t, stream);
+      if (rc) return rc;
+    }
+    e = launch_cdklm_exact(a1, p->has_land, p->f_is_array, p->angle_is_zero, s);
+    if (e != hipSuccess) return hip_check(e, "gpo_cdklm16_trajectory_f32 launch");
+    if (frs) {
+      rc = gpo_boundary_conditions(p->nx, p->ny, boundary_conditions, frs->sponge, c[0], c[1], c[2], pitch,
+                                   frs->ns_cur, frs->ns_next, frs->ns_nx, frs->ew_cur, frs->ew_next, frs->ew_ny,
+                                   frs->t, stream);
+      if (rc) return rc;
+    }
   }
   return 0;
 }
@@ -515,13 +612,15 @@ int gpo_cdklm16_trajectory_f32(const gpo_cdklm16_params* p, int32_t n_steps, flo
 int gpo_cdklm16_adjoint_steps_f32(const gpo_cdklm16_params* p, int32_t n_steps, float dt, const float* const* qs,
                                   const float* const* q1s, float* const* lam, float* const* work, int32_t pitch,
                                   const float* Hi, int32_t Hi_pitch, const float* f_cells,
-                                  uint32_t boundary_conditions, void* workspace, int32_t* flag, void* stream) {
+                                  uint32_t boundary_conditions, const gpo_frs* frs, void* workspace, int32_t* flag,
+                                  void* stream) {
   Geo g;
   int rc = make_geo(p, boundary_conditions, pitch, Hi_pitch, f_cells ? pitch : 4, 4, &g);
   if (rc) return rc;
   if (n_steps < 0 || !qs || !q1s || !lam || !work || !workspace)
     return bad_arg("gpo_cdklm16_adjoint_steps_f32: bad arguments");
   hipStream_t s = (hipStream_t)stream;
+  const Sponge sp = frs ? Sponge{frs->sponge[0], frs->sponge[1], frs->sponge[2], frs->sponge[3]} : Sponge{0, 0, 0, 0};
   float* cur[3] = {lam[0], lam[1], lam[2]};     // lambda at Q^{j+1}, then at Q^j
   float* nxt[3] = {work[0], work[1], work[2]};  // lambda being formed at Q^j
   float* l1[3] = {work[3], work[4], work[5]};   // lambda at the RK stage Q^(1)
@@ -530,9 +629,12 @@ int gpo_cdklm16_adjoint_steps_f32(const gpo_cdklm16_params* p, int32_t n_steps, 
     const float* const* b = q1s + 3 * j;
     const float* const* c = qs + 3 * (j + 1);
     // reverse of the second RK stage (writes its direct Q^n term to nxt), then of
-    // the first (adds its sensitivity of Q^n to nxt)
+    // the first (adds its sensitivity of Q^n to nxt); with flow relaxation,
+    // the transpose of each sponge blend before the stage that produced it
+    if (frs) hipLaunchKernelGGL(frs_scale<float>, cell_grid(g), kBlock, 0, s, g, sp, cur[0], cur[1], cur[2]);
     rc = adjoint_stages<float>(g, dt, 1, b[0], b[1], b[2], c[0], cur[0], cur[1], cur[2], l1[0], l1[1], l1[2], nxt[0],
                                nxt[1], nxt[2], 0, 1, Hi, f_cells, workspace, flag, s);
+    if (!rc && frs) hipLaunchKernelGGL(frs_scale<float>, cell_grid(g), kBlock, 0, s, g, sp, l1[0], l1[1], l1[2]);
     if (!rc)
       rc = adjoint_stages<float>(g, dt, 0, a[0], a[1], a[2], b[0], l1[0], l1[1], l1[2], nxt[0], nxt[1], nxt[2],
                                  nullptr, nullptr, nullptr, 1, 0, Hi, f_cells, workspace, flag, s);
@@ -643,6 +745,43 @@ int gpo_point_misfit_f64(int32_t nx, int32_t ny, double dx, double dy, const dou
   return hip_check(hipGetLastError(), "gpo_point_misfit_f64 launch");
 }
 
+int gpo_frs_adj_f32(const gpo_cdklm16_params* p, uint32_t boundary_conditions, const int32_t sponge[4],
+                    float* lam_eta, float* lam_hu, float* lam_hv, int32_t pitch, void* stream) {
+  Geo g;
+  int rc = make_geo(p, boundary_conditions, pitch, 4 * (p ? p->nx + 5 : 0), 4, 4, &g);
+  if (rc) return rc;
+  if (!sponge || !lam_eta || !lam_hu || !lam_hv) return bad_arg("gpo_frs_adj_f32: bad arguments");
+  hipLaunchKernelGGL(frs_scale<float>, cell_grid(g), kBlock, 0, (hipStream_t)stream, g,
+                     Sponge{sponge[0], sponge[1], sponge[2], sponge[3]}, lam_eta, lam_hu, lam_hv);
+  return hip_check(hipGetLastError(), "gpo_frs_adj_f32 launch");
+}
+
+int gpo_frs_adj_f64(const gpo_cdklm16_params* p, uint32_t boundary_conditions, const int32_t sponge[4],
+                    double* lam_eta, double* lam_hu, double* lam_hv, int32_t pitch, void* stream) {
+  Geo g;
+  int rc = make_geo(p, boundary_conditions, pitch, 8 * (p ? p->nx + 5 : 0), 8, 8, &g);
+  if (rc) return rc;
+  if (!sponge || !lam_eta || !lam_hu || !lam_hv) return bad_arg("gpo_frs_adj_f64: bad arguments");
+  hipLaunchKernelGGL(frs_scale<double>, cell_grid(g), kBlock, 0, (hipStream_t)stream, g,
+                     Sponge{sponge[0], sponge[1], sponge[2], sponge[3]}, lam_eta, lam_hu, lam_hv);
+  return hip_check(hipGetLastError(), "gpo_frs_adj_f64 launch");
+}
+
+int gpo_frs_apply_f64(const gpo_cdklm16_params* p, uint32_t boundary_conditions, const gpo_frs* frs, double* eta,
+                      double* hu, double* hv, int32_t pitch, void* stream) {
+  Geo g;
+  int rc = make_geo(p, boundary_conditions, pitch, 8 * (p ? p->nx + 5 : 0), 8, 8, &g);
+  if (rc) return rc;
+  if (!frs || !eta || !hu || !hv) return bad_arg("gpo_frs_apply_f64: bad arguments");
+  const bool ns = g.bc_n == 3 || g.bc_s == 3, ew = g.bc_e == 3 || g.bc_w == 3;
+  if ((ns && (!frs->ns_cur || !frs->ns_next || frs->ns_nx < 1)) || (ew && (!frs->ew_cur || !frs->ew_next || frs->ew_ny < 1)))
+    return bad_arg("gpo_frs_apply_f64: boundary data missing");
+  hipLaunchKernelGGL(frs_blend_f64, cell_grid(g), kBlock, 0, (hipStream_t)stream, g,
+                     Sponge{frs->sponge[0], frs->sponge[1], frs->sponge[2], frs->sponge[3]}, frs->ns_cur,
+                     frs->ns_next, frs->ns_nx, frs->ew_cur, frs->ew_next, frs->ew_ny, frs->t, eta, hu, hv);
+  return hip_check(hipGetLastError(), "gpo_frs_apply_f64 launch");
+}
+
 // ---- batched differentiable ensemble
 
 int gpo_adj_batch_create(const gpo_adj_member* members, int32_t n, int32_t k, int32_t n_segments,
@@ -665,6 +804,8 @@ int gpo_adj_batch_create(const gpo_adj_member* members, int32_t n, int32_t k, in
     if (!rc && (!M.state || !M.stage || !M.ckpt || !M.qs || !M.q1s || !M.lam || !M.workspace || !M.Hi ||
                 M.plane < (int64_t)M.pitch * (p.ny + 4) || !(M.dt > 0.0f)))
       rc = bad_arg("gpo_adj_batch_create: member buffers, plane or dt");
+    if (!rc && (g.bc_n == 3 || g.bc_s == 3 || g.bc_e == 3 || g.bc_w == 3))
+      rc = bad_arg("gpo_adj_batch_create: flow-relaxation members are not batched (use the per-member path)");
     if (m == 0) b->f_array = p.f_is_array, b->angle0 = p.angle_is_zero;
     if (!rc && (p.f_is_array != b->f_array || p.angle_is_zero != b->angle0 || (p.f_is_array && !M.f_cells)))
       rc = bad_arg("gpo_adj_batch_create: members differ in coriolis texture or north angle");

@@ -14,8 +14,10 @@
 //    or wrap.
 //
 // Scope of the derivative: RK2 substeps of wet cells (h >= desingularisation
-// eps everywhere), wall and periodic boundaries, no friction, wind or
-// pressure forcing and no land.  Branches outside that scope set bits in the
+// eps everywhere), wall and periodic boundaries (flow-relaxation sides read
+// their ghost cells as stored constants; the sponge blend between substeps
+// is handled by the callers), no friction, wind or pressure forcing and no
+// land.  Branches outside that scope set bits in the
 // caller's flag word.  Minmod, max and min take the subgradient of the first
 // minimal/maximal argument, the same choice the tangent-linear model makes.
 //

@@ -345,13 +345,28 @@ int gpo_cdklm16_substep_adj_f64(const gpo_cdklm16_params* p, double dt, int32_t 
                                 const double* Hi, int32_t Hi_pitch, const double* f_cells,
                                 uint32_t boundary_conditions, void* workspace, int32_t* flag, void* stream);
 
+/* Flow-relaxation data of a differentiable run (boundary code 3 sides):
+ * sponge widths N, E, S, W and the exterior solution, packed as for
+ * gpo_boundary_conditions, with the weight t of next against cur. */
+typedef struct {
+  int32_t sponge[4];
+  const float *ns_cur, *ns_next;
+  int32_t ns_nx;
+  const float *ew_cur, *ew_next;
+  int32_t ew_ny;
+  float t;
+} gpo_frs;
+
 /* n_steps RK2 steps keeping every state: Q^{j+1} = S1(S0(Q^j); Q^j) with the
  * stage Q^(1)_j stored.  qs holds 3(n_steps+1) device pointers (eta, hu, hv of
  * Q^0 .. Q^n; Q^0 is read), q1s 3 n_steps (the stages).  The recomputation a
- * checkpointed gradient makes of one segment; exact build. */
+ * checkpointed gradient makes of one segment; exact build.  With frs (NULL
+ * when no side is 3), gpo_boundary_conditions follows each substep, as in
+ * the step of a flow-relaxation simulator. */
 int gpo_cdklm16_trajectory_f32(const gpo_cdklm16_params* p, int32_t n_steps, float dt, float* const* qs,
                                float* const* q1s, int32_t pitch, const float* Hi, int32_t Hi_pitch,
-                               const float* f_cells, uint32_t boundary_conditions, void* stream);
+                               const float* f_cells, uint32_t boundary_conditions, const gpo_frs* frs,
+                               void* stream);
 
 /* Reverse sweep over such a trajectory: lam (3 pointers) holds lambda at
  * Q^n on entry and (dQ^n/dQ^0)^T lambda on return (interior cells; the ghost
@@ -360,7 +375,19 @@ int gpo_cdklm16_trajectory_f32(const gpo_cdklm16_params* p, int32_t n_steps, flo
 int gpo_cdklm16_adjoint_steps_f32(const gpo_cdklm16_params* p, int32_t n_steps, float dt, const float* const* qs,
                                   const float* const* q1s, float* const* lam, float* const* work, int32_t pitch,
                                   const float* Hi, int32_t Hi_pitch, const float* f_cells,
-                                  uint32_t boundary_conditions, void* workspace, int32_t* flag, void* stream);
+                                  uint32_t boundary_conditions, const gpo_frs* frs, void* workspace,
+                                  int32_t* flag, void* stream);
+
+/* Transpose (equally: tangent) of the flow-relaxation blend: lambda *=
+ * (1 - alpha) on the sponge cells of the code-3 sides (the alpha of
+ * boundary_kernels.cu:406-540), east/west pass first. */
+int gpo_frs_adj_f32(const gpo_cdklm16_params* p, uint32_t boundary_conditions, const int32_t sponge[4],
+                    float* lam_eta, float* lam_hu, float* lam_hv, int32_t pitch, void* stream);
+int gpo_frs_adj_f64(const gpo_cdklm16_params* p, uint32_t boundary_conditions, const int32_t sponge[4],
+                    double* lam_eta, double* lam_hu, double* lam_hv, int32_t pitch, void* stream);
+/* The blend itself on an FP64 state (for the FP64 verification runs). */
+int gpo_frs_apply_f64(const gpo_cdklm16_params* p, uint32_t boundary_conditions, const gpo_frs* frs, double* eta,
+                      double* hu, double* hv, int32_t pitch, void* stream);
 
 /* ---- batched differentiable ensemble: many members' checkpointed forward,
  * trajectory recomputation and reverse sweep, one launch per stage for all

@@ -15,11 +15,25 @@ from oracle import oracle as O
 pytestmark = pytest.mark.gpu
 
 
+def _frs_data(kw):
+    """Flow-relaxation data of one time: the initial state's outermost interior rows and columns."""
+    e, u, v = kw["eta0"], kw["hu0"], kw["hv0"]
+    ny, nx = e.shape[0] - 4, e.shape[1] - 4
+
+    def side(sl):
+        return Common.SingleBoundaryConditionData(h=[e[sl].copy()], hu=[u[sl].copy()], hv=[v[sl].copy()])
+
+    return Common.BoundaryConditionsData(north=side(np.s_[ny + 1, :]), south=side(np.s_[2, :]),
+                                         east=side(np.s_[:, nx + 1]), west=side(np.s_[:, 2]))
+
+
 def _model(gpu_ctx, n=64, bc=None, n_steps=20, k=7, n_obs=30, seed=5, f=1.2e-4):
     kw = cases.config_c3(n=n, seed=seed)
     kw["f"] = f
     if bc is not None:
         kw["boundary_conditions"] = bc
+        if any(c == 3 for c in (bc.north, bc.east, bc.south, bc.west)):
+            kw["boundary_conditions_data"] = _frs_data(kw)
     bg, truth = cases.c3_background_and_truth(kw, seed=seed)
     sim = CDKLM16(gpu_ctx, **kw)
     pos = cases.c3_drifters(n, n, 200.0, 200.0, n=n_obs, seed=seed)
@@ -59,6 +73,10 @@ def _interior_noise(shape, rng, scale):
 
 BCS = [Common.BoundaryConditions(1, 1, 1, 1), Common.BoundaryConditions(1, 2, 1, 2),
        Common.BoundaryConditions(2, 2, 2, 2)]
+_SPONGE = {'north': 8, 'east': 8, 'south': 8, 'west': 8}
+BCS_FRS = [Common.BoundaryConditions(3, 3, 3, 3, spongeCells=_SPONGE),
+           Common.BoundaryConditions(3, 2, 3, 2, spongeCells=_SPONGE),
+           Common.BoundaryConditions(1, 3, 1, 3, spongeCells=_SPONGE)]
 
 
 @pytest.mark.parametrize("bc", BCS, ids=["walls", "periodic_x", "periodic"])
@@ -78,7 +96,8 @@ def test_f64_forward_matches_f64_oracle(gpu_ctx, bc):
     assert model.flag_value() == 0
 
 
-@pytest.mark.parametrize("bc", BCS, ids=["walls", "periodic_x", "periodic"])
+@pytest.mark.parametrize("bc", BCS + BCS_FRS, ids=["walls", "periodic_x", "periodic", "frs", "frs_ns_periodic_ew",
+                                                   "walls_ns_frs_ew"])
 def test_dot_product_f64(gpu_ctx, bc):
     """<TLM v, w> == <v, ADJ w> along an FP64 trajectory of 10 steps."""
     print(f"dot product, bc={bc.packed():#x}", flush=True)
@@ -144,7 +163,7 @@ def _fd_check(model, bg, n_dirs, rng, tol64, tol32):
     return g32, g64
 
 
-@pytest.mark.parametrize("bc", BCS[:2], ids=["walls", "periodic_x"])
+@pytest.mark.parametrize("bc", BCS[:2] + BCS_FRS[:2], ids=["walls", "periodic_x", "frs", "frs_ns_periodic_ew"])
 def test_gradient_against_finite_differences_small(gpu_ctx, bc):
     print(f"FD check 64x64, bc={bc.packed():#x}", flush=True)
     kw, sim, model, bg = _model(gpu_ctx, n=64, bc=bc, n_steps=40, k=7, n_obs=30)
