@@ -1,9 +1,9 @@
 """Gradient of a point-observation misfit through CDKLM16 time steps.
 
 No counterpart exists in the reference.  ``DifferentiableCDKLM16`` takes the
-configuration of a ``CDKLM16`` simulator (RK2, wet domain, wall, periodic or
-flow-relaxation boundaries with boundary data of one time, no friction, land
-or forcing) and gives
+configuration of a ``CDKLM16`` simulator (RK2, wet or land cells, wall,
+periodic or flow-relaxation boundaries with boundary data of one time, no
+friction or forcing) and gives
 
   * the FP32 forward run: the simulator's own step kernel (bit-compatible with
     the host oracle), keeping a checkpoint every ``checkpoint_every`` steps;
@@ -72,8 +72,8 @@ class DifferentiableCDKLM16:
         bc = sim.boundary_conditions
         if int(sim.rk_order) != 2 or float(sim.r) != 0.0:
             raise ValueError("the differentiable step needs RK2 and r = 0")
-        if sim.has_land or sim._has_wind or sim._has_pressure or sim.one_dimensional:
-            raise ValueError("the differentiable step supports no land, wind, pressure or one-dimensional mode")
+        if sim._has_wind or sim._has_pressure or sim.one_dimensional:
+            raise ValueError("the differentiable step supports no wind, pressure or one-dimensional mode")
         if any(int(c) not in (1, 2, 3) for c in (bc.north, bc.east, bc.south, bc.west)):
             raise ValueError("the differentiable step supports wall, periodic and flow-relaxation boundaries only")
         self.frs = None

@@ -379,7 +379,7 @@ int make_geo(const gpo_cdklm16_params* p, uint32_t bc, int32_t pitch_bytes, int3
   if (!p || p->nx < 2 || p->ny < 2) return bad_arg("differentiable substep: bad parameters");
   if (p->rk_order != 2) return bad_arg("differentiable substep: RK2 only");
   if (p->r != 0.0f) return bad_arg("differentiable substep: friction is not supported");
-  if (p->has_land) return bad_arg("differentiable substep: land is not supported");
+
   if (p->one_dimensional) return bad_arg("differentiable substep: one-dimensional mode is not supported");
   g->nx = p->nx;
   g->ny = p->ny;
@@ -406,6 +406,8 @@ int make_geo(const gpo_cdklm16_params* p, uint32_t bc, int32_t pitch_bytes, int3
   g->depth_cutoff = p->depth_cutoff;
   g->flux_slope_eps = p->flux_slope_eps;
   g->flux_balancer = p->flux_balancer;
+  g->has_land = p->has_land ? 1 : 0;
+  g->land = p->land_value;
   g->north_x = p->north_x;
   g->north_y = p->north_y;
   return 0;
@@ -451,7 +453,7 @@ struct AdjBatch {
   int n = 0, k = 0, n_seg = 0;
   int max_w = 0, max_h = 0;  // largest (nx+4), (ny+4) over the members
   int n_work = 0, n_blocks = 0;
-  int f_array = 0, angle0 = 1;      // shared by all members (kernel variant)
+  int has_land = 0, f_array = 0, angle0 = 1;  // shared by all members (kernel variant)
   StepArgs* d_fwd = nullptr;         // 2n: forward state -> stage -> state
   StepArgs* d_traj = nullptr;        // k x 2n: qs[j] -> q1s[j] -> qs[j+1]
   int2* d_work = nullptr;            // (member, strip | chunk << 16)
@@ -472,7 +474,7 @@ int adj_batch_copy(AdjBatch* b, const CopyItem* tab, hipStream_t s) {
 
 int adj_batch_steps(AdjBatch* b, const StepArgs* tab, int rk, hipStream_t s) {
   const hipError_t e =
-      launch_cdklm_batched_exact(tab, b->d_work, b->n_work, rk, b->n_blocks, 0, b->f_array, b->angle0, s);
+      launch_cdklm_batched_exact(tab, b->d_work, b->n_work, rk, b->n_blocks, b->has_land, b->f_array, b->angle0, s);
   return e != hipSuccess ? hip_check(e, "batched substep launch") : 0;
 }
 
@@ -806,9 +808,10 @@ int gpo_adj_batch_create(const gpo_adj_member* members, int32_t n, int32_t k, in
       rc = bad_arg("gpo_adj_batch_create: member buffers, plane or dt");
     if (!rc && (g.bc_n == 3 || g.bc_s == 3 || g.bc_e == 3 || g.bc_w == 3))
       rc = bad_arg("gpo_adj_batch_create: flow-relaxation members are not batched (use the per-member path)");
-    if (m == 0) b->f_array = p.f_is_array, b->angle0 = p.angle_is_zero;
-    if (!rc && (p.f_is_array != b->f_array || p.angle_is_zero != b->angle0 || (p.f_is_array && !M.f_cells)))
-      rc = bad_arg("gpo_adj_batch_create: members differ in coriolis texture or north angle");
+    if (m == 0) b->has_land = p.has_land ? 1 : 0, b->f_array = p.f_is_array, b->angle0 = p.angle_is_zero;
+    if (!rc && ((p.has_land ? 1 : 0) != b->has_land || p.f_is_array != b->f_array ||
+                p.angle_is_zero != b->angle0 || (p.f_is_array && !M.f_cells)))
+      rc = bad_arg("gpo_adj_batch_create: members differ in land, coriolis texture or north angle");
     if (rc) {
       adj_batch_free(b);
       return rc;

@@ -14,10 +14,10 @@
 //    or wrap.
 //
 // Scope of the derivative: RK2 substeps of wet cells (h >= desingularisation
-// eps everywhere), wall and periodic boundaries (flow-relaxation sides read
-// their ghost cells as stored constants; the sponge blend between substeps
-// is handled by the callers), no friction, wind or pressure forcing and no
-// land.  Branches outside that scope set bits in the
+// eps everywhere) and land cells (constants, as the step kernel treats
+// them), wall and periodic boundaries (flow-relaxation sides read their ghost
+// cells as stored constants; the sponge blend between substeps is handled by
+// the callers), no friction, wind or pressure forcing.  Branches outside that scope set bits in the
 // caller's flag word.  Minmod, max and min take the subgradient of the first
 // minimal/maximal argument, the same choice the tangent-linear model makes.
 //
@@ -104,7 +104,14 @@ struct Geo {
   int sp, hp, fp;           // state, Hi and f_cells pitches in elements
   int bc_n, bc_e, bc_s, bc_w;
   double dx, dy, g, f, theta, desing_eps, depth_cutoff, flux_slope_eps, flux_balancer, north_x, north_y;
+  int has_land;  // corner depths equal to `land` are land
+  double land;
 };
+
+// Land sentinels of the step kernel (cdklm_common.h): a land corner depth reads
+// as kDry, a land cell's eta is kDry (CDKLM16_kernel.cu:552-655).
+constexpr float kDry = 1.0e-30f;
+constexpr double kDryTol = 1.0e-3;
 
 // Cell of the input array that supplies stencil cell (r, c): wall ghosts
 // mirror (normal momentum negated), periodic ghosts wrap; only for ghost
@@ -152,13 +159,30 @@ template <class S> inline Fwd<S> carve(void* ws, int nx, int ny) {
   return f;
 }
 
+template <class T> GPO_HD inline T corner(const Geo& g, const T* Hi, int r, int c) {
+  const T h = Hi[(size_t)r * g.hp + c];
+  if (g.has_land && fabs((double)val(h) - g.land) < kDryTol) return T(kDry);
+  return h;
+}
+// Mean of the four corner depths (land corners as kDry): Hm of the update.
 template <class T> GPO_HD inline T cell_depth(const Geo& g, const T* Hi, int r, int c) {
-  const T a = Hi[(size_t)r * g.hp + c], b = Hi[(size_t)(r + 1) * g.hp + c];
-  const T c_ = Hi[(size_t)r * g.hp + c + 1], d = Hi[(size_t)(r + 1) * g.hp + c + 1];
+  const T a = corner(g, Hi, r, c), b = corner(g, Hi, r + 1, c);
+  const T c_ = corner(g, Hi, r, c + 1), d = corner(g, Hi, r + 1, c + 1);
   return T(0.25) * (a + b + c_ + d);
 }
-template <class T> GPO_HD inline T corner(const Geo& g, const T* Hi, int r, int c) {
-  return Hi[(size_t)r * g.hp + c];
+// Depth of the conversion (initHm_kernel.cu:81-100): the mean of the wet
+// corners, `land` when all four are land.
+template <class T> GPO_HD inline T cell_bm(const Geo& g, const T* Hi, int r, int c) {
+  const T a = corner(g, Hi, r, c), b = corner(g, Hi, r + 1, c);
+  const T c_ = corner(g, Hi, r, c + 1), d = corner(g, Hi, r + 1, c + 1);
+  const T S = a + b + c_ + d;
+  if (!g.has_land) return T(0.25) * S;
+  const int wet = 4 - (val(a) == (double)kDry) - (val(b) == (double)kDry) - (val(c_) == (double)kDry) -
+                  (val(d) == (double)kDry);
+  return wet == 0 ? T(g.land) : (wet == 4 ? T(0.25) * S : S / T(wet));
+}
+template <class T> GPO_HD inline bool land_cell(const Geo& g, T bm) {
+  return g.has_land && fabs((double)val(bm) - g.land) <= kDryTol;
 }
 template <class T> GPO_HD inline T fcor(const Geo& g, const T* fc, int r, int c) {
   return fc ? fc[(size_t)r * g.fp + c] : T(g.f);
@@ -182,10 +206,14 @@ GPO_HD void f1_cell(Geo g, const S* in_e, const S* in_u, const S* in_v, const S*
   S e = in_e[k];
   S hu = in_u[k] * S(s.shu);
   S hv = in_v[k] * S(s.shv);
-  const S Hm = cell_depth(g, Hi, r, c);
+  const S Hm = cell_bm(g, Hi, r, c);
   const S h = e + Hm;
   S u, v;
-  if (val(h) < g.desing_eps) {
+  if (land_cell(g, Hm)) {  // land: constants
+    e = S(kDry);
+    u = S(0.0);
+    v = S(0.0);
+  } else if (val(h) < g.desing_eps) {
     if (flag && r >= 1 && r <= g.ny + 2 && c >= 1 && c <= g.nx + 2) GPO_FLAG(flag, 1);
     if (val(h) <= g.depth_cutoff) {
       e = -Hm + S(g.depth_cutoff);
@@ -360,6 +388,15 @@ GPO_HD void xface(const Geo& g, const Fwd<S>& w, const S* Hi, const S* fc, int r
   const S vpn = up * nx_ + vp * ny_, vmn = um * nx_ + vm * ny_;
   const S ap = ebp - (w.kx[p] + S(g.dx) * fcor(g, fc, r, s + 1) * vpn) / g2;
   const S am = ebm + (w.kx[m] + S(g.dx) * fcor(g, fc, r, s) * vmn) / g2;
+  const bool dm = val(ebm) == (double)kDry, dp = val(ebp) == (double)kDry;
+  if (dm || dp) {  // land on one or both sides (CDKLM16_kernel.cu:318-338): the wet side's pressure only
+    F[0] = F[1] = F[2] = S(0.0);
+    if (!(dm && dp)) {
+      const S eta = dm ? tmax(-H, ap) : tmax(-H, am);
+      F[1] = S(0.5) * G * (eta + H) * (eta + H);
+    }
+    return;
+  }
   if (flag && (!(val(ap) > val(-H)) || !(val(am) > val(-H)))) GPO_FLAG(flag, 8);
   const S Qp[3] = {tmax(-H, ap), Rpx, Rpy}, Qm[3] = {tmax(-H, am), Rmx, Rmy};
   flux_fwd(g, Qm, Qp, H, F);
@@ -381,6 +418,15 @@ GPO_HD void yface(const Geo& g, const Fwd<S>& w, const S* Hi, const S* fc, int s
   const S upe = up * ex + vp * ey, ume = um * ex + vm * ey;
   const S ap = ebp - (w.ly[p] - S(g.dy) * fcor(g, fc, s + 1, c) * upe) / g2;
   const S am = ebm + (w.ly[m] - S(g.dy) * fcor(g, fc, s, c) * ume) / g2;
+  const bool dm = val(ebm) == (double)kDry, dp = val(ebp) == (double)kDry;
+  if (dm || dp) {  // land (CDKLM16_kernel.cu:394-416): the wet side's pressure in the normal-momentum flux
+    G3[0] = G3[1] = G3[2] = S(0.0);
+    if (!(dm && dp)) {
+      const S eta = dm ? tmax(-H, ap) : tmax(-H, am);
+      G3[2] = S(0.5) * G * (eta + H) * (eta + H);
+    }
+    return;
+  }
   if (flag && (!(val(ap) > val(-H)) || !(val(am) > val(-H)))) GPO_FLAG(flag, 8);
   const S Qp[3] = {tmax(-H, ap), Rpy, Rpx}, Qm[3] = {tmax(-H, am), Rmy, Rmx};
   S fl[3];
@@ -438,7 +484,7 @@ GPO_HD void f4_cell(Geo g, S dt, int rk_step, const S* Hi, const S* fc, Fwd<S> w
   const S eta_north = R0 + (w.ly[o] - dy * f * R1) / g2;
   S st1 = S(0.0), st2 = S(0.0);
   const S h = R0 + hm;
-  if (val(h) >= g.depth_cutoff) {
+  if (val(h) >= g.depth_cutoff && val(R0) != (double)kDry) {
     const S RHxp = S(0.5) * (corner(g, Hi, r, c + 1) + corner(g, Hi, r + 1, c + 1));
     const S RHxm = S(0.5) * (corner(g, Hi, r, c) + corner(g, Hi, r + 1, c));
     const S RHyp = S(0.5) * (corner(g, Hi, r + 1, c) + corner(g, Hi, r + 1, c + 1));
@@ -469,7 +515,7 @@ GPO_HD void f4_cell(Geo g, S dt, int rk_step, const S* Hi, const S* fc, Fwd<S> w
     uhv = S(0.5) * (qn_v[k] + (hv + dt * L3));
   }
   if (val(ue + hm) <= g.depth_cutoff) {
-    if (flag) GPO_FLAG(flag, 2);
+    if (flag && val(R0) != (double)kDry) GPO_FLAG(flag, 2);  // a wet cell dried (land cells always clamp)
     ue = -hm + S(g.depth_cutoff);
     uhu = S(0.0);
     uhv = S(0.0);
@@ -638,7 +684,16 @@ GPO_HD void a1_xface(const Geo& g, T dt, int rk_step, const T* Hi, const T* fc, 
   const T argm = ebm + (km + T(g.dx) * fm * vmn) / g2;
   const T Qp[3] = {(-H < argp) ? argp : -H, Rpx, Rpy}, Qm[3] = {(-H < argm) ? argm : -H, Rmx, Rmy};
   T lQp[3] = {0, 0, 0}, lQm[3] = {0, 0, 0};
-  flux_rev(g, Qm, Qp, H, lF, lQm, lQp);
+  const bool dm = ebm == T(kDry), dp = ebp == T(kDry);
+  if (dm || dp) {  // land: F.y = g/2 (eta + H)^2 of the wet side's reconstruction
+    if (!(dm && dp)) {
+      T* lw = dm ? lQp : lQm;
+      const T eta = dm ? Qp[0] : Qm[0];
+      lw[0] = G * (eta + H) * lF[1];
+    }
+  } else {
+    flux_rev(g, Qm, Qp, H, lF, lQm, lQp);
+  }
   // the reconstructed-eta sensitivity is zero where the -H clamp was active
   rv.xl[0][o] = (-H < argm) ? lQm[0] : T(0);
   rv.xl[1][o] = lQm[1];
@@ -677,7 +732,16 @@ GPO_HD void a1_yface(const Geo& g, T dt, int rk_step, const T* Hi, const T* fc, 
   const T argm = ebm + (lm - T(g.dy) * fm * ume) / g2;
   const T Qp[3] = {(-H < argp) ? argp : -H, Rpy, Rpx}, Qm[3] = {(-H < argm) ? argm : -H, Rmy, Rmx};
   T lQp[3] = {0, 0, 0}, lQm[3] = {0, 0, 0};
-  flux_rev(g, Qm, Qp, H, lfl, lQm, lQp);
+  const bool dm = ebm == T(kDry), dp = ebp == T(kDry);
+  if (dm || dp) {  // land: G.z = g/2 (eta + H)^2 of the wet side's reconstruction
+    if (!(dm && dp)) {
+      T* lw = dm ? lQp : lQm;
+      const T eta = dm ? Qp[0] : Qm[0];
+      lw[0] = G * (eta + H) * lG[2];
+    }
+  } else {
+    flux_rev(g, Qm, Qp, H, lfl, lQm, lQp);
+  }
   rv.yl[0][o] = (-H < argm) ? lQm[0] : T(0);
   rv.yl[1][o] = lQm[1];
   rv.yl[2][o] = lQm[2];
@@ -910,7 +974,7 @@ GPO_HD CellParts<T> a2_parts(const Geo& g, T dt, int rk_step, const T* Hi, const
     T lhu = coef * a.u, lhv = coef * a.v;
     const T lst1 = coef * dt * a.u, lst2 = coef * dt * a.v;
     const T h = R0 + hm;
-    if (h >= T(g.depth_cutoff)) {
+    if (h >= T(g.depth_cutoff) && R0 != T(kDry)) {
       const T RHxp = T(0.5) * (corner(g, Hi, r, c + 1) + corner(g, Hi, r + 1, c + 1));
       const T RHxm = T(0.5) * (corner(g, Hi, r, c) + corner(g, Hi, r + 1, c));
       const T RHyp = T(0.5) * (corner(g, Hi, r + 1, c) + corner(g, Hi, r + 1, c + 1));
@@ -1053,9 +1117,10 @@ GPO_HD void conv_rev(const Geo& g, const T* Hi, const Fwd<T>& w, const Rev<T>& r
   }
   const Src src = map_cell(g, r, c);
   const T e = in_e[(size_t)src.r * g.sp + src.c];
-  const T h = e + cell_depth(g, Hi, r, c);
+  const T bm = cell_bm(g, Hi, r, c);
+  const T h = e + bm;
   oe = ou = ov = T(0);
-  if (h >= T(g.desing_eps)) {
+  if (!land_cell(g, bm) && h >= T(g.desing_eps)) {
     const T u = w.cu[o], v = w.cv[o];
     ou = l1 / h * T(src.shu);
     ov = l2 / h * T(src.shv);

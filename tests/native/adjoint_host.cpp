@@ -14,7 +14,7 @@ using namespace gpo::adj;
 namespace {
 
 int make_geo(const gpo_cdklm16_params* p, uint32_t bc, Geo* g) {
-  if (!p || p->nx < 2 || p->ny < 2 || p->rk_order != 2 || p->r != 0.0f || p->has_land) return -1;
+  if (!p || p->nx < 2 || p->ny < 2 || p->rk_order != 2 || p->r != 0.0f) return -1;
   g->nx = p->nx;
   g->ny = p->ny;
   g->W = p->nx + 4;
@@ -36,6 +36,8 @@ int make_geo(const gpo_cdklm16_params* p, uint32_t bc, Geo* g) {
   g->flux_balancer = p->flux_balancer;
   g->north_x = p->north_x;
   g->north_y = p->north_y;
+  g->has_land = p->has_land ? 1 : 0;
+  g->land = p->land_value;
   return 0;
 }
 

@@ -1,7 +1,7 @@
 """Batched differentiable ensemble: every member's misfit and gradient, from
 single launches over all members, equal bit for bit those of the member's
 own DifferentiableCDKLM16 run (whose gradient the adjoint tests pin by
-dot-product and finite-difference checks)."""
+dot-product and finite-difference checks), with and without land."""
 import numpy as np
 import pytest
 import torch
@@ -15,15 +15,19 @@ pytestmark = pytest.mark.gpu
 SIZES = [(64, 48), (96, 64), (80, 80), (128, 40), (48, 96)]
 
 
-def _member(gpu_ctx, nx, ny, seed, bc):
+def _member(gpu_ctx, nx, ny, seed, bc, island=False):
     kw = cases.config_c0(n=64)
     rng = np.random.default_rng(seed)
     jj, ii = np.meshgrid(np.arange(ny + 4), np.arange(nx + 4), indexing="ij")
     eta = 0.3 * np.exp(-((ii - nx / 2 - 3) ** 2 + (jj - ny / 2 + 2) ** 2) / (0.02 * nx * ny))
     p = cases.smooth_random_field((ny + 5, nx + 5), 6.0, seed)
+    H = (60.0 + 10.0 * p / np.max(np.abs(p))).astype(np.float32)
+    if island:   # land: corner depths of a rectangle in the western half
+        mask = np.zeros(H.shape, bool)
+        mask[ny // 3:ny // 3 + 5, nx // 5:nx // 5 + 7] = True
+        H = np.ma.array(H, mask=mask)
     kw.update(eta0=eta.astype(np.float32), hu0=np.zeros((ny + 4, nx + 4), np.float32),
-              hv0=np.zeros((ny + 4, nx + 4), np.float32), nx=nx, ny=ny,
-              H=(60.0 + 10.0 * p / np.max(np.abs(p))).astype(np.float32), boundary_conditions=bc)
+              hv0=np.zeros((ny + 4, nx + 4), np.float32), nx=nx, ny=ny, H=H, boundary_conditions=bc)
     sim = CDKLM16(gpu_ctx, **kw)
     x = rng.uniform(2 * sim.dx, (nx - 2) * sim.dx, 12)
     y = rng.uniform(2 * sim.dy, (ny - 2) * sim.dy, 12)
@@ -34,12 +38,12 @@ def _member(gpu_ctx, nx, ny, seed, bc):
     return sim, pos, obs, bg
 
 
-@pytest.mark.parametrize("n_steps,k", [(30, 8), (16, 16), (9, 4)])
-def test_batched_gradients_equal_per_member(gpu_ctx, n_steps, k):
-    print(f"batched adjoint, {n_steps} steps, segments of {k}", flush=True)
+@pytest.mark.parametrize("n_steps,k,island", [(30, 8, False), (16, 16, False), (9, 4, False), (12, 5, True)])
+def test_batched_gradients_equal_per_member(gpu_ctx, n_steps, k, island):
+    print(f"batched adjoint, {n_steps} steps, segments of {k}, land {island}", flush=True)
     bcs = [Common.BoundaryConditions(1, 1, 1, 1), Common.BoundaryConditions(2, 2, 2, 2),
            Common.BoundaryConditions(1, 2, 1, 2)]
-    members = [_member(gpu_ctx, nx, ny, 40 + i, bcs[i % 3]) for i, (nx, ny) in enumerate(SIZES)]
+    members = [_member(gpu_ctx, nx, ny, 40 + i, bcs[i % 3], island) for i, (nx, ny) in enumerate(SIZES)]
     ens = DifferentiableEnsemble([m[0] for m in members], [m[1] for m in members], [m[2] for m in members],
                                  n_steps, checkpoint_every=k)
     J = ens.forward([m[3] for m in members])
