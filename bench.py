@@ -384,12 +384,12 @@ def run_coastal(args):
                        "waves_per_simd": ens.waves_per_simd}}
 
 
-def run_adjoint_batch(n_members=64, n_steps=100, k=20, reps=2):
-    """Batched gradient of 64 members of the coastal sizes (nx 256..768, ny
-    256..512) with the double-jet physics of the stochastic ensemble (walls
-    north/south, periodic east/west; the adjoint covers no land or flow
-    relaxation): 100 RK2 steps, checkpoints every 20, 50 observation points
-    per member; against the same members' gradients one after another."""
+def run_adjoint_batch(n_members=128, n_steps=100, k=20, reps=2):
+    """Batched gradient of the 128 coastal members (nx 256..768, ny 256..512;
+    land in the western 10 %, flow relaxation on the other sides; the
+    deterministic step, without the model error): 100 RK2 steps, checkpoints
+    every 20, 50 observation points per member in the open water; against the
+    same members' gradients one after another."""
     import numpy as np
     import torch
     import cases
@@ -398,10 +398,10 @@ def run_adjoint_batch(n_members=64, n_steps=100, k=20, reps=2):
 
     ctx = Common.DeviceContext(0)
     sims, pos, obs, x0 = [], [], [], []
-    for i, (nx, ny) in enumerate(c4_member_sizes(n_members, seed=4049)):
-        sim = CDKLM16(ctx, **cases.config_c2_member(nx, ny, 9000 + i))
+    for i, (nx, ny) in enumerate(c4_member_sizes(n_members)):
+        sim = CDKLM16(ctx, **cases.config_c4_member(nx, ny, 7000 + i))
         rng = np.random.default_rng(9000 + i)
-        p = np.stack([rng.uniform(2e3, (nx - 2) * 1e3, 50), rng.uniform(2e3, (ny - 2) * 1e3, 50)], 1)
+        p = np.stack([rng.uniform(0.3 * nx * 1e3, 0.85 * nx * 1e3, 50), rng.uniform(0.15 * ny * 1e3, 0.85 * ny * 1e3, 50)], 1)
         sims.append(sim)
         pos.append(p.astype(np.float32))
         obs.append(np.zeros((50, 2), np.float32))
@@ -518,7 +518,7 @@ def main():
         args.seq_steps = 3 if args.seq_steps == 10 else args.seq_steps
         out = {"coastal_c4": run_coastal(args)}
         if not args.no_adjoint:
-            out["adjoint_batched_c4_sizes"] = run_adjoint_batch()
+            out["adjoint_batched_c4"] = run_adjoint_batch()
         print(json.dumps(out), flush=True)
         return
 

@@ -75,6 +75,7 @@ class AdjMember(ctypes.Structure):
         ("Hi", c_void_p), ("Hi_pitch", c_int32), ("f_cells", c_void_p), ("pitch", c_int32),
         ("plane", ctypes.c_int64), ("state", c_void_p), ("stage", c_void_p), ("ckpt", c_void_p),
         ("qs", c_void_p), ("q1s", c_void_p), ("lam", c_void_p), ("workspace", c_void_p),
+        ("has_frs", c_int32), ("frs", Frs),
     ]
 
 

@@ -481,6 +481,9 @@ class DifferentiableEnsemble:
             e.q1s = base + (2 + self.n_seg + self.k + 1) * st
             e.lam = base + (2 + self.n_seg + 2 * self.k + 1) * st
             e.workspace = ws.data_ptr()
+            e.has_frs = int(m.frs is not None)
+            if m.frs is not None:
+                e.frs = m.frs
         self._pitch = [_pitch_elems(m.nx + 4) for m in self.models]
         h = ctypes.c_void_p()
         self.stream.synchronize()

@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "adjoint_core.h"
+#include "boundary_cells.h"
 #include "cdklm_common.h"
 
 namespace gpo {
@@ -206,6 +207,8 @@ struct AdjItem {
   const GPO_GLOBAL T *Hi, *fc;
   Fwd<T> w;
   Rev<T> rv;
+  int frs;  // flow relaxation: lambda in is scaled by the sponge transpose first
+  int sn, se, ss, sw;
 };
 
 template <class T>
@@ -236,6 +239,35 @@ __global__ void a3_fold_b(const AdjItem<T>* __restrict__ tab) {
              (const T*)it.out_e, (const T*)it.le, (const T*)it.lu, (const T*)it.lv, (T*)it.lie, (T*)it.liu,
              (T*)it.liv, (T*)it.lqe, (T*)it.lqu, (T*)it.lqv, blockIdx.y * blockDim.y + threadIdx.y + 2,
              blockIdx.x * blockDim.x + threadIdx.x + 2);
+}
+
+template <class T>
+__global__ void frs_scale_b(const AdjItem<T>* __restrict__ tab) {
+  const AdjItem<T>& it = tab[blockIdx.z];
+  if (!it.frs) return;
+  const int ti = blockIdx.x * blockDim.x + threadIdx.x, tj = blockIdx.y * blockDim.y + threadIdx.y;
+  if (ti >= it.g.nx + 4 || tj >= it.g.ny + 4) return;
+  const Sponge sp{it.sn, it.se, it.ss, it.sw};
+  const size_t k = (size_t)tj * it.g.sp + ti;
+  T* a = (T*)it.le;
+  T* b = (T*)it.lu;
+  T* c = (T*)it.lv;
+  bool in_ew, in_ns;
+  const float aew = frs_alpha_ew(it.g, sp, ti, &in_ew), ans = frs_alpha_ns(it.g, sp, tj, &in_ns);
+  if (in_ew) {
+    const T f = T(1.0f - aew);
+    a[k] *= f, b[k] *= f, c[k] *= f;
+  }
+  if (in_ns) {
+    const T f = T(1.0f - ans);
+    a[k] *= f, b[k] *= f, c[k] *= f;
+  }
+}
+
+// Ghost-cell passes of flow-relaxation members on one state each (block row
+// y = item y; members without flow relaxation have no item).
+__global__ void __launch_bounds__(256) bc_targets_b(const cells::BcTarget* __restrict__ tab, int pass) {
+  cells::bc_target_cell(tab[blockIdx.y], pass, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // Interior copy of one state per member: dst[m] = src[m] (3 planes each).
@@ -459,10 +491,15 @@ struct AdjBatch {
   int2* d_work = nullptr;            // (member, strip | chunk << 16)
   AdjItem<float>* d_items = nullptr;  // (2j + rk) x n
   CopyItem* d_copy = nullptr;        // save (n_seg x n), restore (n_seg x n), lambda parity (n)
+  // flow-relaxation members (n_frs of them): ghost-cell passes on the forward
+  // stage, the forward state, then per step j the stage and the new state
+  cells::BcTarget* d_bc = nullptr;   // (2 + 2k) x n_frs
+  int n_frs = 0, bc_blocks = 0;
 };
 
 void adj_batch_free(AdjBatch* b) {
-  for (void* p : {(void*)b->d_fwd, (void*)b->d_traj, (void*)b->d_work, (void*)b->d_items, (void*)b->d_copy})
+  for (void* p : {(void*)b->d_fwd, (void*)b->d_traj, (void*)b->d_work, (void*)b->d_items, (void*)b->d_copy,
+                  (void*)b->d_bc})
     if (p) (void)hipFree(p);
   delete b;
 }
@@ -478,8 +515,21 @@ int adj_batch_steps(AdjBatch* b, const StepArgs* tab, int rk, hipStream_t s) {
   return e != hipSuccess ? hip_check(e, "batched substep launch") : 0;
 }
 
+// Ghost-cell passes (north/south, then east/west) of the flow-relaxation
+// members on table slot `slot` (0: forward stage, 1: forward state, 2 + 2j:
+// stage of step j, 3 + 2j: state after step j).
+int adj_batch_bc(AdjBatch* b, int slot, hipStream_t s) {
+  if (b->n_frs == 0) return 0;
+  const cells::BcTarget* tab = b->d_bc + (size_t)slot * b->n_frs;
+  for (int pass = 0; pass < 2; ++pass)
+    hipLaunchKernelGGL(bc_targets_b, dim3(b->bc_blocks, b->n_frs), dim3(256), 0, s, tab, pass);
+  return hip_check(hipGetLastError(), "batched boundary launch");
+}
+
 int adj_batch_stage(AdjBatch* b, const AdjItem<float>* tab, int* flag, hipStream_t s) {
   const int n = b->n;
+  if (b->n_frs)
+    hipLaunchKernelGGL(frs_scale_b<float>, dim3((b->max_w + 63) / 64, (b->max_h + 3) / 4, n), kBlock, 0, s, tab);
   hipLaunchKernelGGL(f1_convert_b<float>, dim3((b->max_w + 63) / 64, (b->max_h + 3) / 4, n), kBlock, 0, s, tab,
                      flag);
   hipLaunchKernelGGL(a1_faces_b<float>, dim3((b->max_w + 63) / 64, (b->max_h + kSlopeRows - 1) / kSlopeRows, n),
@@ -798,6 +848,8 @@ int gpo_adj_batch_create(const gpo_adj_member* members, int32_t n, int32_t k, in
   std::vector<AdjItem<float>> items(2 * (size_t)n * k);
   std::vector<CopyItem> copies((2 * (size_t)n_segments + 1) * n);
   std::vector<int2> work;
+  std::vector<std::vector<cells::BcTarget>> bcs(2 + 2 * (size_t)k);  // per slot, one per flow-relaxation member
+  long long max_bc_cells = 0;
   for (int m = 0; m < n; ++m) {
     const gpo_adj_member& M = members[m];
     gpo_cdklm16_params p = M.params;
@@ -806,8 +858,8 @@ int gpo_adj_batch_create(const gpo_adj_member* members, int32_t n, int32_t k, in
     if (!rc && (!M.state || !M.stage || !M.ckpt || !M.qs || !M.q1s || !M.lam || !M.workspace || !M.Hi ||
                 M.plane < (int64_t)M.pitch * (p.ny + 4) || !(M.dt > 0.0f)))
       rc = bad_arg("gpo_adj_batch_create: member buffers, plane or dt");
-    if (!rc && (g.bc_n == 3 || g.bc_s == 3 || g.bc_e == 3 || g.bc_w == 3))
-      rc = bad_arg("gpo_adj_batch_create: flow-relaxation members are not batched (use the per-member path)");
+    const bool frs = g.bc_n == 3 || g.bc_s == 3 || g.bc_e == 3 || g.bc_w == 3;
+    if (!rc && frs && !M.has_frs) rc = bad_arg("gpo_adj_batch_create: flow-relaxation member without frs data");
     if (m == 0) b->has_land = p.has_land ? 1 : 0, b->f_array = p.f_is_array, b->angle0 = p.angle_is_zero;
     if (!rc && ((p.has_land ? 1 : 0) != b->has_land || p.f_is_array != b->f_array ||
                 p.angle_is_zero != b->angle0 || (p.f_is_array && !M.f_cells)))
@@ -824,6 +876,36 @@ int gpo_adj_batch_create(const gpo_adj_member* members, int32_t n, int32_t k, in
     float *se, *su, *sv, *ge, *gu, *gv;
     planes(M.state, 0, &se, &su, &sv);
     planes(M.stage, 0, &ge, &gu, &gv);
+    cells::BcTarget bt{};
+    if (frs) {
+      bt.nx = p.nx, bt.ny = p.ny, bt.bn = g.bc_n, bt.be = g.bc_e, bt.bs = g.bc_s, bt.bw = g.bc_w;
+      bt.sn = M.frs.sponge[0], bt.se = M.frs.sponge[1], bt.ss = M.frs.sponge[2], bt.sw = M.frs.sponge[3];
+      cells::bc_target_extent(&bt);
+      bt.pitch = M.pitch;
+      bt.ns_cur = M.frs.ns_cur, bt.ns_next = M.frs.ns_next, bt.ns_nx = M.frs.ns_nx;
+      bt.ew_cur = M.frs.ew_cur, bt.ew_next = M.frs.ew_next, bt.ew_ny = M.frs.ew_ny;
+      bt.frs_t = M.frs.t;
+      if (((bt.bn == 3 || bt.bs == 3) && bt.bn != 2 && (!bt.ns_cur || !bt.ns_next || bt.ns_nx < 1)) ||
+          ((bt.be == 3 || bt.bw == 3) && bt.be != 2 && (!bt.ew_cur || !bt.ew_next || bt.ew_ny < 1))) {
+        adj_batch_free(b);
+        return bad_arg("gpo_adj_batch_create: flow-relaxation data missing");
+      }
+      max_bc_cells = std::max(max_bc_cells, cells::bc_target_cells(bt));
+      auto put = [&](size_t slot, float* e_, float* u_, float* v_) {
+        cells::BcTarget t = bt;
+        t.e = e_, t.u = u_, t.v = v_;
+        bcs[slot].push_back(t);
+      };
+      put(0, ge, gu, gv);
+      put(1, se, su, sv);
+      for (int j = 0; j < k; ++j) {
+        float *be_, *bu_, *bv_, *ce_, *cu_, *cv_;
+        planes(M.q1s, j, &be_, &bu_, &bv_);
+        planes(M.qs, j + 1, &ce_, &cu_, &cv_);
+        put(2 + 2 * (size_t)j, be_, bu_, bv_);
+        put(3 + 2 * (size_t)j, ce_, cu_, cv_);
+      }
+    }
     rc = make_step_args(&p, M.dt, 0, se, su, sv, nullptr, nullptr, nullptr, ge, gu, gv, M.pitch, M.Hi, M.Hi_pitch,
                         M.f_cells, nullptr, M.boundary_conditions, &fwd[2 * m]);
     if (!rc)
@@ -854,6 +936,8 @@ int gpo_adj_batch_create(const gpo_adj_member* members, int32_t n, int32_t k, in
         it.fc = gptr(M.f_cells);
         it.w = w;
         it.rv = rv;
+        it.frs = frs ? 1 : 0;
+        if (frs) it.sn = M.frs.sponge[0], it.se = M.frs.sponge[1], it.ss = M.frs.sponge[2], it.sw = M.frs.sponge[3];
         float* lo = rk ? buf[(j + 1) & 1] : l1;
         float* li = rk ? l1 : buf[j & 1];
         it.le = gptr(lo), it.lu = gptr(lo + pl), it.lv = gptr(lo + 2 * pl);
@@ -915,6 +999,17 @@ int gpo_adj_batch_create(const gpo_adj_member* members, int32_t n, int32_t k, in
     rc = hip_check(hipMemcpyAsync(b->d_copy, copies.data(), sizeof(CopyItem) * copies.size(), hipMemcpyHostToDevice,
                                   s),
                    "gpo_adj_batch_create copy");
+  b->n_frs = (int)bcs[0].size();
+  if (!rc && b->n_frs > 0) {
+    std::vector<cells::BcTarget> flat;
+    for (const auto& v : bcs) flat.insert(flat.end(), v.begin(), v.end());
+    b->bc_blocks = (int)std::max<long long>(1, (max_bc_cells + 255) / 256);
+    rc = hip_check(hipMalloc(&b->d_bc, sizeof(cells::BcTarget) * flat.size()), "gpo_adj_batch_create hipMalloc");
+    if (!rc)
+      rc = hip_check(hipMemcpyAsync(b->d_bc, flat.data(), sizeof(cells::BcTarget) * flat.size(),
+                                    hipMemcpyHostToDevice, s),
+                     "gpo_adj_batch_create copy");
+  }
   if (!rc) rc = hip_check(hipStreamSynchronize(s), "gpo_adj_batch_create sync");
   if (rc) {
     adj_batch_free(b);
@@ -930,7 +1025,9 @@ int gpo_adj_batch_forward(gpo_adj_batch h, int32_t n_steps, void* stream) {
   hipStream_t s = (hipStream_t)stream;
   for (int i = 0; i < n_steps; ++i) {
     int rc = adj_batch_steps(b, b->d_fwd, 0, s);
+    if (!rc) rc = adj_batch_bc(b, 0, s);
     if (!rc) rc = adj_batch_steps(b, b->d_fwd, 1, s);
+    if (!rc) rc = adj_batch_bc(b, 1, s);
     if (rc) return rc;
   }
   return 0;
@@ -950,7 +1047,9 @@ int gpo_adj_batch_trajectory(gpo_adj_batch h, int32_t m_steps, void* stream) {
   for (int j = 0; j < m_steps; ++j) {
     const StepArgs* t = b->d_traj + (size_t)j * 2 * b->n;
     int rc = adj_batch_steps(b, t, 0, s);
+    if (!rc) rc = adj_batch_bc(b, 2 + 2 * j, s);
     if (!rc) rc = adj_batch_steps(b, t, 1, s);
+    if (!rc) rc = adj_batch_bc(b, 3 + 2 * j, s);
     if (rc) return rc;
   }
   return 0;

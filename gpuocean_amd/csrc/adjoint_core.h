@@ -13,9 +13,9 @@
 //    A3 fold of ghost-cell sensitivities onto the interior cells they mirror
 //    or wrap.
 //
-// Scope of the derivative: RK2 substeps of wet cells (h >= desingularisation
-// eps everywhere) and land cells (constants, as the step kernel treats
-// them), wall and periodic boundaries (flow-relaxation sides read their ghost
+// Scope of the derivative: RK2 substeps of wet cells (desingularised below
+// eps, dried to a constant state at the depth cutoff) and land cells
+// (constants, as the step kernel treats them), wall and periodic boundaries (flow-relaxation sides read their ghost
 // cells as stored constants; the sponge blend between substeps is handled by
 // the callers), no friction, wind or pressure forcing.  Branches outside that scope set bits in the
 // caller's flag word.  Minmod, max and min take the subgradient of the first
@@ -214,7 +214,6 @@ GPO_HD void f1_cell(Geo g, const S* in_e, const S* in_u, const S* in_v, const S*
     u = S(0.0);
     v = S(0.0);
   } else if (val(h) < g.desing_eps) {
-    if (flag && r >= 1 && r <= g.ny + 2 && c >= 1 && c <= g.nx + 2) GPO_FLAG(flag, 1);
     if (val(h) <= g.depth_cutoff) {
       e = -Hm + S(g.depth_cutoff);
       u = S(0.0);
@@ -1120,11 +1119,20 @@ GPO_HD void conv_rev(const Geo& g, const T* Hi, const Fwd<T>& w, const Rev<T>& r
   const T bm = cell_bm(g, Hi, r, c);
   const T h = e + bm;
   oe = ou = ov = T(0);
-  if (!land_cell(g, bm) && h >= T(g.desing_eps)) {
-    const T u = w.cu[o], v = w.cv[o];
+  if (land_cell(g, bm) || !(h > T(g.depth_cutoff))) return;  // land, or dried to a constant state
+  const T u = w.cu[o], v = w.cv[o];
+  if (h >= T(g.desing_eps)) {
     ou = l1 / h * T(src.shu);
     ov = l2 / h * T(src.shv);
     oe = l0 - (l1 * u + l2 * v) / h;
+  } else {
+    // desingularised: u = hu / a with a = h^2 / (2 eps) + eps / 2 (between the
+    // depth cutoff and eps, min(a, eps) = a and max(a, |h|) = a), da/dh = h / eps
+    const T eps = T(g.desing_eps);
+    const T a = h * h / T(2.0 * g.desing_eps) + T(0.5 * g.desing_eps);
+    ou = l1 / a * T(src.shu);
+    ov = l2 / a * T(src.shv);
+    oe = l0 - (l1 * u + l2 * v) * (h / eps) / a;
   }
 }
 

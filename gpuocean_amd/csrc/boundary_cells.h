@@ -103,5 +103,70 @@ __device__ inline void frs_ew_cell(int bc_e, int bc_w, int nx, int ny, int se, i
   vr[ti] = (1.0f - alpha) * vr[ti] + alpha * ev;
 }
 
+// The ghost-cell passes of gpo_boundary_conditions for one state of one
+// member, as lists of cells: the north/south pass (periodic rows, or the
+// flow-relaxation sponge rows: the first lo and the last hi rows, all
+// columns) and the east/west pass (likewise columns).
+struct BcTarget {
+  int nx, ny, bn, be, bs, bw;
+  int sn, se, ss, sw;
+  int ns_rows_lo, ns_rows_hi, ew_cols_lo, ew_cols_hi;
+  float *e, *u, *v;
+  int pitch;
+  const float *ns_cur, *ns_next, *ew_cur, *ew_next;
+  int ns_nx, ew_ny;
+  float frs_t;
+};
+
+// Rows and columns each pass writes: periodic when N (E) is 2, else flow
+// relaxation when N or S (E or W) is 3, else none (walls).
+inline void bc_target_extent(BcTarget* b) {
+  b->ns_rows_lo = b->ns_rows_hi = b->ew_cols_lo = b->ew_cols_hi = 0;
+  if (b->bn == 2) {
+    b->ns_rows_lo = b->ns_rows_hi = 2;
+  } else if (b->bn == 3 || b->bs == 3) {
+    const int lo = b->bs == 3 ? (b->ss > 0 ? b->ss : 0) : 0, hi = b->bn == 3 ? (b->sn > 0 ? b->sn : 0) : 0;
+    if (lo + hi >= b->ny + 4) b->ns_rows_lo = b->ny + 4;
+    else b->ns_rows_lo = lo, b->ns_rows_hi = hi;
+  }
+  if (b->be == 2) {
+    b->ew_cols_lo = b->ew_cols_hi = 2;
+  } else if (b->be == 3 || b->bw == 3) {
+    const int lo = b->bw == 3 ? (b->sw > 0 ? b->sw : 0) : 0, hi = b->be == 3 ? (b->se > 0 ? b->se : 0) : 0;
+    if (lo + hi >= b->nx + 4) b->ew_cols_lo = b->nx + 4;
+    else b->ew_cols_lo = lo, b->ew_cols_hi = hi;
+  }
+}
+inline long long bc_target_cells(const BcTarget& b) {
+  const long long ns = (long long)(b.ns_rows_lo + b.ns_rows_hi) * (b.nx + 4);
+  const long long ew = (long long)(b.ew_cols_lo + b.ew_cols_hi) * (b.ny + 4);
+  return ns > ew ? ns : ew;
+}
+
+// Cell k of pass `pass` (0: north/south, 1: east/west) of target b.
+__device__ inline void bc_target_cell(const BcTarget& b, int pass, int k) {
+  if (pass == 0) {
+    const int w = b.nx + 4, rows = b.ns_rows_lo + b.ns_rows_hi;
+    if (k >= rows * w) return;
+    const int r = k / w, ti = k - r * w;
+    const int tj = r < b.ns_rows_lo ? r : (b.ny + 4 - b.ns_rows_hi) + (r - b.ns_rows_lo);
+    if (b.bn == 2)
+      periodic_ns_cell(b.nx, b.ny, b.e, b.u, b.v, b.pitch, ti, tj < 2 ? tj : tj - b.ny);
+    else
+      frs_ns_cell(b.bn, b.bs, b.nx, b.ny, b.sn, b.ss, b.e, b.u, b.v, b.pitch, b.ns_cur, b.ns_next, b.ns_nx, b.frs_t, ti,
+                  tj);
+  } else {
+    const int hgt = b.ny + 4, cols = b.ew_cols_lo + b.ew_cols_hi;
+    if (k >= cols * hgt) return;
+    const int c = k / hgt, tj = k - c * hgt;
+    const int ti = c < b.ew_cols_lo ? c : (b.nx + 4 - b.ew_cols_hi) + (c - b.ew_cols_lo);
+    if (b.be == 2)
+      periodic_ew_cell(b.nx, b.ny, b.e, b.u, b.v, b.pitch, tj, ti < 2 ? ti : ti - b.nx);
+    else
+      frs_ew_cell(b.be, b.bw, b.nx, b.ny, b.se, b.sw, b.e, b.u, b.v, b.pitch, b.ew_cur, b.ew_next, b.ew_ny, b.frs_t, ti,
+                  tj);
+  }
+}
+
 }  // namespace cells
 }  // namespace gpo

@@ -133,20 +133,11 @@ __global__ void advance_draws(uint32_t* draw, int n) {
   if (i < n) draw[i] = draw[i] + 1u;
 }
 
-// Ghost cells of one member, as gpo_boundary_conditions writes them: the
-// north/south pass (periodic rows, or the flow-relaxation sponge rows) and
-// the east/west pass, each over a list of cells of its own.
+// Ghost cells of one member, as gpo_boundary_conditions writes them, on its
+// state (h0) and RK stage (h1).
 struct BcMember {
-  int nx, ny, bn, be, bs, bw;
-  int sn, se, ss, sw;
-  int frs;            // some side is flow relaxation: ghost cells after substep 0 too
-  int ns_rows_lo, ns_rows_hi;  // north/south pass: the first lo and the last hi rows (all columns)
-  int ew_cols_lo, ew_cols_hi;  // east/west pass: the first lo and the last hi columns (all rows)
-  float *h0e, *h0u, *h0v, *h1e, *h1u, *h1v;
-  int pitch;
-  const float *ns_cur, *ns_next, *ew_cur, *ew_next;
-  int ns_nx, ew_ny;
-  float frs_t;
+  cells::BcTarget h0, h1;
+  int frs;  // some side is flow relaxation: ghost cells after substep 0 too
 };
 
 // pass 0: north/south, pass 1: east/west; target 0: h0, 1: h1 (members with
@@ -154,31 +145,7 @@ struct BcMember {
 __global__ void __launch_bounds__(256) bc_batched(const BcMember* __restrict__ members, int pass, int target) {
   const BcMember& b = members[blockIdx.y];
   if (target == 1 && !b.frs) return;
-  float* h = target ? b.h1e : b.h0e;
-  float* u = target ? b.h1u : b.h0u;
-  float* v = target ? b.h1v : b.h0v;
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (pass == 0) {
-    const int w = b.nx + 4, rows = b.ns_rows_lo + b.ns_rows_hi;
-    if (k >= rows * w) return;
-    const int r = k / w, ti = k - r * w;
-    const int tj = r < b.ns_rows_lo ? r : (b.ny + 4 - b.ns_rows_hi) + (r - b.ns_rows_lo);
-    if (b.bn == 2)
-      cells::periodic_ns_cell(b.nx, b.ny, h, u, v, b.pitch, ti, tj < 2 ? tj : tj - b.ny);
-    else
-      cells::frs_ns_cell(b.bn, b.bs, b.nx, b.ny, b.sn, b.ss, h, u, v, b.pitch, b.ns_cur, b.ns_next, b.ns_nx, b.frs_t,
-                         ti, tj);
-  } else {
-    const int hgt = b.ny + 4, cols = b.ew_cols_lo + b.ew_cols_hi;
-    if (k >= cols * hgt) return;
-    const int c = k / hgt, tj = k - c * hgt;
-    const int ti = c < b.ew_cols_lo ? c : (b.nx + 4 - b.ew_cols_hi) + (c - b.ew_cols_lo);
-    if (b.be == 2)
-      cells::periodic_ew_cell(b.nx, b.ny, h, u, v, b.pitch, tj, ti < 2 ? ti : ti - b.nx);
-    else
-      cells::frs_ew_cell(b.be, b.bw, b.nx, b.ny, b.se, b.sw, h, u, v, b.pitch, b.ew_cur, b.ew_next, b.ew_ny,
-                         b.frs_t, ti, tj);
-  }
+  cells::bc_target_cell(target ? b.h1 : b.h0, pass, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // Drifters of one member; the time step is the member's current one.
@@ -376,51 +343,36 @@ int gpo_ensemble_create(const gpo_member* members, int32_t n, int32_t chunk_rows
     draws[m] = M.draw;
     if (M.bc_each_step) {
       BcMember b{};
-      b.nx = p.nx;
-      b.ny = p.ny;
-      b.bn = (M.boundary_conditions >> 24) & 0xFF;
-      b.bs = (M.boundary_conditions >> 16) & 0xFF;
-      b.be = (M.boundary_conditions >> 8) & 0xFF;
-      b.bw = M.boundary_conditions & 0xFF;
-      b.sn = M.sponge[0];
-      b.se = M.sponge[1];
-      b.ss = M.sponge[2];
-      b.sw = M.sponge[3];
+      cells::BcTarget& t = b.h0;
+      t.nx = p.nx;
+      t.ny = p.ny;
+      t.bn = (M.boundary_conditions >> 24) & 0xFF;
+      t.bs = (M.boundary_conditions >> 16) & 0xFF;
+      t.be = (M.boundary_conditions >> 8) & 0xFF;
+      t.bw = M.boundary_conditions & 0xFF;
+      t.sn = M.sponge[0];
+      t.se = M.sponge[1];
+      t.ss = M.sponge[2];
+      t.sw = M.sponge[3];
+      cells::bc_target_extent(&t);
+      if ((t.bn == 3 || t.bs == 3) && t.bn != 2 && (!M.frs_ns_cur || !M.frs_ns_next || M.frs_ns_nx < 1)) {
+        ensemble_free(e);
+        return bad("gpo_ensemble_create: flow-relaxation north/south data missing");
+      }
+      if ((t.be == 3 || t.bw == 3) && t.be != 2 && (!M.frs_ew_cur || !M.frs_ew_next || M.frs_ew_ny < 1)) {
+        ensemble_free(e);
+        return bad("gpo_ensemble_create: flow-relaxation east/west data missing");
+      }
+      t.pitch = M.pitch;
+      t.ns_cur = M.frs_ns_cur, t.ns_next = M.frs_ns_next, t.ns_nx = M.frs_ns_nx;
+      t.ew_cur = M.frs_ew_cur, t.ew_next = M.frs_ew_next, t.ew_ny = M.frs_ew_ny;
+      t.frs_t = M.frs_t;
+      b.h1 = t;
+      t.e = M.h0_eta, t.u = M.h0_hu, t.v = M.h0_hv;
+      b.h1.e = M.h1_eta, b.h1.u = M.h1_hu, b.h1.v = M.h1_hv;
       b.frs = frs;
-      // rows and columns each pass writes (the pass of gpo_boundary_conditions
-      // runs periodic when N (E) is 2, else flow relaxation when N or S (E or W) is 3)
-      if (b.bn == 2) {
-        b.ns_rows_lo = 2, b.ns_rows_hi = 2;
-      } else if (b.bn == 3 || b.bs == 3) {
-        const int lo = b.bs == 3 ? std::max(0, b.ss) : 0, hi = b.bn == 3 ? std::max(0, b.sn) : 0;
-        if (lo + hi >= p.ny + 4) b.ns_rows_lo = p.ny + 4, b.ns_rows_hi = 0;
-        else b.ns_rows_lo = lo, b.ns_rows_hi = hi;
-        if ((b.bn == 3 || b.bs == 3) && (!M.frs_ns_cur || !M.frs_ns_next || M.frs_ns_nx < 1)) {
-          ensemble_free(e);
-          return bad("gpo_ensemble_create: flow-relaxation north/south data missing");
-        }
-      }
-      if (b.be == 2) {
-        b.ew_cols_lo = 2, b.ew_cols_hi = 2;
-      } else if (b.be == 3 || b.bw == 3) {
-        const int lo = b.bw == 3 ? std::max(0, b.sw) : 0, hi = b.be == 3 ? std::max(0, b.se) : 0;
-        if (lo + hi >= p.nx + 4) b.ew_cols_lo = p.nx + 4, b.ew_cols_hi = 0;
-        else b.ew_cols_lo = lo, b.ew_cols_hi = hi;
-        if (!M.frs_ew_cur || !M.frs_ew_next || M.frs_ew_ny < 1) {
-          ensemble_free(e);
-          return bad("gpo_ensemble_create: flow-relaxation east/west data missing");
-        }
-      }
-      b.h0e = M.h0_eta, b.h0u = M.h0_hu, b.h0v = M.h0_hv;
-      b.h1e = M.h1_eta, b.h1u = M.h1_hu, b.h1v = M.h1_hv;
-      b.pitch = M.pitch;
-      b.ns_cur = M.frs_ns_cur, b.ns_next = M.frs_ns_next, b.ns_nx = M.frs_ns_nx;
-      b.ew_cur = M.frs_ew_cur, b.ew_next = M.frs_ew_next, b.ew_ny = M.frs_ew_ny;
-      b.frs_t = M.frs_t;
       bcs.push_back(b);
-      const long long cells = std::max((long long)(b.ns_rows_lo + b.ns_rows_hi) * (p.nx + 4),
-                                       (long long)(b.ew_cols_lo + b.ew_cols_hi) * (p.ny + 4));
-      max_bc_cells = std::max(max_bc_cells, cells);
+      max_bc_cells = std::max(max_bc_cells, cells::bc_target_cells(t));
       e->any_frs |= frs;
     }
     if (M.drifters) {

@@ -391,8 +391,8 @@ int gpo_frs_apply_f64(const gpo_cdklm16_params* p, uint32_t boundary_conditions,
 
 /* ---- batched differentiable ensemble: many members' checkpointed forward,
  * trajectory recomputation and reverse sweep, one launch per stage for all
- * members (no counterpart in the reference).  Members: RK2, wall and periodic
- * boundaries, no land or friction; exact build.  Every state is three planes
+ * members (no counterpart in the reference).  Members: RK2, wall, periodic
+ * and flow-relaxation boundaries, land, no friction; exact build.  Every state is three planes
  * (eta, hu, hv) of (ny+4) rows x pitch bytes, `plane` bytes apart; a
  * buffer of several states holds them one after another (3 planes each). */
 typedef struct {
@@ -411,6 +411,8 @@ typedef struct {
   float* q1s;                      /* k RK stages of that segment */
   float* lam;                      /* 3 states: lambda, then scratch */
   void* workspace;                 /* gpo_adjoint_workspace_bytes(nx, ny, 4) */
+  int32_t has_frs;                 /* some side is flow relaxation (3): frs is used */
+  gpo_frs frs;
 } gpo_adj_member;
 
 typedef void* gpo_adj_batch;

@@ -36,7 +36,8 @@ def _p(a):
 class Model:
     """A small CDKLM16 configuration evaluated with the host build."""
 
-    def __init__(self, nx=24, ny=20, bc=Common.BoundaryConditions(1, 1, 1, 1), seed=4, f=1.2e-4, island=False):
+    def __init__(self, nx=24, ny=20, bc=Common.BoundaryConditions(1, 1, 1, 1), seed=4, f=1.2e-4, island=False,
+                 shelf=False):
         kw = cases.config_c3(n=max(nx, ny), seed=seed)
         n = max(nx, ny)
         kw = dict(kw, nx=nx, ny=ny, f=f, boundary_conditions=bc)
@@ -45,10 +46,16 @@ class Model:
             mask = np.zeros(kw["H"].shape, bool)
             mask[ny // 2 - 2:ny // 2 + 3, nx // 2 - 3:nx // 2 + 2] = True
             kw["H"] = np.ma.array(kw["H"], mask=mask)
+        if shelf:    # a band of water shallower than the desingularisation depth (0.1 m)
+            kw["H"][:, 3:7] = 0.06
         for k in ("eta0", "hu0", "hv0"):
             kw[k] = kw[k][:ny + 4, :nx + 4].copy()
         bg, _ = cases.c3_background_and_truth(dict(kw, eta0=kw["eta0"]), seed=seed)
         self.bg = [np.asarray(a, np.float64) for a in bg]
+        if shelf:    # at rest with tiny momenta over the shelf, so that 0.06 + eta stays in (cutoff, 0.1)
+            self.bg[0][:, 2:9] = 0.0
+            self.bg[1][:, 2:9] *= 1e-3
+            self.bg[2][:, 2:9] *= 1e-3
         ora = O.OracleCDKLM16(**kw, flux_balancer=float(np.float32(0.8)), precision="f64")
         self.kw, self.ora = kw, ora
         self.Hi = np.ascontiguousarray(ora.Hi, np.float64)
@@ -161,9 +168,9 @@ def test_host_forward_matches_f64_oracle(bc, island):
 
 
 @pytest.mark.parametrize("bc", BCS, ids=IDS)
-@pytest.mark.parametrize("island", [False, True], ids=["", "island"])
-def test_host_dot_product(bc, island):
-    m = Model(bc=bc, island=island)
+@pytest.mark.parametrize("island,shelf", [(False, False), (True, False), (False, True)], ids=["", "island", "shelf"])
+def test_host_dot_product(bc, island, shelf):
+    m = Model(bc=bc, island=island, shelf=shelf)
     Qs, Q1s = m.trajectory([a.copy() for a in m.bg], 4)
     rng = np.random.default_rng(11)
     v = [_interior_noise(m.shape, rng, s) for s in (1e-3, 1e-1, 1e-1)]
@@ -195,21 +202,24 @@ def test_host_single_substep_transposes(bc, island):
 
 
 @pytest.mark.parametrize("bc", BCS[:2], ids=IDS[:2])
-@pytest.mark.parametrize("island", [False, True], ids=["", "island"])
-def test_host_adjoint_against_finite_differences(bc, island):
+@pytest.mark.parametrize("island,shelf", [(False, False), (True, False), (False, True)], ids=["", "island", "shelf"])
+def test_host_adjoint_against_finite_differences(bc, island, shelf):
     """J = <w, Q^n>: the adjoint's <lambda_0, d> against central differences."""
-    m = Model(bc=bc, island=island)
+    m = Model(bc=bc, island=island, shelf=shelf)
     n = 5
     rng = np.random.default_rng(3)
     w = [_interior_noise(m.shape, rng, 1.0) for _ in range(3)]
     Qs, Q1s = m.trajectory([a.copy() for a in m.bg], n)
     lam0 = m.adjoint(Qs, Q1s, w)
-    # The coast adds limiter kinks (adjustSlopes next to near-zero face depths)
-    # that a white-noise step can cross; there the best of several step sizes
-    # is compared (the function is smooth between kinks).
-    steps, tol = ((1e-6, 3e-7, 1e-7, 3e-8), 1e-5) if island else ((1e-6,), 1e-6)
+    # A coast or a shelf adds limiter kinks (adjustSlopes next to near-zero
+    # face depths) that a white-noise step can cross; there the best of
+    # several step sizes is compared (the function is smooth between kinks).
+    steps, tol = ((1e-6, 3e-7, 1e-7, 3e-8), 1e-5) if (island or shelf) else ((1e-6,), 1e-6)
     for _ in range(3):
         d = [_interior_noise(m.shape, rng, s) for s in (1e-2, 1.0, 1.0)]
+        if shelf:   # perturbations in proportion to the shelf's tiny depth and momenta
+            for a in d:
+                a[:, 2:9] *= 1e-3
         ad = _dot(lam0, d)
         errs = []
         for eps in steps:

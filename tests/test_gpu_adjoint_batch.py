@@ -26,8 +26,14 @@ def _member(gpu_ctx, nx, ny, seed, bc, island=False):
         mask = np.zeros(H.shape, bool)
         mask[ny // 3:ny // 3 + 5, nx // 5:nx // 5 + 7] = True
         H = np.ma.array(H, mask=mask)
-    kw.update(eta0=eta.astype(np.float32), hu0=np.zeros((ny + 4, nx + 4), np.float32),
-              hv0=np.zeros((ny + 4, nx + 4), np.float32), nx=nx, ny=ny, H=H, boundary_conditions=bc)
+    eta, hu, hv = eta.astype(np.float32), np.zeros((ny + 4, nx + 4), np.float32), np.zeros((ny + 4, nx + 4), np.float32)
+    kw.update(eta0=eta, hu0=hu, hv0=hv, nx=nx, ny=ny, H=H, boundary_conditions=bc)
+    if any(c == 3 for c in (bc.north, bc.east, bc.south, bc.west)):   # exterior: the initial boundary rows
+        def side(sl):
+            return Common.SingleBoundaryConditionData(h=[eta[sl].copy()], hu=[hu[sl].copy()], hv=[hv[sl].copy()])
+        kw["boundary_conditions_data"] = Common.BoundaryConditionsData(
+            north=side(np.s_[ny + 1, :]), south=side(np.s_[2, :]), east=side(np.s_[:, nx + 1]),
+            west=side(np.s_[:, 2]))
     sim = CDKLM16(gpu_ctx, **kw)
     x = rng.uniform(2 * sim.dx, (nx - 2) * sim.dx, 12)
     y = rng.uniform(2 * sim.dy, (ny - 2) * sim.dy, 12)
@@ -38,11 +44,18 @@ def _member(gpu_ctx, nx, ny, seed, bc, island=False):
     return sim, pos, obs, bg
 
 
-@pytest.mark.parametrize("n_steps,k,island", [(30, 8, False), (16, 16, False), (9, 4, False), (12, 5, True)])
-def test_batched_gradients_equal_per_member(gpu_ctx, n_steps, k, island):
-    print(f"batched adjoint, {n_steps} steps, segments of {k}, land {island}", flush=True)
-    bcs = [Common.BoundaryConditions(1, 1, 1, 1), Common.BoundaryConditions(2, 2, 2, 2),
-           Common.BoundaryConditions(1, 2, 1, 2)]
+_SP = {'north': 6, 'east': 6, 'south': 6, 'west': 6}
+BCS = [Common.BoundaryConditions(1, 1, 1, 1), Common.BoundaryConditions(2, 2, 2, 2),
+       Common.BoundaryConditions(1, 2, 1, 2)]
+BCS_FRS = [Common.BoundaryConditions(3, 3, 3, 3, spongeCells=_SP), Common.BoundaryConditions(1, 1, 1, 1),
+           Common.BoundaryConditions(3, 2, 3, 2, spongeCells=_SP)]
+
+
+@pytest.mark.parametrize("n_steps,k,island,frs", [(30, 8, False, False), (16, 16, False, False), (9, 4, False, False),
+                                                  (12, 5, True, False), (11, 4, True, True)])
+def test_batched_gradients_equal_per_member(gpu_ctx, n_steps, k, island, frs):
+    print(f"batched adjoint, {n_steps} steps, segments of {k}, land {island}, flow relaxation {frs}", flush=True)
+    bcs = BCS_FRS if frs else BCS
     members = [_member(gpu_ctx, nx, ny, 40 + i, bcs[i % 3], island) for i, (nx, ny) in enumerate(SIZES)]
     ens = DifferentiableEnsemble([m[0] for m in members], [m[1] for m in members], [m[2] for m in members],
                                  n_steps, checkpoint_every=k)
