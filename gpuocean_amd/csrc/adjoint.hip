@@ -292,6 +292,13 @@ __global__ void pack_dual(Geo g, const double* v, const double* d, Dual* out) {
   const size_t k = (size_t)r * g.sp + c;
   out[(size_t)r * g.W + c] = Dual(v[k], d ? d[k] : 0.0);
 }
+// Hi (the caller's, land unflagged; geometry gc) into a dense Dual copy with
+// land corners flagged (rows of gh.W).
+__global__ void pack_dual_hi(Geo gh, Geo gc, const double* Hi, Dual* out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y * blockDim.y + threadIdx.y;
+  if (c >= gh.nx + 4 || r >= gh.ny + 4) return;
+  out[(size_t)r * gh.W + c] = Dual(corner_land(gc, Hi, r, c), 0.0);
+}
 __global__ void unpack_dual(Geo g, const Dual* in, double* v, double* d) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x + 2, r = blockIdx.y * blockDim.y + threadIdx.y + 2;
   if (c > g.nx + 1 || r > g.ny + 1) return;
@@ -350,7 +357,7 @@ __global__ void misfit(Geo g, int n, const float* pos, const float* obs, const T
     T u = 0, v = 0, hh[4];
     for (int q = 0; q < 4; ++q) {
       const size_t k = (size_t)rr[q] * g.sp + cc[q];
-      hh[q] = e[k] + cell_depth(g, Hi, rr[q], cc[q]);
+      hh[q] = e[k] + cell_depth_land(g, Hi, rr[q], cc[q]);
       u += wgt[q] * hu[k] / hh[q];
       v += wgt[q] * hv[k] / hh[q];
     }
@@ -449,6 +456,23 @@ dim3 cell_grid(const Geo& g) { return dim3((g.nx + 4 + 63) / 64, (g.ny + 4 + 3) 
 dim3 inner_grid(const Geo& g) { return dim3((g.nx + 63) / 64, (g.ny + 3) / 4); }
 const dim3 kBlock(64, 4);
 
+// Land-flagged copy of Hi (land corners -> kDry) in the workspace tail, dense
+// rows of nx+5; the stages then read corners without testing them.
+template <class T>
+__global__ void flag_hi(Geo g, const T* Hi, T* out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y * blockDim.y + threadIdx.y;
+  if (c >= g.nx + 5 || r >= g.ny + 5) return;
+  out[(size_t)r * (g.nx + 5) + c] = corner_land(g, Hi, r, c);
+}
+template <class T>
+const T* land_hi(Geo* g, const T* Hi, void* ws, hipStream_t s) {
+  if (!g->has_land) return Hi;
+  T* out = (T*)ws + (size_t)kAdjWorkArrays * (g->nx + 4) * (g->ny + 4);
+  hipLaunchKernelGGL(flag_hi<T>, dim3((g->nx + 5 + 63) / 64, (g->ny + 5 + 3) / 4), kBlock, 0, s, *g, Hi, out);
+  g->hp = g->nx + 5;
+  return out;
+}
+
 template <class S>
 int forward_stages(const Geo& g, S dt, int rk, const S* ie, const S* iu, const S* iv, const S* qe, const S* qu,
                    const S* qv, S* oe, S* ou, S* ov, const S* Hi, const S* fc, void* ws, int* flag, hipStream_t s) {
@@ -545,7 +569,7 @@ extern "C" {
 
 size_t gpo_adjoint_workspace_bytes(int32_t nx, int32_t ny, int32_t elem_bytes) {
   if (nx < 1 || ny < 1 || (elem_bytes != 4 && elem_bytes != 8)) return 0;
-  return (size_t)kAdjWorkArrays * (nx + 4) * (ny + 4) * elem_bytes;
+  return ((size_t)kAdjWorkArrays * (nx + 4) * (ny + 4) + (size_t)(nx + 5) * (ny + 5)) * elem_bytes;
 }
 
 int gpo_cdklm16_substep_f64(const gpo_cdklm16_params* p, double dt, int32_t rk_step, const double* in_eta,
@@ -558,8 +582,9 @@ int gpo_cdklm16_substep_f64(const gpo_cdklm16_params* p, double dt, int32_t rk_s
   if (rc) return rc;
   if (!in_eta || !out_eta || !Hi || !workspace || (rk_step == 1 && !qn_eta) || rk_step < 0 || rk_step > 1)
     return bad_arg("gpo_cdklm16_substep_f64: bad arguments");
+  const double* hi = land_hi(&g, Hi, workspace, (hipStream_t)stream);
   return forward_stages<double>(g, dt, rk_step, in_eta, in_hu, in_hv, qn_eta, qn_hu, qn_hv, out_eta, out_hu, out_hv,
-                                Hi, f_cells, workspace, flag, (hipStream_t)stream);
+                                hi, f_cells, workspace, flag, (hipStream_t)stream);
 }
 
 int gpo_cdklm16_substep_tlm_f64(const gpo_cdklm16_params* p, double dt, int32_t rk_step, const double* in_eta,
@@ -604,7 +629,7 @@ int gpo_cdklm16_substep_tlm_f64(const gpo_cdklm16_params* p, double dt, int32_t 
     gh.W = g.nx + 5;
     gh.nx = g.nx + 1;
     gh.ny = g.ny + 1;
-    hipLaunchKernelGGL(pack_dual, cell_grid(gh), kBlock, 0, s, gh, Hi, (const double*)nullptr, hi);
+    hipLaunchKernelGGL(pack_dual_hi, cell_grid(gh), kBlock, 0, s, gh, g, Hi, hi);
     if (f_cells) hipLaunchKernelGGL(pack_dual, cell_grid(g), kBlock, 0, s, g, f_cells, (const double*)nullptr, fcd);
   }
   rc = forward_stages<Dual>(gd, Dual(dt), rk_step, ie, iu, iv, qe, qu, qv, oe, ou, ov, hi, f_cells ? fcd : nullptr,
@@ -673,6 +698,7 @@ int gpo_cdklm16_adjoint_steps_f32(const gpo_cdklm16_params* p, int32_t n_steps, 
     return bad_arg("gpo_cdklm16_adjoint_steps_f32: bad arguments");
   hipStream_t s = (hipStream_t)stream;
   const Sponge sp = frs ? Sponge{frs->sponge[0], frs->sponge[1], frs->sponge[2], frs->sponge[3]} : Sponge{0, 0, 0, 0};
+  Hi = land_hi(&g, Hi, workspace, s);
   float* cur[3] = {lam[0], lam[1], lam[2]};     // lambda at Q^{j+1}, then at Q^j
   float* nxt[3] = {work[0], work[1], work[2]};  // lambda being formed at Q^j
   float* l1[3] = {work[3], work[4], work[5]};   // lambda at the RK stage Q^(1)
@@ -715,8 +741,9 @@ int gpo_cdklm16_substep_adj_f32(const gpo_cdklm16_params* p, float dt, int32_t r
   if (rc) return rc;
   if (!in_eta || !out_eta || !lam_out_eta || !lam_in_eta || !Hi || !workspace || rk_step < 0 || rk_step > 1)
     return bad_arg("gpo_cdklm16_substep_adj_f32: bad arguments");
+  const float* hi = land_hi(&g, Hi, workspace, (hipStream_t)stream);
   return adjoint_stages<float>(g, dt, rk_step, in_eta, in_hu, in_hv, out_eta, lam_out_eta, lam_out_hu, lam_out_hv,
-                               lam_in_eta, lam_in_hu, lam_in_hv, lam_qn_eta, lam_qn_hu, lam_qn_hv, accumulate, 0, Hi,
+                               lam_in_eta, lam_in_hu, lam_in_hv, lam_qn_eta, lam_qn_hu, lam_qn_hv, accumulate, 0, hi,
                                f_cells, workspace, flag, (hipStream_t)stream);
 }
 
@@ -732,8 +759,9 @@ int gpo_cdklm16_substep_adj_f64(const gpo_cdklm16_params* p, double dt, int32_t 
   if (rc) return rc;
   if (!in_eta || !out_eta || !lam_out_eta || !lam_in_eta || !Hi || !workspace || rk_step < 0 || rk_step > 1)
     return bad_arg("gpo_cdklm16_substep_adj_f64: bad arguments");
+  const double* hi = land_hi(&g, Hi, workspace, (hipStream_t)stream);
   return adjoint_stages<double>(g, dt, rk_step, in_eta, in_hu, in_hv, out_eta, lam_out_eta, lam_out_hu, lam_out_hv,
-                                lam_in_eta, lam_in_hu, lam_in_hv, lam_qn_eta, lam_qn_hu, lam_qn_hv, accumulate, 0, Hi,
+                                lam_in_eta, lam_in_hu, lam_in_hv, lam_qn_eta, lam_qn_hu, lam_qn_hv, accumulate, 0, hi,
                                 f_cells, workspace, flag, (hipStream_t)stream);
 }
 
@@ -869,6 +897,8 @@ int gpo_adj_batch_create(const gpo_adj_member* members, int32_t n, int32_t k, in
       return rc;
     }
     if (p.chunk_rows == 0) p.chunk_rows = 16;
+    Geo ga = g;  // the reverse stages read the land-flagged copy of Hi
+    const float* hi_adj = land_hi(&ga, M.Hi, M.workspace, s);
     const size_t pl = (size_t)M.plane / 4, st = 3 * pl;  // elements per plane / per state
     auto planes = [&](float* base, size_t i, float** e, float** u, float** v) {
       *e = base + i * st, *u = base + i * st + pl, *v = base + i * st + 2 * pl;
@@ -929,10 +959,10 @@ int gpo_adj_batch_create(const gpo_adj_member* members, int32_t n, int32_t k, in
       const Rev<float> rv = carve_rev<float>(M.workspace, p.nx, p.ny);
       for (int rk = 0; rk < 2; ++rk) {
         AdjItem<float>& it = items[((size_t)2 * j + rk) * n + m];
-        it.g = g;
+        it.g = ga;
         it.dt = M.dt;
         it.rk = rk;
-        it.Hi = gptr(M.Hi);
+        it.Hi = gptr(hi_adj);
         it.fc = gptr(M.f_cells);
         it.w = w;
         it.rv = rv;

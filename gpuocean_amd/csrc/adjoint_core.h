@@ -159,7 +159,13 @@ template <class S> inline Fwd<S> carve(void* ws, int nx, int ny) {
   return f;
 }
 
+// Corner depth.  With land, the stages read a copy of Hi whose land corners
+// already hold kDry (made once per call by the entry points); corner_land
+// reads the caller's Hi and flags on the fly.
 template <class T> GPO_HD inline T corner(const Geo& g, const T* Hi, int r, int c) {
+  return Hi[(size_t)r * g.hp + c];
+}
+template <class T> GPO_HD inline T corner_land(const Geo& g, const T* Hi, int r, int c) {
   const T h = Hi[(size_t)r * g.hp + c];
   if (g.has_land && fabs((double)val(h) - g.land) < kDryTol) return T(kDry);
   return h;
@@ -168,6 +174,11 @@ template <class T> GPO_HD inline T corner(const Geo& g, const T* Hi, int r, int 
 template <class T> GPO_HD inline T cell_depth(const Geo& g, const T* Hi, int r, int c) {
   const T a = corner(g, Hi, r, c), b = corner(g, Hi, r + 1, c);
   const T c_ = corner(g, Hi, r, c + 1), d = corner(g, Hi, r + 1, c + 1);
+  return T(0.25) * (a + b + c_ + d);
+}
+template <class T> GPO_HD inline T cell_depth_land(const Geo& g, const T* Hi, int r, int c) {
+  const T a = corner_land(g, Hi, r, c), b = corner_land(g, Hi, r + 1, c);
+  const T c_ = corner_land(g, Hi, r, c + 1), d = corner_land(g, Hi, r + 1, c + 1);
   return T(0.25) * (a + b + c_ + d);
 }
 // Depth of the conversion (initHm_kernel.cu:81-100): the mean of the wet
@@ -1121,19 +1132,15 @@ GPO_HD void conv_rev(const Geo& g, const T* Hi, const Fwd<T>& w, const Rev<T>& r
   oe = ou = ov = T(0);
   if (land_cell(g, bm) || !(h > T(g.depth_cutoff))) return;  // land, or dried to a constant state
   const T u = w.cu[o], v = w.cv[o];
-  if (h >= T(g.desing_eps)) {
-    ou = l1 / h * T(src.shu);
-    ov = l2 / h * T(src.shv);
-    oe = l0 - (l1 * u + l2 * v) / h;
-  } else {
-    // desingularised: u = hu / a with a = h^2 / (2 eps) + eps / 2 (between the
-    // depth cutoff and eps, min(a, eps) = a and max(a, |h|) = a), da/dh = h / eps
-    const T eps = T(g.desing_eps);
-    const T a = h * h / T(2.0 * g.desing_eps) + T(0.5 * g.desing_eps);
-    ou = l1 / a * T(src.shu);
-    ov = l2 / a * T(src.shv);
-    oe = l0 - (l1 * u + l2 * v) * (h / eps) / a;
-  }
+  // u = hu / den with den = h, or below eps the desingularised
+  // a = h^2 / (2 eps) + eps / 2 (between the depth cutoff and eps,
+  // min(a, eps) = a and max(a, |h|) = a), d den / dh = 1 or h / eps
+  const bool shallow = h < T(g.desing_eps);
+  const T den = shallow ? h * h * T(0.5 / g.desing_eps) + T(0.5 * g.desing_eps) : h;
+  const T dh = shallow ? h * T(1.0 / g.desing_eps) : T(1);
+  ou = l1 / den * T(src.shu);
+  ov = l2 / den * T(src.shv);
+  oe = l0 - (l1 * u + l2 * v) * dh / den;
 }
 
 // A3: the sensitivity of interior input cell (r, c): its own converted cell

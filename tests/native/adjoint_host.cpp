@@ -41,6 +41,15 @@ int make_geo(const gpo_cdklm16_params* p, uint32_t bc, Geo* g) {
   return 0;
 }
 
+// Hi with land corners flagged (kDry), as the library's entry points prepare
+// it before the stages run.
+std::vector<double> flagged_hi(const Geo& g, const double* Hi) {
+  std::vector<double> out((size_t)(g.nx + 5) * (g.ny + 5));
+  for (int r = 0; r < g.ny + 5; ++r)
+    for (int c = 0; c < g.nx + 5; ++c) out[(size_t)r * (g.nx + 5) + c] = corner_land(g, Hi, r, c);
+  return out;
+}
+
 template <class S> Fwd<S> carve_vec(std::vector<S>& ws, const Geo& g) {
   ws.assign((size_t)kAdjWorkArrays * (g.nx + 4) * (g.ny + 4), S(0.0));
   return carve<S>(ws.data(), g.nx, g.ny);
@@ -71,7 +80,8 @@ int gpo_host_substep_f64(const gpo_cdklm16_params* p, double dt, int rk, const d
                          double* ou, double* ov, const double* Hi, const double* fc, uint32_t bc, int* flag) {
   Geo g;
   if (make_geo(p, bc, &g)) return -1;
-  forward<double>(g, dt, rk, ie, iu, iv, qe, qu, qv, oe, ou, ov, Hi, fc, flag);
+  const std::vector<double> hf = flagged_hi(g, Hi);
+  forward<double>(g, dt, rk, ie, iu, iv, qe, qu, qv, oe, ou, ov, hf.data(), fc, flag);
   return 0;
 }
 
@@ -96,7 +106,8 @@ int gpo_host_substep_tlm_f64(const gpo_cdklm16_params* p, double dt, int rk, con
     if (fc) f[k] = Dual(fc[k]);
     oa[k] = ob[k] = oc[k] = Dual(0.0);
   }
-  for (size_t k = 0; k < nh; ++k) hi[k] = Dual(Hi[k]);
+  const std::vector<double> hf = flagged_hi(g, Hi);
+  for (size_t k = 0; k < nh; ++k) hi[k] = Dual(hf[k]);
   forward<Dual>(g, Dual(dt), rk, a.data(), b.data(), c.data(), qa.data(), qb.data(), qc.data(), oa.data(),
                 ob.data(), oc.data(), hi.data(), fc ? f.data() : nullptr, flag);
   for (int r = 2; r <= g.ny + 1; ++r)
@@ -119,6 +130,8 @@ int gpo_host_substep_adj_f64(const gpo_cdklm16_params* p, double dt, int rk, con
                              int* flag) {
   Geo g;
   if (make_geo(p, bc, &g)) return -1;
+  const std::vector<double> hf = flagged_hi(g, Hi);
+  Hi = hf.data();
   std::vector<double> ws;
   Fwd<double> w = carve_vec(ws, g);
   Rev<double> rv = carve_rev<double>(ws.data(), g.nx, g.ny);
