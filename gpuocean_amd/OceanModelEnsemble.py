@@ -25,6 +25,9 @@ import torch
 from . import _lib as L
 from .CDKLM16 import CDKLM16
 from .GPUDrifterCollection import GPUDrifterCollection
+from .Observation import Observation
+from .ParticleInfo import ParticleInfo
+from . import observe
 
 
 def _vp(t):
@@ -357,6 +360,7 @@ class OceanModelEnsemble:
                 self.particles[i].setSOARModelError(**model_error_args)
             elif 'kl_scaling' in model_error_args:
                 raise NotImplementedError("KL model error is not part of gpuocean_amd")
+            self.particleInfos[i] = ParticleInfo()
             if self.initialization_variance_factor_ocean_field != 0.0:
                 self.particles[i].perturbState(q0_scale=self.initialization_variance_factor_ocean_field)
 
@@ -369,6 +373,7 @@ class OceanModelEnsemble:
         self.autotune = autotune
         self.graph_steps = graph_steps
         self._ensemble = None
+        self._obs_table = observe._Table()
 
     def getNumParticles(self):
         return self.numParticles
@@ -398,6 +403,8 @@ class OceanModelEnsemble:
                                                 gpu_stream=p.gpu_stream)
                 drifters.setDrifterPositions(drifter_positions)
                 p.attachDrifters(drifters)
+                self.drifterForecast[i] = Observation()
+                self.drifterForecast[i].add_observation_from_sim(p)
 
     def cleanUp(self):
         if self._ensemble is not None:
@@ -471,20 +478,58 @@ class OceanModelEnsemble:
             c = p.courant_number if courant_number is None else courant_number
             p.dt = c * host[i:i + 1][0]
 
+    def _cell_sizes(self):
+        """The dx, dy the reference divides positions by: data_args' own."""
+        if isinstance(self.data_args, (list, tuple)):
+            return None, None
+        return self.data_args.get('dx'), self.data_args.get('dy')
+
     def observeParticles(self, drifter_positions):
         """hu, hv at the cells holding the given positions, per member:
-        (numParticles, num_drifters, 2) (OceanModelEnsemble.py:168-198)."""
+        (numParticles, num_drifters, 2) (OceanModelEnsemble.py:168-198).
+        One observation launch over all members and one read-back; the cell
+        is np.floor(x/dx) on the interior, as the reference indexes it."""
         drifter_positions = np.asarray(drifter_positions)
-        out = np.empty((self.numParticles, len(drifter_positions), 2))
-        for p in range(self.numParticles):
-            sim = self.particles[p]
-            eta, hu, hv = sim.download(interior_domain_only=True)
-            for d in range(len(drifter_positions)):
-                id_x = int(np.floor(drifter_positions[d, 0] / sim.dx))
-                id_y = int(np.floor(drifter_positions[d, 1] / sim.dy))
-                out[p, d, 0] = hu[id_y, id_x]
-                out[p, d, 1] = hv[id_y, id_x]
-        return out
+        if len(drifter_positions) == 0:
+            return np.empty((self.numParticles, 0, 2))
+        dx, dy = self._cell_sizes()
+        return observe.observe_members(self.particles, observe.FLOW, positions=drifter_positions, dx=dx, dy=dy,
+                                       table=self._obs_table)
+
+    def dumpParticleSample(self, drifter_cells):
+        """(eta, hu, hv) of every member at drifter_cells (and each member's
+        extra cells) into its ParticleInfo (OceanModelEnsemble.py:160-162),
+        from one observation launch when the members share their extra cells."""
+        drifter_cells = np.asarray(drifter_cells)
+        extras = [pi.extraCells for pi in self.particleInfos]
+        same = all((e is None and extras[0] is None) or
+                   (e is not None and extras[0] is not None and np.array_equal(e, extras[0])) for e in extras)
+        if not same:
+            for i in range(self.numParticles):
+                self.particleInfos[i].add_state_sample_from_sim(self.particles[i], drifter_cells)
+            return
+        cells = drifter_cells if extras[0] is None else np.concatenate([drifter_cells, np.asarray(extras[0])])
+        values = observe.observe_members(self.particles, observe.STATE, cells=cells, table=self._obs_table)
+        D = drifter_cells.shape[0]
+        for i in range(self.numParticles):
+            extra = None if extras[0] is None else values[i, D:].copy()
+            self.particleInfos[i]._add(self.particles[i].t, values[i, :D].copy(), extra)
+
+    def dumpForecastParticleSample(self):
+        """Each member's drifter positions into its forecast Observation
+        (OceanModelEnsemble.py:164-166)."""
+        for i in range(self.numParticles):
+            self.drifterForecast[i].add_observation_from_sim(self.particles[i])
+
+    def dumpParticleInfosToFiles(self, filename_prefix):
+        """Each member's state samples to <prefix>_<member>.bz2 (OceanModelEnsemble.py:200-206)."""
+        for p in range(self.getNumParticles()):
+            self.particleInfos[p].to_pickle(filename_prefix + "_" + str(p) + ".bz2")
+
+    def dumpDrifterForecastToFiles(self, filename_prefix):
+        """Each member's drifter forecast to <prefix>_<member>.bz2 (OceanModelEnsemble.py:208-214)."""
+        for p in range(self.getNumParticles()):
+            self.drifterForecast[p].to_pickle(filename_prefix + "_" + str(p) + ".bz2")
 
     def syncGPU(self):
         for p in self.particles:

@@ -10,6 +10,9 @@ from .forcing import AtmosphericPressure, WindStress  # noqa: F401
 from .GPUDrifterCollection import GPUDrifterCollection  # noqa: F401
 from .OceanStateNoise import OceanStateNoise  # noqa: F401
 from .OceanModelEnsemble import Autotuner, BatchedEnsemble, OceanModelEnsemble  # noqa: F401
+from .DataAssimilationUtils import ObservationType  # noqa: F401
+from .Observation import Observation  # noqa: F401
+from .ParticleInfo import ParticleInfo  # noqa: F401
 
 __all__ = ["CDKLM16", "Common", "WindStress", "AtmosphericPressure", "GPUDrifterCollection", "OceanStateNoise",
-           "OceanModelEnsemble", "BatchedEnsemble", "Autotuner"]
+           "OceanModelEnsemble", "BatchedEnsemble", "Autotuner", "ObservationType", "Observation", "ParticleInfo"]

@@ -79,6 +79,18 @@ class AdjMember(ctypes.Structure):
     ]
 
 
+class ObsMember(ctypes.Structure):
+    """gpo_obs_member: one member seen by the observation operator."""
+    _fields_ = [
+        ("eta", c_void_p), ("hu", c_void_p), ("hv", c_void_p), ("pitch", c_int32),
+        ("Bm", c_void_p), ("Bm_pitch", c_int32), ("nx", c_int32), ("ny", c_int32),
+        ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("points", c_void_p), ("cells", c_void_p),
+    ]
+
+
+OBS_FLOW, OBS_STATE, OBS_BUOY, OBS_INNOVATION, OBS_GHOSTED = 0, 1, 2, 3, 16
+
+
 class GpoError(RuntimeError):
     pass
 
@@ -118,6 +130,11 @@ _SIGNATURES = [
      [c_int32, c_int32, c_float, c_float, c_float, c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_int32,
       c_void_p, c_int32, c_int32, c_int32, c_int32, c_void_p, c_int32, c_float, c_void_p, c_void_p, c_void_p,
       c_void_p, c_int32, c_int32, c_float, c_float, c_void_p]),
+    ("gpo_observe_underlying_flow", c_int32,
+     [c_int32, c_int32, c_float, c_float, c_int32, c_int32, c_void_p, c_int32, c_void_p, c_int32, c_void_p,
+      c_int32, c_float, c_int32, c_void_p, c_int32, c_void_p, c_int32, c_void_p]),
+    ("gpo_observe", c_int32,
+     [c_void_p, c_int32, c_int32, c_int32, c_void_p, c_float, c_void_p, c_void_p, c_void_p]),
     ("gpo_ensemble_create", c_int32,
      [ctypes.POINTER(Member), c_int32, c_int32, c_int32, c_int32, ctypes.POINTER(c_void_p), c_void_p]),
     ("gpo_ensemble_update", c_int32, [c_void_p, c_fp, ctypes.POINTER(c_uint32), c_void_p]),

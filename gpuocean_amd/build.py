@@ -32,6 +32,7 @@ UNITS = [
     ("noise_drift.hip", ["-ffp-contract=off"]),
     ("ensemble.hip", ["-ffp-contract=off"]),
     ("adjoint.hip", ["-ffp-contract=off"]),
+    ("observe.hip", ["-ffp-contract=off"]),
 ]
 
 

@@ -190,6 +190,61 @@ int gpo_drift(int32_t nx, int32_t ny, float dx, float dy, float dt, int32_t x0re
               const float* wind_y_next, int32_t wind_nx, int32_t wind_ny, float wind_t, float wind_drift_factor,
               void* stream);
 
+/* ---- observation operators ---------------------------------------------------- */
+
+/* Mirror of observeUnderlyingFlow (gpu_kernels/observationKernels.cu:29-80),
+ * launched by BaseOceanStateEnsemble.observeParticles(gpu=True)
+ * (ensembles/BaseOceanStateEnsemble.py:117-131): (hu, hv) of the cell
+ * floor(x/dx) + x0ref, floor(y/dy) + y0ref (FP32 quotient) for each of the
+ * n_drifters positions (rows of (x, y), pos_pitch bytes), written to rows of
+ * (hu, hv) floats, obs_pitch bytes.  eta and H are accepted and unused, as in
+ * the reference.  A position whose cell falls outside the (ny+4) x (nx+4)
+ * array gives NaN (the reference reads outside the array there). */
+int gpo_observe_underlying_flow(int32_t nx, int32_t ny, float dx, float dy, int32_t x0ref, int32_t y0ref,
+                                const float* eta, int32_t eta_pitch, const float* hu, int32_t hu_pitch,
+                                const float* hv, int32_t hv_pitch, float H, int32_t n_drifters,
+                                const float* positions, int32_t pos_pitch, float* observation, int32_t obs_pitch,
+                                void* stream);
+
+/* One member observed by gpo_observe: its state (rows of `pitch` bytes,
+ * ghost cells included), cell-centre depth Bm (GPO_OBS_BUOY only) and where to
+ * observe it: either `points` (n_points (x, y) pairs in metres, FP64) or
+ * `cells` (n_points (x, y) interior cell indices); members may share them. */
+typedef struct {
+  const float* eta;
+  const float* hu;
+  const float* hv;
+  int32_t pitch;
+  const float* Bm;
+  int32_t Bm_pitch;
+  int32_t nx, ny;
+  double dx, dy;
+  const double* points;
+  const int32_t* cells;
+} gpo_obs_member;
+
+#define GPO_OBS_FLOW 0       /* (hu, hv) */
+#define GPO_OBS_STATE 1      /* (eta, hu, hv) */
+#define GPO_OBS_BUOY 2       /* (hu, hv) * H / (H + eta) in FP32 */
+#define GPO_OBS_INNOVATION 3 /* (hu_obs, hv_obs) * (D + eta) / D - (hu, hv), D = mean_depth (1 if 0) */
+#define GPO_OBS_GHOSTED 16   /* or-ed in: observeUnderlyingFlow's cell rule instead of numpy's */
+
+/* Observation operator over many members in one launch.  members: device
+ * array of n_members entries.  out: device doubles, (n_members, n_points, k)
+ * with k = 3 for GPO_OBS_STATE, else 2.  Cells follow numpy indexing of the
+ * interior arrays, as the reference's host paths do: floor(x/dx) in FP64 (or
+ * the cell given), -n..n-1 with negative indices counting from the end.
+ * A point outside that range gives NaN and adds one to *n_bad (device int,
+ * may be NULL), where numpy raises IndexError.  observed (GPO_OBS_INNOVATION):
+ * device doubles, n_points rows of (x, y, hu, hv) as Observation.get_observation
+ * returns them.  Replaces the per-member download-and-index loops of
+ * OceanModelEnsemble.observeParticles (ensembles/OceanModelEnsemble.py:168-198),
+ * BaseOceanStateEnsemble.observeParticles (ensembles/BaseOceanStateEnsemble.py:79-159),
+ * ParticleInfo.add_state_sample_from_sim (utils/ParticleInfo.py:81-111) and the
+ * buoy readings of Observation.add_observation_from_sim (utils/Observation.py:137-180). */
+int gpo_observe(const gpo_obs_member* members, int32_t n_members, int32_t n_points, int32_t mode,
+                const double* observed, float mean_depth, double* out, int32_t* n_bad, void* stream);
+
 /* ---- batched ensemble ------------------------------------------------------- */
 
 /* One member of a batched ensemble: its parameter block (after
