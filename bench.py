@@ -262,6 +262,7 @@ def run_ensemble(args):
     for s in sims[:4]:
         assert np.all(np.isfinite(s.download()[0])), "non-finite state"
     ens.destroy()
+    observe_r = run_observe(sims)
 
     sizes = [(int(s.nx), int(s.ny)) for s in sims]
     return members_kw, {
@@ -288,11 +289,49 @@ def run_ensemble(args):
                        "speedup_batched": value / value_seq,
                        "path": "each member's own CDKLM16.step, one member after another"},
         "parts_us": {"substep0": part_us[0], "substep1": part_us[1], "model_error": part_us[2]},
+        "observe": observe_r,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic("cdklm_march_batched", "c2"),
                      "kernel": "cdklm_march_batched", "bytes_per_cell_per_launch": BYTES_PER_CELL_STEP / 2,
                      "cells_per_launch": cells, "avg_launch_us": avg_launch_us},
     }
+
+
+def run_observe(sims, n_points=1000, reps=5):
+    """hu, hv of every member at the same n_points positions: the library's
+    one-launch observation operator against the reference's way (each
+    member's interior state downloaded, then numpy indexing), both ending
+    with the values on the host."""
+    import numpy as np
+    import torch
+    from gpuocean_amd import observe
+
+    nx = min(int(s.nx) for s in sims)
+    ny = min(int(s.ny) for s in sims)
+    rng = np.random.default_rng(17)
+    pts = np.stack([rng.uniform(0, nx * float(sims[0].dx), n_points),
+                    rng.uniform(0, ny * float(sims[0].dy), n_points)], axis=1)
+    table = observe._Table()
+    observe.observe_members(sims, observe.FLOW, positions=pts, table=table)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        got = observe.observe_members(sims, observe.FLOW, positions=pts, table=table)
+    ms = 1e3 * (time.perf_counter() - t0) / reps
+    ix = np.floor(pts[:, 0] / float(sims[0].dx)).astype(int)
+    iy = np.floor(pts[:, 1] / float(sims[0].dy)).astype(int)
+    t0 = time.perf_counter()
+    ref = np.empty((len(sims), n_points, 2))
+    for p, s in enumerate(sims):
+        _, hu, hv = s.download(interior_domain_only=True)
+        ref[p, :, 0] = hu[iy, ix]
+        ref[p, :, 1] = hv[iy, ix]
+    ms_ref = 1e3 * (time.perf_counter() - t0)
+    assert np.array_equal(got, ref), "observation operator differs from the download path"
+    log(f"[observe] {len(sims)} members x {n_points} points: {ms:.2f} ms (one launch, one read-back) vs "
+        f"{ms_ref:.1f} ms downloading every member; {ms_ref / ms:.0f}x")
+    return {"members": len(sims), "points": n_points, "ms": ms, "ms_download_path": ms_ref,
+            "speedup": ms_ref / ms}
 
 
 def run_copy(n_bytes=256 << 20, reps=5):

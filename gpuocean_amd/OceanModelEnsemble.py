@@ -90,6 +90,8 @@ class BatchedEnsemble:
                     return False, "drifters whose periodicity differs from their member's"
             if (s.model_error is None) != (s0.model_error is None):
                 return False, "some members carry model error and some do not"
+            if s.model_error is not None and int(s.model_error.interpolation_factor) != 1:
+                return False, "model error generated on a coarser grid (interpolation_factor > 1)"
             if s.device != s0.device or bool(s.exact_arithmetic) != bool(s0.exact_arithmetic):
                 return False, "members on different devices or arithmetic modes"
             p, q = s.params, s0.params

@@ -9,8 +9,13 @@ the seed (``xorwow_seed``/``np_seed``), with counter (element, draw, stream), so
 every draw is reproducible and any part of it can be regenerated on the fly.
 The reference's XORWOW (curand) and LCG streams are not reproduced; the
 statistical properties it tests are (zero mean, unit variance, seeded
-reproducibility).  Coarse-grid interpolation (interpolation_factor > 1) is not
-implemented.
+reproducibility).
+
+With ``interpolation_factor`` > 1 the random field and SOAR live on a grid
+that factor times coarser, and ``gpo_bicubic_interpolation`` carries the
+perturbation to the computational grid, optionally aligned with a given cell
+(``align_with_cell_i/j``), with its balanced momentum
+(OceanStateNoise.py:418-441, 461-475).
 """
 
 import ctypes
@@ -45,8 +50,10 @@ class OceanStateNoise:
                  device=None, stream_id=0):
         assert interpolation_factor > 0 and interpolation_factor % 2 == 1, \
             'interpolation_factor must be a positive odd integer'
-        if interpolation_factor != 1:
-            raise NotImplementedError("coarse-grid (bicubic) SOAR perturbations are not implemented")
+        if boundaryConditions.isPeriodicNorthSouth():
+            assert ny % interpolation_factor == 0, 'ny must be divisible by the interpolation factor'
+        if boundaryConditions.isPeriodicEastWest():
+            assert nx % interpolation_factor == 0, 'nx must be divisible by the interpolation factor'
         if staggered:
             raise NotImplementedError("staggered grids (FBL/CTCS) are not part of gpuocean_amd")
         L.lib()
@@ -60,8 +67,10 @@ class OceanStateNoise:
         self.interpolation_factor = np.int32(interpolation_factor)
         self.periodicNorthSouth = np.int32(boundaryConditions.isPeriodicNorthSouth())
         self.periodicEastWest = np.int32(boundaryConditions.isPeriodicEastWest())
-        self.coarse_nx, self.coarse_ny = np.int32(nx), np.int32(ny)
-        self.coarse_dx, self.coarse_dy = np.float32(dx), np.float32(dy)
+        self.coarse_nx = np.int32(np.ceil(nx / self.interpolation_factor))
+        self.coarse_ny = np.int32(np.ceil(ny / self.interpolation_factor))
+        self.coarse_dx = np.float32(dx * self.interpolation_factor)
+        self.coarse_dy = np.float32(dy * self.interpolation_factor)
         self.rand_ghost_cells_x = np.int32(0 if self.periodicEastWest else 2 + self.cutoff)
         self.rand_ghost_cells_y = np.int32(0 if self.periodicNorthSouth else 2 + self.cutoff)
         self.rand_nx = np.int32(self.coarse_nx + 2 * self.rand_ghost_cells_x)
@@ -162,6 +171,7 @@ class OceanStateNoise:
         self.perturbOceanState(sim.h0[0], sim.h0[1], sim.h0[2], sim.Hi, sim.f, beta=sim.coriolis_beta, g=sim.g,
                                q0_scale=q0_scale, update_random_field=update_random_field,
                                perturbation_scale=perturbation_scale, perpendicular_scale=perpendicular_scale,
+                               align_with_cell_i=align_with_cell_i, align_with_cell_j=align_with_cell_j,
                                land_mask_value=sim.mask_value, north=sim.params)
 
     def perturbSimSimilarAs(self, simToPerturb, modelError=None, perturbation_scale=1.0):
@@ -182,8 +192,6 @@ class OceanStateNoise:
         (OceanStateNoise.py:356-459).  eta/hu/hv/H are DeviceArray2D buffers."""
         if perpendicular_scale > 0:
             raise NotImplementedError("perpendicular perturbations are not implemented")
-        if align_with_cell_i is not None or align_with_cell_j is not None:
-            raise NotImplementedError("coarse-grid alignment offsets are not implemented")
         if update_random_field:
             self.generateNormalDistribution()
         else:
@@ -198,6 +206,18 @@ class OceanStateNoise:
             nxv, ny_ = float(np.sin(self.angle.reshape(-1)[0])), float(np.cos(self.angle.reshape(-1)[0]))
         fc = self._f_cells
         f_scalar = float(np.float32(f)) if np.asarray(f).size == 1 else 0.0
+        if self.interpolation_factor > 1:
+            oi, oj = self._obtain_coarse_grid_offset(align_with_cell_i, align_with_cell_j)
+            L.check(lib.gpo_bicubic_interpolation(
+                int(self.nx), int(self.ny), float(self.dx), float(self.dy), int(self.coarse_nx), int(self.coarse_ny),
+                float(self.coarse_dx), float(self.coarse_dy), int(oi), int(oj), float(np.float32(g)), f_scalar,
+                float(np.float32(beta)), float(nxv), float(ny_),
+                None if fc is None else ctypes.c_void_p(fc.data_ptr()), 0 if fc is None else fc.pitch,
+                ctypes.c_void_p(self.coarse_buffer.data_ptr()), 4 * (int(self.coarse_nx) + 4),
+                ctypes.c_void_p(eta.data_ptr()), ctypes.c_void_p(hu.data_ptr()), ctypes.c_void_p(hv.data_ptr()),
+                eta.pitch, ctypes.c_void_p(H.data_ptr()), H.pitch, float(land_mask_value), self._stream()),
+                "gpo_bicubic_interpolation")
+            return
         L.check(lib.gpo_geostrophic_balance(
             int(self.nx), int(self.ny), float(self.dx), float(self.dy), float(np.float32(g)), f_scalar,
             float(np.float32(beta)), float(nxv), float(ny_),
@@ -206,6 +226,17 @@ class OceanStateNoise:
             ctypes.c_void_p(eta.data_ptr()), ctypes.c_void_p(hu.data_ptr()), ctypes.c_void_p(hv.data_ptr()),
             eta.pitch, ctypes.c_void_p(H.data_ptr()), H.pitch, float(land_mask_value), self._stream()),
             "gpo_geostrophic_balance")
+
+    def _obtain_coarse_grid_offset(self, fine_index_i, fine_index_j):
+        """Fine-cell shift of the coarse grid that puts a coarse cell centre on
+        the given fine cell (OceanStateNoise.py:461-475); 0 when not given."""
+        default_offset = int(self.interpolation_factor) // 2
+        offset_i = offset_j = 0
+        if fine_index_i is not None:
+            offset_i = -int(fine_index_i % int(self.interpolation_factor) - default_offset)
+        if fine_index_j is not None:
+            offset_j = -int(fine_index_j % int(self.interpolation_factor) - default_offset)
+        return offset_i, offset_j
 
     def cleanUp(self, do_gc=True):
         self.random_numbers = None

@@ -126,6 +126,10 @@ _SIGNATURES = [
     ("gpo_geostrophic_balance", c_int32,
      [c_int32, c_int32, c_float, c_float, c_float, c_float, c_float, c_float, c_float, c_void_p, c_int32,
       c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_int32, c_float, c_void_p]),
+    ("gpo_bicubic_interpolation", c_int32,
+     [c_int32, c_int32, c_float, c_float, c_int32, c_int32, c_float, c_float, c_int32, c_int32, c_float, c_float,
+      c_float, c_float, c_float, c_void_p, c_int32, c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_int32,
+      c_void_p, c_int32, c_float, c_void_p]),
     ("gpo_drift", c_int32,
      [c_int32, c_int32, c_float, c_float, c_float, c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_int32,
       c_void_p, c_int32, c_int32, c_int32, c_int32, c_void_p, c_int32, c_float, c_void_p, c_void_p, c_void_p,

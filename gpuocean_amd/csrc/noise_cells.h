@@ -79,4 +79,100 @@ __device__ __forceinline__ void balance_cell(float dx, float dy, float g, float 
                 land_value, tj, ti);
 }
 
+// ---- coarse-grid perturbations (interpolation factor > 1) ---------------------
+
+__device__ __forceinline__ float dot4(float a0, float a1, float a2, float a3, float b0, float b1, float b2, float b3) {
+  return a0 * b0 + a1 * b1 + a2 * b2 + a3 * b3;
+}
+
+// Coarse perturbation at fine cell (tj, ti) of the ghosted fine grid by bicubic
+// interpolation of the (cny+4) x (cnx+4) coarse buffer (2 ghost cells on both
+// grids): bicubicInterpolation (gpu_kernels/ocean_noise.cu:523-611) with the
+// coefficient and evaluation steps of interpolation.cu:39-133.  The corner
+// values and central-difference derivatives of the coarse cell below-left of
+// the fine centre form F; the coefficients are B F B^T, B the Hermite basis
+// matrix, evaluated at the fine centre's offset in that coarse cell.  Coarse
+// indices are clamped to the buffer as the reference's shared-memory load does.
+__device__ __forceinline__ float bicubic_fine(int cnx, int cny, float dx, float dy, float cdx, float cdy, int oi,
+                                              int oj, const float* coarse, int c_pitch, int tj, int ti) {
+  const float x = ((float)(ti - 2) + 0.5f + (float)oi) * dx;
+  const float y = ((float)(tj - 2) + 0.5f + (float)oj) * dy;
+  int ci = (int)floorf(x / cdx + 2.0f - 0.5f);
+  int cj = (int)floorf(y / cdy + 2.0f - 0.5f);
+  if (ci > cnx + 1) ci -= 1;  // first ghost cell right/top of the domain: rel = 1
+  if (cj > cny + 1) cj -= 1;
+  const float cx = ((float)(ci - 2) + 0.5f) * cdx;
+  const float cy = ((float)(cj - 2) + 0.5f) * cdy;
+  auto C = [&](int j, int i) {
+    j = j < 0 ? 0 : (j > cny + 3 ? cny + 3 : j);
+    i = i < 0 ? 0 : (i > cnx + 3 ? cnx + 3 : i);
+    return crow(coarse, c_pitch, j)[i];
+  };
+  // F = [[f00, f01, fy00, fy01], [f10, f11, fy10, fy11], [fx00, fx01, fxy00, fxy01], [fx10, fx11, fxy10, fxy11]]
+  const float f00 = C(cj, ci), f01 = C(cj + 1, ci), f10 = C(cj, ci + 1), f11 = C(cj + 1, ci + 1);
+  const float fx00 = (C(cj, ci + 1) - C(cj, ci - 1)) / 2.0f;
+  const float fx01 = (C(cj + 1, ci + 1) - C(cj + 1, ci - 1)) / 2.0f;
+  const float fx10 = (C(cj, ci + 2) - C(cj, ci)) / 2.0f;
+  const float fx11 = (C(cj + 1, ci + 2) - C(cj + 1, ci)) / 2.0f;
+  const float fy00 = (C(cj + 1, ci) - C(cj - 1, ci)) / 2.0f;
+  const float fy01 = (C(cj + 2, ci) - C(cj, ci)) / 2.0f;
+  const float fy10 = (C(cj + 1, ci + 1) - C(cj - 1, ci + 1)) / 2.0f;
+  const float fy11 = (C(cj + 2, ci + 1) - C(cj, ci + 1)) / 2.0f;
+  const float fym0 = (C(cj + 1, ci - 1) - C(cj - 1, ci - 1)) / 2.0f;
+  const float fym1 = (C(cj + 2, ci - 1) - C(cj, ci - 1)) / 2.0f;
+  const float fy20 = (C(cj + 1, ci + 2) - C(cj - 1, ci + 2)) / 2.0f;
+  const float fy21 = (C(cj + 2, ci + 2) - C(cj, ci + 2)) / 2.0f;
+  const float F[4][4] = {{f00, f01, fy00, fy01},
+                         {f10, f11, fy10, fy11},
+                         {fx00, fx01, (fy10 - fym0) / 2.0f, (fy11 - fym1) / 2.0f},
+                         {fx10, fx11, (fy20 - fy00) / 2.0f, (fy21 - fy01) / 2.0f}};
+  const float B[4][4] = {{1.0f, 0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 1.0f, 0.0f},
+                         {-3.0f, 3.0f, -2.0f, -1.0f}, {2.0f, -2.0f, 1.0f, 1.0f}};
+  float T[4][4];  // (F B^T)^T: T[i][j] = F[j] . B[i]
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) T[i][j] = dot4(F[j][0], F[j][1], F[j][2], F[j][3], B[i][0], B[i][1], B[i][2], B[i][3]);
+  const float rx = (x - cx) / cdx, ry = (y - cy) / cdy;
+  const float rx2 = rx * rx, ry2 = ry * ry;
+  const float xv[4] = {1.0f, rx, rx2, rx2 * rx};
+  const float yv[4] = {1.0f, ry, ry2, ry2 * ry};
+  float t[4];  // (B F B^T) y
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    float a[4];  // row i of B F B^T: a[j] = B[i] . T[j]
+#pragma unroll
+    for (int j = 0; j < 4; ++j) a[j] = dot4(B[i][0], B[i][1], B[i][2], B[i][3], T[j][0], T[j][1], T[j][2], T[j][3]);
+    t[i] = dot4(a[0], a[1], a[2], a[3], yv[0], yv[1], yv[2], yv[3]);
+  }
+  return dot4(xv[0], xv[1], xv[2], xv[3], t[0], t[1], t[2], t[3]);
+}
+
+// The geostrophic step of bicubicInterpolation (gpu_kernels/ocean_noise.cu:616-657)
+// at interior cell (tj, ti): d the interpolated perturbation there and at its
+// four neighbours; the balanced momentum uses the still depth only, and a cell
+// with a land corner is left unchanged.
+__device__ __forceinline__ void bicubic_balance_apply(float dx, float dy, float g, float f, float beta,
+                                                      float north_x, float north_y, const float* f_cells,
+                                                      int f_pitch, float d, float d_xm, float d_xp, float d_ym,
+                                                      float d_yp, float* eta, float* hu, float* hv, int pitch,
+                                                      const float* Hi, int h_pitch, float land_value, int tj,
+                                                      int ti) {
+  const float h00 = crow(Hi, h_pitch, tj)[ti], h01 = crow(Hi, h_pitch, tj)[ti + 1];
+  const float h10 = crow(Hi, h_pitch, tj + 1)[ti], h11 = crow(Hi, h_pitch, tj + 1)[ti + 1];
+  const bool dry = (h00 == land_value) || (h01 == land_value) || (h10 == land_value) || (h11 == land_value);
+  const float H_mid = 0.25f * (h00 + h01 + h10 + h11);
+  float coriolis = f + beta * (((float)ti + 0.5f) * dx * north_x + ((float)tj + 0.5f) * dy * north_y);
+  if (f == 0.0f) coriolis = f_cells ? crow(f_cells, f_pitch, tj)[ti] : 0.0f;
+  const float ddx = (d_xp - d_xm) / (2.0f * dx);
+  const float ddy = (d_yp - d_ym) / (2.0f * dy);
+  const float gc = g / coriolis;
+  float d_hu = -gc * (H_mid + d) * ddy;
+  float d_hv = gc * (H_mid + d) * ddx;
+  if (dry) d = d_hu = d_hv = 0.0f;
+  mrow(eta, pitch, tj)[ti] += d;
+  mrow(hu, pitch, tj)[ti] += d_hu;
+  mrow(hv, pitch, tj)[ti] += d_hv;
+}
+
 }  // namespace gpo

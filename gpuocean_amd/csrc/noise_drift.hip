@@ -77,6 +77,36 @@ __global__ void geo_balance(int nx, int ny, float dx, float dy, float g, float f
                h_pitch, land_value, tj, ti);
 }
 
+// bicubicInterpolation (gpu_kernels/ocean_noise.cu:401-659): a 64 x 4 tile
+// of interior cells interpolates the coarse perturbation for itself and a
+// one-cell ring into LDS, then adds it and its balanced momentum.
+constexpr int kBiW = 64, kBiH = 4;
+struct BicubicGeo {
+  int nx, ny, cnx, cny, oi, oj;
+  float dx, dy, cdx, cdy;
+};
+__global__ __launch_bounds__(kBiW* kBiH) void bicubic_balance(BicubicGeo q, float g, float f, float beta,
+                                                               float north_x, float north_y, const float* f_cells,
+                                                               int f_pitch, const float* coarse, int c_pitch,
+                                                               float* eta, float* hu, float* hv, int pitch,
+                                                               const float* Hi, int h_pitch, float land_value) {
+  __shared__ float d_eta[kBiH + 2][kBiW + 2];
+  const int bx = blockIdx.x * kBiW + 2, by = blockIdx.y * kBiH + 2;  // first interior cell of the tile
+  const int tid = threadIdx.y * kBiW + threadIdx.x;
+  for (int k = tid; k < (kBiH + 2) * (kBiW + 2); k += kBiW * kBiH) {
+    const int j = k / (kBiW + 2), i = k % (kBiW + 2);
+    d_eta[j][i] = bicubic_fine(q.cnx, q.cny, q.dx, q.dy, q.cdx, q.cdy, q.oi, q.oj, coarse, c_pitch, by + j - 1,
+                               bx + i - 1);
+  }
+  __syncthreads();
+  const int ti = bx + threadIdx.x, tj = by + threadIdx.y;
+  if (ti >= q.nx + 2 || tj >= q.ny + 2) return;
+  const int x = threadIdx.x + 1, y = threadIdx.y + 1;
+  bicubic_balance_apply(q.dx, q.dy, g, f, beta, north_x, north_y, f_cells, f_pitch, d_eta[y][x], d_eta[y][x - 1],
+                        d_eta[y][x + 1], d_eta[y - 1][x], d_eta[y + 1][x], eta, hu, hv, pitch, Hi, h_pitch,
+                        land_value, tj, ti);
+}
+
 __global__ void drift(int nx, int ny, float dx, float dy, float dt, int x0ref, int y0ref, const float* eta,
                       const float* hu, const float* hv, int pitch, const float* Hm, int hm_pitch, int per_ns,
                       int per_ew, int n_drifters, float* pos, int dpitch, float sensitivity, const float* wx_cur,
@@ -146,6 +176,23 @@ int gpo_geostrophic_balance(int32_t nx, int32_t ny, float dx, float dy, float g,
                      ny, dx, dy, g, f, beta, north_x, north_y, f_cells, f_pitch, coarse, coarse_pitch, eta, hu, hv,
                      pitch, Hi, Hi_pitch, land_value);
   return check(hipGetLastError(), "gpo_geostrophic_balance launch");
+}
+
+int gpo_bicubic_interpolation(int32_t nx, int32_t ny, float dx, float dy, int32_t coarse_nx, int32_t coarse_ny,
+                              float coarse_dx, float coarse_dy, int32_t offset_i, int32_t offset_j, float g, float f,
+                              float beta, float north_x, float north_y, const float* f_cells, int32_t f_pitch,
+                              const float* coarse, int32_t coarse_pitch, float* eta, float* hu, float* hv,
+                              int32_t pitch, const float* Hi, int32_t Hi_pitch, float land_value, void* stream) {
+  if (nx < 1 || ny < 1 || coarse_nx < 1 || coarse_ny < 1 || !coarse || !eta || !hu || !hv || !Hi ||
+      !(coarse_dx > 0.0f) || !(coarse_dy > 0.0f))
+    return bad("gpo_bicubic_interpolation: bad arguments");
+  if (offset_i < -coarse_nx || offset_i > coarse_nx || offset_j < -coarse_ny || offset_j > coarse_ny)
+    return bad("gpo_bicubic_interpolation: offset larger than the coarse grid");
+  const BicubicGeo q{nx, ny, coarse_nx, coarse_ny, offset_i, offset_j, dx, dy, coarse_dx, coarse_dy};
+  hipLaunchKernelGGL(bicubic_balance, dim3(ceil_div(nx, kBiW), ceil_div(ny, kBiH)), dim3(kBiW, kBiH), 0,
+                     (hipStream_t)stream, q, g, f, beta, north_x, north_y, f_cells, f_pitch, coarse, coarse_pitch,
+                     eta, hu, hv, pitch, Hi, Hi_pitch, land_value);
+  return check(hipGetLastError(), "gpo_bicubic_interpolation launch");
 }
 
 int gpo_drift(int32_t nx, int32_t ny, float dx, float dy, float dt, int32_t x0ref, int32_t y0ref, const float* eta,

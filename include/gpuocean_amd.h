@@ -175,6 +175,21 @@ int gpo_geostrophic_balance(int32_t nx, int32_t ny, float dx, float dy, float g,
                             const float* coarse, int32_t coarse_pitch, float* eta, float* hu, float* hv,
                             int32_t pitch, const float* Hi, int32_t Hi_pitch, float land_value, void* stream);
 
+/* Adds a perturbation generated on a coarse grid (coarse_nx x coarse_ny cells
+ * of coarse_dx x coarse_dy, buffer (coarse_ny+4) x (coarse_nx+4), pitch
+ * coarse_pitch bytes) to the interior state: bicubic interpolation from the
+ * coarse cell centres (shifted by offset_i, offset_j fine cells) onto the fine
+ * cell centres, and the geostrophically balanced hu, hv from the still depth;
+ * cells with a land corner are left unchanged.  Two ghost cells on both grids.
+ * Replaces bicubicInterpolation (gpu_kernels/ocean_noise.cu:401-659) launched by
+ * OceanStateNoise.perturbOceanState when interpolation_factor > 1
+ * (SWEsimulators/OceanStateNoise.py:418-441). */
+int gpo_bicubic_interpolation(int32_t nx, int32_t ny, float dx, float dy, int32_t coarse_nx, int32_t coarse_ny,
+                              float coarse_dx, float coarse_dy, int32_t offset_i, int32_t offset_j, float g, float f,
+                              float beta, float north_x, float north_y, const float* f_cells, int32_t f_pitch,
+                              const float* coarse, int32_t coarse_pitch, float* eta, float* hu, float* hv,
+                              int32_t pitch, const float* Hi, int32_t Hi_pitch, float land_value, void* stream);
+
 /* ---- drifters ----------------------------------------------------------------- */
 
 /* One forward-Euler step of n_drifters + 1 passive drifters (the last one is the

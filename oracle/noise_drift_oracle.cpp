@@ -4,8 +4,10 @@
 // specifies.  Random numbers are an input: the tests pass the generator's
 // output so SOAR, balance and drift are checked on identical xi.
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <vector>
 
 #include "oracle.h"
 
@@ -56,6 +58,106 @@ void gpo_oracle_geostrophic_balance(int nx, int ny, float dx, float dy, float g,
       eta[k] = e;
       hu[k] = u;
       hv[k] = v;
+    }
+}
+
+}  // extern "C"
+
+namespace {
+
+float dot4(const float* a, const float* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + a[3] * b[3]; }
+
+// Bicubic value at fine cell (tj, ti) of the coarse buffer (cny+4) x (cnx+4):
+// the interpolation loop of bicubicInterpolation (gpu_kernels/ocean_noise.cu:523-611),
+// bicubic_interpolation_coefficients (interpolation.cu:39-117) and
+// bicubic_evaluation (interpolation.cu:119-133).
+float bicubic_value(int cnx, int cny, float dx, float dy, float cdx, float cdy, int oi, int oj, const float* coarse,
+                    int tj, int ti) {
+  const float x = ((float)(ti - 2) + 0.5f + (float)oi) * dx;
+  const float y = ((float)(tj - 2) + 0.5f + (float)oj) * dy;
+  int ci = (int)std::floor(x / cdx + 2.0f - 0.5f);
+  int cj = (int)std::floor(y / cdy + 2.0f - 0.5f);
+  if (ci > cnx + 1) ci -= 1;
+  if (cj > cny + 1) cj -= 1;
+  const float cx = ((float)(ci - 2) + 0.5f) * cdx;
+  const float cy = ((float)(cj - 2) + 0.5f) * cdy;
+  const int CW = cnx + 4;
+  auto c = [&](int j, int i) {
+    j = std::min(std::max(j, 0), cny + 3);
+    i = std::min(std::max(i, 0), cnx + 3);
+    return coarse[(size_t)j * CW + i];
+  };
+  // f matrix of corner values and central-difference derivatives (ocean_noise.cu:554-601)
+  float f[4][4];
+  f[0][0] = c(cj, ci);
+  f[0][1] = c(cj + 1, ci);
+  f[1][0] = c(cj, ci + 1);
+  f[1][1] = c(cj + 1, ci + 1);
+  f[2][0] = (c(cj, ci + 1) - c(cj, ci - 1)) / 2.0f;
+  f[2][1] = (c(cj + 1, ci + 1) - c(cj + 1, ci - 1)) / 2.0f;
+  f[3][0] = (c(cj, ci + 2) - c(cj, ci)) / 2.0f;
+  f[3][1] = (c(cj + 1, ci + 2) - c(cj + 1, ci)) / 2.0f;
+  f[0][2] = (c(cj + 1, ci) - c(cj - 1, ci)) / 2.0f;
+  f[0][3] = (c(cj + 2, ci) - c(cj, ci)) / 2.0f;
+  f[1][2] = (c(cj + 1, ci + 1) - c(cj - 1, ci + 1)) / 2.0f;
+  f[1][3] = (c(cj + 2, ci + 1) - c(cj, ci + 1)) / 2.0f;
+  const float fy_10 = (c(cj + 1, ci - 1) - c(cj - 1, ci - 1)) / 2.0f;
+  const float fy_11 = (c(cj + 2, ci - 1) - c(cj, ci - 1)) / 2.0f;
+  const float fy20 = (c(cj + 1, ci + 2) - c(cj - 1, ci + 2)) / 2.0f;
+  const float fy21 = (c(cj + 2, ci + 2) - c(cj, ci + 2)) / 2.0f;
+  f[2][2] = (f[1][2] - fy_10) / 2.0f;
+  f[2][3] = (f[1][3] - fy_11) / 2.0f;
+  f[3][2] = (fy20 - f[0][2]) / 2.0f;
+  f[3][3] = (fy21 - f[0][3]) / 2.0f;
+  static const float b[4][4] = {{1, 0, 0, 0}, {0, 0, 1, 0}, {-3, 3, -2, -1}, {2, -2, 1, 1}};
+  float fbT[4][4], a[4][4];
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) fbT[i][j] = dot4(f[j], b[i]);
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) a[i][j] = dot4(b[i], fbT[j]);
+  const float rx = (x - cx) / cdx, ry = (y - cy) / cdy;
+  const float xv[4] = {1.0f, rx, rx * rx, rx * rx * rx};
+  const float yv[4] = {1.0f, ry, ry * ry, ry * ry * ry};
+  float tmp[4];
+  for (int i = 0; i < 4; ++i) tmp[i] = dot4(a[i], yv);
+  return dot4(xv, tmp);
+}
+
+}  // namespace
+
+extern "C" {
+
+// bicubicInterpolation (gpu_kernels/ocean_noise.cu:401-659): the interpolated
+// perturbation d_eta (interior and first ghost ring) and its balanced hu, hv
+// from the still depth, added to the interior; cells with a land corner
+// unchanged.  Dense arrays: state and f_cells rows of nx+4, Hi rows of nx+5.
+void gpo_oracle_bicubic_interpolation(int nx, int ny, float dx, float dy, int cnx, int cny, float cdx, float cdy,
+                                      int oi, int oj, float g, float f, float beta, float north_x, float north_y,
+                                      const float* f_cells, const float* coarse, float* eta, float* hu, float* hv,
+                                      const float* Hi, float land_value) {
+  const int W = nx + 4, WI = nx + 5;
+  std::vector<float> d((size_t)(ny + 4) * W, 0.0f);
+  for (int tj = 1; tj < ny + 3; ++tj)
+    for (int ti = 1; ti < nx + 3; ++ti)
+      d[(size_t)tj * W + ti] = bicubic_value(cnx, cny, dx, dy, cdx, cdy, oi, oj, coarse, tj, ti);
+  for (int tj = 2; tj < ny + 2; ++tj)
+    for (int ti = 2; ti < nx + 2; ++ti) {
+      const float h00 = Hi[(size_t)tj * WI + ti], h01 = Hi[(size_t)tj * WI + ti + 1];
+      const float h10 = Hi[(size_t)(tj + 1) * WI + ti], h11 = Hi[(size_t)(tj + 1) * WI + ti + 1];
+      const bool dry = (h00 == land_value) || (h01 == land_value) || (h10 == land_value) || (h11 == land_value);
+      const float H_mid = 0.25f * (h00 + h01 + h10 + h11);
+      float coriolis = f + beta * (((float)ti + 0.5f) * dx * north_x + ((float)tj + 0.5f) * dy * north_y);
+      if (f == 0.0f) coriolis = f_cells ? f_cells[(size_t)tj * W + ti] : 0.0f;
+      const size_t k = (size_t)tj * W + ti;
+      const float eta_diff_x = (d[k + 1] - d[k - 1]) / (2.0f * dx);
+      const float eta_diff_y = (d[k + W] - d[k - W]) / (2.0f * dy);
+      float dd = d[k];
+      float d_hu = -(g / coriolis) * (H_mid + dd) * eta_diff_y;
+      float d_hv = (g / coriolis) * (H_mid + dd) * eta_diff_x;
+      if (dry) dd = d_hu = d_hv = 0.0f;
+      eta[k] += dd;
+      hu[k] += d_hu;
+      hv[k] += d_hv;
     }
 }
 

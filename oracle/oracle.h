@@ -66,6 +66,10 @@ void gpo_oracle_soar(int nx, int ny, const float* W, float scale, int per_ns, in
 void gpo_oracle_geostrophic_balance(int nx, int ny, float dx, float dy, float g, float f, float beta, float north_x,
                                     float north_y, const float* f_cells, const float* coarse, float* eta, float* hu,
                                     float* hv, const float* Hi, float land_value);
+void gpo_oracle_bicubic_interpolation(int nx, int ny, float dx, float dy, int cnx, int cny, float cdx, float cdy,
+                                      int oi, int oj, float g, float f, float beta, float north_x, float north_y,
+                                      const float* f_cells, const float* coarse, float* eta, float* hu, float* hv,
+                                      const float* Hi, float land_value);
 void gpo_oracle_drift(int nx, int ny, float dx, float dy, float dt, int x0ref, int y0ref, const float* eta,
                       const float* hu, const float* hv, const float* Hm, int per_ns, int per_ew, int n_drifters,
                       float* pos, float sensitivity, float wind_x, float wind_y, float wind_drift_factor);

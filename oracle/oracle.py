@@ -297,6 +297,19 @@ def geostrophic_balance(nx, ny, dx, dy, g, f, beta, north_x, north_y, f_cells, c
                                      _fp(hv), _fp(np.ascontiguousarray(Hi, np.float32)), ctypes.c_float(land_value))
 
 
+def bicubic_interpolation(nx, ny, dx, dy, cnx, cny, cdx, cdy, oi, oj, g, f, beta, north_x, north_y, f_cells, coarse,
+                          eta, hu, hv, Hi, land_value=np.float32(1e20)):
+    """Adds the coarse perturbation, bicubically interpolated, and its balanced
+    momentum to eta/hu/hv in place (float32 (ny+4) x (nx+4) arrays)."""
+    L = lib()
+    args = [ctypes.c_int(nx), ctypes.c_int(ny), ctypes.c_float(dx), ctypes.c_float(dy), ctypes.c_int(cnx),
+            ctypes.c_int(cny), ctypes.c_float(cdx), ctypes.c_float(cdy), ctypes.c_int(oi), ctypes.c_int(oj)]
+    args += [ctypes.c_float(v) for v in (g, f, beta, north_x, north_y)]
+    fc = None if f_cells is None else _fp(np.ascontiguousarray(f_cells, np.float32))
+    L.gpo_oracle_bicubic_interpolation(*args, fc, _fp(np.ascontiguousarray(coarse, np.float32)), _fp(eta), _fp(hu),
+                                       _fp(hv), _fp(np.ascontiguousarray(Hi, np.float32)), ctypes.c_float(land_value))
+
+
 def drift(nx, ny, dx, dy, dt, eta, hu, hv, Hm, per_ns, per_ew, pos, sensitivity=1.0, wind=(0.0, 0.0), factor=0.0):
     """One drifter step in place on pos ((n+1) x 2 float32)."""
     L = lib()
