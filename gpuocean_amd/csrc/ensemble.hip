@@ -47,6 +47,14 @@ constexpr int kTileW = 64, kTileH = 16;
 constexpr int kXiW = kTileW + 6, kXiH = kTileH + 6;  // xi: coarse halo 1 + SOAR radius 2 (+ column offset)
 constexpr int kCoW = kTileW + 2, kCoH = kTileH + 2;  // coarse: balance halo 1
 
+// x mod n for n >= 1, without a division when x lies in [-n, 2n).
+__device__ __forceinline__ int wrap(int x, int n) {
+  if (x < 0) x += n;
+  if (x >= n) x -= n;
+  if (x < 0 || x >= n) x = ((x % n) + n) % n;
+  return x;
+}
+
 __global__ void __launch_bounds__(256) noise_fused(const NoiseMember* __restrict__ members,
                                                    const int2* __restrict__ tiles,
                                                    const uint32_t* __restrict__ draw) {
@@ -62,17 +70,32 @@ __global__ void __launch_bounds__(256) noise_fused(const NoiseMember* __restrict
   const int nrow = cj_hi + 4 - (T0 - 1) + 1, ncol = ci_hi + 4 - (S0 - 1) + 1;
   const int cols = m.xi_cols;
 
+  // 0. the state and depths of this thread's four cells, loaded first so that
+  // the HBM reads overlap the random numbers and SOAR below
+  const int lx = threadIdx.x & 63, ly0 = threadIdx.x >> 6;
+  constexpr int kRowsPerThread = kTileH / 4;
+  BalanceIn in[kRowsPerThread];
+#pragma unroll
+  for (int k = 0; k < kRowsPerThread; ++k) {
+    const int tj = T0 + ly0 + 4 * k, ti = S0 + lx;
+    if (tj <= m.ny + 1 && ti <= m.nx + 1)
+      in[k] = balance_load(m.f == 0.0f ? m.f_cells : nullptr, m.f_pitch, m.eta, m.hu, m.hv, m.pitch, m.Hi,
+                           m.hi_pitch, tj, ti);
+  }
+
   // 1. normal field into LDS
-  const int a0 = m.per_ew ? ((S0 - 1 - 4 + 2 * m.nx) % m.nx) : (S0 - 1);  // actual column of U = S0-1
+  const int a0 = m.per_ew ? wrap(S0 - 1 - 4, m.nx) : (S0 - 1);  // actual column of U = S0-1
   if (!m.per_ew || ncol <= m.nx) {
     // the needed actual columns form one run, or two when a periodic run wraps
     const int seg_len0 = m.per_ew ? min(ncol, m.nx - a0) : ncol;
     constexpr int kMaxQ = kXiW / 4 + 2;
-    for (int u = threadIdx.x; u < kXiH * 2 * kMaxQ; u += 256) {
-      const int lr = u / (2 * kMaxQ), rem = u - lr * (2 * kMaxQ), seg = rem / kMaxQ, j = rem - seg * kMaxQ;
-      if (lr >= nrow) continue;
+    // work items (segment, row, quad): the second segment exists only where a
+    // periodic window wraps, so most tiles keep every lane busy
+    const int n_seg = seg_len0 < ncol ? 2 : 1;
+    for (int u = threadIdx.x; u < n_seg * nrow * kMaxQ; u += 256) {
+      const int seg = u / (nrow * kMaxQ), rem = u - seg * (nrow * kMaxQ), lr = rem / kMaxQ, j = rem - lr * kMaxQ;
       const int V = T0 - 1 + lr;
-      const int r = m.per_ns ? (V - 4 + 2 * m.ny) % m.ny : V;
+      const int r = m.per_ns ? wrap(V - 4, m.ny) : V;
       const int ca = seg == 0 ? a0 : 0;
       const int len = seg == 0 ? seg_len0 : ncol - seg_len0;
       if (len <= 0) continue;
@@ -82,12 +105,14 @@ __global__ void __launch_bounds__(256) noise_fused(const NoiseMember* __restrict
       if (q > ((base + (uint32_t)cb) >> 2)) continue;
       float v4[4];
       normals4(q, d, m.stream_id, m.k0, m.k1, v4);
+      // window column of actual column c: c - a0 in the first run, c - a0 + nx
+      // in the wrapped second run (a0 = S0 - 1 without periodicity)
+      const int c0 = (int)(4u * q - base), shift = seg == 0 ? -a0 : m.nx - a0;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const int c = (int)(4u * q + (uint32_t)i - base);
+        const int c = c0 + i;
         if (c < ca || c > cb) continue;
-        const int lc = m.per_ew ? (c - a0 + m.nx) % m.nx : c - (S0 - 1);
-        xs[lr][lc] = v4[i];
+        xs[lr][c + shift] = v4[i];
       }
     }
   } else {
@@ -117,13 +142,15 @@ __global__ void __launch_bounds__(256) noise_fused(const NoiseMember* __restrict
   __syncthreads();
 
   // 3. geostrophic balance on the tile
-  const int lx = threadIdx.x & 63;
-  for (int ly = threadIdx.x >> 6; ly < kTileH; ly += 4) {
+#pragma unroll
+  for (int k = 0; k < kRowsPerThread; ++k) {
+    const int ly = ly0 + 4 * k;
     const int tj = T0 + ly, ti = S0 + lx;
     if (tj > m.ny + 1 || ti > m.nx + 1) continue;
-    balance_apply(m.dx, m.dy, m.g, m.f, m.beta, m.north_x, m.north_y, m.f_cells, m.f_pitch, cs[ly + 1][lx + 1],
-                  cs[ly + 1][lx], cs[ly + 1][lx + 2], cs[ly][lx + 1], cs[ly + 2][lx + 1], m.eta, m.hu, m.hv, m.pitch,
-                  m.Hi, m.hi_pitch, m.land, tj, ti);
+    float e, u, v;
+    balance_compute(m.dx, m.dy, m.g, m.f, m.beta, m.north_x, m.north_y, cs[ly + 1][lx + 1], cs[ly + 1][lx],
+                    cs[ly + 1][lx + 2], cs[ly][lx + 1], cs[ly + 2][lx + 1], in[k], m.land, tj, ti, &e, &u, &v);
+    balance_store(m.eta, m.hu, m.hv, m.pitch, tj, ti, e, u, v);
   }
 }
 

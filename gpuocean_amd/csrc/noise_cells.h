@@ -34,38 +34,66 @@ __device__ __forceinline__ float soar_cell(int nx, int ny, const float* W, int p
   return acc;
 }
 
+// What the balance of one cell reads from memory: its state, the four
+// corner depths and (when f is a texture) its coriolis parameter.
+struct BalanceIn {
+  float e, u, v, h00, h01, h10, h11, fc;
+};
+__device__ __forceinline__ BalanceIn balance_load(const float* f_cells, int f_pitch, const float* eta,
+                                                  const float* hu, const float* hv, int pitch, const float* Hi,
+                                                  int h_pitch, int tj, int ti) {
+  BalanceIn b;
+  b.e = crow(eta, pitch, tj)[ti];
+  b.u = crow(hu, pitch, tj)[ti];
+  b.v = crow(hv, pitch, tj)[ti];
+  b.h00 = crow(Hi, h_pitch, tj)[ti];
+  b.h01 = crow(Hi, h_pitch, tj)[ti + 1];
+  b.h10 = crow(Hi, h_pitch, tj + 1)[ti];
+  b.h11 = crow(Hi, h_pitch, tj + 1)[ti + 1];
+  b.fc = f_cells ? crow(f_cells, f_pitch, tj)[ti] : 0.0f;
+  return b;
+}
+
 // geostrophicBalance (gpu_kernels/ocean_noise.cu:256-390) at interior cell
 // (tj, ti), coarse grid equal to the computational grid, given the coarse
-// value d at the cell and at its four neighbours.
-__device__ __forceinline__ void balance_apply(float dx, float dy, float g, float f, float beta, float north_x,
-                                              float north_y, const float* f_cells, int f_pitch, float d, float d_xm,
-                                              float d_xp, float d_ym, float d_yp, float* eta, float* hu, float* hv,
-                                              int pitch, const float* Hi, int h_pitch, float land_value, int tj,
-                                              int ti) {
-  const float h00 = crow(Hi, h_pitch, tj)[ti], h01 = crow(Hi, h_pitch, tj)[ti + 1];
-  const float h10 = crow(Hi, h_pitch, tj + 1)[ti], h11 = crow(Hi, h_pitch, tj + 1)[ti + 1];
-  const bool dry = (h00 == land_value) || (h01 == land_value) || (h10 == land_value) || (h11 == land_value);
-  const float H_mid = 0.25f * (h00 + h01 + h10 + h11);
+// value d at the cell and at its four neighbours and what balance_load read:
+// the new (eta, hu, hv) of the cell.
+__device__ __forceinline__ void balance_compute(float dx, float dy, float g, float f, float beta, float north_x,
+                                                float north_y, float d, float d_xm, float d_xp, float d_ym,
+                                                float d_yp, const BalanceIn& b, float land_value, int tj, int ti,
+                                                float* e, float* u, float* v) {
+  const bool dry = (b.h00 == land_value) || (b.h01 == land_value) || (b.h10 == land_value) || (b.h11 == land_value);
+  const float H_mid = 0.25f * (b.h00 + b.h01 + b.h10 + b.h11);
   float coriolis = f + beta * (((float)ti + 0.5f) * dx * north_x + ((float)tj + 0.5f) * dy * north_y);
-  if (f == 0.0f) coriolis = f_cells ? crow(f_cells, f_pitch, tj)[ti] : 0.0f;
-  float* er = mrow(eta, pitch, tj);
-  float* ur = mrow(hu, pitch, tj);
-  float* vr = mrow(hv, pitch, tj);
-  const float h_mid = d + H_mid + er[ti];
+  if (f == 0.0f) coriolis = b.fc;
+  const float h_mid = d + H_mid + b.e;
   const float ddx = (d_xp - d_xm) / (2.0f * dx);
   const float ddy = (d_yp - d_ym) / (2.0f * dy);
   const float gc = g / coriolis;
   const float d_hu = -gc * h_mid * ddy;
   const float d_hv = gc * h_mid * ddx;
-  float e = er[ti] + d, u = ur[ti] + d_hu, v = vr[ti] + d_hv;
-  if (dry) {
-    e = 0.0f;
-    u = 0.0f;
-    v = 0.0f;
-  }
-  er[ti] = e;
-  ur[ti] = u;
-  vr[ti] = v;
+  *e = dry ? 0.0f : b.e + d;
+  *u = dry ? 0.0f : b.u + d_hu;
+  *v = dry ? 0.0f : b.v + d_hv;
+}
+
+__device__ __forceinline__ void balance_store(float* eta, float* hu, float* hv, int pitch, int tj, int ti, float e,
+                                              float u, float v) {
+  mrow(eta, pitch, tj)[ti] = e;
+  mrow(hu, pitch, tj)[ti] = u;
+  mrow(hv, pitch, tj)[ti] = v;
+}
+
+__device__ __forceinline__ void balance_apply(float dx, float dy, float g, float f, float beta, float north_x,
+                                              float north_y, const float* f_cells, int f_pitch, float d, float d_xm,
+                                              float d_xp, float d_ym, float d_yp, float* eta, float* hu, float* hv,
+                                              int pitch, const float* Hi, int h_pitch, float land_value, int tj,
+                                              int ti) {
+  const BalanceIn b = balance_load(f_cells, f_pitch, eta, hu, hv, pitch, Hi, h_pitch, tj, ti);
+  float e, u, v;
+  balance_compute(dx, dy, g, f, beta, north_x, north_y, d, d_xm, d_xp, d_ym, d_yp, b, land_value, tj, ti, &e, &u,
+                  &v);
+  balance_store(eta, hu, hv, pitch, tj, ti, e, u, v);
 }
 
 // The same, reading the coarse field from a pitched array.

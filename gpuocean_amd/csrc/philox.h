@@ -39,15 +39,18 @@ __host__ __device__ inline U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1) {
 }
 
 // Standard normal pair from two 32-bit words: uniforms at bin centres of a
-// 2^-24 grid (never 0 or 1), r = sqrt(-2 ln u1), angle 2 pi u2.
+// 2^-24 grid (never 0 or 1), r = sqrt(-2 ln u1), angle 2 pi u2.  The
+// transcendentals are the hardware ones: v_log_f32 (log2, u1 is a normal
+// number), v_sqrt_f32, and v_sin_f32 / v_cos_f32, whose argument is in
+// revolutions, so u2 itself is the angle 2 pi u2.  Their errors are a few
+// ulp, far below what the statistics of the field can show.
 __device__ inline void box_muller(uint32_t a, uint32_t b, float* z0, float* z1) {
   const float u1 = (float)(a >> 8) * 0x1p-24f + 0x1p-25f;
   const float u2 = (float)(b >> 8) * 0x1p-24f + 0x1p-25f;
-  const float r = sqrtf(-2.0f * logf(u1));
-  float s, c;
-  sincospif(2.0f * u2, &s, &c);
-  *z0 = r * c;
-  *z1 = r * s;
+  constexpr float kMinus2Ln2 = -1.38629436f;  // -2 ln 2: ln u1 = ln 2 * log2 u1
+  const float r = __builtin_amdgcn_sqrtf(kMinus2Ln2 * __builtin_amdgcn_logf(u1));
+  *z0 = r * __builtin_amdgcn_cosf(u2);
+  *z1 = r * __builtin_amdgcn_sinf(u2);
 }
 
 // The normal number of element `index` of draw `draw` in stream `stream`:

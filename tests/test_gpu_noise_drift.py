@@ -52,9 +52,11 @@ def test_random_normal_statistics_and_reproducibility(gpu_ctx):
     assert np.array_equal(a, _normals(1024, 2048, 1234, 7, 0))       # seeded reproducibility
     assert not np.array_equal(a, _normals(1024, 2048, 1234, 7, 1))   # next draw differs
     assert not np.array_equal(a, _normals(1024, 2048, 1234, 8, 0))   # other stream differs
-    # against the generator restated on the host (FP64 Box-Muller)
-    flat = a.reshape(-1)[:64]
-    for idx in range(64):
+    # against the generator restated on the host (FP64 Box-Muller), over the
+    # first 2048 elements: the hardware log2/sin/cos keep every normal within
+    # 1e-5 of it
+    flat = a.reshape(-1)[:2048]
+    for idx in range(2048):
         words = O.philox4x32_10((idx >> 2, 0, 7, 0x534F4152), (1234, 0))
         x0, x1 = (words[0], words[1]) if (idx & 2) == 0 else (words[2], words[3])
         u1 = (x0 >> 8) * 2.0 ** -24 + 2.0 ** -25
