@@ -252,6 +252,88 @@ class BatchedEnsemble:
         for s, dt in zip(self.sims, self._dts):
             s._after_batched_steps(int(n_steps), dt, ghosts_written=self._bc_in_batch(s))
 
+    def _select(self, active=None, scales=None):
+        a = None if active is None else (ctypes.c_uint8 * self.n)(*[1 if x else 0 for x in active])
+        sc = None if scales is None else (ctypes.c_float * self.n)(*[float(np.float32(x)) for x in scales])
+        L.check(L.lib().gpo_ensemble_select(self.handle, a, sc, self._sh()), "gpo_ensemble_select")
+
+    def _parts(self, parts, count=1):
+        sh = self._sh()
+        lib = L.lib()
+        for _ in range(count):
+            for part in parts:
+                L.check(lib.gpo_ensemble_part(self.handle, int(part), sh), "gpo_ensemble_part")
+
+    def model_step(self, seqs, noise_scales):
+        """One model time step of each member as CDKLM16.dataAssimilationStep
+        takes it (SWEsimulators/CDKLM16.py:776-795): member p takes the time
+        steps seqs[p] (none: it is left alone) with no model error between
+        them, then, where noise_scales[p] is not None, the model error once
+        with that perturbation scale.  Members with different step counts
+        share launches; each launch acts on the members still stepping.  Host
+        bookkeeping (time, iteration count, drifter time, draw index) follows
+        the per-member path."""
+        sims = self.sims
+        if not any(len(q) for q in seqs):
+            return
+        for s, q in zip(sims, seqs):
+            if q and s.t == 0:   # CDKLM16.step writes the ghost cells at t = 0 first
+                s._update_bc_values()
+                s._apply_bc(s.h0)
+        self._sync_in()
+        if self._dts is None or self._draws is None:
+            self._dts, self._draws = None, None
+            self._refresh_all()
+        k_max = max(len(q) for q in seqs)
+        j = 0
+        while j < k_max:
+            act = [j < len(q) for q in seqs]
+            dts = [np.float32(q[j]) if a else self._dts[p] for p, (q, a) in enumerate(zip(seqs, act))]
+            j2 = j + 1
+            while j2 < k_max and [j2 < len(q) for q in seqs] == act and \
+                    all(np.float32(q[j2]) == dts[p] for p, q in enumerate(seqs) if act[p]):
+                j2 += 1
+            if dts != self._dts:
+                arr = (ctypes.c_float * self.n)(*[float(d) for d in dts])
+                L.check(L.lib().gpo_ensemble_update(self.handle, arr, None, self._sh()), "gpo_ensemble_update")
+                self._dts = dts
+            self._select(None if all(act) else act)
+            self._parts((0, 1, 3), j2 - j)
+            j = j2
+        # ghost cells of the new state where the batched parts do not write them
+        # (the per-member path writes them after each step, before the model error)
+        self._sync_out()
+        for s, q in zip(sims, seqs):
+            if q and not self._bc_in_batch(s):
+                s._apply_bc(s.h0)
+        self._sync_in()
+        noisy = [bool(q) and sc is not None for q, sc in zip(seqs, noise_scales)]
+        if self.has_noise and any(noisy):
+            self._select(noisy, [1.0 if sc is None else sc for sc in noise_scales])
+            self._parts((4,))
+            self._select(None, [1.0] * self.n)
+        else:
+            self._select(None)
+        self._sync_out()
+        for p, (s, q) in enumerate(zip(sims, seqs)):
+            for dt in q:
+                s.t += np.float64(np.float32(dt))
+                s.num_iterations += 1
+                if s.drifters is not None:
+                    s.drifter_t += np.float32(dt)
+            if self.has_noise and noisy[p]:
+                s.model_error.draw += 1
+                s.model_error._last_scales = (1.0, noise_scales[p])
+                self._draws[p] += 1
+
+    def _refresh_all(self):
+        dts = [np.float32(s.dt) for s in self.sims]
+        draws = [s.model_error.draw if self.has_noise else 0 for s in self.sims]
+        dt_arr = (ctypes.c_float * self.n)(*[float(d) for d in dts])
+        dr_arr = (ctypes.c_uint32 * self.n)(*draws)
+        L.check(L.lib().gpo_ensemble_update(self.handle, dt_arr, dr_arr, self._sh()), "gpo_ensemble_update")
+        self._dts, self._draws = dts, draws
+
     def destroy(self):
         if self.handle is not None:
             torch.cuda.synchronize(self.device)
@@ -417,11 +499,64 @@ class OceanModelEnsemble:
                 p.cleanUp(do_gc=False)
 
     def stepToObservation(self, observation_time, write_now=False):
-        """dataAssimilationStep on every active member (OceanModelEnsemble.py:121-134)."""
-        for p in range(self.numParticles):
-            if self.particlesActive[p]:
+        """dataAssimilationStep on every active member (OceanModelEnsemble.py:121-134).
+        When the active members share their time and model time step (and
+        carry no flow relaxation), the whole cadence runs batched: the same
+        model steps, each member's own dt within them, the model error after
+        each, the dt updates every fifth model step, with every member's
+        results equal to its own dataAssimilationStep."""
+        active = [p for p in range(self.numParticles) if self.particlesActive[p]]
+        ens = self._batched_ensemble() if active else None
+        if ens is not None and self._da_batchable(active):
+            self._batched_da_step(ens, active, observation_time)
+        else:
+            for p in active:
                 self.particles[p].dataAssimilationStep(observation_time, write_now=write_now)
         self.t = observation_time
+
+    def _da_batchable(self, active):
+        s0 = self.particles[active[0]]
+        for p in active:
+            s = self.particles[p]
+            if s.t != s0.t or s.model_time_step != s0.model_time_step or s.total_time_steps != s0.total_time_steps:
+                return False
+            if s._needs_explicit_bc_each_substep():
+                return False
+        return True
+
+    def _batched_da_step(self, ens, active, observation_time, courant_number=0.8, model_error_final_step=True):
+        """CDKLM16.dataAssimilationStep (SWEsimulators/CDKLM16.py:742-806) of
+        the given members at once."""
+        s0 = self.particles[active[0]]
+        mts = s0.model_time_step
+        full = int(round(observation_time - s0.t) / mts)
+        leftover = observation_time - s0.t - full * mts
+        if leftover / mts < 0.1 and full > 1:
+            leftover += mts
+            full -= 1
+        if leftover / mts < 0.00001:
+            leftover = 0
+        assert full > 0, "There is less than CDKLM16.model_time_step until the observation"
+        self.updateDt(courant_number=courant_number, members=active)
+        on = set(active)
+        for i in range(full + 1):
+            if i == 0 and leftover == 0:
+                continue
+            target = leftover if i == 0 else mts
+            scale = np.sqrt(leftover / mts) if i == 0 else 1.0
+            noise = i == 0 or i < full or model_error_final_step
+            seqs = [self.particles[p].local_dts(target) if p in on else [] for p in range(self.numParticles)]
+            scales = [scale if (p in on and noise) else None for p in range(self.numParticles)]
+            ens.model_step(seqs, scales)
+            for p in active:
+                self.particles[p].total_time_steps += 1
+            if self.particles[active[0]].total_time_steps % 5 == 0:
+                self.updateDt(courant_number=courant_number, members=active)
+        for p in active:
+            s = self.particles[p]
+            assert round(observation_time) == round(s.t), \
+                "The simulation time is not the same as observation time after dataAssimilationStep! " \
+                f"(self.t, observation_time, diff): {(s.t, observation_time, s.t - observation_time)}"
 
     def modelStep(self, sub_t, rank="", update_dt=True):
         """Every member steps sub_t seconds, then (update_dt) takes a new dt
@@ -458,25 +593,27 @@ class OceanModelEnsemble:
             self.updateDt()
         return self.t
 
-    def updateDt(self, courant_number=None):
-        """New dt for every member from one batched set of CFL reductions and a
-        single read-back (replaces the per-member blocking updateDt loop)."""
-        n = self.numParticles
+    def updateDt(self, courant_number=None, members=None):
+        """New dt for every member (or the given member indices) from one
+        batched set of CFL reductions and a single read-back (replaces the
+        per-member blocking updateDt loop)."""
+        sims = self.particles if members is None else [self.particles[p] for p in members]
+        n = len(sims)
         dev = self.particles[0].device
         out = torch.empty(n, dtype=torch.float32, device=dev)
         stream = torch.cuda.Stream(device=dev)
         sh = ctypes.c_void_p(stream.cuda_stream)
-        for p in self.particles:
+        for p in sims:
             stream.wait_stream(p.gpu_stream)
         lib = L.lib()
         with torch.cuda.stream(stream):
-            for i, p in enumerate(self.particles):
+            for i, p in enumerate(sims):
                 L.check(lib.gpo_max_dt(int(p.nx), int(p.ny), float(p.dx), float(p.dy), float(p.g),
                                        float(p.mask_value), _vp(p.h0[0]), _vp(p.h0[1]), _vp(p.h0[2]), p.h0[0].pitch,
                                        _vp(p.Bm), p.Bm.pitch, ctypes.c_void_p(out.data_ptr() + 4 * i), sh),
                         "gpo_max_dt")
             host = out.cpu().numpy()
-        for i, p in enumerate(self.particles):
+        for i, p in enumerate(sims):
             c = p.courant_number if courant_number is None else courant_number
             p.dt = c * host[i:i + 1][0]
 

@@ -154,10 +154,11 @@ __global__ void __launch_bounds__(256) noise_fused(const NoiseMember* __restrict
   }
 }
 
-// Every member's draw index advances once the step's model error is applied.
-__global__ void advance_draws(uint32_t* draw, int n) {
+// Every member's draw index (or every selected one's) advances once the
+// step's model error is applied.
+__global__ void advance_draws(uint32_t* draw, int n, const uint8_t* active) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) draw[i] = draw[i] + 1u;
+  if (i < n && (!active || active[i])) draw[i] = draw[i] + 1u;
 }
 
 // Ghost cells of one member, as gpo_boundary_conditions writes them, on its
@@ -227,39 +228,69 @@ struct Ensemble {
   hipGraphExec_t exec = nullptr;
   int graph_steps = 0;
   std::vector<StepArgs> h_args;  // host copy of the argument table (2 per member)
+  // Host copies of the full tables, with the member of each BC and drifter
+  // entry, from which a selection of members builds its own tables.
+  std::vector<int2> h_work, h_tiles;
+  std::vector<NoiseMember> h_noise;
+  std::vector<BcMember> h_bc;
+  std::vector<DriftMember> h_drift;
+  std::vector<int> bc_member, drift_member;
+  // Tables the launches use: the full ones, or those of the selected members.
+  int selected = 0;
+  const int2* cur_work = nullptr;
+  const int2* cur_tiles = nullptr;
+  const BcMember* cur_bc = nullptr;
+  const DriftMember* cur_drift = nullptr;
+  int cur_n_work = 0, cur_n_tiles = 0, cur_n_bc = 0, cur_n_drift = 0;
+  int2* d_work_sel = nullptr;
+  int2* d_tiles_sel = nullptr;
+  BcMember* d_bc_sel = nullptr;
+  DriftMember* d_drift_sel = nullptr;
+  uint8_t* d_active = nullptr;
 };
+
+static void use_full_tables(Ensemble* e) {
+  e->selected = 0;
+  e->cur_work = e->d_work, e->cur_n_work = e->n_work;
+  e->cur_tiles = e->d_tiles, e->cur_n_tiles = e->n_tiles;
+  e->cur_bc = e->d_bc, e->cur_n_bc = e->n_bc;
+  e->cur_drift = e->d_drift, e->cur_n_drift = e->n_drift;
+}
 
 static int ensemble_launch_substep(Ensemble* e, int rk, hipStream_t s) {
   const hipError_t err =
-      e->exact ? launch_cdklm_batched_exact(e->d_args, e->d_work, e->n_work, rk, e->n_blocks, e->has_land,
+      e->exact ? launch_cdklm_batched_exact(e->d_args, e->cur_work, e->cur_n_work, rk, e->n_blocks, e->has_land,
                                             e->f_array, e->angle0, s)
-               : launch_cdklm_batched_fast(e->d_args, e->d_work, e->n_work, rk, e->n_blocks, e->has_land,
+               : launch_cdklm_batched_fast(e->d_args, e->cur_work, e->cur_n_work, rk, e->n_blocks, e->has_land,
                                            e->f_array, e->angle0, s);
   return err != hipSuccess ? check(err, "ensemble substep launch") : 0;
 }
 
 static int ensemble_launch_noise(Ensemble* e, hipStream_t s) {
   if (!e->has_noise) return 0;
-  hipLaunchKernelGGL(noise_fused, dim3(e->n_tiles), dim3(256), 0, s, e->d_noise, e->d_tiles, e->d_draw);
-  hipLaunchKernelGGL(advance_draws, dim3((e->n + 255) / 256), dim3(256), 0, s, e->d_draw, e->n);
+  if (e->cur_n_tiles > 0)
+    hipLaunchKernelGGL(noise_fused, dim3(e->cur_n_tiles), dim3(256), 0, s, e->d_noise, e->cur_tiles, e->d_draw);
+  hipLaunchKernelGGL(advance_draws, dim3((e->n + 255) / 256), dim3(256), 0, s, e->d_draw, e->n,
+                     (const uint8_t*)(e->selected ? e->d_active : nullptr));
   return check(hipGetLastError(), "ensemble noise launch");
 }
 
 static int ensemble_launch_bc(Ensemble* e, int target, hipStream_t s) {
-  if (e->n_bc == 0 || (target == 1 && !e->any_frs)) return 0;
+  if (e->cur_n_bc == 0 || (target == 1 && !e->any_frs)) return 0;
   for (int pass = 0; pass < 2; ++pass)
-    hipLaunchKernelGGL(bc_batched, dim3(e->bc_blocks, e->n_bc), dim3(256), 0, s, e->d_bc, pass, target);
+    hipLaunchKernelGGL(bc_batched, dim3(e->bc_blocks, e->cur_n_bc), dim3(256), 0, s, e->cur_bc, pass, target);
   return check(hipGetLastError(), "ensemble boundary launch");
 }
 
 static int ensemble_launch_drift(Ensemble* e, hipStream_t s) {
-  if (e->n_drift == 0) return 0;
-  hipLaunchKernelGGL(drift_batched, dim3(e->drift_blocks, e->n_drift), dim3(64), 0, s, e->d_drift, e->d_args);
+  if (e->cur_n_drift == 0) return 0;
+  hipLaunchKernelGGL(drift_batched, dim3(e->drift_blocks, e->cur_n_drift), dim3(64), 0, s, e->cur_drift, e->d_args);
   return check(hipGetLastError(), "ensemble drift launch");
 }
 
 // Part 0: substep 0 (+ flow-relaxation ghost cells of the stage); part 1:
-// substep 1; part 2: model error, ghost cells of the new state, drifters.
+// substep 1; part 2: model error, ghost cells of the new state, drifters;
+// part 3: part 2 without the model error; part 4: the model error alone.
 static int ensemble_launch_part(Ensemble* e, int part, hipStream_t s) {
   int rc = 0;
   if (part == 0) {
@@ -267,8 +298,10 @@ static int ensemble_launch_part(Ensemble* e, int part, hipStream_t s) {
     if (!rc) rc = ensemble_launch_bc(e, 1, s);
   } else if (part == 1) {
     rc = ensemble_launch_substep(e, 1, s);
-  } else {
+  } else if (part == 4) {
     rc = ensemble_launch_noise(e, s);
+  } else {
+    if (part == 2) rc = ensemble_launch_noise(e, s);
     if (!rc) rc = ensemble_launch_bc(e, 0, s);
     if (!rc) rc = ensemble_launch_drift(e, s);
   }
@@ -291,6 +324,11 @@ static void ensemble_free(Ensemble* e) {
   if (e->d_draw) (void)hipFree(e->d_draw);
   if (e->d_bc) (void)hipFree(e->d_bc);
   if (e->d_drift) (void)hipFree(e->d_drift);
+  if (e->d_work_sel) (void)hipFree(e->d_work_sel);
+  if (e->d_tiles_sel) (void)hipFree(e->d_tiles_sel);
+  if (e->d_bc_sel) (void)hipFree(e->d_bc_sel);
+  if (e->d_drift_sel) (void)hipFree(e->d_drift_sel);
+  if (e->d_active) (void)hipFree(e->d_active);
   delete e;
 }
 
@@ -399,6 +437,7 @@ int gpo_ensemble_create(const gpo_member* members, int32_t n, int32_t chunk_rows
       b.h1.e = M.h1_eta, b.h1.u = M.h1_hu, b.h1.v = M.h1_hv;
       b.frs = frs;
       bcs.push_back(b);
+      e->bc_member.push_back(m);
       max_bc_cells = std::max(max_bc_cells, cells::bc_target_cells(t));
       e->any_frs |= frs;
     }
@@ -424,6 +463,7 @@ int gpo_ensemble_create(const gpo_member* members, int32_t n, int32_t chunk_rows
       d.Bm = M.Bm;
       d.pos = M.positions;
       drifts.push_back(d);
+      e->drift_member.push_back(m);
       max_drift_rows = std::max(max_drift_rows, d.rows);
     }
     for (int sy = 0; sy < a.n_chunks; ++sy)
@@ -527,11 +567,27 @@ int gpo_ensemble_create(const gpo_member* members, int32_t n, int32_t chunk_rows
                                 hipMemcpyHostToDevice, s),
                  "gpo_ensemble_create copy");
   }
+  // buffers for the tables of a selection of members (at most the full sizes)
+  if (!rc) rc = check(hipMalloc(&e->d_work_sel, sizeof(int2) * work.size()), "gpo_ensemble_create hipMalloc");
+  if (!rc) rc = check(hipMalloc(&e->d_active, (size_t)n), "gpo_ensemble_create hipMalloc");
+  if (!rc && !tiles.empty())
+    rc = check(hipMalloc(&e->d_tiles_sel, sizeof(int2) * tiles.size()), "gpo_ensemble_create hipMalloc");
+  if (!rc && e->n_bc > 0)
+    rc = check(hipMalloc(&e->d_bc_sel, sizeof(BcMember) * bcs.size()), "gpo_ensemble_create hipMalloc");
+  if (!rc && e->n_drift > 0)
+    rc = check(hipMalloc(&e->d_drift_sel, sizeof(DriftMember) * drifts.size()), "gpo_ensemble_create hipMalloc");
   if (!rc) rc = check(hipStreamSynchronize(s), "gpo_ensemble_create sync");
   if (rc) {
     ensemble_free(e);
     return rc;
   }
+  e->h_work = std::move(work);
+  e->h_tiles = std::move(tiles);
+  e->h_noise = std::move(noise);
+  e->h_bc = std::move(bcs);
+  e->h_drift = std::move(drifts);
+  if (e->n_bc == 0) e->h_bc.clear(), e->bc_member.clear();
+  use_full_tables(e);
   *out = (gpo_ensemble)e;
   return 0;
 }
@@ -556,13 +612,14 @@ int gpo_ensemble_step(gpo_ensemble h, int32_t n_steps, void* stream) {
 
 int gpo_ensemble_part(gpo_ensemble h, int32_t part, void* stream) {
   Ensemble* e = (Ensemble*)h;
-  if (!e || part < 0 || part > 2) return bad("gpo_ensemble_part: bad arguments");
+  if (!e || part < 0 || part > 4) return bad("gpo_ensemble_part: bad arguments");
   return ensemble_launch_part(e, part, (hipStream_t)stream);
 }
 
 int gpo_ensemble_capture(gpo_ensemble h, int32_t steps_per_graph, void* stream) {
   Ensemble* e = (Ensemble*)h;
   if (!e || steps_per_graph < 1) return bad("gpo_ensemble_capture: bad arguments");
+  if (e->selected) return bad("gpo_ensemble_capture: a selection of members is active (select all first)");
   hipStream_t s = (hipStream_t)stream;
   if (e->exec) {
     (void)hipGraphExecDestroy(e->exec);
@@ -626,6 +683,52 @@ int gpo_ensemble_update(gpo_ensemble h, const float* dts, const uint32_t* draws,
                "gpo_ensemble_update copy");
   // The host sources may change as soon as this returns.
   if (!rc) rc = check(hipStreamSynchronize(s), "gpo_ensemble_update sync");
+  return rc;
+}
+
+int gpo_ensemble_select(gpo_ensemble h, const uint8_t* active, const float* noise_scale, void* stream) {
+  Ensemble* e = (Ensemble*)h;
+  if (!e) return bad("gpo_ensemble_select: NULL ensemble");
+  hipStream_t s = (hipStream_t)stream;
+  int rc = 0;
+  if (noise_scale && e->has_noise) {
+    for (int m = 0; m < e->n; ++m) e->h_noise[m].scale = noise_scale[m];
+    rc = check(hipMemcpyAsync(e->d_noise, e->h_noise.data(), sizeof(NoiseMember) * e->h_noise.size(),
+                              hipMemcpyHostToDevice, s),
+               "gpo_ensemble_select copy");
+  }
+  if (!rc && !active) use_full_tables(e);
+  if (!rc && active) {
+    std::vector<int2> w, t;
+    std::vector<BcMember> b;
+    std::vector<DriftMember> d;
+    for (const int2& x : e->h_work)
+      if (active[x.x]) w.push_back(x);
+    for (const int2& x : e->h_tiles)
+      if (active[x.x]) t.push_back(x);
+    for (size_t k = 0; k < e->h_bc.size(); ++k)
+      if (active[e->bc_member[k]]) b.push_back(e->h_bc[k]);
+    for (size_t k = 0; k < e->h_drift.size(); ++k)
+      if (active[e->drift_member[k]]) d.push_back(e->h_drift[k]);
+    auto put = [&](void* dst, const void* src, size_t bytes) {
+      if (!rc && bytes) rc = check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s), "gpo_ensemble_select copy");
+    };
+    put(e->d_work_sel, w.data(), sizeof(int2) * w.size());
+    put(e->d_tiles_sel, t.data(), sizeof(int2) * t.size());
+    put(e->d_bc_sel, b.data(), sizeof(BcMember) * b.size());
+    put(e->d_drift_sel, d.data(), sizeof(DriftMember) * d.size());
+    put(e->d_active, active, (size_t)e->n);
+    // the host vectors are released when this returns: finish the copies first
+    if (!rc) rc = check(hipStreamSynchronize(s), "gpo_ensemble_select sync");
+    if (!rc) {
+      e->selected = 1;
+      e->cur_work = e->d_work_sel, e->cur_n_work = (int)w.size();
+      e->cur_tiles = e->d_tiles_sel, e->cur_n_tiles = (int)t.size();
+      e->cur_bc = e->d_bc_sel, e->cur_n_bc = (int)b.size();
+      e->cur_drift = e->d_drift_sel, e->cur_n_drift = (int)d.size();
+    }
+  }
+  if (!rc) rc = check(hipStreamSynchronize(s), "gpo_ensemble_select sync");
   return rc;
 }
 

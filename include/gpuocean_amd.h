@@ -343,11 +343,25 @@ int gpo_ensemble_info(gpo_ensemble e, int32_t* n_work, int32_t* n_blocks);
  * are not written (apply gpo_boundary_conditions afterwards). */
 int gpo_ensemble_step(gpo_ensemble e, int32_t n_steps, void* stream);
 
-/* One part of a step, for timing and profiling: part 0 and 1 = RK substep 0
- * and 1 of every member, part 2 = the model error of every member (random
- * field, SOAR, balance; nothing without model error).  Parts 0, 1, 2 in this
- * order make one gpo_ensemble_step. */
+/* One part of a step: part 0 and 1 = RK substep 0 and 1 (with the
+ * flow-relaxation ghost cells of the stage), part 2 = the model error (random
+ * field, SOAR, balance; nothing without model error), then the ghost cells of
+ * the new state and the drifters.  Parts 0, 1, 2 in this order make one
+ * gpo_ensemble_step.  Part 3 = part 2 without the model error (a step with
+ * apply_stochastic_term=False), part 4 = the model error alone
+ * (CDKLM16.perturbState after such steps, as dataAssimilationStep does,
+ * SWEsimulators/CDKLM16.py:776-795).  Parts act on the selected members. */
 int gpo_ensemble_part(gpo_ensemble e, int32_t part, void* stream);
+
+/* Selects the members the following parts act on (active: n host bytes,
+ * nonzero = selected; NULL = all members) and/or sets each member's model-error
+ * scale (noise_scale: n host floats, the perturbation_scale of
+ * OceanStateNoise.perturbSim; NULL = unchanged).  Unselected members are not
+ * stepped, perturbed or drifted and keep their draw index.  Graph capture and
+ * replay need all members selected.  Synchronises `stream`.  Lets members with
+ * different numbers of time steps to an observation time share launches
+ * (OceanModelEnsemble.stepToObservation, ensembles/OceanModelEnsemble.py:121-135). */
+int gpo_ensemble_select(gpo_ensemble e, const uint8_t* active, const float* noise_scale, void* stream);
 
 /* Captures steps_per_graph steps on `stream` into a hipGraph (replacing any
  * earlier one); gpo_ensemble_replay launches it n_replays times. */
