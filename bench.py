@@ -10,7 +10,7 @@ captured hipGraph; the states are resident in HBM.  value = member-cell-steps/s
 through the per-member path give the sequential figure the speedup is quoted
 against, and the 2048x1024 single member (c1) is measured alongside.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c1|c0]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c1|c0|c3|c4|da|copy]
 
 Prints one JSON line.
 """
@@ -334,6 +334,56 @@ def run_observe(sims, n_points=1000, reps=5):
             "speedup": ms_ref / ms}
 
 
+def run_da(n_members=64, cycles=2, obs_interval=300.0):
+    """OceanModelEnsemble.stepToObservation over uneven stochastic members
+    (model time step 60 s, observations every obs_interval s): the batched
+    cadence against the members' own dataAssimilationStep, one after another."""
+    import numpy as np
+    import torch
+    import cases
+    from gpuocean_amd import Common, OceanModelEnsemble
+
+    ctx = Common.DeviceContext(0)
+    sizes = cases.c2_member_sizes(n_members, 2024)
+
+    def build(batched):
+        data = []
+        for i, (nx, ny) in enumerate(sizes):
+            kw = cases.config_c2_member(nx, ny, 2025 + i)
+            data.append({k: kw[k] for k in ("eta0", "hu0", "hv0", "H", "nx", "ny", "dx", "dy",
+                                             "boundary_conditions")})
+        sim_args = {"dt": 0.0, "g": 9.81, "f": 1e-4, "r": 0.0, "courant_number": 0.7, "model_time_step": 60.0}
+        return OceanModelEnsemble(ctx, sim_args, data, {"small_scale_perturbation_amplitude": 5e-3,
+                                                        "xorwow_seed": 11}, numParticles=n_members,
+                                  batched=batched)
+
+    cells = sum(nx * ny for nx, ny in sizes)
+    out = {}
+    for batched in (True, False):
+        ens = build(batched)
+        ens.stepToObservation(obs_interval)   # warm-up cycle
+        torch.cuda.synchronize()
+        steps0 = sum(s.num_iterations for s in ens.particles)
+        t0 = time.perf_counter()
+        for c in range(cycles):
+            ens.stepToObservation(obs_interval * (c + 2))
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        member_steps = sum(s.num_iterations for s in ens.particles) - steps0
+        mean_cells = cells / n_members
+        out["batched" if batched else "sequential"] = {
+            "seconds": wall, "member_steps": member_steps,
+            "value": member_steps * mean_cells / wall}
+        log(f"[da] {'batched' if batched else 'per member'}: {cycles} observation cycles of {obs_interval:.0f} s, "
+            f"{member_steps} member steps in {wall:.2f} s -> {member_steps * mean_cells / wall:.4g} member-cell-steps/s")
+        ens.cleanUp()
+    out["speedup"] = out["batched"]["value"] / out["sequential"]["value"]
+    out["config"] = (f"{n_members} uneven members (c2 sizes), SOAR model error, model time step 60 s, "
+                     f"observation every {obs_interval:.0f} s, dt updated every 5 model steps")
+    log(f"[da] batched speedup {out['speedup']:.2f}x")
+    return out
+
+
 def run_copy(n_bytes=256 << 20, reps=5):
     """A plain device-to-device copy of n_bytes, reps times: the reference
     kernel whose FETCH_SIZE / WRITE_SIZE counters calibrate the HBM traffic
@@ -521,10 +571,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c1", "c0", "c3", "c4", "copy"],
+    ap.add_argument("--workload", default="c2", choices=["c2", "c1", "c0", "c3", "c4", "da", "copy"],
                     help="c3 = the adjoint/forward timing alone; c4 = the batched coastal ensemble (land, flow "
-                         "relaxation, drifters); copy = HBM copy of 256 MiB, the calibration kernel for the "
-                         "traffic counters")
+                         "relaxation, drifters); da = OceanModelEnsemble.stepToObservation batched against "
+                         "per member; copy = HBM copy of 256 MiB, the calibration kernel for the traffic counters")
     ap.add_argument("--members", type=int, default=256)
     ap.add_argument("--graph-steps", type=int, default=10)
     ap.add_argument("--seq-steps", type=int, default=10)
@@ -559,6 +609,10 @@ def main():
         if not args.no_adjoint:
             out["adjoint_batched_c4"] = run_adjoint_batch()
         print(json.dumps(out), flush=True)
+        return
+
+    if args.workload == "da":
+        print(json.dumps({"step_to_observation": run_da()}), flush=True)
         return
 
     if args.workload == "c3":

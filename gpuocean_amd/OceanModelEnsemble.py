@@ -264,7 +264,7 @@ class BatchedEnsemble:
             for part in parts:
                 L.check(lib.gpo_ensemble_part(self.handle, int(part), sh), "gpo_ensemble_part")
 
-    def model_step(self, seqs, noise_scales):
+    def model_step(self, seqs, noise_scales, write_ghosts=True):
         """One model time step of each member as CDKLM16.dataAssimilationStep
         takes it (SWEsimulators/CDKLM16.py:776-795): member p takes the time
         steps seqs[p] (none: it is left alone) with no model error between
@@ -272,7 +272,11 @@ class BatchedEnsemble:
         with that perturbation scale.  Members with different step counts
         share launches; each launch acts on the members still stepping.  Host
         bookkeeping (time, iteration count, drifter time, draw index) follows
-        the per-member path."""
+        the per-member path.  write_ghosts=False leaves the ghost cells of
+        wall/periodic members without drifters unwritten: nothing reads them
+        (the step kernels resolve those boundaries while loading, the model
+        error reads only interior cells), and a later model step writes them
+        as the per-member path would."""
         sims = self.sims
         if not any(len(q) for q in seqs):
             return
@@ -302,11 +306,12 @@ class BatchedEnsemble:
             j = j2
         # ghost cells of the new state where the batched parts do not write them
         # (the per-member path writes them after each step, before the model error)
-        self._sync_out()
-        for s, q in zip(sims, seqs):
-            if q and not self._bc_in_batch(s):
-                s._apply_bc(s.h0)
-        self._sync_in()
+        if write_ghosts:
+            self._sync_out()
+            for s, q in zip(sims, seqs):
+                if q and not self._bc_in_batch(s):
+                    s._apply_bc(s.h0)
+            self._sync_in()
         noisy = [bool(q) and sc is not None for q, sc in zip(seqs, noise_scales)]
         if self.has_noise and any(noisy):
             self._select(noisy, [1.0 if sc is None else sc for sc in noise_scales])
@@ -547,7 +552,7 @@ class OceanModelEnsemble:
             noise = i == 0 or i < full or model_error_final_step
             seqs = [self.particles[p].local_dts(target) if p in on else [] for p in range(self.numParticles)]
             scales = [scale if (p in on and noise) else None for p in range(self.numParticles)]
-            ens.model_step(seqs, scales)
+            ens.model_step(seqs, scales, write_ghosts=(i == full))
             for p in active:
                 self.particles[p].total_time_steps += 1
             if self.particles[active[0]].total_time_steps % 5 == 0:
