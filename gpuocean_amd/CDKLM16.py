@@ -105,8 +105,8 @@ class CDKLM16:
             assert r == 0.0, "3rd order Runge Kutta supported only without friction"
         if latitude is not None:
             raise NotImplementedError("latitude-based coriolis textures are not supported; pass f or f+beta")
-        if write_netcdf:
-            raise NotImplementedError("netCDF output is not part of gpuocean_amd")
+        if comm is not None:
+            raise NotImplementedError("MPI communicators (MPIOceanModelEnsemble) are not part of gpuocean_amd")
         if reportGeostrophicEquilibrium:
             raise NotImplementedError("reportGeostrophicEquilibrium is not supported")
         if use_direct_lookup:
@@ -259,6 +259,18 @@ class CDKLM16:
         self.total_time_steps = 0
         self.local_particle_id = local_particle_id
         self.constant_equilibrium_depth = np.max(H)
+        self.comm = None
+        self.ignore_ghostcells = ignore_ghostcells
+        self.offset_x, self.offset_y = offset_x, offset_y
+
+        # netCDF output, written before dt is computed as the reference does (CDKLM16.py:367-373)
+        self.write_netcdf = write_netcdf
+        self.sim_writer = None
+        if self.write_netcdf:
+            from .SimWriter import SimNetCDFWriter
+            self.sim_writer = SimNetCDFWriter(self, super_dir_name=super_dir_name, filename=netcdf_filename,
+                                              ignore_ghostcells=self.ignore_ghostcells, offset_x=self.offset_x,
+                                              offset_y=self.offset_y)
 
         if self.dt <= 0:
             self.updateDt()
@@ -442,6 +454,8 @@ class CDKLM16:
                 self.t += np.float64(local_dt)
                 self.num_iterations += 1
             self._apply_bc(self.h0)
+            if self.write_netcdf and write_now:
+                self.writeState()
             return self.t
 
         t_now = 0.0
@@ -453,6 +467,8 @@ class CDKLM16:
             t_now += np.float64(local_dt)
         if not self._bc_each_step():
             self._apply_bc(self.h0)   # idempotent wall/periodic ghosts, written once at the end
+        if self.write_netcdf and write_now:
+            self.writeState()
         return self.t
 
     def step_dts(self, dts, apply_stochastic_term=True):
@@ -538,6 +554,8 @@ class CDKLM16:
             self.total_time_steps += 1
             if self.total_time_steps % 5 == 0:
                 self.updateDt(courant_number=courant_number)
+        if self.write_netcdf and write_now:
+            self.sim_writer.writeTimestep(self)
         assert round(observation_time) == round(self.t), \
             "The simulation time is not the same as observation time after dataAssimilationStep! " \
             f"(self.t, observation_time, diff): {(self.t, observation_time, self.t - observation_time)}"
@@ -568,6 +586,64 @@ class CDKLM16:
         self.model_error = OceanStateNoise.fromsim(self, soar_q0=small_scale_perturbation_amplitude,
                                                    interpolation_factor=small_scale_perturbation_interpolation_factor,
                                                    use_lcg=use_lcg, xorwow_seed=xorwow_seed)
+        if self.write_netcdf:
+            self.sim_writer.writeModelError(self)
+
+    def setModelErrorFromFile(self, filename, use_lcg=False, xorwow_seed=None):
+        """Model error as recorded in a netCDF file this simulator wrote
+        (CDKLM16.py:600-633); SOAR only."""
+        from .SimReader import SimNetCDFReader
+        reader = SimNetCDFReader(filename, ignore_ghostcells=False)
+        sim_name = str(reader.get('simulator_short'))
+        assert sim_name == self.__class__.__name__, \
+            "Trying to initialize a " + self.__class__.__name__ + " simulator with netCDF file based on " + \
+            sim_name + " results."
+        args = None
+        if reader.has('small_scale_perturbation'):
+            if reader.get('small_scale_perturbation') == 'True':
+                args = {}
+        elif reader.has('has_model_error') and reader.get('has_model_error') == "True":
+            if reader.get('model_error_name') != "OceanStateNoise":
+                raise NotImplementedError("only OceanStateNoise model error is part of gpuocean_amd")
+            args = {}
+        if args is not None:
+            args['small_scale_perturbation_amplitude'] = reader.get('small_scale_perturbation_amplitude')
+            args['small_scale_perturbation_interpolation_factor'] = \
+                int(reader.get('small_scale_perturbation_interpolation_factor'))
+            self.setSOARModelError(**args, use_lcg=use_lcg, xorwow_seed=xorwow_seed)
+        reader.close()
+
+    @classmethod
+    def fromfilename(cls, gpu_ctx, filename, cont_write_netcdf=True, new_netcdf_filename=None, time0=None):
+        """A simulator hot-started from the last (or the time0) record of a
+        netCDF file, with the parameters recorded there (CDKLM16.py:403-473)."""
+        from .SimReader import SimNetCDFReader
+        reader = SimNetCDFReader(filename, ignore_ghostcells=False)
+        sim_name = str(reader.get('simulator_short'))
+        assert sim_name == cls.__name__, \
+            "Trying to initialize a " + cls.__name__ + " simulator with netCDF file based on " + sim_name + " results."
+        H = reader.getH()
+        if time0 is None:
+            eta0, hu0, hv0, time0 = reader.getLastTimeStep()
+        else:
+            eta0, hu0, hv0, time0 = reader.getStateAtTime(time0)
+        if len(H.shape) == 3:
+            H = H[0, :, :]
+        params = dict(gpu_ctx=gpu_ctx, eta0=eta0, hu0=hu0, hv0=hv0, H=H, nx=reader.get("nx"), ny=reader.get("ny"),
+                      dx=reader.get("dx"), dy=reader.get("dy"), dt=reader.get("dt"), g=reader.get("g"),
+                      f=reader.get("coriolis_force"), r=reader.get("bottom_friction_r"), t=float(time0),
+                      theta=reader.get("minmod_theta"), rk_order=reader.get("time_integrator"),
+                      coriolis_beta=reader.get("coriolis_beta"), write_netcdf=cont_write_netcdf,
+                      netcdf_filename=new_netcdf_filename, wind=WindStress(),
+                      boundary_conditions=reader.getBC())
+        if reader.has('model_time_step'):
+            params['model_time_step'] = reader.get('model_time_step')
+        reader.close()
+        return cls(**params)
+
+    def writeState(self):
+        if self.write_netcdf:
+            self.sim_writer.writeTimestep(self)
 
     def perturbState(self, perturbation_scale=1.0, update_random_field=True, q0_scale=1):
         if not q0_scale == 1:
@@ -669,6 +745,7 @@ class CDKLM16:
         return mask[2:-2, 2:-2] if interior_domain_only else mask
 
     def cleanUp(self, do_gc=True):
+        self.closeNetCDF()
         self.h0 = self.h1 = self.h_stage = None
         self.Hi = self.Bm = self.f_cells = None
         if self.model_error is not None and hasattr(self.model_error, "cleanUp"):
@@ -677,4 +754,7 @@ class CDKLM16:
         self.gpu_ctx = None
 
     def closeNetCDF(self):
-        pass
+        """Closes the netCDF file, if there is one (Simulator.py:318-324)."""
+        if self.write_netcdf:
+            self.sim_writer.close()
+            self.write_netcdf = False

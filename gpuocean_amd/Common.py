@@ -70,9 +70,11 @@ class BoundaryConditions:
                 4: "Open_Linear_Interpolation"}.get(int(cond), "Invalid_:|")
 
     def __str__(self):
+        # sponge widths as plain integers, as NumPy 1.x printed them, so the
+        # text parses back (fromstring) and matches files the reference wrote
         return ("north: " + self._toString(self.north) + ", east: " + self._toString(self.east)
                 + ", south: " + self._toString(self.south) + ", west: " + self._toString(self.west)
-                + ", spongeCells: " + str(self.spongeCells))
+                + ", spongeCells: " + str({k: int(v) for k, v in self.spongeCells.items()}))
 
     @classmethod
     def fromstring(cls, bc_string):

@@ -80,6 +80,8 @@ class BatchedEnsemble:
                 return False, "flow-relaxation data that changes in time"
             if s._has_wind or s._has_pressure:
                 return False, "wind or pressure forcing"
+            if s.write_netcdf:
+                return False, "members writing netCDF output after every step"
             if s.hasDrifters:
                 d = s.drifters
                 if not hasattr(d, "positions") or float(getattr(d, "wind_drift_factor", 0.0)) != 0.0:
