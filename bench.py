@@ -297,6 +297,60 @@ def run_ensemble(args):
     }
 
 
+def run_coarse_noise(n=2048, factor=3, reps=100):
+    """Model error generated on a grid `factor` times coarser (2048 x 2048,
+    walls): the bicubic interpolation + balance kernel timed alone with HIP
+    events, and one whole perturbation (random field, SOAR, interpolation)."""
+    import ctypes
+    import numpy as np
+    import torch
+    import cases
+    from gpuocean_amd import CDKLM16, Common
+    from gpuocean_amd import _lib as L
+
+    ctx = Common.DeviceContext(0)
+    kw = cases.config_c0(n=n)
+    sim = CDKLM16(ctx, **kw)
+    sim.setSOARModelError(5e-3, small_scale_perturbation_interpolation_factor=factor, xorwow_seed=5)
+    me = sim.model_error
+    st = sim.gpu_stream
+    sim.perturbState()
+    st.synchronize()
+    ev0, ev1 = event_pair(st)
+    ev0.record(st)
+    for _ in range(reps):
+        sim.perturbState()
+    ev1.record(st)
+    st.synchronize()
+    whole_us = 1e3 * ev0.elapsed_time(ev1) / reps
+    args = [int(sim.nx), int(sim.ny), float(sim.dx), float(sim.dy), int(me.coarse_nx), int(me.coarse_ny),
+            float(me.coarse_dx), float(me.coarse_dy), 0, 0, float(sim.g), float(sim.f), 0.0,
+            float(sim.params.north_x), float(sim.params.north_y), None, 0,
+            ctypes.c_void_p(me.coarse_buffer.data_ptr()), 4 * (int(me.coarse_nx) + 4),
+            ctypes.c_void_p(sim.h0[0].data_ptr()), ctypes.c_void_p(sim.h0[1].data_ptr()),
+            ctypes.c_void_p(sim.h0[2].data_ptr()), sim.h0[0].pitch, ctypes.c_void_p(sim.Hi.data_ptr()), sim.Hi.pitch,
+            float(sim.mask_value), ctypes.c_void_p(st.cuda_stream)]
+    lib = L.lib()
+    ev0, ev1 = event_pair(st)
+    ev0.record(st)
+    for _ in range(reps):
+        L.check(lib.gpo_bicubic_interpolation(*args), "gpo_bicubic_interpolation")
+    ev1.record(st)
+    st.synchronize()
+    kernel_us = 1e3 * ev0.elapsed_time(ev1) / reps
+    cells = n * n
+    achieved = BYTES_NOISE * cells / (kernel_us * 1e-6) / 1e9
+    assert np.all(np.isfinite(sim.download()[0])), "non-finite state"
+    log(f"[coarse model error] {n}x{n}, factor {factor}: interpolation + balance {kernel_us:.1f} us "
+        f"({achieved:.0f} GB/s algorithmic, {BYTES_NOISE} B/cell), whole perturbation {whole_us:.1f} us")
+    sim.cleanUp()
+    return {"workload": f"{n}x{n} walls, SOAR on a grid {factor} times coarser, bicubic interpolation",
+            "interpolation_us": kernel_us, "perturbation_us": whole_us,
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS, "kernel": "bicubic_balance",
+                         "bytes_per_cell_per_launch": BYTES_NOISE, "cells_per_launch": cells}}
+
+
 def run_observe(sims, n_points=1000, reps=5):
     """hu, hv of every member at the same n_points positions: the library's
     one-launch observation operator against the reference's way (each
@@ -627,6 +681,7 @@ def main():
         if not args.no_c1:
             c1 = run_single(cases.config_c1(), "c1", 2000, 50, args.fast, 0)
             result["single_member_c1"] = c1
+            result["coarse_model_error"] = run_coarse_noise()
         if not args.no_adjoint:
             result["adjoint_c3"] = run_adjoint()
         result["cpu_baseline"] = None

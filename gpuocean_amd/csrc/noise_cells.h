@@ -113,24 +113,37 @@ __device__ __forceinline__ float dot4(float a0, float a1, float a2, float a3, fl
   return a0 * b0 + a1 * b1 + a2 * b2 + a3 * b3;
 }
 
-// Coarse perturbation at fine cell (tj, ti) of the ghosted fine grid by bicubic
-// interpolation of the (cny+4) x (cnx+4) coarse buffer (2 ghost cells on both
-// grids): bicubicInterpolation (gpu_kernels/ocean_noise.cu:523-611) with the
-// coefficient and evaluation steps of interpolation.cu:39-133.  The corner
-// values and central-difference derivatives of the coarse cell below-left of
-// the fine centre form F; the coefficients are B F B^T, B the Hermite basis
-// matrix, evaluated at the fine centre's offset in that coarse cell.  Coarse
-// indices are clamped to the buffer as the reference's shared-memory load does.
-__device__ __forceinline__ float bicubic_fine(int cnx, int cny, float dx, float dy, float cdx, float cdy, int oi,
-                                              int oj, const float* coarse, int c_pitch, int tj, int ti) {
+// Where fine cell (tj, ti) of the ghosted fine grid lies on the coarse grid
+// (2 ghost cells on both): the coarse cell (cj, ci) below-left of its centre
+// and the centre's offset (ry, rx) in units of the coarse cell, as
+// bicubicInterpolation finds them (gpu_kernels/ocean_noise.cu:523-545, 603-604).
+struct CoarseSpot {
+  int ci, cj;
+  float rx, ry;
+};
+__device__ __forceinline__ CoarseSpot coarse_spot(int cnx, int cny, float dx, float dy, float cdx, float cdy, int oi,
+                                                  int oj, int tj, int ti) {
   const float x = ((float)(ti - 2) + 0.5f + (float)oi) * dx;
   const float y = ((float)(tj - 2) + 0.5f + (float)oj) * dy;
-  int ci = (int)floorf(x / cdx + 2.0f - 0.5f);
-  int cj = (int)floorf(y / cdy + 2.0f - 0.5f);
-  if (ci > cnx + 1) ci -= 1;  // first ghost cell right/top of the domain: rel = 1
-  if (cj > cny + 1) cj -= 1;
-  const float cx = ((float)(ci - 2) + 0.5f) * cdx;
-  const float cy = ((float)(cj - 2) + 0.5f) * cdy;
+  CoarseSpot p;
+  p.ci = (int)floorf(x / cdx + 2.0f - 0.5f);
+  p.cj = (int)floorf(y / cdy + 2.0f - 0.5f);
+  if (p.ci > cnx + 1) p.ci -= 1;  // first ghost cell right/top of the domain: rel = 1
+  if (p.cj > cny + 1) p.cj -= 1;
+  const float cx = ((float)(p.ci - 2) + 0.5f) * cdx;
+  const float cy = ((float)(p.cj - 2) + 0.5f) * cdy;
+  p.rx = (x - cx) / cdx;
+  p.ry = (y - cy) / cdy;
+  return p;
+}
+
+// Bicubic coefficients B F B^T of coarse cell (cj, ci) (interpolation.cu:39-117):
+// F holds the corner values and central-difference derivatives of the
+// (cny+4) x (cnx+4) coarse buffer (ocean_noise.cu:554-601), B is the Hermite
+// basis matrix.  Coarse indices are clamped to the buffer as the reference's
+// shared-memory load does.
+__device__ __forceinline__ void bicubic_coeffs(int cnx, int cny, const float* coarse, int c_pitch, int cj, int ci,
+                                               float a[16]) {
   auto C = [&](int j, int i) {
     j = j < 0 ? 0 : (j > cny + 3 ? cny + 3 : j);
     i = i < 0 ? 0 : (i > cnx + 3 ? cnx + 3 : i);
@@ -161,19 +174,21 @@ __device__ __forceinline__ float bicubic_fine(int cnx, int cny, float dx, float 
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) T[i][j] = dot4(F[j][0], F[j][1], F[j][2], F[j][3], B[i][0], B[i][1], B[i][2], B[i][3]);
-  const float rx = (x - cx) / cdx, ry = (y - cy) / cdy;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)  // row i of B F B^T: a[4i + j] = B[i] . T[j]
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      a[4 * i + j] = dot4(B[i][0], B[i][1], B[i][2], B[i][3], T[j][0], T[j][1], T[j][2], T[j][3]);
+}
+
+// x^T A y with x = (1, rx, rx^2, rx^3), y likewise (interpolation.cu:119-133).
+__device__ __forceinline__ float bicubic_eval(const float* a, float rx, float ry) {
   const float rx2 = rx * rx, ry2 = ry * ry;
-  const float xv[4] = {1.0f, rx, rx2, rx2 * rx};
   const float yv[4] = {1.0f, ry, ry2, ry2 * ry};
-  float t[4];  // (B F B^T) y
+  float t[4];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    float a[4];  // row i of B F B^T: a[j] = B[i] . T[j]
-#pragma unroll
-    for (int j = 0; j < 4; ++j) a[j] = dot4(B[i][0], B[i][1], B[i][2], B[i][3], T[j][0], T[j][1], T[j][2], T[j][3]);
-    t[i] = dot4(a[0], a[1], a[2], a[3], yv[0], yv[1], yv[2], yv[3]);
-  }
-  return dot4(xv[0], xv[1], xv[2], xv[3], t[0], t[1], t[2], t[3]);
+  for (int i = 0; i < 4; ++i) t[i] = dot4(a[4 * i], a[4 * i + 1], a[4 * i + 2], a[4 * i + 3], yv[0], yv[1], yv[2], yv[3]);
+  return dot4(1.0f, rx, rx2, rx2 * rx, t[0], t[1], t[2], t[3]);
 }
 
 // The geostrophic step of bicubicInterpolation (gpu_kernels/ocean_noise.cu:616-657)

@@ -77,34 +77,64 @@ __global__ void geo_balance(int nx, int ny, float dx, float dy, float g, float f
                h_pitch, land_value, tj, ti);
 }
 
-// bicubicInterpolation (gpu_kernels/ocean_noise.cu:401-659): a 64 x 4 tile
-// of interior cells interpolates the coarse perturbation for itself and a
-// one-cell ring into LDS, then adds it and its balanced momentum.
-constexpr int kBiW = 64, kBiH = 4;
+// bicubicInterpolation (gpu_kernels/ocean_noise.cu:401-659).  A 64 x 16 tile
+// of interior cells and its one-cell ring (the balance needs the neighbours'
+// perturbation) covers a small block of coarse cells: their bicubic
+// coefficients are formed once, in LDS, and every fine point of the tile
+// evaluates the polynomial of its coarse cell from there, so the coefficient
+// product is paid per coarse cell, not per fine cell.  Then the balance.
+constexpr int kBiW = 64, kBiH = 16;
+constexpr int kBiMaxCoarse = 256;  // coarse cells per tile at factor >= 3: (66/3+2) x (18/3+2) = 192
 struct BicubicGeo {
   int nx, ny, cnx, cny, oi, oj;
   float dx, dy, cdx, cdy;
 };
-__global__ __launch_bounds__(kBiW* kBiH) void bicubic_balance(BicubicGeo q, float g, float f, float beta,
-                                                               float north_x, float north_y, const float* f_cells,
-                                                               int f_pitch, const float* coarse, int c_pitch,
-                                                               float* eta, float* hu, float* hv, int pitch,
-                                                               const float* Hi, int h_pitch, float land_value) {
+__global__ __launch_bounds__(256) void bicubic_balance(BicubicGeo q, float g, float f, float beta, float north_x,
+                                                       float north_y, const float* f_cells, int f_pitch,
+                                                       const float* coarse, int c_pitch, float* eta, float* hu,
+                                                       float* hv, int pitch, const float* Hi, int h_pitch,
+                                                       float land_value) {
+  __shared__ float coef[kBiMaxCoarse][17];  // 16 coefficients per coarse cell (+1 against bank conflicts)
   __shared__ float d_eta[kBiH + 2][kBiW + 2];
   const int bx = blockIdx.x * kBiW + 2, by = blockIdx.y * kBiH + 2;  // first interior cell of the tile
-  const int tid = threadIdx.y * kBiW + threadIdx.x;
-  for (int k = tid; k < (kBiH + 2) * (kBiW + 2); k += kBiW * kBiH) {
-    const int j = k / (kBiW + 2), i = k % (kBiW + 2);
-    d_eta[j][i] = bicubic_fine(q.cnx, q.cny, q.dx, q.dy, q.cdx, q.cdy, q.oi, q.oj, coarse, c_pitch, by + j - 1,
-                               bx + i - 1);
+  const int tid = threadIdx.x;
+  // the coarse cells under the tile and its ring: the spot of a fine point
+  // never decreases with its index, so the corners bound them
+  const CoarseSpot lo = coarse_spot(q.cnx, q.cny, q.dx, q.dy, q.cdx, q.cdy, q.oi, q.oj, by - 1, bx - 1);
+  const CoarseSpot hi = coarse_spot(q.cnx, q.cny, q.dx, q.dy, q.cdx, q.cdy, q.oi, q.oj, by + kBiH, bx + kBiW);
+  const int ncw = hi.ci - lo.ci + 1, nch = hi.cj - lo.cj + 1;
+  const bool shared = ncw * nch <= kBiMaxCoarse;  // false only for factors below 3
+  if (shared) {
+    for (int k = tid; k < ncw * nch; k += 256) {
+      float a[16];
+      bicubic_coeffs(q.cnx, q.cny, coarse, c_pitch, lo.cj + k / ncw, lo.ci + k % ncw, a);
+#pragma unroll
+      for (int i = 0; i < 16; ++i) coef[k][i] = a[i];
+    }
   }
   __syncthreads();
-  const int ti = bx + threadIdx.x, tj = by + threadIdx.y;
-  if (ti >= q.nx + 2 || tj >= q.ny + 2) return;
-  const int x = threadIdx.x + 1, y = threadIdx.y + 1;
-  bicubic_balance_apply(q.dx, q.dy, g, f, beta, north_x, north_y, f_cells, f_pitch, d_eta[y][x], d_eta[y][x - 1],
-                        d_eta[y][x + 1], d_eta[y - 1][x], d_eta[y + 1][x], eta, hu, hv, pitch, Hi, h_pitch,
-                        land_value, tj, ti);
+  for (int k = tid; k < (kBiH + 2) * (kBiW + 2); k += 256) {
+    const int j = k / (kBiW + 2), i = k % (kBiW + 2);
+    const CoarseSpot p = coarse_spot(q.cnx, q.cny, q.dx, q.dy, q.cdx, q.cdy, q.oi, q.oj, by + j - 1, bx + i - 1);
+    float a[16];
+    if (shared) {
+      const int c = (p.cj - lo.cj) * ncw + (p.ci - lo.ci);
+#pragma unroll
+      for (int t = 0; t < 16; ++t) a[t] = coef[c][t];
+    } else {
+      bicubic_coeffs(q.cnx, q.cny, coarse, c_pitch, p.cj, p.ci, a);
+    }
+    d_eta[j][i] = bicubic_eval(a, p.rx, p.ry);
+  }
+  __syncthreads();
+  const int x = (tid & 63) + 1;
+  for (int y = (tid >> 6) + 1; y <= kBiH; y += 4) {
+    const int ti = bx + x - 1, tj = by + y - 1;
+    if (ti >= q.nx + 2 || tj >= q.ny + 2) continue;
+    bicubic_balance_apply(q.dx, q.dy, g, f, beta, north_x, north_y, f_cells, f_pitch, d_eta[y][x], d_eta[y][x - 1],
+                          d_eta[y][x + 1], d_eta[y - 1][x], d_eta[y + 1][x], eta, hu, hv, pitch, Hi, h_pitch,
+                          land_value, tj, ti);
+  }
 }
 
 __global__ void drift(int nx, int ny, float dx, float dy, float dt, int x0ref, int y0ref, const float* eta,
@@ -189,7 +219,7 @@ int gpo_bicubic_interpolation(int32_t nx, int32_t ny, float dx, float dy, int32_
   if (offset_i < -coarse_nx || offset_i > coarse_nx || offset_j < -coarse_ny || offset_j > coarse_ny)
     return bad("gpo_bicubic_interpolation: offset larger than the coarse grid");
   const BicubicGeo q{nx, ny, coarse_nx, coarse_ny, offset_i, offset_j, dx, dy, coarse_dx, coarse_dy};
-  hipLaunchKernelGGL(bicubic_balance, dim3(ceil_div(nx, kBiW), ceil_div(ny, kBiH)), dim3(kBiW, kBiH), 0,
+  hipLaunchKernelGGL(bicubic_balance, dim3(ceil_div(nx, kBiW), ceil_div(ny, kBiH)), dim3(256), 0,
                      (hipStream_t)stream, q, g, f, beta, north_x, north_y, f_cells, f_pitch, coarse, coarse_pitch,
                      eta, hu, hv, pitch, Hi, Hi_pitch, land_value);
   return check(hipGetLastError(), "gpo_bicubic_interpolation launch");

@@ -61,6 +61,15 @@ class _Table:
 
 
 _default_table = _Table()
+_streams = {}
+
+
+def _stream(dev):
+    """One observation stream per device, made once."""
+    key = (dev.type, dev.index)
+    if key not in _streams:
+        _streams[key] = torch.cuda.Stream(device=dev)
+    return _streams[key]
 
 
 def observe_members(sims, mode=FLOW, positions=None, cells=None, observations=None, mean_depth=None,
@@ -120,9 +129,12 @@ def observe_members(sims, mode=FLOW, positions=None, cells=None, observations=No
         keep.append(obs_dev)
     dxs = [float(s.dx if dx is None else dx) for s in sims]
     dys = [float(s.dy if dy is None else dy) for s in sims]
-    stream = torch.cuda.Stream(device=dev)
-    for s in sims:
-        stream.wait_stream(s.gpu_stream)
+    stream = _stream(dev)
+    seen = set()
+    for s in sims:   # members may share a stream: wait on each once
+        if id(s.gpu_stream) not in seen:
+            seen.add(id(s.gpu_stream))
+            stream.wait_stream(s.gpu_stream)
     stream.wait_stream(torch.cuda.current_stream(dev))
     with torch.cuda.stream(stream):
         tab = (table or _default_table).get(sims, point_ptrs, cell_ptrs, dxs, dys, dev)
