@@ -188,7 +188,12 @@ class BatchedEnsemble:
             s.gpu_stream.wait_stream(self.stream)
 
     def _refresh(self):
-        """Push members' dt and draw index to the device tables if they changed."""
+        """Push members' dt and draw index to the device tables if they changed
+        (and the unit model-error scale of a regular step, if a
+        data-assimilation step left other scales)."""
+        if getattr(self, "_scales_dirty", False):
+            self._select(None, [1.0] * self.n)
+            self._scales_dirty = False
         dts = [np.float32(s.dt) for s in self.sims]
         draws = [s.model_error.draw if self.has_noise else 0 for s in self.sims]
         new_dt = dts != self._dts
@@ -290,22 +295,32 @@ class BatchedEnsemble:
         if self._dts is None or self._draws is None:
             self._dts, self._draws = None, None
             self._refresh_all()
+        # groups of consecutive inner steps with the same members and time steps
         k_max = max(len(q) for q in seqs)
-        j = 0
-        while j < k_max:
-            act = [j < len(q) for q in seqs]
-            dts = [np.float32(q[j]) if a else self._dts[p] for p, (q, a) in enumerate(zip(seqs, act))]
-            j2 = j + 1
-            while j2 < k_max and [j2 < len(q) for q in seqs] == act and \
-                    all(np.float32(q[j2]) == dts[p] for p, q in enumerate(seqs) if act[p]):
-                j2 += 1
-            if dts != self._dts:
-                arr = (ctypes.c_float * self.n)(*[float(d) for d in dts])
-                L.check(L.lib().gpo_ensemble_update(self.handle, arr, None, self._sh()), "gpo_ensemble_update")
-                self._dts = dts
-            self._select(None if all(act) else act)
-            self._parts((0, 1, 3), j2 - j)
-            j = j2
+        groups = []   # [active, dts, count]
+        for j in range(k_max):
+            act = tuple(j < len(q) for q in seqs)
+            dts = tuple(float(np.float32(q[j])) if a else 1.0 for q, a in zip(seqs, act))
+            if groups and groups[-1][0] == act and groups[-1][1] == dts:
+                groups[-1][2] += 1
+            else:
+                groups.append([act, dts, 1])
+        noisy = [bool(q) and sc is not None for q, sc in zip(seqs, noise_scales)]
+        with_noise = self.has_noise and any(noisy)
+        sets = [(g[0], g[1]) for g in groups]
+        if with_noise:
+            sets.append((tuple(noisy), tuple([1.0] * self.n)))
+        n = self.n
+        act_arr = (ctypes.c_uint8 * (len(sets) * n))(*[1 if a else 0 for st in sets for a in st[0]])
+        dt_arr = (ctypes.c_float * (len(sets) * n))(*[d for st in sets for d in st[1]])
+        scales = [1.0 if sc is None else sc for sc in noise_scales] if with_noise else [1.0] * n
+        sc_arr = (ctypes.c_float * n)(*[float(np.float32(x)) for x in scales])
+        lib = L.lib()
+        sh = self._sh()
+        L.check(lib.gpo_ensemble_prepare_sets(self.handle, len(sets), act_arr, dt_arr, sc_arr, sh),
+                "gpo_ensemble_prepare_sets")
+        for k, g in enumerate(groups):   # parts 0, 1, 3: substeps, ghost cells and drifters
+            L.check(lib.gpo_ensemble_run_set(self.handle, k, g[2], 0b1011, sh), "gpo_ensemble_run_set")
         # ghost cells of the new state where the batched parts do not write them
         # (the per-member path writes them after each step, before the model error)
         if write_ghosts:
@@ -314,13 +329,9 @@ class BatchedEnsemble:
                 if q and not self._bc_in_batch(s):
                     s._apply_bc(s.h0)
             self._sync_in()
-        noisy = [bool(q) and sc is not None for q, sc in zip(seqs, noise_scales)]
-        if self.has_noise and any(noisy):
-            self._select(noisy, [1.0 if sc is None else sc for sc in noise_scales])
-            self._parts((4,))
-            self._select(None, [1.0] * self.n)
-        else:
-            self._select(None)
+        if with_noise:   # part 4: the model error with each member's scale
+            L.check(lib.gpo_ensemble_run_set(self.handle, len(sets) - 1, 1, 0b10000, sh), "gpo_ensemble_run_set")
+            self._scales_dirty = True
         self._sync_out()
         for p, (s, q) in enumerate(zip(sims, seqs)):
             for dt in q:

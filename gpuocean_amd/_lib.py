@@ -146,6 +146,8 @@ _SIGNATURES = [
     ("gpo_ensemble_step", c_int32, [c_void_p, c_int32, c_void_p]),
     ("gpo_ensemble_part", c_int32, [c_void_p, c_int32, c_void_p]),
     ("gpo_ensemble_select", c_int32, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("gpo_ensemble_prepare_sets", c_int32, [c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("gpo_ensemble_run_set", c_int32, [c_void_p, c_int32, c_int32, c_uint32, c_void_p]),
     ("gpo_ensemble_capture", c_int32, [c_void_p, c_int32, c_void_p]),
     ("gpo_ensemble_replay", c_int32, [c_void_p, c_int32, c_void_p]),
     ("gpo_ensemble_draws", c_int32, [c_void_p, ctypes.POINTER(c_uint32), c_void_p]),

@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstring>
 #include <new>
 #include <vector>
 
@@ -247,10 +248,24 @@ struct Ensemble {
   BcMember* d_bc_sel = nullptr;
   DriftMember* d_drift_sel = nullptr;
   uint8_t* d_active = nullptr;
+  const StepArgs* cur_args = nullptr;
+  const uint8_t* cur_active = nullptr;  // per-member flags of the current selection (NULL: all)
+  // Prepared selections (gpo_ensemble_prepare_sets): each set's tables, an
+  // argument table with its time steps and its member flags, in one arena.
+  struct Set {
+    size_t work, tiles, bc, drift, args, active;  // byte offsets in the arena
+    int n_work, n_tiles, n_bc, n_drift;
+  };
+  std::vector<Set> sets;
+  char* d_arena = nullptr;
+  size_t arena_bytes = 0;
+  std::vector<char> h_arena;
 };
 
 static void use_full_tables(Ensemble* e) {
   e->selected = 0;
+  e->cur_args = e->d_args;
+  e->cur_active = nullptr;
   e->cur_work = e->d_work, e->cur_n_work = e->n_work;
   e->cur_tiles = e->d_tiles, e->cur_n_tiles = e->n_tiles;
   e->cur_bc = e->d_bc, e->cur_n_bc = e->n_bc;
@@ -259,9 +274,9 @@ static void use_full_tables(Ensemble* e) {
 
 static int ensemble_launch_substep(Ensemble* e, int rk, hipStream_t s) {
   const hipError_t err =
-      e->exact ? launch_cdklm_batched_exact(e->d_args, e->cur_work, e->cur_n_work, rk, e->n_blocks, e->has_land,
-                                            e->f_array, e->angle0, s)
-               : launch_cdklm_batched_fast(e->d_args, e->cur_work, e->cur_n_work, rk, e->n_blocks, e->has_land,
+      e->exact ? launch_cdklm_batched_exact(e->cur_args, e->cur_work, e->cur_n_work, rk, e->n_blocks,
+                                            e->has_land, e->f_array, e->angle0, s)
+               : launch_cdklm_batched_fast(e->cur_args, e->cur_work, e->cur_n_work, rk, e->n_blocks, e->has_land,
                                            e->f_array, e->angle0, s);
   return err != hipSuccess ? check(err, "ensemble substep launch") : 0;
 }
@@ -270,8 +285,7 @@ static int ensemble_launch_noise(Ensemble* e, hipStream_t s) {
   if (!e->has_noise) return 0;
   if (e->cur_n_tiles > 0)
     hipLaunchKernelGGL(noise_fused, dim3(e->cur_n_tiles), dim3(256), 0, s, e->d_noise, e->cur_tiles, e->d_draw);
-  hipLaunchKernelGGL(advance_draws, dim3((e->n + 255) / 256), dim3(256), 0, s, e->d_draw, e->n,
-                     (const uint8_t*)(e->selected ? e->d_active : nullptr));
+  hipLaunchKernelGGL(advance_draws, dim3((e->n + 255) / 256), dim3(256), 0, s, e->d_draw, e->n, e->cur_active);
   return check(hipGetLastError(), "ensemble noise launch");
 }
 
@@ -284,7 +298,7 @@ static int ensemble_launch_bc(Ensemble* e, int target, hipStream_t s) {
 
 static int ensemble_launch_drift(Ensemble* e, hipStream_t s) {
   if (e->cur_n_drift == 0) return 0;
-  hipLaunchKernelGGL(drift_batched, dim3(e->drift_blocks, e->cur_n_drift), dim3(64), 0, s, e->cur_drift, e->d_args);
+  hipLaunchKernelGGL(drift_batched, dim3(e->drift_blocks, e->cur_n_drift), dim3(64), 0, s, e->cur_drift, e->cur_args);
   return check(hipGetLastError(), "ensemble drift launch");
 }
 
@@ -329,6 +343,7 @@ static void ensemble_free(Ensemble* e) {
   if (e->d_bc_sel) (void)hipFree(e->d_bc_sel);
   if (e->d_drift_sel) (void)hipFree(e->d_drift_sel);
   if (e->d_active) (void)hipFree(e->d_active);
+  if (e->d_arena) (void)hipFree(e->d_arena);
   delete e;
 }
 
@@ -722,6 +737,7 @@ int gpo_ensemble_select(gpo_ensemble h, const uint8_t* active, const float* nois
     if (!rc) rc = check(hipStreamSynchronize(s), "gpo_ensemble_select sync");
     if (!rc) {
       e->selected = 1;
+      e->cur_active = e->d_active;
       e->cur_work = e->d_work_sel, e->cur_n_work = (int)w.size();
       e->cur_tiles = e->d_tiles_sel, e->cur_n_tiles = (int)t.size();
       e->cur_bc = e->d_bc_sel, e->cur_n_bc = (int)b.size();
@@ -729,6 +745,90 @@ int gpo_ensemble_select(gpo_ensemble h, const uint8_t* active, const float* nois
     }
   }
   if (!rc) rc = check(hipStreamSynchronize(s), "gpo_ensemble_select sync");
+  return rc;
+}
+
+int gpo_ensemble_prepare_sets(gpo_ensemble h, int32_t n_sets, const uint8_t* active, const float* dts,
+                              const float* noise_scale, void* stream) {
+  Ensemble* e = (Ensemble*)h;
+  if (!e || n_sets < 0 || (n_sets > 0 && !active)) return bad("gpo_ensemble_prepare_sets: bad arguments");
+  if (dts)
+    for (size_t k = 0; k < (size_t)n_sets * e->n; ++k)
+      if (active[k] && !(dts[k] > 0.0f)) return bad("gpo_ensemble_prepare_sets: time steps must be positive");
+  hipStream_t s = (hipStream_t)stream;
+  std::vector<char>& A = e->h_arena;
+  A.clear();
+  e->sets.assign((size_t)n_sets, Ensemble::Set{});
+  auto put = [&](const void* src, size_t bytes) {
+    const size_t off = (A.size() + 255) & ~(size_t)255;
+    A.resize(off + bytes);
+    if (bytes) memcpy(A.data() + off, src, bytes);
+    return off;
+  };
+  for (int k = 0; k < n_sets; ++k) {
+    const uint8_t* act = active + (size_t)k * e->n;
+    std::vector<int2> w, t;
+    std::vector<BcMember> b;
+    std::vector<DriftMember> d;
+    for (const int2& x : e->h_work)
+      if (act[x.x]) w.push_back(x);
+    for (const int2& x : e->h_tiles)
+      if (act[x.x]) t.push_back(x);
+    for (size_t i = 0; i < e->h_bc.size(); ++i)
+      if (act[e->bc_member[i]]) b.push_back(e->h_bc[i]);
+    for (size_t i = 0; i < e->h_drift.size(); ++i)
+      if (act[e->drift_member[i]]) d.push_back(e->h_drift[i]);
+    std::vector<StepArgs> args = e->h_args;
+    if (dts)
+      for (int m = 0; m < e->n; ++m)
+        if (act[m]) args[2 * m].dt = args[2 * m + 1].dt = dts[(size_t)k * e->n + m];
+    Ensemble::Set& S = e->sets[k];
+    S.work = put(w.data(), sizeof(int2) * w.size()), S.n_work = (int)w.size();
+    S.tiles = put(t.data(), sizeof(int2) * t.size()), S.n_tiles = (int)t.size();
+    S.bc = put(b.data(), sizeof(BcMember) * b.size()), S.n_bc = (int)b.size();
+    S.drift = put(d.data(), sizeof(DriftMember) * d.size()), S.n_drift = (int)d.size();
+    S.args = put(args.data(), sizeof(StepArgs) * args.size());
+    S.active = put(act, (size_t)e->n);
+  }
+  int rc = 0;
+  // the arena may still be read by launches of earlier sets: finish them first
+  if (A.size() > e->arena_bytes) {
+    rc = check(hipStreamSynchronize(s), "gpo_ensemble_prepare_sets sync");
+    if (!rc && e->d_arena) (void)hipFree(e->d_arena), e->d_arena = nullptr, e->arena_bytes = 0;
+    if (!rc) rc = check(hipMalloc(&e->d_arena, A.size()), "gpo_ensemble_prepare_sets hipMalloc");
+    if (!rc) e->arena_bytes = A.size();
+  }
+  if (!rc && !A.empty())
+    rc = check(hipMemcpyAsync(e->d_arena, A.data(), A.size(), hipMemcpyHostToDevice, s),
+               "gpo_ensemble_prepare_sets copy");
+  if (!rc && noise_scale && e->has_noise) {
+    for (int m = 0; m < e->n; ++m) e->h_noise[m].scale = noise_scale[m];
+    rc = check(hipMemcpyAsync(e->d_noise, e->h_noise.data(), sizeof(NoiseMember) * e->h_noise.size(),
+                              hipMemcpyHostToDevice, s),
+               "gpo_ensemble_prepare_sets copy");
+  }
+  // the host sources are reused by the next call: let the copies finish
+  if (!rc) rc = check(hipStreamSynchronize(s), "gpo_ensemble_prepare_sets sync");
+  return rc;
+}
+
+int gpo_ensemble_run_set(gpo_ensemble h, int32_t set, int32_t n_steps, uint32_t parts, void* stream) {
+  Ensemble* e = (Ensemble*)h;
+  if (!e || set < 0 || set >= (int)e->sets.size() || n_steps < 0 || (parts & ~0x1Fu))
+    return bad("gpo_ensemble_run_set: bad arguments");
+  const Ensemble::Set& S = e->sets[set];
+  e->selected = 1;
+  e->cur_work = (const int2*)(e->d_arena + S.work), e->cur_n_work = S.n_work;
+  e->cur_tiles = (const int2*)(e->d_arena + S.tiles), e->cur_n_tiles = S.n_tiles;
+  e->cur_bc = (const BcMember*)(e->d_arena + S.bc), e->cur_n_bc = S.n_bc;
+  e->cur_drift = (const DriftMember*)(e->d_arena + S.drift), e->cur_n_drift = S.n_drift;
+  e->cur_args = (const StepArgs*)(e->d_arena + S.args);
+  e->cur_active = (const uint8_t*)(e->d_arena + S.active);
+  int rc = 0;
+  for (int i = 0; i < n_steps && !rc; ++i)
+    for (int part = 0; part < 5 && !rc; ++part)
+      if (parts & (1u << part)) rc = ensemble_launch_part(e, part, (hipStream_t)stream);
+  use_full_tables(e);
   return rc;
 }
 

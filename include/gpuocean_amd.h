@@ -363,6 +363,22 @@ int gpo_ensemble_part(gpo_ensemble e, int32_t part, void* stream);
  * (OceanModelEnsemble.stepToObservation, ensembles/OceanModelEnsemble.py:121-135). */
 int gpo_ensemble_select(gpo_ensemble e, const uint8_t* active, const float* noise_scale, void* stream);
 
+/* Prepares n_sets selections at once, for gpo_ensemble_run_set: set k selects
+ * the members with active[k*n + m] != 0 and steps them with time steps
+ * dts[k*n + m] (NULL: each member's current dt); noise_scale (n floats, may be
+ * NULL) sets every member's model-error scale.  One upload and one
+ * synchronisation of `stream` for all sets (allocates device memory when the
+ * sets outgrow the previous ones).  No reference counterpart: it batches the
+ * member loop of OceanModelEnsemble.stepToObservation
+ * (ensembles/OceanModelEnsemble.py:121-135). */
+int gpo_ensemble_prepare_sets(gpo_ensemble e, int32_t n_sets, const uint8_t* active, const float* dts,
+                              const float* noise_scale, void* stream);
+
+/* n_steps steps of prepared set `set`, each made of the parts whose bit is set
+ * in `parts` (bit p = gpo_ensemble_part p), launched without host
+ * synchronisation.  Afterwards all members are selected again. */
+int gpo_ensemble_run_set(gpo_ensemble e, int32_t set, int32_t n_steps, uint32_t parts, void* stream);
+
 /* Captures steps_per_graph steps on `stream` into a hipGraph (replacing any
  * earlier one); gpo_ensemble_replay launches it n_replays times. */
 int gpo_ensemble_capture(gpo_ensemble e, int32_t steps_per_graph, void* stream);
