@@ -58,11 +58,12 @@ def event_pair(stream):
 
 # ---------------------------------------------------------------------------- CPU baselines
 
-def cpu_baseline_single(kw, workload, budget_s=15.0):
-    """FP32 oracle (OpenMP, all host threads) on a bounded number of steps."""
+def cpu_baseline_single(kw, workload, budget_s=15.0, precision="f32"):
+    """The oracle (OpenMP, all host threads; FP32, or FP64 for the double-precision
+    build) on a bounded number of steps."""
     import numpy as np
     from oracle.oracle import OracleCDKLM16, num_threads
-    ora = OracleCDKLM16(**kw)
+    ora = OracleCDKLM16(**kw, precision=precision)
     dt = np.float32(ora.dt)
     cells = kw["nx"] * kw["ny"]
     t0 = time.perf_counter()
@@ -73,7 +74,7 @@ def cpu_baseline_single(kw, workload, budget_s=15.0):
     ora.run_steps(n, dt)
     el = time.perf_counter() - t0
     return {"value": cells * n / el, "unit": "cell-steps/s", "cores": num_threads(), "kind": "port",
-            "sample": f"{workload} {kw['nx']}x{kw['ny']}, {n} RK2 steps of the FP32 oracle "
+            "sample": f"{workload} {kw['nx']}x{kw['ny']}, {n} RK2 steps of the {precision.upper()} oracle "
                       f"({el:.1f} s), wall and periodic ghosts in C"}
 
 
@@ -172,7 +173,7 @@ def run_single(kw, workload, steps, warmup, fast, chunk_rows, waves_per_simd=0):
     return {
         "value": value, "unit": "cell-steps/s", "ms_per_step": 1e3 * wall / steps, "steps": steps,
         "config": {"workload": f"CDKLM16 single member {nx}x{ny}, RK2, "
-                               f"{'periodic-x walls-y double jet' if workload == 'c1' else 'walls, Gaussian bump'}",
+                               f"{'periodic-x walls-y double jet' if workload.startswith('c1') else 'walls, Gaussian bump'}",
                    "nx": nx, "ny": ny, "dt": float(dt), "members": 1,
                    "kernel_build": "fast" if fast else "exact"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -625,7 +626,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c1", "c0", "c3", "c4", "da", "copy"],
+    ap.add_argument("--workload", default="c2", choices=["c2", "c1", "c1big", "c0", "c3", "c4", "da", "copy"],
                     help="c3 = the adjoint/forward timing alone; c4 = the batched coastal ensemble (land, flow "
                          "relaxation, drifters); da = OceanModelEnsemble.stepToObservation batched against "
                          "per member; copy = HBM copy of 256 MiB, the calibration kernel for the traffic counters")
@@ -681,6 +682,9 @@ def main():
         if not args.no_c1:
             c1 = run_single(cases.config_c1(), "c1", 2000, 50, args.fast, 0)
             result["single_member_c1"] = c1
+            big = run_single(cases.config_c1_large(), "c1big", 200, 10, args.fast, 0)
+            big.pop("config", None)
+            result["single_member_8192x4096"] = big
             result["coarse_model_error"] = run_coarse_noise()
         if not args.no_adjoint:
             result["adjoint_c3"] = run_adjoint()
@@ -688,12 +692,15 @@ def main():
         if not args.no_cpu:
             log("cpu baseline (FP32 oracle, ensemble members one after another)...")
             result["cpu_baseline"] = cpu_baseline_ensemble(members_kw)
+            log("cpu baseline of the FP64 oracle (C1, bounded sample)...")
+            result["cpu_baseline_f64_c1"] = cpu_baseline_single(cases.config_c1(), "c1", budget_s=5.0,
+                                                                precision="f64")
         print(json.dumps(result), flush=True)
         return
 
     args.steps = 2000 if args.steps is None else args.steps
     args.warmup = 50 if args.warmup is None else args.warmup
-    kw = cases.config_c1() if args.workload == "c1" else cases.config_c0()
+    kw = {"c1": cases.config_c1, "c1big": cases.config_c1_large}.get(args.workload, cases.config_c0)()
     r = run_single(kw, args.workload, args.steps, args.warmup, args.fast, args.chunk_rows, args.waves_per_simd)
     result = {
         "metric": "member-cell-steps/s at 256 members; % HBM roofline; adjoint/forward time",

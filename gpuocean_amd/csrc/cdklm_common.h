@@ -19,6 +19,7 @@ constexpr int kStripOut = 60;
 constexpr int kWave = 64;
 constexpr int kWavesPerBlock = 4;
 constexpr int kDefaultChunkRows = 6;
+constexpr int kLargeChunkRows = 24;
 constexpr int kDefaultWavesPerSimd = 5;
 
 // Lane exchange through DPP wave shifts (verified on gfx950: wave_shr:1 gives

@@ -178,6 +178,26 @@ def config_c1(nx=2048, ny=1024, seed=1):
                 courant_number=0.7, boundary_conditions=Common.BoundaryConditions(1, 2, 1, 2))
 
 
+def config_c1_large(nx=8192, ny=4096):
+    """The double jet of config_c1 on a grid larger than the 256 MB last-level
+    cache (8192 x 4096: about 0.8 GB of state, bathymetry and stage), with an
+    analytic bathymetry 250 + 30 sin(6 pi x / Lx) cos(4 pi y / Ly) instead of
+    the filtered random field, whose construction at this size takes minutes."""
+    dx = dy = 1000.0
+    g, f = 9.81, 1e-4
+    xc = (np.arange(nx + 5) - 2.0) / nx
+    yc = (np.arange(ny + 5) - 2.0) / ny
+    H = (250.0 + 30.0 * np.sin(6 * np.pi * xc)[None, :] * np.cos(4 * np.pi * yc)[:, None]).astype(np.float32)
+    y = (np.arange(ny + 4) - 2 + 0.5) * dy
+    Ly = ny * dy
+    eta_y = 0.3 * (np.tanh((y - Ly / 4) / 5e4) - np.tanh((y - 3 * Ly / 4) / 5e4))
+    eta0 = np.repeat(eta_y[:, None], nx + 4, axis=1).astype(np.float32)
+    Hm = 0.25 * (H[:-1, :-1] + H[1:, :-1] + H[:-1, 1:] + H[1:, 1:])
+    hu0 = (-(g / f) * (Hm + eta0) * np.gradient(eta_y, dy)[:, None]).astype(np.float32)
+    return dict(eta0=eta0, hu0=hu0, hv0=np.zeros_like(eta0), H=H, nx=nx, ny=ny, dx=dx, dy=dy, dt=0.0, g=g, f=f,
+                r=0.0, courant_number=0.7, boundary_conditions=Common.BoundaryConditions(1, 2, 1, 2))
+
+
 def c2_member_sizes(n_members=256, seed=2024):
     """Seeded uneven member sizes: nx = 16*round(U{128..512}/16),
     ny = 16*round(U{128..384}/16)."""

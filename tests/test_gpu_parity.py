@@ -146,6 +146,24 @@ def test_nonsquare_sizes_and_bcs(gpu_ctx, bcs):
             assert np.array_equal(a, b), f"chunk {chunk}"
 
 
+def test_large_grid_default_chunks(gpu_ctx):
+    """On a grid with far more rows than the persistent grid has waves, the
+    default work items grow to 24-row chunks; the result equals 6-row chunks
+    bit for bit (both equal the oracle on the smaller grids above)."""
+    from gpuocean_amd import CDKLM16
+    print("8192x3600: default chunks vs 6-row chunks", flush=True)
+    kw = cases.config_c1_large(nx=8192, ny=3600)
+    kw["dt"] = 3.0
+    a = CDKLM16(gpu_ctx, **kw)
+    b = CDKLM16(gpu_ctx, **kw, chunk_rows=6)
+    a.step(9.0)
+    b.step(9.0)
+    for x, y in zip(a.download(), b.download()):
+        assert np.array_equal(x, y)
+    a.cleanUp()
+    b.cleanUp()
+
+
 def test_friction_and_rk1_rk3(gpu_ctx):
     from gpuocean_amd import CDKLM16
     kw, _ = cases.regression_setup(cases.REGRESSION_CASES[12])
