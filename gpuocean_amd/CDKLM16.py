@@ -125,9 +125,6 @@ class CDKLM16:
         self.boundary_conditions = Common.BoundaryConditions() if boundary_conditions is None \
             else boundary_conditions
         bc = self.boundary_conditions
-        for side in (bc.north, bc.east, bc.south, bc.west):
-            if int(side) == 4:
-                raise NotImplementedError("open linear interpolation boundaries (code 4) are not supported")
         self.bc_data = Common.BoundaryConditionsData() if boundary_conditions_data is None \
             else boundary_conditions_data
 
@@ -344,7 +341,7 @@ class CDKLM16:
     def _apply_bc(self, h):
         """Write ghost cells of a state triple (utils/Common.py:1310-1330)."""
         bc = self.boundary_conditions
-        if not (bc.north in (2, 3) or bc.south == 3 or bc.east in (2, 3) or bc.west == 3):
+        if not (bc.north in (2, 3, 4) or bc.south in (3, 4) or bc.east in (2, 3, 4) or bc.west in (3, 4)):
             return  # walls only: the reference writes nothing
         f = self._frs or {}
         ns_nx = int(self.bc_data.north.h[0].shape[0])
@@ -358,8 +355,11 @@ class CDKLM16:
             float(self._bc_times.bc_t), self._stream_handle()), "gpo_boundary_conditions")
 
     def _needs_explicit_bc_each_substep(self):
+        """Flow relaxation (3) and linear interpolation (4) ghost cells are
+        written by their own kernels between the RK stages; walls and periodic
+        sides are handled inside the step kernel."""
         bc = self.boundary_conditions
-        return bc.north == 3 or bc.south == 3 or bc.east == 3 or bc.west == 3
+        return any(int(c) in (3, 4) for c in (bc.north, bc.south, bc.east, bc.west))
 
     def _substep(self, src, dst, qn, dt, rk_step, forcing):
         p = self.params

@@ -56,6 +56,18 @@ __global__ void frs_ew(int bc_e, int bc_w, int nx, int ny, int se, int sw, float
 
 // periodic_boundary_intersections_NS / closed_boundary_intersections_NS
 // (boundary_kernels.cu:122-147, 188-212): one thread per corner column.
+__global__ void lin_ns(int bc_n, int bc_s, int nx, int ny, int sn, int ss, float* h, float* u, float* v, int pitch) {
+  const int ti = blockIdx.x * blockDim.x + threadIdx.x, tj = blockIdx.y * blockDim.y + threadIdx.y;
+  if (ti >= nx + 4 || tj >= ny + 4) return;
+  cells::lin_ns_cell(bc_n, bc_s, nx, ny, sn, ss, h, u, v, pitch, ti, tj);
+}
+
+__global__ void lin_ew(int bc_e, int bc_w, int nx, int ny, int se, int sw, float* h, float* u, float* v, int pitch) {
+  const int ti = blockIdx.x * blockDim.x + threadIdx.x, tj = blockIdx.y * blockDim.y + threadIdx.y;
+  if (ti >= nx + 4 || tj >= ny + 4) return;
+  cells::lin_ew_cell(bc_e, bc_w, nx, ny, se, sw, h, u, v, pitch, ti, tj);
+}
+
 __global__ void hi_ns(int nx, int ny, int periodic, float* Hi, int pitch) {
   const int ti = blockIdx.x * blockDim.x + threadIdx.x;
   if (ti >= nx + 5) return;
@@ -449,25 +461,28 @@ int gpo_boundary_conditions(int32_t nx, int32_t ny, uint32_t bc, const int32_t s
   hipStream_t s = (hipStream_t)stream;
   const int sn = sponge ? sponge[0] : 0, se = sponge ? sponge[1] : 0, ss = sponge ? sponge[2] : 0,
             sw = sponge ? sponge[3] : 0;
+  const dim3 grid(ceil_div(nx + 4, 16), ceil_div(ny + 4, 16)), block(16, 16);
   if (bn == 2) {
     hipLaunchKernelGGL(periodic_ns, dim3(ceil_div(nx + 4, 256), 4), dim3(256), 0, s, nx, ny, eta, hu, hv, pitch);
-  } else if (bn == 3 || bs == 3) {
-    if (!ns_cur || !ns_next || ns_nx < 1) return bad("gpo_boundary_conditions: FRS north/south data missing");
-    hipLaunchKernelGGL(frs_ns, dim3(ceil_div(nx + 4, 16), ceil_div(ny + 4, 16)), dim3(16, 16), 0, s, bn, bs, nx, ny,
-                       sn, ss, eta, hu, hv, pitch, ns_cur, ns_next, ns_nx, frs_t);
-  } else if (bn == 4 || bs == 4) {
-    set_error("gpo_boundary_conditions: open linear interpolation (code 4) is not implemented");
-    return GPO_ERR_UNSUPPORTED;
+  } else {
+    if (bn == 3 || bs == 3) {
+      if (!ns_cur || !ns_next || ns_nx < 1) return bad("gpo_boundary_conditions: FRS north/south data missing");
+      hipLaunchKernelGGL(frs_ns, grid, block, 0, s, bn, bs, nx, ny, sn, ss, eta, hu, hv, pitch, ns_cur, ns_next, ns_nx,
+                         frs_t);
+    }
+    if (bn == 4 || bs == 4)
+      hipLaunchKernelGGL(lin_ns, grid, block, 0, s, bn, bs, nx, ny, sn, ss, eta, hu, hv, pitch);
   }
   if (be == 2) {
     hipLaunchKernelGGL(periodic_ew, dim3(ceil_div(ny + 4, 256), 4), dim3(256), 0, s, nx, ny, eta, hu, hv, pitch);
-  } else if (be == 3 || bw == 3) {
-    if (!ew_cur || !ew_next || ew_ny < 1) return bad("gpo_boundary_conditions: FRS east/west data missing");
-    hipLaunchKernelGGL(frs_ew, dim3(ceil_div(nx + 4, 16), ceil_div(ny + 4, 16)), dim3(16, 16), 0, s, be, bw, nx, ny,
-                       se, sw, eta, hu, hv, pitch, ew_cur, ew_next, ew_ny, frs_t);
-  } else if (be == 4 || bw == 4) {
-    set_error("gpo_boundary_conditions: open linear interpolation (code 4) is not implemented");
-    return GPO_ERR_UNSUPPORTED;
+  } else {
+    if (be == 3 || bw == 3) {
+      if (!ew_cur || !ew_next || ew_ny < 1) return bad("gpo_boundary_conditions: FRS east/west data missing");
+      hipLaunchKernelGGL(frs_ew, grid, block, 0, s, be, bw, nx, ny, se, sw, eta, hu, hv, pitch, ew_cur, ew_next, ew_ny,
+                         frs_t);
+    }
+    if (be == 4 || bw == 4)
+      hipLaunchKernelGGL(lin_ew, grid, block, 0, s, be, bw, nx, ny, se, sw, eta, hu, hv, pitch);
   }
   return check(hipGetLastError(), "gpo_boundary_conditions launch");
 }

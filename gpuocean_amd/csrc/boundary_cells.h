@@ -56,6 +56,42 @@ __device__ inline float3 bilinear3(const float* data, int dnx, int dny, float nx
 
 // flowRelaxationScheme_NS (boundary_kernels.cu:406-472) at cell (ti, tj); cells
 // outside the south/north sponges are left alone.
+// linearInterpolation_NS (gpu_kernels/boundary_kernels.cu:265-325): on a
+// code-4 side, the rows strictly between the outer ghost row and the row
+// `sponge` cells in are interpolated linearly between those two rows, which
+// are kept (two ghost cells).  Within one launch no cell reads a row another
+// cell writes.
+__device__ inline void lin_ns_cell(int bc_n, int bc_s, int nx, int ny, int sn, int ss, float* h, float* u, float* v,
+                                   int pitch, int ti, int tj) {
+  const int top = ny + 3;
+  const bool in = ((bc_s == 4 && tj < ss && tj > 0) || (bc_n == 4 && tj > top - sn && tj < top)) && ti > 0 &&
+                  ti < nx + 3;
+  if (!in) return;
+  int inner = ss, outer = 0;
+  if (tj > ss) inner = top - sn, outer = top;
+  const float ratio = (float)(tj - outer) / (float)(inner - outer);
+  for (float* a : {h, u, v}) {
+    const float vi = bc_row(a, pitch, inner)[ti], vo = bc_row(a, pitch, outer)[ti];
+    bc_row(a, pitch, tj)[ti] = vo + ratio * (vi - vo);
+  }
+}
+
+// linearInterpolation_EW (gpu_kernels/boundary_kernels.cu:330-384), by columns.
+__device__ inline void lin_ew_cell(int bc_e, int bc_w, int nx, int ny, int se, int sw, float* h, float* u, float* v,
+                                   int pitch, int ti, int tj) {
+  const int right = nx + 3;
+  const bool in = ((bc_w == 4 && ti < sw && ti > 0) || (bc_e == 4 && ti > right - se && ti < right)) && tj > 0 &&
+                  tj < ny + 3;
+  if (!in) return;
+  int inner = sw, outer = 0;
+  if (ti > sw) inner = right - se, outer = right;
+  const float ratio = (float)(ti - outer) / (float)(inner - outer);
+  for (float* a : {h, u, v}) {
+    float* r = bc_row(a, pitch, tj);
+    r[ti] = r[outer] + ratio * (r[inner] - r[outer]);
+  }
+}
+
 __device__ inline void frs_ns_cell(int bc_n, int bc_s, int nx, int ny, int sn, int ss, float* h, float* u, float* v,
                                    int pitch, const float* cur, const float* next, int bc_nx, float t_, int ti,
                                    int tj) {

@@ -109,9 +109,12 @@ int gpo_cdklm16_rk2_steps(const gpo_cdklm16_params* p, int32_t n_steps, float dt
 /* Ghost-cell boundary conditions on a state triple.
  * Replaces BoundaryConditionsArakawaA.boundaryCondition
  * (utils/Common.py:1310-1330) with periodicBoundary_NS/_EW
- * (gpu_kernels/boundary_kernels.cu:36-110) and flowRelaxationScheme_NS/_EW
- * (boundary_kernels.cu:406-540).  Codes 1 (wall) write nothing, as in the
- * reference.  frs_* may be NULL when no side is 3.
+ * (gpu_kernels/boundary_kernels.cu:36-110), linearInterpolation_NS/_EW
+ * (boundary_kernels.cu:265-384) and flowRelaxationScheme_NS/_EW
+ * (boundary_kernels.cu:406-540).  Per axis: periodic, or else flow relaxation
+ * on code-3 sides followed by linear interpolation on code-4 sides.  Codes 1
+ * (wall) write nothing, as in the reference.  frs_* may be NULL when no side
+ * is 3.
  *   frs_ns_* : [2][frs_ns_nx][3] (south, north), frs_ew_* : [frs_ew_ny][2][3]. */
 int gpo_boundary_conditions(int32_t nx, int32_t ny, uint32_t boundary_conditions,
                             const int32_t sponge_cells[4] /* n, e, s, w */,

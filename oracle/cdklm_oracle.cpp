@@ -705,22 +705,66 @@ void frs_ew(const gpo_oracle_params* q, T* h, T* u, T* v, const float* cur, cons
     }
 }
 
-// BoundaryConditionsArakawaA.boundaryCondition (utils/Common.py:1310-1330)
+// linearInterpolation_NS (boundary_kernels.cu:265-325): on a code-4 side the
+// rows strictly between the outer ghost row and the row `sponge` cells in are
+// interpolated linearly between those two rows, which are kept.
+template <typename T> void lin_ns(const gpo_oracle_params* q, T* h, T* u, T* v) {
+  const int nx = q->nx, ny = q->ny, W = nx + 4, top = ny + 4 - 1;
+  const int sn = q->sponge_north, ss = q->sponge_south;
+  for (int tj = 0; tj < ny + 4; ++tj)
+    for (int ti = 0; ti < nx + 4; ++ti) {
+      const bool in = ((q->bc_south == 4 && tj < ss && tj > 0) || (q->bc_north == 4 && tj > top - sn && tj < top)) &&
+                      ti > 0 && ti < nx + 4 - 1;
+      if (!in) continue;
+      int inner = ss, outer = 0;
+      if (tj > ss) inner = top - sn, outer = top;
+      const float ratio = (float)(tj - outer) / (float)(inner - outer);
+      for (T* a : {h, u, v}) {
+        const T vi = a[size_t(inner) * W + ti], vo = a[size_t(outer) * W + ti];
+        a[size_t(tj) * W + ti] = vo + T(ratio) * (vi - vo);
+      }
+    }
+}
+
+// linearInterpolation_EW (boundary_kernels.cu:330-384), by columns.
+template <typename T> void lin_ew(const gpo_oracle_params* q, T* h, T* u, T* v) {
+  const int nx = q->nx, ny = q->ny, W = nx + 4, right = nx + 4 - 1;
+  const int se = q->sponge_east, sw = q->sponge_west;
+  for (int tj = 0; tj < ny + 4; ++tj)
+    for (int ti = 0; ti < nx + 4; ++ti) {
+      const bool in = ((q->bc_west == 4 && ti < sw && ti > 0) || (q->bc_east == 4 && ti > right - se && ti < right)) &&
+                      tj > 0 && tj < ny + 4 - 1;
+      if (!in) continue;
+      int inner = sw, outer = 0;
+      if (ti > sw) inner = right - se, outer = right;
+      const float ratio = (float)(ti - outer) / (float)(inner - outer);
+      for (T* a : {h, u, v}) {
+        const T vi = a[size_t(tj) * W + inner], vo = a[size_t(tj) * W + outer];
+        a[size_t(tj) * W + ti] = vo + T(ratio) * (vi - vo);
+      }
+    }
+}
+
+// BoundaryConditionsArakawaA.boundaryCondition (utils/Common.py:1310-1330):
+// north/south then east/west; on each axis periodic, or else flow relaxation
+// and then linear interpolation for the sides that have them.
 template <typename T>
 void apply_bc(const gpo_oracle_params* q, T* h, T* u, T* v, const gpo_oracle_frs* frs) {
   if (q->bc_north == 2) {
     periodic_ns(q->nx, q->ny, h);
     periodic_ns(q->nx, q->ny, u);
     periodic_ns(q->nx, q->ny, v);
-  } else if ((q->bc_north == 3 || q->bc_south == 3) && frs) {
-    frs_ns(q, h, u, v, frs->ns_cur, frs->ns_next, frs->ns_nx, frs->t);
+  } else {
+    if ((q->bc_north == 3 || q->bc_south == 3) && frs) frs_ns(q, h, u, v, frs->ns_cur, frs->ns_next, frs->ns_nx, frs->t);
+    if (q->bc_north == 4 || q->bc_south == 4) lin_ns(q, h, u, v);
   }
   if (q->bc_east == 2) {
     periodic_ew(q->nx, q->ny, h);
     periodic_ew(q->nx, q->ny, u);
     periodic_ew(q->nx, q->ny, v);
-  } else if ((q->bc_east == 3 || q->bc_west == 3) && frs) {
-    frs_ew(q, h, u, v, frs->ew_cur, frs->ew_next, frs->ew_ny, frs->t);
+  } else {
+    if ((q->bc_east == 3 || q->bc_west == 3) && frs) frs_ew(q, h, u, v, frs->ew_cur, frs->ew_next, frs->ew_ny, frs->t);
+    if (q->bc_east == 4 || q->bc_west == 4) lin_ew(q, h, u, v);
   }
 }
 

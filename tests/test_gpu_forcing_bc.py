@@ -52,6 +52,32 @@ def test_frs_boundaries_match_oracle(gpu_ctx, codes):
             assert np.array_equal(a, b), f"t {sim.t}"
 
 
+@pytest.mark.parametrize("codes", [(4, 1, 4, 1), (1, 4, 1, 4), (4, 4, 4, 4), (4, 2, 4, 2), (2, 4, 2, 4),
+                                   (3, 4, 4, 3), (4, 3, 1, 4)])
+def test_linear_interpolation_boundaries_match_oracle(gpu_ctx, codes):
+    """Open boundaries by linear interpolation (code 4), alone and mixed with
+    walls, periodic sides and flow relaxation on the same axis, exact build,
+    value for value against the oracle."""
+    print(f"linear interpolation {codes}", flush=True)
+    nx, ny = 90, 60
+    kw = _base(nx, ny, 2)
+    sponge = {'north': 8, 'south': 10, 'east': 6, 'west': 9}
+    bc = Common.BoundaryConditions(*codes, spongeCells=sponge)
+    kw.update(boundary_conditions=bc)
+    if 3 in codes:
+        kw.update(boundary_conditions_data=_frs_data(nx, ny))
+    sim = CDKLM16(gpu_ctx, **kw, exact_arithmetic=True)
+    ora = OracleCDKLM16(**kw)
+    for a, b in zip(sim.download(), ora.download()):
+        assert np.array_equal(a, b), "initial boundary conditions"
+    for t_end in (9.0, 27.0):
+        sim.step(t_end)
+        ora.step(t_end)
+        for a, b in zip(sim.download(), ora.download()):
+            assert np.array_equal(a, b), f"t {sim.t}"
+    sim.cleanUp()
+
+
 def test_wind_and_pressure_forcing_match_oracle(gpu_ctx):
     print("wind + pressure", flush=True)
     nx, ny = 80, 64

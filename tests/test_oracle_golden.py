@@ -169,3 +169,88 @@ def test_periodic_ghosts_copy_opposite_interior():
     assert np.array_equal(e[ny + 2:ny + 4, :], e[2:4, :])
     assert np.array_equal(e[:, 0:2], e[:, nx:nx + 2])
     assert np.array_equal(e[:, nx + 2:nx + 4], e[:, 2:4])
+
+
+def _lin_bc_state(codes, sponge, nx=40, ny=30, seed=9):
+    from gpuocean_amd import Common
+    rng = np.random.default_rng(seed)
+    h0 = [rng.standard_normal((ny + 4, nx + 4)).astype(np.float32) for _ in range(3)]
+    H = (50.0 + rng.random((ny + 5, nx + 5))).astype(np.float32)
+    o = OracleCDKLM16(*[a.copy() for a in h0], H, nx, ny, 100.0, 100.0, 1.0, 9.81, 0.0, 0.0,
+                      boundary_conditions=Common.BoundaryConditions(*codes, spongeCells=sponge))
+    o.step(0.0)   # at t = 0 this applies the boundary conditions only
+    return h0, o.download()
+
+
+def _on_line(vals, outer, inner, k_outer, k_inner):
+    """vals[k] for k strictly between k_outer and k_inner lie on the straight
+    line through (k_outer, outer) and (k_inner, inner), to FP32 rounding."""
+    k = np.arange(min(k_outer, k_inner) + 1, max(k_outer, k_inner))
+    w = (k - k_outer) / (k_inner - k_outer)
+    line = outer[None].astype(np.float64) + w[:, None] * (inner[None].astype(np.float64) - outer[None])
+    scale = np.maximum(np.abs(outer), np.abs(inner))[None] + 1e-30
+    return np.max(np.abs(vals[k] - line) / scale) < 4e-7
+
+
+@pytest.mark.parametrize("codes", [(4, 1, 4, 1), (4, 2, 1, 2), (1, 1, 4, 1)])
+def test_linear_interpolation_ns_is_a_straight_line(codes):
+    """Open boundary by linear interpolation (boundary_kernels.cu:265-325): on
+    a code-4 side the rows strictly between the outermost ghost row and the
+    row `sponge` cells in become the straight line between those two rows,
+    which keep their values; the outermost columns and every other row are
+    left alone."""
+    sn, ss, nx, ny = 6, 9, 40, 30
+    h0, h1 = _lin_bc_state(codes, {'north': sn, 'south': ss, 'east': 0, 'west': 0})
+    top = ny + 3
+    # periodic east/west overwrites the two ghost columns each side afterwards
+    c = slice(2, -2) if codes[1] == 2 else slice(1, -1)
+    for a0, a in zip(h0, h1):
+        touched = np.zeros(a.shape, bool)
+        if codes[2] == 4:
+            assert _on_line(a[:, c], a0[0, c], a0[ss, c], 0, ss)
+            touched[1:ss, 1:-1] = True
+        if codes[0] == 4:
+            assert _on_line(a[:, c], a0[top, c], a0[top - sn, c], top, top - sn)
+            touched[top - sn + 1:top, 1:-1] = True
+        if codes[1] == 2:
+            touched[:, :2] = touched[:, -2:] = True
+        assert np.array_equal(a[~touched], a0[~touched])
+        assert not np.array_equal(a[touched], a0[touched])
+
+
+@pytest.mark.parametrize("codes", [(1, 4, 1, 4), (2, 4, 2, 4), (1, 1, 1, 4)])
+def test_linear_interpolation_ew_is_a_straight_line(codes):
+    """The east/west counterpart (boundary_kernels.cu:330-384), by columns."""
+    se, sw, nx, ny = 5, 7, 40, 30
+    h0, h1 = _lin_bc_state(codes, {'north': 0, 'south': 0, 'east': se, 'west': sw})
+    right = nx + 3
+    for a0, a in zip(h0, h1):
+        if codes[0] == 2:   # periodic north/south first: the columns interpolate the updated rows
+            a0 = a0.copy()
+            a0[:2], a0[-2:] = a0[ny:ny + 2].copy(), a0[2:4].copy()
+        touched = np.zeros(a.shape, bool)
+        if codes[3] == 4:
+            assert _on_line(a[1:-1].T, a0[1:-1, 0], a0[1:-1, sw], 0, sw)
+            touched[1:-1, 1:sw] = True
+        if codes[1] == 4:
+            assert _on_line(a[1:-1].T, a0[1:-1, right], a0[1:-1, right - se], right, right - se)
+            touched[1:-1, right - se + 1:right] = True
+        assert np.array_equal(a[~touched], a0[~touched])
+
+
+def test_linear_interpolation_all_sides_rows_then_columns():
+    """With code 4 on every side the rows are interpolated first and the
+    columns afterwards from the interpolated rows (utils/Common.py:1310-1330),
+    so every band cell away from the corners lies on its column's line and
+    every east/west band cell on its row's line."""
+    sp = {'north': 5, 'south': 5, 'east': 6, 'west': 6}
+    nx, ny = 40, 30
+    h0, h1 = _lin_bc_state((4, 4, 4, 4), sp)
+    top, right = ny + 3, nx + 3
+    for a0, a in zip(h0, h1):
+        mid = slice(6, right - 6 + 1)   # columns outside the east/west bands
+        assert _on_line(a[:, mid], a0[0, mid], a0[5, mid], 0, 5)
+        assert _on_line(a[:, mid], a0[top, mid], a0[top - 5, mid], top, top - 5)
+        assert _on_line(a[1:-1].T, a[1:-1, 0], a[1:-1, 6], 0, 6)
+        assert _on_line(a[1:-1].T, a[1:-1, right], a[1:-1, right - 6], right, right - 6)
+        assert np.array_equal(a[[0, -1]], a0[[0, -1]])   # outermost rows are kept
