@@ -535,7 +535,8 @@ int adj_batch_copy(AdjBatch* b, const CopyItem* tab, hipStream_t s) {
 
 int adj_batch_steps(AdjBatch* b, const StepArgs* tab, int rk, hipStream_t s) {
   const hipError_t e =
-      launch_cdklm_batched_exact(tab, b->d_work, b->n_work, rk, b->n_blocks, b->has_land, b->f_array, b->angle0, s);
+      launch_cdklm_batched_exact(tab, b->d_work, b->n_work, rk, b->n_blocks, b->has_land, b->f_array, b->angle0,
+                                 /*any_stage=*/0, s);  // RK2 without friction
   return e != hipSuccess ? hip_check(e, "batched substep launch") : 0;
 }
 

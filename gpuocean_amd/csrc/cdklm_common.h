@@ -60,10 +60,14 @@ struct StepArgs {
 hipError_t launch_cdklm_fast(const StepArgs& a, int has_land, int f_array, int angle0, hipStream_t s);
 hipError_t launch_cdklm_exact(const StepArgs& a, int has_land, int f_array, int angle0, hipStream_t s);
 
+// any_stage: some member has RK3 or bottom friction (the general kernel);
+// otherwise rk_step 0/1 select the kernels specialised to the RK2 stages.
 hipError_t launch_cdklm_batched_fast(const StepArgs* members, const int2* work, int n_work, int rk_step,
-                                     int n_blocks, int has_land, int f_array, int angle0, hipStream_t s);
+                                     int n_blocks, int has_land, int f_array, int angle0, int any_stage,
+                                     hipStream_t s);
 hipError_t launch_cdklm_batched_exact(const StepArgs* members, const int2* work, int n_work, int rk_step,
-                                      int n_blocks, int has_land, int f_array, int angle0, hipStream_t s);
+                                      int n_blocks, int has_land, int f_array, int angle0, int any_stage,
+                                      hipStream_t s);
 
 // Builds the argument block of one substep (aux_kernels.hip); returns 0 or an error code.
 int make_step_args(const gpo_cdklm16_params* p, float dt, int32_t rk_step, const float* in_eta, const float* in_hu,

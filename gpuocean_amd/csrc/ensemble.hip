@@ -215,6 +215,7 @@ int bad(const char* msg) {
 struct Ensemble {
   int n = 0, n_work = 0, n_blocks = 0, exact = 1, has_noise = 0;
   int has_land = 0, f_array = 0, angle0 = 1;
+  int any_stage = 0;  // some member has bottom friction: the general substep kernel
   int n_tiles = 0;  // model-error tiles over all members
   StepArgs* d_args = nullptr;
   int2* d_tiles = nullptr;
@@ -275,9 +276,9 @@ static void use_full_tables(Ensemble* e) {
 static int ensemble_launch_substep(Ensemble* e, int rk, hipStream_t s) {
   const hipError_t err =
       e->exact ? launch_cdklm_batched_exact(e->cur_args, e->cur_work, e->cur_n_work, rk, e->n_blocks,
-                                            e->has_land, e->f_array, e->angle0, s)
+                                            e->has_land, e->f_array, e->angle0, e->any_stage, s)
                : launch_cdklm_batched_fast(e->cur_args, e->cur_work, e->cur_n_work, rk, e->n_blocks, e->has_land,
-                                           e->f_array, e->angle0, s);
+                                           e->f_array, e->angle0, e->any_stage, s);
   return err != hipSuccess ? check(err, "ensemble substep launch") : 0;
 }
 
@@ -399,6 +400,7 @@ int gpo_ensemble_create(const gpo_member* members, int32_t n, int32_t chunk_rows
       }
       frs |= code == 3;
     }
+    if (p.r > 0.0f) e->any_stage = 1;
     if (frs && !M.bc_each_step) {
       ensemble_free(e);
       return bad("gpo_ensemble_create: flow-relaxation members need bc_each_step");
