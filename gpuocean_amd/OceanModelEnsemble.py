@@ -92,8 +92,9 @@ class BatchedEnsemble:
                     return False, "drifters whose periodicity differs from their member's"
             if (s.model_error is None) != (s0.model_error is None):
                 return False, "some members carry model error and some do not"
-            if s.model_error is not None and int(s.model_error.interpolation_factor) != 1:
-                return False, "model error generated on a coarser grid (interpolation_factor > 1)"
+            if s.model_error is not None and \
+                    (int(s.model_error.interpolation_factor) > 1) != (int(s0.model_error.interpolation_factor) > 1):
+                return False, "some members generate model error on a coarser grid and some do not"
             if s.device != s0.device or bool(s.exact_arithmetic) != bool(s0.exact_arithmetic):
                 return False, "members on different devices or arithmetic modes"
             p, q = s.params, s0.params
@@ -127,6 +128,11 @@ class BatchedEnsemble:
                 m.g = float(np.float32(s.g))
                 m.f = float(np.float32(s.f)) if np.asarray(s.f).size == 1 else 0.0
                 m.beta = float(np.float32(s.coriolis_beta))
+                if int(me.interpolation_factor) > 1:
+                    # perturbSim's coarse grid, aligned as it aligns it by default
+                    m.coarse_nx, m.coarse_ny = int(me.coarse_nx), int(me.coarse_ny)
+                    m.coarse_dx, m.coarse_dy = float(me.coarse_dx), float(me.coarse_dy)
+                    m.coarse_offset_i, m.coarse_offset_j = me._obtain_coarse_grid_offset(None, None)
             m.bc_each_step = int(self._bc_in_batch(s))
             for k in range(4):
                 m.sponge[k] = int(s.sponge[k])

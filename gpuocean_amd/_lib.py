@@ -57,6 +57,8 @@ class Member(ctypes.Structure):
         ("frs_ew_cur", c_void_p), ("frs_ew_next", c_void_p), ("frs_ew_ny", c_int32), ("frs_t", c_float),
         ("drifters", c_int32), ("n_drifters", c_int32), ("positions", c_void_p), ("pos_pitch", c_int32),
         ("sensitivity", c_float), ("Bm", c_void_p), ("Bm_pitch", c_int32),
+        ("coarse_nx", c_int32), ("coarse_ny", c_int32), ("coarse_dx", c_float), ("coarse_dy", c_float),
+        ("coarse_offset_i", c_int32), ("coarse_offset_j", c_int32),
     ]
 
 

@@ -35,6 +35,10 @@ struct NoiseMember {
   float* hv;
   const float* Hi;
   const float* f_cells;
+  // model error on a coarser grid: xi and SOAR over cnx x cny, then the
+  // bicubic interpolation to this member's grid (coarse = 0: on the grid)
+  int coarse_grid;
+  BicubicGeo geo;
 };
 
 // Model error of one output tile (kTileH rows x kTileW columns of interior
@@ -155,6 +159,59 @@ __global__ void __launch_bounds__(256) noise_fused(const NoiseMember* __restrict
   }
 }
 
+// The model-error tile table serves both the fused kernel and the bicubic one.
+static_assert(kTileW == kBiW && kTileH == kBiH, "model-error tiles of the two paths must agree");
+
+// Coarse-grid model error in three launches, each over every member (block
+// row y = member y; blocks past a member's extent and members left out of a
+// selection return at once):
+// 1. the member's normal field xi (rows x cols) of the current draw, one
+//    Philox call per four consecutive elements (gpo_random_normal's values);
+__global__ void __launch_bounds__(256) xi_batched(const NoiseMember* __restrict__ members,
+                                                  const uint32_t* __restrict__ draw,
+                                                  const uint8_t* __restrict__ active) {
+  const NoiseMember& m = members[blockIdx.y];
+  if (active && !active[blockIdx.y]) return;
+  const uint32_t n = (uint32_t)m.xi_rows * (uint32_t)m.xi_cols;
+  const uint32_t q = blockIdx.x * 256u + threadIdx.x;
+  if (4u * q >= n) return;
+  float v4[4];
+  normals4(q, draw[blockIdx.y], m.stream_id, m.k0, m.k1, v4);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const uint32_t e = 4u * q + (uint32_t)i;
+    if (e >= n) break;
+    const uint32_t r = e / (uint32_t)m.xi_cols, c = e - r * (uint32_t)m.xi_cols;
+    ((float*)((char*)m.xi + (size_t)r * m.xi_pitch))[c] = v4[i];
+  }
+}
+
+// 2. the SOAR field over the coarse grid and its two ghost rings (gpo_soar);
+__global__ void __launch_bounds__(256) soar_batched(const NoiseMember* __restrict__ members,
+                                                    const uint8_t* __restrict__ active) {
+  const NoiseMember& m = members[blockIdx.y];
+  if (active && !active[blockIdx.y]) return;
+  const int W = m.geo.cnx + 4, n = W * (m.geo.cny + 4);
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= n) return;
+  const int tj = k / W, ti = k - tj * W;
+  const float acc = soar_cell(m.geo.cnx, m.geo.cny, m.W, m.per_ns, m.per_ew, m.xi, m.xi_pitch, tj, ti);
+  ((float*)((char*)m.coarse + (size_t)tj * m.c_pitch))[ti] = m.scale * acc;
+}
+
+// 3. the bicubic interpolation and its balance, one tile of the table per
+//    block (gpo_bicubic_interpolation's tiles and arithmetic).
+__global__ void __launch_bounds__(256) bicubic_batched(const NoiseMember* __restrict__ members,
+                                                       const int2* __restrict__ tiles) {
+  __shared__ float coef[kBiMaxCoarse][17];
+  __shared__ float d_eta[kBiH + 2][kBiW + 2];
+  const int2 t = tiles[blockIdx.x];
+  const NoiseMember& m = members[t.x];
+  bicubic_tile(m.geo, m.g, m.f, m.beta, m.north_x, m.north_y, m.f == 0.0f ? m.f_cells : nullptr, m.f_pitch,
+               m.coarse, m.c_pitch, m.eta, m.hu, m.hv, m.pitch, m.Hi, m.hi_pitch, m.land,
+               2 + kBiW * (t.y & 0xFFFF), 2 + kBiH * (t.y >> 16), coef, d_eta);
+}
+
 // Every member's draw index (or every selected one's) advances once the
 // step's model error is applied.
 __global__ void advance_draws(uint32_t* draw, int n, const uint8_t* active) {
@@ -217,6 +274,9 @@ struct Ensemble {
   int has_land = 0, f_array = 0, angle0 = 1;
   int any_stage = 0;  // some member has bottom friction: the general substep kernel
   int n_tiles = 0;  // model-error tiles over all members
+  // model error on coarser grids: the largest normal field (in Philox quads)
+  // and coarse buffer over the members, the x extents of the first two launches
+  int coarse_noise = 0, max_xi_quads = 0, max_coarse_cells = 0;
   StepArgs* d_args = nullptr;
   int2* d_tiles = nullptr;
   int2* d_work = nullptr;
@@ -284,8 +344,16 @@ static int ensemble_launch_substep(Ensemble* e, int rk, hipStream_t s) {
 
 static int ensemble_launch_noise(Ensemble* e, hipStream_t s) {
   if (!e->has_noise) return 0;
-  if (e->cur_n_tiles > 0)
+  if (e->coarse_noise) {
+    hipLaunchKernelGGL(xi_batched, dim3((e->max_xi_quads + 255) / 256, e->n), dim3(256), 0, s, e->d_noise, e->d_draw,
+                       e->cur_active);
+    hipLaunchKernelGGL(soar_batched, dim3((e->max_coarse_cells + 255) / 256, e->n), dim3(256), 0, s, e->d_noise,
+                       e->cur_active);
+    if (e->cur_n_tiles > 0)
+      hipLaunchKernelGGL(bicubic_batched, dim3(e->cur_n_tiles), dim3(256), 0, s, e->d_noise, e->cur_tiles);
+  } else if (e->cur_n_tiles > 0) {
     hipLaunchKernelGGL(noise_fused, dim3(e->cur_n_tiles), dim3(256), 0, s, e->d_noise, e->cur_tiles, e->d_draw);
+  }
   hipLaunchKernelGGL(advance_draws, dim3((e->n + 255) / 256), dim3(256), 0, s, e->d_draw, e->n, e->cur_active);
   return check(hipGetLastError(), "ensemble noise launch");
 }
@@ -518,10 +586,36 @@ int gpo_ensemble_create(const gpo_member* members, int32_t n, int32_t chunk_rows
       nm.hv = M.h0_hv;
       nm.Hi = M.Hi;
       nm.f_cells = M.f_cells;
-      if (!nm.xi || !nm.coarse || nm.xi_rows != (nm.per_ns ? nm.ny : nm.ny + 8) ||
-          nm.xi_cols != (nm.per_ew ? nm.nx : nm.nx + 8)) {
+      nm.coarse_grid = M.coarse_nx > 0 ? 1 : 0;
+      const int cnx = nm.coarse_grid ? M.coarse_nx : p.nx, cny = nm.coarse_grid ? M.coarse_ny : p.ny;
+      nm.geo = BicubicGeo{p.nx, p.ny, cnx, cny, M.coarse_offset_i, M.coarse_offset_j,
+                          p.dx, p.dy, M.coarse_dx, M.coarse_dy};
+      if (m == 0) e->coarse_noise = nm.coarse_grid;
+      if (nm.coarse_grid != e->coarse_noise) {
+        ensemble_free(e);
+        return bad("gpo_ensemble_create: members differ in whether their model error lives on a coarser grid");
+      }
+      if (nm.coarse_grid &&
+          (cny < 1 || !(M.coarse_dx > 0.0f) || !(M.coarse_dy > 0.0f) || M.coarse_offset_i < -cnx ||
+           M.coarse_offset_i > cnx || M.coarse_offset_j < -cny || M.coarse_offset_j > cny)) {
+        ensemble_free(e);
+        return bad("gpo_ensemble_create: bad coarse model-error grid");
+      }
+      if (!nm.xi || !nm.coarse || nm.xi_rows != (nm.per_ns ? cny : cny + 8) ||
+          nm.xi_cols != (nm.per_ew ? cnx : cnx + 8) || nm.xi_pitch < 4 * nm.xi_cols ||
+          nm.c_pitch < 4 * (cnx + 4)) {
         ensemble_free(e);
         return bad("gpo_ensemble_create: model error buffers missing or mis-sized");
+      }
+      if (nm.coarse_grid) {
+        const long long quads = ((long long)nm.xi_rows * nm.xi_cols + 3) / 4;
+        const long long cells = (long long)(cnx + 4) * (cny + 4);
+        if (quads > (1ll << 30) || cells > (1ll << 30)) {
+          ensemble_free(e);
+          return bad("gpo_ensemble_create: coarse model-error grid too large");
+        }
+        e->max_xi_quads = std::max(e->max_xi_quads, (int)quads);
+        e->max_coarse_cells = std::max(e->max_coarse_cells, (int)cells);
       }
       noise.push_back(nm);
       const int tx_n = (p.nx + kTileW - 1) / kTileW, ty_n = (p.ny + kTileH - 1) / kTileH;

@@ -218,4 +218,62 @@ __device__ __forceinline__ void bicubic_balance_apply(float dx, float dy, float 
   mrow(hv, pitch, tj)[ti] += d_hv;
 }
 
+constexpr int kBiW = 64, kBiH = 16;
+constexpr int kBiMaxCoarse = 256;  // coarse cells per tile at factor >= 3: (66/3+2) x (18/3+2) = 192
+struct BicubicGeo {
+  int nx, ny, cnx, cny, oi, oj;
+  float dx, dy, cdx, cdy;
+};
+// Coarse-grid model error of one tile of kBiH x kBiW interior cells whose
+// first cell is (by, bx): bicubic interpolation of the coarse field to the
+// tile and its one-cell ring (coefficients formed once per coarse cell, in
+// LDS, when the tile spans few enough coarse cells) and its geostrophic
+// balance added to the state.  One 256-thread block; coef and d_eta are its
+// LDS arrays.
+__device__ __forceinline__ void bicubic_tile(const BicubicGeo& q, float g, float f, float beta, float north_x,
+                                             float north_y, const float* f_cells, int f_pitch, const float* coarse,
+                                             int c_pitch, float* eta, float* hu, float* hv, int pitch,
+                                             const float* Hi, int h_pitch, float land_value, const int bx,
+                                             const int by, float (*coef)[17], float (*d_eta)[kBiW + 2]) {
+  const int tid = threadIdx.x;
+  // the coarse cells under the tile and its ring: the spot of a fine point
+  // never decreases with its index, so the corners bound them
+  const CoarseSpot lo = coarse_spot(q.cnx, q.cny, q.dx, q.dy, q.cdx, q.cdy, q.oi, q.oj, by - 1, bx - 1);
+  const CoarseSpot hi = coarse_spot(q.cnx, q.cny, q.dx, q.dy, q.cdx, q.cdy, q.oi, q.oj, by + kBiH, bx + kBiW);
+  const int ncw = hi.ci - lo.ci + 1, nch = hi.cj - lo.cj + 1;
+  const bool shared = ncw * nch <= kBiMaxCoarse;  // false only for factors below 3
+  if (shared) {
+    for (int k = tid; k < ncw * nch; k += 256) {
+      float a[16];
+      bicubic_coeffs(q.cnx, q.cny, coarse, c_pitch, lo.cj + k / ncw, lo.ci + k % ncw, a);
+#pragma unroll
+      for (int i = 0; i < 16; ++i) coef[k][i] = a[i];
+    }
+  }
+  __syncthreads();
+  for (int k = tid; k < (kBiH + 2) * (kBiW + 2); k += 256) {
+    const int j = k / (kBiW + 2), i = k % (kBiW + 2);
+    const CoarseSpot p = coarse_spot(q.cnx, q.cny, q.dx, q.dy, q.cdx, q.cdy, q.oi, q.oj, by + j - 1, bx + i - 1);
+    float a[16];
+    if (shared) {
+      const int c = (p.cj - lo.cj) * ncw + (p.ci - lo.ci);
+#pragma unroll
+      for (int t = 0; t < 16; ++t) a[t] = coef[c][t];
+    } else {
+      bicubic_coeffs(q.cnx, q.cny, coarse, c_pitch, p.cj, p.ci, a);
+    }
+    d_eta[j][i] = bicubic_eval(a, p.rx, p.ry);
+  }
+  __syncthreads();
+  const int x = (tid & 63) + 1;
+  for (int y = (tid >> 6) + 1; y <= kBiH; y += 4) {
+    const int ti = bx + x - 1, tj = by + y - 1;
+    if (ti >= q.nx + 2 || tj >= q.ny + 2) continue;
+    bicubic_balance_apply(q.dx, q.dy, g, f, beta, north_x, north_y, f_cells, f_pitch, d_eta[y][x], d_eta[y][x - 1],
+                          d_eta[y][x + 1], d_eta[y - 1][x], d_eta[y + 1][x], eta, hu, hv, pitch, Hi, h_pitch,
+                          land_value, tj, ti);
+  }
+}
+
+
 }  // namespace gpo

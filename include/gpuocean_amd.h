@@ -311,6 +311,14 @@ typedef struct {
   float sensitivity;
   const float* Bm;                 /* cell-centre depth, (ny+4) rows */
   int32_t Bm_pitch;
+  /* model error on a coarser grid (OceanStateNoise interpolation_factor > 1):
+   * coarse_nx > 0 gives its interior size and spacing and the fine-cell offset
+   * of gpo_bicubic_interpolation; xi is then (coarse_ny+8) x (coarse_nx+8)
+   * (coarse_ny x coarse_nx on periodic axes) and coarse (coarse_ny+4) x
+   * (coarse_nx+4); coarse_nx = 0: the model error lives on the grid itself */
+  int32_t coarse_nx, coarse_ny;
+  float coarse_dx, coarse_dy;
+  int32_t coarse_offset_i, coarse_offset_j;
 } gpo_member;
 
 typedef void* gpo_ensemble;
@@ -325,7 +333,9 @@ typedef void* gpo_ensemble;
  * member's draw index lives on the device and advances once per step, so
  * replays of a captured graph draw fresh random numbers.  Results are bit for
  * bit those of the per-member sequence gpo_cdklm16_substep, (gpo_boundary_conditions),
- * gpo_cdklm16_substep, gpo_random_normal + gpo_soar + gpo_geostrophic_balance,
+ * gpo_cdklm16_substep, gpo_random_normal + gpo_soar + gpo_geostrophic_balance
+ * (gpo_bicubic_interpolation in place of the balance for coarse-grid model
+ * error, run as three batched launches: normal fields, SOAR, interpolation),
  * (gpo_boundary_conditions), (gpo_drift).  chunk_rows (0 = each member's own)
  * and waves_per_simd (0 = 4) set the work-item height and the persistent grid.
  * Allocates the device tables; synchronises `stream`.

@@ -151,3 +151,58 @@ def test_ocean_model_ensemble_batched_equals_sequential(gpu_ctx):
         assert pa.dt == pb.dt
     obs = np.array([[5000.0, 7000.0], [20000.0, 15000.0]])
     assert np.array_equal(ea.observeParticles(obs), eb.observeParticles(obs))
+
+
+COARSE = [((96, 48), 3), ((160, 64), 5), ((112, 96), 7), ((48, 80), 3), ((150, 30), 5)]
+
+
+def _coarse_members(gpu_ctx, bc=None):
+    sims = []
+    for i, ((nx, ny), factor) in enumerate(COARSE):
+        kw = cases.config_c2_member(nx, ny, 31 + i)
+        if bc is not None:
+            kw["boundary_conditions"] = bc
+        sim = CDKLM16(gpu_ctx, **kw, local_particle_id=i)
+        sim.setSOARModelError(5e-3, small_scale_perturbation_interpolation_factor=factor, xorwow_seed=500 + i)
+        sims.append(sim)
+    return sims
+
+
+@pytest.mark.parametrize("bc", [None, (1, 1, 1, 1)], ids=["periodic_x", "walls"])
+def test_batched_coarse_model_error_matches_per_member(gpu_ctx, bc):
+    """Model error generated on grids 3, 5 and 7 times coarser (normal field
+    and SOAR on the coarse grid, bicubic interpolation and balance on the
+    member's): three batched launches give the per-member values bit for bit,
+    directly and through graph replays, and leave the members' normal and
+    coarse fields as the per-member path leaves them."""
+    print(f"batched coarse model error, bc={bc}", flush=True)
+    bcs = None if bc is None else Common.BoundaryConditions(*bc)
+    plain = _coarse_members(gpu_ctx, bcs)
+    batched = _coarse_members(gpu_ctx, bcs)
+    ens = BatchedEnsemble(batched)
+    for s in plain:
+        for _ in range(4):
+            s.step(np.float32(s.dt))
+    ens.step(4)
+    _assert_same(batched, plain)
+    assert ens.device_draws() == [4] * len(COARSE)
+    for a, b in zip(batched, plain):
+        assert np.array_equal(a.model_error.getRandomNumbers(), b.model_error.getRandomNumbers())
+        assert np.array_equal(a.model_error.getCoarseBuffer(), b.model_error.getCoarseBuffer())
+    # more steps through a captured graph of three steps, replayed twice
+    for s in plain:
+        for _ in range(6):
+            s.step(np.float32(s.dt))
+    ens.step(6, graph_steps=3)
+    _assert_same(batched, plain)
+    ens.destroy()
+
+
+def test_coarse_and_fine_model_error_do_not_batch_together(gpu_ctx):
+    sims = _coarse_members(gpu_ctx)[:2]
+    kw = cases.config_c2_member(64, 48, 3)
+    fine = CDKLM16(gpu_ctx, **kw, local_particle_id=9)
+    fine.setSOARModelError(5e-3, xorwow_seed=7)
+    ok, why = BatchedEnsemble.eligible(sims + [fine])
+    assert not ok and "coarser grid" in why
+    assert BatchedEnsemble.eligible(sims)[0]

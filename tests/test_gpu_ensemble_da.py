@@ -12,17 +12,20 @@ from gpuocean_amd import OceanModelEnsemble
 pytestmark = pytest.mark.gpu
 
 SIZES = [(64, 48), (96, 64), (48, 80), (128, 32)]
+SIZES_3 = [(66, 48), (96, 64), (48, 80), (129, 32)]   # periodic x: nx divisible by 3
 
 
-def _ensemble(gpu_ctx, batched, drifters=False, noise=True):
+def _ensemble(gpu_ctx, batched, drifters=False, noise=True, factor=1):
     data = []
-    for i, (nx, ny) in enumerate(SIZES):
+    for i, (nx, ny) in enumerate(SIZES if factor == 1 else SIZES_3):
         kw = cases.config_c2_member(nx, ny, 40 + i)
         data.append({k: kw[k] for k in ("eta0", "hu0", "hv0", "H", "nx", "ny", "dx", "dy", "boundary_conditions")})
     kw = cases.config_c2_member(*SIZES[0], 40)
     sim_args = {k: kw[k] for k in ("dt", "g", "f", "r", "courant_number")}
     sim_args["model_time_step"] = 60.0
     me = {"small_scale_perturbation_amplitude": 5e-3, "xorwow_seed": 77} if noise else {}
+    if factor > 1:
+        me["small_scale_perturbation_interpolation_factor"] = factor
     ens = OceanModelEnsemble(gpu_ctx, sim_args, data, me, numParticles=len(SIZES), batched=batched)
     if drifters:
         rng = np.random.default_rng(5)
@@ -45,11 +48,13 @@ def _same(a, b, label):
             assert x.drifter_t == y.drifter_t
 
 
-@pytest.mark.parametrize("drifters,noise", [(False, True), (True, True), (False, False)])
-def test_batched_step_to_observation_matches_members(gpu_ctx, drifters, noise):
-    print(f"stepToObservation batched vs per member, drifters={drifters}, noise={noise}", flush=True)
-    a = _ensemble(gpu_ctx, True, drifters, noise)
-    b = _ensemble(gpu_ctx, False, drifters, noise)
+@pytest.mark.parametrize("drifters,noise,factor", [(False, True, 1), (True, True, 1), (False, False, 1),
+                                                   (False, True, 3)])
+def test_batched_step_to_observation_matches_members(gpu_ctx, drifters, noise, factor):
+    print(f"stepToObservation batched vs per member, drifters={drifters}, noise={noise}, factor={factor}",
+          flush=True)
+    a = _ensemble(gpu_ctx, True, drifters, noise, factor)
+    b = _ensemble(gpu_ctx, False, drifters, noise, factor)
     assert len({round(float(s.dt), 3) for s in a.particles}) > 1   # members step with different dt
     # full model steps only, a leftover step first, and the short-leftover merge
     for t_obs in (300.0, 610.0, 1215.0, 1500.0):
