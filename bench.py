@@ -298,6 +298,70 @@ def run_ensemble(args):
     }
 
 
+def run_coarse_ensemble(n_members=128, factor=3, steps=50, seq_steps=4, graph_steps=10):
+    """Batched ensemble whose model error lives on a grid `factor` times
+    coarser (the double-jet data-assimilation set-up): uneven members with nx
+    a multiple of 16 * factor (periodic in x), the step rate, the model-error
+    part alone, and the per-member path for comparison."""
+    import ctypes
+    import numpy as np
+    import torch
+    import cases
+    from gpuocean_amd import BatchedEnsemble, CDKLM16, Common
+    from gpuocean_amd import _lib as L
+
+    ctx = Common.DeviceContext(0)
+    q = 16 * factor
+    sims = []
+    for i, (nx, ny) in enumerate(cases.c2_member_sizes(n_members, seed=77)):
+        nx = q * max(3, int(round(nx / q)))
+        sim = CDKLM16(ctx, **cases.config_c2_member(nx, ny, 5000 + i), local_particle_id=i)
+        sim.setSOARModelError(5e-3, small_scale_perturbation_interpolation_factor=factor, xorwow_seed=900 + i)
+        sims.append(sim)
+    cells = sum(int(s.nx) * int(s.ny) for s in sims)
+    ens = BatchedEnsemble(sims)
+    ens.step(graph_steps, graph_steps=graph_steps)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ens.step(steps, graph_steps=graph_steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    value = cells * steps / wall
+    st = ens.stream
+    sh = ctypes.c_void_p(st.cuda_stream)
+    ens._sync_in()
+    n_prof = 20
+    ev0, ev1 = event_pair(st)
+    ev0.record(st)
+    for _ in range(n_prof):
+        L.check(L.lib().gpo_ensemble_part(ens.handle, 4, sh), "gpo_ensemble_part")
+    ev1.record(st)
+    st.synchronize()
+    noise_us = 1e3 * ev0.elapsed_time(ev1) / n_prof
+    for s in sims:
+        s.model_error.draw += n_prof
+    ens._draws = [d + n_prof for d in ens._draws]
+    ens.destroy()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in sims:
+        for _ in range(seq_steps):
+            s.step(np.float32(s.dt))
+    torch.cuda.synchronize()
+    value_seq = cells * seq_steps / (time.perf_counter() - t0)
+    log(f"[coarse ensemble] {len(sims)} members, {cells / 1e6:.2f} M cells, model error {factor}x coarser: "
+        f"{1e3 * wall / steps:.3f} ms/step, {value:.4g} member-cell-steps/s (model error {noise_us:.1f} us); "
+        f"per member {value_seq:.4g}, {value / value_seq:.1f}x")
+    for s in sims:
+        s.cleanUp()
+    return {"value": value, "unit": "member-cell-steps/s", "members": len(sims), "interior_cells": cells,
+            "interpolation_factor": factor, "steps": steps, "ms_per_step": 1e3 * wall / steps,
+            "model_error_us": noise_us, "per_member_value": value_seq, "speedup_batched": value / value_seq,
+            "workload": f"batched stochastic ensemble, {len(sims)} uneven members (nx multiples of {q}), "
+                        f"SOAR q0=5e-3 on a grid {factor}x coarser, bicubic interpolation, hipGraph of "
+                        f"{graph_steps} steps"}
+
+
 def run_coarse_noise(n=2048, factor=3, reps=100):
     """Model error generated on a grid `factor` times coarser (2048 x 2048,
     walls): the bicubic interpolation + balance kernel timed alone with HIP
@@ -686,6 +750,7 @@ def main():
             big.pop("config", None)
             result["single_member_8192x4096"] = big
             result["coarse_model_error"] = run_coarse_noise()
+            result["coarse_model_error_ensemble"] = run_coarse_ensemble()
         if not args.no_adjoint:
             result["adjoint_c3"] = run_adjoint()
         result["cpu_baseline"] = None

@@ -203,13 +203,12 @@ __global__ void __launch_bounds__(256) soar_batched(const NoiseMember* __restric
 //    block (gpo_bicubic_interpolation's tiles and arithmetic).
 __global__ void __launch_bounds__(256) bicubic_batched(const NoiseMember* __restrict__ members,
                                                        const int2* __restrict__ tiles) {
-  __shared__ float coef[kBiMaxCoarse][17];
-  __shared__ float d_eta[kBiH + 2][kBiW + 2];
+  __shared__ BicubicLds lds;
   const int2 t = tiles[blockIdx.x];
   const NoiseMember& m = members[t.x];
   bicubic_tile(m.geo, m.g, m.f, m.beta, m.north_x, m.north_y, m.f == 0.0f ? m.f_cells : nullptr, m.f_pitch,
                m.coarse, m.c_pitch, m.eta, m.hu, m.hv, m.pitch, m.Hi, m.hi_pitch, m.land,
-               2 + kBiW * (t.y & 0xFFFF), 2 + kBiH * (t.y >> 16), coef, d_eta);
+               2 + kBiW * (t.y & 0xFFFF), 2 + kBiH * (t.y >> 16), lds);
 }
 
 // Every member's draw index (or every selected one's) advances once the

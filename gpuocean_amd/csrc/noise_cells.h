@@ -121,19 +121,22 @@ struct CoarseSpot {
   int ci, cj;
   float rx, ry;
 };
+// One axis of coarse_spot: fine index t (ghost-inclusive) with offset o ->
+// coarse index c and relative position r in [0, 1] within it.
+__device__ __forceinline__ void coarse_axis(int cn, float d, float cd, int o, int t, int* c, float* r) {
+  const float x = ((float)(t - 2) + 0.5f + (float)o) * d;
+  int ci = (int)floorf(x / cd + 2.0f - 0.5f);
+  if (ci > cn + 1) ci -= 1;  // first ghost cell right/top of the domain: rel = 1
+  const float cx = ((float)(ci - 2) + 0.5f) * cd;
+  *c = ci;
+  *r = (x - cx) / cd;
+}
+
 __device__ __forceinline__ CoarseSpot coarse_spot(int cnx, int cny, float dx, float dy, float cdx, float cdy, int oi,
                                                   int oj, int tj, int ti) {
-  const float x = ((float)(ti - 2) + 0.5f + (float)oi) * dx;
-  const float y = ((float)(tj - 2) + 0.5f + (float)oj) * dy;
   CoarseSpot p;
-  p.ci = (int)floorf(x / cdx + 2.0f - 0.5f);
-  p.cj = (int)floorf(y / cdy + 2.0f - 0.5f);
-  if (p.ci > cnx + 1) p.ci -= 1;  // first ghost cell right/top of the domain: rel = 1
-  if (p.cj > cny + 1) p.cj -= 1;
-  const float cx = ((float)(p.ci - 2) + 0.5f) * cdx;
-  const float cy = ((float)(p.cj - 2) + 0.5f) * cdy;
-  p.rx = (x - cx) / cdx;
-  p.ry = (y - cy) / cdy;
+  coarse_axis(cnx, dx, cdx, oi, ti, &p.ci, &p.rx);
+  coarse_axis(cny, dy, cdy, oj, tj, &p.cj, &p.ry);
   return p;
 }
 
@@ -219,61 +222,103 @@ __device__ __forceinline__ void bicubic_balance_apply(float dx, float dy, float 
 }
 
 constexpr int kBiW = 64, kBiH = 16;
-constexpr int kBiMaxCoarse = 256;  // coarse cells per tile at factor >= 3: (66/3+2) x (18/3+2) = 192
+// A tile and its ring (66 x 18 fine cells) span at most 66/3 + 1 = 23 coarse
+// columns and 18/3 + 1 = 7 coarse rows at factor 3 (fewer at 5 and 7).
+constexpr int kBiMaxCoarse = 168;
+constexpr int kBiMaxCw = 24;
 struct BicubicGeo {
   int nx, ny, cnx, cny, oi, oj;
   float dx, dy, cdx, cdy;
 };
+// LDS of one bicubic tile: coefficients per coarse cell (dead once the row
+// sums are formed, so the interpolated field reuses their space), the row
+// sums sum_j a_ij ry^j per (tile row, coarse column), and the coarse index
+// and relative position of every tile column and row.
+struct BicubicLds {
+  union {
+    float coef[kBiMaxCoarse][17];  // +1 against bank conflicts
+    float d_eta[kBiH + 2][kBiW + 2];
+  };
+  float tv[kBiH + 2][kBiMaxCw][4];
+  float rx[kBiW + 2], ry[kBiH + 2];
+  int ci[kBiW + 2], cj[kBiH + 2];
+};
 // Coarse-grid model error of one tile of kBiH x kBiW interior cells whose
 // first cell is (by, bx): bicubic interpolation of the coarse field to the
-// tile and its one-cell ring (coefficients formed once per coarse cell, in
-// LDS, when the tile spans few enough coarse cells) and its geostrophic
-// balance added to the state.  One 256-thread block; coef and d_eta are its
-// LDS arrays.
+// tile and its one-cell ring, and its geostrophic balance added to the state.
+// A fine point's coarse column and x position depend on its column only, its
+// coarse row and y position on its row only; so when the tile spans few
+// enough coarse cells, the coefficients are formed once per coarse cell, the
+// row sums t_i = sum_j a_ij ry^j once per (row, coarse column), and each point
+// takes sum_i rx^i t_i: bicubic_eval's operations in bicubic_eval's order.
+// One 256-thread block; L is its LDS.
 __device__ __forceinline__ void bicubic_tile(const BicubicGeo& q, float g, float f, float beta, float north_x,
                                              float north_y, const float* f_cells, int f_pitch, const float* coarse,
                                              int c_pitch, float* eta, float* hu, float* hv, int pitch,
                                              const float* Hi, int h_pitch, float land_value, const int bx,
-                                             const int by, float (*coef)[17], float (*d_eta)[kBiW + 2]) {
+                                             const int by, BicubicLds& L) {
   const int tid = threadIdx.x;
   // the coarse cells under the tile and its ring: the spot of a fine point
   // never decreases with its index, so the corners bound them
   const CoarseSpot lo = coarse_spot(q.cnx, q.cny, q.dx, q.dy, q.cdx, q.cdy, q.oi, q.oj, by - 1, bx - 1);
   const CoarseSpot hi = coarse_spot(q.cnx, q.cny, q.dx, q.dy, q.cdx, q.cdy, q.oi, q.oj, by + kBiH, bx + kBiW);
   const int ncw = hi.ci - lo.ci + 1, nch = hi.cj - lo.cj + 1;
-  const bool shared = ncw * nch <= kBiMaxCoarse;  // false only for factors below 3
+  const bool shared = ncw * nch <= kBiMaxCoarse && ncw <= kBiMaxCw;  // false only for factors below 3
   if (shared) {
     for (int k = tid; k < ncw * nch; k += 256) {
       float a[16];
       bicubic_coeffs(q.cnx, q.cny, coarse, c_pitch, lo.cj + k / ncw, lo.ci + k % ncw, a);
 #pragma unroll
-      for (int i = 0; i < 16; ++i) coef[k][i] = a[i];
+      for (int i = 0; i < 16; ++i) L.coef[k][i] = a[i];
     }
-  }
-  __syncthreads();
-  for (int k = tid; k < (kBiH + 2) * (kBiW + 2); k += 256) {
-    const int j = k / (kBiW + 2), i = k % (kBiW + 2);
-    const CoarseSpot p = coarse_spot(q.cnx, q.cny, q.dx, q.dy, q.cdx, q.cdy, q.oi, q.oj, by + j - 1, bx + i - 1);
-    float a[16];
-    if (shared) {
-      const int c = (p.cj - lo.cj) * ncw + (p.ci - lo.ci);
+    for (int k = tid; k < (kBiW + 2) + (kBiH + 2); k += 256) {
+      int c;
+      float r;
+      if (k < kBiW + 2) {
+        coarse_axis(q.cnx, q.dx, q.cdx, q.oi, bx + k - 1, &c, &r);
+        L.ci[k] = c - lo.ci;
+        L.rx[k] = r;
+      } else {
+        coarse_axis(q.cny, q.dy, q.cdy, q.oj, by + (k - (kBiW + 2)) - 1, &c, &r);
+        L.cj[k - (kBiW + 2)] = c - lo.cj;
+        L.ry[k - (kBiW + 2)] = r;
+      }
+    }
+    __syncthreads();
+    for (int k = tid; k < (kBiH + 2) * ncw; k += 256) {
+      const int j = k / ncw, c = k - j * ncw;
+      const float* a = L.coef[L.cj[j] * ncw + c];
+      const float ry = L.ry[j], ry2 = ry * ry;
+      const float yv[4] = {1.0f, ry, ry2, ry2 * ry};
 #pragma unroll
-      for (int t = 0; t < 16; ++t) a[t] = coef[c][t];
-    } else {
-      bicubic_coeffs(q.cnx, q.cny, coarse, c_pitch, p.cj, p.ci, a);
+      for (int i = 0; i < 4; ++i)
+        L.tv[j][c][i] = dot4(a[4 * i], a[4 * i + 1], a[4 * i + 2], a[4 * i + 3], yv[0], yv[1], yv[2], yv[3]);
     }
-    d_eta[j][i] = bicubic_eval(a, p.rx, p.ry);
+    __syncthreads();
+    for (int k = tid; k < (kBiH + 2) * (kBiW + 2); k += 256) {
+      const int j = k / (kBiW + 2), i = k % (kBiW + 2);
+      const float rx = L.rx[i], rx2 = rx * rx;
+      const float* t = L.tv[j][L.ci[i]];
+      L.d_eta[j][i] = dot4(1.0f, rx, rx2, rx2 * rx, t[0], t[1], t[2], t[3]);
+    }
+  } else {
+    for (int k = tid; k < (kBiH + 2) * (kBiW + 2); k += 256) {
+      const int j = k / (kBiW + 2), i = k % (kBiW + 2);
+      const CoarseSpot p = coarse_spot(q.cnx, q.cny, q.dx, q.dy, q.cdx, q.cdy, q.oi, q.oj, by + j - 1, bx + i - 1);
+      float a[16];
+      bicubic_coeffs(q.cnx, q.cny, coarse, c_pitch, p.cj, p.ci, a);
+      L.d_eta[j][i] = bicubic_eval(a, p.rx, p.ry);
+    }
   }
   __syncthreads();
   const int x = (tid & 63) + 1;
   for (int y = (tid >> 6) + 1; y <= kBiH; y += 4) {
     const int ti = bx + x - 1, tj = by + y - 1;
     if (ti >= q.nx + 2 || tj >= q.ny + 2) continue;
-    bicubic_balance_apply(q.dx, q.dy, g, f, beta, north_x, north_y, f_cells, f_pitch, d_eta[y][x], d_eta[y][x - 1],
-                          d_eta[y][x + 1], d_eta[y - 1][x], d_eta[y + 1][x], eta, hu, hv, pitch, Hi, h_pitch,
-                          land_value, tj, ti);
+    bicubic_balance_apply(q.dx, q.dy, g, f, beta, north_x, north_y, f_cells, f_pitch, L.d_eta[y][x],
+                          L.d_eta[y][x - 1], L.d_eta[y][x + 1], L.d_eta[y - 1][x], L.d_eta[y + 1][x], eta, hu, hv,
+                          pitch, Hi, h_pitch, land_value, tj, ti);
   }
 }
-
 
 }  // namespace gpo

@@ -88,10 +88,9 @@ __global__ __launch_bounds__(256) void bicubic_balance(BicubicGeo q, float g, fl
                                                        const float* coarse, int c_pitch, float* eta, float* hu,
                                                        float* hv, int pitch, const float* Hi, int h_pitch,
                                                        float land_value) {
-  __shared__ float coef[kBiMaxCoarse][17];
-  __shared__ float d_eta[kBiH + 2][kBiW + 2];
+  __shared__ BicubicLds lds;
   bicubic_tile(q, g, f, beta, north_x, north_y, f_cells, f_pitch, coarse, c_pitch, eta, hu, hv, pitch, Hi, h_pitch,
-               land_value, blockIdx.x * kBiW + 2, blockIdx.y * kBiH + 2, coef, d_eta);
+               land_value, blockIdx.x * kBiW + 2, blockIdx.y * kBiH + 2, lds);
 }
 
 __global__ void drift(int nx, int ny, float dx, float dy, float dt, int x0ref, int y0ref, const float* eta,
