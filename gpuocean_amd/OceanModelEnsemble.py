@@ -22,6 +22,7 @@ import logging
 import numpy as np
 import torch
 
+from . import Common
 from . import _lib as L
 from .CDKLM16 import CDKLM16
 from .GPUDrifterCollection import GPUDrifterCollection
@@ -76,8 +77,6 @@ class BatchedEnsemble:
                 return False, "RK order other than 2"
             if any(int(c) not in (1, 2, 3) for c in (bc.north, bc.east, bc.south, bc.west)):
                 return False, "boundaries other than wall, periodic and flow relaxation"
-            if s._needs_explicit_bc_each_substep() and len(s.bc_data.t) != 1:
-                return False, "flow-relaxation data that changes in time"
             if s._has_wind or s._has_pressure:
                 return False, "wind or pressure forcing"
             if s.write_netcdf:
@@ -146,6 +145,18 @@ class BatchedEnsemble:
                 m.frs_ew_ny = int(s.bc_data.east.h[0].shape[0])
                 # boundary data of one time: the weight is 1 after the first step (which the per-member path takes)
                 m.frs_t = 1.0
+                if len(s.bc_data.t) > 1:
+                    # several times: every level on the device, the weight derived there from the member's time
+                    n_t = len(s.bc_data.t)
+                    times = torch.tensor(np.asarray(s.bc_data.t, dtype=np.float64), device=s.device)
+                    ns = torch.from_numpy(np.stack([Common.pack_bc_data_ns(s.bc_data, k) for k in range(n_t)])
+                                          .astype(np.float32)).to(s.device)
+                    ew = torch.from_numpy(np.stack([Common.pack_bc_data_ew(s.bc_data, k) for k in range(n_t)])
+                                          .astype(np.float32)).to(s.device)
+                    self._frs_levels.append((times, ns, ew))
+                    m.frs_n_times = n_t
+                    m.frs_times, m.frs_ns_levels, m.frs_ew_levels = times.data_ptr(), ns.data_ptr(), ew.data_ptr()
+            m.t = float(s.t)
             if s.hasDrifters:
                 d = s.drifters
                 m.drifters = 1
@@ -162,10 +173,10 @@ class BatchedEnsemble:
         return s._needs_explicit_bc_each_substep() or s.hasDrifters
 
     def _frs_first_step_pending(self):
-        """Whether a flow-relaxation member still has to take the step whose
-        boundary-data weight is not 1 (the first step of the data's time)."""
+        """Whether a flow-relaxation member with boundary data of one time
+        still has to take the step whose weight is not 1 (the first step)."""
         for s in self.sims:
-            if s._needs_explicit_bc_each_substep():
+            if s._needs_explicit_bc_each_substep() and len(s.bc_data.t) == 1:
                 s._update_bc_values()
                 if s._bc_times.bc_t != np.float32(1.0):
                     return True
@@ -175,12 +186,15 @@ class BatchedEnsemble:
         self.destroy()
         h = ctypes.c_void_p()
         self._sync_in()
+        self._frs_levels = []   # device copies of time-varying boundary data, alive with the ensemble
         L.check(L.lib().gpo_ensemble_create(self._member_table(), self.n, self.chunk_rows, self.waves_per_simd,
                                             int(self.exact), ctypes.byref(h), self._sh()), "gpo_ensemble_create")
         self.handle = h
         self._graph_steps = 0
         self._dts = [np.float32(s.dt) for s in self.sims]
         self._draws = [s.model_error.draw if self.has_noise else 0 for s in self.sims]
+        self._times = [s.t for s in self.sims]
+        self._has_clocks = bool(self._frs_levels)
 
     def _sh(self):
         return ctypes.c_void_p(self.stream.cuda_stream)
@@ -209,6 +223,12 @@ class BatchedEnsemble:
             dr_arr = (ctypes.c_uint32 * self.n)(*draws) if new_draw else None
             L.check(L.lib().gpo_ensemble_update(self.handle, dt_arr, dr_arr, self._sh()), "gpo_ensemble_update")
             self._dts, self._draws = dts, draws
+        if getattr(self, "_has_clocks", False):
+            times = [s.t for s in self.sims]
+            if times != self._times:   # members stepped (or were reset) outside the batched path
+                arr = (ctypes.c_double * self.n)(*[float(t) for t in times])
+                L.check(L.lib().gpo_ensemble_set_times(self.handle, arr, self._sh()), "gpo_ensemble_set_times")
+                self._times = times
 
     def info(self):
         nw, nb = ctypes.c_int32(), ctypes.c_int32()
@@ -264,6 +284,7 @@ class BatchedEnsemble:
         self._sync_out()
         for s, dt in zip(self.sims, self._dts):
             s._after_batched_steps(int(n_steps), dt, ghosts_written=self._bc_in_batch(s))
+        self._times = [s.t for s in self.sims]   # the device clocks advanced the same way
 
     def _select(self, active=None, scales=None):
         a = None if active is None else (ctypes.c_uint8 * self.n)(*[1 if x else 0 for x in active])

@@ -59,6 +59,8 @@ class Member(ctypes.Structure):
         ("sensitivity", c_float), ("Bm", c_void_p), ("Bm_pitch", c_int32),
         ("coarse_nx", c_int32), ("coarse_ny", c_int32), ("coarse_dx", c_float), ("coarse_dy", c_float),
         ("coarse_offset_i", c_int32), ("coarse_offset_j", c_int32),
+        ("frs_n_times", c_int32), ("frs_times", c_void_p), ("frs_ns_levels", c_void_p), ("frs_ew_levels", c_void_p),
+        ("t", ctypes.c_double),
     ]
 
 
@@ -144,6 +146,7 @@ _SIGNATURES = [
     ("gpo_ensemble_create", c_int32,
      [ctypes.POINTER(Member), c_int32, c_int32, c_int32, c_int32, ctypes.POINTER(c_void_p), c_void_p]),
     ("gpo_ensemble_update", c_int32, [c_void_p, c_fp, ctypes.POINTER(c_uint32), c_void_p]),
+    ("gpo_ensemble_set_times", c_int32, [c_void_p, ctypes.POINTER(ctypes.c_double), c_void_p]),
     ("gpo_ensemble_info", c_int32, [c_void_p, ctypes.POINTER(c_int32), ctypes.POINTER(c_int32)]),
     ("gpo_ensemble_step", c_int32, [c_void_p, c_int32, c_void_p]),
     ("gpo_ensemble_part", c_int32, [c_void_p, c_int32, c_void_p]),

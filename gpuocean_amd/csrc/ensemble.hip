@@ -218,11 +218,51 @@ __global__ void advance_draws(uint32_t* draw, int n, const uint8_t* active) {
   if (i < n && (!active || active[i])) draw[i] = draw[i] + 1u;
 }
 
+// Flow-relaxation data of several times for one member: every level, the
+// member's model time, and the two levels and weight of the current step
+// (written by frs_clock at the start of each step).
+struct FrsClock {
+  double t;
+  const double* times;
+  const float* ns_levels;
+  const float* ew_levels;
+  long long ns_stride, ew_stride;  // floats per level
+  const float *ns_cur, *ns_next, *ew_cur, *ew_next;
+  float w;
+  int n_times, member;
+};
+
+// BoundaryConditionsArakawaA.update_bc_values (utils/Common.py:1193-1308) at
+// the member's time t, in FP64 as the host evaluates it: the levels around t
+// (numpy.searchsorted's left insertion point k gives k-1 and k, clamped) and
+// the weight clamp((t - t0) / max(1e-10, t1 - t0), 0, 1) rounded to FP32;
+// then t advances by the member's dt, as CDKLM16.step accumulates it.
+__global__ void frs_clock(FrsClock* __restrict__ clocks, int n, const StepArgs* __restrict__ args,
+                          const uint8_t* __restrict__ active) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  FrsClock& c = clocks[i];
+  if (active && !active[c.member]) return;
+  const double t = c.t;
+  int k = 0;
+  while (k < c.n_times && c.times[k] < t) ++k;
+  const int i0 = k > 0 ? k - 1 : 0, i1 = k < c.n_times - 1 ? k : c.n_times - 1;
+  const double elapsed = t - c.times[i0];
+  const double interval = fmax(1.0e-10, c.times[i1] - c.times[i0]);
+  c.w = (float)fmax(0.0, fmin(1.0, elapsed / interval));
+  c.ns_cur = c.ns_levels ? c.ns_levels + i0 * c.ns_stride : nullptr;
+  c.ns_next = c.ns_levels ? c.ns_levels + i1 * c.ns_stride : nullptr;
+  c.ew_cur = c.ew_levels ? c.ew_levels + i0 * c.ew_stride : nullptr;
+  c.ew_next = c.ew_levels ? c.ew_levels + i1 * c.ew_stride : nullptr;
+  c.t = t + (double)args[2 * c.member].dt;
+}
+
 // Ghost cells of one member, as gpo_boundary_conditions writes them, on its
 // state (h0) and RK stage (h1).
 struct BcMember {
   cells::BcTarget h0, h1;
   int frs;  // some side is flow relaxation: ghost cells after substep 0 too
+  const FrsClock* clock;  // flow-relaxation levels and weight of this step, or NULL (fixed data)
 };
 
 // pass 0: north/south, pass 1: east/west; target 0: h0, 1: h1 (members with
@@ -230,6 +270,14 @@ struct BcMember {
 __global__ void __launch_bounds__(256) bc_batched(const BcMember* __restrict__ members, int pass, int target) {
   const BcMember& b = members[blockIdx.y];
   if (target == 1 && !b.frs) return;
+  if (b.clock) {
+    cells::BcTarget t = target ? b.h1 : b.h0;
+    const FrsClock& c = *b.clock;
+    t.ns_cur = c.ns_cur, t.ns_next = c.ns_next, t.ew_cur = c.ew_cur, t.ew_next = c.ew_next;
+    t.frs_t = c.w;
+    cells::bc_target_cell(t, pass, blockIdx.x * blockDim.x + threadIdx.x);
+    return;
+  }
   cells::bc_target_cell(target ? b.h1 : b.h0, pass, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
@@ -276,6 +324,9 @@ struct Ensemble {
   // model error on coarser grids: the largest normal field (in Philox quads)
   // and coarse buffer over the members, the x extents of the first two launches
   int coarse_noise = 0, max_xi_quads = 0, max_coarse_cells = 0;
+  // flow-relaxation clocks (members whose boundary data has several times)
+  std::vector<FrsClock> h_clock;
+  FrsClock* d_clock = nullptr;
   StepArgs* d_args = nullptr;
   int2* d_tiles = nullptr;
   int2* d_work = nullptr;
@@ -376,7 +427,12 @@ static int ensemble_launch_drift(Ensemble* e, hipStream_t s) {
 static int ensemble_launch_part(Ensemble* e, int part, hipStream_t s) {
   int rc = 0;
   if (part == 0) {
-    rc = ensemble_launch_substep(e, 0, s);
+    if (!e->h_clock.empty()) {
+      const int nc = (int)e->h_clock.size();
+      hipLaunchKernelGGL(frs_clock, dim3((nc + 63) / 64), dim3(64), 0, s, e->d_clock, nc, e->d_args, e->cur_active);
+      rc = check(hipGetLastError(), "ensemble boundary-data clock launch");
+    }
+    if (!rc) rc = ensemble_launch_substep(e, 0, s);
     if (!rc) rc = ensemble_launch_bc(e, 1, s);
   } else if (part == 1) {
     rc = ensemble_launch_substep(e, 1, s);
@@ -412,6 +468,7 @@ static void ensemble_free(Ensemble* e) {
   if (e->d_drift_sel) (void)hipFree(e->d_drift_sel);
   if (e->d_active) (void)hipFree(e->d_active);
   if (e->d_arena) (void)hipFree(e->d_arena);
+  if (e->d_clock) (void)hipFree(e->d_clock);
   delete e;
 }
 
@@ -520,6 +577,26 @@ int gpo_ensemble_create(const gpo_member* members, int32_t n, int32_t chunk_rows
       t.e = M.h0_eta, t.u = M.h0_hu, t.v = M.h0_hv;
       b.h1.e = M.h1_eta, b.h1.u = M.h1_hu, b.h1.v = M.h1_hv;
       b.frs = frs;
+      b.clock = nullptr;
+      if (frs && M.frs_n_times > 1) {
+        if (!M.frs_times || ((t.bn == 3 || t.bs == 3) && t.bn != 2 && !M.frs_ns_levels) ||
+            ((t.be == 3 || t.bw == 3) && t.be != 2 && !M.frs_ew_levels)) {
+          ensemble_free(e);
+          return bad("gpo_ensemble_create: flow-relaxation time levels missing");
+        }
+        FrsClock c{};
+        c.t = M.t;
+        c.times = M.frs_times;
+        c.n_times = M.frs_n_times;
+        c.ns_levels = M.frs_ns_levels;
+        c.ew_levels = M.frs_ew_levels;
+        c.ns_stride = 2ll * M.frs_ns_nx * 3;
+        c.ew_stride = 2ll * M.frs_ew_ny * 3;
+        c.member = m;
+        // the device address is known once the array exists: store the index + 1 for now
+        b.clock = (const FrsClock*)(uintptr_t)(e->h_clock.size() + 1);
+        e->h_clock.push_back(c);
+      }
       bcs.push_back(b);
       e->bc_member.push_back(m);
       max_bc_cells = std::max(max_bc_cells, cells::bc_target_cells(t));
@@ -662,6 +739,16 @@ int gpo_ensemble_create(const gpo_member* members, int32_t n, int32_t chunk_rows
   e->bc_blocks = (int)((max_bc_cells + 255) / 256);
   e->n_drift = (int)drifts.size();
   e->drift_blocks = (max_drift_rows + 63) / 64;
+  if (!rc && !e->h_clock.empty()) {
+    rc = check(hipMalloc(&e->d_clock, sizeof(FrsClock) * e->h_clock.size()), "gpo_ensemble_create hipMalloc");
+    if (!rc)
+      rc = check(hipMemcpyAsync(e->d_clock, e->h_clock.data(), sizeof(FrsClock) * e->h_clock.size(),
+                                hipMemcpyHostToDevice, s),
+                 "gpo_ensemble_create copy");
+    // the members' clock indices (+1) become device addresses
+    for (BcMember& b : bcs)
+      if (b.clock) b.clock = e->d_clock + ((uintptr_t)b.clock - 1);
+  }
   if (!rc && e->n_bc > 0 && e->bc_blocks > 0) {
     rc = check(hipMalloc(&e->d_bc, sizeof(BcMember) * bcs.size()), "gpo_ensemble_create hipMalloc");
     if (!rc)
@@ -793,6 +880,20 @@ int gpo_ensemble_update(gpo_ensemble h, const float* dts, const uint32_t* draws,
                "gpo_ensemble_update copy");
   // The host sources may change as soon as this returns.
   if (!rc) rc = check(hipStreamSynchronize(s), "gpo_ensemble_update sync");
+  return rc;
+}
+
+int gpo_ensemble_set_times(gpo_ensemble h, const double* t, void* stream) {
+  Ensemble* e = (Ensemble*)h;
+  if (!e || !t) return bad("gpo_ensemble_set_times: NULL argument");
+  if (e->h_clock.empty()) return 0;
+  hipStream_t s = (hipStream_t)stream;
+  // the derived fields are rewritten by the next step's clock launch
+  for (FrsClock& c : e->h_clock) c.t = t[c.member];
+  int rc = check(hipMemcpyAsync(e->d_clock, e->h_clock.data(), sizeof(FrsClock) * e->h_clock.size(),
+                                hipMemcpyHostToDevice, s),
+                 "gpo_ensemble_set_times copy");
+  if (!rc) rc = check(hipStreamSynchronize(s), "gpo_ensemble_set_times sync");
   return rc;
 }
 

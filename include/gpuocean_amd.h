@@ -319,6 +319,17 @@ typedef struct {
   int32_t coarse_nx, coarse_ny;
   float coarse_dx, coarse_dy;
   int32_t coarse_offset_i, coarse_offset_j;
+  /* flow-relaxation data of several times (frs_n_times > 1): every level on
+   * the device; each step takes its two levels and weight from the member's
+   * model time t as BoundaryConditionsArakawaA.update_bc_values does
+   * (utils/Common.py:1193-1308, in FP64), and t advances by the member's dt.
+   * frs_*_cur/next and frs_t are then not used.  frs_n_times <= 1: the data
+   * of frs_*_cur/next with weight frs_t. */
+  int32_t frs_n_times;
+  const double* frs_times;         /* frs_n_times ascending times */
+  const float* frs_ns_levels;      /* frs_n_times x [2][frs_ns_nx][3], or NULL */
+  const float* frs_ew_levels;      /* frs_n_times x [frs_ew_ny][2][3], or NULL */
+  double t;                        /* the member's model time */
 } gpo_member;
 
 typedef void* gpo_ensemble;
@@ -348,6 +359,12 @@ int gpo_ensemble_create(const gpo_member* members, int32_t n, int32_t chunk_rows
  * uint32) of every member; either may be NULL.  Synchronises `stream`.  A
  * captured graph stays valid (it reads both tables at run time). */
 int gpo_ensemble_update(gpo_ensemble e, const float* dts, const uint32_t* draws, void* stream);
+
+/* Model times (n host doubles, one per member) of the members whose
+ * flow-relaxation data has several times, as they stand before the next
+ * step; the others' entries are ignored.  No reference counterpart: the
+ * reference keeps each member's time on the host.  Synchronises `stream`. */
+int gpo_ensemble_set_times(gpo_ensemble e, const double* t, void* stream);
 
 /* Number of work items and persistent workgroups of an ensemble. */
 int gpo_ensemble_info(gpo_ensemble e, int32_t* n_work, int32_t* n_blocks);

@@ -67,16 +67,62 @@ def test_batched_coastal_matches_per_member(gpu_ctx, codes, noise, n_drifters):
     _assert_same(batched, plain)
 
 
-def test_time_varying_frs_data_is_not_batched(gpu_ctx):
-    kw = cases.config_c4_member(64, 48, 3)
+def _time_varying(kw, factors=(1.0, 1.15, 0.9, 1.25), times=(0.0, 12.0, 30.0, 45.0)):
+    """The member's boundary data at four times: its jet scaled per level."""
     d = kw["boundary_conditions_data"]
     for side in (d.north, d.south, d.east, d.west):
-        side.h, side.hu, side.hv = side.h * 2, side.hu * 2, side.hv * 2
-        side.numSteps = 2
-    d.t, d.numSteps = [0.0, 100.0], 2
-    sim = CDKLM16(gpu_ctx, **kw)
-    ok, why = BatchedEnsemble.eligible([sim])
-    assert not ok and "changes in time" in why
+        side.h = [np.float32(f) * side.h[0] for f in factors]
+        side.hu = [np.float32(f) * side.hu[0] for f in factors]
+        side.hv = [np.float32(f) * side.hv[0] for f in factors]
+        side.numSteps = len(factors)
+    d.t, d.numSteps = list(times), len(times)
+    return kw
+
+
+def _frs_members(gpu_ctx, codes):
+    sims = []
+    for i, (nx, ny) in enumerate(SIZES):
+        kw = cases.config_c4_member(nx, ny, 40 + i, codes=codes)
+        if i % 2 == 0:
+            kw = _time_varying(kw)
+        sims.append(CDKLM16(gpu_ctx, **kw, local_particle_id=i))
+    return sims
+
+
+@pytest.mark.parametrize("codes", [(3, 3, 3, 1), (3, 2, 3, 2), (1, 3, 1, 3)])
+def test_time_varying_frs_data_batched(gpu_ctx, codes):
+    """Flow-relaxation data of several times (members 0, 2, 4; the others
+    hold one time): the batched step derives each step's two levels and weight
+    on the device from the member's time and gives each member's own values
+    bit for bit, across level changes, through graph replays and after
+    members took steps of their own."""
+    print(f"time-varying flow relaxation, codes={codes}", flush=True)
+    plain = _frs_members(gpu_ctx, codes)
+    batched = _frs_members(gpu_ctx, codes)
+    ok, why = BatchedEnsemble.eligible(batched)
+    assert ok, why
+    ens = BatchedEnsemble(batched)
+    for s in plain:
+        for _ in range(8):
+            s.step(np.float32(s.dt))
+    ens.step(8)
+    _assert_same(batched, plain)
+    for s in plain:
+        for _ in range(8):
+            s.step(np.float32(s.dt))
+    ens.step(8, graph_steps=4)
+    _assert_same(batched, plain)
+    assert min(s.t for s in batched) > 45.0   # every level change crossed, and past the last level
+    # a step of each member's own moves its time outside the batched path
+    for a, b in zip(batched, plain):
+        a.step(np.float32(a.dt))
+        b.step(np.float32(b.dt))
+    for s in plain:
+        for _ in range(5):
+            s.step(np.float32(s.dt))
+    ens.step(5)
+    _assert_same(batched, plain)
+    ens.destroy()
 
 
 def test_ocean_model_ensemble_coastal_batched_equals_sequential(gpu_ctx):
