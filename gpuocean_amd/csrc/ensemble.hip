@@ -429,7 +429,7 @@ static int ensemble_launch_part(Ensemble* e, int part, hipStream_t s) {
   if (part == 0) {
     if (!e->h_clock.empty()) {
       const int nc = (int)e->h_clock.size();
-      hipLaunchKernelGGL(frs_clock, dim3((nc + 63) / 64), dim3(64), 0, s, e->d_clock, nc, e->d_args, e->cur_active);
+      hipLaunchKernelGGL(frs_clock, dim3((nc + 63) / 64), dim3(64), 0, s, e->d_clock, nc, e->cur_args, e->cur_active);
       rc = check(hipGetLastError(), "ensemble boundary-data clock launch");
     }
     if (!rc) rc = ensemble_launch_substep(e, 0, s);
