@@ -12,10 +12,6 @@ struct U4 {
   uint32_t x, y, z, w;
 };
 
-__host__ __device__ inline uint32_t mulhi32(uint32_t a, uint32_t b) {
-  return (uint32_t)(((uint64_t)a * (uint64_t)b) >> 32);
-}
-
 // One call: counter (c0..c3), key (k0, k1), ten rounds with key schedule.
 __host__ __device__ inline U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1) {
   constexpr uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
@@ -26,8 +22,10 @@ __host__ __device__ inline U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1) {
       k0 += W0;
       k1 += W1;
     }
-    const uint32_t hi0 = mulhi32(M0, c.x), lo0 = M0 * c.x;
-    const uint32_t hi1 = mulhi32(M1, c.z), lo1 = M1 * c.z;
+    // one 32 x 32 -> 64-bit product gives both halves
+    const uint64_t p0 = (uint64_t)M0 * (uint64_t)c.x, p1 = (uint64_t)M1 * (uint64_t)c.z;
+    const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
     U4 n;
     n.x = hi1 ^ c.y ^ k0;
     n.y = lo1;
