@@ -143,19 +143,18 @@ class BatchedEnsemble:
                 m.frs_ew_cur, m.frs_ew_next = addr(f.get('ew_cur')), addr(f.get('ew_next'))
                 m.frs_ns_nx = int(s.bc_data.north.h[0].shape[0])
                 m.frs_ew_ny = int(s.bc_data.east.h[0].shape[0])
-                # boundary data of one time: the weight is 1 after the first step (which the per-member path takes)
                 m.frs_t = 1.0
-                if len(s.bc_data.t) > 1:
-                    # several times: every level on the device, the weight derived there from the member's time
-                    n_t = len(s.bc_data.t)
-                    times = torch.tensor(np.asarray(s.bc_data.t, dtype=np.float64), device=s.device)
-                    ns = torch.from_numpy(np.stack([Common.pack_bc_data_ns(s.bc_data, k) for k in range(n_t)])
-                                          .astype(np.float32)).to(s.device)
-                    ew = torch.from_numpy(np.stack([Common.pack_bc_data_ew(s.bc_data, k) for k in range(n_t)])
-                                          .astype(np.float32)).to(s.device)
-                    self._frs_levels.append((times, ns, ew))
-                    m.frs_n_times = n_t
-                    m.frs_times, m.frs_ns_levels, m.frs_ew_levels = times.data_ptr(), ns.data_ptr(), ew.data_ptr()
+                # every level on the device (one or several times); each step's
+                # levels and weight are derived there from the member's time
+                n_t = len(s.bc_data.t)
+                times = torch.tensor(np.asarray(s.bc_data.t, dtype=np.float64), device=s.device)
+                ns = torch.from_numpy(np.stack([Common.pack_bc_data_ns(s.bc_data, k) for k in range(n_t)])
+                                      .astype(np.float32)).to(s.device)
+                ew = torch.from_numpy(np.stack([Common.pack_bc_data_ew(s.bc_data, k) for k in range(n_t)])
+                                      .astype(np.float32)).to(s.device)
+                self._frs_levels.append((times, ns, ew))
+                m.frs_n_times = n_t
+                m.frs_times, m.frs_ns_levels, m.frs_ew_levels = times.data_ptr(), ns.data_ptr(), ew.data_ptr()
             m.t = float(s.t)
             if s.hasDrifters:
                 d = s.drifters
@@ -171,16 +170,6 @@ class BatchedEnsemble:
         """Whether the batched step writes this member's ghost cells every
         step (flow relaxation, or drifters that read them)."""
         return s._needs_explicit_bc_each_substep() or s.hasDrifters
-
-    def _frs_first_step_pending(self):
-        """Whether a flow-relaxation member with boundary data of one time
-        still has to take the step whose weight is not 1 (the first step)."""
-        for s in self.sims:
-            if s._needs_explicit_bc_each_substep() and len(s.bc_data.t) == 1:
-                s._update_bc_values()
-                if s._bc_times.bc_t != np.float32(1.0):
-                    return True
-        return False
 
     def _create(self):
         self.destroy()
@@ -269,14 +258,6 @@ class BatchedEnsemble:
                 if s.t == 0:
                     s._update_bc_values()
                     s._apply_bc(s.h0)
-        if self._frs_first_step_pending():
-            # the step with boundary-data weight below 1 goes through each member's own path
-            for s in self.sims:
-                s.step_dts([np.float32(s.dt)])
-            n_steps -= 1
-            self._draws = None   # the members' draw indices moved on the host
-            if n_steps <= 0:
-                return
         self._sync_in()
         self._refresh()
         self.launch(n_steps, graph_steps)

@@ -578,7 +578,7 @@ int gpo_ensemble_create(const gpo_member* members, int32_t n, int32_t chunk_rows
       b.h1.e = M.h1_eta, b.h1.u = M.h1_hu, b.h1.v = M.h1_hv;
       b.frs = frs;
       b.clock = nullptr;
-      if (frs && M.frs_n_times > 1) {
+      if (frs && M.frs_n_times >= 1) {
         if (!M.frs_times || ((t.bn == 3 || t.bs == 3) && t.bn != 2 && !M.frs_ns_levels) ||
             ((t.be == 3 || t.bw == 3) && t.be != 2 && !M.frs_ew_levels)) {
           ensemble_free(e);

@@ -319,12 +319,12 @@ typedef struct {
   int32_t coarse_nx, coarse_ny;
   float coarse_dx, coarse_dy;
   int32_t coarse_offset_i, coarse_offset_j;
-  /* flow-relaxation data of several times (frs_n_times > 1): every level on
+  /* flow-relaxation data as time levels (frs_n_times >= 1): every level on
    * the device; each step takes its two levels and weight from the member's
    * model time t as BoundaryConditionsArakawaA.update_bc_values does
    * (utils/Common.py:1193-1308, in FP64), and t advances by the member's dt.
-   * frs_*_cur/next and frs_t are then not used.  frs_n_times <= 1: the data
-   * of frs_*_cur/next with weight frs_t. */
+   * frs_*_cur/next and frs_t are then not used.  frs_n_times = 0: the data
+   * of frs_*_cur/next with the fixed weight frs_t. */
   int32_t frs_n_times;
   const double* frs_times;         /* frs_n_times ascending times */
   const float* frs_ns_levels;      /* frs_n_times x [2][frs_ns_nx][3], or NULL */
