@@ -245,6 +245,7 @@ class BatchedEnsemble:
             rest = int(n_steps)
         if rest:
             L.check(L.lib().gpo_ensemble_step(self.handle, rest, self._sh()), "gpo_ensemble_step")
+        self._times = None   # the device clocks moved on; step() records the times its bookkeeping reaches
 
     def step(self, n_steps, graph_steps=0, initial_bc=True):
         """n_steps steps of every member, each with its own dt; then the host
@@ -278,6 +279,7 @@ class BatchedEnsemble:
         for _ in range(count):
             for part in parts:
                 L.check(lib.gpo_ensemble_part(self.handle, int(part), sh), "gpo_ensemble_part")
+        self._times = None   # part 0 moves the device clocks: upload the members' times before the next steps
 
     def model_step(self, seqs, noise_scales, write_ghosts=True):
         """One model time step of each member as CDKLM16.dataAssimilationStep
@@ -340,6 +342,7 @@ class BatchedEnsemble:
         if with_noise:   # part 4: the model error with each member's scale
             L.check(lib.gpo_ensemble_run_set(self.handle, len(sets) - 1, 1, 0b10000, sh), "gpo_ensemble_run_set")
             self._scales_dirty = True
+        self._times = None   # the sets' part 0 moved any device clocks
         self._sync_out()
         for p, (s, q) in enumerate(zip(sims, seqs)):
             for dt in q:
