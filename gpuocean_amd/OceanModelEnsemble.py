@@ -212,9 +212,14 @@ class BatchedEnsemble:
             dr_arr = (ctypes.c_uint32 * self.n)(*draws) if new_draw else None
             L.check(L.lib().gpo_ensemble_update(self.handle, dt_arr, dr_arr, self._sh()), "gpo_ensemble_update")
             self._dts, self._draws = dts, draws
+        self._sync_times()
+
+    def _sync_times(self):
+        """Upload the members' model times to the device boundary-data clocks
+        if they moved outside the batched steps."""
         if getattr(self, "_has_clocks", False):
             times = [s.t for s in self.sims]
-            if times != self._times:   # members stepped (or were reset) outside the batched path
+            if times != self._times:
                 arr = (ctypes.c_double * self.n)(*[float(t) for t in times])
                 L.check(L.lib().gpo_ensemble_set_times(self.handle, arr, self._sh()), "gpo_ensemble_set_times")
                 self._times = times
@@ -305,6 +310,7 @@ class BatchedEnsemble:
         if self._dts is None or self._draws is None:
             self._dts, self._draws = None, None
             self._refresh_all()
+        self._sync_times()
         # groups of consecutive inner steps with the same members and time steps
         k_max = max(len(q) for q in seqs)
         groups = []   # [active, dts, count]
@@ -548,8 +554,6 @@ class OceanModelEnsemble:
         for p in active:
             s = self.particles[p]
             if s.t != s0.t or s.model_time_step != s0.model_time_step or s.total_time_steps != s0.total_time_steps:
-                return False
-            if s._needs_explicit_bc_each_substep():
                 return False
         return True
 

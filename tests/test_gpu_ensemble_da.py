@@ -75,3 +75,41 @@ def test_batched_step_to_observation_matches_members(gpu_ctx, drifters, noise, f
     _same(a, b, "modelStep after")
     a.cleanUp()
     b.cleanUp()
+
+
+def _coastal_ensemble(gpu_ctx, batched):
+    data = []
+    for i, (nx, ny) in enumerate([(64, 48), (96, 64), (80, 56)]):
+        kw = cases.config_c4_member(nx, ny, 70 + i)
+        if i == 1:   # boundary data at three times for one member
+            d = kw["boundary_conditions_data"]
+            for side in (d.north, d.south, d.east, d.west):
+                side.h = [np.float32(f) * side.h[0] for f in (1.0, 1.2, 0.8)]
+                side.hu = [np.float32(f) * side.hu[0] for f in (1.0, 1.2, 0.8)]
+                side.hv = [np.float32(f) * side.hv[0] for f in (1.0, 1.2, 0.8)]
+                side.numSteps = 3
+            d.t, d.numSteps = [0.0, 200.0, 500.0], 3
+        data.append({k: kw[k] for k in ("eta0", "hu0", "hv0", "H", "nx", "ny", "dx", "dy", "boundary_conditions",
+                                        "boundary_conditions_data")})
+    sim_args = {"dt": 0.0, "g": 9.81, "f": 1e-4, "r": 0.0, "courant_number": 0.7, "model_time_step": 60.0}
+    me = {"small_scale_perturbation_amplitude": 5e-3, "xorwow_seed": 55}
+    return OceanModelEnsemble(gpu_ctx, sim_args, data, me, numParticles=3, batched=batched)
+
+
+def test_batched_step_to_observation_coastal(gpu_ctx):
+    """Coastal members (land, flow relaxation, one of them with boundary data
+    at three times) through the batched data-assimilation stepping: the device
+    boundary-data clocks follow each member's own dataAssimilationStep."""
+    print("stepToObservation batched vs per member, coastal", flush=True)
+    a = _coastal_ensemble(gpu_ctx, True)
+    b = _coastal_ensemble(gpu_ctx, False)
+    calls = []
+    batched_step = a._batched_da_step
+    a._batched_da_step = lambda *args, **kw: (calls.append(1), batched_step(*args, **kw))[1]
+    for t_obs in (300.0, 610.0):
+        a.stepToObservation(t_obs)
+        b.stepToObservation(t_obs)
+        _same(a, b, f"coastal t={t_obs}")
+    assert a._ensemble is not None and len(calls) == 2   # the batched data-assimilation path ran
+    a.cleanUp()
+    b.cleanUp()
