@@ -571,6 +571,20 @@ class CDKLM16:
             self.drifter_t += dt
             return self.drifter_t
 
+    def _adopt_stream(self, stream):
+        """Run this member (and its model error and drifters) on `stream`
+        from now on, after the work already queued on its own stream.  Members
+        of a batched ensemble share the ensemble's stream, so the ensemble
+        needs no per-member stream joins around its launches."""
+        if stream is self.gpu_stream:
+            return
+        stream.wait_stream(self.gpu_stream)
+        self.gpu_stream = stream
+        if self.model_error is not None:
+            self.model_error.gpu_stream = stream
+        if self.drifters is not None and hasattr(self.drifters, "setGPUStream"):
+            self.drifters.setGPUStream(stream)
+
     def attachDrifters(self, drifters):
         self.drifters = drifters
         self.hasDrifters = True

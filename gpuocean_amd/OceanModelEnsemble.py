@@ -53,6 +53,8 @@ class BatchedEnsemble:
         self.n = len(self.sims)
         self.device = self.sims[0].device
         self.stream = torch.cuda.Stream(device=self.device)
+        for s in self.sims:   # one stream for all members: no joins around each launch
+            s._adopt_stream(self.stream)
         self.exact = bool(self.sims[0].exact_arithmetic)
         self.has_noise = self.sims[0].model_error is not None
         self.handle = None
@@ -190,11 +192,13 @@ class BatchedEnsemble:
 
     def _sync_in(self):
         for s in self.sims:
-            self.stream.wait_stream(s.gpu_stream)
+            if s.gpu_stream is not self.stream:   # a member moved to another stream since
+                self.stream.wait_stream(s.gpu_stream)
 
     def _sync_out(self):
         for s in self.sims:
-            s.gpu_stream.wait_stream(self.stream)
+            if s.gpu_stream is not self.stream:
+                s.gpu_stream.wait_stream(self.stream)
 
     def _refresh(self):
         """Push members' dt and draw index to the device tables if they changed
@@ -267,24 +271,18 @@ class BatchedEnsemble:
         self._sync_in()
         self._refresh()
         self.launch(n_steps, graph_steps)
+        # ghost cells of the members whose steps do not write them, once
+        L.check(L.lib().gpo_ensemble_part(self.handle, 5, self._sh()), "gpo_ensemble_part")
         self._draws = [d + n_steps for d in self._draws] if self.has_noise else self._draws
         self._sync_out()
         for s, dt in zip(self.sims, self._dts):
-            s._after_batched_steps(int(n_steps), dt, ghosts_written=self._bc_in_batch(s))
+            s._after_batched_steps(int(n_steps), dt, ghosts_written=True)
         self._times = [s.t for s in self.sims]   # the device clocks advanced the same way
 
     def _select(self, active=None, scales=None):
         a = None if active is None else (ctypes.c_uint8 * self.n)(*[1 if x else 0 for x in active])
         sc = None if scales is None else (ctypes.c_float * self.n)(*[float(np.float32(x)) for x in scales])
         L.check(L.lib().gpo_ensemble_select(self.handle, a, sc, self._sh()), "gpo_ensemble_select")
-
-    def _parts(self, parts, count=1):
-        sh = self._sh()
-        lib = L.lib()
-        for _ in range(count):
-            for part in parts:
-                L.check(lib.gpo_ensemble_part(self.handle, int(part), sh), "gpo_ensemble_part")
-        self._times = None   # part 0 moves the device clocks: upload the members' times before the next steps
 
     def model_step(self, seqs, noise_scales, write_ghosts=True):
         """One model time step of each member as CDKLM16.dataAssimilationStep
@@ -324,6 +322,8 @@ class BatchedEnsemble:
         noisy = [bool(q) and sc is not None for q, sc in zip(seqs, noise_scales)]
         with_noise = self.has_noise and any(noisy)
         sets = [(g[0], g[1]) for g in groups]
+        stepped_set = len(sets)   # the members that step: their ghost cells (part 5)
+        sets.append((tuple(bool(q) for q in seqs), tuple([1.0] * self.n)))
         if with_noise:
             sets.append((tuple(noisy), tuple([1.0] * self.n)))
         n = self.n
@@ -340,11 +340,7 @@ class BatchedEnsemble:
         # ghost cells of the new state where the batched parts do not write them
         # (the per-member path writes them after each step, before the model error)
         if write_ghosts:
-            self._sync_out()
-            for s, q in zip(sims, seqs):
-                if q and not self._bc_in_batch(s):
-                    s._apply_bc(s.h0)
-            self._sync_in()
+            L.check(lib.gpo_ensemble_run_set(self.handle, stepped_set, 1, 0b100000, sh), "gpo_ensemble_run_set")
         if with_noise:   # part 4: the model error with each member's scale
             L.check(lib.gpo_ensemble_run_set(self.handle, len(sets) - 1, 1, 0b10000, sh), "gpo_ensemble_run_set")
             self._scales_dirty = True
@@ -535,8 +531,8 @@ class OceanModelEnsemble:
 
     def stepToObservation(self, observation_time, write_now=False):
         """dataAssimilationStep on every active member (OceanModelEnsemble.py:121-134).
-        When the active members share their time and model time step (and
-        carry no flow relaxation), the whole cadence runs batched: the same
+        When the active members share their time, model time step and step
+        count (total_time_steps), the whole cadence runs batched: the same
         model steps, each member's own dt within them, the model error after
         each, the dt updates every fifth model step, with every member's
         results equal to its own dataAssimilationStep."""

@@ -38,7 +38,10 @@ __device__ inline float water_velocity(const float* eta, const float* mom, const
 
 // passiveDrifterKernel (gpu_kernels/driftKernels.cu:74-183) for the drifter at
 // d = (x, y): bilinear velocity between cell centres, optional wind drag,
-// forward Euler, periodic wrap.
+// forward Euler, periodic wrap.  A drifter whose four surrounding cell
+// centres are not all inside the ghost-inclusive arrays (more than the ghost
+// cells beyond an open side, or a non-finite position) is left where it is:
+// the reference reads outside its arrays there.
 __device__ inline void drift_one(int nx, int ny, float dx, float dy, float dt, int x0ref, int y0ref, const float* eta,
                                  const float* hu, const float* hv, int pitch, const float* Hm, int hm_pitch,
                                  int per_ns, int per_ew, float* d, float sensitivity, const float* wx_cur,
@@ -46,14 +49,17 @@ __device__ inline void drift_one(int nx, int ny, float dx, float dy, float dt, i
                                  float wind_t, float wind_drift_factor) {
   float px = d[0], py = d[1];
   const float qx = px / dx, qy = py / dy;
-  const int cx = (int)(floorf(qx) + (float)x0ref);
-  const int cy = (int)(floorf(qy) + (float)y0ref);
+  const float fcx = floorf(qx) + (float)x0ref, fcy = floorf(qy) + (float)y0ref;
+  if (!(fcx >= 0.0f && fcx <= (float)(nx + 3) && fcy >= 0.0f && fcy <= (float)(ny + 3))) return;  // NaN too
+  const int cx = (int)fcx;
+  const int cy = (int)fcy;
   const float frx = qx - floorf(qx), fry = qy - floorf(qy);
   const int cx0 = frx < 0.5f ? cx - 1 : cx;
   const float xf = frx < 0.5f ? frx + 0.5f : frx - 0.5f;
   const int cy0 = fry < 0.5f ? cy - 1 : cy;
   const float yf = fry < 0.5f ? fry + 0.5f : fry - 0.5f;
   const int cx1 = cx0 + 1, cy1 = cy0 + 1;
+  if (cx0 < 0 || cy0 < 0 || cx1 > nx + 3 || cy1 > ny + 3) return;
   const float u00 = water_velocity(eta, hu, Hm, pitch, hm_pitch, cx0, cy0);
   const float u10 = water_velocity(eta, hu, Hm, pitch, hm_pitch, cx1, cy0);
   const float u01 = water_velocity(eta, hu, Hm, pitch, hm_pitch, cx0, cy1);

@@ -237,12 +237,7 @@ struct FrsClock {
 // (numpy.searchsorted's left insertion point k gives k-1 and k, clamped) and
 // the weight clamp((t - t0) / max(1e-10, t1 - t0), 0, 1) rounded to FP32;
 // then t advances by the member's dt, as CDKLM16.step accumulates it.
-__global__ void frs_clock(FrsClock* __restrict__ clocks, int n, const StepArgs* __restrict__ args,
-                          const uint8_t* __restrict__ active) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  FrsClock& c = clocks[i];
-  if (active && !active[c.member]) return;
+__device__ void clock_levels(FrsClock& c) {
   const double t = c.t;
   int k = 0;
   while (k < c.n_times && c.times[k] < t) ++k;
@@ -254,7 +249,27 @@ __global__ void frs_clock(FrsClock* __restrict__ clocks, int n, const StepArgs* 
   c.ns_next = c.ns_levels ? c.ns_levels + i1 * c.ns_stride : nullptr;
   c.ew_cur = c.ew_levels ? c.ew_levels + i0 * c.ew_stride : nullptr;
   c.ew_next = c.ew_levels ? c.ew_levels + i1 * c.ew_stride : nullptr;
-  c.t = t + (double)args[2 * c.member].dt;
+}
+
+__global__ void frs_clock(FrsClock* __restrict__ clocks, int n, const StepArgs* __restrict__ args,
+                          const uint8_t* __restrict__ active) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  FrsClock& c = clocks[i];
+  if (active && !active[c.member]) return;
+  clock_levels(c);
+  c.t = c.t + (double)args[2 * c.member].dt;
+}
+
+// Puts the members' model times t[member] into their clocks (NULL t: keep the
+// times) and derives the levels and weight at those times, without advancing
+// them, so the ghost-cell passes are valid before the next step's clock launch.
+__global__ void clock_set_times(FrsClock* __restrict__ clocks, int n, const double* __restrict__ t) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  FrsClock& c = clocks[i];
+  if (t) c.t = t[c.member];
+  clock_levels(c);
 }
 
 // Ghost cells of one member, as gpo_boundary_conditions writes them, on its
@@ -334,6 +349,11 @@ struct Ensemble {
   uint32_t* d_draw = nullptr;
   BcMember* d_bc = nullptr;   // members whose ghost cells are written every step
   int n_bc = 0, bc_blocks = 0, any_frs = 0;
+  // members with a periodic side whose ghost cells the steps do not write:
+  // part 5 writes them once, after a run of steps
+  BcMember* d_bc_end = nullptr;
+  int n_bc_end = 0, bc_end_blocks = 0;
+  double* d_times = nullptr;  // staging of gpo_ensemble_set_times (one per member)
   DriftMember* d_drift = nullptr;
   int n_drift = 0, drift_blocks = 0;
   hipGraph_t graph = nullptr;
@@ -344,19 +364,21 @@ struct Ensemble {
   // entry, from which a selection of members builds its own tables.
   std::vector<int2> h_work, h_tiles;
   std::vector<NoiseMember> h_noise;
-  std::vector<BcMember> h_bc;
+  std::vector<BcMember> h_bc, h_bc_end;
   std::vector<DriftMember> h_drift;
-  std::vector<int> bc_member, drift_member;
+  std::vector<int> bc_member, bc_end_member, drift_member;
   // Tables the launches use: the full ones, or those of the selected members.
   int selected = 0;
   const int2* cur_work = nullptr;
   const int2* cur_tiles = nullptr;
   const BcMember* cur_bc = nullptr;
+  const BcMember* cur_bc_end = nullptr;
   const DriftMember* cur_drift = nullptr;
-  int cur_n_work = 0, cur_n_tiles = 0, cur_n_bc = 0, cur_n_drift = 0;
+  int cur_n_work = 0, cur_n_tiles = 0, cur_n_bc = 0, cur_n_bc_end = 0, cur_n_drift = 0;
   int2* d_work_sel = nullptr;
   int2* d_tiles_sel = nullptr;
   BcMember* d_bc_sel = nullptr;
+  BcMember* d_bc_end_sel = nullptr;
   DriftMember* d_drift_sel = nullptr;
   uint8_t* d_active = nullptr;
   const StepArgs* cur_args = nullptr;
@@ -364,8 +386,8 @@ struct Ensemble {
   // Prepared selections (gpo_ensemble_prepare_sets): each set's tables, an
   // argument table with its time steps and its member flags, in one arena.
   struct Set {
-    size_t work, tiles, bc, drift, args, active;  // byte offsets in the arena
-    int n_work, n_tiles, n_bc, n_drift;
+    size_t work, tiles, bc, bc_end, drift, args, active;  // byte offsets in the arena
+    int n_work, n_tiles, n_bc, n_bc_end, n_drift;
   };
   std::vector<Set> sets;
   char* d_arena = nullptr;
@@ -380,6 +402,7 @@ static void use_full_tables(Ensemble* e) {
   e->cur_work = e->d_work, e->cur_n_work = e->n_work;
   e->cur_tiles = e->d_tiles, e->cur_n_tiles = e->n_tiles;
   e->cur_bc = e->d_bc, e->cur_n_bc = e->n_bc;
+  e->cur_bc_end = e->d_bc_end, e->cur_n_bc_end = e->n_bc_end;
   e->cur_drift = e->d_drift, e->cur_n_drift = e->n_drift;
 }
 
@@ -415,6 +438,15 @@ static int ensemble_launch_bc(Ensemble* e, int target, hipStream_t s) {
   return check(hipGetLastError(), "ensemble boundary launch");
 }
 
+// Ghost cells of the state of members whose steps leave them unwritten
+// (wall/periodic members without drifters), once after a run of steps.
+static int ensemble_launch_bc_end(Ensemble* e, hipStream_t s) {
+  if (e->cur_n_bc_end == 0) return 0;
+  for (int pass = 0; pass < 2; ++pass)
+    hipLaunchKernelGGL(bc_batched, dim3(e->bc_end_blocks, e->cur_n_bc_end), dim3(256), 0, s, e->cur_bc_end, pass, 0);
+  return check(hipGetLastError(), "ensemble end-of-run boundary launch");
+}
+
 static int ensemble_launch_drift(Ensemble* e, hipStream_t s) {
   if (e->cur_n_drift == 0) return 0;
   hipLaunchKernelGGL(drift_batched, dim3(e->drift_blocks, e->cur_n_drift), dim3(64), 0, s, e->cur_drift, e->cur_args);
@@ -423,9 +455,11 @@ static int ensemble_launch_drift(Ensemble* e, hipStream_t s) {
 
 // Part 0: substep 0 (+ flow-relaxation ghost cells of the stage); part 1:
 // substep 1; part 2: model error, ghost cells of the new state, drifters;
-// part 3: part 2 without the model error; part 4: the model error alone.
+// part 3: part 2 without the model error; part 4: the model error alone;
+// part 5: ghost cells of the members whose steps do not write them.
 static int ensemble_launch_part(Ensemble* e, int part, hipStream_t s) {
   int rc = 0;
+  if (part == 5) return ensemble_launch_bc_end(e, s);
   if (part == 0) {
     if (!e->h_clock.empty()) {
       const int nc = (int)e->h_clock.size();
@@ -461,6 +495,9 @@ static void ensemble_free(Ensemble* e) {
   if (e->d_tiles) (void)hipFree(e->d_tiles);
   if (e->d_draw) (void)hipFree(e->d_draw);
   if (e->d_bc) (void)hipFree(e->d_bc);
+  if (e->d_bc_end) (void)hipFree(e->d_bc_end);
+  if (e->d_bc_end_sel) (void)hipFree(e->d_bc_end_sel);
+  if (e->d_times) (void)hipFree(e->d_times);
   if (e->d_drift) (void)hipFree(e->d_drift);
   if (e->d_work_sel) (void)hipFree(e->d_work_sel);
   if (e->d_tiles_sel) (void)hipFree(e->d_tiles_sel);
@@ -500,7 +537,7 @@ int gpo_ensemble_create(const gpo_member* members, int32_t n, int32_t chunk_rows
   std::vector<NoiseMember> noise;
   std::vector<BcMember> bcs;
   std::vector<DriftMember> drifts;
-  long long max_bc_cells = 0;
+  long long max_bc_cells = 0, max_bc_end_cells = 0;
   int max_drift_rows = 0;
   std::vector<uint32_t> draws((size_t)n);
   for (int m = 0; m < n; ++m) {
@@ -601,6 +638,24 @@ int gpo_ensemble_create(const gpo_member* members, int32_t n, int32_t chunk_rows
       e->bc_member.push_back(m);
       max_bc_cells = std::max(max_bc_cells, cells::bc_target_cells(t));
       e->any_frs |= frs;
+    }
+    if (!M.bc_each_step && (((M.boundary_conditions >> 24) & 0xFF) == 2 || ((M.boundary_conditions >> 8) & 0xFF) == 2)) {
+      // periodic ghost cells, written once after a run of steps (part 5)
+      BcMember b{};
+      cells::BcTarget& t = b.h0;
+      t.nx = p.nx;
+      t.ny = p.ny;
+      t.bn = (M.boundary_conditions >> 24) & 0xFF;
+      t.bs = (M.boundary_conditions >> 16) & 0xFF;
+      t.be = (M.boundary_conditions >> 8) & 0xFF;
+      t.bw = M.boundary_conditions & 0xFF;
+      cells::bc_target_extent(&t);
+      t.pitch = M.pitch;
+      t.e = M.h0_eta, t.u = M.h0_hu, t.v = M.h0_hv;
+      b.h1 = t;
+      e->h_bc_end.push_back(b);
+      e->bc_end_member.push_back(m);
+      max_bc_end_cells = std::max(max_bc_end_cells, cells::bc_target_cells(t));
     }
     if (M.drifters) {
       if (M.n_drifters < 0 || !M.positions || !M.Bm || M.pos_pitch < 8) {
@@ -748,6 +803,26 @@ int gpo_ensemble_create(const gpo_member* members, int32_t n, int32_t chunk_rows
     // the members' clock indices (+1) become device addresses
     for (BcMember& b : bcs)
       if (b.clock) b.clock = e->d_clock + ((uintptr_t)b.clock - 1);
+    // levels and weight at the members' current times, so that every clock
+    // holds valid data pointers from the start
+    if (!rc) {
+      const int nc = (int)e->h_clock.size();
+      hipLaunchKernelGGL(clock_set_times, dim3((nc + 63) / 64), dim3(64), 0, s, e->d_clock, nc,
+                         (const double*)nullptr);
+      rc = check(hipGetLastError(), "gpo_ensemble_create clock launch");
+    }
+    if (!rc) rc = check(hipMalloc(&e->d_times, sizeof(double) * n), "gpo_ensemble_create hipMalloc");
+  }
+  e->n_bc_end = (int)e->h_bc_end.size();
+  e->bc_end_blocks = (int)((max_bc_end_cells + 255) / 256);
+  if (!rc && e->n_bc_end > 0) {
+    rc = check(hipMalloc(&e->d_bc_end, sizeof(BcMember) * e->h_bc_end.size()), "gpo_ensemble_create hipMalloc");
+    if (!rc)
+      rc = check(hipMemcpyAsync(e->d_bc_end, e->h_bc_end.data(), sizeof(BcMember) * e->h_bc_end.size(),
+                                hipMemcpyHostToDevice, s),
+                 "gpo_ensemble_create copy");
+    if (!rc)
+      rc = check(hipMalloc(&e->d_bc_end_sel, sizeof(BcMember) * e->h_bc_end.size()), "gpo_ensemble_create hipMalloc");
   }
   if (!rc && e->n_bc > 0 && e->bc_blocks > 0) {
     rc = check(hipMalloc(&e->d_bc, sizeof(BcMember) * bcs.size()), "gpo_ensemble_create hipMalloc");
@@ -809,7 +884,7 @@ int gpo_ensemble_step(gpo_ensemble h, int32_t n_steps, void* stream) {
 
 int gpo_ensemble_part(gpo_ensemble h, int32_t part, void* stream) {
   Ensemble* e = (Ensemble*)h;
-  if (!e || part < 0 || part > 4) return bad("gpo_ensemble_part: bad arguments");
+  if (!e || part < 0 || part > 5) return bad("gpo_ensemble_part: bad arguments");
   return ensemble_launch_part(e, part, (hipStream_t)stream);
 }
 
@@ -888,11 +963,17 @@ int gpo_ensemble_set_times(gpo_ensemble h, const double* t, void* stream) {
   if (!e || !t) return bad("gpo_ensemble_set_times: NULL argument");
   if (e->h_clock.empty()) return 0;
   hipStream_t s = (hipStream_t)stream;
-  // the derived fields are rewritten by the next step's clock launch
-  for (FrsClock& c : e->h_clock) c.t = t[c.member];
-  int rc = check(hipMemcpyAsync(e->d_clock, e->h_clock.data(), sizeof(FrsClock) * e->h_clock.size(),
-                                hipMemcpyHostToDevice, s),
+  // Only the times change: the kernel derives each clock's levels and weight
+  // at its new time, so ghost-cell passes before the next step read valid data.
+  int rc = check(hipMemcpyAsync(e->d_times, t, sizeof(double) * e->n, hipMemcpyHostToDevice, s),
                  "gpo_ensemble_set_times copy");
+  if (!rc) {
+    const int nc = (int)e->h_clock.size();
+    hipLaunchKernelGGL(clock_set_times, dim3((nc + 63) / 64), dim3(64), 0, s, e->d_clock, nc,
+                       (const double*)e->d_times);
+    rc = check(hipGetLastError(), "gpo_ensemble_set_times launch");
+  }
+  // the host array may change as soon as this returns
   if (!rc) rc = check(hipStreamSynchronize(s), "gpo_ensemble_set_times sync");
   return rc;
 }
@@ -917,8 +998,11 @@ int gpo_ensemble_select(gpo_ensemble h, const uint8_t* active, const float* nois
       if (active[x.x]) w.push_back(x);
     for (const int2& x : e->h_tiles)
       if (active[x.x]) t.push_back(x);
+    std::vector<BcMember> be;
     for (size_t k = 0; k < e->h_bc.size(); ++k)
       if (active[e->bc_member[k]]) b.push_back(e->h_bc[k]);
+    for (size_t k = 0; k < e->h_bc_end.size(); ++k)
+      if (active[e->bc_end_member[k]]) be.push_back(e->h_bc_end[k]);
     for (size_t k = 0; k < e->h_drift.size(); ++k)
       if (active[e->drift_member[k]]) d.push_back(e->h_drift[k]);
     auto put = [&](void* dst, const void* src, size_t bytes) {
@@ -927,6 +1011,7 @@ int gpo_ensemble_select(gpo_ensemble h, const uint8_t* active, const float* nois
     put(e->d_work_sel, w.data(), sizeof(int2) * w.size());
     put(e->d_tiles_sel, t.data(), sizeof(int2) * t.size());
     put(e->d_bc_sel, b.data(), sizeof(BcMember) * b.size());
+    put(e->d_bc_end_sel, be.data(), sizeof(BcMember) * be.size());
     put(e->d_drift_sel, d.data(), sizeof(DriftMember) * d.size());
     put(e->d_active, active, (size_t)e->n);
     // the host vectors are released when this returns: finish the copies first
@@ -937,6 +1022,7 @@ int gpo_ensemble_select(gpo_ensemble h, const uint8_t* active, const float* nois
       e->cur_work = e->d_work_sel, e->cur_n_work = (int)w.size();
       e->cur_tiles = e->d_tiles_sel, e->cur_n_tiles = (int)t.size();
       e->cur_bc = e->d_bc_sel, e->cur_n_bc = (int)b.size();
+      e->cur_bc_end = e->d_bc_end_sel, e->cur_n_bc_end = (int)be.size();
       e->cur_drift = e->d_drift_sel, e->cur_n_drift = (int)d.size();
     }
   }
@@ -964,7 +1050,7 @@ int gpo_ensemble_prepare_sets(gpo_ensemble h, int32_t n_sets, const uint8_t* act
   for (int k = 0; k < n_sets; ++k) {
     const uint8_t* act = active + (size_t)k * e->n;
     std::vector<int2> w, t;
-    std::vector<BcMember> b;
+    std::vector<BcMember> b, be;
     std::vector<DriftMember> d;
     for (const int2& x : e->h_work)
       if (act[x.x]) w.push_back(x);
@@ -972,6 +1058,8 @@ int gpo_ensemble_prepare_sets(gpo_ensemble h, int32_t n_sets, const uint8_t* act
       if (act[x.x]) t.push_back(x);
     for (size_t i = 0; i < e->h_bc.size(); ++i)
       if (act[e->bc_member[i]]) b.push_back(e->h_bc[i]);
+    for (size_t i = 0; i < e->h_bc_end.size(); ++i)
+      if (act[e->bc_end_member[i]]) be.push_back(e->h_bc_end[i]);
     for (size_t i = 0; i < e->h_drift.size(); ++i)
       if (act[e->drift_member[i]]) d.push_back(e->h_drift[i]);
     std::vector<StepArgs> args = e->h_args;
@@ -982,6 +1070,7 @@ int gpo_ensemble_prepare_sets(gpo_ensemble h, int32_t n_sets, const uint8_t* act
     S.work = put(w.data(), sizeof(int2) * w.size()), S.n_work = (int)w.size();
     S.tiles = put(t.data(), sizeof(int2) * t.size()), S.n_tiles = (int)t.size();
     S.bc = put(b.data(), sizeof(BcMember) * b.size()), S.n_bc = (int)b.size();
+    S.bc_end = put(be.data(), sizeof(BcMember) * be.size()), S.n_bc_end = (int)be.size();
     S.drift = put(d.data(), sizeof(DriftMember) * d.size()), S.n_drift = (int)d.size();
     S.args = put(args.data(), sizeof(StepArgs) * args.size());
     S.active = put(act, (size_t)e->n);
@@ -1010,19 +1099,20 @@ int gpo_ensemble_prepare_sets(gpo_ensemble h, int32_t n_sets, const uint8_t* act
 
 int gpo_ensemble_run_set(gpo_ensemble h, int32_t set, int32_t n_steps, uint32_t parts, void* stream) {
   Ensemble* e = (Ensemble*)h;
-  if (!e || set < 0 || set >= (int)e->sets.size() || n_steps < 0 || (parts & ~0x1Fu))
+  if (!e || set < 0 || set >= (int)e->sets.size() || n_steps < 0 || (parts & ~0x3Fu))
     return bad("gpo_ensemble_run_set: bad arguments");
   const Ensemble::Set& S = e->sets[set];
   e->selected = 1;
   e->cur_work = (const int2*)(e->d_arena + S.work), e->cur_n_work = S.n_work;
   e->cur_tiles = (const int2*)(e->d_arena + S.tiles), e->cur_n_tiles = S.n_tiles;
   e->cur_bc = (const BcMember*)(e->d_arena + S.bc), e->cur_n_bc = S.n_bc;
+  e->cur_bc_end = (const BcMember*)(e->d_arena + S.bc_end), e->cur_n_bc_end = S.n_bc_end;
   e->cur_drift = (const DriftMember*)(e->d_arena + S.drift), e->cur_n_drift = S.n_drift;
   e->cur_args = (const StepArgs*)(e->d_arena + S.args);
   e->cur_active = (const uint8_t*)(e->d_arena + S.active);
   int rc = 0;
   for (int i = 0; i < n_steps && !rc; ++i)
-    for (int part = 0; part < 5 && !rc; ++part)
+    for (int part = 0; part < 6 && !rc; ++part)
       if (parts & (1u << part)) rc = ensemble_launch_part(e, part, (hipStream_t)stream);
   use_full_tables(e);
   return rc;

@@ -362,15 +362,19 @@ int gpo_ensemble_update(gpo_ensemble e, const float* dts, const uint32_t* draws,
 
 /* Model times (n host doubles, one per member) of the members whose
  * flow-relaxation data has several times, as they stand before the next
- * step; the others' entries are ignored.  No reference counterpart: the
- * reference keeps each member's time on the host.  Synchronises `stream`. */
+ * step; the others' entries are ignored.  Only the times change: each
+ * member's boundary-data levels and weight are derived at its new time on
+ * the device, so ghost-cell parts launched before the next step read valid
+ * data.  No reference counterpart: the reference keeps each member's time on
+ * the host.  Synchronises `stream`. */
 int gpo_ensemble_set_times(gpo_ensemble e, const double* t, void* stream);
 
 /* Number of work items and persistent workgroups of an ensemble. */
 int gpo_ensemble_info(gpo_ensemble e, int32_t* n_work, int32_t* n_blocks);
 
 /* n_steps steps of every member, launched directly on `stream`.  Ghost cells
- * are not written (apply gpo_boundary_conditions afterwards). */
+ * of wall/periodic members without drifters are not written (part 5 writes
+ * them, once after a run of steps). */
 int gpo_ensemble_step(gpo_ensemble e, int32_t n_steps, void* stream);
 
 /* One part of a step: part 0 and 1 = RK substep 0 and 1 (with the
@@ -380,7 +384,12 @@ int gpo_ensemble_step(gpo_ensemble e, int32_t n_steps, void* stream);
  * gpo_ensemble_step.  Part 3 = part 2 without the model error (a step with
  * apply_stochastic_term=False), part 4 = the model error alone
  * (CDKLM16.perturbState after such steps, as dataAssimilationStep does,
- * SWEsimulators/CDKLM16.py:776-795).  Parts act on the selected members. */
+ * SWEsimulators/CDKLM16.py:776-795), part 5 = the ghost cells of the state
+ * of members with a periodic side whose steps do not write them (two batched
+ * launches, north/south then east/west): the gpo_boundary_conditions call
+ * CDKLM16.step ends each step with (SWEsimulators/CDKLM16.py:569-575), made
+ * once after a run of steps, since nothing in between reads those cells.
+ * Parts act on the selected members. */
 int gpo_ensemble_part(gpo_ensemble e, int32_t part, void* stream);
 
 /* Selects the members the following parts act on (active: n host bytes,

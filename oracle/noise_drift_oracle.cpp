@@ -174,13 +174,18 @@ void gpo_oracle_drift(int nx, int ny, float dx, float dy, float dt, int x0ref, i
   for (int i = 0; i < n_drifters + 1; ++i) {
     float px = pos[2 * i], py = pos[2 * i + 1];
     const float qx = px / dx, qy = py / dy;
-    const int cx = (int)(std::floor(qx) + (float)x0ref);
-    const int cy = (int)(std::floor(qy) + (float)y0ref);
+    // Cells outside the ghost-inclusive arrays (the reference reads them
+    // unchecked, driftKernels.cu:118-141): the drifter stays where it is.
+    const float fcx = std::floor(qx) + (float)x0ref, fcy = std::floor(qy) + (float)y0ref;
+    if (!(fcx >= 0.0f && fcx <= (float)(nx + 3) && fcy >= 0.0f && fcy <= (float)(ny + 3))) continue;
+    const int cx = (int)fcx;
+    const int cy = (int)fcy;
     const float frx = qx - std::floor(qx), fry = qy - std::floor(qy);
     const int cx0 = frx < 0.5f ? cx - 1 : cx;
     const float xf = frx < 0.5f ? frx + 0.5f : frx - 0.5f;
     const int cy0 = fry < 0.5f ? cy - 1 : cy;
     const float yf = fry < 0.5f ? fry + 0.5f : fry - 0.5f;
+    if (cx0 < 0 || cy0 < 0 || cx0 + 1 > nx + 3 || cy0 + 1 > ny + 3) continue;
     const float u00 = vel(hu, cx0, cy0), u10 = vel(hu, cx0 + 1, cy0);
     const float u01 = vel(hu, cx0, cy0 + 1), u11 = vel(hu, cx0 + 1, cy0 + 1);
     const float v00 = vel(hv, cx0, cy0), v10 = vel(hv, cx0 + 1, cy0);

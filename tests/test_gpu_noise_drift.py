@@ -172,3 +172,27 @@ def test_drift_matches_oracle(gpu_ctx, periodic):
         O.drift(nx, ny, dx, dy, 30.0, e, u, v, Bm, periodic, periodic, ref)
     got = np.concatenate([d.getDrifterPositions(), [d.getObservationPosition()]], axis=0)
     assert np.array_equal(got, ref)
+
+
+def test_drift_outside_the_arrays_matches_oracle(gpu_ctx):
+    """Drifters just outside an open side: the GPU leaves those whose stencil
+    leaves the arrays in place, as the oracle does, and moves the rest
+    identically."""
+    from test_noise_drift_cpu import _drift_guard_case
+    nx, ny, dx, dy, eta, hu, hv, Hm, pos, inside = _drift_guard_case()
+    Hi = np.full((ny + 5, nx + 5), 10.0, np.float32)
+    sim = CDKLM16(gpu_ctx, eta, hu, hv, Hi, nx, ny, dx, dy, 1.0, 9.81, 0.0, 0.0)
+    bc = Common.BoundaryConditions()
+    d = GPUDrifterCollection(gpu_ctx, len(pos) - 1, boundaryConditions=bc, domain_size_x=nx * dx,
+                             domain_size_y=ny * dy, gpu_stream=sim.gpu_stream)
+    d.setDrifterPositions(pos[:-1])
+    d.setObservationPosition(pos[-1])
+    Bm = np.asarray(sim.downloadBathymetry()[1], np.float32)
+    e, u, v = sim.download()
+    ref = pos.copy()
+    for _ in range(3):
+        d.driftFromSim(sim, 10.0)
+        O.drift(nx, ny, dx, dy, 10.0, e, u, v, Bm, False, False, ref)
+    got = np.concatenate([d.getDrifterPositions(), [d.getObservationPosition()]], axis=0)
+    assert np.array_equal(got, ref, equal_nan=True)
+    assert not np.array_equal(got[inside], pos[inside])

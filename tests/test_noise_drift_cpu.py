@@ -106,3 +106,33 @@ def test_wind_drift_known_answer_on_the_oracle():
     _wind_drift_known_answer(lambda: o.step(1.0), drift_all)
     got = np.array([p[0] for p in pos], np.float64)
     assert np.max(np.abs(got - np.array(WIND_DRIFT_REF))) < 5e-5   # places=4, as the reference asserts
+
+
+def _drift_guard_case():
+    """A uniform eastward flow (u = 2 m/s) and drifters inside, at the edge of
+    and beyond the two ghost cells of an open (non-periodic) domain."""
+    nx, ny, dx, dy = 40, 30, 100.0, 100.0
+    eta = np.zeros((ny + 4, nx + 4), np.float32)
+    Hm = np.full((ny + 4, nx + 4), 10.0, np.float32)
+    hu = np.full_like(eta, 20.0)
+    hv = np.zeros_like(eta)
+    x = np.float32([5.5, nx + 1.2, nx + 1.7, nx + 2.6, -1.2, -1.7, -2.6, 20.5, 20.5, np.nan, 1e30]) * dx
+    y = np.float32([15.5, 15.5, 15.5, 15.5, 15.5, 15.5, 15.5, ny + 2.6, -2.6, 15.5, 15.5]) * dy
+    pos = np.stack([x, y], axis=1).astype(np.float32)
+    # drifters whose four surrounding cell centres lie inside the ghost-inclusive arrays
+    inside = np.array([True, True, False, False, True, False, False, False, False, False, False])
+    return nx, ny, dx, dy, eta, hu, hv, Hm, pos, inside
+
+
+def test_drift_outside_the_arrays_is_left_in_place():
+    """Drifters whose bilinear stencil would leave the ghost-inclusive arrays
+    (the reference reads outside them, driftKernels.cu:118-141) stay put;
+    the others move by u dt exactly in a uniform flow."""
+    nx, ny, dx, dy, eta, hu, hv, Hm, pos, inside = _drift_guard_case()
+    ref = pos.copy()
+    O.drift(nx, ny, dx, dy, 10.0, eta, hu, hv, Hm, False, False, ref)
+    moved = ref[inside]
+    assert np.array_equal(moved[:, 0], pos[inside, 0] + np.float32(2.0) * np.float32(10.0))
+    assert np.array_equal(moved[:, 1], pos[inside, 1])
+    out = ~inside
+    assert np.array_equal(ref[out], pos[out], equal_nan=True)
