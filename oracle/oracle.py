@@ -79,6 +79,7 @@ def lib():
         _lib.gpo_oracle_max_dt_f32.restype = ctypes.c_float
         _lib.gpo_oracle_max_dt_f64.restype = ctypes.c_double
         _lib.gpo_oracle_fcell_f32.argtypes = [P, vp]
+        _lib.gpo_oracle_substep_f32.argtypes = [P, vp, vp, vp, vp, vp, vp, vp, vp, ctypes.c_float, ctypes.c_int]
         _lib.gpo_oracle_num_threads.restype = ctypes.c_int
     return _lib
 
@@ -250,6 +251,28 @@ class OracleCDKLM16:
         FRS time levels are held at the current time."""
         self._run(np.float32(dt), n, self.t == 0)
         self.t += n * np.float64(np.float32(dt))
+
+    def rk2_step(self, dt, perturb=None):
+        """One RK2 step in the order CDKLM16._one_step takes it (FP32 only):
+        substep 0, ghost cells of the stage, substep 1, then perturb(h0) (the
+        model error, when given), then the ghost cells of the new state.  The
+        ghost cells of h0 are written first when t = 0, as CDKLM16.step does."""
+        assert self.suf == "f32"
+        L = lib()
+        frs = self._frs_struct()
+        fr = None if frs is None else ctypes.byref(frs)
+        q = ctypes.byref(self.q)
+        if self.t == 0:
+            L.gpo_oracle_apply_bc_f32(q, *[_p(a) for a in self.h0], fr)
+        sub = L.gpo_oracle_substep_f32
+        common = (_p(self.Hi), _p(self.Bm), ctypes.c_float(dt))
+        sub(q, *[_p(a) for a in self.h0], *[_p(a) for a in self.h1], *common, ctypes.c_int(0))
+        L.gpo_oracle_apply_bc_f32(q, *[_p(a) for a in self.h1], fr)
+        sub(q, *[_p(a) for a in self.h1], *[_p(a) for a in self.h0], *common, ctypes.c_int(1))
+        if perturb is not None:
+            perturb(self.h0)
+        L.gpo_oracle_apply_bc_f32(q, *[_p(a) for a in self.h0], fr)
+        self.t += np.float64(np.float32(dt))
 
     def download(self, interior_domain_only=False):
         out = [a.copy() for a in self.h0]

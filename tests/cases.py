@@ -315,3 +315,34 @@ def c4_drifter_positions(nx, ny, dx, dy, n, seed):
     x = rng.uniform(0.3 * nx * dx, 0.8 * nx * dx, n)
     y = rng.uniform(0.25 * ny * dy, 0.75 * ny * dy, n)
     return np.stack([x, y], axis=1).astype(np.float32)
+
+
+def config_coast_island(nx=160, ny=120, seed=5):
+    """Land, a drying shelf and an island in one member: dx=dy=500 m, walls,
+    f=1e-4; corner depths ramp from -4 m at the west edge to 60 m at the east
+    edge; corners at or above the datum are masked land.  Beside the coast
+    lie cells a few centimetres to a metre deep (desingularised cells,
+    h < 0.1 m); those shallower than 2 m start dry (eta = -Hm) and the
+    waves wet and dry them; a masked island of corners sits offshore; a 0.6 m
+    Gaussian bump east of the island sends waves across the shelf and around
+    the island."""
+    rng = np.random.default_rng(seed)
+    dx = dy = 500.0
+    xc = (np.arange(nx + 5) - 2.0) / nx
+    H = np.repeat((-4.0 + 64.0 * xc)[None, :], ny + 5, axis=0)
+    H += 0.3 * rng.standard_normal(H.shape)
+    H = H.astype(np.float32)
+    mask = H <= 0.0
+    cy, cx = ny // 2, int(0.45 * nx)
+    jj, ii = np.meshgrid(np.arange(ny + 5), np.arange(nx + 5), indexing="ij")
+    mask |= ((jj - cy) / 8.0) ** 2 + ((ii - cx) / 5.0) ** 2 <= 1.0
+    Hm_ = np.where(mask, 0.0, H)
+    wet_c = (~mask).astype(np.float64)
+    n_wet = wet_c[:-1, :-1] + wet_c[1:, :-1] + wet_c[:-1, 1:] + wet_c[1:, 1:]
+    Hm = (Hm_[:-1, :-1] + Hm_[1:, :-1] + Hm_[:-1, 1:] + Hm_[1:, 1:]) / np.maximum(n_wet, 1)
+    jc, ic = np.meshgrid(np.arange(ny + 4), np.arange(nx + 4), indexing="ij")
+    eta = 0.6 * np.exp(-((ic - 0.65 * nx) ** 2 + (jc - cy) ** 2) / (2 * 6.0 ** 2))
+    eta = np.where((n_wet > 0) & (Hm < 2.0), -Hm, eta).astype(np.float32)
+    z = np.zeros_like(eta)
+    return dict(eta0=eta, hu0=z, hv0=z.copy(), H=np.ma.masked_array(H, mask), nx=nx, ny=ny, dx=dx, dy=dy, dt=0.0,
+                g=9.81, f=1e-4, r=0.0, courant_number=0.7, boundary_conditions=Common.BoundaryConditions(1, 1, 1, 1))
