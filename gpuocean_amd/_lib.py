@@ -178,6 +178,10 @@ _SIGNATURES = [
      [ctypes.POINTER(CDKLM16Params), c_int32, c_float, ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p),
       ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p), c_int32, c_void_p, c_int32, c_void_p, c_uint32,
       ctypes.POINTER(Frs), c_void_p, c_void_p, c_void_p]),
+    ("gpo_cdklm16_adjoint_steps_f32_noalloc", c_int32,
+     [ctypes.POINTER(CDKLM16Params), c_int32, c_float, ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p),
+      ctypes.POINTER(c_void_p), c_int32, c_void_p, c_int32, c_void_p, c_uint32, ctypes.POINTER(Frs), c_void_p,
+      c_void_p]),
     ("gpo_frs_adj_f32", c_int32,
      [ctypes.POINTER(CDKLM16Params), c_uint32, ctypes.POINTER(c_int32), c_void_p, c_void_p, c_void_p, c_int32,
       c_void_p]),

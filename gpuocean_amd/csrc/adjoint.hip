@@ -4,6 +4,7 @@
 // misfit and their C ABI.  Built with -ffp-contract=off.
 
 #include <cmath>
+#include <mutex>
 #include <cstdint>
 #include <algorithm>
 #include <cstdio>
@@ -564,6 +565,81 @@ int adj_batch_stage(AdjBatch* b, const AdjItem<float>* tab, int* flag, hipStream
   return hip_check(hipGetLastError(), "batched adjoint launch");
 }
 
+// The reverse sweep of gpo_cdklm16_adjoint_steps_f32 over caller-given (or
+// library scratch) lambda arrays and stage workspace.
+int adjoint_steps_impl(const gpo_cdklm16_params* p, int32_t n_steps, float dt, const float* const* qs,
+                       const float* const* q1s, float* const* lam, float* const* work, int32_t pitch, const float* Hi,
+                       int32_t Hi_pitch, const float* f_cells, uint32_t boundary_conditions, const gpo_frs* frs,
+                       void* workspace, int32_t* flag, void* stream) {
+  Geo g;
+  int rc = make_geo(p, boundary_conditions, pitch, Hi_pitch, f_cells ? pitch : 4, 4, &g);
+  if (rc) return rc;
+  if (n_steps < 0 || !qs || !q1s || !lam || !work || !workspace)
+    return bad_arg("gpo_cdklm16_adjoint_steps_f32: bad arguments");
+  hipStream_t s = (hipStream_t)stream;
+  const Sponge sp = frs ? Sponge{frs->sponge[0], frs->sponge[1], frs->sponge[2], frs->sponge[3]} : Sponge{0, 0, 0, 0};
+  Hi = land_hi(&g, Hi, workspace, s);
+  float* cur[3] = {lam[0], lam[1], lam[2]};     // lambda at Q^{j+1}, then at Q^j
+  float* nxt[3] = {work[0], work[1], work[2]};  // lambda being formed at Q^j
+  float* l1[3] = {work[3], work[4], work[5]};   // lambda at the RK stage Q^(1)
+  for (int j = n_steps - 1; j >= 0; --j) {
+    const float* const* a = qs + 3 * j;
+    const float* const* b = q1s + 3 * j;
+    const float* const* c = qs + 3 * (j + 1);
+    // reverse of the second RK stage (writes its direct Q^n term to nxt), then of
+    // the first (adds its sensitivity of Q^n to nxt); with flow relaxation,
+    // the transpose of each sponge blend before the stage that produced it
+    if (frs) hipLaunchKernelGGL(frs_scale<float>, cell_grid(g), kBlock, 0, s, g, sp, cur[0], cur[1], cur[2]);
+    rc = adjoint_stages<float>(g, dt, 1, b[0], b[1], b[2], c[0], cur[0], cur[1], cur[2], l1[0], l1[1], l1[2], nxt[0],
+                               nxt[1], nxt[2], 0, 1, Hi, f_cells, workspace, flag, s);
+    if (!rc && frs) hipLaunchKernelGGL(frs_scale<float>, cell_grid(g), kBlock, 0, s, g, sp, l1[0], l1[1], l1[2]);
+    if (!rc)
+      rc = adjoint_stages<float>(g, dt, 0, a[0], a[1], a[2], b[0], l1[0], l1[1], l1[2], nxt[0], nxt[1], nxt[2],
+                                 nullptr, nullptr, nullptr, 1, 0, Hi, f_cells, workspace, flag, s);
+    if (rc) return rc;
+    for (int k = 0; k < 3; ++k) {
+      float* t = cur[k];
+      cur[k] = nxt[k];
+      nxt[k] = t;
+    }
+  }
+  if (cur[0] != lam[0])
+    hipLaunchKernelGGL(state_copy<float>, inner_grid(g), kBlock, 0, s, g, cur[0], cur[1], cur[2], lam[0], lam[1],
+                       lam[2]);
+  return hip_check(hipGetLastError(), "gpo_cdklm16_adjoint_steps_f32 launch");
+}
+
+// Scratch of gpo_cdklm16_adjoint_steps_f32_noalloc: one array per device,
+// held until the process ends.  Host code never passes the symbol itself (its
+// host-side name is no device address): the device address is looked up once
+// per device, with that device current, and kept.
+constexpr size_t kAdjScratchBytes = (size_t)48 << 20;
+__device__ __align__(256) unsigned char scratch[kAdjScratchBytes];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+char* g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_base(char** out) {
+  int dev = 0;
+  int rc = hip_check(hipGetDevice(&dev), "scratch: hipGetDevice");
+  if (rc) return rc;
+  if (dev < 0 || dev >= kMaxDevices) return bad_arg("scratch: device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    rc = hip_check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "scratch: hipGetSymbolAddress");
+    if (rc) return rc;
+    if (!ptr) return bad_arg("scratch: no device address");
+    g_scratch_ptr[dev] = (char*)ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
+__global__ void fill_zero_words(uint32_t* p, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0u;
+}
+
 }  // namespace
 
 extern "C" {
@@ -692,42 +768,44 @@ int gpo_cdklm16_adjoint_steps_f32(const gpo_cdklm16_params* p, int32_t n_steps, 
                                   const float* Hi, int32_t Hi_pitch, const float* f_cells,
                                   uint32_t boundary_conditions, const gpo_frs* frs, void* workspace, int32_t* flag,
                                   void* stream) {
-  Geo g;
-  int rc = make_geo(p, boundary_conditions, pitch, Hi_pitch, f_cells ? pitch : 4, 4, &g);
-  if (rc) return rc;
-  if (n_steps < 0 || !qs || !q1s || !lam || !work || !workspace)
-    return bad_arg("gpo_cdklm16_adjoint_steps_f32: bad arguments");
-  hipStream_t s = (hipStream_t)stream;
-  const Sponge sp = frs ? Sponge{frs->sponge[0], frs->sponge[1], frs->sponge[2], frs->sponge[3]} : Sponge{0, 0, 0, 0};
-  Hi = land_hi(&g, Hi, workspace, s);
-  float* cur[3] = {lam[0], lam[1], lam[2]};     // lambda at Q^{j+1}, then at Q^j
-  float* nxt[3] = {work[0], work[1], work[2]};  // lambda being formed at Q^j
-  float* l1[3] = {work[3], work[4], work[5]};   // lambda at the RK stage Q^(1)
-  for (int j = n_steps - 1; j >= 0; --j) {
-    const float* const* a = qs + 3 * j;
-    const float* const* b = q1s + 3 * j;
-    const float* const* c = qs + 3 * (j + 1);
-    // reverse of the second RK stage (writes its direct Q^n term to nxt), then of
-    // the first (adds its sensitivity of Q^n to nxt); with flow relaxation,
-    // the transpose of each sponge blend before the stage that produced it
-    if (frs) hipLaunchKernelGGL(frs_scale<float>, cell_grid(g), kBlock, 0, s, g, sp, cur[0], cur[1], cur[2]);
-    rc = adjoint_stages<float>(g, dt, 1, b[0], b[1], b[2], c[0], cur[0], cur[1], cur[2], l1[0], l1[1], l1[2], nxt[0],
-                               nxt[1], nxt[2], 0, 1, Hi, f_cells, workspace, flag, s);
-    if (!rc && frs) hipLaunchKernelGGL(frs_scale<float>, cell_grid(g), kBlock, 0, s, g, sp, l1[0], l1[1], l1[2]);
-    if (!rc)
-      rc = adjoint_stages<float>(g, dt, 0, a[0], a[1], a[2], b[0], l1[0], l1[1], l1[2], nxt[0], nxt[1], nxt[2],
-                                 nullptr, nullptr, nullptr, 1, 0, Hi, f_cells, workspace, flag, s);
-    if (rc) return rc;
-    for (int k = 0; k < 3; ++k) {
-      float* t = cur[k];
-      cur[k] = nxt[k];
-      nxt[k] = t;
-    }
+  return adjoint_steps_impl(p, n_steps, dt, qs, q1s, lam, work, pitch, Hi, Hi_pitch, f_cells, boundary_conditions,
+                            frs, workspace, flag, stream);
+}
+
+int gpo_cdklm16_adjoint_steps_f32_noalloc(const gpo_cdklm16_params* p, int32_t n_steps, float dt,
+                                          const float* const* qs, const float* const* q1s, float* const* lam,
+                                          int32_t pitch, const float* Hi, int32_t Hi_pitch, const float* f_cells,
+                                          uint32_t boundary_conditions, const gpo_frs* frs, int32_t* flag,
+                                          void* stream) {
+  if (!p || p->nx < 1 || p->ny < 1 || pitch < 4 * (p->nx + 4) || pitch % 4)
+    return bad_arg("gpo_cdklm16_adjoint_steps_f32_noalloc: bad parameters");
+  // scratch layout: six lambda arrays of the state's shape, then the stage workspace
+  const size_t plane = (((size_t)pitch * (p->ny + 4)) + 255) & ~(size_t)255;
+  const size_t need = 6 * plane + gpo_adjoint_workspace_bytes(p->nx, p->ny, 4);
+  if (need > kAdjScratchBytes) {
+    char buf[240];
+    snprintf(buf, sizeof buf,
+             "gpo_cdklm16_adjoint_steps_f32_noalloc: this grid needs %zu bytes of scratch, more than the library's "
+             "%zu (use gpo_cdklm16_adjoint_steps_f32 with caller buffers)",
+             need, kAdjScratchBytes);
+    return bad_arg(buf);
   }
-  if (cur[0] != lam[0])
-    hipLaunchKernelGGL(state_copy<float>, inner_grid(g), kBlock, 0, s, g, cur[0], cur[1], cur[2], lam[0], lam[1],
-                       lam[2]);
-  return hip_check(hipGetLastError(), "gpo_cdklm16_adjoint_steps_f32 launch");
+  char* base = nullptr;
+  int rc = scratch_base(&base);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  // The lambda arrays' ghost cells are read as zeros and the previous call's
+  // values remain: clear all six, with a fill kernel (no memset node, so the
+  // call can be captured in a graph).
+  const size_t words = 6 * plane / 4;
+  hipLaunchKernelGGL(fill_zero_words, dim3((unsigned)std::min<size_t>((words + 255) / 256, 65535)), dim3(256), 0, s,
+                     (uint32_t*)base, words);
+  rc = hip_check(hipGetLastError(), "gpo_cdklm16_adjoint_steps_f32_noalloc clear");
+  if (rc) return rc;
+  float* work[6];
+  for (int q = 0; q < 6; ++q) work[q] = (float*)(base + q * plane);
+  return adjoint_steps_impl(p, n_steps, dt, qs, q1s, lam, work, pitch, Hi, Hi_pitch, f_cells, boundary_conditions,
+                            frs, base + 6 * plane, flag, stream);
 }
 
 int gpo_cdklm16_substep_adj_f32(const gpo_cdklm16_params* p, float dt, int32_t rk_step, const float* in_eta,

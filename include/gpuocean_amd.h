@@ -517,6 +517,23 @@ int gpo_cdklm16_adjoint_steps_f32(const gpo_cdklm16_params* p, int32_t n_steps, 
                                   uint32_t boundary_conditions, const gpo_frs* frs, void* workspace,
                                   int32_t* flag, void* stream);
 
+/* gpo_cdklm16_adjoint_steps_f32 without the caller's scratch (`work` and
+ * `workspace`): the six lambda arrays and the stage workspace live in a
+ * 48 MiB array of library device memory, one per GPU, kept until the process
+ * ends, so the call allocates nothing (no hipMalloc/hipFree, no allocating
+ * library routine) and can be captured in a hipGraph.  Grids whose scratch,
+ * 6 * round_up(pitch * (ny+4), 256) + gpo_adjoint_workspace_bytes(nx, ny, 4)
+ * bytes, exceeds 48 MiB (C3's 512 x 512 needs about 40 MiB) are rejected.  Every
+ * call on a GPU shares that GPU's array: calls that overlap on one GPU (other
+ * streams, other threads) race.  Results equal gpo_cdklm16_adjoint_steps_f32's
+ * bit for bit.  No counterpart in the reference: it serves callers that cannot
+ * allocate (code under graph capture, tight serving loops). */
+int gpo_cdklm16_adjoint_steps_f32_noalloc(const gpo_cdklm16_params* p, int32_t n_steps, float dt,
+                                          const float* const* qs, const float* const* q1s, float* const* lam,
+                                          int32_t pitch, const float* Hi, int32_t Hi_pitch, const float* f_cells,
+                                          uint32_t boundary_conditions, const gpo_frs* frs, int32_t* flag,
+                                          void* stream);
+
 /* Transpose (equally: tangent) of the flow-relaxation blend: lambda *=
  * (1 - alpha) on the sponge cells of the code-3 sides (the alpha of
  * boundary_kernels.cu:406-540), east/west pass first. */

@@ -70,3 +70,15 @@ def test_oracle_is_not_imported_by_the_product():
                 with open(os.path.join(dirpath, fn)) as f:
                     text = f.read()
                 assert "from oracle" not in text and "import oracle" not in text, fn
+
+
+def test_noalloc_adjoint_sweep_rejects_oversized_grids_without_a_gpu():
+    """The size check of the no-allocation sweep comes before any device call."""
+    L = _lib.lib()
+    p = _lib.CDKLM16Params()
+    p.nx, p.ny, p.rk_order = 2000, 2000, 2
+    lam = (ctypes.c_void_p * 3)(None, None, None)
+    rc = L.gpo_cdklm16_adjoint_steps_f32_noalloc(ctypes.byref(p), 1, 1.0, lam, lam, lam, 4 * 2048, None, 4 * 2048,
+                                                  None, 0x01010101, None, None, None)
+    assert rc == -1
+    assert b"scratch" in L.gpo_last_error()
