@@ -201,6 +201,24 @@ _SIGNATURES = [
     ("gpo_balance_adj_f32", c_int32,
      [ctypes.POINTER(CDKLM16Params), c_float, c_float, c_float, c_void_p, c_int32, c_void_p, c_int32,
       c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_int32, c_void_p]),
+    ("gpo_balance_adj_f64", c_int32,
+     [ctypes.POINTER(CDKLM16Params), ctypes.c_double, ctypes.c_double, ctypes.c_double, c_void_p, c_int32,
+      c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_int32, c_void_p]),
+    ("gpo_balance_f64", c_int32,
+     [ctypes.POINTER(CDKLM16Params), ctypes.c_double, ctypes.c_double, ctypes.c_double, c_void_p, c_int32,
+      c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_int32,
+      c_void_p]),
+    ("gpo_drift_f64", c_int32,
+     [c_int32, c_int32, ctypes.c_double, ctypes.c_double, ctypes.c_double, c_void_p, c_void_p, c_void_p, c_int32,
+      c_void_p, c_int32, c_uint32, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_double,
+      c_void_p]),
+    ("gpo_drift_adj_f32", c_int32,
+     [c_int32, c_int32, c_float, c_float, c_float, c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_int32,
+      c_uint32, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_void_p]),
+    ("gpo_drift_adj_f64", c_int32,
+     [c_int32, c_int32, ctypes.c_double, ctypes.c_double, ctypes.c_double, c_void_p, c_void_p, c_void_p, c_int32,
+      c_void_p, c_int32, c_uint32, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_double,
+      c_void_p]),
     ("gpo_point_misfit_f32", c_int32,
      [c_int32, c_int32, c_float, c_float, c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_int32, c_int32,
       c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),

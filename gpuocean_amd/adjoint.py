@@ -15,6 +15,15 @@ friction or forcing) and gives
   * an FP64 build of the forward, tangent-linear and adjoint runs, used by the
     finite-difference and dot-product checks.
 
+With ``model_error=True`` every step adds the member's SOAR model error for
+fixed normals (the Philox draws of the member's own stream), whose balanced
+momentum depends on the state, and with ``drifter_positions`` the drifters are
+advected after every step as ``CDKLM16`` moves them, and J adds the
+drifter-position misfit sum |x_d(T) - x_obs_d|^2.  Both run step by step: the
+forward repeats the product's step (substeps, model error, ghost cells,
+drift), and the reverse sweep takes drift, sponge, model error and the two
+substeps in the opposite order.
+
 ``CDKLM16Misfit`` wraps the FP32 path as a ``torch.autograd.Function``;
 ``DifferentiableEnsemble`` runs the FP32 path for many members at once.
 """
@@ -67,7 +76,8 @@ class DifferentiableCDKLM16:
     """Misfit and its gradient for n_steps RK2 steps of fixed dt of the given
     simulator's configuration (its bathymetry, coriolis, boundaries, dt)."""
 
-    def __init__(self, sim, positions, observed, n_steps, dt=None, checkpoint_every=20, _workspace=True):
+    def __init__(self, sim, positions, observed, n_steps, dt=None, checkpoint_every=20, _workspace=True,
+                 model_error=False, drifter_positions=None, drifter_observed=None):
         p = sim.params
         bc = sim.boundary_conditions
         if int(sim.rk_order) != 2 or float(sim.r) != 0.0:
@@ -120,6 +130,33 @@ class DifferentiableCDKLM16:
             self.J = torch.zeros(1, dtype=torch.float64, device=self.dev)
         self._f64 = None
         self._seg = None
+        # stochastic member (fixed normals) and drifters: the step-by-step path
+        self.gen = None
+        if model_error:
+            me = sim.model_error
+            if me is None or int(me.interpolation_factor) != 1:
+                raise ValueError("model_error needs the simulator's SOAR model error on its own grid (factor 1)")
+            from .OceanStateNoise import OceanStateNoise
+            self.gen = OceanStateNoise.fromsim(sim, soar_q0=me.soar_q0, soar_L=me.soar_L, xorwow_seed=me.seed,
+                                               stream_id=me.stream_id)
+            self.gen.gpu_stream = self.stream
+            self.draw0 = int(me.draw)
+            self.g32, self.f32, self.beta32 = float(np.float32(sim.g)), float(np.float32(sim.f)), \
+                float(np.float32(sim.coriolis_beta))
+            if np.asarray(sim.f).size != 1:
+                raise ValueError("model_error needs a scalar coriolis parameter")
+        self.n_drifters = 0
+        if drifter_positions is not None:
+            dp = np.asarray(drifter_positions, np.float32).reshape(-1, 2)
+            do = np.asarray(drifter_observed, np.float32).reshape(-1, 2)
+            assert dp.shape == do.shape and dp.shape[0] >= 1
+            self.n_drifters = dp.shape[0]
+            self.x0 = torch.from_numpy(dp).to(self.dev)
+            self.xobs = torch.from_numpy(do).to(self.dev)
+            self.Bm = sim.Bm
+            self.per_ns = int(bc.isPeriodicNorthSouth())
+            self.per_ew = int(bc.isPeriodicEastWest())
+        self.generic = self.gen is not None or self.n_drifters > 0
 
     # ------------------------------------------------------------------ helpers
     def _sh(self):
@@ -189,6 +226,8 @@ class DifferentiableCDKLM16:
         """Run n_steps from (eta0, hu0, hv0) ((ny+4) x (nx+4) device float32),
         keeping a checkpoint every checkpoint_every steps.  Returns J (python
         float) and the final state."""
+        if self.generic:
+            return self._forward_generic32(eta0, hu0, hv0)
         s = self.stream
         self._enter()
         A, B = self._new_state(), self._new_state()
@@ -260,6 +299,8 @@ class DifferentiableCDKLM16:
         three (ny+4) x (nx+4) float32 tensors (ghost cells zero).  Each
         segment between checkpoints is recomputed once, keeping every state
         and stage, then swept backwards, both by single library calls."""
+        if self.generic:
+            return self._backward_generic32(grad_J)
         s = self.stream
         self._enter()
         seg = self._segment_buffers()
@@ -338,6 +379,8 @@ class DifferentiableCDKLM16:
 
     def trajectory_f64(self, eta0, hu0, hv0, n_steps=None):
         """FP64 states Q^0 .. Q^n and stages Q1^0 .. Q1^(n-1) (kept whole)."""
+        if self.generic:
+            return self._trajectory_generic64(eta0, hu0, hv0, n_steps)
         self._init_f64()
         self._enter()
         n = self.n_steps if n_steps is None else n_steps
@@ -355,6 +398,8 @@ class DifferentiableCDKLM16:
 
     def misfit_f64(self, eta0, hu0, hv0):
         """J of the FP64 forward run from the given (float64) initial state."""
+        if self.generic:
+            return self._misfit_generic64(eta0, hu0, hv0)
         self._init_f64()
         self._enter()
         A, B = self._new_state(torch.float64), self._new_state(torch.float64)
@@ -407,6 +452,13 @@ class DifferentiableCDKLM16:
 
     def gradient_f64(self, eta0, hu0, hv0):
         """J and dJ/dQ^0 of the FP64 run (trajectory kept whole)."""
+        if self.generic:
+            tr = self._trajectory_generic64(eta0, hu0, hv0)
+            J = self._j_generic64(tr)
+            lam, lx = self._final_sens64(tr)
+            lam0, _ = self._adjoint_generic64(tr, lam, lx)
+            self.stream.synchronize()
+            return J, [a.clone() for a in lam0.views()]
         Qs, Q1s = self.trajectory_f64(eta0, hu0, hv0)
         lamN = self._new_state(torch.float64)
         with torch.cuda.stream(self.stream):
@@ -417,6 +469,293 @@ class DifferentiableCDKLM16:
         lam0 = self.adjoint_f64(Qs, Q1s, lamN)
         self.stream.synchronize()
         return J, [a.clone() for a in lam0.views()]
+
+
+    # ------------------------------------------- step-by-step path (model error, drifters)
+    def _stage_bc32(self, st):
+        """Ghost cells as the product's step writes them: flow-relaxation
+        blend and sides, or periodic copies (drifters read ghost cells)."""
+        bc = self.sim.boundary_conditions
+        if self.frs is None and not (bc.isPeriodicNorthSouth() or bc.isPeriodicEastWest()):
+            return
+        f = self.frs
+        L.check(L.lib().gpo_boundary_conditions(
+            self.nx, self.ny, self.bc, self.sim.sponge, *st.ptrs(), st.pitch,
+            None if f is None else f.ns_cur, None if f is None else f.ns_next, 0 if f is None else f.ns_nx,
+            None if f is None else f.ew_cur, None if f is None else f.ew_next, 0 if f is None else f.ew_ny,
+            1.0 if f is None else float(f.t), self._sh()), "gpo_boundary_conditions")
+
+    def _coarse(self, j):
+        """The model error's SOAR field of step j (draw draw0 + j)."""
+        self.gen._fill(self.draw0 + j)
+        self.gen._soar(1.0, 1.0)
+        return self.gen.coarse_buffer
+
+    def _noise32(self, st, j):
+        c = self._coarse(j)
+        sim = self.sim
+        L.check(L.lib().gpo_geostrophic_balance(
+            self.nx, self.ny, self.dx, self.dy, self.g32, self.f32, self.beta32, float(self.params.north_x),
+            float(self.params.north_y), self._fc_ptr(), 0 if self.fc is None else self.fc.pitch, _vp(c),
+            4 * (self.nx + 4), *st.ptrs(), st.pitch, _vp(self.Hi), self.Hi.pitch, float(sim.mask_value),
+            self._sh()), "gpo_geostrophic_balance")
+
+    def _drift32(self, st, x):
+        L.check(L.lib().gpo_drift(
+            self.nx, self.ny, self.dx, self.dy, float(self.dt), 2, 2, *st.ptrs(), st.pitch, _vp(self.Bm),
+            self.Bm.pitch, self.per_ns, self.per_ew, self.n_drifters - 1, _vp(x), 8, 1.0, None, None, None, None, 0,
+            0, 0.0, 0.0, self._sh()), "gpo_drift")
+
+    def _step_generic32(self, src, stage, pre, dst, j, x):
+        """CDKLM16._one_step on private buffers: stage = S0(src), ghost cells,
+        pre = S1(stage; src), dst = model error(pre), ghost cells, drift of x
+        (pre may be dst; dst may be src)."""
+        self._substep32(src, stage, None, 0)
+        if self.frs is not None:
+            self._stage_bc32(stage)
+        self._substep32(stage, pre, src, 1)
+        if dst is not pre:
+            with torch.cuda.stream(self.stream):
+                dst.copy_from(pre)
+        if self.gen is not None:
+            self._noise32(dst, j)
+        self._stage_bc32(dst)
+        if self.n_drifters:
+            self._drift32(dst, x)
+
+    def _drifter_j(self, x):
+        if not self.n_drifters:
+            return 0.0
+        d = x.double() - self.xobs.double()
+        return float((d * d).sum().item())
+
+    def _forward_generic32(self, eta0, hu0, hv0):
+        s = self.stream
+        self._enter()
+        A, B = self._new_state(), self._new_state()
+        ckpts, xck = [], []
+        with torch.cuda.stream(s):
+            A.load(eta0, hu0, hv0)
+            x = self.x0.clone() if self.n_drifters else None
+        for j in range(self.n_steps):
+            if j % self.k == 0:
+                c = self._new_state()
+                with torch.cuda.stream(s):
+                    c.copy_from(A)
+                    xck.append(x.clone() if x is not None else None)
+                ckpts.append(c)
+            self._step_generic32(A, B, A, A, j, x)
+        with torch.cuda.stream(s):
+            self.J.zero_()
+        if self.n_obs:
+            self._misfit32(A)
+        s.synchronize()
+        J = float(self.J.item()) + self._drifter_j(x)
+        self._final, self._final_x = A, x
+        self._ckpts, self._xck = ckpts, xck
+        return J, A
+
+    def _backward_generic32(self, grad_J=1.0):
+        s = self.stream
+        self._enter()
+        k, n = self.k, self.n_steps
+        if self._seg is None or "Pre" not in self._seg:
+            self._seg = dict(Qs=[self._new_state() for _ in range(k + 1)], Q1s=[self._new_state() for _ in range(k)],
+                             Pre=[self._new_state() for _ in range(k)])
+            self._lam = [self._new_state() for _ in range(3)]
+        seg = self._seg
+        lam, L1, X = self._lam
+        with torch.cuda.stream(s):
+            lam.zero_()
+            self.J.zero_()
+            xs = [torch.empty_like(self.x0) for _ in range(k + 1)] if self.n_drifters else None
+            lx = (2.0 * (self._final_x.double() - self.xobs.double())).float() * float(grad_J) \
+                if self.n_drifters else None
+        if self.n_obs:
+            self._misfit32(self._final, lam)
+        with torch.cuda.stream(s):
+            if grad_J != 1.0:
+                for a in lam.t:
+                    a.mul_(float(grad_J))
+        starts = list(range(0, n, k))
+        for si in reversed(range(len(starts))):
+            m = min(k, n - starts[si])
+            with torch.cuda.stream(s):
+                seg["Qs"][0].copy_from(self._ckpts[si])
+                if xs is not None:
+                    xs[0].copy_(self._xck[si])
+            for jj in range(m):
+                if xs is not None:
+                    with torch.cuda.stream(s):
+                        xs[jj + 1].copy_(xs[jj])
+                self._step_generic32(seg["Qs"][jj], seg["Q1s"][jj], seg["Pre"][jj], seg["Qs"][jj + 1],
+                                     starts[si] + jj, None if xs is None else xs[jj + 1])
+            for jj in reversed(range(m)):
+                j = starts[si] + jj
+                q, q1, pre, q2 = seg["Qs"][jj], seg["Q1s"][jj], seg["Pre"][jj], seg["Qs"][jj + 1]
+                if xs is not None:
+                    L.check(L.lib().gpo_drift_adj_f32(
+                        self.nx, self.ny, self.dx, self.dy, float(self.dt), *q2.ptrs(), q2.pitch, _vp(self.Bm),
+                        self.Bm.pitch, self.bc, self.n_drifters, _vp(xs[jj]), _vp(lx), *lam.ptrs(), 1.0,
+                        self._sh()), "gpo_drift_adj_f32")
+                self._frs_scale(lam)
+                if self.gen is not None:
+                    c = self._coarse(j)
+                    L.check(L.lib().gpo_balance_adj_f32(
+                        ctypes.byref(self.params), self.g32, self.f32, self.beta32, self._fc_ptr(),
+                        0 if self.fc is None else self.fc.pitch, _vp(c), 4 * (self.nx + 4), *lam.ptrs(), lam.pitch,
+                        _vp(self.Hi), self.Hi.pitch, self._sh()), "gpo_balance_adj_f32")
+                with torch.cuda.stream(s):
+                    X.zero_()
+                self._adj32(1, q1, pre, lam, L1, X, 0)
+                self._frs_scale(L1)
+                self._adj32(0, q, q1, L1, X, None, 1)
+                lam, X = X, lam
+        self._lam = [lam, L1, X]
+        s.synchronize()
+        return [a.clone() for a in lam.views()]
+
+    # FP64 counterparts (the verification build)
+    def _init_generic64(self):
+        self._init_f64()
+        if self.n_drifters and "Bm" not in self._f64:
+            with torch.cuda.stream(self.stream):
+                self._f64["Bm"] = self.Bm.buf.double()
+
+    def _balance64(self, st, j, d=None):
+        dd = self._f64
+        c = self._coarse(j)
+        L.check(L.lib().gpo_balance_f64(
+            ctypes.byref(self.params), self.g32, self.f32, self.beta32, self._fc_ptr(),
+            0 if self.fc is None else self.fc.pitch, _vp(c), 4 * (self.nx + 4), *st.ptrs(),
+            *(d.ptrs() if d is not None else [None] * 3), st.pitch, _vp(dd["Hi"]), dd["Hi_pitch"], self._sh()),
+            "gpo_balance_f64")
+
+    def _drift64(self, st, x, dx=None, dst=None):
+        Bm = self._f64["Bm"]
+        L.check(L.lib().gpo_drift_f64(
+            self.nx, self.ny, self.dx, self.dy, float(self.dt), *st.ptrs(), st.pitch, _vp(Bm), 8 * Bm.shape[1],
+            self.bc, self.n_drifters, _vp(x), None if dx is None else _vp(dx),
+            *(dst.ptrs() if dst is not None else [None] * 3), 1.0, self._sh()), "gpo_drift_f64")
+
+    def _trajectory_generic64(self, eta0, hu0, hv0, n_steps=None):
+        """FP64 run kept whole: Qs (states after each step), Q1s (stages),
+        Pre (second-substep outputs before model error) and Xs (drifter
+        positions before each step, and the final ones)."""
+        self._init_generic64()
+        self._enter()
+        s = self.stream
+        n = self.n_steps if n_steps is None else n_steps
+        Qs = [self._new_state(torch.float64) for _ in range(n + 1)]
+        Q1s = [self._new_state(torch.float64) for _ in range(n)]
+        Pre = [self._new_state(torch.float64) for _ in range(n)]
+        with torch.cuda.stream(s):
+            Qs[0].load(eta0, hu0, hv0)
+            Xs = [self.x0.double().clone()] if self.n_drifters else None
+        for j in range(n):
+            self._substep64(Qs[j], Q1s[j], None, 0)
+            self._frs_apply64(Q1s[j])
+            self._substep64(Q1s[j], Pre[j], Qs[j], 1)
+            with torch.cuda.stream(s):
+                Qs[j + 1].copy_from(Pre[j])
+            if self.gen is not None:
+                self._balance64(Qs[j + 1], j)
+            self._frs_apply64(Qs[j + 1])
+            if Xs is not None:
+                with torch.cuda.stream(s):
+                    Xs.append(Xs[-1].clone())
+                self._drift64(Qs[j + 1], Xs[-1])
+        self._leave()
+        return dict(Qs=Qs, Q1s=Q1s, Pre=Pre, Xs=Xs)
+
+    def _j_generic64(self, tr):
+        with torch.cuda.stream(self.stream):
+            self.J.zero_()
+        if self.n_obs:
+            self._misfit64(tr["Qs"][-1])
+        self.stream.synchronize()
+        J = float(self.J.item())
+        if self.n_drifters:
+            d = tr["Xs"][-1] - self.xobs.double()
+            J += float((d * d).sum().item())
+        return J
+
+    def _misfit_generic64(self, eta0, hu0, hv0):
+        return self._j_generic64(self._trajectory_generic64(eta0, hu0, hv0))
+
+    def _final_sens64(self, tr):
+        """dJ/d(final state) and dJ/d(final positions)."""
+        lam = self._new_state(torch.float64)
+        with torch.cuda.stream(self.stream):
+            self.J.zero_()
+        if self.n_obs:
+            self._misfit64(tr["Qs"][-1], lam)
+        lx = None
+        if self.n_drifters:
+            with torch.cuda.stream(self.stream):
+                lx = 2.0 * (tr["Xs"][-1] - self.xobs.double())
+        return lam, lx
+
+    def _adjoint_generic64(self, tr, lam_final, lx_final):
+        """(lambda_0, lambda_x0) = transposed sensitivities of the FP64 run."""
+        Qs, Q1s, Pre, Xs = tr["Qs"], tr["Q1s"], tr["Pre"], tr["Xs"]
+        s = self.stream
+        self._enter()
+        lam, L1, X = (self._new_state(torch.float64) for _ in range(3))
+        with torch.cuda.stream(s):
+            lam.copy_from(lam_final)
+            lx = lx_final.clone() if lx_final is not None else None
+        for j in reversed(range(len(Q1s))):
+            if lx is not None:
+                Bm = self._f64["Bm"]
+                L.check(L.lib().gpo_drift_adj_f64(
+                    self.nx, self.ny, self.dx, self.dy, float(self.dt), *Qs[j + 1].ptrs(), Qs[j + 1].pitch, _vp(Bm),
+                    8 * Bm.shape[1], self.bc, self.n_drifters, _vp(Xs[j]), _vp(lx), *lam.ptrs(), 1.0, self._sh()),
+                    "gpo_drift_adj_f64")
+            self._frs_scale(lam, f64=True)
+            if self.gen is not None:
+                c = self._coarse(j)
+                dd = self._f64
+                L.check(L.lib().gpo_balance_adj_f64(
+                    ctypes.byref(self.params), self.g32, self.f32, self.beta32, self._fc_ptr(),
+                    0 if self.fc is None else self.fc.pitch, _vp(c), 4 * (self.nx + 4), *lam.ptrs(), lam.pitch,
+                    _vp(dd["Hi"]), dd["Hi_pitch"], self._sh()), "gpo_balance_adj_f64")
+            with torch.cuda.stream(s):
+                X.zero_()
+            self._adj64(1, Q1s[j], Pre[j], lam, L1, X, 0)
+            self._frs_scale(L1, f64=True)
+            self._adj64(0, Qs[j], Q1s[j], L1, X, None, 1)
+            lam, X = X, lam
+        self._leave()
+        return lam, lx
+
+    def _tangent_generic64(self, tr, d0, dx0):
+        """(dQ^n, dx^n) for initial perturbations (d0, dx0) along the FP64 run."""
+        Qs, Q1s, Pre, Xs = tr["Qs"], tr["Q1s"], tr["Pre"], tr["Xs"]
+        s = self.stream
+        self._enter()
+        out = self._new_state(torch.float64)
+        val = self._new_state(torch.float64)
+        d = d0
+        with torch.cuda.stream(s):
+            dx = dx0.clone() if dx0 is not None else None
+        for j in range(len(Q1s)):
+            d1, dn = self._new_state(torch.float64), self._new_state(torch.float64)
+            self._tlm64(Qs[j], d, out, d1, None, None, 0)
+            self._frs_scale(d1, f64=True)
+            self._tlm64(Q1s[j], d1, out, dn, Qs[j], d, 1)
+            if self.gen is not None:
+                with torch.cuda.stream(s):
+                    val.copy_from(Pre[j])
+                self._balance64(val, j, dn)
+            self._frs_scale(dn, f64=True)
+            if dx is not None:
+                with torch.cuda.stream(s):
+                    xv = Xs[j].clone()
+                self._drift64(Qs[j + 1], xv, dx, dn)
+            d = dn
+        self._leave()
+        return d, dx
 
 
 class CDKLM16Misfit(torch.autograd.Function):

@@ -339,6 +339,184 @@ __global__ void balance_rev(Geo g, double beta, const float* f_cells, int f_pitc
   l_e[k] = l_e[k] - gc * ddy * l_u[k] + gc * ddx * l_v[k];
 }
 
+// geostrophicBalance (gpu_kernels/ocean_noise.cu:256-390) on a state of type
+// T, the verification counterpart of the product's FP32 balance, with its
+// tangent (d* may be NULL): eta += d, hu -= g/f (d + H + eta) d_y d,
+// hv += g/f (d + H + eta) d_x d with the eta before the update; a cell with a
+// land corner is set to 0.  The coriolis value is formed in FP32 as the
+// product forms it.
+template <class T>
+__global__ void balance_fwd(Geo g, double beta, const float* f_cells, int f_pitch, const float* coarse, int c_pitch,
+                            const T* Hi, float land, T* e, T* u, T* v, T* de, T* du, T* dv) {
+  const int ti = blockIdx.x * blockDim.x + threadIdx.x + 2, tj = blockIdx.y * blockDim.y + threadIdx.y + 2;
+  if (ti > g.nx + 1 || tj > g.ny + 1) return;
+  const size_t k = (size_t)tj * g.sp + ti;
+  const T h00 = corner(g, Hi, tj, ti), h01 = corner(g, Hi, tj, ti + 1);
+  const T h10 = corner(g, Hi, tj + 1, ti), h11 = corner(g, Hi, tj + 1, ti + 1);
+  if (h00 == T(land) || h01 == T(land) || h10 == T(land) || h11 == T(land)) {
+    e[k] = T(0), u[k] = T(0), v[k] = T(0);
+    if (de) de[k] = T(0), du[k] = T(0), dv[k] = T(0);
+    return;
+  }
+  float cor = (float)g.f + (float)beta * (((float)ti + 0.5f) * (float)g.dx * (float)g.north_x +
+                                          ((float)tj + 0.5f) * (float)g.dy * (float)g.north_y);
+  if ((float)g.f == 0.0f) cor = f_cells ? ((const float*)((const char*)f_cells + (size_t)f_pitch * tj))[ti] : 0.0f;
+  auto crow = [&](int j) { return (const float*)((const char*)coarse + (size_t)c_pitch * j); };
+  const T d = T(crow(tj)[ti]);
+  const T ddx = T(crow(tj)[ti + 1] - crow(tj)[ti - 1]) / (T(2) * T(g.dx));
+  const T ddy = T(crow(tj + 1)[ti] - crow(tj - 1)[ti]) / (T(2) * T(g.dy));
+  const T gc = T(g.g) / T(cor);
+  const T H_mid = T(0.25) * (h00 + h01 + h10 + h11);
+  const T h_mid = d + H_mid + e[k];
+  u[k] = u[k] - gc * h_mid * ddy;
+  v[k] = v[k] + gc * h_mid * ddx;
+  e[k] = e[k] + d;
+  if (de) {
+    du[k] = du[k] - gc * ddy * de[k];
+    dv[k] = dv[k] + gc * ddx * de[k];
+  }
+}
+
+// ----------------------------------------------------------------------------- drifters
+
+// Where a drifter reads the state (passiveDrifterKernel, gpu_kernels/
+// driftKernels.cu:74-183, reference cell 2): the lower-left of the four cell
+// centres around it and its position between them, or !ok when the four
+// cells are not all inside the ghost-inclusive arrays (it then stays put).
+template <class T>
+struct DriftAt {
+  bool ok;
+  int cx0, cy0;
+  T xf, yf;
+};
+template <class T>
+__device__ DriftAt<T> drift_at(int nx, int ny, T dx, T dy, T px, T py) {
+  DriftAt<T> a{false, 0, 0, T(0), T(0)};
+  const T qx = px / dx, qy = py / dy;
+  const T fx = floor(qx), fy = floor(qy);
+  const T fcx = fx + T(2), fcy = fy + T(2);
+  if (!(fcx >= T(0) && fcx <= T(nx + 3) && fcy >= T(0) && fcy <= T(ny + 3))) return a;
+  const int cx = (int)fcx, cy = (int)fcy;
+  const T frx = qx - fx, fry = qy - fy;
+  a.cx0 = frx < T(0.5) ? cx - 1 : cx;
+  a.xf = frx < T(0.5) ? frx + T(0.5) : frx - T(0.5);
+  a.cy0 = fry < T(0.5) ? cy - 1 : cy;
+  a.yf = fry < T(0.5) ? fry + T(0.5) : fry - T(0.5);
+  a.ok = !(a.cx0 < 0 || a.cy0 < 0 || a.cx0 + 1 > nx + 3 || a.cy0 + 1 > ny + 3);
+  return a;
+}
+
+struct DriftGeo {
+  int nx, ny, sp, bp;  // state and Bm pitches in elements
+  int per_ns, per_ew;
+  int n;               // drifters (rows of the position array, 2 values each)
+  double dx, dy, dt, sens;
+};
+
+// Cell whose state a stencil cell reads: periodic ghosts hold copies of the
+// interior cells they wrap (the ghost-cell pass runs before the drift), so
+// they map there; other ghost cells keep stored constants (mapped = -1 row).
+__device__ inline int wrap_axis(int c, int n, int periodic) {
+  if (c >= 2 && c <= n + 1) return c;
+  if (!periodic) return -1;
+  return c < 2 ? c + n : c - n;
+}
+
+// Forward Euler drift of the positions in T with its tangent: d_pos and the
+// state tangent d_* (NULL: none).  The FP64 counterpart of the product drift.
+template <class T>
+__global__ void drift_fwd(DriftGeo q, const T* e, const T* hu, const T* hv, const T* Bm, T* pos, T* dpos,
+                          const T* de, const T* du, const T* dv) {
+  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= q.n) return;
+  const T px = pos[2 * d], py = pos[2 * d + 1];
+  const DriftAt<T> a = drift_at<T>(q.nx, q.ny, T(q.dx), T(q.dy), px, py);
+  if (!a.ok) return;
+  T uu[4], vv[4], hh[4], w[4], du_[4], dv_[4];
+  const T wq[4] = {(T(1) - a.xf) * (T(1) - a.yf), a.xf * (T(1) - a.yf), (T(1) - a.xf) * a.yf, a.xf * a.yf};
+  T u = T(0), v = T(0), tu = T(0), tv = T(0);
+  for (int c = 0; c < 4; ++c) {
+    const int cc = a.cx0 + (c & 1), rr = a.cy0 + (c >> 1);
+    const int mc = wrap_axis(cc, q.nx, q.per_ew), mr = wrap_axis(rr, q.ny, q.per_ns);
+    const bool dyn = mc >= 0 && mr >= 0;
+    const size_t ks = dyn ? (size_t)mr * q.sp + mc : (size_t)rr * q.sp + cc;
+    const T h = Bm[(size_t)rr * q.bp + cc] + e[ks];
+    uu[c] = hu[ks] / h;
+    vv[c] = hv[ks] / h;
+    hh[c] = h;
+    w[c] = wq[c];
+    u = u + w[c] * uu[c];
+    v = v + w[c] * vv[c];
+    if (dpos && dyn && de) {
+      const T dh = de[ks];
+      du_[c] = (du[ks] - uu[c] * dh) / h;
+      dv_[c] = (dv[ks] - vv[c] * dh) / h;
+    } else {
+      du_[c] = dv_[c] = T(0);
+    }
+  }
+  const T s_dt = T(q.sens) * T(q.dt);
+  T nx_ = px + T(q.sens) * u * T(q.dt), ny_ = py + T(q.sens) * v * T(q.dt);
+  if (dpos) {
+    const T dudx = ((T(1) - a.yf) * (uu[1] - uu[0]) + a.yf * (uu[3] - uu[2])) / T(q.dx);
+    const T dudy = ((T(1) - a.xf) * (uu[2] - uu[0]) + a.xf * (uu[3] - uu[1])) / T(q.dy);
+    const T dvdx = ((T(1) - a.yf) * (vv[1] - vv[0]) + a.yf * (vv[3] - vv[2])) / T(q.dx);
+    const T dvdy = ((T(1) - a.xf) * (vv[2] - vv[0]) + a.xf * (vv[3] - vv[1])) / T(q.dy);
+    T su = T(0), sv = T(0);
+    for (int c = 0; c < 4; ++c) su = su + w[c] * du_[c], sv = sv + w[c] * dv_[c];
+    const T dx0 = dpos[2 * d], dy0 = dpos[2 * d + 1];
+    dpos[2 * d] = dx0 + s_dt * (dudx * dx0 + dudy * dy0 + su);
+    dpos[2 * d + 1] = dy0 + s_dt * (dvdx * dx0 + dvdy * dy0 + sv);
+  }
+  const T Lx = T(q.nx) * T(q.dx), Ly = T(q.ny) * T(q.dy);
+  if (q.per_ew && nx_ < T(0)) nx_ = nx_ + Lx;
+  if (q.per_ew && nx_ > Lx) nx_ = nx_ - Lx;
+  if (q.per_ns && ny_ < T(0)) ny_ = ny_ + Ly;
+  if (q.per_ns && ny_ > Ly) ny_ = ny_ - Ly;
+  pos[2 * d] = nx_;
+  pos[2 * d + 1] = ny_;
+  (void)hh;
+}
+
+// Reverse of one drift step at positions pos (before the step) over the state
+// it read: lam_pos (in: at the new positions; out: at the old ones) and the
+// state sensitivities l_* (accumulated; cells whose values are stored
+// constants receive nothing).
+template <class T>
+__global__ void drift_rev(DriftGeo q, const T* e, const T* hu, const T* hv, const T* Bm, const T* pos, T* lpos,
+                          T* l_e, T* l_u, T* l_v) {
+  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= q.n) return;
+  const DriftAt<T> a = drift_at<T>(q.nx, q.ny, T(q.dx), T(q.dy), pos[2 * d], pos[2 * d + 1]);
+  if (!a.ok) return;  // the drifter stayed put: lam_pos passes through unchanged
+  const T s_dt = T(q.sens) * T(q.dt);
+  const T lx = lpos[2 * d], ly = lpos[2 * d + 1];
+  const T ga = s_dt * lx, gb = s_dt * ly;  // sensitivities of u and v
+  const T wq[4] = {(T(1) - a.xf) * (T(1) - a.yf), a.xf * (T(1) - a.yf), (T(1) - a.xf) * a.yf, a.xf * a.yf};
+  T uu[4], vv[4];
+  for (int c = 0; c < 4; ++c) {
+    const int cc = a.cx0 + (c & 1), rr = a.cy0 + (c >> 1);
+    const int mc = wrap_axis(cc, q.nx, q.per_ew), mr = wrap_axis(rr, q.ny, q.per_ns);
+    const bool dyn = mc >= 0 && mr >= 0;
+    const size_t ks = dyn ? (size_t)mr * q.sp + mc : (size_t)rr * q.sp + cc;
+    const T h = Bm[(size_t)rr * q.bp + cc] + e[ks];
+    uu[c] = hu[ks] / h;
+    vv[c] = hv[ks] / h;
+    if (dyn) {
+      const T a_u = ga * wq[c] / h, a_v = gb * wq[c] / h;
+      atomicAdd(&l_u[ks], a_u);
+      atomicAdd(&l_v[ks], a_v);
+      atomicAdd(&l_e[ks], -(a_u * uu[c] + a_v * vv[c]));
+    }
+  }
+  const T dudx = ((T(1) - a.yf) * (uu[1] - uu[0]) + a.yf * (uu[3] - uu[2])) / T(q.dx);
+  const T dudy = ((T(1) - a.xf) * (uu[2] - uu[0]) + a.xf * (uu[3] - uu[1])) / T(q.dy);
+  const T dvdx = ((T(1) - a.yf) * (vv[1] - vv[0]) + a.yf * (vv[3] - vv[2])) / T(q.dx);
+  const T dvdy = ((T(1) - a.xf) * (vv[2] - vv[0]) + a.xf * (vv[3] - vv[1])) / T(q.dy);
+  lpos[2 * d] = lx + ga * dudx + gb * dvdx;
+  lpos[2 * d + 1] = ly + ga * dudy + gb * dvdy;
+}
+
 // Observation of the cell-centre velocity (u, v) = (hu, hv)/(eta + Hm),
 // bilinear between the four cell centres around each position (x, y).
 // J += sum (u - u_obs)^2 + (v - v_obs)^2; lambda += dJ/d(eta, hu, hv).
@@ -864,6 +1042,95 @@ int gpo_balance_adj_f32(const gpo_cdklm16_params* p, float g_, float f, float be
   hipLaunchKernelGGL(balance_rev<float>, inner_grid(g), kBlock, 0, (hipStream_t)stream, g, (double)beta, f_cells,
                      f_pitch, coarse, coarse_pitch, Hi, p->land_value, lam_eta, lam_hu, lam_hv);
   return hip_check(hipGetLastError(), "gpo_balance_adj_f32 launch");
+}
+
+int gpo_balance_adj_f64(const gpo_cdklm16_params* p, double g_, double f, double beta, const float* f_cells,
+                        int32_t f_pitch, const float* coarse, int32_t coarse_pitch, double* lam_eta,
+                        double* lam_hu, double* lam_hv, int32_t pitch, const double* Hi, int32_t Hi_pitch,
+                        void* stream) {
+  if (!p || !coarse || !lam_eta || !lam_hu || !lam_hv || !Hi) return bad_arg("gpo_balance_adj_f64: bad arguments");
+  Geo g{};
+  g.nx = p->nx, g.ny = p->ny, g.W = p->nx + 4, g.sp = pitch / 8, g.hp = Hi_pitch / 8;
+  g.dx = p->dx, g.dy = p->dy, g.g = g_, g.f = f, g.north_x = p->north_x, g.north_y = p->north_y;
+  hipLaunchKernelGGL(balance_rev<double>, inner_grid(g), kBlock, 0, (hipStream_t)stream, g, beta, f_cells, f_pitch,
+                     coarse, coarse_pitch, Hi, p->land_value, lam_eta, lam_hu, lam_hv);
+  return hip_check(hipGetLastError(), "gpo_balance_adj_f64 launch");
+}
+
+int gpo_balance_f64(const gpo_cdklm16_params* p, double g_, double f, double beta, const float* f_cells,
+                    int32_t f_pitch, const float* coarse, int32_t coarse_pitch, double* eta, double* hu, double* hv,
+                    double* d_eta, double* d_hu, double* d_hv, int32_t pitch, const double* Hi, int32_t Hi_pitch,
+                    void* stream) {
+  if (!p || !coarse || !eta || !hu || !hv || !Hi || (d_eta && (!d_hu || !d_hv)))
+    return bad_arg("gpo_balance_f64: bad arguments");
+  Geo g{};
+  g.nx = p->nx, g.ny = p->ny, g.W = p->nx + 4, g.sp = pitch / 8, g.hp = Hi_pitch / 8;
+  g.dx = p->dx, g.dy = p->dy, g.g = g_, g.f = f, g.north_x = p->north_x, g.north_y = p->north_y;
+  hipLaunchKernelGGL(balance_fwd<double>, inner_grid(g), kBlock, 0, (hipStream_t)stream, g, beta, f_cells, f_pitch,
+                     coarse, coarse_pitch, Hi, p->land_value, eta, hu, hv, d_eta, d_hu, d_hv);
+  return hip_check(hipGetLastError(), "gpo_balance_f64 launch");
+}
+
+}  // extern "C"
+
+namespace {
+template <class T>
+int drift_geo(int32_t nx, int32_t ny, double dx, double dy, double dt, int32_t pitch, int32_t Bm_pitch,
+              uint32_t bc, int32_t n, double sens, DriftGeo* q) {
+  if (nx < 1 || ny < 1 || n < 0 || !(dx > 0) || !(dy > 0) || pitch % (int)sizeof(T) || Bm_pitch % (int)sizeof(T))
+    return bad_arg("drifter derivative: bad arguments");
+  q->nx = nx, q->ny = ny, q->sp = pitch / (int)sizeof(T), q->bp = Bm_pitch / (int)sizeof(T);
+  q->per_ns = ((bc >> 24) & 0xFF) == 2 && ((bc >> 16) & 0xFF) == 2;
+  q->per_ew = ((bc >> 8) & 0xFF) == 2 && (bc & 0xFF) == 2;
+  q->n = n, q->dx = dx, q->dy = dy, q->dt = dt, q->sens = sens;
+  return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int gpo_drift_f64(int32_t nx, int32_t ny, double dx, double dy, double dt, const double* eta, const double* hu,
+                  const double* hv, int32_t pitch, const double* Bm, int32_t Bm_pitch, uint32_t boundary_conditions,
+                  int32_t n, double* positions, double* d_positions, const double* d_eta, const double* d_hu,
+                  const double* d_hv, double sensitivity, void* stream) {
+  DriftGeo q;
+  int rc = drift_geo<double>(nx, ny, dx, dy, dt, pitch, Bm_pitch, boundary_conditions, n, sensitivity, &q);
+  if (rc) return rc;
+  if (!eta || !hu || !hv || !Bm || !positions) return bad_arg("gpo_drift_f64: bad arguments");
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(drift_fwd<double>, dim3((n + 63) / 64), dim3(64), 0, (hipStream_t)stream, q, eta, hu, hv, Bm,
+                     positions, d_positions, d_eta, d_hu, d_hv);
+  return hip_check(hipGetLastError(), "gpo_drift_f64 launch");
+}
+
+int gpo_drift_adj_f32(int32_t nx, int32_t ny, float dx, float dy, float dt, const float* eta, const float* hu,
+                      const float* hv, int32_t pitch, const float* Bm, int32_t Bm_pitch, uint32_t boundary_conditions,
+                      int32_t n, const float* positions, float* lam_positions, float* lam_eta, float* lam_hu,
+                      float* lam_hv, float sensitivity, void* stream) {
+  DriftGeo q;
+  int rc = drift_geo<float>(nx, ny, dx, dy, dt, pitch, Bm_pitch, boundary_conditions, n, sensitivity, &q);
+  if (rc) return rc;
+  if (!eta || !hu || !hv || !Bm || !positions || !lam_positions || !lam_eta || !lam_hu || !lam_hv)
+    return bad_arg("gpo_drift_adj_f32: bad arguments");
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(drift_rev<float>, dim3((n + 63) / 64), dim3(64), 0, (hipStream_t)stream, q, eta, hu, hv, Bm,
+                     positions, lam_positions, lam_eta, lam_hu, lam_hv);
+  return hip_check(hipGetLastError(), "gpo_drift_adj_f32 launch");
+}
+
+int gpo_drift_adj_f64(int32_t nx, int32_t ny, double dx, double dy, double dt, const double* eta, const double* hu,
+                      const double* hv, int32_t pitch, const double* Bm, int32_t Bm_pitch,
+                      uint32_t boundary_conditions, int32_t n, const double* positions, double* lam_positions,
+                      double* lam_eta, double* lam_hu, double* lam_hv, double sensitivity, void* stream) {
+  DriftGeo q;
+  int rc = drift_geo<double>(nx, ny, dx, dy, dt, pitch, Bm_pitch, boundary_conditions, n, sensitivity, &q);
+  if (rc) return rc;
+  if (!eta || !hu || !hv || !Bm || !positions || !lam_positions || !lam_eta || !lam_hu || !lam_hv)
+    return bad_arg("gpo_drift_adj_f64: bad arguments");
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(drift_rev<double>, dim3((n + 63) / 64), dim3(64), 0, (hipStream_t)stream, q, eta, hu, hv, Bm,
+                     positions, lam_positions, lam_eta, lam_hu, lam_hv);
+  return hip_check(hipGetLastError(), "gpo_drift_adj_f64 launch");
 }
 
 int gpo_point_misfit_f32(int32_t nx, int32_t ny, float dx, float dy, const float* eta, const float* hu,

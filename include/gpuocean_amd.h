@@ -596,6 +596,52 @@ int gpo_balance_adj_f32(const gpo_cdklm16_params* p, float g, float f, float bet
                         float* lam_hu, float* lam_hv, int32_t pitch, const float* Hi, int32_t Hi_pitch,
                         void* stream);
 
+/* The same in FP64 (the verification build).  Replaces nothing in the
+ * reference (no derivative there); it transposes the balance of
+ * gpu_kernels/ocean_noise.cu:256-390. */
+int gpo_balance_adj_f64(const gpo_cdklm16_params* p, double g, double f, double beta, const float* f_cells,
+                        int32_t f_pitch, const float* coarse, int32_t coarse_pitch, double* lam_eta,
+                        double* lam_hu, double* lam_hv, int32_t pitch, const double* Hi, int32_t Hi_pitch,
+                        void* stream);
+
+/* The geostrophic balance of a given coarse field added to an FP64 state, the
+ * verification counterpart of gpo_geostrophic_balance
+ * (gpu_kernels/ocean_noise.cu:256-390, interpolation factor 1), and with
+ * d_* (may be NULL) its tangent: d_hu -= g/f d_y(coarse) d_eta,
+ * d_hv += g/f d_x(coarse) d_eta; cells with a land corner are set to 0. */
+int gpo_balance_f64(const gpo_cdklm16_params* p, double g, double f, double beta, const float* f_cells,
+                    int32_t f_pitch, const float* coarse, int32_t coarse_pitch, double* eta, double* hu, double* hv,
+                    double* d_eta, double* d_hu, double* d_hv, int32_t pitch, const double* Hi, int32_t Hi_pitch,
+                    void* stream);
+
+/* Drifter advection in FP64 (the verification build of gpo_drift, which
+ * replaces passiveDrifterKernel, gpu_kernels/driftKernels.cu:74-183; no wind):
+ * n positions (x, y pairs, dense) move by dt * sensitivity * (u, v), (u, v) =
+ * (hu, hv)/(Bm + eta) bilinear between the cell centres around each position,
+ * periodic wrap on periodic axes; a drifter whose stencil leaves the arrays
+ * stays put.  Periodic ghost cells are read as the interior cells they copy.
+ * With d_positions (may be NULL) the tangent d_positions += dt sensitivity
+ * (du/dx d_pos + du/dstate d_state) is formed too (d_eta, d_hu, d_hv: the
+ * tangent of the state, or NULL for a fixed state). */
+int gpo_drift_f64(int32_t nx, int32_t ny, double dx, double dy, double dt, const double* eta, const double* hu,
+                  const double* hv, int32_t pitch, const double* Bm, int32_t Bm_pitch, uint32_t boundary_conditions,
+                  int32_t n, double* positions, double* d_positions, const double* d_eta, const double* d_hu,
+                  const double* d_hv, double sensitivity, void* stream);
+
+/* Reverse of one drift step from `positions` (before the step): lam_positions
+ * (n pairs) holds the sensitivity of the new positions on entry and of the old
+ * ones on return; lam_eta/hu/hv accumulate the sensitivity of the state the
+ * drift read (periodic ghost cells fold onto the interior cells they copy;
+ * other ghost cells are stored constants and receive nothing). */
+int gpo_drift_adj_f32(int32_t nx, int32_t ny, float dx, float dy, float dt, const float* eta, const float* hu,
+                      const float* hv, int32_t pitch, const float* Bm, int32_t Bm_pitch, uint32_t boundary_conditions,
+                      int32_t n, const float* positions, float* lam_positions, float* lam_eta, float* lam_hu,
+                      float* lam_hv, float sensitivity, void* stream);
+int gpo_drift_adj_f64(int32_t nx, int32_t ny, double dx, double dy, double dt, const double* eta, const double* hu,
+                      const double* hv, int32_t pitch, const double* Bm, int32_t Bm_pitch,
+                      uint32_t boundary_conditions, int32_t n, const double* positions, double* lam_positions,
+                      double* lam_eta, double* lam_hu, double* lam_hv, double sensitivity, void* stream);
+
 /* Point-observation misfit J += sum_d (u - u_obs)^2 + (v - v_obs)^2 with
  * (u, v) = (hu, hv)/(eta + Hm) interpolated bilinearly between the four cell
  * centres around positions[2d], positions[2d+1] (metres); observed holds
