@@ -781,9 +781,14 @@ class DifferentiableEnsemble:
     member with its own grid size, dt, observations and checkpoints.  The
     members share the number of steps and the checkpoint spacing."""
 
-    def __init__(self, sims, positions, observed, n_steps, checkpoint_every=20, dts=None):
+    def __init__(self, sims, positions, observed, n_steps, checkpoint_every=20, dts=None, model_error=False,
+                 drifter_positions=None, drifter_observed=None):
+        dpos = drifter_positions if drifter_positions is not None else [None] * len(sims)
+        dobs = drifter_observed if drifter_observed is not None else [None] * len(sims)
         self.models = [DifferentiableCDKLM16(s, p, o, n_steps, dt=None if dts is None else dts[i],
-                                             checkpoint_every=checkpoint_every, _workspace=False)
+                                             checkpoint_every=checkpoint_every, _workspace=False,
+                                             model_error=model_error, drifter_positions=dpos[i],
+                                             drifter_observed=dobs[i])
                        for i, (s, p, o) in enumerate(zip(sims, positions, observed))]
         self.n = len(self.models)
         self.n_steps = int(n_steps)
@@ -794,15 +799,22 @@ class DifferentiableEnsemble:
         lib = L.lib()
         table = (L.AdjMember * self.n)()
         self._bufs = []
+        self._pos = []
         for i, m in enumerate(self.models):
             P = _pitch_elems(m.nx + 4)
             rows = m.ny + 4
-            # one allocation per member: state, stage, checkpoints, k+1 states, k stages, 3 lambda states
-            n_states = 2 + self.n_seg + (self.k + 1) + self.k + 3
+            # one allocation per member: state, stage, checkpoints, k+1 states, k stages, 3 lambda states,
+            # and with model error k pre-noise states
+            n_states = 2 + self.n_seg + (self.k + 1) + self.k + 3 + (self.k if m.gen is not None else 0)
             with torch.cuda.stream(self.stream):
                 buf = torch.zeros((n_states, 3, rows, P), dtype=torch.float32, device=self.dev)
                 ws = torch.empty(lib.gpo_adjoint_workspace_bytes(m.nx, m.ny, 4), dtype=torch.uint8, device=self.dev)
+                # drifter positions: forward, checkpoints, segment, sensitivity
+                nd = m.n_drifters
+                pos = torch.zeros((1 + self.n_seg + self.k + 1 + 1, max(nd, 1), 2), dtype=torch.float32,
+                                  device=self.dev)
             self._bufs.append((buf, ws))
+            self._pos.append(pos)
             base = buf.data_ptr()
             st = 3 * rows * P * 4
             e = table[i]
@@ -823,6 +835,25 @@ class DifferentiableEnsemble:
             e.has_frs = int(m.frs is not None)
             if m.frs is not None:
                 e.frs = m.frs
+            if m.gen is not None:
+                g = m.gen
+                e.has_noise = 1
+                e.seed, e.stream_id, e.draw0 = g.seed, g.stream_id, m.draw0
+                W = g.weights(1.0).reshape(-1)
+                for q in range(25):
+                    e.soar_weights[q] = float(W[q])
+                e.g, e.f, e.beta = m.g32, m.f32, m.beta32
+                e.xi, (e.xi_rows, e.xi_cols) = g.random_numbers.data_ptr(), g.random_numbers.shape
+                e.coarse = g.coarse_buffer.data_ptr()
+                e.pre = base + (2 + self.n_seg + 2 * self.k + 1 + 3) * st
+            if m.n_drifters:
+                pb = pos.data_ptr()
+                sp = 2 * 4 * m.n_drifters
+                e.n_drifters = m.n_drifters
+                e.Bm, e.Bm_pitch = m.Bm.data_ptr(), m.Bm.pitch
+                e.pos, e.pos_ckpt = pb, pb + sp
+                e.pos_seg = pb + (1 + self.n_seg) * sp
+                e.lam_pos = pb + (1 + self.n_seg + self.k + 1) * sp
         self._pitch = [_pitch_elems(m.nx + 4) for m in self.models]
         h = ctypes.c_void_p()
         self.stream.synchronize()
@@ -862,6 +893,9 @@ class DifferentiableEnsemble:
             for i, st in enumerate(states):
                 for a, b in zip(self._state(i, 0), st):
                     a.copy_(b)
+                m = self.models[i]
+                if m.n_drifters:
+                    self._pos[i][0, :m.n_drifters].copy_(m.x0)
         for sg in range(self.n_seg):
             m = min(self.k, self.n_steps - sg * self.k)
             L.check(lib.gpo_adj_batch_checkpoint(self.handle, sg, 0, self._sh()), "gpo_adj_batch_checkpoint")
@@ -869,9 +903,20 @@ class DifferentiableEnsemble:
         with torch.cuda.stream(self.stream):
             self.J.zero_()
         for i in range(self.n):
-            self._misfit(i, 0)
+            if self.models[i].n_obs:
+                self._misfit(i, 0)
         self.stream.synchronize()
-        return self.J.cpu().numpy().copy()
+        J = self.J.cpu().numpy().copy()
+        for i, m in enumerate(self.models):
+            if m.n_drifters:
+                J[i] += m._drifter_j(self._pos[i][0, :m.n_drifters])
+        return J
+
+    def final_positions(self, i):
+        """Drifter positions of member i after the last forward()."""
+        m = self.models[i]
+        self.stream.synchronize()
+        return self._pos[i][0, :m.n_drifters].clone()
 
     def backward(self):
         """dJ_i/d(eta0, hu0, hv0) of every member i of the last forward(), as
@@ -882,8 +927,14 @@ class DifferentiableEnsemble:
             for buf, _ in self._bufs:
                 buf[lam_idx:lam_idx + 3].zero_()
             self.J.zero_()
+            for i, m in enumerate(self.models):
+                if m.n_drifters:
+                    x = self._pos[i][0, :m.n_drifters]
+                    self._pos[i][1 + self.n_seg + self.k + 1, :m.n_drifters].copy_(
+                        (2.0 * (x.double() - m.xobs.double())).float())
         for i in range(self.n):
-            self._misfit(i, 0, lam_idx)
+            if self.models[i].n_obs:
+                self._misfit(i, 0, lam_idx)
         for sg in reversed(range(self.n_seg)):
             m = min(self.k, self.n_steps - sg * self.k)
             L.check(lib.gpo_adj_batch_checkpoint(self.handle, sg, 1, self._sh()), "gpo_adj_batch_checkpoint")

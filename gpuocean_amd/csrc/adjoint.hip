@@ -14,6 +14,8 @@
 #include "adjoint_core.h"
 #include "boundary_cells.h"
 #include "cdklm_common.h"
+#include "noise_cells.h"
+#include "philox.h"
 
 namespace gpo {
 namespace adj {
@@ -315,9 +317,8 @@ __global__ void unpack_dual(Geo g, const Dual* in, double* v, double* d) {
 // hv += g/f (d + H + eta) d_x d, eta += d, so only eta feeds back:
 // lambda_eta += -g/f d_y d lambda_hu + g/f d_x d lambda_hv.
 template <class T>
-__global__ void balance_rev(Geo g, double beta, const float* f_cells, int f_pitch, const float* coarse, int c_pitch,
-                            const T* Hi, float land, T* l_e, T* l_u, T* l_v) {
-  const int ti = blockIdx.x * blockDim.x + threadIdx.x + 2, tj = blockIdx.y * blockDim.y + threadIdx.y + 2;
+__device__ void balance_rev_cell(const Geo& g, double beta, const float* f_cells, int f_pitch, const float* coarse,
+                                 int c_pitch, const T* Hi, float land, T* l_e, T* l_u, T* l_v, int ti, int tj) {
   if (ti > g.nx + 1 || tj > g.ny + 1) return;
   const T h00 = corner(g, Hi, tj, ti), h01 = corner(g, Hi, tj, ti + 1);
   const T h10 = corner(g, Hi, tj + 1, ti), h11 = corner(g, Hi, tj + 1, ti + 1);
@@ -337,6 +338,12 @@ __global__ void balance_rev(Geo g, double beta, const float* f_cells, int f_pitc
   const T gc = T(g.g) / T(cor);
   const size_t k = (size_t)tj * g.sp + ti;
   l_e[k] = l_e[k] - gc * ddy * l_u[k] + gc * ddx * l_v[k];
+}
+template <class T>
+__global__ void balance_rev(Geo g, double beta, const float* f_cells, int f_pitch, const float* coarse, int c_pitch,
+                            const T* Hi, float land, T* l_e, T* l_u, T* l_v) {
+  balance_rev_cell<T>(g, beta, f_cells, f_pitch, coarse, c_pitch, Hi, land, l_e, l_u, l_v,
+                      blockIdx.x * blockDim.x + threadIdx.x + 2, blockIdx.y * blockDim.y + threadIdx.y + 2);
 }
 
 // geostrophicBalance (gpu_kernels/ocean_noise.cu:256-390) on a state of type
@@ -483,9 +490,8 @@ __global__ void drift_fwd(DriftGeo q, const T* e, const T* hu, const T* hv, cons
 // state sensitivities l_* (accumulated; cells whose values are stored
 // constants receive nothing).
 template <class T>
-__global__ void drift_rev(DriftGeo q, const T* e, const T* hu, const T* hv, const T* Bm, const T* pos, T* lpos,
-                          T* l_e, T* l_u, T* l_v) {
-  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ void drift_rev_one(const DriftGeo& q, const T* e, const T* hu, const T* hv, const T* Bm, const T* pos,
+                              T* lpos, T* l_e, T* l_u, T* l_v, int d) {
   if (d >= q.n) return;
   const DriftAt<T> a = drift_at<T>(q.nx, q.ny, T(q.dx), T(q.dy), pos[2 * d], pos[2 * d + 1]);
   if (!a.ok) return;  // the drifter stayed put: lam_pos passes through unchanged
@@ -515,6 +521,168 @@ __global__ void drift_rev(DriftGeo q, const T* e, const T* hu, const T* hv, cons
   const T dvdy = ((T(1) - a.xf) * (vv[2] - vv[0]) + a.xf * (vv[3] - vv[1])) / T(q.dy);
   lpos[2 * d] = lx + ga * dudx + gb * dvdx;
   lpos[2 * d + 1] = ly + ga * dudy + gb * dvdy;
+}
+template <class T>
+__global__ void drift_rev(DriftGeo q, const T* e, const T* hu, const T* hv, const T* Bm, const T* pos, T* lpos,
+                          T* l_e, T* l_u, T* l_v) {
+  drift_rev_one<T>(q, e, hu, hv, Bm, pos, lpos, l_e, l_u, l_v, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// ---- batched model error and drifters of the differentiable ensemble
+
+// One member's model error (fixed normals) and drifters, with the buffers the
+// batched forward, trajectory and sweep address by step.
+struct NoiseDriftItem {
+  Geo g;                 // state pitch sp, Hi pitch hp (elements), dx, dy, g, f, north
+  int per_ns, per_ew;
+  int noise;
+  int xi_rows, xi_cols;
+  uint32_t k0, k1, stream_id, draw0;
+  float W[25];
+  float beta, land;
+  float *xi, *coarse;    // dense rows: xi_cols, nx + 4
+  const float* Hi;
+  float *state, *qs, *pre, *lam;  // forward state; segment states; pre-noise outputs; lambda buffers
+  size_t plane, st;      // elements per plane and per state
+  int n_drifters;
+  const float* Bm;
+  int bm_pitch;          // elements
+  float *pos, *pos_ckpt, *pos_seg, *lam_pos;
+  float dt;
+};
+
+__device__ inline float* st_plane(const NoiseDriftItem& m, float* base, size_t i, int q) {
+  return base + i * m.st + (size_t)q * m.plane;
+}
+
+// The normal field of draw draw0 + step (gpo_random_normal's values).
+__global__ void __launch_bounds__(256) nd_xi_b(const NoiseDriftItem* __restrict__ tab, int step) {
+  const NoiseDriftItem& m = tab[blockIdx.y];
+  const uint32_t n = (uint32_t)m.xi_rows * (uint32_t)m.xi_cols;
+  const uint32_t e = blockIdx.x * 256u + threadIdx.x;
+  if (!m.noise || e >= n) return;
+  m.xi[e] = normal_at(e, m.draw0 + (uint32_t)step, m.stream_id, m.k0, m.k1);
+}
+
+// Its SOAR field over the grid and two ghost rings (gpo_soar, scale 1).
+__global__ void __launch_bounds__(256) nd_soar_b(const NoiseDriftItem* __restrict__ tab) {
+  const NoiseDriftItem& m = tab[blockIdx.y];
+  const int W = m.g.nx + 4, n = W * (m.g.ny + 4);
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (!m.noise || k >= n) return;
+  const int tj = k / W, ti = k - tj * W;
+  m.coarse[k] = 1.0f * soar_cell(m.g.nx, m.g.ny, m.W, m.per_ns, m.per_ew, m.xi, 4 * m.xi_cols, tj, ti);
+}
+
+// The balanced perturbation added to a state (gpo_geostrophic_balance's
+// arithmetic): the forward state (which < 0) or segment state `which`; with
+// keep >= 0 the state is first copied to pre-noise buffer `keep`.
+__global__ void __launch_bounds__(256) nd_balance_b(const NoiseDriftItem* __restrict__ tab, int which, int keep) {
+  const NoiseDriftItem& m = tab[blockIdx.z];
+  const int ti = blockIdx.x * 64 + (threadIdx.x & 63) + 2, tj = blockIdx.y * 4 + (threadIdx.x >> 6) + 2;
+  if (!m.noise || ti > m.g.nx + 1 || tj > m.g.ny + 1) return;
+  float* base = which < 0 ? m.state : m.qs + (size_t)which * m.st;
+  float *e = base, *u = base + m.plane, *v = base + 2 * m.plane;
+  const size_t k = (size_t)tj * m.g.sp + ti;
+  if (keep >= 0) {
+    float* p = m.pre + (size_t)keep * m.st;
+    p[k] = e[k], p[m.plane + k] = u[k], p[2 * m.plane + k] = v[k];
+  }
+  balance_cell((float)m.g.dx, (float)m.g.dy, (float)m.g.g, (float)m.g.f, m.beta, (float)m.g.north_x,
+               (float)m.g.north_y, nullptr, 0, m.coarse, 4 * (m.g.nx + 4), e, u, v, 4 * m.g.sp, m.Hi, 4 * m.g.hp,
+               m.land, tj, ti);
+}
+
+// Drift of the forward positions over the forward state (which < 0), or of
+// segment positions which-1 into `which` over segment state `which`, with
+// gpo_drift's arithmetic; periodic ghost cells are read as the interior cells
+// they copy (the values the ghost-cell pass would write).
+__global__ void __launch_bounds__(64) nd_drift_b(const NoiseDriftItem* __restrict__ tab, int which) {
+  const NoiseDriftItem& m = tab[blockIdx.y];
+  const int d = blockIdx.x * 64 + threadIdx.x;
+  if (d >= m.n_drifters) return;
+  const float* base = which < 0 ? m.state : m.qs + (size_t)which * m.st;
+  const float *eta = base, *hu = base + m.plane, *hv = base + 2 * m.plane;
+  float* out = which < 0 ? m.pos : m.pos_seg + (size_t)which * 2 * m.n_drifters;
+  const float* in = which < 0 ? m.pos : m.pos_seg + (size_t)(which - 1) * 2 * m.n_drifters;
+  float px = in[2 * d], py = in[2 * d + 1];
+  const int nx = m.g.nx, ny = m.g.ny;
+  const float dx = (float)m.g.dx, dy = (float)m.g.dy;
+  const float qx = px / dx, qy = py / dy;
+  const float fcx = floorf(qx) + 2.0f, fcy = floorf(qy) + 2.0f;
+  bool ok = fcx >= 0.0f && fcx <= (float)(nx + 3) && fcy >= 0.0f && fcy <= (float)(ny + 3);
+  if (ok) {
+    const int cx = (int)fcx, cy = (int)fcy;
+    const float frx = qx - floorf(qx), fry = qy - floorf(qy);
+    const int cx0 = frx < 0.5f ? cx - 1 : cx;
+    const float xf = frx < 0.5f ? frx + 0.5f : frx - 0.5f;
+    const int cy0 = fry < 0.5f ? cy - 1 : cy;
+    const float yf = fry < 0.5f ? fry + 0.5f : fry - 0.5f;
+    ok = !(cx0 < 0 || cy0 < 0 || cx0 + 1 > nx + 3 || cy0 + 1 > ny + 3);
+    if (ok) {
+      auto vel = [&](const float* mom, int c, int r) {
+        const int mc = wrap_axis(c, nx, m.per_ew), mr = wrap_axis(r, ny, m.per_ns);
+        const size_t ks = (mc >= 0 && mr >= 0) ? (size_t)mr * m.g.sp + mc : (size_t)r * m.g.sp + c;
+        const float h = m.Bm[(size_t)r * m.bm_pitch + c] + eta[ks];
+        return mom[ks] / h;
+      };
+      const float u00 = vel(hu, cx0, cy0), u10 = vel(hu, cx0 + 1, cy0);
+      const float u01 = vel(hu, cx0, cy0 + 1), u11 = vel(hu, cx0 + 1, cy0 + 1);
+      const float v00 = vel(hv, cx0, cy0), v10 = vel(hv, cx0 + 1, cy0);
+      const float v01 = vel(hv, cx0, cy0 + 1), v11 = vel(hv, cx0 + 1, cy0 + 1);
+      const float uy0 = (1.0f - xf) * u00 + xf * u10;
+      const float uy1 = (1.0f - xf) * u01 + xf * u11;
+      const float vy0 = (1.0f - xf) * v00 + xf * v10;
+      const float vy1 = (1.0f - xf) * v01 + xf * v11;
+      const float u = (1.0f - yf) * uy0 + yf * uy1;
+      const float v = (1.0f - yf) * vy0 + yf * vy1;
+      px += 1.0f * u * m.dt;
+      py += 1.0f * v * m.dt;
+      const float Lx = (float)nx * dx, Ly = (float)ny * dy;
+      if (m.per_ew && px < 0.0f) px += Lx;
+      if (m.per_ew && px > Lx) px -= Lx;
+      if (m.per_ns && py < 0.0f) py += Ly;
+      if (m.per_ns && py > Ly) py -= Ly;
+    }
+  }
+  out[2 * d] = px;
+  out[2 * d + 1] = py;
+}
+
+// Reverse of the drift of step j of the swept segment: positions pos_seg[j],
+// state qs[j+1], lambda buffer (j+1) & 1.
+__global__ void __launch_bounds__(64) nd_drift_rev_b(const NoiseDriftItem* __restrict__ tab, int j) {
+  const NoiseDriftItem& m = tab[blockIdx.y];
+  if (m.n_drifters == 0) return;
+  const DriftGeo q{m.g.nx, m.g.ny, m.g.sp, m.bm_pitch, m.per_ns, m.per_ew, m.n_drifters, m.g.dx, m.g.dy,
+                   (double)m.dt, 1.0};
+  const float* s = m.qs + (size_t)(j + 1) * m.st;
+  float* l = m.lam + (size_t)((j + 1) & 1) * m.st;
+  drift_rev_one<float>(q, s, s + m.plane, s + 2 * m.plane, m.Bm, m.pos_seg + (size_t)j * 2 * m.n_drifters,
+                       m.lam_pos, l, l + m.plane, l + 2 * m.plane, blockIdx.x * 64 + threadIdx.x);
+}
+
+// Reverse of the model error of step j on lambda buffer (j+1) & 1 (the
+// member's coarse field holds that step's SOAR field).
+__global__ void __launch_bounds__(256) nd_balance_rev_b(const NoiseDriftItem* __restrict__ tab, int j) {
+  const NoiseDriftItem& m = tab[blockIdx.z];
+  if (!m.noise) return;
+  float* l = m.lam + (size_t)((j + 1) & 1) * m.st;
+  Geo g = m.g;
+  balance_rev_cell<float>(g, (double)m.beta, nullptr, 0, m.coarse, 4 * (m.g.nx + 4), m.Hi, m.land, l, l + m.plane,
+                          l + 2 * m.plane, blockIdx.x * 64 + (threadIdx.x & 63) + 2,
+                          blockIdx.y * 4 + (threadIdx.x >> 6) + 2);
+}
+
+// Drifter positions: mode 0 forward -> checkpoint seg, mode 1 checkpoint seg ->
+// segment start.
+__global__ void __launch_bounds__(64) nd_pos_copy_b(const NoiseDriftItem* __restrict__ tab, int mode, int seg) {
+  const NoiseDriftItem& m = tab[blockIdx.y];
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  if (i >= 2 * m.n_drifters) return;
+  float* ck = m.pos_ckpt + (size_t)seg * 2 * m.n_drifters;
+  if (mode == 0) ck[i] = m.pos[i];
+  else m.pos_seg[i] = ck[i];
 }
 
 // Observation of the cell-centre velocity (u, v) = (hu, hv)/(eta + Hm),
@@ -698,13 +866,35 @@ struct AdjBatch {
   // stage, the forward state, then per step j the stage and the new state
   cells::BcTarget* d_bc = nullptr;   // (2 + 2k) x n_frs
   int n_frs = 0, bc_blocks = 0;
+  // model error and drifters (per member, n items)
+  NoiseDriftItem* d_nd = nullptr;
+  int any_noise = 0, any_drift = 0, max_xi = 0, max_cells = 0, max_drifters = 0;
+  int step = 0;       // step index of the forward run (the draw offset of its model error)
+  int seg_start = 0;  // first step of the segment being recomputed and swept
 };
 
 void adj_batch_free(AdjBatch* b) {
   for (void* p : {(void*)b->d_fwd, (void*)b->d_traj, (void*)b->d_work, (void*)b->d_items, (void*)b->d_copy,
-                  (void*)b->d_bc})
+                  (void*)b->d_bc, (void*)b->d_nd})
     if (p) (void)hipFree(p);
   delete b;
+}
+
+// The model error of step `step` added to the forward state (which < 0) or to
+// segment state `which` (its pre-noise copy kept in `keep`).
+int adj_batch_noise(AdjBatch* b, int step, int which, int keep, hipStream_t s) {
+  if (!b->any_noise) return 0;
+  hipLaunchKernelGGL(nd_xi_b, dim3((b->max_xi + 255) / 256, b->n), dim3(256), 0, s, b->d_nd, step);
+  hipLaunchKernelGGL(nd_soar_b, dim3((b->max_cells + 255) / 256, b->n), dim3(256), 0, s, b->d_nd);
+  hipLaunchKernelGGL(nd_balance_b, dim3((b->max_w - 4 + 63) / 64, (b->max_h - 4 + 3) / 4, b->n), dim3(256), 0, s,
+                     b->d_nd, which, keep);
+  return hip_check(hipGetLastError(), "batched model error launch");
+}
+
+int adj_batch_drift(AdjBatch* b, int which, hipStream_t s) {
+  if (!b->any_drift) return 0;
+  hipLaunchKernelGGL(nd_drift_b, dim3((b->max_drifters + 63) / 64, b->n), dim3(64), 0, s, b->d_nd, which);
+  return hip_check(hipGetLastError(), "batched drifter launch");
 }
 
 int adj_batch_copy(AdjBatch* b, const CopyItem* tab, hipStream_t s) {
@@ -730,9 +920,9 @@ int adj_batch_bc(AdjBatch* b, int slot, hipStream_t s) {
   return hip_check(hipGetLastError(), "batched boundary launch");
 }
 
-int adj_batch_stage(AdjBatch* b, const AdjItem<float>* tab, int* flag, hipStream_t s) {
+int adj_batch_stage(AdjBatch* b, const AdjItem<float>* tab, int* flag, hipStream_t s, bool scale_first = true) {
   const int n = b->n;
-  if (b->n_frs)
+  if (b->n_frs && scale_first)
     hipLaunchKernelGGL(frs_scale_b<float>, dim3((b->max_w + 63) / 64, (b->max_h + 3) / 4, n), kBlock, 0, s, tab);
   hipLaunchKernelGGL(f1_convert_b<float>, dim3((b->max_w + 63) / 64, (b->max_h + 3) / 4, n), kBlock, 0, s, tab,
                      flag);
@@ -1222,6 +1412,7 @@ int gpo_adj_batch_create(const gpo_adj_member* members, int32_t n, int32_t k, in
   std::vector<AdjItem<float>> items(2 * (size_t)n * k);
   std::vector<CopyItem> copies((2 * (size_t)n_segments + 1) * n);
   std::vector<int2> work;
+  std::vector<NoiseDriftItem> nds;
   std::vector<std::vector<cells::BcTarget>> bcs(2 + 2 * (size_t)k);  // per slot, one per flow-relaxation member
   long long max_bc_cells = 0;
   for (int m = 0; m < n; ++m) {
@@ -1319,7 +1510,8 @@ int gpo_adj_batch_create(const gpo_adj_member* members, int32_t n, int32_t k, in
         it.le = gptr(lo), it.lu = gptr(lo + pl), it.lv = gptr(lo + 2 * pl);
         it.lie = gptr(li), it.liu = gptr(li + pl), it.liv = gptr(li + 2 * pl);
         if (rk) {
-          it.ie = gptr(be), it.iu = gptr(bu), it.iv = gptr(bv), it.out_e = gptr(ce);
+          // the second substep's own output: before the model error when there is one
+          it.ie = gptr(be), it.iu = gptr(bu), it.iv = gptr(bv), it.out_e = gptr(M.has_noise ? M.pre + j * st : ce);
           float* lq = buf[j & 1];
           it.lqe = gptr(lq), it.lqu = gptr(lq + pl), it.lqv = gptr(lq + 2 * pl);
           it.accumulate = 0, it.lq_mode = 1;
@@ -1348,6 +1540,46 @@ int gpo_adj_batch_create(const gpo_adj_member* members, int32_t n, int32_t k, in
       for (int sx = 0; sx < a.n_strips; ++sx) work.push_back(make_int2(m, sx | (sy << 16)));
     b->max_w = std::max(b->max_w, p.nx + 4);
     b->max_h = std::max(b->max_h, p.ny + 4);
+    // model error and drifters
+    NoiseDriftItem nd{};
+    nd.g = g;
+    nd.per_ns = g.bc_n == 2 && g.bc_s == 2;
+    nd.per_ew = g.bc_e == 2 && g.bc_w == 2;
+    nd.Hi = M.Hi;
+    nd.state = M.state, nd.qs = M.qs, nd.pre = M.pre, nd.lam = M.lam;
+    nd.plane = pl, nd.st = st;
+    nd.dt = M.dt;
+    nd.land = p.land_value;
+    if (M.has_noise) {
+      if (!M.xi || !M.coarse || !M.pre || M.xi_rows != (nd.per_ns ? p.ny : p.ny + 8) ||
+          M.xi_cols != (nd.per_ew ? p.nx : p.nx + 8) || p.f_is_array || M.f_cells) {
+        adj_batch_free(b);
+        return bad_arg("gpo_adj_batch_create: model-error buffers (xi, coarse, pre) missing or mis-sized, or f not "
+                       "scalar");
+      }
+      nd.noise = 1;
+      nd.xi_rows = M.xi_rows, nd.xi_cols = M.xi_cols;
+      nd.k0 = (uint32_t)(M.seed & 0xffffffffu), nd.k1 = (uint32_t)(M.seed >> 32);
+      nd.stream_id = M.stream_id, nd.draw0 = M.draw0;
+      for (int i = 0; i < 25; ++i) nd.W[i] = M.soar_weights[i];
+      nd.g.g = M.g, nd.g.f = M.f, nd.beta = M.beta;
+      nd.xi = M.xi, nd.coarse = M.coarse;
+      b->any_noise = 1;
+      b->max_xi = std::max(b->max_xi, M.xi_rows * M.xi_cols);
+      b->max_cells = std::max(b->max_cells, (p.nx + 4) * (p.ny + 4));
+    }
+    if (M.n_drifters > 0) {
+      if (!M.Bm || !M.pos || !M.pos_ckpt || !M.pos_seg || !M.lam_pos || M.Bm_pitch % 4) {
+        adj_batch_free(b);
+        return bad_arg("gpo_adj_batch_create: drifter buffers missing");
+      }
+      nd.n_drifters = M.n_drifters;
+      nd.Bm = M.Bm, nd.bm_pitch = M.Bm_pitch / 4;
+      nd.pos = M.pos, nd.pos_ckpt = M.pos_ckpt, nd.pos_seg = M.pos_seg, nd.lam_pos = M.lam_pos;
+      b->any_drift = 1;
+      b->max_drifters = std::max(b->max_drifters, M.n_drifters);
+    }
+    nds.push_back(nd);
   }
   b->n_work = (int)work.size();
   const long long waves = std::min<long long>((long long)persistent_waves(8), b->n_work);
@@ -1386,6 +1618,13 @@ int gpo_adj_batch_create(const gpo_adj_member* members, int32_t n, int32_t k, in
                                     hipMemcpyHostToDevice, s),
                      "gpo_adj_batch_create copy");
   }
+  if (!rc && (b->any_noise || b->any_drift)) {
+    rc = hip_check(hipMalloc(&b->d_nd, sizeof(NoiseDriftItem) * nds.size()), "gpo_adj_batch_create hipMalloc");
+    if (!rc)
+      rc = hip_check(hipMemcpyAsync(b->d_nd, nds.data(), sizeof(NoiseDriftItem) * nds.size(), hipMemcpyHostToDevice,
+                                    s),
+                     "gpo_adj_batch_create copy");
+  }
   if (!rc) rc = hip_check(hipStreamSynchronize(s), "gpo_adj_batch_create sync");
   if (rc) {
     adj_batch_free(b);
@@ -1403,8 +1642,11 @@ int gpo_adj_batch_forward(gpo_adj_batch h, int32_t n_steps, void* stream) {
     int rc = adj_batch_steps(b, b->d_fwd, 0, s);
     if (!rc) rc = adj_batch_bc(b, 0, s);
     if (!rc) rc = adj_batch_steps(b, b->d_fwd, 1, s);
+    if (!rc) rc = adj_batch_noise(b, b->step, -1, -1, s);
     if (!rc) rc = adj_batch_bc(b, 1, s);
+    if (!rc) rc = adj_batch_drift(b, -1, s);
     if (rc) return rc;
+    ++b->step;
   }
   return 0;
 }
@@ -1413,7 +1655,18 @@ int gpo_adj_batch_checkpoint(gpo_adj_batch h, int32_t segment, int32_t restore, 
   AdjBatch* b = (AdjBatch*)h;
   if (!b || segment < 0 || segment >= b->n_seg) return bad_arg("gpo_adj_batch_checkpoint: bad segment");
   const size_t base = (restore ? (size_t)b->n_seg + segment : (size_t)segment) * b->n;
-  return adj_batch_copy(b, b->d_copy + base, (hipStream_t)stream);
+  hipStream_t s = (hipStream_t)stream;
+  if (restore)
+    b->seg_start = segment * b->k;
+  else
+    b->step = segment * b->k;
+  if (b->any_drift) {
+    hipLaunchKernelGGL(nd_pos_copy_b, dim3((2 * b->max_drifters + 63) / 64, b->n), dim3(64), 0, s, b->d_nd,
+                       restore ? 1 : 0, segment);
+    const int rc = hip_check(hipGetLastError(), "batched drifter checkpoint");
+    if (rc) return rc;
+  }
+  return adj_batch_copy(b, b->d_copy + base, s);
 }
 
 int gpo_adj_batch_trajectory(gpo_adj_batch h, int32_t m_steps, void* stream) {
@@ -1425,7 +1678,9 @@ int gpo_adj_batch_trajectory(gpo_adj_batch h, int32_t m_steps, void* stream) {
     int rc = adj_batch_steps(b, t, 0, s);
     if (!rc) rc = adj_batch_bc(b, 2 + 2 * j, s);
     if (!rc) rc = adj_batch_steps(b, t, 1, s);
+    if (!rc) rc = adj_batch_noise(b, b->seg_start + j, j + 1, j, s);
     if (!rc) rc = adj_batch_bc(b, 3 + 2 * j, s);
+    if (!rc) rc = adj_batch_drift(b, j + 1, s);
     if (rc) return rc;
   }
   return 0;
@@ -1438,7 +1693,22 @@ int gpo_adj_batch_sweep(gpo_adj_batch h, int32_t m_steps, int32_t* flag, void* s
   // lambda at Q^{j+1} lives in buffer (j+1) & 1: start from buffer m & 1
   int rc = (m_steps & 1) ? adj_batch_copy(b, b->d_copy + (size_t)2 * b->n_seg * b->n, s) : 0;
   for (int j = m_steps - 1; j >= 0 && !rc; --j) {
-    rc = adj_batch_stage(b, b->d_items + ((size_t)2 * j + 1) * b->n, flag, s);
+    const AdjItem<float>* t1 = b->d_items + ((size_t)2 * j + 1) * b->n;
+    if (b->any_drift) {  // the drift read the state after the step
+      hipLaunchKernelGGL(nd_drift_rev_b, dim3((b->max_drifters + 63) / 64, b->n), dim3(64), 0, s, b->d_nd, j);
+      rc = hip_check(hipGetLastError(), "batched drifter reverse launch");
+    }
+    if (!rc && b->any_noise) {  // sponge blend, then the model error, in reverse
+      if (b->n_frs)
+        hipLaunchKernelGGL(frs_scale_b<float>, dim3((b->max_w + 63) / 64, (b->max_h + 3) / 4, b->n), kBlock, 0, s,
+                           t1);
+      hipLaunchKernelGGL(nd_xi_b, dim3((b->max_xi + 255) / 256, b->n), dim3(256), 0, s, b->d_nd, b->seg_start + j);
+      hipLaunchKernelGGL(nd_soar_b, dim3((b->max_cells + 255) / 256, b->n), dim3(256), 0, s, b->d_nd);
+      hipLaunchKernelGGL(nd_balance_rev_b, dim3((b->max_w - 4 + 63) / 64, (b->max_h - 4 + 3) / 4, b->n), dim3(256),
+                         0, s, b->d_nd, j);
+      rc = hip_check(hipGetLastError(), "batched model-error reverse launch");
+    }
+    if (!rc) rc = adj_batch_stage(b, t1, flag, s, !b->any_noise);
     if (!rc) rc = adj_batch_stage(b, b->d_items + ((size_t)2 * j) * b->n, flag, s);
   }
   return rc;

@@ -569,6 +569,28 @@ typedef struct {
   void* workspace;                 /* gpo_adjoint_workspace_bytes(nx, ny, 4) */
   int32_t has_frs;                 /* some side is flow relaxation (3): frs is used */
   gpo_frs frs;
+  /* stochastic member: after every step the SOAR model error of fixed normals
+   * (draws draw0, draw0 + 1, ... of the member's Philox stream, interpolation
+   * factor 1, scalar f), as gpo_random_normal + gpo_soar +
+   * gpo_geostrophic_balance add it; all members or none */
+  int32_t has_noise;
+  uint64_t seed;
+  uint32_t stream_id, draw0;
+  float soar_weights[25];
+  float g, f, beta;
+  float* xi;                       /* xi_rows x xi_cols, dense */
+  int32_t xi_rows, xi_cols;
+  float* coarse;                   /* (ny+4) x (nx+4), dense */
+  float* pre;                      /* k states: second-substep outputs before the model error */
+  /* drifters advected after every step (gpo_drift without wind); all members
+   * or none */
+  int32_t n_drifters;
+  const float* Bm;
+  int32_t Bm_pitch;
+  float* pos;                      /* forward positions, n_drifters x 2 */
+  float* pos_ckpt;                 /* n_segments x n_drifters x 2 */
+  float* pos_seg;                  /* (k+1) x n_drifters x 2: positions within the swept segment */
+  float* lam_pos;                  /* n_drifters x 2: sensitivity of the positions */
 } gpo_adj_member;
 
 typedef void* gpo_adj_batch;

@@ -73,3 +73,45 @@ def test_batched_gradients_equal_per_member(gpu_ctx, n_steps, k, island, frs):
             assert np.array_equal(a, b), f"member {i} field {q}"
         assert grads[i][0].abs().max().item() > 0
     ens.destroy()
+
+
+@pytest.mark.parametrize("frs,island", [(False, False), (True, True)])
+def test_batched_stochastic_drifter_gradients_equal_per_member(gpu_ctx, frs, island):
+    """Members with SOAR model error (fixed normals) and drifters, and the
+    drifter-position misfit added to the point misfit: the batched misfits,
+    final drifter positions and gradients equal each member's own
+    step-by-step DifferentiableCDKLM16 run bit for bit."""
+    n_steps, k = 13, 5
+    print(f"batched stochastic drifter adjoint, flow relaxation {frs}, land {island}", flush=True)
+    bcs = BCS_FRS if frs else BCS
+    members = [_member(gpu_ctx, nx, ny, 60 + i, bcs[i % 3], island) for i, (nx, ny) in enumerate(SIZES)]
+    dpos, dobs = [], []
+    for i, (sim, pos, obs, bg) in enumerate(members):
+        sim.setSOARModelError(1e-4, xorwow_seed=500 + i)
+        rng = np.random.default_rng(90 + i)
+        nd = 20 + 7 * i
+        x = np.stack([rng.uniform(0.3 * sim.nx * sim.dx, 0.7 * sim.nx * sim.dx, nd),
+                      rng.uniform(0.3 * sim.ny * sim.dy, 0.7 * sim.ny * sim.dy, nd)], 1).astype(np.float32)
+        dpos.append(x)
+        dobs.append((x + rng.normal(0, 30.0, x.shape)).astype(np.float32))
+    ens = DifferentiableEnsemble([m[0] for m in members], [m[1] for m in members], [m[2] for m in members],
+                                 n_steps, checkpoint_every=k, model_error=True, drifter_positions=dpos,
+                                 drifter_observed=dobs)
+    J = ens.forward([m[3] for m in members])
+    xs = [ens.final_positions(i) for i in range(len(members))]
+    grads = ens.backward()
+    assert ens.flag_value() == 0
+    for i, (sim, pos, obs, bg) in enumerate(members):
+        one = DifferentiableCDKLM16(sim, pos, obs, n_steps, checkpoint_every=k, model_error=True,
+                                    drifter_positions=dpos[i], drifter_observed=dobs[i])
+        J1, _ = one.forward(*bg)
+        assert torch.equal(xs[i], one._final_x), f"member {i}: drifter positions"
+        g1 = one.backward()
+        assert J[i] == J1, f"member {i}"
+        for q in range(3):
+            a = grads[i][q][2:-2, 2:-2].cpu().numpy()
+            b = g1[q][2:-2, 2:-2].cpu().numpy()
+            assert np.array_equal(a, b), f"member {i} field {q}"
+        assert grads[i][0].abs().max().item() > 0
+        assert not torch.equal(xs[i], torch.from_numpy(dpos[i]).cuda())
+    ens.destroy()
