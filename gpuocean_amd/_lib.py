@@ -82,7 +82,8 @@ class AdjMember(ctypes.Structure):
         ("has_frs", c_int32), ("frs", Frs),
         ("has_noise", c_int32), ("seed", ctypes.c_uint64), ("stream_id", c_uint32), ("draw0", c_uint32),
         ("soar_weights", c_float * 25), ("g", c_float), ("f", c_float), ("beta", c_float),
-        ("xi", c_void_p), ("xi_rows", c_int32), ("xi_cols", c_int32), ("coarse", c_void_p), ("pre", c_void_p),
+        ("xi", c_void_p), ("xi_rows", c_int32), ("xi_cols", c_int32), ("coarse", c_void_p), ("coarse_seg", c_void_p),
+        ("pre", c_void_p),
         ("n_drifters", c_int32), ("Bm", c_void_p), ("Bm_pitch", c_int32), ("pos", c_void_p),
         ("pos_ckpt", c_void_p), ("pos_seg", c_void_p), ("lam_pos", c_void_p),
     ]

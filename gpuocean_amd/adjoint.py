@@ -800,6 +800,7 @@ class DifferentiableEnsemble:
         table = (L.AdjMember * self.n)()
         self._bufs = []
         self._pos = []
+        self._coarse_seg = []
         for i, m in enumerate(self.models):
             P = _pitch_elems(m.nx + 4)
             rows = m.ny + 4
@@ -845,6 +846,10 @@ class DifferentiableEnsemble:
                 e.g, e.f, e.beta = m.g32, m.f32, m.beta32
                 e.xi, (e.xi_rows, e.xi_cols) = g.random_numbers.data_ptr(), g.random_numbers.shape
                 e.coarse = g.coarse_buffer.data_ptr()
+                with torch.cuda.stream(self.stream):   # the segment's SOAR fields, kept for the sweep
+                    cs = torch.empty((self.k, m.ny + 4, m.nx + 4), dtype=torch.float32, device=self.dev)
+                self._coarse_seg.append(cs)
+                e.coarse_seg = cs.data_ptr()
                 e.pre = base + (2 + self.n_seg + 2 * self.k + 1 + 3) * st
             if m.n_drifters:
                 pb = pos.data_ptr()

@@ -267,6 +267,41 @@ __global__ void frs_scale_b(const AdjItem<T>* __restrict__ tab) {
   }
 }
 
+// The transpose of the sponge blend over the sponge cells only (frs_scale_b
+// over the whole grid, in its order: east/west factor first (pass 0), then
+// north/south (pass 1)); block row y = member y.
+template <class T>
+__global__ void frs_scale_sponge_b(const AdjItem<T>* __restrict__ tab, int pass) {
+  const AdjItem<T>& it = tab[blockIdx.y];
+  if (!it.frs) return;
+  const Geo& g = it.g;
+  const int W = g.nx + 4, H = g.ny + 4;
+  int lo, hi;  // sponge columns (pass 0) or rows (pass 1) at each end
+  if (pass == 0) {
+    lo = g.bc_w == 3 ? min(it.sw, W) : 0, hi = g.bc_e == 3 ? min(it.se, W) : 0;
+    if (lo + hi > W) lo = W, hi = 0;
+  } else {
+    lo = g.bc_s == 3 ? min(it.ss, H) : 0, hi = g.bc_n == 3 ? min(it.sn, H) : 0;
+    if (lo + hi > H) lo = H, hi = 0;
+  }
+  const int along = pass == 0 ? H : W;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (lo + hi) * along) return;
+  const int c = k / along, t = k - c * along;
+  const int across = c < lo ? c : (pass == 0 ? W : H) - hi + (c - lo);
+  const int ti = pass == 0 ? across : t, tj = pass == 0 ? t : across;
+  const Sponge sp{it.sn, it.se, it.ss, it.sw};
+  bool in;
+  const float a = pass == 0 ? frs_alpha_ew(g, sp, ti, &in) : frs_alpha_ns(g, sp, tj, &in);
+  if (!in) return;
+  const size_t kk = (size_t)tj * g.sp + ti;
+  const T f = T(1.0f - a);
+  T* x = (T*)it.le;
+  T* y = (T*)it.lu;
+  T* z = (T*)it.lv;
+  x[kk] *= f, y[kk] *= f, z[kk] *= f;
+}
+
 // Ghost-cell passes of flow-relaxation members on one state each (block row
 // y = item y; members without flow relaxation have no item).
 __global__ void __launch_bounds__(256) bc_targets_b(const cells::BcTarget* __restrict__ tab, int pass) {
@@ -541,6 +576,7 @@ struct NoiseDriftItem {
   float W[25];
   float beta, land;
   float *xi, *coarse;    // dense rows: xi_cols, nx + 4
+  float* coarse_seg;     // k SOAR fields of the segment being recomputed (NULL: regenerated in the sweep)
   const float* Hi;
   float *state, *qs, *pre, *lam;  // forward state; segment states; pre-noise outputs; lambda buffers
   size_t plane, st;      // elements per plane and per state
@@ -551,8 +587,9 @@ struct NoiseDriftItem {
   float dt;
 };
 
-__device__ inline float* st_plane(const NoiseDriftItem& m, float* base, size_t i, int q) {
-  return base + i * m.st + (size_t)q * m.plane;
+// SOAR field buffer: the member's own (slot < 0) or the segment's slot.
+__device__ inline float* nd_coarse(const NoiseDriftItem& m, int slot) {
+  return (slot < 0 || !m.coarse_seg) ? m.coarse : m.coarse_seg + (size_t)slot * (m.g.nx + 4) * (m.g.ny + 4);
 }
 
 // The normal field of draw draw0 + step (gpo_random_normal's values).
@@ -565,19 +602,20 @@ __global__ void __launch_bounds__(256) nd_xi_b(const NoiseDriftItem* __restrict_
 }
 
 // Its SOAR field over the grid and two ghost rings (gpo_soar, scale 1).
-__global__ void __launch_bounds__(256) nd_soar_b(const NoiseDriftItem* __restrict__ tab) {
+__global__ void __launch_bounds__(256) nd_soar_b(const NoiseDriftItem* __restrict__ tab, int slot) {
   const NoiseDriftItem& m = tab[blockIdx.y];
   const int W = m.g.nx + 4, n = W * (m.g.ny + 4);
   const int k = blockIdx.x * 256 + threadIdx.x;
   if (!m.noise || k >= n) return;
   const int tj = k / W, ti = k - tj * W;
-  m.coarse[k] = 1.0f * soar_cell(m.g.nx, m.g.ny, m.W, m.per_ns, m.per_ew, m.xi, 4 * m.xi_cols, tj, ti);
+  nd_coarse(m, slot)[k] = 1.0f * soar_cell(m.g.nx, m.g.ny, m.W, m.per_ns, m.per_ew, m.xi, 4 * m.xi_cols, tj, ti);
 }
 
 // The balanced perturbation added to a state (gpo_geostrophic_balance's
 // arithmetic): the forward state (which < 0) or segment state `which`; with
 // keep >= 0 the state is first copied to pre-noise buffer `keep`.
-__global__ void __launch_bounds__(256) nd_balance_b(const NoiseDriftItem* __restrict__ tab, int which, int keep) {
+__global__ void __launch_bounds__(256) nd_balance_b(const NoiseDriftItem* __restrict__ tab, int which, int keep,
+                                                    int slot) {
   const NoiseDriftItem& m = tab[blockIdx.z];
   const int ti = blockIdx.x * 64 + (threadIdx.x & 63) + 2, tj = blockIdx.y * 4 + (threadIdx.x >> 6) + 2;
   if (!m.noise || ti > m.g.nx + 1 || tj > m.g.ny + 1) return;
@@ -589,8 +627,8 @@ __global__ void __launch_bounds__(256) nd_balance_b(const NoiseDriftItem* __rest
     p[k] = e[k], p[m.plane + k] = u[k], p[2 * m.plane + k] = v[k];
   }
   balance_cell((float)m.g.dx, (float)m.g.dy, (float)m.g.g, (float)m.g.f, m.beta, (float)m.g.north_x,
-               (float)m.g.north_y, nullptr, 0, m.coarse, 4 * (m.g.nx + 4), e, u, v, 4 * m.g.sp, m.Hi, 4 * m.g.hp,
-               m.land, tj, ti);
+               (float)m.g.north_y, nullptr, 0, nd_coarse(m, slot), 4 * (m.g.nx + 4), e, u, v, 4 * m.g.sp, m.Hi,
+               4 * m.g.hp, m.land, tj, ti);
 }
 
 // Drift of the forward positions over the forward state (which < 0), or of
@@ -664,12 +702,13 @@ __global__ void __launch_bounds__(64) nd_drift_rev_b(const NoiseDriftItem* __res
 
 // Reverse of the model error of step j on lambda buffer (j+1) & 1 (the
 // member's coarse field holds that step's SOAR field).
-__global__ void __launch_bounds__(256) nd_balance_rev_b(const NoiseDriftItem* __restrict__ tab, int j) {
+__global__ void __launch_bounds__(256) nd_balance_rev_b(const NoiseDriftItem* __restrict__ tab, int j, int slot) {
   const NoiseDriftItem& m = tab[blockIdx.z];
   if (!m.noise) return;
   float* l = m.lam + (size_t)((j + 1) & 1) * m.st;
   Geo g = m.g;
-  balance_rev_cell<float>(g, (double)m.beta, nullptr, 0, m.coarse, 4 * (m.g.nx + 4), m.Hi, m.land, l, l + m.plane,
+  balance_rev_cell<float>(g, (double)m.beta, nullptr, 0, nd_coarse(m, slot), 4 * (m.g.nx + 4), m.Hi, m.land, l,
+                          l + m.plane,
                           l + 2 * m.plane, blockIdx.x * 64 + (threadIdx.x & 63) + 2,
                           blockIdx.y * 4 + (threadIdx.x >> 6) + 2);
 }
@@ -869,6 +908,8 @@ struct AdjBatch {
   // model error and drifters (per member, n items)
   NoiseDriftItem* d_nd = nullptr;
   int any_noise = 0, any_drift = 0, max_xi = 0, max_cells = 0, max_drifters = 0;
+  int sponge_cells = 0;  // most sponge cells of one pass over the members
+  int coarse_saved = 0;  // the segment's SOAR fields are kept for the sweep
   int step = 0;       // step index of the forward run (the draw offset of its model error)
   int seg_start = 0;  // first step of the segment being recomputed and swept
 };
@@ -884,10 +925,11 @@ void adj_batch_free(AdjBatch* b) {
 // segment state `which` (its pre-noise copy kept in `keep`).
 int adj_batch_noise(AdjBatch* b, int step, int which, int keep, hipStream_t s) {
   if (!b->any_noise) return 0;
+  const int slot = keep;  // the segment's slot of this step's SOAR field (forward: the member's own buffer)
   hipLaunchKernelGGL(nd_xi_b, dim3((b->max_xi + 255) / 256, b->n), dim3(256), 0, s, b->d_nd, step);
-  hipLaunchKernelGGL(nd_soar_b, dim3((b->max_cells + 255) / 256, b->n), dim3(256), 0, s, b->d_nd);
+  hipLaunchKernelGGL(nd_soar_b, dim3((b->max_cells + 255) / 256, b->n), dim3(256), 0, s, b->d_nd, slot);
   hipLaunchKernelGGL(nd_balance_b, dim3((b->max_w - 4 + 63) / 64, (b->max_h - 4 + 3) / 4, b->n), dim3(256), 0, s,
-                     b->d_nd, which, keep);
+                     b->d_nd, which, keep, slot);
   return hip_check(hipGetLastError(), "batched model error launch");
 }
 
@@ -920,10 +962,15 @@ int adj_batch_bc(AdjBatch* b, int slot, hipStream_t s) {
   return hip_check(hipGetLastError(), "batched boundary launch");
 }
 
+void adj_batch_sponge(AdjBatch* b, const AdjItem<float>* tab, hipStream_t s) {
+  for (int pass = 0; pass < 2; ++pass)
+    hipLaunchKernelGGL(frs_scale_sponge_b<float>, dim3((b->sponge_cells + 255) / 256, b->n), dim3(256), 0, s, tab,
+                       pass);
+}
+
 int adj_batch_stage(AdjBatch* b, const AdjItem<float>* tab, int* flag, hipStream_t s, bool scale_first = true) {
   const int n = b->n;
-  if (b->n_frs && scale_first)
-    hipLaunchKernelGGL(frs_scale_b<float>, dim3((b->max_w + 63) / 64, (b->max_h + 3) / 4, n), kBlock, 0, s, tab);
+  if (b->n_frs && scale_first) adj_batch_sponge(b, tab, s);
   hipLaunchKernelGGL(f1_convert_b<float>, dim3((b->max_w + 63) / 64, (b->max_h + 3) / 4, n), kBlock, 0, s, tab,
                      flag);
   hipLaunchKernelGGL(a1_faces_b<float>, dim3((b->max_w + 63) / 64, (b->max_h + kSlopeRows - 1) / kSlopeRows, n),
@@ -1536,6 +1583,12 @@ int gpo_adj_batch_create(const gpo_adj_member* members, int32_t n, int32_t k, in
       adj_batch_free(b);
       return bad_arg("gpo_adj_batch_create: member too large for the work table");
     }
+    if (frs) {
+      const int W = p.nx + 4, H = p.ny + 4;
+      const int cols = std::min(W, (g.bc_w == 3 ? M.frs.sponge[3] : 0) + (g.bc_e == 3 ? M.frs.sponge[1] : 0));
+      const int rows = std::min(H, (g.bc_s == 3 ? M.frs.sponge[2] : 0) + (g.bc_n == 3 ? M.frs.sponge[0] : 0));
+      b->sponge_cells = std::max(b->sponge_cells, std::max(cols * H, rows * W));
+    }
     for (int sy = 0; sy < a.n_chunks; ++sy)
       for (int sx = 0; sx < a.n_strips; ++sx) work.push_back(make_int2(m, sx | (sy << 16)));
     b->max_w = std::max(b->max_w, p.nx + 4);
@@ -1564,6 +1617,12 @@ int gpo_adj_batch_create(const gpo_adj_member* members, int32_t n, int32_t k, in
       for (int i = 0; i < 25; ++i) nd.W[i] = M.soar_weights[i];
       nd.g.g = M.g, nd.g.f = M.f, nd.beta = M.beta;
       nd.xi = M.xi, nd.coarse = M.coarse;
+      nd.coarse_seg = M.coarse_seg;
+      if (m == 0) b->coarse_saved = M.coarse_seg ? 1 : 0;
+      if ((M.coarse_seg ? 1 : 0) != b->coarse_saved) {
+        adj_batch_free(b);
+        return bad_arg("gpo_adj_batch_create: members differ in whether their segment SOAR fields are kept");
+      }
       b->any_noise = 1;
       b->max_xi = std::max(b->max_xi, M.xi_rows * M.xi_cols);
       b->max_cells = std::max(b->max_cells, (p.nx + 4) * (p.ny + 4));
@@ -1699,13 +1758,13 @@ int gpo_adj_batch_sweep(gpo_adj_batch h, int32_t m_steps, int32_t* flag, void* s
       rc = hip_check(hipGetLastError(), "batched drifter reverse launch");
     }
     if (!rc && b->any_noise) {  // sponge blend, then the model error, in reverse
-      if (b->n_frs)
-        hipLaunchKernelGGL(frs_scale_b<float>, dim3((b->max_w + 63) / 64, (b->max_h + 3) / 4, b->n), kBlock, 0, s,
-                           t1);
-      hipLaunchKernelGGL(nd_xi_b, dim3((b->max_xi + 255) / 256, b->n), dim3(256), 0, s, b->d_nd, b->seg_start + j);
-      hipLaunchKernelGGL(nd_soar_b, dim3((b->max_cells + 255) / 256, b->n), dim3(256), 0, s, b->d_nd);
+      if (b->n_frs) adj_batch_sponge(b, t1, s);
+      if (!b->coarse_saved) {   // regenerate the step's SOAR fields
+        hipLaunchKernelGGL(nd_xi_b, dim3((b->max_xi + 255) / 256, b->n), dim3(256), 0, s, b->d_nd, b->seg_start + j);
+        hipLaunchKernelGGL(nd_soar_b, dim3((b->max_cells + 255) / 256, b->n), dim3(256), 0, s, b->d_nd, -1);
+      }
       hipLaunchKernelGGL(nd_balance_rev_b, dim3((b->max_w - 4 + 63) / 64, (b->max_h - 4 + 3) / 4, b->n), dim3(256),
-                         0, s, b->d_nd, j);
+                         0, s, b->d_nd, j, b->coarse_saved ? j : -1);
       rc = hip_check(hipGetLastError(), "batched model-error reverse launch");
     }
     if (!rc) rc = adj_batch_stage(b, t1, flag, s, !b->any_noise);

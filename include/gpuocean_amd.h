@@ -581,6 +581,8 @@ typedef struct {
   float* xi;                       /* xi_rows x xi_cols, dense */
   int32_t xi_rows, xi_cols;
   float* coarse;                   /* (ny+4) x (nx+4), dense */
+  float* coarse_seg;               /* k such fields: the swept segment's, kept from its recomputation
+                                    * (NULL: regenerated in the sweep); all members or none */
   float* pre;                      /* k states: second-substep outputs before the model error */
   /* drifters advected after every step (gpo_drift without wind); all members
    * or none */
