@@ -144,8 +144,10 @@ __global__ void coriolis_cells(int nx, int ny, const float* tex, int fnx, int fn
 __global__ void fill_scalar(float* p, float v) { *p = v; }
 
 // Exhaustive check that one Markstein step q + (a - c q) y with y = RN(1/c)
-// equals IEEE a / c for every a with |a| in [2^-96, 2^100] (both signs); +0
-// gives +0 either way (the step kernel sends -0 to IEEE division).
+// equals IEEE a / c for every a with |a| in [2^-96, 2^100] (both signs).  For
+// c > 0 the IEEE quotient has the sign of a, so the step kernel's copysign of
+// the result onto a changes nothing on that set and turns the +0 that the
+// step gives for a = -0 into IEEE's -0; +0 gives +0 either way.
 constexpr uint64_t kDivcDomain = 2ull * (0x62000000ull + 1ull);
 __global__ void __launch_bounds__(256) verify_divc(float c, float y, unsigned int* bad) {
   const uint64_t base = ((uint64_t)blockIdx.x * 256u + threadIdx.x) * 64u;
