@@ -91,6 +91,34 @@ def test_c0_1000_steps(gpu_ctx, exact):
             assert e64 <= 1e-3
 
 
+@pytest.mark.parametrize("general", ["divisions_unverified", "tiny_depth_cutoff"])
+@pytest.mark.parametrize("bc", [(1, 1, 1, 1), (1, 2, 1, 2)])
+def test_general_march_equals_oracle(gpu_ctx, general, bc):
+    """The step kernel's general march, which checks at run time whether the
+    fast divisions by dx, dy and 2g and the unscaled square root may be used,
+    equals the oracle value for value: taken when the host has not verified
+    the divisions (forced here by clearing divc_ok) and when g * depth_cutoff
+    is below 2^-96."""
+    from gpuocean_amd import CDKLM16, Common
+    kw = cases.config_c0(n=96)
+    kw["boundary_conditions"] = Common.BoundaryConditions(*bc)
+    rng = np.random.default_rng(11)
+    kw["hu0"] = (rng.standard_normal(kw["hu0"].shape) * 0.5).astype(np.float32)
+    if general == "tiny_depth_cutoff":
+        kw["depth_cutoff"] = 1e-30
+    print(f"general march: {general}, boundary conditions {bc}", flush=True)
+    sim = CDKLM16(gpu_ctx, **kw)
+    if general == "divisions_unverified":
+        sim.params.divc_ok = 0
+    ora = OracleCDKLM16(**kw)
+    assert np.float32(sim.dt) == np.float32(ora.dt)
+    dt = np.float32(ora.dt)
+    sim.step(60 * np.float64(dt))
+    ora.run_steps(60, dt)
+    for name, a, b in zip(("eta", "hu", "hv"), sim.download(), ora.download()):
+        assert np.array_equal(a, b), name
+
+
 @pytest.mark.parametrize("bathy", [False, True])
 def test_lake_at_rest(gpu_ctx, bathy):
     from gpuocean_amd import CDKLM16
