@@ -896,6 +896,7 @@ struct AdjBatch {
   int max_w = 0, max_h = 0;  // largest (nx+4), (ny+4) over the members
   int n_work = 0, n_blocks = 0;
   int has_land = 0, f_array = 0, angle0 = 1;  // shared by all members (kernel variant)
+  int all_ok = 1;                             // every member's fast divisions and square root verified
   StepArgs* d_fwd = nullptr;         // 2n: forward state -> stage -> state
   StepArgs* d_traj = nullptr;        // k x 2n: qs[j] -> q1s[j] -> qs[j+1]
   int2* d_work = nullptr;            // (member, strip | chunk << 16)
@@ -947,7 +948,7 @@ int adj_batch_copy(AdjBatch* b, const CopyItem* tab, hipStream_t s) {
 int adj_batch_steps(AdjBatch* b, const StepArgs* tab, int rk, hipStream_t s) {
   const hipError_t e =
       launch_cdklm_batched_exact(tab, b->d_work, b->n_work, rk, b->n_blocks, b->has_land, b->f_array, b->angle0,
-                                 /*any_stage=*/0, s);  // RK2 without friction
+                                 /*any_stage=*/0, b->all_ok, s);  // RK2 without friction
   return e != hipSuccess ? hip_check(e, "batched substep launch") : 0;
 }
 
@@ -1525,6 +1526,7 @@ int gpo_adj_batch_create(const gpo_adj_member* members, int32_t n, int32_t k, in
     if (!rc)
       rc = make_step_args(&p, M.dt, 1, ge, gu, gv, se, su, sv, se, su, sv, M.pitch, M.Hi, M.Hi_pitch, M.f_cells,
                           nullptr, M.boundary_conditions, &fwd[2 * m + 1]);
+    if (!rc && !(fast_arith_ok(fwd[2 * m]) && fast_arith_ok(fwd[2 * m + 1]))) b->all_ok = 0;
     for (int j = 0; j < k && !rc; ++j) {
       float *ae, *au, *av, *be, *bu, *bv, *ce, *cu, *cv;
       planes(M.qs, j, &ae, &au, &av);

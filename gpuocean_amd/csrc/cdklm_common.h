@@ -56,18 +56,24 @@ struct StepArgs {
   gpo_forcing forcing;
 };
 
+// The exact build's fast divisions by dx, dy and 2g and its unscaled square
+// root are all verified for this argument block (the step kernel then needs
+// no run-time flags for them).
+inline bool fast_arith_ok(const StepArgs& a) { return a.divc_ok == 7 && a.sqrt_ok; }
+
 // Launchers of the two builds of the step kernel (fast / exact).
 hipError_t launch_cdklm_fast(const StepArgs& a, int has_land, int f_array, int angle0, hipStream_t s);
 hipError_t launch_cdklm_exact(const StepArgs& a, int has_land, int f_array, int angle0, hipStream_t s);
 
 // any_stage: some member has RK3 or bottom friction (the general kernel);
 // otherwise rk_step 0/1 select the kernels specialised to the RK2 stages.
+// all_ok: fast_arith_ok holds for every member's argument blocks.
 hipError_t launch_cdklm_batched_fast(const StepArgs* members, const int2* work, int n_work, int rk_step,
-                                     int n_blocks, int has_land, int f_array, int angle0, int any_stage,
+                                     int n_blocks, int has_land, int f_array, int angle0, int any_stage, int all_ok,
                                      hipStream_t s);
 hipError_t launch_cdklm_batched_exact(const StepArgs* members, const int2* work, int n_work, int rk_step,
                                       int n_blocks, int has_land, int f_array, int angle0, int any_stage,
-                                      hipStream_t s);
+                                      int all_ok, hipStream_t s);
 
 // Builds the argument block of one substep (aux_kernels.hip); returns 0 or an error code.
 int make_step_args(const gpo_cdklm16_params* p, float dt, int32_t rk_step, const float* in_eta, const float* in_hu,

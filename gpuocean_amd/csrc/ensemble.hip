@@ -335,6 +335,7 @@ struct Ensemble {
   int n = 0, n_work = 0, n_blocks = 0, exact = 1, has_noise = 0;
   int has_land = 0, f_array = 0, angle0 = 1;
   int any_stage = 0;  // some member has bottom friction: the general substep kernel
+  int all_ok = 1;     // every member's fast divisions and square root verified
   int n_tiles = 0;  // model-error tiles over all members
   // model error on coarser grids: the largest normal field (in Philox quads)
   // and coarse buffer over the members, the x extents of the first two launches
@@ -409,9 +410,9 @@ static void use_full_tables(Ensemble* e) {
 static int ensemble_launch_substep(Ensemble* e, int rk, hipStream_t s) {
   const hipError_t err =
       e->exact ? launch_cdklm_batched_exact(e->cur_args, e->cur_work, e->cur_n_work, rk, e->n_blocks,
-                                            e->has_land, e->f_array, e->angle0, e->any_stage, s)
+                                            e->has_land, e->f_array, e->angle0, e->any_stage, e->all_ok, s)
                : launch_cdklm_batched_fast(e->cur_args, e->cur_work, e->cur_n_work, rk, e->n_blocks, e->has_land,
-                                           e->f_array, e->angle0, e->any_stage, s);
+                                           e->f_array, e->angle0, e->any_stage, e->all_ok, s);
   return err != hipSuccess ? check(err, "ensemble substep launch") : 0;
 }
 
@@ -574,6 +575,7 @@ int gpo_ensemble_create(const gpo_member* members, int32_t n, int32_t chunk_rows
       rc = make_step_args(&p, M.dt, 1, M.h1_eta, M.h1_hu, M.h1_hv, M.h0_eta, M.h0_hu, M.h0_hv, M.h0_eta, M.h0_hu,
                           M.h0_hv, M.pitch, M.Hi, M.Hi_pitch, M.f_cells, nullptr, M.boundary_conditions,
                           &args[2 * m + 1]);
+    if (!rc && !(fast_arith_ok(args[2 * m]) && fast_arith_ok(args[2 * m + 1]))) e->all_ok = 0;
     if (rc) {
       ensemble_free(e);
       return rc;
