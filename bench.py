@@ -719,6 +719,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c1", action="store_true", help="skip the single-member line of the c2 run")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward timing of the c2 run")
+    ap.add_argument("--no-c4", action="store_true",
+                    help="skip the coastal ensemble (C4) forward and batched-gradient lines of the c2 run")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -767,6 +769,13 @@ def main():
             result["coarse_model_error_ensemble"] = run_coarse_ensemble()
         if not args.no_adjoint:
             result["adjoint_c3"] = run_adjoint()
+        if not args.no_c4:
+            c4 = argparse.Namespace(**vars(args))
+            c4.members, c4.steps, c4.warmup, c4.seq_steps = 128, 300, 10, 3
+            c4.chunk_rows = c4.waves_per_simd = 0
+            result["coastal_c4"] = run_coastal(c4)
+            if not args.no_adjoint:
+                result["adjoint_batched_c4"] = run_adjoint_batch()
         result["cpu_baseline"] = None
         if not args.no_cpu:
             log("cpu baseline (FP32 oracle, ensemble members one after another)...")
