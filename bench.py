@@ -721,6 +721,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward timing of the c2 run")
     ap.add_argument("--no-c4", action="store_true",
                     help="skip the coastal ensemble (C4) forward and batched-gradient lines of the c2 run")
+    ap.add_argument("--adjoint-steps", type=int, default=300,
+                    help="RK2 steps of the C4 batched gradient (300 is the stated configuration; fewer for profiling)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -742,7 +744,7 @@ def main():
         args.seq_steps = 3 if args.seq_steps == 10 else args.seq_steps
         out = {"coastal_c4": run_coastal(args)}
         if not args.no_adjoint:
-            out["adjoint_batched_c4"] = run_adjoint_batch()
+            out["adjoint_batched_c4"] = run_adjoint_batch(n_steps=args.adjoint_steps)
         print(json.dumps(out), flush=True)
         return
 
@@ -775,7 +777,7 @@ def main():
             c4.chunk_rows = c4.waves_per_simd = 0
             result["coastal_c4"] = run_coastal(c4)
             if not args.no_adjoint:
-                result["adjoint_batched_c4"] = run_adjoint_batch()
+                result["adjoint_batched_c4"] = run_adjoint_batch(n_steps=args.adjoint_steps)
         result["cpu_baseline"] = None
         if not args.no_cpu:
             log("cpu baseline (FP32 oracle, ensemble members one after another)...")
