@@ -386,7 +386,7 @@ class Autotuner:
     per set of member shapes (and ghost-cell / drifter work)."""
 
     CHUNK_ROWS = (4, 8, 12, 16, 24, 32)
-    WAVES_PER_SIMD = (2, 4, 8)
+    WAVES_PER_SIMD = (2, 4, 5, 6, 8)
     _cache = {}
 
     @staticmethod
