@@ -15,7 +15,7 @@ from oracle.oracle import OracleCDKLM16
 
 pytestmark = pytest.mark.gpu
 
-TOL_BAR = 1e-5    # relative L2 per field against the FP32 oracle
+TOL_BAR = 1e-5    # required agreement: relative L2 per field against the FP32 oracle
 TOL_FAST = 5e-5   # fast build after 56 steps (FP32 noise level; FP32 vs FP64 oracle is ~4e-5)
 
 
