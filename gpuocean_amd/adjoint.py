@@ -782,7 +782,7 @@ class DifferentiableEnsemble:
     members share the number of steps and the checkpoint spacing."""
 
     def __init__(self, sims, positions, observed, n_steps, checkpoint_every=20, dts=None, model_error=False,
-                 drifter_positions=None, drifter_observed=None):
+                 drifter_positions=None, drifter_observed=None, fused_reverse=True, march_rows=0):
         dpos = drifter_positions if drifter_positions is not None else [None] * len(sims)
         dobs = drifter_observed if drifter_observed is not None else [None] * len(sims)
         self.models = [DifferentiableCDKLM16(s, p, o, n_steps, dt=None if dts is None else dts[i],
@@ -865,12 +865,20 @@ class DifferentiableEnsemble:
         L.check(lib.gpo_adj_batch_create(table, self.n, self.k, self.n_seg, ctypes.byref(h), self._sh()),
                 "gpo_adj_batch_create")
         self.handle = h
+        self.set_reverse(fused_reverse, march_rows)
         with torch.cuda.stream(self.stream):
             self.J = torch.zeros(self.n, dtype=torch.float64, device=self.dev)
             self.flag = torch.zeros(1, dtype=torch.int32, device=self.dev)
 
     def _sh(self):
         return ctypes.c_void_p(self.stream.cuda_stream)
+
+    def set_reverse(self, fused=True, rows=0):
+        """Reverse substeps as one fused strip march per (member, strip,
+        chunk of `rows` rows; 0 keeps the current height) or as the four
+        staged kernels; both give the same gradient bit for bit."""
+        L.check(L.lib().gpo_adj_batch_set_reverse(self.handle, int(bool(fused)), int(rows), self._sh()),
+                "gpo_adj_batch_set_reverse")
 
     def _state(self, i, idx):
         """(eta, hu, hv) views of state idx of member i's buffer."""

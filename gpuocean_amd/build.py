@@ -32,6 +32,7 @@ UNITS = [
     ("noise_drift.hip", ["-ffp-contract=off"]),
     ("ensemble.hip", ["-ffp-contract=off"]),
     ("adjoint.hip", ["-ffp-contract=off"]),
+    ("adjoint_rev32.hip", ["-ffp-contract=on"]),
     ("observe.hip", ["-ffp-contract=off"]),
 ]
 

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "adjoint_core.h"
+#include "adjoint_stages.h"
 #include "boundary_cells.h"
 #include "cdklm_common.h"
 #include "noise_cells.h"
@@ -41,79 +42,17 @@ __global__ void f4_update(Geo g, S dt, int rk_step, const S* Hi, const S* fc, Fw
   f4_cell(g, dt, rk_step, Hi, fc, w, qn_e, qn_u, qn_v, out_e, out_u, out_v, flag,
           blockIdx.y * blockDim.y + threadIdx.y + 2, blockIdx.x * blockDim.x + threadIdx.x + 2);
 }
-// A1 with the slopes of F2 formed in the same launch: a block of 64 x kSlopeRows
-// cells computes the x- and y-slopes of its cells (plus the column east of
-// it and the row north of it) into LDS, then the reverse of its cells'
-// east and north faces.
-constexpr int kSlopeRows = 8;
-template <class T>
-__device__ __forceinline__ void a1_faces_body(const Geo& g, T dt, int rk_step, const T* Hi, const T* fc,
-                                              const Fwd<T>& w, const Rev<T>& rv, const T* out_e, const T* lo_e,
-                                              const T* lo_u, const T* lo_v, int bx, int by) {
-  __shared__ T sx[3][kSlopeRows][65];
-  __shared__ T sy[3][kSlopeRows + 1][64];
-  const int tx = threadIdx.x, ty = threadIdx.y;
-  const int c0 = bx * 64, r0 = by * kSlopeRows;
-  const int c = c0 + tx, r = r0 + ty;
-  T a = 0, b = 0, k = 0;
-  if (has_xslope(g, r, c)) xslope(g, w, Hi, fc, r, c, a, b, k);
-  sx[0][ty][tx] = a, sx[1][ty][tx] = b, sx[2][ty][tx] = k;
-  a = b = k = 0;
-  if (has_yslope(g, r, c)) yslope(g, w, Hi, fc, r, c, a, b, k);
-  sy[0][ty][tx] = a, sy[1][ty][tx] = b, sy[2][ty][tx] = k;
-  if (ty == 0) {  // the row north of the block
-    a = b = k = 0;
-    if (has_yslope(g, r0 + kSlopeRows, c)) yslope(g, w, Hi, fc, r0 + kSlopeRows, c, a, b, k);
-    sy[0][kSlopeRows][tx] = a, sy[1][kSlopeRows][tx] = b, sy[2][kSlopeRows][tx] = k;
-  } else if (ty == 1 && tx < kSlopeRows) {  // the column east of the block
-    a = b = k = 0;
-    if (has_xslope(g, r0 + tx, c0 + 64)) xslope(g, w, Hi, fc, r0 + tx, c0 + 64, a, b, k);
-    sx[0][tx][64] = a, sx[1][tx][64] = b, sx[2][tx][64] = k;
-  }
-  __syncthreads();
-  if (has_xface(g, r, c))
-    a1_xface(g, dt, rk_step, Hi, fc, w, rv, out_e, lo_e, lo_u, lo_v, r, c, sx[0][ty][tx], sx[1][ty][tx],
-             sx[2][ty][tx], sx[0][ty][tx + 1], sx[1][ty][tx + 1], sx[2][ty][tx + 1]);
-  if (has_yface(g, r, c))
-    a1_yface(g, dt, rk_step, Hi, fc, w, rv, out_e, lo_e, lo_u, lo_v, r, c, sy[0][ty][tx], sy[1][ty][tx],
-             sy[2][ty][tx], sy[0][ty + 1][tx], sy[1][ty + 1][tx], sy[2][ty + 1][tx]);
+// The reverse-stage kernels with IEEE arithmetic (the FP64 build).
+namespace exact {
+#include "adjoint_stages.inc"
+void launch_stages(const Geo& g, double dt, int rk, const double* ie, const double* iu, const double* iv,
+                   const double* out_e, const double* le, const double* lu, const double* lv, double* lie, double* liu,
+                   double* liv, double* lqe, double* lqu, double* lqv, int accumulate, int lq_mode, const double* Hi,
+                   const double* fc, void* ws, int* flag, hipStream_t s) {
+  stages_t<double>(g, dt, rk, ie, iu, iv, out_e, le, lu, lv, lie, liu, liv, lqe, lqu, lqv, accumulate, lq_mode, Hi,
+                   fc, ws, flag, s);
 }
-template <class T>
-__global__ __launch_bounds__(64 * kSlopeRows) void a1_faces(Geo g, T dt, int rk_step, const T* Hi, const T* fc,
-                                                            Fwd<T> w, Rev<T> rv, const T* out_e, const T* lo_e,
-                                                            const T* lo_u, const T* lo_v) {
-  a1_faces_body(g, dt, rk_step, Hi, fc, w, rv, out_e, lo_e, lo_u, lo_v, blockIdx.x, blockIdx.y);
-}
-// A2: one wave per row segment of 64 cells, of which the inner 62 are stored;
-// the x-slope terms a cell sends to its west and east neighbours cross lanes.
-template <class T>
-__device__ __forceinline__ void a2_cells_body(const Geo& g, T dt, int rk_step, const T* Hi, const T* fc,
-                                              const Fwd<T>& w, const Rev<T>& rv, const T* out_e, const T* lo_e,
-                                              const T* lo_u, const T* lo_v, int bx, int by) {
-  const int lane = threadIdx.x;
-  const int r = by * blockDim.y + threadIdx.y;
-  const int c = bx * 62 - 1 + lane;
-  const CellParts<T> p = a2_parts(g, dt, rk_step, Hi, fc, w, rv, out_e, lo_e, lo_u, lo_v, r, c);
-  T xp_west[3], xm_east[3];
-  for (int i = 0; i < 3; ++i) {
-    xp_west[i] = __shfl_up(p.xp[i], 1, 64);
-    xm_east[i] = __shfl_down(p.xm[i], 1, 64);
-  }
-  if (lane >= 1 && lane <= 62 && c < g.nx + 4 && r < g.ny + 4) store_parts(g, rv, r, c, xp_west, p, xm_east);
-}
-template <class T>
-__global__ __launch_bounds__(256) void a2_cells(Geo g, T dt, int rk_step, const T* Hi, const T* fc, Fwd<T> w,
-                                                Rev<T> rv, const T* out_e, const T* lo_e, const T* lo_u,
-                                                const T* lo_v) {
-  a2_cells_body(g, dt, rk_step, Hi, fc, w, rv, out_e, lo_e, lo_u, lo_v, blockIdx.x, blockIdx.y);
-}
-template <class T>
-__global__ void a3_fold(Geo g, int rk_step, int accumulate, int lq_mode, const T* Hi, Fwd<T> w, Rev<T> rv,
-                        const T* in_e, const T* out_e, const T* lo_e, const T* lo_u, const T* lo_v, T* li_e, T* li_u,
-                        T* li_v, T* lq_e, T* lq_u, T* lq_v) {
-  a3_cell(g, rk_step, accumulate, lq_mode, Hi, w, rv, in_e, out_e, lo_e, lo_u, lo_v, li_e, li_u, li_v, lq_e, lq_u,
-          lq_v, blockIdx.y * blockDim.y + threadIdx.y + 2, blockIdx.x * blockDim.x + threadIdx.x + 2);
-}
+}  // namespace exact
 
 // ---- flow relaxation (boundary_kernels.cu:406-540): q <- (1 - alpha) q + alpha q_ext
 // on the sponge rows (north/south pass) and then columns (east/west pass),
@@ -195,53 +134,6 @@ __global__ void frs_blend_f64(Geo g, Sponge sp, const float* ns_cur, const float
     u[k] = (1.0 - a) * u[k] + a * (double)(t_ * n.y + (1.0f - t_) * c.y);
     v[k] = (1.0 - a) * v[k] + a * (double)(t_ * n.z + (1.0f - t_) * c.z);
   }
-}
-
-// ---- batched reverse substep: block row z is member z of a table
-// One member's reverse substep: its geometry, the stage input and output, the
-// sensitivities in and out, and its workspace.
-template <class T>
-struct AdjItem {
-  Geo g;
-  T dt;
-  int rk, accumulate, lq_mode;
-  const GPO_GLOBAL T *ie, *iu, *iv, *out_e, *le, *lu, *lv;
-  GPO_GLOBAL T *lie, *liu, *liv, *lqe, *lqu, *lqv;
-  const GPO_GLOBAL T *Hi, *fc;
-  Fwd<T> w;
-  Rev<T> rv;
-  int frs;  // flow relaxation: lambda in is scaled by the sponge transpose first
-  int sn, se, ss, sw;
-};
-
-template <class T>
-__global__ void f1_convert_b(const AdjItem<T>* __restrict__ tab, int* flag) {
-  const AdjItem<T>& it = tab[blockIdx.z];
-  f1_cell<T>(it.g, (const T*)it.ie, (const T*)it.iu, (const T*)it.iv, (const T*)it.Hi, it.w, flag,
-             blockIdx.y * blockDim.y + threadIdx.y,
-          blockIdx.x * blockDim.x + threadIdx.x);
-}
-template <class T>
-__global__ __launch_bounds__(64 * kSlopeRows) void a1_faces_b(const AdjItem<T>* __restrict__ tab) {
-  const AdjItem<T>& it = tab[blockIdx.z];
-  if ((int)blockIdx.x * 64 >= it.g.nx + 4 || (int)blockIdx.y * kSlopeRows >= it.g.ny + 4) return;
-  a1_faces_body<T>(it.g, it.dt, it.rk, (const T*)it.Hi, (const T*)it.fc, it.w, it.rv, (const T*)it.out_e,
-                   (const T*)it.le, (const T*)it.lu, (const T*)it.lv, blockIdx.x, blockIdx.y);
-}
-template <class T>
-__global__ __launch_bounds__(256) void a2_cells_b(const AdjItem<T>* __restrict__ tab) {
-  const AdjItem<T>& it = tab[blockIdx.z];
-  if ((int)blockIdx.x * 62 - 1 >= it.g.nx + 4 || (int)(blockIdx.y * blockDim.y) >= it.g.ny + 4) return;
-  a2_cells_body<T>(it.g, it.dt, it.rk, (const T*)it.Hi, (const T*)it.fc, it.w, it.rv, (const T*)it.out_e,
-                   (const T*)it.le, (const T*)it.lu, (const T*)it.lv, blockIdx.x, blockIdx.y);
-}
-template <class T>
-__global__ void a3_fold_b(const AdjItem<T>* __restrict__ tab) {
-  const AdjItem<T>& it = tab[blockIdx.z];
-  a3_cell<T>(it.g, it.rk, it.accumulate, it.lq_mode, (const T*)it.Hi, it.w, it.rv, (const T*)it.ie,
-             (const T*)it.out_e, (const T*)it.le, (const T*)it.lu, (const T*)it.lv, (T*)it.lie, (T*)it.liu,
-             (T*)it.liv, (T*)it.lqe, (T*)it.lqu, (T*)it.lqv, blockIdx.y * blockDim.y + threadIdx.y + 2,
-             blockIdx.x * blockDim.x + threadIdx.x + 2);
 }
 
 template <class T>
@@ -872,30 +764,48 @@ int forward_stages(const Geo& g, S dt, int rk, const S* ie, const S* iu, const S
 
 // Reverse of one substep.  lambda_in = (or +=, with accumulate) the
 // sensitivity of the stage input; for rk == 1 the direct Q^n term is added to
-// lambda_qn, or written to it with lq_mode 1.
+// lambda_qn, or written to it with lq_mode 1.  The FP32 reverse runs the
+// stage kernels built with fast arithmetic (adjoint_rev32.hip), the FP64 one
+// those built here.
 template <class T>
 int adjoint_stages(const Geo& g, T dt, int rk, const T* ie, const T* iu, const T* iv, const T* out_e, const T* le,
                    const T* lu, const T* lv, T* lie, T* liu, T* liv, T* lqe, T* lqu, T* lqv, int accumulate,
-                   int lq_mode, const T* Hi, const T* fc, void* ws, int* flag, hipStream_t s) {
-  Fwd<T> w = carve<T>(ws, g.nx, g.ny);
-  Rev<T> rv = carve_rev<T>(ws, g.nx, g.ny);
-  hipLaunchKernelGGL(f1_convert<T>, cell_grid(g), kBlock, 0, s, g, ie, iu, iv, Hi, w, flag);
-  const dim3 face_grid((g.nx + 4 + 63) / 64, (g.ny + 4 + kSlopeRows - 1) / kSlopeRows);
-  hipLaunchKernelGGL(a1_faces<T>, face_grid, dim3(64, kSlopeRows), 0, s, g, dt, rk, Hi, fc, w, rv, out_e, le, lu, lv);
-  const dim3 part_grid((g.nx + 4 + 61) / 62, (g.ny + 4 + 3) / 4);
-  hipLaunchKernelGGL(a2_cells<T>, part_grid, kBlock, 0, s, g, dt, rk, Hi, fc, w, rv, out_e, le, lu, lv);
-  hipLaunchKernelGGL(a3_fold<T>, inner_grid(g), kBlock, 0, s, g, rk, accumulate, lq_mode, Hi, w, rv, ie, out_e, le,
-                     lu, lv, lie, liu, liv, lqe, lqu, lqv);
+                   int lq_mode, const T* Hi, const T* fc, void* ws, int* flag, hipStream_t s);
+template <>
+int adjoint_stages<float>(const Geo& g, float dt, int rk, const float* ie, const float* iu, const float* iv,
+                          const float* out_e, const float* le, const float* lu, const float* lv, float* lie,
+                          float* liu, float* liv, float* lqe, float* lqu, float* lqv, int accumulate, int lq_mode,
+                          const float* Hi, const float* fc, void* ws, int* flag, hipStream_t s) {
+  fast32::launch_stages(g, dt, rk, ie, iu, iv, out_e, le, lu, lv, lie, liu, liv, lqe, lqu, lqv, accumulate, lq_mode, Hi,
+                        fc, ws, flag, s);
+  return hip_check(hipGetLastError(), "adjoint launch");
+}
+template <>
+int adjoint_stages<double>(const Geo& g, double dt, int rk, const double* ie, const double* iu, const double* iv,
+                           const double* out_e, const double* le, const double* lu, const double* lv, double* lie,
+                           double* liu, double* liv, double* lqe, double* lqu, double* lqv, int accumulate,
+                           int lq_mode, const double* Hi, const double* fc, void* ws, int* flag, hipStream_t s) {
+  exact::launch_stages(g, dt, rk, ie, iu, iv, out_e, le, lu, lv, lie, liu, liv, lqe, lqu, lqv, accumulate, lq_mode, Hi,
+                       fc, ws, flag, s);
   return hip_check(hipGetLastError(), "adjoint launch");
 }
 
 // ---- batched differentiable ensemble
+
+// Rows per work item of the fused reverse march.
+constexpr int kDefaultMarchRows = 32;
 
 struct AdjBatch {
   int n = 0, k = 0, n_seg = 0;
   int max_w = 0, max_h = 0;  // largest (nx+4), (ny+4) over the members
   int n_work = 0, n_blocks = 0;
   int has_land = 0, f_array = 0, angle0 = 1;  // shared by all members (kernel variant)
+  int any_fc = 0;                             // some member has a coriolis array
+  // reverse substeps as one strip march per work item (adjoint_march.inc)
+  // rather than the staged kernels; the work table for chunks of march_rows
+  int use_march = 1, march_rows = 0, n_march = 0, max_fold = 0;
+  int4* d_march = nullptr;
+  std::vector<int> nxs, nys;
   int all_ok = 1;                             // every member's fast divisions and square root verified
   StepArgs* d_fwd = nullptr;         // 2n: forward state -> stage -> state
   StepArgs* d_traj = nullptr;        // k x 2n: qs[j] -> q1s[j] -> qs[j+1]
@@ -917,7 +827,7 @@ struct AdjBatch {
 
 void adj_batch_free(AdjBatch* b) {
   for (void* p : {(void*)b->d_fwd, (void*)b->d_traj, (void*)b->d_work, (void*)b->d_items, (void*)b->d_copy,
-                  (void*)b->d_bc, (void*)b->d_nd})
+                  (void*)b->d_bc, (void*)b->d_nd, (void*)b->d_march})
     if (p) (void)hipFree(p);
   delete b;
 }
@@ -970,15 +880,34 @@ void adj_batch_sponge(AdjBatch* b, const AdjItem<float>* tab, hipStream_t s) {
 }
 
 int adj_batch_stage(AdjBatch* b, const AdjItem<float>* tab, int* flag, hipStream_t s, bool scale_first = true) {
-  const int n = b->n;
   if (b->n_frs && scale_first) adj_batch_sponge(b, tab, s);
-  hipLaunchKernelGGL(f1_convert_b<float>, dim3((b->max_w + 63) / 64, (b->max_h + 3) / 4, n), kBlock, 0, s, tab,
-                     flag);
-  hipLaunchKernelGGL(a1_faces_b<float>, dim3((b->max_w + 63) / 64, (b->max_h + kSlopeRows - 1) / kSlopeRows, n),
-                     dim3(64, kSlopeRows), 0, s, tab);
-  hipLaunchKernelGGL(a2_cells_b<float>, dim3((b->max_w + 61) / 62, (b->max_h + 3) / 4, n), kBlock, 0, s, tab);
-  hipLaunchKernelGGL(a3_fold_b<float>, dim3((b->max_w - 4 + 63) / 64, (b->max_h - 4 + 3) / 4, n), kBlock, 0, s, tab);
+  if (b->use_march)
+    fast32::launch_batch_march(tab, b->d_march, b->n_march, b->n, b->max_fold, b->any_fc != 0, s);
+  else
+    fast32::launch_batch_stage(tab, b->max_w, b->max_h, b->n, b->any_fc != 0, flag, s);
   return hip_check(hipGetLastError(), "batched adjoint launch");
+}
+
+// (Re)build the march work table for chunks of `rows` rows (waits for the
+// stream: the old table may be in use).
+int adj_batch_march_table(AdjBatch* b, int rows, hipStream_t s) {
+  if (rows == b->march_rows && b->d_march) return 0;
+  std::vector<int4> work;
+  fast32::march_work_items(b->nxs.data(), b->nys.data(), b->n, rows, work);
+  int rc = hip_check(hipStreamSynchronize(s), "march table sync");
+  if (!rc && b->d_march) rc = hip_check(hipFree(b->d_march), "march table free");
+  b->d_march = nullptr;
+  if (!rc) rc = hip_check(hipMalloc(&b->d_march, sizeof(int4) * work.size()), "march table hipMalloc");
+  if (!rc)
+    rc = hip_check(hipMemcpyAsync(b->d_march, work.data(), sizeof(int4) * work.size(), hipMemcpyHostToDevice, s),
+                   "march table copy");
+  if (!rc) rc = hip_check(hipStreamSynchronize(s), "march table sync");
+  if (rc) return rc;
+  b->n_march = (int)work.size();
+  b->march_rows = rows;
+  b->max_fold = 0;
+  for (int m = 0; m < b->n; ++m) b->max_fold = std::max(b->max_fold, fast32::march_fold_cells(b->nxs[m], b->nys[m]));
+  return 0;
 }
 
 // The reverse sweep of gpo_cdklm16_adjoint_steps_f32 over caller-given (or
@@ -1550,6 +1479,7 @@ int gpo_adj_batch_create(const gpo_adj_member* members, int32_t n, int32_t k, in
         it.rk = rk;
         it.Hi = gptr(hi_adj);
         it.fc = gptr(M.f_cells);
+        if (M.f_cells) b->any_fc = 1;
         it.w = w;
         it.rv = rv;
         it.frs = frs ? 1 : 0;
@@ -1642,6 +1572,7 @@ int gpo_adj_batch_create(const gpo_adj_member* members, int32_t n, int32_t k, in
     }
     nds.push_back(nd);
   }
+  for (int m = 0; m < n; ++m) b->nxs.push_back(members[m].params.nx), b->nys.push_back(members[m].params.ny);
   b->n_work = (int)work.size();
   const long long waves = std::min<long long>((long long)persistent_waves(8), b->n_work);
   b->n_blocks = (int)((waves + kWavesPerBlock - 1) / kWavesPerBlock);
@@ -1687,12 +1618,22 @@ int gpo_adj_batch_create(const gpo_adj_member* members, int32_t n, int32_t k, in
                      "gpo_adj_batch_create copy");
   }
   if (!rc) rc = hip_check(hipStreamSynchronize(s), "gpo_adj_batch_create sync");
+  if (!rc) rc = adj_batch_march_table(b, kDefaultMarchRows, s);
   if (rc) {
     adj_batch_free(b);
     return rc;
   }
   *out = (gpo_adj_batch)b;
   return 0;
+}
+
+int gpo_adj_batch_set_reverse(gpo_adj_batch h, int32_t fused, int32_t rows, void* stream) {
+  AdjBatch* b = (AdjBatch*)h;
+  if (!b || rows < 0 || rows > 4096) return bad_arg("gpo_adj_batch_set_reverse: bad arguments");
+  int rc = hip_check(hipStreamSynchronize((hipStream_t)stream), "gpo_adj_batch_set_reverse sync");
+  if (!rc && rows > 0) rc = adj_batch_march_table(b, rows, (hipStream_t)stream);
+  if (!rc) b->use_march = fused ? 1 : 0;
+  return rc;
 }
 
 int gpo_adj_batch_forward(gpo_adj_batch h, int32_t n_steps, void* stream) {

@@ -27,6 +27,7 @@
 #include <cmath>
 #include <cstddef>
 #include <cstdint>
+#include <type_traits>
 
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
@@ -81,9 +82,28 @@ GPO_HD inline Dual tsqrt(Dual a) {
   const double s = sqrt(a.v);
   return {s, s > 0.0 ? a.d / (2.0 * s) : 0.0};
 }
-GPO_HD inline float tsqrt(float a) { return std::sqrt(a); }
+GPO_HD inline float tsqrt(float a) {
+#if defined(GPO_ADJ_FAST_F32) && defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_sqrtf(a);
+#else
+  return std::sqrt(a);
+#endif
+}
 GPO_HD inline double tsqrt(double a) { return sqrt(a); }
 
+// Division and square root of the stages.  IEEE for double and Dual, and for
+// float by default; a translation unit that defines GPO_ADJ_FAST_F32 before
+// including this header (the product's FP32 reverse) gets the hardware
+// reciprocal (1 ulp) times the numerator and the hardware square root for
+// float on the device.  Written as builtins, these are the same instructions
+// wherever the code is inlined, so two kernels that evaluate the same
+// expression on the same values agree bit for bit.
+template <class T> GPO_HD inline T qdiv(T a, T b) {
+#if defined(GPO_ADJ_FAST_F32) && defined(__HIP_DEVICE_COMPILE__)
+  if constexpr (std::is_same<T, float>::value) return a * __builtin_amdgcn_rcpf(b);
+#endif
+  return a / b;
+}
 GPO_HD inline double val(Dual a) { return a.v; }
 GPO_HD inline float val(float a) { return a; }
 GPO_HD inline double val(double a) { return a; }
@@ -181,16 +201,18 @@ template <class T> GPO_HD inline T cell_depth_land(const Geo& g, const T* Hi, in
   const T c_ = corner_land(g, Hi, r, c + 1), d = corner_land(g, Hi, r + 1, c + 1);
   return T(0.25) * (a + b + c_ + d);
 }
-// Depth of the conversion (initHm_kernel.cu:81-100): the mean of the wet
-// corners, `land` when all four are land.
-template <class T> GPO_HD inline T cell_bm(const Geo& g, const T* Hi, int r, int c) {
-  const T a = corner(g, Hi, r, c), b = corner(g, Hi, r + 1, c);
-  const T c_ = corner(g, Hi, r, c + 1), d = corner(g, Hi, r + 1, c + 1);
+// Depth of the conversion (initHm_kernel.cu:81-100) from the four corner
+// depths (r,c), (r+1,c), (r,c+1), (r+1,c+1): the mean of the wet corners,
+// `land` when all four are land.
+template <class T> GPO_HD inline T bm_of(const Geo& g, T a, T b, T c_, T d) {
   const T S = a + b + c_ + d;
   if (!g.has_land) return T(0.25) * S;
   const int wet = 4 - (val(a) == (double)kDry) - (val(b) == (double)kDry) - (val(c_) == (double)kDry) -
                   (val(d) == (double)kDry);
   return wet == 0 ? T(g.land) : (wet == 4 ? T(0.25) * S : S / T(wet));
+}
+template <class T> GPO_HD inline T cell_bm(const Geo& g, const T* Hi, int r, int c) {
+  return bm_of(g, corner(g, Hi, r, c), corner(g, Hi, r + 1, c), corner(g, Hi, r, c + 1), corner(g, Hi, r + 1, c + 1));
 }
 template <class T> GPO_HD inline bool land_cell(const Geo& g, T bm) {
   return g.has_land && fabs((double)val(bm) - g.land) <= kDryTol;
@@ -199,44 +221,74 @@ template <class T> GPO_HD inline T fcor(const Geo& g, const T* fc, int r, int c)
   return fc ? fc[(size_t)r * g.fp + c] : T(g.f);
 }
 
+// Converted cells, coriolis values and corner depths of a slope stencil:
+// cells 0, 1, 2 are s-1, s, s+1 along the slope's axis; the corners are
+// (r,s), (r+1,s), (r,s+1), (r+1,s+1) for an x-slope at (r, s) and
+// (s,c), (s,c+1), (s+1,c), (s+1,c+1) for a y-slope at (s, c).
+template <class T> struct Sten {
+  T e0, e1, e2, u0, u1, u2, v0, v1, v2;
+  T F0, F1, F2;
+  T Ha, Hb, Hc, Hd;
+};
+template <class T>
+GPO_HD inline Sten<T> xsten(const Geo& g, const Fwd<T>& w, const T* Hi, const T* fc, int r, int s) {
+  const size_t o = (size_t)r * g.W + s;
+  return {w.ce[o - 1], w.ce[o], w.ce[o + 1], w.cu[o - 1], w.cu[o], w.cu[o + 1], w.cv[o - 1], w.cv[o], w.cv[o + 1],
+          fcor(g, fc, r, s - 1), fcor(g, fc, r, s), fcor(g, fc, r, s + 1), corner(g, Hi, r, s),
+          corner(g, Hi, r + 1, s), corner(g, Hi, r, s + 1), corner(g, Hi, r + 1, s + 1)};
+}
+template <class T>
+GPO_HD inline Sten<T> ysten(const Geo& g, const Fwd<T>& w, const T* Hi, const T* fc, int s, int c) {
+  const size_t o = (size_t)s * g.W + c, up = g.W;
+  return {w.ce[o - up], w.ce[o], w.ce[o + up], w.cu[o - up], w.cu[o], w.cu[o + up], w.cv[o - up], w.cv[o],
+          w.cv[o + up], fcor(g, fc, s - 1, c), fcor(g, fc, s, c), fcor(g, fc, s + 1, c), corner(g, Hi, s, c),
+          corner(g, Hi, s, c + 1), corner(g, Hi, s + 1, c), corner(g, Hi, s + 1, c + 1)};
+}
+
 // ----------------------------------------------------------------------------- forward stages
 
 // desingularize (common.cu:664-666), the double formula
 template <class S> GPO_HD inline S desing(S h, S hu, double eps) {
-  const S a = h * h / S(2.0 * eps) + S(0.5 * eps);
-  return hu / tmax(tmin(a, S(eps)), tabs(h));
+  const S a = qdiv(h * h, S(2.0 * eps)) + S(0.5 * eps);
+  return qdiv(hu, tmax(tmin(a, S(eps)), tabs(h)));
 }
 
-// F1: conversion of the stage input (CDKLM16_kernel.cu:614-655).  `in_*` are
-// the caller's arrays (pitch sp) of type S.
+// The conversion of one cell (CDKLM16_kernel.cu:614-655): loaded eta and
+// momenta (wall mirror signs applied), the cell's conversion depth Hm ->
+// (eta, u, v).
+template <class S>
+GPO_HD inline void conv_v(const Geo& g, S e, S hu, S hv, S Hm, S& oe, S& ou, S& ov) {
+  const S h = e + Hm;
+  if (land_cell(g, Hm)) {  // land: constants
+    oe = S(kDry);
+    ou = S(0.0);
+    ov = S(0.0);
+  } else if (val(h) < g.desing_eps) {
+    if (val(h) <= g.depth_cutoff) {
+      oe = -Hm + S(g.depth_cutoff);
+      ou = S(0.0);
+      ov = S(0.0);
+    } else {
+      oe = e;
+      ou = desing(h, hu, g.desing_eps);
+      ov = desing(h, hv, g.desing_eps);
+    }
+  } else {
+    oe = e;
+    ou = qdiv(hu, h);
+    ov = qdiv(hv, h);
+  }
+}
+
+// F1: conversion of the stage input.  `in_*` are the caller's arrays (pitch
+// sp) of type S.
 template <class S>
 GPO_HD void f1_cell(Geo g, const S* in_e, const S* in_u, const S* in_v, const S* Hi, Fwd<S> w, int* flag, int r, int c) {
   if (c >= g.nx + 4 || r >= g.ny + 4) return;
   const Src s = map_cell(g, r, c);
   const size_t k = (size_t)s.r * g.sp + s.c;
-  S e = in_e[k];
-  S hu = in_u[k] * S(s.shu);
-  S hv = in_v[k] * S(s.shv);
-  const S Hm = cell_bm(g, Hi, r, c);
-  const S h = e + Hm;
-  S u, v;
-  if (land_cell(g, Hm)) {  // land: constants
-    e = S(kDry);
-    u = S(0.0);
-    v = S(0.0);
-  } else if (val(h) < g.desing_eps) {
-    if (val(h) <= g.depth_cutoff) {
-      e = -Hm + S(g.depth_cutoff);
-      u = S(0.0);
-      v = S(0.0);
-    } else {
-      u = desing(h, hu, g.desing_eps);
-      v = desing(h, hv, g.desing_eps);
-    }
-  } else {
-    u = hu / h;
-    v = hv / h;
-  }
+  S e, u, v;
+  conv_v(g, in_e[k], in_u[k] * S(s.shu), in_v[k] * S(s.shv), cell_bm(g, Hi, r, c), e, u, v);
   const size_t o = (size_t)r * g.W + c;
   w.ce[o] = e;
   w.cu[o] = u;
@@ -254,11 +306,10 @@ template <class S> GPO_HD inline S minmod_slope(S l, S c, S r, S theta) {
 
 // x-slopes at (r, s): CDKLM16_kernel.cu:673-737 with adjustSlopes_x (:159-203)
 template <class S>
-GPO_HD void xslope(const Geo& g, const Fwd<S>& w, const S* Hi, const S* fc, int r, int s, S& q0, S& q1, S& K) {
-  const size_t o = (size_t)r * g.W + s;
-  const S le = w.ce[o - 1], ce = w.ce[o], re = w.ce[o + 1];
-  const S lu = w.cu[o - 1], cu = w.cu[o], ru = w.cu[o + 1];
-  S lv = w.cv[o - 1], cv = w.cv[o], rv = w.cv[o + 1];
+GPO_HD void xslope_v(const Geo& g, const Sten<S>& st, int s, S& q0, S& q1, S& K) {
+  const S le = st.e0, ce = st.e1, re = st.e2;
+  const S lu = st.u0, cu = st.u1, ru = st.u2;
+  S lv = st.v0, cv = st.v1, rv = st.v2;
   const S th = S(g.theta);
   q0 = minmod_slope(lu, cu, ru, th);
   q1 = minmod_slope(lv, cv, rv, th);
@@ -271,34 +322,38 @@ GPO_HD void xslope(const Geo& g, const Fwd<S>& w, const S* Hi, const S* fc, int 
     if (s > g.nx + 1) cv = -cv;
   }
   const S nx_ = S(g.north_x), ny_ = S(g.north_y), G = S(g.g), Vc = S(g.dx) / (S(2.0) * G);
-  const S lfv = (nx_ * lu + ny_ * lv) * fcor(g, fc, r, s - 1);
-  const S cfv = (nx_ * cu + ny_ * cv) * fcor(g, fc, r, s);
-  const S rfv = (nx_ * ru + ny_ * rv) * fcor(g, fc, r, s + 1);
+  const S lfv = (nx_ * lu + ny_ * lv) * st.F0;
+  const S cfv = (nx_ * cu + ny_ * cv) * st.F1;
+  const S rfv = (nx_ * ru + ny_ * rv) * st.F2;
   const S b = th * G * (ce - le - Vc * (cfv + lfv));
   const S c = S(0.5) * G * (re - le - Vc * (rfv + S(2.0) * cfv + lfv));
   const S f = th * G * (re - ce - Vc * (cfv + rfv));
   S k = minmod_raw(b, c, f);
-  S v = w.cv[o];
+  S v = st.v1;
   if (g.bc_w == 1 && s < 2) v = -v;
   if (g.bc_e == 1 && s > g.nx + 2) v = -v;
   const S g2 = S(2.0) * G;
-  const S dxfv = S(g.dx) * fcor(g, fc, r, s) * v;
-  const S Hw = S(0.5) * (corner(g, Hi, r, s) + corner(g, Hi, r + 1, s));
-  const S He = S(0.5) * (corner(g, Hi, r, s + 1) + corner(g, Hi, r + 1, s + 1));
-  const S hw = ce + Hw - (k + dxfv) / g2;
-  const S he = ce + He + (k + dxfv) / g2;
+  const S dxfv = S(g.dx) * st.F1 * v;
+  const S Hw = S(0.5) * (st.Ha + st.Hb);
+  const S He = S(0.5) * (st.Hc + st.Hd);
+  const S hw = ce + Hw - qdiv(k + dxfv, g2);
+  const S he = ce + He + qdiv(k + dxfv, g2);
   k = (val(hw) > 0) ? k : -dxfv + g2 * (ce + Hw);
   k = (val(he) > 0) ? k : -dxfv - g2 * (ce + He);
   K = k;
 }
 
+template <class S>
+GPO_HD void xslope(const Geo& g, const Fwd<S>& w, const S* Hi, const S* fc, int r, int s, S& q0, S& q1, S& K) {
+  xslope_v(g, xsten(g, w, Hi, fc, r, s), s, q0, q1, K);
+}
+
 // y-slopes at (s, c): CDKLM16_kernel.cu:779-842 with adjustSlopes_y (:210-254)
 template <class S>
-GPO_HD void yslope(const Geo& g, const Fwd<S>& w, const S* Hi, const S* fc, int s, int c, S& q0, S& q1, S& L) {
-  const size_t o = (size_t)s * g.W + c, up = g.W;
-  const S le = w.ce[o - up], ce = w.ce[o], ue = w.ce[o + up];
-  S lu = w.cu[o - up], cu = w.cu[o], uu = w.cu[o + up];
-  const S lv = w.cv[o - up], cv = w.cv[o], uv = w.cv[o + up];
+GPO_HD void yslope_v(const Geo& g, const Sten<S>& st, int s, S& q0, S& q1, S& L) {
+  const S le = st.e0, ce = st.e1, ue = st.e2;
+  S lu = st.u0, cu = st.u1, uu = st.u2;
+  const S lv = st.v0, cv = st.v1, uv = st.v2;
   const S th = S(g.theta);
   q0 = minmod_slope(lu, cu, uu, th);
   q1 = minmod_slope(lv, cv, uv, th);
@@ -311,25 +366,30 @@ GPO_HD void yslope(const Geo& g, const Fwd<S>& w, const S* Hi, const S* fc, int 
     if (s > g.ny + 1) cu = -cu;
   }
   const S ex = S(g.north_y), ey = S(-g.north_x), G = S(g.g), Uc = S(g.dy) / (S(2.0) * G);
-  const S lfu = (ex * lu + ey * lv) * fcor(g, fc, s - 1, c);
-  const S cfu = (ex * cu + ey * cv) * fcor(g, fc, s, c);
-  const S ufu = (ex * uu + ey * uv) * fcor(g, fc, s + 1, c);
+  const S lfu = (ex * lu + ey * lv) * st.F0;
+  const S cfu = (ex * cu + ey * cv) * st.F1;
+  const S ufu = (ex * uu + ey * uv) * st.F2;
   const S b = th * G * (ce - le + Uc * (cfu + lfu));
   const S cc = S(0.5) * G * (ue - le + Uc * (ufu + S(2.0) * cfu + lfu));
   const S f = th * G * (ue - ce + Uc * (cfu + ufu));
   S l = minmod_raw(b, cc, f);
-  S u = w.cu[o];
+  S u = st.u1;
   if (g.bc_s == 1 && s < 2) u = -u;
   if (g.bc_n == 1 && s > g.ny + 2) u = -u;
   const S g2 = S(2.0) * G;
-  const S dyfu = S(g.dy) * fcor(g, fc, s, c) * u;
-  const S Hs = S(0.5) * (corner(g, Hi, s, c) + corner(g, Hi, s, c + 1));
-  const S Hn = S(0.5) * (corner(g, Hi, s + 1, c) + corner(g, Hi, s + 1, c + 1));
-  const S hs = ce + Hs - (l - dyfu) / g2;
-  const S hn = ce + Hn + (l - dyfu) / g2;
+  const S dyfu = S(g.dy) * st.F1 * u;
+  const S Hs = S(0.5) * (st.Ha + st.Hb);
+  const S Hn = S(0.5) * (st.Hc + st.Hd);
+  const S hs = ce + Hs - qdiv(l - dyfu, g2);
+  const S hn = ce + Hn + qdiv(l - dyfu, g2);
   l = (val(hs) > 0) ? l : dyfu + g2 * (ce + Hs);
   l = (val(hn) > 0) ? l : dyfu - g2 * (ce + Hn);
   L = l;
+}
+
+template <class S>
+GPO_HD void yslope(const Geo& g, const Fwd<S>& w, const S* Hi, const S* fc, int s, int c, S& q0, S& q1, S& L) {
+  yslope_v(g, ysten(g, w, Hi, fc, s, c), s, q0, q1, L);
 }
 
 template <class S>
@@ -606,7 +666,7 @@ GPO_HD void flux_rev(const Geo& g, const T* Qm, const T* Qp, T H, const T* lF, T
   const T Nx = (ap * Fm0 - am * Fp0) + ap * am * dq;
   const T Ny = (ap * Fm1 - am * Fp1) + ap * am * dfx;
   const T Z = (ap * Fm2 - am * Fp2) + ap * am * dhv;
-  const T rd = T(1) / d;
+  const T rd = qdiv(T(1), d);
   const T lNx = lF[0] * rd, lNy = lF[1] * rd, lZ = (T(1) - fb) * lF[2] * rd;
   T ld = -(lNx * Nx + lNy * Ny + lZ * Z) * rd;
   T lap = lNx * (Fm0 + am * dq) + lNy * (Fm1 + am * dfx) + lZ * (Fm2 + am * dhv) + ld;
@@ -631,13 +691,13 @@ GPO_HD void flux_rev(const Geo& g, const T* Qm, const T* Qp, T H, const T* lF, T
     lhp += lFp0 * Qp[1] + lFp1 * (Qp[1] * Qp[1] + G * hp) + lFp2 * Qp[1] * Qp[2];
     lup += lFp0 * hp + lFp1 * T(2) * hp * Qp[1] + lFp2 * hp * Qp[2];
     lvp += lFp2 * hp * Qp[1];
-    if (cp > T(0)) lhp += lcp * G / (T(2) * cp);
+    if (cp > T(0)) lhp += qdiv(lcp * G, T(2) * cp);
   }
   if (wm) {
     lhm += lFm0 * Qm[1] + lFm1 * (Qm[1] * Qm[1] + G * hm) + lFm2 * Qm[1] * Qm[2];
     lum += lFm0 * hm + lFm1 * T(2) * hm * Qm[1] + lFm2 * hm * Qm[2];
     lvm += lFm2 * hm * Qm[1];
-    if (cm > T(0)) lhm += lcm * G / (T(2) * cm);
+    if (cm > T(0)) lhm += qdiv(lcm * G, T(2) * cm);
   }
   lQp[0] += lQp0 + lhp;
   lQp[1] += lup;
@@ -665,33 +725,34 @@ GPO_HD inline CellAdj<T> out_adj(const Geo& g, const T* Hi, const T* out_e, cons
   return a;
 }
 
+// Face sensitivities of one face: those of the reconstructed (eta, first,
+// second) state on its minus (m) and plus (p) side, the eta one zeroed where
+// the -H clamp was active.
+template <class T> struct FaceSens {
+  T m[3], p[3];
+};
+
 // A1: reverse of the face fluxes.  x-face c of row r (between cells c and
-// c+1), given the x-slopes (q0, q1, K) of both cells.
+// c+1) from loaded values: the two cells' update sensitivities (out_adj),
+// converted states and x-slopes (q0, q1, K), the face depth H and the
+// coriolis values.
 template <class T>
-GPO_HD void a1_xface(const Geo& g, T dt, int rk_step, const T* Hi, const T* fc, const Fwd<T>& w, const Rev<T>& rv,
-                     const T* out_e, const T* lo_e, const T* lo_u, const T* lo_v, int r, int c, T qm0, T qm1, T km,
-                     T qp0, T qp1, T kp) {
-  const size_t o = (size_t)r * g.W + c;
+GPO_HD FaceSens<T> xface_rev_v(const Geo& g, T dt, int rk_step, const CellAdj<T>& am, const CellAdj<T>& ap, T ebm,
+                               T um, T vm0, T ebp, T up, T vp0, T qm0, T qm1, T km, T qp0, T qp1, T kp, T H, T fm,
+                               T fp, int c) {
   const T coef = (rk_step == 0 ? T(1) : T(0.5)) * dt;
   const T G = T(g.g), g2 = T(2) * G;
   // lambda of the three x-fluxes at face c: L_k = -(F_c - F_{c-1})/dx - ...
-  const CellAdj<T> am = out_adj(g, Hi, out_e, lo_e, lo_u, lo_v, r, c);
-  const CellAdj<T> ap = out_adj(g, Hi, out_e, lo_e, lo_u, lo_v, r, c + 1);
   const T cd = coef / T(g.dx);
   const T lF[3] = {cd * (ap.e - am.e), cd * (ap.u - am.u), cd * (ap.v - am.v)};
-  const size_t m = o, p = o + 1;
-  const T ebp = w.ce[p], ebm = w.ce[m], up = w.cu[p], um = w.cu[m];
-  const T vp0 = w.cv[p], vm0 = w.cv[m];
   const T Rpx = up - T(0.5) * qp0, Rpy = vp0 - T(0.5) * qp1;
   const T Rmx = um + T(0.5) * qm0, Rmy = vm0 + T(0.5) * qm1;
-  const T H = T(0.5) * (corner(g, Hi, r, c + 1) + corner(g, Hi, r + 1, c + 1));
   const T sm = (g.bc_w == 1 && c + 1 == 2) ? T(-1) : T(1);
   const T sp = (g.bc_e == 1 && c + 1 == g.nx + 2) ? T(-1) : T(1);
   const T nx_ = T(g.north_x), ny_ = T(g.north_y);
   const T vpn = up * nx_ + (sp * vp0) * ny_, vmn = um * nx_ + (sm * vm0) * ny_;
-  const T fp = fcor(g, fc, r, c + 1), fm = fcor(g, fc, r, c);
-  const T argp = ebp - (kp + T(g.dx) * fp * vpn) / g2;
-  const T argm = ebm + (km + T(g.dx) * fm * vmn) / g2;
+  const T argp = ebp - qdiv(kp + T(g.dx) * fp * vpn, g2);
+  const T argm = ebm + qdiv(km + T(g.dx) * fm * vmn, g2);
   const T Qp[3] = {(-H < argp) ? argp : -H, Rpx, Rpy}, Qm[3] = {(-H < argm) ? argm : -H, Rmx, Rmy};
   T lQp[3] = {0, 0, 0}, lQm[3] = {0, 0, 0};
   const bool dm = ebm == T(kDry), dp = ebp == T(kDry);
@@ -704,42 +765,46 @@ GPO_HD void a1_xface(const Geo& g, T dt, int rk_step, const T* Hi, const T* fc, 
   } else {
     flux_rev(g, Qm, Qp, H, lF, lQm, lQp);
   }
-  // the reconstructed-eta sensitivity is zero where the -H clamp was active
-  rv.xl[0][o] = (-H < argm) ? lQm[0] : T(0);
-  rv.xl[1][o] = lQm[1];
-  rv.xl[2][o] = lQm[2];
-  rv.xr[0][o] = (-H < argp) ? lQp[0] : T(0);
-  rv.xr[1][o] = lQp[1];
-  rv.xr[2][o] = lQp[2];
+  FaceSens<T> o;
+  o.m[0] = (-H < argm) ? lQm[0] : T(0);
+  o.m[1] = lQm[1];
+  o.m[2] = lQm[2];
+  o.p[0] = (-H < argp) ? lQp[0] : T(0);
+  o.p[1] = lQp[1];
+  o.p[2] = lQp[2];
+  return o;
 }
-
-// y-face r of column c (between rows r and r+1), given the y-slopes
-// (q0, q1, L) of both cells.
 template <class T>
-GPO_HD void a1_yface(const Geo& g, T dt, int rk_step, const T* Hi, const T* fc, const Fwd<T>& w, const Rev<T>& rv,
-                     const T* out_e, const T* lo_e, const T* lo_u, const T* lo_v, int r, int c, T qm0, T qm1, T lm,
-                     T qp0, T qp1, T lp) {
-  const size_t o = (size_t)r * g.W + c;
+GPO_HD void a1_xface(const Geo& g, T dt, int rk_step, const T* Hi, const T* fc, const Fwd<T>& w, const Rev<T>& rv,
+                     const T* out_e, const T* lo_e, const T* lo_u, const T* lo_v, int r, int c, T qm0, T qm1, T km,
+                     T qp0, T qp1, T kp) {
+  const size_t o = (size_t)r * g.W + c, p = o + 1;
+  const CellAdj<T> am = out_adj(g, Hi, out_e, lo_e, lo_u, lo_v, r, c);
+  const CellAdj<T> ap = out_adj(g, Hi, out_e, lo_e, lo_u, lo_v, r, c + 1);
+  const T H = T(0.5) * (corner(g, Hi, r, c + 1) + corner(g, Hi, r + 1, c + 1));
+  const FaceSens<T> fs = xface_rev_v(g, dt, rk_step, am, ap, w.ce[o], w.cu[o], w.cv[o], w.ce[p], w.cu[p], w.cv[p],
+                                     qm0, qm1, km, qp0, qp1, kp, H, fcor(g, fc, r, c), fcor(g, fc, r, c + 1), c);
+  for (int i = 0; i < 3; ++i) rv.xl[i][o] = fs.m[i], rv.xr[i][o] = fs.p[i];
+}
+// y-face r of column c (between rows r and r+1) from loaded values (the
+// reconstructed state is stored as (eta, v, u)).
+template <class T>
+GPO_HD FaceSens<T> yface_rev_v(const Geo& g, T dt, int rk_step, const CellAdj<T>& am, const CellAdj<T>& ap, T ebm,
+                               T um0, T vm, T ebp, T up0, T vp, T qm0, T qm1, T lm, T qp0, T qp1, T lp, T H, T fm,
+                               T fp, int r) {
   const T coef = (rk_step == 0 ? T(1) : T(0.5)) * dt;
   const T G = T(g.g), g2 = T(2) * G;
-  const CellAdj<T> am = out_adj(g, Hi, out_e, lo_e, lo_u, lo_v, r, c);
-  const CellAdj<T> ap = out_adj(g, Hi, out_e, lo_e, lo_u, lo_v, r + 1, c);
   const T cd = coef / T(g.dy);
   const T lG[3] = {cd * (ap.e - am.e), cd * (ap.u - am.u), cd * (ap.v - am.v)};
   const T lfl[3] = {lG[0], lG[2], lG[1]};  // G = (fl.x, fl.z, fl.y)
-  const size_t m = o, p = o + g.W;
-  const T ebp = w.ce[p], ebm = w.ce[m], vp = w.cv[p], vm = w.cv[m];
-  const T up0 = w.cu[p], um0 = w.cu[m];
   const T Rpx = up0 - T(0.5) * qp0, Rpy = vp - T(0.5) * qp1;
   const T Rmx = um0 + T(0.5) * qm0, Rmy = vm + T(0.5) * qm1;
-  const T H = T(0.5) * (corner(g, Hi, r + 1, c) + corner(g, Hi, r + 1, c + 1));
   const T sm = (g.bc_s == 1 && r + 1 == 2) ? T(-1) : T(1);
   const T sp = (g.bc_n == 1 && r + 1 == g.ny + 2) ? T(-1) : T(1);
   const T ex = T(g.north_y), ey = T(-g.north_x);
   const T upe = (sp * up0) * ex + vp * ey, ume = (sm * um0) * ex + vm * ey;
-  const T fp = fcor(g, fc, r + 1, c), fm = fcor(g, fc, r, c);
-  const T argp = ebp - (lp - T(g.dy) * fp * upe) / g2;
-  const T argm = ebm + (lm - T(g.dy) * fm * ume) / g2;
+  const T argp = ebp - qdiv(lp - T(g.dy) * fp * upe, g2);
+  const T argm = ebm + qdiv(lm - T(g.dy) * fm * ume, g2);
   const T Qp[3] = {(-H < argp) ? argp : -H, Rpy, Rpx}, Qm[3] = {(-H < argm) ? argm : -H, Rmy, Rmx};
   T lQp[3] = {0, 0, 0}, lQm[3] = {0, 0, 0};
   const bool dm = ebm == T(kDry), dp = ebp == T(kDry);
@@ -752,12 +817,26 @@ GPO_HD void a1_yface(const Geo& g, T dt, int rk_step, const T* Hi, const T* fc, 
   } else {
     flux_rev(g, Qm, Qp, H, lfl, lQm, lQp);
   }
-  rv.yl[0][o] = (-H < argm) ? lQm[0] : T(0);
-  rv.yl[1][o] = lQm[1];
-  rv.yl[2][o] = lQm[2];
-  rv.yr[0][o] = (-H < argp) ? lQp[0] : T(0);
-  rv.yr[1][o] = lQp[1];
-  rv.yr[2][o] = lQp[2];
+  FaceSens<T> o;
+  o.m[0] = (-H < argm) ? lQm[0] : T(0);
+  o.m[1] = lQm[1];
+  o.m[2] = lQm[2];
+  o.p[0] = (-H < argp) ? lQp[0] : T(0);
+  o.p[1] = lQp[1];
+  o.p[2] = lQp[2];
+  return o;
+}
+template <class T>
+GPO_HD void a1_yface(const Geo& g, T dt, int rk_step, const T* Hi, const T* fc, const Fwd<T>& w, const Rev<T>& rv,
+                     const T* out_e, const T* lo_e, const T* lo_u, const T* lo_v, int r, int c, T qm0, T qm1, T lm,
+                     T qp0, T qp1, T lp) {
+  const size_t o = (size_t)r * g.W + c, p = o + g.W;
+  const CellAdj<T> am = out_adj(g, Hi, out_e, lo_e, lo_u, lo_v, r, c);
+  const CellAdj<T> ap = out_adj(g, Hi, out_e, lo_e, lo_u, lo_v, r + 1, c);
+  const T H = T(0.5) * (corner(g, Hi, r + 1, c) + corner(g, Hi, r + 1, c + 1));
+  const FaceSens<T> fs = yface_rev_v(g, dt, rk_step, am, ap, w.ce[o], w.cu[o], w.cv[o], w.ce[p], w.cu[p], w.cv[p],
+                                     qm0, qm1, lm, qp0, qp1, lp, H, fcor(g, fc, r, c), fcor(g, fc, r + 1, c), r);
+  for (int i = 0; i < 3; ++i) rv.yl[i][o] = fs.m[i], rv.yr[i][o] = fs.p[i];
 }
 
 // Which faces exist: x-faces c in [1, nx+1] of interior rows, y-faces r in
@@ -801,12 +880,10 @@ template <class T> GPO_HD inline void minmod_rev(T b, T c, T f, T lout, T& lb, T
 // Reverse of the x-slope computation at (r, s) given the slope sensitivities;
 // adds the contributions to cells s-1, s, s+1 (index 0, 1, 2) into le/lu/lv.
 template <class T>
-GPO_HD void xslope_rev(const Geo& g, const Fwd<T>& w, const T* Hi, const T* fc, int r, int s, T lq0, T lq1,
-                           T lK, T* le, T* lu, T* lv) {
-  const size_t o = (size_t)r * g.W + s;
-  const T le_ = w.ce[o - 1], ce = w.ce[o], re = w.ce[o + 1];
-  const T lu_ = w.cu[o - 1], cu = w.cu[o], ru = w.cu[o + 1];
-  const T lv0 = w.cv[o - 1], cv0 = w.cv[o], rv0 = w.cv[o + 1];
+GPO_HD void xslope_rev_v(const Geo& g, const Sten<T>& st, int s, T lq0, T lq1, T lK, T* le, T* lu, T* lv) {
+  const T le_ = st.e0, ce = st.e1, re = st.e2;
+  const T lu_ = st.u0, cu = st.u1, ru = st.u2;
+  const T lv0 = st.v0, cv0 = st.v1, rv0 = st.v2;
   const T th = T(g.theta);
   {  // q0 = minmod_slope(lu, cu, ru), q1 = minmod_slope(lv, cv, rv)
     T lb = 0, lc = 0, lf = 0;
@@ -830,7 +907,7 @@ GPO_HD void xslope_rev(const Geo& g, const Fwd<T>& w, const T* Hi, const T* fc, 
     if (s > g.nx + 1) sc = -1;
   }
   const T nx_ = T(g.north_x), ny_ = T(g.north_y), G = T(g.g), Vc = T(g.dx) / (T(2) * G), g2 = T(2) * G;
-  const T Fl = fcor(g, fc, r, s - 1), Fc = fcor(g, fc, r, s), Fr = fcor(g, fc, r, s + 1);
+  const T Fl = st.F0, Fc = st.F1, Fr = st.F2;
   const T lfv = (nx_ * lu_ + ny_ * (sl * lv0)) * Fl;
   const T cfv = (nx_ * cu + ny_ * (sc * cv0)) * Fc;
   const T rfv = (nx_ * ru + ny_ * (sr * rv0)) * Fr;
@@ -840,10 +917,10 @@ GPO_HD void xslope_rev(const Geo& g, const Fwd<T>& w, const T* Hi, const T* fc, 
   const T K0 = minmod_raw(b, c, f);
   const T sa = ((g.bc_w == 1 && s < 2) || (g.bc_e == 1 && s > g.nx + 2)) ? T(-1) : T(1);
   const T dxfv = T(g.dx) * Fc * (sa * cv0);
-  const T Hw = T(0.5) * (corner(g, Hi, r, s) + corner(g, Hi, r + 1, s));
-  const T He = T(0.5) * (corner(g, Hi, r, s + 1) + corner(g, Hi, r + 1, s + 1));
-  const T hw = ce + Hw - (K0 + dxfv) / g2;
-  const T he = ce + He + (K0 + dxfv) / g2;
+  const T Hw = T(0.5) * (st.Ha + st.Hb);
+  const T He = T(0.5) * (st.Hc + st.Hd);
+  const T hw = ce + Hw - qdiv(K0 + dxfv, g2);
+  const T he = ce + He + qdiv(K0 + dxfv, g2);
   T lK1 = lK, ldxfv = 0, lce = 0;
   if (!(he > T(0))) {
     ldxfv -= lK;
@@ -877,12 +954,16 @@ GPO_HD void xslope_rev(const Geo& g, const Fwd<T>& w, const T* Hi, const T* fc, 
 }
 
 template <class T>
-GPO_HD void yslope_rev(const Geo& g, const Fwd<T>& w, const T* Hi, const T* fc, int s, int c, T lq0, T lq1,
-                           T lL, T* le, T* lu, T* lv) {
-  const size_t o = (size_t)s * g.W + c, up = g.W;
-  const T le_ = w.ce[o - up], ce = w.ce[o], ue = w.ce[o + up];
-  const T lu0 = w.cu[o - up], cu0 = w.cu[o], uu0 = w.cu[o + up];
-  const T lv_ = w.cv[o - up], cv = w.cv[o], uv = w.cv[o + up];
+GPO_HD void xslope_rev(const Geo& g, const Fwd<T>& w, const T* Hi, const T* fc, int r, int s, T lq0, T lq1, T lK,
+                       T* le, T* lu, T* lv) {
+  xslope_rev_v(g, xsten(g, w, Hi, fc, r, s), s, lq0, lq1, lK, le, lu, lv);
+}
+
+template <class T>
+GPO_HD void yslope_rev_v(const Geo& g, const Sten<T>& st, int s, T lq0, T lq1, T lL, T* le, T* lu, T* lv) {
+  const T le_ = st.e0, ce = st.e1, ue = st.e2;
+  const T lu0 = st.u0, cu0 = st.u1, uu0 = st.u2;
+  const T lv_ = st.v0, cv = st.v1, uv = st.v2;
   const T th = T(g.theta);
   {
     T lb = 0, lc = 0, lf = 0;
@@ -906,7 +987,7 @@ GPO_HD void yslope_rev(const Geo& g, const Fwd<T>& w, const T* Hi, const T* fc, 
     if (s > g.ny + 1) sc = -1;
   }
   const T ex = T(g.north_y), ey = T(-g.north_x), G = T(g.g), Uc = T(g.dy) / (T(2) * G), g2 = T(2) * G;
-  const T Fl = fcor(g, fc, s - 1, c), Fc = fcor(g, fc, s, c), Fu = fcor(g, fc, s + 1, c);
+  const T Fl = st.F0, Fc = st.F1, Fu = st.F2;
   const T lfu = (ex * (sl * lu0) + ey * lv_) * Fl;
   const T cfu = (ex * (sc * cu0) + ey * cv) * Fc;
   const T ufu = (ex * (su * uu0) + ey * uv) * Fu;
@@ -916,10 +997,10 @@ GPO_HD void yslope_rev(const Geo& g, const Fwd<T>& w, const T* Hi, const T* fc, 
   const T L0 = minmod_raw(b, cc, f);
   const T sa = ((g.bc_s == 1 && s < 2) || (g.bc_n == 1 && s > g.ny + 2)) ? T(-1) : T(1);
   const T dyfu = T(g.dy) * Fc * (sa * cu0);
-  const T Hs = T(0.5) * (corner(g, Hi, s, c) + corner(g, Hi, s, c + 1));
-  const T Hn = T(0.5) * (corner(g, Hi, s + 1, c) + corner(g, Hi, s + 1, c + 1));
-  const T hs = ce + Hs - (L0 - dyfu) / g2;
-  const T hn = ce + Hn + (L0 - dyfu) / g2;
+  const T Hs = T(0.5) * (st.Ha + st.Hb);
+  const T Hn = T(0.5) * (st.Hc + st.Hd);
+  const T hs = ce + Hs - qdiv(L0 - dyfu, g2);
+  const T hn = ce + Hn + qdiv(L0 - dyfu, g2);
   T lL1 = lL, ldyfu = 0, lce = 0;
   if (!(hn > T(0))) {
     ldyfu += lL;
@@ -952,6 +1033,12 @@ GPO_HD void yslope_rev(const Geo& g, const Fwd<T>& w, const T* Hi, const T* fc, 
   lv[2] += ey * Fu * lufu;
 }
 
+template <class T>
+GPO_HD void yslope_rev(const Geo& g, const Fwd<T>& w, const T* Hi, const T* fc, int s, int c, T lq0, T lq1, T lL,
+                       T* le, T* lu, T* lv) {
+  yslope_rev_v(g, ysten(g, w, Hi, fc, s, c), s, lq0, lq1, lL, le, lu, lv);
+}
+
 // A2: per converted cell (r, c): the reverse of its update and sources, its
 // face parts, and the reverse of its own x- and y-slopes, split into the part
 // for the cell itself and the parts for its four neighbours.  Zero outside
@@ -959,41 +1046,46 @@ GPO_HD void yslope_rev(const Geo& g, const Fwd<T>& w, const T* Hi, const T* fc, 
 template <class T> struct CellParts {
   T own[3], xm[3], xp[3], ym[3], yp[3];
 };
+// A2 on loaded values: the cell's four corner depths (H_rc with r, c the
+// offsets), its coriolis value F, its x- and y-slope stencils, its update
+// sensitivity a (out_adj's rule), the sensitivities of its side of x-face c
+// (xl), of x-face c-1 (xr), of y-face r (yl) and of y-face r-1 (yr).
 template <class T>
-GPO_HD CellParts<T> a2_parts(const Geo& g, T dt, int rk_step, const T* Hi, const T* fc, const Fwd<T>& w,
-                             const Rev<T>& rv, const T* out_e, const T* lo_e, const T* lo_u, const T* lo_v, int r,
-                             int c) {
+GPO_HD CellParts<T> a2_parts_v(const Geo& g, T dt, int rk_step, int r, int c, T H00, T H10, T H01, T H11, T F,
+                               const Sten<T>& sx, const Sten<T>& sy, const CellAdj<T>& a, const T* xl, const T* xr,
+                               const T* yl, const T* yr) {
   CellParts<T> out;
   for (int i = 0; i < 3; ++i) out.own[i] = out.xm[i] = out.xp[i] = out.ym[i] = out.yp[i] = T(0);
   if (c < 0 || r < 0 || c >= g.nx + 4 || r >= g.ny + 4) return out;
-  const size_t o = (size_t)r * g.W + c;
   const bool row_in = r >= 2 && r <= g.ny + 1, col_in = c >= 2 && c <= g.nx + 1;
+  const bool left = row_in && c >= 1 && c <= g.nx + 1, right = row_in && c - 1 >= 1 && c - 1 <= g.nx + 1;
+  const bool low = col_in && r >= 1 && r <= g.ny + 1, high = col_in && r - 1 >= 1 && r - 1 <= g.ny + 1;
+  const bool upd = row_in && col_in;
   T lR0 = 0, lR1 = 0, lR2 = 0;
   T* xm = out.xm;
   T* xp = out.xp;
   T* ym = out.ym;
   T* yp = out.yp;
   const T G = T(g.g);
-  if (row_in && col_in) {
+  if (upd) {
     // update + sources (CDKLM16_kernel.cu:869-1089), reversed
-    const CellAdj<T> a = out_adj(g, Hi, out_e, lo_e, lo_u, lo_v, r, c);
+    const T hm = T(0.25) * (H00 + H10 + H01 + H11);
     const T coef = rk_step == 0 ? T(1) : T(0.5);
-    const T hm = cell_depth(g, Hi, r, c);
-    const T R0 = w.ce[o], R1 = w.cu[o], R2 = w.cv[o];
+    const T R0 = sx.e1, R1 = sx.u1, R2 = sx.v1;
     lR0 += coef * a.e;
     T lhu = coef * a.u, lhv = coef * a.v;
     const T lst1 = coef * dt * a.u, lst2 = coef * dt * a.v;
     const T h = R0 + hm;
     if (h >= T(g.depth_cutoff) && R0 != T(kDry)) {
-      const T RHxp = T(0.5) * (corner(g, Hi, r, c + 1) + corner(g, Hi, r + 1, c + 1));
-      const T RHxm = T(0.5) * (corner(g, Hi, r, c) + corner(g, Hi, r + 1, c));
-      const T RHyp = T(0.5) * (corner(g, Hi, r + 1, c) + corner(g, Hi, r + 1, c + 1));
-      const T RHym = T(0.5) * (corner(g, Hi, r, c) + corner(g, Hi, r, c + 1));
+      const T RHxp = T(0.5) * (H01 + H11);
+      const T RHxm = T(0.5) * (H00 + H10);
+      const T RHyp = T(0.5) * (H10 + H11);
+      const T RHym = T(0.5) * (H00 + H01);
       // bathymetry terms: eta_we = (eta_west + eta_east)/2 depends on R0 only
       lR0 += G * (RHxp - RHxm) * (lst1 * T(1.0 / g.dx));
       lR0 += G * (RHyp - RHym) * (lst2 * T(1.0 / g.dy));
       // Coriolis terms (linear in hu, hv)
-      const T f = fcor(g, fc, r, c);
+      const T f = F;
       const T nx_ = T(g.north_x), ny_ = T(g.north_y), ex = ny_, ey = -nx_;
       const T upx = -nx_, upy = ny_, rtx = upy, rty = -upx;
       const T lhu_east_cor = rtx * lst1 + upx * lst2;
@@ -1013,13 +1105,12 @@ GPO_HD CellParts<T> a2_parts(const Geo& g, T dt, int rk_step, const T* Hi, const
     // face parts: the cell is the left side of x-face c and the right side of
     // x-face c-1; the reconstruction there is eta -/+ (K + dx f v_n)/(2g),
     // u -/+ q0/2, v -/+ q1/2, with v mirrored at a wall's first ghost cell
-    const bool left = c >= 1 && c <= g.nx + 1, right = c - 1 >= 1 && c - 1 <= g.nx + 1;
     T lq0 = 0, lq1 = 0, lK = 0;
     if (left || right) {
-      const T rg2 = T(0.5 / g.g), f = fcor(g, fc, r, c), nx_ = T(g.north_x), ny_ = T(g.north_y);
+      const T rg2 = T(0.5 / g.g), f = F, nx_ = T(g.north_x), ny_ = T(g.north_y);
       const T fdx = T(g.dx) * f * rg2;
       if (left) {
-        const T a = rv.xl[0][o], b = rv.xl[1][o], d = rv.xl[2][o];
+        const T a = xl[0], b = xl[1], d = xl[2];
         const T sm = (g.bc_w == 1 && c == 1) ? T(-1) : T(1);
         const T lvn = a * fdx;
         lR0 += a;
@@ -1030,7 +1121,7 @@ GPO_HD CellParts<T> a2_parts(const Geo& g, T dt, int rk_step, const T* Hi, const
         lK += a * rg2;
       }
       if (right) {
-        const T a = rv.xr[0][o - 1], b = rv.xr[1][o - 1], d = rv.xr[2][o - 1];
+        const T a = xr[0], b = xr[1], d = xr[2];
         const T sp = (g.bc_e == 1 && c == g.nx + 2) ? T(-1) : T(1);
         const T lvn = -a * fdx;
         lR0 += a;
@@ -1043,7 +1134,7 @@ GPO_HD CellParts<T> a2_parts(const Geo& g, T dt, int rk_step, const T* Hi, const
     }
     if ((left || right) && !(lq0 == T(0) && lq1 == T(0) && lK == T(0))) {
       T e3[3] = {0, 0, 0}, u3[3] = {0, 0, 0}, v3[3] = {0, 0, 0};
-      xslope_rev(g, w, Hi, fc, r, c, lq0, lq1, lK, e3, u3, v3);
+      xslope_rev_v(g, sx, c, lq0, lq1, lK, e3, u3, v3);
       xm[0] = e3[0], xm[1] = u3[0], xm[2] = v3[0];
       lR0 += e3[1], lR1 += u3[1], lR2 += v3[1];
       xp[0] = e3[2], xp[1] = u3[2], xp[2] = v3[2];
@@ -1052,13 +1143,12 @@ GPO_HD CellParts<T> a2_parts(const Geo& g, T dt, int rk_step, const T* Hi, const
   if (col_in) {
     // y-faces r (cell below) and r-1 (cell above): eta -/+ (L - dy f u_e)/(2g),
     // v -/+ q1/2, u -/+ q0/2; stored as (eta, v, u)
-    const bool low = r >= 1 && r <= g.ny + 1, high = r - 1 >= 1 && r - 1 <= g.ny + 1;
     T lq0 = 0, lq1 = 0, lL = 0;
     if (low || high) {
-      const T rg2 = T(0.5 / g.g), f = fcor(g, fc, r, c), ex = T(g.north_y), ey = T(-g.north_x);
+      const T rg2 = T(0.5 / g.g), f = F, ex = T(g.north_y), ey = T(-g.north_x);
       const T fdy = T(g.dy) * f * rg2;
       if (low) {
-        const T a = rv.yl[0][o], bv = rv.yl[1][o], bu = rv.yl[2][o];
+        const T a = yl[0], bv = yl[1], bu = yl[2];
         const T sm = (g.bc_s == 1 && r == 1) ? T(-1) : T(1);
         const T lue = -a * fdy;
         lR0 += a;
@@ -1069,7 +1159,7 @@ GPO_HD CellParts<T> a2_parts(const Geo& g, T dt, int rk_step, const T* Hi, const
         lL += a * rg2;
       }
       if (high) {
-        const T a = rv.yr[0][o - g.W], bv = rv.yr[1][o - g.W], bu = rv.yr[2][o - g.W];
+        const T a = yr[0], bv = yr[1], bu = yr[2];
         const T sp = (g.bc_n == 1 && r == g.ny + 2) ? T(-1) : T(1);
         const T lue = a * fdy;
         lR0 += a;
@@ -1082,7 +1172,7 @@ GPO_HD CellParts<T> a2_parts(const Geo& g, T dt, int rk_step, const T* Hi, const
     }
     if ((low || high) && !(lq0 == T(0) && lq1 == T(0) && lL == T(0))) {
       T e3[3] = {0, 0, 0}, u3[3] = {0, 0, 0}, v3[3] = {0, 0, 0};
-      yslope_rev(g, w, Hi, fc, r, c, lq0, lq1, lL, e3, u3, v3);
+      yslope_rev_v(g, sy, r, lq0, lq1, lL, e3, u3, v3);
       ym[0] = e3[0], ym[1] = u3[0], ym[2] = v3[0];
       lR0 += e3[1], lR1 += u3[1], lR2 += v3[1];
       yp[0] = e3[2], yp[1] = u3[2], yp[2] = v3[2];
@@ -1092,6 +1182,47 @@ GPO_HD CellParts<T> a2_parts(const Geo& g, T dt, int rk_step, const T* Hi, const
   out.own[1] = lR1;
   out.own[2] = lR2;
   return out;
+}
+
+template <class T>
+GPO_HD CellParts<T> a2_parts(const Geo& g, T dt, int rk_step, const T* Hi, const T* fc, const Fwd<T>& w,
+                             const Rev<T>& rv, const T* out_e, const T* lo_e, const T* lo_u, const T* lo_v, int r,
+                             int c) {
+  if (c < 0 || r < 0 || c >= g.nx + 4 || r >= g.ny + 4) {
+    CellParts<T> out;
+    for (int i = 0; i < 3; ++i) out.own[i] = out.xm[i] = out.xp[i] = out.ym[i] = out.yp[i] = T(0);
+    return out;
+  }
+  const size_t o = (size_t)r * g.W + c;
+  const bool row_in = r >= 2 && r <= g.ny + 1, col_in = c >= 2 && c <= g.nx + 1;
+  const bool left = row_in && c >= 1 && c <= g.nx + 1, right = row_in && c - 1 >= 1 && c - 1 <= g.nx + 1;
+  const bool low = col_in && r >= 1 && r <= g.ny + 1, high = col_in && r - 1 >= 1 && r - 1 <= g.ny + 1;
+  // Every value the cell may use is loaded here, before any branch on the
+  // data, so the loads are in flight together; where a value is not used its
+  // load reads the cell itself.
+  const bool xs = left || right, ys = low || high;
+  const int cw = xs ? c - 1 : c, ce_ = xs ? c + 1 : c, rs = ys ? r - 1 : r, rn = ys ? r + 1 : r;
+  const size_t ow = (size_t)r * g.W + cw, oe = (size_t)r * g.W + ce_;
+  const size_t os = (size_t)rs * g.W + c, on = (size_t)rn * g.W + c;
+  const T H00 = corner(g, Hi, r, c), H10 = corner(g, Hi, r + 1, c);
+  const T H01 = corner(g, Hi, r, c + 1), H11 = corner(g, Hi, r + 1, c + 1);
+  const T F = fcor(g, fc, r, c);
+  const Sten<T> sx = {w.ce[ow], w.ce[o], w.ce[oe], w.cu[ow], w.cu[o], w.cu[oe], w.cv[ow], w.cv[o], w.cv[oe],
+                      fcor(g, fc, r, cw), F, fcor(g, fc, r, ce_), H00, H10, H01, H11};
+  const Sten<T> sy = {w.ce[os], sx.e1, w.ce[on], w.cu[os], sx.u1, w.cu[on], w.cv[os], sx.v1, w.cv[on],
+                      fcor(g, fc, rs, c), F, fcor(g, fc, rn, c), H00, H01, H10, H11};
+  const size_t oxr = right ? o - 1 : o, oyr = high ? o - g.W : o;
+  const T xl[3] = {rv.xl[0][o], rv.xl[1][o], rv.xl[2][o]};
+  const T xr[3] = {rv.xr[0][oxr], rv.xr[1][oxr], rv.xr[2][oxr]};
+  const T yl[3] = {rv.yl[0][o], rv.yl[1][o], rv.yl[2][o]};
+  const T yr[3] = {rv.yr[0][oyr], rv.yr[1][oyr], rv.yr[2][oyr]};
+  const size_t k = (size_t)r * g.sp + c;
+  const T oute = out_e[k], loe = lo_e[k], lou = lo_u[k], lov = lo_v[k];
+  // out_adj's rule on the loaded values (it applies only where the update does)
+  const T hm = T(0.25) * (H00 + H10 + H01 + H11);
+  CellAdj<T> a{0, 0, 0};
+  if (!(oute == -hm + T(g.depth_cutoff))) a = {loe, lou, lov};
+  return a2_parts_v(g, dt, rk_step, r, c, H00, H10, H01, H11, F, sx, sy, a, xl, xr, yl, yr);
 }
 
 // px of a cell from the parts of its west neighbour, itself and its east neighbour.
@@ -1106,10 +1237,29 @@ GPO_HD inline void store_parts(const Geo& g, const Rev<T>& rv, int r, int c, con
   }
 }
 
+// The reverse of the conversion on loaded values: the gathered sensitivity
+// (l0, l1, l2) of a converted cell with input eta e, depth bm and velocities
+// (u, v) -> that of its loaded (eta, hu, hv); shu/shv the signs of a
+// wall-mirrored momentum.  u = hu / den with den = h, or below eps the
+// desingularised a = h^2 / (2 eps) + eps / 2 (between the depth cutoff and
+// eps, min(a, eps) = a and max(a, |h|) = a), d den / dh = 1 or h / eps.
+template <class T>
+GPO_HD inline void conv_rev_vals(const Geo& g, T l0, T l1, T l2, T e, T bm, T u, T v, float shu, float shv, T& oe,
+                                 T& ou, T& ov) {
+  const T h = e + bm;
+  oe = ou = ov = T(0);
+  if (land_cell(g, bm) || !(h > T(g.depth_cutoff))) return;
+  const bool shallow = h < T(g.desing_eps);
+  const T den = shallow ? h * h * T(0.5 / g.desing_eps) + T(0.5 * g.desing_eps) : h;
+  const T dh = shallow ? h * T(1.0 / g.desing_eps) : T(1);
+  ou = qdiv(l1, den) * T(shu);
+  ov = qdiv(l2, den) * T(shv);
+  oe = l0 - qdiv((l1 * u + l2 * v) * dh, den);
+}
+
 // Sensitivity of the loaded stage input of converted cell (r, c) (any cell of
 // the grid): the gathered sensitivity of its (eta, u, v) through the
-// conversion u = hu/h, v = hv/h, h = eta + Hm (wet branch), with the sign of a
-// wall-mirrored momentum.
+// conversion (wet branch), with the sign of a wall-mirrored momentum.
 template <class T>
 GPO_HD void conv_rev(const Geo& g, const T* Hi, const Fwd<T>& w, const Rev<T>& rv, const T* in_e, int r, int c, T& oe,
                      T& ou, T& ov) {
@@ -1126,77 +1276,96 @@ GPO_HD void conv_rev(const Geo& g, const T* Hi, const Fwd<T>& w, const Rev<T>& r
     l2 += rv.ym[2][o + g.W];
   }
   const Src src = map_cell(g, r, c);
-  const T e = in_e[(size_t)src.r * g.sp + src.c];
-  const T bm = cell_bm(g, Hi, r, c);
-  const T h = e + bm;
-  oe = ou = ov = T(0);
-  if (land_cell(g, bm) || !(h > T(g.depth_cutoff))) return;  // land, or dried to a constant state
-  const T u = w.cu[o], v = w.cv[o];
-  // u = hu / den with den = h, or below eps the desingularised
-  // a = h^2 / (2 eps) + eps / 2 (between the depth cutoff and eps,
-  // min(a, eps) = a and max(a, |h|) = a), d den / dh = 1 or h / eps
-  const bool shallow = h < T(g.desing_eps);
-  const T den = shallow ? h * h * T(0.5 / g.desing_eps) + T(0.5 * g.desing_eps) : h;
-  const T dh = shallow ? h * T(1.0 / g.desing_eps) : T(1);
-  ou = l1 / den * T(src.shu);
-  ov = l2 / den * T(src.shv);
-  oe = l0 - (l1 * u + l2 * v) * dh / den;
+  conv_rev_vals(g, l0, l1, l2, in_e[(size_t)src.r * g.sp + src.c], cell_bm(g, Hi, r, c), w.cu[o], w.cv[o], src.shu,
+                src.shv, oe, ou, ov);
+}
+
+// A ghost cell of row r or column c may load from interior cell (r, c) only
+// in the two columns (rows) next to each side.
+GPO_HD inline bool may_receive_fold(const Geo& g, int r, int c) {
+  return c <= 3 || c >= g.nx || r <= 3 || r >= g.ny;
 }
 
 // A3: the sensitivity of interior input cell (r, c): its own converted cell
 // plus the ghost cells loaded from it; written to lambda_in, and for the
 // second RK stage the direct Q^n term to lambda_qn.  lq_mode 0 adds to
-// lambda_qn, 1 overwrites it.
+// lambda_qn, 1 overwrites it.  lambda_in may alias lambda_out (every value
+// of the cell is loaded before anything is stored); lambda_qn must not alias
+// lambda_in.  The cell is interior, so it is its own stencil cell and both
+// of its y-neighbours' parts exist: all of its loads are issued together,
+// then conv_rev's arithmetic runs on them (the same operations in the same
+// order as conv_rev).
 template <class T>
 GPO_HD void a3_cell(Geo g, int rk_step, int accumulate, int lq_mode, const T* Hi, const Fwd<T>& w, const Rev<T>& rv,
                     const T* in_e, const T* out_e, const T* lo_e, const T* lo_u, const T* lo_v, T* li_e, T* li_u,
                     T* li_v, T* lq_e, T* lq_u, T* lq_v, int r, int c) {
   if (c > g.nx + 1 || r > g.ny + 1) return;
-  T e, u, v;
-  conv_rev(g, Hi, w, rv, in_e, r, c, e, u, v);
-  const int gc[4] = {0, 1, g.nx + 2, g.nx + 3};
-  for (int i = 0; i < 4; ++i) {
-    const Src s = map_cell(g, r, gc[i]);
-    if (s.r == r && s.c == c && !(gc[i] == c)) {
-      T ge, gu, gv;
-      conv_rev(g, Hi, w, rv, in_e, r, gc[i], ge, gu, gv);
-      e += ge;
-      u += gu;
-      v += gv;
+  const size_t o = (size_t)r * g.W + c, k = (size_t)r * g.sp + c;
+  const T p0 = rv.px[0][o], p1 = rv.px[1][o], p2 = rv.px[2][o];
+  const T s0 = rv.yp[0][o - g.W], s1 = rv.yp[1][o - g.W], s2 = rv.yp[2][o - g.W];
+  const T n0 = rv.ym[0][o + g.W], n1 = rv.ym[1][o + g.W], n2 = rv.ym[2][o + g.W];
+  const T ein = in_e[k];
+  const T ha = corner(g, Hi, r, c), hb = corner(g, Hi, r + 1, c);
+  const T hc = corner(g, Hi, r, c + 1), hd = corner(g, Hi, r + 1, c + 1);
+  const T u = w.cu[o], v = w.cv[o];
+  const bool with_lq = rk_step == 1 && lq_e;
+  T oute = T(0), loe = T(0), lou = T(0), lov = T(0), qe = T(0), qu = T(0), qv = T(0);
+  if (with_lq) {
+    oute = out_e[k], loe = lo_e[k], lou = lo_u[k], lov = lo_v[k];
+    if (!lq_mode) qe = lq_e[k], qu = lq_u[k], qv = lq_v[k];
+  }
+  T ie = T(0), iu = T(0), iv = T(0);
+  if (accumulate) ie = li_e[k], iu = li_u[k], iv = li_v[k];
+
+  T e, uu, vv;
+  conv_rev_vals(g, (p0 + s0) + n0, (p1 + s1) + n1, (p2 + s2) + n2, ein, bm_of(g, ha, hb, hc, hd), u, v, 1.0f, 1.0f,
+                e, uu, vv);
+  if (may_receive_fold(g, r, c)) {
+    const int gc[4] = {0, 1, g.nx + 2, g.nx + 3};
+    for (int i = 0; i < 4; ++i) {
+      const Src s = map_cell(g, r, gc[i]);
+      if (s.r == r && s.c == c && !(gc[i] == c)) {
+        T ge, gu, gv;
+        conv_rev(g, Hi, w, rv, in_e, r, gc[i], ge, gu, gv);
+        e += ge;
+        uu += gu;
+        vv += gv;
+      }
+    }
+    const int gr[4] = {0, 1, g.ny + 2, g.ny + 3};
+    for (int i = 0; i < 4; ++i) {
+      const Src s = map_cell(g, gr[i], c);
+      if (s.r == r && s.c == c && !(gr[i] == r)) {
+        T ge, gu, gv;
+        conv_rev(g, Hi, w, rv, in_e, gr[i], c, ge, gu, gv);
+        e += ge;
+        uu += gu;
+        vv += gv;
+      }
     }
   }
-  const int gr[4] = {0, 1, g.ny + 2, g.ny + 3};
-  for (int i = 0; i < 4; ++i) {
-    const Src s = map_cell(g, gr[i], c);
-    if (s.r == r && s.c == c && !(gr[i] == r)) {
-      T ge, gu, gv;
-      conv_rev(g, Hi, w, rv, in_e, gr[i], c, ge, gu, gv);
-      e += ge;
-      u += gu;
-      v += gv;
-    }
-  }
-  const size_t k = (size_t)r * g.sp + c;
-  if (rk_step == 1 && lq_e) {  // read lambda_out before lambda_in may overwrite it (they may alias)
-    const CellAdj<T> a = out_adj(g, Hi, out_e, lo_e, lo_u, lo_v, r, c);
+  if (with_lq) {  // out_adj's rule on the loaded values
+    const T hm = T(0.25) * (ha + hb + hc + hd);
+    T ae = T(0), au = T(0), av = T(0);
+    if (!(oute == -hm + T(g.depth_cutoff))) ae = loe, au = lou, av = lov;
     if (lq_mode) {
-      lq_e[k] = T(0.5) * a.e;
-      lq_u[k] = T(0.5) * a.u;
-      lq_v[k] = T(0.5) * a.v;
+      lq_e[k] = T(0.5) * ae;
+      lq_u[k] = T(0.5) * au;
+      lq_v[k] = T(0.5) * av;
     } else {
-      lq_e[k] += T(0.5) * a.e;
-      lq_u[k] += T(0.5) * a.u;
-      lq_v[k] += T(0.5) * a.v;
+      lq_e[k] = qe + T(0.5) * ae;
+      lq_u[k] = qu + T(0.5) * au;
+      lq_v[k] = qv + T(0.5) * av;
     }
   }
   if (accumulate) {
-    li_e[k] += e;
-    li_u[k] += u;
-    li_v[k] += v;
+    li_e[k] = ie + e;
+    li_u[k] = iu + uu;
+    li_v[k] = iv + vv;
   } else {
     li_e[k] = e;
-    li_u[k] = u;
-    li_v[k] = v;
+    li_u[k] = uu;
+    li_v[k] = vv;
   }
 }
 

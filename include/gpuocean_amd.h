@@ -611,6 +611,14 @@ int gpo_adj_batch_trajectory(gpo_adj_batch b, int32_t m, void* stream);
  * over the trajectory of the last gpo_adj_batch_trajectory(b, m). */
 int gpo_adj_batch_sweep(gpo_adj_batch b, int32_t m, int32_t* flag, void* stream);
 int gpo_adj_batch_destroy(gpo_adj_batch b);
+/* How the sweep reverses each substep: fused = 1 (default) runs one strip
+ * march per (member, strip of 58 columns, chunk of `rows` rows) that keeps
+ * the stage's intermediates in registers, then folds the ghost-cell
+ * sensitivities; fused = 0 runs the four staged kernels (F1, A1, A2, A3)
+ * through workspace arrays.  Both give the same gradient bit for bit.
+ * rows = 0 keeps the current chunk height.  No counterpart in the reference
+ * (its adjoint does not exist); waits for the stream. */
+int gpo_adj_batch_set_reverse(gpo_adj_batch b, int32_t fused, int32_t rows, void* stream);
 
 /* Adjoint of the state dependence of gpo_geostrophic_balance for a given
  * coarse field: lambda_eta += -g/f d_y(coarse) lambda_hu + g/f d_x(coarse)
