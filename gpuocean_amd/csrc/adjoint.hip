@@ -793,7 +793,7 @@ int adjoint_stages<double>(const Geo& g, double dt, int rk, const double* ie, co
 // ---- batched differentiable ensemble
 
 // Rows per work item of the fused reverse march.
-constexpr int kDefaultMarchRows = 32;
+constexpr int kDefaultMarchRows = 24;
 
 struct AdjBatch {
   int n = 0, k = 0, n_seg = 0;
@@ -910,6 +910,39 @@ int adj_batch_march_table(AdjBatch* b, int rows, hipStream_t s) {
   return 0;
 }
 
+// One member's march has one wavefront per (strip, chunk): below this many the
+// GPU is mostly idle and the staged kernels, which spread a grid over many
+// more wavefronts, are faster (512 x 512: 198 march wavefronts, 2.8x slower).
+constexpr int kMarchMinItems = 2048;
+bool member_march(const Geo& g) {
+  const int strips = (g.nx + 4 + fast32::kMarchColsHost - 1) / fast32::kMarchColsHost;
+  return (long long)strips * ((g.ny + 4 + kDefaultMarchRows - 1) / kDefaultMarchRows) >= kMarchMinItems;
+}
+
+// One member's reverse substep through the fused march (fast32::launch_march):
+// lambda_in (li) must not alias lambda_out (lo) or lambda_qn (lq).
+int march_stage(const Geo& g, float dt, int rk, const float* const* in, const float* out_e, float* const* lo,
+                float* const* li, float* const* lq, int accumulate, int lq_mode, const float* Hi, const float* fc,
+                void* ws, hipStream_t s) {
+  AdjItem<float> it{};
+  it.g = g;
+  it.dt = dt;
+  it.rk = rk;
+  it.accumulate = accumulate;
+  it.lq_mode = lq_mode;
+  it.ie = gptr(in[0]), it.iu = gptr(in[1]), it.iv = gptr(in[2]);
+  it.out_e = gptr(out_e);
+  it.le = gptr((const float*)lo[0]), it.lu = gptr((const float*)lo[1]), it.lv = gptr((const float*)lo[2]);
+  it.lie = gptr(li[0]), it.liu = gptr(li[1]), it.liv = gptr(li[2]);
+  if (lq) it.lqe = gptr(lq[0]), it.lqu = gptr(lq[1]), it.lqv = gptr(lq[2]);
+  it.Hi = gptr(Hi);
+  it.fc = gptr(fc);
+  it.w = carve<float>(ws, g.nx, g.ny);
+  it.rv = carve_rev<float>(ws, g.nx, g.ny);
+  fast32::launch_march(it, kDefaultMarchRows, s);
+  return hip_check(hipGetLastError(), "adjoint march launch");
+}
+
 // The reverse sweep of gpo_cdklm16_adjoint_steps_f32 over caller-given (or
 // library scratch) lambda arrays and stage workspace.
 int adjoint_steps_impl(const gpo_cdklm16_params* p, int32_t n_steps, float dt, const float* const* qs,
@@ -927,6 +960,7 @@ int adjoint_steps_impl(const gpo_cdklm16_params* p, int32_t n_steps, float dt, c
   float* cur[3] = {lam[0], lam[1], lam[2]};     // lambda at Q^{j+1}, then at Q^j
   float* nxt[3] = {work[0], work[1], work[2]};  // lambda being formed at Q^j
   float* l1[3] = {work[3], work[4], work[5]};   // lambda at the RK stage Q^(1)
+  const bool march = member_march(g);
   for (int j = n_steps - 1; j >= 0; --j) {
     const float* const* a = qs + 3 * j;
     const float* const* b = q1s + 3 * j;
@@ -935,10 +969,14 @@ int adjoint_steps_impl(const gpo_cdklm16_params* p, int32_t n_steps, float dt, c
     // the first (adds its sensitivity of Q^n to nxt); with flow relaxation,
     // the transpose of each sponge blend before the stage that produced it
     if (frs) hipLaunchKernelGGL(frs_scale<float>, cell_grid(g), kBlock, 0, s, g, sp, cur[0], cur[1], cur[2]);
-    rc = adjoint_stages<float>(g, dt, 1, b[0], b[1], b[2], c[0], cur[0], cur[1], cur[2], l1[0], l1[1], l1[2], nxt[0],
-                               nxt[1], nxt[2], 0, 1, Hi, f_cells, workspace, flag, s);
+    if (march)
+      rc = march_stage(g, dt, 1, b, c[0], cur, l1, nxt, 0, 1, Hi, f_cells, workspace, s);
+    else
+      rc = adjoint_stages<float>(g, dt, 1, b[0], b[1], b[2], c[0], cur[0], cur[1], cur[2], l1[0], l1[1], l1[2],
+                                 nxt[0], nxt[1], nxt[2], 0, 1, Hi, f_cells, workspace, flag, s);
     if (!rc && frs) hipLaunchKernelGGL(frs_scale<float>, cell_grid(g), kBlock, 0, s, g, sp, l1[0], l1[1], l1[2]);
-    if (!rc)
+    if (!rc && march) rc = march_stage(g, dt, 0, a, b[0], l1, nxt, nullptr, 1, 0, Hi, f_cells, workspace, s);
+    if (!rc && !march)
       rc = adjoint_stages<float>(g, dt, 0, a[0], a[1], a[2], b[0], l1[0], l1[1], l1[2], nxt[0], nxt[1], nxt[2],
                                  nullptr, nullptr, nullptr, 1, 0, Hi, f_cells, workspace, flag, s);
     if (rc) return rc;

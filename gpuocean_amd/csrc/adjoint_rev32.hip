@@ -47,6 +47,18 @@ void launch_batch_march(const AdjItem<float>* tab, const int4* work, int n_work,
   if (max_fold > 0) hipLaunchKernelGGL(fold_fix_b, dim3((max_fold + 255) / 256, n), dim3(256), 0, s, tab);
 }
 
+void launch_march(const AdjItem<float>& it, int rows, hipStream_t s) {
+  const int n_strips = (it.g.nx + 4 + kMarchCols - 1) / kMarchCols;
+  const int n_items = n_strips * ((it.g.ny + 4 + rows - 1) / rows);
+  const dim3 grid((n_items + 3) / 4);
+  if (it.fc)
+    hipLaunchKernelGGL(march_rev_1<true>, grid, dim3(256), 0, s, it, n_strips, n_items, rows);
+  else
+    hipLaunchKernelGGL(march_rev_1<false>, grid, dim3(256), 0, s, it, n_strips, n_items, rows);
+  const int nf = fold_cells(it.g.nx, it.g.ny);
+  if (nf > 0) hipLaunchKernelGGL(fold_fix_1, dim3((nf + 255) / 256), dim3(256), 0, s, it);
+}
+
 void march_work_items(const int* nx, const int* ny, int n, int rows, std::vector<int4>& out) {
   march_work(nx, ny, n, rows, out);
 }

@@ -60,6 +60,11 @@ void launch_batch_stage(const AdjItem<float>* tab, int max_w, int max_h, int n, 
 // next to a side of one member); bit-identical to launch_batch_stage.
 void launch_batch_march(const AdjItem<float>* tab, const int4* work, int n_work, int n, int max_fold, bool any_fc,
                         hipStream_t s);
+// Output columns of one march wavefront (kMarchCols of adjoint_march.inc).
+constexpr int kMarchColsHost = 58;
+// One member's reverse substep as the march (the item's lambda_in must not
+// alias its lambda_out or lambda_qn), chunks of `rows` rows.
+void launch_march(const AdjItem<float>& it, int rows, hipStream_t s);
 // Work items of the march: members of sizes nx[m] x ny[m], `rows` rows each.
 void march_work_items(const int* nx, const int* ny, int n, int rows, std::vector<int4>& out);
 int march_fold_cells(int nx, int ny);

@@ -155,3 +155,24 @@ def test_fused_reverse_equals_staged(gpu_ctx, frs, island, noise):
                 assert np.array_equal(a, g0[i][q][2:-2, 2:-2]), f"rows {rows} member {i} field {q}"
             assert np.abs(g0[i][0]).max() > 0
     ens.destroy()
+
+
+def test_member_march_equals_staged(gpu_ctx):
+    """A member large enough for the one-member sweep to take the fused march
+    (at least 2048 strips x row chunks: 1240 x 2300 cells) gives the same
+    gradient as the staged kernels (the batched path with fused_reverse=False)
+    bit for bit, with land and periodic east/west sides."""
+    nx, ny, n_steps, k = 1240, 2300, 3, 3
+    print(f"one-member march against staged, {nx} x {ny}", flush=True)
+    sim, pos, obs, bg = _member(gpu_ctx, nx, ny, 140, Common.BoundaryConditions(1, 2, 1, 2), island=True)
+    one = DifferentiableCDKLM16(sim, pos, obs, n_steps, checkpoint_every=k)
+    J1, _ = one.forward(*bg)
+    g1 = one.backward()
+    ens = DifferentiableEnsemble([sim], [pos], [obs], n_steps, checkpoint_every=k, fused_reverse=False)
+    J = ens.forward([bg])
+    g = ens.backward()
+    assert J[0] == J1
+    for q in range(3):
+        assert np.array_equal(g[0][q][2:-2, 2:-2].cpu().numpy(), g1[q][2:-2, 2:-2].cpu().numpy()), f"field {q}"
+    assert g1[0].abs().max().item() > 0
+    ens.destroy()
