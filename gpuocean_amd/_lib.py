@@ -205,6 +205,7 @@ _SIGNATURES = [
     ("gpo_adj_batch_sweep", c_int32, [c_void_p, c_int32, c_void_p, c_void_p]),
     ("gpo_adj_batch_destroy", c_int32, [c_void_p]),
     ("gpo_adj_batch_set_reverse", c_int32, [c_void_p, c_int32, c_int32, c_void_p]),
+    ("gpo_adj_batch_slot", c_int32, [c_void_p, c_int32, c_int32, c_int32, c_void_p]),
     ("gpo_balance_adj_f32", c_int32,
      [ctypes.POINTER(CDKLM16Params), c_float, c_float, c_float, c_void_p, c_int32, c_void_p, c_int32,
       c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_int32, c_void_p]),

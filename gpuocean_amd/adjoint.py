@@ -774,15 +774,92 @@ class CDKLM16Misfit(torch.autograd.Function):
         return g[0], g[1], g[2], None
 
 
+def binomial_schedule(n_seg, n_slots):
+    """Binomial checkpointing (Griewank and Walther's revolve) over n_seg
+    segments with n_slots checkpoint slots, slot 0 holding the initial
+    state: the actions that reverse every segment, last first, with the
+    fewest segment advances.  Actions:
+
+      ("load", slot, a)     the forward run continues from the start of
+                            segment a, held in slot
+      ("advance", a, m)     the forward run steps through segments a..m-1
+      ("save", slot, m)     the start of segment m goes into slot
+      ("turn", a)           segment a is recomputed from the forward run's
+                            state with every intermediate kept, then swept
+
+    The split points come from the recursion cost(t, s) = min over m of
+    m + cost(t - m, s - 1) + cost(m, s) (t segments whose start is held, s
+    free slots; cost(t, 0) = t (t - 1) / 2).  Its value is the minimal
+    number of advances of binomial checkpointing with S = s + 1 slots,
+    r t - C(S + r, S + 1) with r the least repetition number such that
+    C(S + r, S) >= t."""
+    n_seg, n_slots = int(n_seg), int(n_slots)
+    if n_seg < 1 or n_slots < 1:
+        raise ValueError("binomial_schedule: need at least one segment and one slot")
+    memo = {}
+
+    def cost(t, s):
+        if t <= 1:
+            return 0
+        if s == 0:
+            return t * (t - 1) // 2
+        if (t, s) not in memo:
+            memo[(t, s)] = min(m + cost(t - m, s - 1) + cost(m, s) for m in range(1, t))
+        return memo[(t, s)]
+
+    ops = []
+
+    def rev(a, b, slot, free):
+        t = b - a
+        if t == 1:
+            ops.extend([("load", slot, a), ("turn", a)])
+        elif not free:
+            for i in range(b - 1, a - 1, -1):
+                ops.append(("load", slot, a))
+                if i > a:
+                    ops.append(("advance", a, i))
+                ops.append(("turn", i))
+        else:
+            s = len(free)
+            m = a + min(range(1, t), key=lambda q: (q + cost(t - q, s - 1) + cost(q, s), q))
+            ops.extend([("load", slot, a), ("advance", a, m), ("save", free[0], m)])
+            rev(m, b, free[0], free[1:])
+            rev(a, m, slot, free)
+
+    rev(0, n_seg, 0, list(range(1, min(n_slots, n_seg))))
+    return ops
+
+
+def schedule_split(ops):
+    """Where the forward run's share of a binomial_schedule ends: before the
+    group of the first turn (its load and, without a free slot, its
+    advance).  The forward run executes ops[:j] and then steps on to the end
+    for the misfit; the sweep executes ops[j:], whose first load brings the
+    forward run back."""
+    j = next(i for i, op in enumerate(ops) if op[0] == "turn")
+    if j > 0 and ops[j - 1][0] == "advance":
+        j -= 1
+    while j > 0 and ops[j - 1][0] == "load":
+        j -= 1
+    return j
+
+
 class DifferentiableEnsemble:
     """The misfit and gradient of DifferentiableCDKLM16 for many members at
     once: every member's forward steps, trajectory recomputation and reverse
     sweep run as single launches over all members (gpo_adj_batch_*), each
     member with its own grid size, dt, observations and checkpoints.  The
-    members share the number of steps and the checkpoint spacing."""
+    members share the number of steps and the checkpoint spacing.
+
+    checkpoint_slots: states kept per member between the forward run and the
+    sweep.  None (default) keeps the start of every segment, so each segment
+    is recomputed once; fewer slots follow binomial_schedule (revolve), which
+    advances through segments more often to fit memory.  Every choice gives
+    the same gradient bit for bit."""
 
     def __init__(self, sims, positions, observed, n_steps, checkpoint_every=20, dts=None, model_error=False,
-                 drifter_positions=None, drifter_observed=None, fused_reverse=True, march_rows=0):
+                 drifter_positions=None, drifter_observed=None, fused_reverse=True, march_rows=0,
+                 checkpoint_slots=None):
         dpos = drifter_positions if drifter_positions is not None else [None] * len(sims)
         dobs = drifter_observed if drifter_observed is not None else [None] * len(sims)
         self.models = [DifferentiableCDKLM16(s, p, o, n_steps, dt=None if dts is None else dts[i],
@@ -794,6 +871,12 @@ class DifferentiableEnsemble:
         self.n_steps = int(n_steps)
         self.k = max(1, int(checkpoint_every))
         self.n_seg = max(1, (self.n_steps + self.k - 1) // self.k)
+        if checkpoint_slots is None or int(checkpoint_slots) >= self.n_seg:
+            self.n_slots, self.schedule = self.n_seg, None
+        else:
+            self.n_slots = max(1, int(checkpoint_slots))
+            self.schedule = binomial_schedule(self.n_seg, self.n_slots)
+            self._first_back = schedule_split(self.schedule)
         self.dev = sims[0].device
         self.stream = torch.cuda.Stream(device=self.dev)
         lib = L.lib()
@@ -806,13 +889,13 @@ class DifferentiableEnsemble:
             rows = m.ny + 4
             # one allocation per member: state, stage, checkpoints, k+1 states, k stages, 3 lambda states,
             # and with model error k pre-noise states
-            n_states = 2 + self.n_seg + (self.k + 1) + self.k + 3 + (self.k if m.gen is not None else 0)
+            n_states = 2 + self.n_slots + (self.k + 1) + self.k + 3 + (self.k if m.gen is not None else 0)
             with torch.cuda.stream(self.stream):
                 buf = torch.zeros((n_states, 3, rows, P), dtype=torch.float32, device=self.dev)
                 ws = torch.empty(lib.gpo_adjoint_workspace_bytes(m.nx, m.ny, 4), dtype=torch.uint8, device=self.dev)
                 # drifter positions: forward, checkpoints, segment, sensitivity
                 nd = m.n_drifters
-                pos = torch.zeros((1 + self.n_seg + self.k + 1 + 1, max(nd, 1), 2), dtype=torch.float32,
+                pos = torch.zeros((1 + self.n_slots + self.k + 1 + 1, max(nd, 1), 2), dtype=torch.float32,
                                   device=self.dev)
             self._bufs.append((buf, ws))
             self._pos.append(pos)
@@ -829,9 +912,9 @@ class DifferentiableEnsemble:
             e.state = base
             e.stage = base + st
             e.ckpt = base + 2 * st
-            e.qs = base + (2 + self.n_seg) * st
-            e.q1s = base + (2 + self.n_seg + self.k + 1) * st
-            e.lam = base + (2 + self.n_seg + 2 * self.k + 1) * st
+            e.qs = base + (2 + self.n_slots) * st
+            e.q1s = base + (2 + self.n_slots + self.k + 1) * st
+            e.lam = base + (2 + self.n_slots + 2 * self.k + 1) * st
             e.workspace = ws.data_ptr()
             e.has_frs = int(m.frs is not None)
             if m.frs is not None:
@@ -850,19 +933,19 @@ class DifferentiableEnsemble:
                     cs = torch.empty((self.k, m.ny + 4, m.nx + 4), dtype=torch.float32, device=self.dev)
                 self._coarse_seg.append(cs)
                 e.coarse_seg = cs.data_ptr()
-                e.pre = base + (2 + self.n_seg + 2 * self.k + 1 + 3) * st
+                e.pre = base + (2 + self.n_slots + 2 * self.k + 1 + 3) * st
             if m.n_drifters:
                 pb = pos.data_ptr()
                 sp = 2 * 4 * m.n_drifters
                 e.n_drifters = m.n_drifters
                 e.Bm, e.Bm_pitch = m.Bm.data_ptr(), m.Bm.pitch
                 e.pos, e.pos_ckpt = pb, pb + sp
-                e.pos_seg = pb + (1 + self.n_seg) * sp
-                e.lam_pos = pb + (1 + self.n_seg + self.k + 1) * sp
+                e.pos_seg = pb + (1 + self.n_slots) * sp
+                e.lam_pos = pb + (1 + self.n_slots + self.k + 1) * sp
         self._pitch = [_pitch_elems(m.nx + 4) for m in self.models]
         h = ctypes.c_void_p()
         self.stream.synchronize()
-        L.check(lib.gpo_adj_batch_create(table, self.n, self.k, self.n_seg, ctypes.byref(h), self._sh()),
+        L.check(lib.gpo_adj_batch_create(table, self.n, self.k, self.n_slots, ctypes.byref(h), self._sh()),
                 "gpo_adj_batch_create")
         self.handle = h
         self.set_reverse(fused_reverse, march_rows)
@@ -909,10 +992,16 @@ class DifferentiableEnsemble:
                 m = self.models[i]
                 if m.n_drifters:
                     self._pos[i][0, :m.n_drifters].copy_(m.x0)
-        for sg in range(self.n_seg):
-            m = min(self.k, self.n_steps - sg * self.k)
-            L.check(lib.gpo_adj_batch_checkpoint(self.handle, sg, 0, self._sh()), "gpo_adj_batch_checkpoint")
-            L.check(lib.gpo_adj_batch_forward(self.handle, m, self._sh()), "gpo_adj_batch_forward")
+        if self.schedule is None:
+            for sg in range(self.n_seg):
+                m = min(self.k, self.n_steps - sg * self.k)
+                L.check(lib.gpo_adj_batch_checkpoint(self.handle, sg, 0, self._sh()), "gpo_adj_batch_checkpoint")
+                L.check(lib.gpo_adj_batch_forward(self.handle, m, self._sh()), "gpo_adj_batch_forward")
+        else:
+            self._slot(0, 0, 0)
+            self._at = 0
+            self._run_schedule(self.schedule[:self._first_back])
+            self._advance(self._at, self.n_seg)
         with torch.cuda.stream(self.stream):
             self.J.zero_()
         for i in range(self.n):
@@ -925,6 +1014,37 @@ class DifferentiableEnsemble:
                 J[i] += m._drifter_j(self._pos[i][0, :m.n_drifters])
         return J
 
+    # ---- binomial checkpointing: the forward run's position is self._at (a segment index)
+    def _slot(self, slot, mode, segment):
+        L.check(L.lib().gpo_adj_batch_slot(self.handle, slot, mode, segment, self._sh()), "gpo_adj_batch_slot")
+
+    def _advance(self, a, m):
+        steps = min(m * self.k, self.n_steps) - a * self.k
+        if steps > 0:
+            L.check(L.lib().gpo_adj_batch_forward(self.handle, steps, self._sh()), "gpo_adj_batch_forward")
+        self._at = m
+
+    def _run_schedule(self, ops):
+        lib = L.lib()
+        for op in ops:
+            if op[0] == "load":
+                if self._at != op[2]:
+                    self._slot(op[1], 2, op[2])
+                    self._at = op[2]
+            elif op[0] == "advance":
+                assert self._at == op[1]
+                self._advance(op[1], op[2])
+            elif op[0] == "save":
+                assert self._at == op[2]
+                self._slot(op[1], 0, op[2])
+            else:   # turn: recompute segment op[1] from the forward run's state, then sweep it
+                sg = op[1]
+                assert self._at == sg
+                m = min(self.k, self.n_steps - sg * self.k)
+                self._slot(0, 3, sg)
+                L.check(lib.gpo_adj_batch_trajectory(self.handle, m, self._sh()), "gpo_adj_batch_trajectory")
+                L.check(lib.gpo_adj_batch_sweep(self.handle, m, _vp(self.flag), self._sh()), "gpo_adj_batch_sweep")
+
     def final_positions(self, i):
         """Drifter positions of member i after the last forward()."""
         m = self.models[i]
@@ -935,7 +1055,7 @@ class DifferentiableEnsemble:
         """dJ_i/d(eta0, hu0, hv0) of every member i of the last forward(), as
         a list of three (ny+4) x (nx+4) float32 tensors per member."""
         lib = L.lib()
-        lam_idx = 2 + self.n_seg + 2 * self.k + 1
+        lam_idx = 2 + self.n_slots + 2 * self.k + 1
         with torch.cuda.stream(self.stream):
             for buf, _ in self._bufs:
                 buf[lam_idx:lam_idx + 3].zero_()
@@ -943,16 +1063,19 @@ class DifferentiableEnsemble:
             for i, m in enumerate(self.models):
                 if m.n_drifters:
                     x = self._pos[i][0, :m.n_drifters]
-                    self._pos[i][1 + self.n_seg + self.k + 1, :m.n_drifters].copy_(
+                    self._pos[i][1 + self.n_slots + self.k + 1, :m.n_drifters].copy_(
                         (2.0 * (x.double() - m.xobs.double())).float())
         for i in range(self.n):
             if self.models[i].n_obs:
                 self._misfit(i, 0, lam_idx)
-        for sg in reversed(range(self.n_seg)):
-            m = min(self.k, self.n_steps - sg * self.k)
-            L.check(lib.gpo_adj_batch_checkpoint(self.handle, sg, 1, self._sh()), "gpo_adj_batch_checkpoint")
-            L.check(lib.gpo_adj_batch_trajectory(self.handle, m, self._sh()), "gpo_adj_batch_trajectory")
-            L.check(lib.gpo_adj_batch_sweep(self.handle, m, _vp(self.flag), self._sh()), "gpo_adj_batch_sweep")
+        if self.schedule is None:
+            for sg in reversed(range(self.n_seg)):
+                m = min(self.k, self.n_steps - sg * self.k)
+                L.check(lib.gpo_adj_batch_checkpoint(self.handle, sg, 1, self._sh()), "gpo_adj_batch_checkpoint")
+                L.check(lib.gpo_adj_batch_trajectory(self.handle, m, self._sh()), "gpo_adj_batch_trajectory")
+                L.check(lib.gpo_adj_batch_sweep(self.handle, m, _vp(self.flag), self._sh()), "gpo_adj_batch_sweep")
+        else:
+            self._run_schedule(self.schedule[self._first_back:])
         self.stream.synchronize()
         return [[a.clone() for a in self._state(i, lam_idx)] for i in range(self.n)]
 

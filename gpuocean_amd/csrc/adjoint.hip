@@ -606,14 +606,17 @@ __global__ void __launch_bounds__(256) nd_balance_rev_b(const NoiseDriftItem* __
 }
 
 // Drifter positions: mode 0 forward -> checkpoint seg, mode 1 checkpoint seg ->
-// segment start.
+// segment start, mode 2 checkpoint seg -> forward, mode 3 forward -> segment
+// start.
 __global__ void __launch_bounds__(64) nd_pos_copy_b(const NoiseDriftItem* __restrict__ tab, int mode, int seg) {
   const NoiseDriftItem& m = tab[blockIdx.y];
   const int i = blockIdx.x * 64 + threadIdx.x;
   if (i >= 2 * m.n_drifters) return;
   float* ck = m.pos_ckpt + (size_t)seg * 2 * m.n_drifters;
   if (mode == 0) ck[i] = m.pos[i];
-  else m.pos_seg[i] = ck[i];
+  else if (mode == 1) m.pos_seg[i] = ck[i];
+  else if (mode == 2) m.pos[i] = ck[i];
+  else m.pos_seg[i] = m.pos[i];
 }
 
 // Observation of the cell-centre velocity (u, v) = (hu, hv)/(eta + Hm),
@@ -811,7 +814,9 @@ struct AdjBatch {
   StepArgs* d_traj = nullptr;        // k x 2n: qs[j] -> q1s[j] -> qs[j+1]
   int2* d_work = nullptr;            // (member, strip | chunk << 16)
   AdjItem<float>* d_items = nullptr;  // (2j + rk) x n
-  CopyItem* d_copy = nullptr;        // save (n_seg x n), restore (n_seg x n), lambda parity (n)
+  // save (n_seg x n), restore (n_seg x n), lambda parity (n), load (n_seg x n),
+  // forward state -> segment start (n); n_seg is the number of checkpoint slots
+  CopyItem* d_copy = nullptr;
   // flow-relaxation members (n_frs of them): ghost-cell passes on the forward
   // stage, the forward state, then per step j the stage and the new state
   cells::BcTarget* d_bc = nullptr;   // (2 + 2k) x n_frs
@@ -1425,7 +1430,7 @@ int gpo_adj_batch_create(const gpo_adj_member* members, int32_t n, int32_t k, in
   b->n = n, b->k = k, b->n_seg = n_segments;
   std::vector<StepArgs> fwd(2 * (size_t)n), traj(2 * (size_t)n * k);
   std::vector<AdjItem<float>> items(2 * (size_t)n * k);
-  std::vector<CopyItem> copies((2 * (size_t)n_segments + 1) * n);
+  std::vector<CopyItem> copies((3 * (size_t)n_segments + 2) * n);
   std::vector<int2> work;
   std::vector<NoiseDriftItem> nds;
   std::vector<std::vector<cells::BcTarget>> bcs(2 + 2 * (size_t)k);  // per slot, one per flow-relaxation member
@@ -1548,6 +1553,10 @@ int gpo_adj_batch_create(const gpo_adj_member* members, int32_t n, int32_t k, in
       copies[((size_t)n_segments + sg) * n + m] = CopyItem{gptr(M.ckpt + sg * st), gptr(M.qs), p.nx, p.ny, g.sp, pl};
     }
     copies[(size_t)2 * n_segments * n + m] = CopyItem{gptr(M.lam), gptr(M.lam + st), p.nx, p.ny, g.sp, pl};
+    for (int sg = 0; sg < n_segments; ++sg)
+      copies[((size_t)2 * n_segments + 1 + sg) * n + m] =
+          CopyItem{gptr(M.ckpt + sg * st), gptr(M.state), p.nx, p.ny, g.sp, pl};
+    copies[((size_t)3 * n_segments + 1) * n + m] = CopyItem{gptr(M.state), gptr(M.qs), p.nx, p.ny, g.sp, pl};
     const StepArgs& a = fwd[2 * m];
     if (a.n_strips > 0xFFFF || a.n_chunks > 0x7FFF) {
       adj_batch_free(b);
@@ -1691,22 +1700,33 @@ int gpo_adj_batch_forward(gpo_adj_batch h, int32_t n_steps, void* stream) {
   return 0;
 }
 
-int gpo_adj_batch_checkpoint(gpo_adj_batch h, int32_t segment, int32_t restore, void* stream) {
+int gpo_adj_batch_slot(gpo_adj_batch h, int32_t slot, int32_t mode, int32_t segment, void* stream) {
   AdjBatch* b = (AdjBatch*)h;
-  if (!b || segment < 0 || segment >= b->n_seg) return bad_arg("gpo_adj_batch_checkpoint: bad segment");
-  const size_t base = (restore ? (size_t)b->n_seg + segment : (size_t)segment) * b->n;
+  if (!b || slot < 0 || slot >= b->n_seg || mode < 0 || mode > 3 || segment < 0)
+    return bad_arg("gpo_adj_batch_slot: bad slot, mode or segment");
+  size_t base = 0;
+  if (mode == 0) base = (size_t)slot * b->n;                            // forward state -> slot
+  if (mode == 1) base = ((size_t)b->n_seg + slot) * b->n;               // slot -> segment start
+  if (mode == 2) base = ((size_t)2 * b->n_seg + 1 + slot) * b->n;       // slot -> forward state
+  if (mode == 3) base = ((size_t)3 * b->n_seg + 1) * b->n;              // forward state -> segment start
   hipStream_t s = (hipStream_t)stream;
-  if (restore)
+  if (mode == 1 || mode == 3)
     b->seg_start = segment * b->k;
   else
     b->step = segment * b->k;
   if (b->any_drift) {
-    hipLaunchKernelGGL(nd_pos_copy_b, dim3((2 * b->max_drifters + 63) / 64, b->n), dim3(64), 0, s, b->d_nd,
-                       restore ? 1 : 0, segment);
+    hipLaunchKernelGGL(nd_pos_copy_b, dim3((2 * b->max_drifters + 63) / 64, b->n), dim3(64), 0, s, b->d_nd, mode,
+                       slot);
     const int rc = hip_check(hipGetLastError(), "batched drifter checkpoint");
     if (rc) return rc;
   }
   return adj_batch_copy(b, b->d_copy + base, s);
+}
+
+int gpo_adj_batch_checkpoint(gpo_adj_batch h, int32_t segment, int32_t restore, void* stream) {
+  AdjBatch* b = (AdjBatch*)h;
+  if (!b || segment < 0 || segment >= b->n_seg) return bad_arg("gpo_adj_batch_checkpoint: bad segment");
+  return gpo_adj_batch_slot(h, segment, restore ? 1 : 0, segment, stream);
 }
 
 int gpo_adj_batch_trajectory(gpo_adj_batch h, int32_t m_steps, void* stream) {

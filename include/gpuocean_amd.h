@@ -598,13 +598,22 @@ typedef struct {
 typedef void* gpo_adj_batch;
 
 /* Builds the argument tables of n members with segments of at most k steps
- * and n_segments checkpoints.  Allocates device tables; synchronises. */
+ * and n_segments checkpoint slots (ckpt holds n_segments states per member).
+ * Allocates device tables; synchronises. */
 int gpo_adj_batch_create(const gpo_adj_member* members, int32_t n, int32_t k, int32_t n_segments,
                          gpo_adj_batch* out, void* stream);
 /* n_steps RK2 steps of every member's state (each with its own dt). */
 int gpo_adj_batch_forward(gpo_adj_batch b, int32_t n_steps, void* stream);
-/* restore = 0: state -> ckpt[segment]; restore = 1: ckpt[segment] -> qs[0]. */
+/* restore = 0: state -> ckpt[segment]; restore = 1: ckpt[segment] -> qs[0]
+ * (gpo_adj_batch_slot with slot = segment). */
 int gpo_adj_batch_checkpoint(gpo_adj_batch b, int32_t segment, int32_t restore, void* stream);
+/* Checkpoint slots for any schedule (binomial checkpointing): the state and
+ * drifter positions of every member, with `segment` the segment whose start
+ * they hold.  mode 0: state -> ckpt[slot] (the forward run is at the start of
+ * segment); 1: ckpt[slot] -> qs[0], the segment start of the next trajectory;
+ * 2: ckpt[slot] -> state, the forward run continues from segment; 3: state ->
+ * qs[0].  No counterpart in the reference (it has no adjoint). */
+int gpo_adj_batch_slot(gpo_adj_batch b, int32_t slot, int32_t mode, int32_t segment, void* stream);
 /* qs[0] -> qs[1..m], with the stages in q1s[0..m-1] (m <= k). */
 int gpo_adj_batch_trajectory(gpo_adj_batch b, int32_t m, void* stream);
 /* lam: lambda at qs[m] in, (dqs[m]/dqs[0])^T lambda out (interior cells),

@@ -176,3 +176,42 @@ def test_member_march_equals_staged(gpu_ctx):
         assert np.array_equal(g[0][q][2:-2, 2:-2].cpu().numpy(), g1[q][2:-2, 2:-2].cpu().numpy()), f"field {q}"
     assert g1[0].abs().max().item() > 0
     ens.destroy()
+
+
+@pytest.mark.parametrize("slots", [1, 2, 3])
+def test_binomial_checkpointing_equals_every_segment(gpu_ctx, slots):
+    """Binomial checkpointing with 1, 2 or 3 state slots (segments recomputed
+    more often) gives the misfits, final drifter positions and gradients of
+    the default schedule (every segment start kept) bit for bit, with flow
+    relaxation, land, model error (its draw index follows the restored
+    segment) and drifters."""
+    n_steps, k = 23, 4
+    print(f"binomial checkpointing, {slots} slots", flush=True)
+    members = [_member(gpu_ctx, nx, ny, 160 + i, BCS_FRS[i % 3], True) for i, (nx, ny) in enumerate(SIZES[:3])]
+    dpos, dobs = [], []
+    for i, (sim, pos, obs, bg) in enumerate(members):
+        sim.setSOARModelError(1e-4, xorwow_seed=900 + i)
+        rng = np.random.default_rng(170 + i)
+        x = np.stack([rng.uniform(0.3 * sim.nx * sim.dx, 0.7 * sim.nx * sim.dx, 30),
+                      rng.uniform(0.3 * sim.ny * sim.dy, 0.7 * sim.ny * sim.dy, 30)], 1).astype(np.float32)
+        dpos.append(x)
+        dobs.append((x + rng.normal(0, 30.0, x.shape)).astype(np.float32))
+    args = ([m[0] for m in members], [m[1] for m in members], [m[2] for m in members], n_steps)
+    kw = dict(checkpoint_every=k, model_error=True, drifter_positions=dpos, drifter_observed=dobs)
+    ref = DifferentiableEnsemble(*args, **kw)
+    J0 = ref.forward([m[3] for m in members])
+    x0 = [ref.final_positions(i) for i in range(len(members))]
+    g0 = [[a[2:-2, 2:-2].cpu().numpy() for a in g] for g in ref.backward()]
+    ref.destroy()
+    ens = DifferentiableEnsemble(*args, checkpoint_slots=slots, **kw)
+    assert ens.schedule is not None and ens.n_slots == slots
+    J = ens.forward([m[3] for m in members])
+    xs = [ens.final_positions(i) for i in range(len(members))]
+    g = ens.backward()
+    assert ens.flag_value() == 0
+    assert np.array_equal(J, J0)
+    for i in range(len(members)):
+        assert torch.equal(xs[i], x0[i]), f"member {i}: drifter positions"
+        for q in range(3):
+            assert np.array_equal(g[i][q][2:-2, 2:-2].cpu().numpy(), g0[i][q]), f"member {i} field {q}"
+    ens.destroy()
