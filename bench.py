@@ -638,8 +638,28 @@ def run_adjoint_batch(n_members=128, n_steps=300, k=20, reps=2, n_seq=4):
         log(f"[c4-adjoint] forward {1e3 * (t1 - t0):.1f} ms, gradient {1e3 * (t2 - t1):.1f} ms")
     f_ms, b_ms = 1e3 * float(np.median(tf[1:])), 1e3 * float(np.median(tb[1:]))
     assert ens.flag_value() == 0 and all(torch.isfinite(a).all().item() for a in g[0]) and np.all(np.isfinite(J))
+    state_gb = sum(b.numel() * b.element_size() for b, _ in ens._bufs) / 1e9
+    slots_all = ens.n_slots
     ens.destroy()
     del ens, g
+    # binomial checkpointing with 4 state slots per member instead of one per segment
+    ens = DifferentiableEnsemble(sims, none, none, n_steps, checkpoint_every=k, model_error=True,
+                                 drifter_positions=dpos, drifter_observed=dobs, checkpoint_slots=4)
+    tb4 = []
+    for _ in range(2):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ens.forward(x0)
+        t1 = time.perf_counter()
+        ens.backward()
+        torch.cuda.synchronize()
+        tb4.append((t1 - t0, time.perf_counter() - t1))
+    f4_ms, b4_ms = 1e3 * tb4[-1][0], 1e3 * tb4[-1][1]
+    state4_gb = sum(b.numel() * b.element_size() for b, _ in ens._bufs) / 1e9
+    ens.destroy()
+    del ens
+    log(f"[c4-adjoint] binomial checkpointing, 4 slots: forward {f4_ms:.1f} ms, gradient {b4_ms:.1f} ms "
+        f"(gradient/forward {b4_ms / f4_ms:.2f}); state buffers {state4_gb:.1f} GB against {state_gb:.1f} GB")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     seq_cells = 0
@@ -658,7 +678,9 @@ def run_adjoint_batch(n_members=128, n_steps=300, k=20, reps=2, n_seq=4):
             "drifters_per_member": 1000, "misfit": "drifter positions after the last step",
             "model_error": "SOAR q0=5e-3, fixed normals", "forward_ms": f_ms, "gradient_ms": b_ms,
             "ratio": b_ms / f_ms, "sequential_ms": seq_ms, "sequential_sample_members": n_seq,
-            "speedup_batched": seq_ms / (f_ms + b_ms)}
+            "speedup_batched": seq_ms / (f_ms + b_ms), "state_slots": slots_all, "state_buffers_gb": state_gb,
+            "binomial_4_slots": {"forward_ms": f4_ms, "gradient_ms": b4_ms, "ratio": b4_ms / f4_ms,
+                                 "state_buffers_gb": state4_gb}}
 
 
 def run_adjoint(reps=3):
