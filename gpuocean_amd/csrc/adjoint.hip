@@ -16,6 +16,7 @@
 #include "boundary_cells.h"
 #include "cdklm_common.h"
 #include "noise_cells.h"
+#include "noise_tile.h"
 #include "philox.h"
 
 namespace gpo {
@@ -503,24 +504,35 @@ __global__ void __launch_bounds__(256) nd_soar_b(const NoiseDriftItem* __restric
   nd_coarse(m, slot)[k] = 1.0f * soar_cell(m.g.nx, m.g.ny, m.W, m.per_ns, m.per_ew, m.xi, 4 * m.xi_cols, tj, ti);
 }
 
-// The balanced perturbation added to a state (gpo_geostrophic_balance's
-// arithmetic): the forward state (which < 0) or segment state `which`; with
-// keep >= 0 the state is first copied to pre-noise buffer `keep`.
-__global__ void __launch_bounds__(256) nd_balance_b(const NoiseDriftItem* __restrict__ tab, int which, int keep,
-                                                    int slot) {
-  const NoiseDriftItem& m = tab[blockIdx.z];
-  const int ti = blockIdx.x * 64 + (threadIdx.x & 63) + 2, tj = blockIdx.y * 4 + (threadIdx.x >> 6) + 2;
-  if (!m.noise || ti > m.g.nx + 1 || tj > m.g.ny + 1) return;
-  float* base = which < 0 ? m.state : m.qs + (size_t)which * m.st;
-  float *e = base, *u = base + m.plane, *v = base + 2 * m.plane;
-  const size_t k = (size_t)tj * m.g.sp + ti;
-  if (keep >= 0) {
+// The model error of step `step` (draw draw0 + step) in one launch over the
+// tiles of every member with model error (noise_tile: normal field, SOAR and
+// geostrophic balance of a 64 x 16 tile, the values of gpo_random_normal,
+// gpo_soar with scale 1 and gpo_geostrophic_balance): on the forward state
+// (TRAJ false), or on segment state `which` keeping its pre-noise copy in
+// `keep` and the SOAR field in the segment's slot (TRAJ true).
+template <bool TRAJ>
+__global__ void __launch_bounds__(256) nd_noise_b(const NoiseDriftItem* __restrict__ tab,
+                                                  const int2* __restrict__ tiles, int step, int which, int keep,
+                                                  int slot) {
+  const int2 t = tiles[blockIdx.x];
+  const NoiseDriftItem& m = tab[t.x];
+  NoiseTile a;
+  a.nx = m.g.nx, a.ny = m.g.ny, a.per_ns = m.per_ns, a.per_ew = m.per_ew;
+  a.xi_cols = m.xi_cols, a.pitch = 4 * m.g.sp, a.hi_pitch = 4 * m.g.hp, a.f_pitch = 0;
+  a.k0 = m.k0, a.k1 = m.k1, a.stream_id = m.stream_id, a.draw = m.draw0 + (uint32_t)step;
+  a.W = m.W;
+  a.scale = 1.0f, a.dx = (float)m.g.dx, a.dy = (float)m.g.dy, a.g = (float)m.g.g, a.f = (float)m.g.f;
+  a.beta = m.beta, a.north_x = (float)m.g.north_x, a.north_y = (float)m.g.north_y, a.land = m.land;
+  float* base = TRAJ ? m.qs + (size_t)which * m.st : m.state;
+  a.eta = base, a.hu = base + m.plane, a.hv = base + 2 * m.plane;
+  a.Hi = m.Hi, a.f_cells = nullptr;
+  a.pre_e = a.pre_u = a.pre_v = nullptr, a.d_out = nullptr, a.d_row = m.g.nx + 4;
+  if (TRAJ) {
     float* p = m.pre + (size_t)keep * m.st;
-    p[k] = e[k], p[m.plane + k] = u[k], p[2 * m.plane + k] = v[k];
+    a.pre_e = p, a.pre_u = p + m.plane, a.pre_v = p + 2 * m.plane;
+    a.d_out = nd_coarse(m, slot);
   }
-  balance_cell((float)m.g.dx, (float)m.g.dy, (float)m.g.g, (float)m.g.f, m.beta, (float)m.g.north_x,
-               (float)m.g.north_y, nullptr, 0, nd_coarse(m, slot), 4 * (m.g.nx + 4), e, u, v, 4 * m.g.sp, m.Hi,
-               4 * m.g.hp, m.land, tj, ti);
+  noise_tile<TRAJ, TRAJ>(a, 2 + kTileH * (t.y >> 16), 2 + kTileW * (t.y & 0xFFFF));
 }
 
 // Drift of the forward positions over the forward state (which < 0), or of
@@ -809,6 +821,9 @@ struct AdjBatch {
   int use_march = 1, march_rows = 0, n_march = 0, max_fold = 0;
   int4* d_march = nullptr;
   std::vector<int> nxs, nys;
+  // model-error tiles (member, tile row << 16 | tile column) of members with model error
+  int2* d_ntiles = nullptr;
+  int n_ntiles = 0;
   int all_ok = 1;                             // every member's fast divisions and square root verified
   StepArgs* d_fwd = nullptr;         // 2n: forward state -> stage -> state
   StepArgs* d_traj = nullptr;        // k x 2n: qs[j] -> q1s[j] -> qs[j+1]
@@ -832,7 +847,7 @@ struct AdjBatch {
 
 void adj_batch_free(AdjBatch* b) {
   for (void* p : {(void*)b->d_fwd, (void*)b->d_traj, (void*)b->d_work, (void*)b->d_items, (void*)b->d_copy,
-                  (void*)b->d_bc, (void*)b->d_nd, (void*)b->d_march})
+                  (void*)b->d_bc, (void*)b->d_nd, (void*)b->d_march, (void*)b->d_ntiles})
     if (p) (void)hipFree(p);
   delete b;
 }
@@ -841,11 +856,12 @@ void adj_batch_free(AdjBatch* b) {
 // segment state `which` (its pre-noise copy kept in `keep`).
 int adj_batch_noise(AdjBatch* b, int step, int which, int keep, hipStream_t s) {
   if (!b->any_noise) return 0;
-  const int slot = keep;  // the segment's slot of this step's SOAR field (forward: the member's own buffer)
-  hipLaunchKernelGGL(nd_xi_b, dim3((b->max_xi + 255) / 256, b->n), dim3(256), 0, s, b->d_nd, step);
-  hipLaunchKernelGGL(nd_soar_b, dim3((b->max_cells + 255) / 256, b->n), dim3(256), 0, s, b->d_nd, slot);
-  hipLaunchKernelGGL(nd_balance_b, dim3((b->max_w - 4 + 63) / 64, (b->max_h - 4 + 3) / 4, b->n), dim3(256), 0, s,
-                     b->d_nd, which, keep, slot);
+  if (which < 0)
+    hipLaunchKernelGGL(nd_noise_b<false>, dim3(b->n_ntiles), dim3(256), 0, s, b->d_nd, b->d_ntiles, step, which, keep,
+                       keep);
+  else  // the segment's slot of this step's SOAR field is `keep`
+    hipLaunchKernelGGL(nd_noise_b<true>, dim3(b->n_ntiles), dim3(256), 0, s, b->d_nd, b->d_ntiles, step, which, keep,
+                       keep);
   return hip_check(hipGetLastError(), "batched model error launch");
 }
 
@@ -1433,6 +1449,7 @@ int gpo_adj_batch_create(const gpo_adj_member* members, int32_t n, int32_t k, in
   std::vector<CopyItem> copies((3 * (size_t)n_segments + 2) * n);
   std::vector<int2> work;
   std::vector<NoiseDriftItem> nds;
+  std::vector<int2> ntiles;
   std::vector<std::vector<cells::BcTarget>> bcs(2 + 2 * (size_t)k);  // per slot, one per flow-relaxation member
   long long max_bc_cells = 0;
   for (int m = 0; m < n; ++m) {
@@ -1603,6 +1620,8 @@ int gpo_adj_batch_create(const gpo_adj_member* members, int32_t n, int32_t k, in
         return bad_arg("gpo_adj_batch_create: members differ in whether their segment SOAR fields are kept");
       }
       b->any_noise = 1;
+      for (int ty = 0; ty < (p.ny + kTileH - 1) / kTileH; ++ty)
+        for (int tx = 0; tx < (p.nx + kTileW - 1) / kTileW; ++tx) ntiles.push_back(make_int2(m, (ty << 16) | tx));
       b->max_xi = std::max(b->max_xi, M.xi_rows * M.xi_cols);
       b->max_cells = std::max(b->max_cells, (p.nx + 4) * (p.ny + 4));
     }
@@ -1655,6 +1674,14 @@ int gpo_adj_batch_create(const gpo_adj_member* members, int32_t n, int32_t k, in
     if (!rc)
       rc = hip_check(hipMemcpyAsync(b->d_bc, flat.data(), sizeof(cells::BcTarget) * flat.size(),
                                     hipMemcpyHostToDevice, s),
+                     "gpo_adj_batch_create copy");
+  }
+  b->n_ntiles = (int)ntiles.size();
+  if (!rc && b->n_ntiles > 0) {
+    rc = hip_check(hipMalloc(&b->d_ntiles, sizeof(int2) * ntiles.size()), "gpo_adj_batch_create hipMalloc");
+    if (!rc)
+      rc = hip_check(hipMemcpyAsync(b->d_ntiles, ntiles.data(), sizeof(int2) * ntiles.size(), hipMemcpyHostToDevice,
+                                    s),
                      "gpo_adj_batch_create copy");
   }
   if (!rc && (b->any_noise || b->any_drift)) {

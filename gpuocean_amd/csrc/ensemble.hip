@@ -17,6 +17,7 @@
 #include "cdklm_common.h"
 #include "drift_cells.h"
 #include "noise_cells.h"
+#include "noise_tile.h"
 #include "philox.h"
 
 namespace gpo {
@@ -41,122 +42,21 @@ struct NoiseMember {
   BicubicGeo geo;
 };
 
-// Model error of one output tile (kTileH rows x kTileW columns of interior
-// cells of one member), fused: the normal field of the tile and its halo is
-// generated into LDS (one Philox call per four consecutive elements, as the
-// element index orders them), the SOAR field of the tile and a one-cell halo
-// is formed from it in LDS, and the geostrophic balance is added to the
-// state.  Every value equals the per-member path's (same element indices,
-// same Box-Muller, same fused multiply-add order, same balance arithmetic).
-constexpr int kTileW = 64, kTileH = 16;
-constexpr int kXiW = kTileW + 6, kXiH = kTileH + 6;  // xi: coarse halo 1 + SOAR radius 2 (+ column offset)
-constexpr int kCoW = kTileW + 2, kCoH = kTileH + 2;  // coarse: balance halo 1
-
-// x mod n for n >= 1, without a division when x lies in [-n, 2n).
-__device__ __forceinline__ int wrap(int x, int n) {
-  if (x < 0) x += n;
-  if (x >= n) x -= n;
-  if (x < 0 || x >= n) x = ((x % n) + n) % n;
-  return x;
-}
-
 __global__ void __launch_bounds__(256) noise_fused(const NoiseMember* __restrict__ members,
                                                    const int2* __restrict__ tiles,
                                                    const uint32_t* __restrict__ draw) {
-  __shared__ float xs[kXiH][kXiW];
-  __shared__ float cs[kCoH][kCoW];
   const int2 t = tiles[blockIdx.x];
   const NoiseMember& m = members[t.x];
-  const uint32_t d = draw[t.x];
-  const int T0 = 2 + kTileH * (t.y >> 16), S0 = 2 + kTileW * (t.y & 0xFFFF);
-  // coarse cells needed: rows T0-1 .. cj_hi, columns S0-1 .. ci_hi
-  const int cj_hi = min(T0 + kTileH, m.ny + 2), ci_hi = min(S0 + kTileW, m.nx + 2);
-  // xi virtual rows V = T0-1+lr and columns U = S0-1+lc are needed up to cj_hi+4, ci_hi+4
-  const int nrow = cj_hi + 4 - (T0 - 1) + 1, ncol = ci_hi + 4 - (S0 - 1) + 1;
-  const int cols = m.xi_cols;
-
-  // 0. the state and depths of this thread's four cells, loaded first so that
-  // the HBM reads overlap the random numbers and SOAR below
-  const int lx = threadIdx.x & 63, ly0 = threadIdx.x >> 6;
-  constexpr int kRowsPerThread = kTileH / 4;
-  BalanceIn in[kRowsPerThread];
-#pragma unroll
-  for (int k = 0; k < kRowsPerThread; ++k) {
-    const int tj = T0 + ly0 + 4 * k, ti = S0 + lx;
-    if (tj <= m.ny + 1 && ti <= m.nx + 1)
-      in[k] = balance_load(m.f == 0.0f ? m.f_cells : nullptr, m.f_pitch, m.eta, m.hu, m.hv, m.pitch, m.Hi,
-                           m.hi_pitch, tj, ti);
-  }
-
-  // 1. normal field into LDS
-  const int a0 = m.per_ew ? wrap(S0 - 1 - 4, m.nx) : (S0 - 1);  // actual column of U = S0-1
-  if (!m.per_ew || ncol <= m.nx) {
-    // the needed actual columns form one run, or two when a periodic run wraps
-    const int seg_len0 = m.per_ew ? min(ncol, m.nx - a0) : ncol;
-    constexpr int kMaxQ = kXiW / 4 + 2;
-    // work items (segment, row, quad): the second segment exists only where a
-    // periodic window wraps, so most tiles keep every lane busy
-    const int n_seg = seg_len0 < ncol ? 2 : 1;
-    for (int u = threadIdx.x; u < n_seg * nrow * kMaxQ; u += 256) {
-      const int seg = u / (nrow * kMaxQ), rem = u - seg * (nrow * kMaxQ), lr = rem / kMaxQ, j = rem - lr * kMaxQ;
-      const int V = T0 - 1 + lr;
-      const int r = m.per_ns ? wrap(V - 4, m.ny) : V;
-      const int ca = seg == 0 ? a0 : 0;
-      const int len = seg == 0 ? seg_len0 : ncol - seg_len0;
-      if (len <= 0) continue;
-      const int cb = ca + len - 1;
-      const uint32_t base = (uint32_t)r * (uint32_t)cols;
-      const uint32_t q = ((base + (uint32_t)ca) >> 2) + (uint32_t)j;
-      if (q > ((base + (uint32_t)cb) >> 2)) continue;
-      float v4[4];
-      normals4(q, d, m.stream_id, m.k0, m.k1, v4);
-      // window column of actual column c: c - a0 in the first run, c - a0 + nx
-      // in the wrapped second run (a0 = S0 - 1 without periodicity)
-      const int c0 = (int)(4u * q - base), shift = seg == 0 ? -a0 : m.nx - a0;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int c = c0 + i;
-        if (c < ca || c > cb) continue;
-        xs[lr][c + shift] = v4[i];
-      }
-    }
-  } else {
-    // periodic member narrower than the tile window: element by element
-    for (int u = threadIdx.x; u < kXiH * kXiW; u += 256) {
-      const int lr = u / kXiW, lc = u - lr * kXiW;
-      if (lr >= nrow || lc >= ncol) continue;
-      const int V = T0 - 1 + lr, U = S0 - 1 + lc;
-      const int r = m.per_ns ? (V - 4 + 2 * m.ny) % m.ny : V;
-      const int c = (U - 4 + 2 * m.nx) % m.nx;
-      xs[lr][lc] = normal_at((uint32_t)r * (uint32_t)cols + (uint32_t)c, d, m.stream_id, m.k0, m.k1);
-    }
-  }
-  __syncthreads();
-
-  // 2. SOAR field on the tile and its one-cell halo (soar_cell's order)
-  for (int u = threadIdx.x; u < kCoH * kCoW; u += 256) {
-    const int lr = u / kCoW, lc = u - lr * kCoW;
-    if (T0 - 1 + lr > cj_hi || S0 - 1 + lc > ci_hi) continue;
-    float acc = 0.0f;
-#pragma unroll
-    for (int k = 0; k < 5; ++k)
-#pragma unroll
-      for (int l = 0; l < 5; ++l) acc = __builtin_fmaf(m.W[k * 5 + l], xs[lr + k][lc + l], acc);
-    cs[lr][lc] = m.scale * acc;
-  }
-  __syncthreads();
-
-  // 3. geostrophic balance on the tile
-#pragma unroll
-  for (int k = 0; k < kRowsPerThread; ++k) {
-    const int ly = ly0 + 4 * k;
-    const int tj = T0 + ly, ti = S0 + lx;
-    if (tj > m.ny + 1 || ti > m.nx + 1) continue;
-    float e, u, v;
-    balance_compute(m.dx, m.dy, m.g, m.f, m.beta, m.north_x, m.north_y, cs[ly + 1][lx + 1], cs[ly + 1][lx],
-                    cs[ly + 1][lx + 2], cs[ly][lx + 1], cs[ly + 2][lx + 1], in[k], m.land, tj, ti, &e, &u, &v);
-    balance_store(m.eta, m.hu, m.hv, m.pitch, tj, ti, e, u, v);
-  }
+  NoiseTile a;
+  a.nx = m.nx, a.ny = m.ny, a.per_ns = m.per_ns, a.per_ew = m.per_ew;
+  a.xi_cols = m.xi_cols, a.pitch = m.pitch, a.hi_pitch = m.hi_pitch, a.f_pitch = m.f_pitch;
+  a.k0 = m.k0, a.k1 = m.k1, a.stream_id = m.stream_id, a.draw = draw[t.x];
+  a.W = m.W;
+  a.scale = m.scale, a.dx = m.dx, a.dy = m.dy, a.g = m.g, a.f = m.f, a.beta = m.beta;
+  a.north_x = m.north_x, a.north_y = m.north_y, a.land = m.land;
+  a.eta = m.eta, a.hu = m.hu, a.hv = m.hv, a.Hi = m.Hi, a.f_cells = m.f_cells;
+  a.pre_e = a.pre_u = a.pre_v = nullptr, a.d_out = nullptr, a.d_row = 0;
+  noise_tile<false, false>(a, 2 + kTileH * (t.y >> 16), 2 + kTileW * (t.y & 0xFFFF));
 }
 
 // The model-error tile table serves both the fused kernel and the bicubic one.
