@@ -246,6 +246,13 @@ class OracleCDKLM16:
                                                              None if frs is None else ctypes.byref(frs))
         return self.t
 
+    def apply_bc(self):
+        """The ghost-cell pass on the current state (the FRS data at the
+        current time)."""
+        frs = self._frs_struct()
+        getattr(lib(), f"gpo_oracle_apply_bc_{self.suf}")(ctypes.byref(self.q), *[_p(a) for a in self.h0],
+                                                         None if frs is None else ctypes.byref(frs))
+
     def run_steps(self, n, dt):
         """n steps of a fixed dt in one call (no per-step Python); forcing and
         FRS time levels are held at the current time."""

@@ -192,3 +192,71 @@ def test_bathymetry_init_matches_oracle(gpu_ctx, case):
     assert np.array_equal(np.ma.getdata(Hi), ora.Hi)
     assert np.array_equal(np.ma.getdata(Bm), ora.Bm)
     assert np.any(ora.Bm == np.float32(host.LAND_VALUE))
+
+
+def _oracle_da_step(ora, obs_time, mts, courant, perturb, total):
+    """dataAssimilationStep's cadence (SWEsimulators/CDKLM16.py:742-806) on the
+    oracle: a leftover step first if there is one, then full model time
+    steps, each followed by the model error (perturb(scale)), dt from the CFL
+    condition at the start and after every fifth model step.  Returns the
+    new total_time_steps."""
+    full = int(round(obs_time - ora.t) / mts)
+    leftover = obs_time - ora.t - full * mts
+    if leftover / mts < 0.1 and full > 1:
+        leftover += mts
+        full -= 1
+    if leftover / mts < 0.00001:
+        leftover = 0
+    ora.updateDt(courant)
+    for i in range(full + 1):
+        if i == 0 and leftover == 0:
+            continue
+        ora.step(leftover if i == 0 else mts)
+        perturb(np.sqrt(leftover / mts) if i == 0 else 1.0)
+        total += 1
+        if total % 5 == 0:
+            ora.updateDt(courant)
+    return total
+
+
+def test_step_to_observation_against_oracle(gpu_ctx):
+    """Data-assimilation stepping (OceanModelEnsemble.stepToObservation, the
+    batched path: members with different time steps in shared launches) against
+    the oracle: each member's state after every observation time equals the
+    oracle's run of dataAssimilationStep's cadence bit for bit, with the
+    member's own normal field of each model step (model time step 60 s, six
+    observations one model step apart, so dt is also updated after the fifth
+    model step), and its dt equals the oracle's."""
+    from gpuocean_amd import OceanModelEnsemble
+    sizes = [(192, 128), (128, 96), (160, 64)]
+    print(f"stepToObservation of {sizes} against the oracle's dataAssimilationStep cadence", flush=True)
+    data, oras = [], []
+    for i, (nx, ny) in enumerate(sizes):
+        kw = cases.config_c2_member(nx, ny, 300 + i)
+        data.append({k: kw[k] for k in ("eta0", "hu0", "hv0", "H", "nx", "ny", "dx", "dy", "boundary_conditions")})
+        oras.append((kw, O.OracleCDKLM16(**kw)))
+    kw0 = oras[0][0]
+    sim_args = {k: kw0[k] for k in ("dt", "g", "f", "r", "courant_number")}
+    sim_args["model_time_step"] = 60.0
+    me = {"small_scale_perturbation_amplitude": 5e-3, "xorwow_seed": 91}
+    assert all(kw["f"] == kw0["f"] and kw["g"] == kw0["g"] for kw, _ in oras)
+    ens = OceanModelEnsemble(gpu_ctx, sim_args, data, me, numParticles=len(sizes), batched=True)
+    totals = [0] * len(sizes)
+    for t_obs in (60.0, 120.0, 180.0, 240.0, 300.0, 360.0):
+        ens.stepToObservation(t_obs)
+        for i, (s, (kw, o)) in enumerate(zip(ens.particles, oras)):
+            perturb = _oracle_noise(s, o)
+            # dataAssimilationStep's own courant_number default, 0.8; the
+            # ghost cells follow the perturbed interior (the step kernel
+            # reads wall and periodic neighbours from the interior cells)
+            def model_error(scale):
+                perturb(o.h0)
+                o.apply_bc()
+            totals[i] = _oracle_da_step(o, t_obs, 60.0, 0.8, model_error, totals[i])
+            assert s.total_time_steps == totals[i]
+            assert np.float32(s.dt) == np.float32(o.dt), f"t={t_obs}, member {i}: dt"
+            for k, (a, b) in enumerate(zip(s.download(True), o.download(True))):
+                assert np.array_equal(a, b), f"t={t_obs}, member {i} {sizes[i]}, field {k}"
+    assert ens._ensemble is not None   # the batched path ran
+    assert len({round(float(s.dt), 3) for s in ens.particles}) > 1
+    ens.cleanUp()
