@@ -180,7 +180,8 @@ __global__ void frs_scale_sponge_b(const AdjItem<T>* __restrict__ tab, int pass)
   const int along = pass == 0 ? H : W;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= (lo + hi) * along) return;
-  const int c = k / along, t = k - c * along;
+  // lanes follow rows in both passes (row by row over the sponge columns)
+  const int c = pass == 0 ? k % (lo + hi) : k / along, t = pass == 0 ? k / (lo + hi) : k - c * along;
   const int across = c < lo ? c : (pass == 0 ? W : H) - hi + (c - lo);
   const int ti = pass == 0 ? across : t, tj = pass == 0 ? t : across;
   const Sponge sp{it.sn, it.se, it.ss, it.sw};

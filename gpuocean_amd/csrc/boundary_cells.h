@@ -191,10 +191,10 @@ __device__ inline void bc_target_cell(const BcTarget& b, int pass, int k) {
     else
       frs_ns_cell(b.bn, b.bs, b.nx, b.ny, b.sn, b.ss, b.e, b.u, b.v, b.pitch, b.ns_cur, b.ns_next, b.ns_nx, b.frs_t, ti,
                   tj);
-  } else {
+  } else {  // row by row, so that the lanes of a wavefront share rows
     const int hgt = b.ny + 4, cols = b.ew_cols_lo + b.ew_cols_hi;
     if (k >= cols * hgt) return;
-    const int c = k / hgt, tj = k - c * hgt;
+    const int tj = k / cols, c = k - tj * cols;
     const int ti = c < b.ew_cols_lo ? c : (b.nx + 4 - b.ew_cols_hi) + (c - b.ew_cols_lo);
     if (b.be == 2)
       periodic_ew_cell(b.nx, b.ny, b.e, b.u, b.v, b.pitch, tj, ti < 2 ? ti : ti - b.nx);
