@@ -304,9 +304,46 @@ template <class S> GPO_HD inline S minmod_slope(S l, S c, S r, S theta) {
   return minmod_raw((c - l) * theta, (r - l) * S(0.5), (r - c) * theta);
 }
 
-// x-slopes at (r, s): CDKLM16_kernel.cu:673-737 with adjustSlopes_x (:159-203)
+// The argument minmod_raw(b, c, f) passes on, for its reverse: 0, 1, 2 for
+// b, c, f (the first of equal magnitudes), 3 when the signs differ and the
+// result is the constant 0.  Where it is not 3 the three signs agree, so
+// minmod_raw = s * min|.| with s the common sign, and d out / d arg = s * s = 1.
+template <class S> GPO_HD inline int minmod_sel(S b, S c, S f) {
+  const S sb = sgn1(b), sc = sgn1(c), sf = sgn1(f);
+  const S s = S(0.25) * sb * (sb + sc) * (sc + sf);
+  if (val(s) == 0) return 3;
+  const S ab = tabs(b), ac = tabs(c), af = tabs(f);
+  int sel = (val(ac) < val(ab)) ? 1 : 0;
+  const S m1 = sel ? ac : ab;
+  if (val(af) < val(m1)) sel = 2;
+  return sel;
+}
+// The same on sign bits and magnitudes (tabs(x) = |x|, copysign(1, x) has
+// x's sign bit, and s = 0 exactly when two neighbouring signs differ).
+template <> GPO_HD inline int minmod_sel<float>(float b, float c, float f) {
+  const bool nb = std::signbit(b), nc = std::signbit(c), nf = std::signbit(f);
+  if (nb != nc || nc != nf) return 3;
+  const float ab = std::fabs(b), ac = std::fabs(c), af = std::fabs(f);
+  int sel = (ac < ab) ? 1 : 0;
+  const float m1 = sel ? ac : ab;
+  if (af < m1) sel = 2;
+  return sel;
+}
+template <class S> GPO_HD inline int minmod_slope_sel(S l, S c, S r, S theta) {
+  return minmod_sel((c - l) * theta, (r - l) * S(0.5), (r - c) * theta);
+}
+
+// Selections of a slope evaluation, for its reverse (xslope_rev_code,
+// yslope_rev_code): bits 0-1, 2-3 and 4-5 hold minmod_sel of the two
+// limited slopes and of the potential-energy slope before its adjustment,
+// bit 6 is set where the low-side depth test failed (hw, hs <= 0) and bit 7
+// where the high-side one did (he, hn <= 0).
+constexpr int kSelLowDry = 64, kSelHighDry = 128;
+
+// x-slopes at (r, s): CDKLM16_kernel.cu:673-737 with adjustSlopes_x (:159-203);
+// `code`, when given, receives the selections.
 template <class S>
-GPO_HD void xslope_v(const Geo& g, const Sten<S>& st, int s, S& q0, S& q1, S& K) {
+GPO_HD void xslope_v(const Geo& g, const Sten<S>& st, int s, S& q0, S& q1, S& K, int* code = nullptr) {
   const S le = st.e0, ce = st.e1, re = st.e2;
   const S lu = st.u0, cu = st.u1, ru = st.u2;
   S lv = st.v0, cv = st.v1, rv = st.v2;
@@ -338,6 +375,9 @@ GPO_HD void xslope_v(const Geo& g, const Sten<S>& st, int s, S& q0, S& q1, S& K)
   const S He = S(0.5) * (st.Hc + st.Hd);
   const S hw = ce + Hw - qdiv(k + dxfv, g2);
   const S he = ce + He + qdiv(k + dxfv, g2);
+  if (code)
+    *code = minmod_slope_sel(lu, cu, ru, th) | (minmod_slope_sel(st.v0, st.v1, st.v2, th) << 2) |
+            (minmod_sel(b, c, f) << 4) | ((val(hw) > 0) ? 0 : kSelLowDry) | ((val(he) > 0) ? 0 : kSelHighDry);
   k = (val(hw) > 0) ? k : -dxfv + g2 * (ce + Hw);
   k = (val(he) > 0) ? k : -dxfv - g2 * (ce + He);
   K = k;
@@ -348,9 +388,10 @@ GPO_HD void xslope(const Geo& g, const Fwd<S>& w, const S* Hi, const S* fc, int 
   xslope_v(g, xsten(g, w, Hi, fc, r, s), s, q0, q1, K);
 }
 
-// y-slopes at (s, c): CDKLM16_kernel.cu:779-842 with adjustSlopes_y (:210-254)
+// y-slopes at (s, c): CDKLM16_kernel.cu:779-842 with adjustSlopes_y (:210-254);
+// `code`, when given, receives the selections.
 template <class S>
-GPO_HD void yslope_v(const Geo& g, const Sten<S>& st, int s, S& q0, S& q1, S& L) {
+GPO_HD void yslope_v(const Geo& g, const Sten<S>& st, int s, S& q0, S& q1, S& L, int* code = nullptr) {
   const S le = st.e0, ce = st.e1, ue = st.e2;
   S lu = st.u0, cu = st.u1, uu = st.u2;
   const S lv = st.v0, cv = st.v1, uv = st.v2;
@@ -382,6 +423,9 @@ GPO_HD void yslope_v(const Geo& g, const Sten<S>& st, int s, S& q0, S& q1, S& L)
   const S Hn = S(0.5) * (st.Hc + st.Hd);
   const S hs = ce + Hs - qdiv(l - dyfu, g2);
   const S hn = ce + Hn + qdiv(l - dyfu, g2);
+  if (code)
+    *code = minmod_slope_sel(st.u0, st.u1, st.u2, th) | (minmod_slope_sel(lv, cv, uv, th) << 2) |
+            (minmod_sel(b, cc, f) << 4) | ((val(hs) > 0) ? 0 : kSelLowDry) | ((val(hn) > 0) ? 0 : kSelHighDry);
   l = (val(hs) > 0) ? l : dyfu + g2 * (ce + Hs);
   l = (val(hn) > 0) ? l : dyfu - g2 * (ce + Hn);
   L = l;
@@ -861,38 +905,30 @@ GPO_HD void a1_cell(Geo g, T dt, int rk_step, const T* Hi, const T* fc, Fwd<T> w
              w.qy0[o + g.W], w.qy1[o + g.W], w.ly[o + g.W]);
 }
 
-// minmod_raw reverse: d out / d (b, c, f) for the selected argument
-template <class T> GPO_HD inline void minmod_rev(T b, T c, T f, T lout, T& lb, T& lc, T& lf) {
-  const T sb = sgn1(b), sc = sgn1(c), sf = sgn1(f);
-  const T s = T(0.25) * sb * (sb + sc) * (sc + sf);
-  if (s == T(0)) return;
-  const T ab = tabs(b), ac = tabs(c), af = tabs(f);
-  int sel = (ac < ab) ? 1 : 0;
-  const T m1 = sel ? ac : ab;
-  if (af < m1) sel = 2;
-  // selects rather than a branch on sel: the compiler keeps lb, lc, lf in registers
-  const T l = lout * s;
-  lb += (sel == 0) ? l * sb : T(0);
-  lc += (sel == 1) ? l * sc : T(0);
-  lf += (sel == 2) ? l * sf : T(0);
+// minmod_raw reverse from its selection (minmod_sel): the sensitivity goes
+// to the selected argument.
+template <class T> GPO_HD inline void sel_rev(int sel, T lout, T& lb, T& lc, T& lf) {
+  lb += (sel == 0) ? lout : T(0);
+  lc += (sel == 1) ? lout : T(0);
+  lf += (sel == 2) ? lout : T(0);
 }
 
-// Reverse of the x-slope computation at (r, s) given the slope sensitivities;
-// adds the contributions to cells s-1, s, s+1 (index 0, 1, 2) into le/lu/lv.
+// Reverse of the x-slope computation at (r, s) given the slope sensitivities
+// and the selections of its forward evaluation (xslope_v's code); Fl, Fc, Fr
+// are the coriolis values of cells s-1, s, s+1.  Adds the contributions to
+// those cells (index 0, 1, 2) into le/lu/lv.
 template <class T>
-GPO_HD void xslope_rev_v(const Geo& g, const Sten<T>& st, int s, T lq0, T lq1, T lK, T* le, T* lu, T* lv) {
-  const T le_ = st.e0, ce = st.e1, re = st.e2;
-  const T lu_ = st.u0, cu = st.u1, ru = st.u2;
-  const T lv0 = st.v0, cv0 = st.v1, rv0 = st.v2;
+GPO_HD void xslope_rev_code(const Geo& g, T Fl, T Fc, T Fr, int s, int code, T lq0, T lq1, T lK, T* le, T* lu,
+                            T* lv) {
   const T th = T(g.theta);
   {  // q0 = minmod_slope(lu, cu, ru), q1 = minmod_slope(lv, cv, rv)
     T lb = 0, lc = 0, lf = 0;
-    minmod_rev((cu - lu_) * th, (ru - lu_) * T(0.5), (ru - cu) * th, lq0, lb, lc, lf);
+    sel_rev(code & 3, lq0, lb, lc, lf);
     lu[1] += th * lb - th * lf;
     lu[0] += -th * lb - T(0.5) * lc;
     lu[2] += T(0.5) * lc + th * lf;
     lb = lc = lf = 0;
-    minmod_rev((cv0 - lv0) * th, (rv0 - lv0) * T(0.5), (rv0 - cv0) * th, lq1, lb, lc, lf);
+    sel_rev((code >> 2) & 3, lq1, lb, lc, lf);
     lv[1] += th * lb - th * lf;
     lv[0] += -th * lb - T(0.5) * lc;
     lv[2] += T(0.5) * lc + th * lf;
@@ -907,35 +943,24 @@ GPO_HD void xslope_rev_v(const Geo& g, const Sten<T>& st, int s, T lq0, T lq1, T
     if (s > g.nx + 1) sc = -1;
   }
   const T nx_ = T(g.north_x), ny_ = T(g.north_y), G = T(g.g), Vc = T(g.dx) / (T(2) * G), g2 = T(2) * G;
-  const T Fl = st.F0, Fc = st.F1, Fr = st.F2;
-  const T lfv = (nx_ * lu_ + ny_ * (sl * lv0)) * Fl;
-  const T cfv = (nx_ * cu + ny_ * (sc * cv0)) * Fc;
-  const T rfv = (nx_ * ru + ny_ * (sr * rv0)) * Fr;
-  const T b = th * G * (ce - le_ - Vc * (cfv + lfv));
-  const T c = T(0.5) * G * (re - le_ - Vc * (rfv + T(2) * cfv + lfv));
-  const T f = th * G * (re - ce - Vc * (cfv + rfv));
-  const T K0 = minmod_raw(b, c, f);
-  const T sa = ((g.bc_w == 1 && s < 2) || (g.bc_e == 1 && s > g.nx + 2)) ? T(-1) : T(1);
-  const T dxfv = T(g.dx) * Fc * (sa * cv0);
-  const T Hw = T(0.5) * (st.Ha + st.Hb);
-  const T He = T(0.5) * (st.Hc + st.Hd);
-  const T hw = ce + Hw - qdiv(K0 + dxfv, g2);
-  const T he = ce + He + qdiv(K0 + dxfv, g2);
+  // K = minmod_raw(b, c, f) of the potential-energy differences, then replaced
+  // where a face depth hw (low side) or he (high side) is not positive
   T lK1 = lK, ldxfv = 0, lce = 0;
-  if (!(he > T(0))) {
+  if (code & kSelHighDry) {
     ldxfv -= lK;
     lce -= g2 * lK;
     lK1 = 0;
   }
   T lK0 = lK1;
-  if (!(hw > T(0))) {
+  if (code & kSelLowDry) {
     ldxfv -= lK1;
     lce += g2 * lK1;
     lK0 = 0;
   }
+  const T sa = ((g.bc_w == 1 && s < 2) || (g.bc_e == 1 && s > g.nx + 2)) ? T(-1) : T(1);
   lv[1] += sa * T(g.dx) * Fc * ldxfv;
   T lb = 0, lc = 0, lf = 0;
-  minmod_rev(b, c, f, lK0, lb, lc, lf);
+  sel_rev((code >> 4) & 3, lK0, lb, lc, lf);
   lce += th * G * lb - th * G * lf;
   const T lle = -th * G * lb - T(0.5) * G * lc;
   const T lre = T(0.5) * G * lc + th * G * lf;
@@ -953,26 +978,35 @@ GPO_HD void xslope_rev_v(const Geo& g, const Sten<T>& st, int s, T lq0, T lq1, T
   lv[2] += sr * ny_ * Fr * lrfv;
 }
 
+// The same from the stencil: the forward evaluation gives the selections.
+template <class T>
+GPO_HD void xslope_rev_v(const Geo& g, const Sten<T>& st, int s, T lq0, T lq1, T lK, T* le, T* lu, T* lv) {
+  T q0, q1, K;
+  int code = 0;
+  xslope_v(g, st, s, q0, q1, K, &code);
+  xslope_rev_code(g, st.F0, st.F1, st.F2, s, code, lq0, lq1, lK, le, lu, lv);
+}
+
 template <class T>
 GPO_HD void xslope_rev(const Geo& g, const Fwd<T>& w, const T* Hi, const T* fc, int r, int s, T lq0, T lq1, T lK,
                        T* le, T* lu, T* lv) {
   xslope_rev_v(g, xsten(g, w, Hi, fc, r, s), s, lq0, lq1, lK, le, lu, lv);
 }
 
+// Reverse of the y-slope computation at (s, c) from its selections (yslope_v's
+// code); Fl, Fc, Fu are the coriolis values of rows s-1, s, s+1.
 template <class T>
-GPO_HD void yslope_rev_v(const Geo& g, const Sten<T>& st, int s, T lq0, T lq1, T lL, T* le, T* lu, T* lv) {
-  const T le_ = st.e0, ce = st.e1, ue = st.e2;
-  const T lu0 = st.u0, cu0 = st.u1, uu0 = st.u2;
-  const T lv_ = st.v0, cv = st.v1, uv = st.v2;
+GPO_HD void yslope_rev_code(const Geo& g, T Fl, T Fc, T Fu, int s, int code, T lq0, T lq1, T lL, T* le, T* lu,
+                            T* lv) {
   const T th = T(g.theta);
   {
     T lb = 0, lc = 0, lf = 0;
-    minmod_rev((cu0 - lu0) * th, (uu0 - lu0) * T(0.5), (uu0 - cu0) * th, lq0, lb, lc, lf);
+    sel_rev(code & 3, lq0, lb, lc, lf);
     lu[1] += th * lb - th * lf;
     lu[0] += -th * lb - T(0.5) * lc;
     lu[2] += T(0.5) * lc + th * lf;
     lb = lc = lf = 0;
-    minmod_rev((cv - lv_) * th, (uv - lv_) * T(0.5), (uv - cv) * th, lq1, lb, lc, lf);
+    sel_rev((code >> 2) & 3, lq1, lb, lc, lf);
     lv[1] += th * lb - th * lf;
     lv[0] += -th * lb - T(0.5) * lc;
     lv[2] += T(0.5) * lc + th * lf;
@@ -987,35 +1021,22 @@ GPO_HD void yslope_rev_v(const Geo& g, const Sten<T>& st, int s, T lq0, T lq1, T
     if (s > g.ny + 1) sc = -1;
   }
   const T ex = T(g.north_y), ey = T(-g.north_x), G = T(g.g), Uc = T(g.dy) / (T(2) * G), g2 = T(2) * G;
-  const T Fl = st.F0, Fc = st.F1, Fu = st.F2;
-  const T lfu = (ex * (sl * lu0) + ey * lv_) * Fl;
-  const T cfu = (ex * (sc * cu0) + ey * cv) * Fc;
-  const T ufu = (ex * (su * uu0) + ey * uv) * Fu;
-  const T b = th * G * (ce - le_ + Uc * (cfu + lfu));
-  const T cc = T(0.5) * G * (ue - le_ + Uc * (ufu + T(2) * cfu + lfu));
-  const T f = th * G * (ue - ce + Uc * (cfu + ufu));
-  const T L0 = minmod_raw(b, cc, f);
-  const T sa = ((g.bc_s == 1 && s < 2) || (g.bc_n == 1 && s > g.ny + 2)) ? T(-1) : T(1);
-  const T dyfu = T(g.dy) * Fc * (sa * cu0);
-  const T Hs = T(0.5) * (st.Ha + st.Hb);
-  const T Hn = T(0.5) * (st.Hc + st.Hd);
-  const T hs = ce + Hs - qdiv(L0 - dyfu, g2);
-  const T hn = ce + Hn + qdiv(L0 - dyfu, g2);
   T lL1 = lL, ldyfu = 0, lce = 0;
-  if (!(hn > T(0))) {
+  if (code & kSelHighDry) {
     ldyfu += lL;
     lce -= g2 * lL;
     lL1 = 0;
   }
   T lL0 = lL1;
-  if (!(hs > T(0))) {
+  if (code & kSelLowDry) {
     ldyfu += lL1;
     lce += g2 * lL1;
     lL0 = 0;
   }
+  const T sa = ((g.bc_s == 1 && s < 2) || (g.bc_n == 1 && s > g.ny + 2)) ? T(-1) : T(1);
   lu[1] += sa * T(g.dy) * Fc * ldyfu;
   T lb = 0, lc = 0, lf = 0;
-  minmod_rev(b, cc, f, lL0, lb, lc, lf);
+  sel_rev((code >> 4) & 3, lL0, lb, lc, lf);
   lce += th * G * lb - th * G * lf;
   const T lle = -th * G * lb - T(0.5) * G * lc;
   const T lue = T(0.5) * G * lc + th * G * lf;
@@ -1034,6 +1055,14 @@ GPO_HD void yslope_rev_v(const Geo& g, const Sten<T>& st, int s, T lq0, T lq1, T
 }
 
 template <class T>
+GPO_HD void yslope_rev_v(const Geo& g, const Sten<T>& st, int s, T lq0, T lq1, T lL, T* le, T* lu, T* lv) {
+  T q0, q1, L;
+  int code = 0;
+  yslope_v(g, st, s, q0, q1, L, &code);
+  yslope_rev_code(g, st.F0, st.F1, st.F2, s, code, lq0, lq1, lL, le, lu, lv);
+}
+
+template <class T>
 GPO_HD void yslope_rev(const Geo& g, const Fwd<T>& w, const T* Hi, const T* fc, int s, int c, T lq0, T lq1, T lL,
                        T* le, T* lu, T* lv) {
   yslope_rev_v(g, ysten(g, w, Hi, fc, s, c), s, lq0, lq1, lL, le, lu, lv);
@@ -1049,11 +1078,13 @@ template <class T> struct CellParts {
 // A2 on loaded values: the cell's four corner depths (H_rc with r, c the
 // offsets), its coriolis value F, its x- and y-slope stencils, its update
 // sensitivity a (out_adj's rule), the sensitivities of its side of x-face c
-// (xl), of x-face c-1 (xr), of y-face r (yl) and of y-face r-1 (yr).
-template <class T>
+// (xl), of x-face c-1 (xr), of y-face r (yl) and of y-face r-1 (yr).  With
+// SEL the slope reverses take the selections of the cell's forward slope
+// evaluations (xcode, ycode) instead of evaluating them again.
+template <class T, bool SEL = false>
 GPO_HD CellParts<T> a2_parts_v(const Geo& g, T dt, int rk_step, int r, int c, T H00, T H10, T H01, T H11, T F,
                                const Sten<T>& sx, const Sten<T>& sy, const CellAdj<T>& a, const T* xl, const T* xr,
-                               const T* yl, const T* yr) {
+                               const T* yl, const T* yr, int xcode = 0, int ycode = 0) {
   CellParts<T> out;
   for (int i = 0; i < 3; ++i) out.own[i] = out.xm[i] = out.xp[i] = out.ym[i] = out.yp[i] = T(0);
   if (c < 0 || r < 0 || c >= g.nx + 4 || r >= g.ny + 4) return out;
@@ -1134,7 +1165,10 @@ GPO_HD CellParts<T> a2_parts_v(const Geo& g, T dt, int rk_step, int r, int c, T 
     }
     if ((left || right) && !(lq0 == T(0) && lq1 == T(0) && lK == T(0))) {
       T e3[3] = {0, 0, 0}, u3[3] = {0, 0, 0}, v3[3] = {0, 0, 0};
-      xslope_rev_v(g, sx, c, lq0, lq1, lK, e3, u3, v3);
+      if constexpr (SEL)
+        xslope_rev_code(g, sx.F0, sx.F1, sx.F2, c, xcode, lq0, lq1, lK, e3, u3, v3);
+      else
+        xslope_rev_v(g, sx, c, lq0, lq1, lK, e3, u3, v3);
       xm[0] = e3[0], xm[1] = u3[0], xm[2] = v3[0];
       lR0 += e3[1], lR1 += u3[1], lR2 += v3[1];
       xp[0] = e3[2], xp[1] = u3[2], xp[2] = v3[2];
@@ -1172,7 +1206,10 @@ GPO_HD CellParts<T> a2_parts_v(const Geo& g, T dt, int rk_step, int r, int c, T 
     }
     if ((low || high) && !(lq0 == T(0) && lq1 == T(0) && lL == T(0))) {
       T e3[3] = {0, 0, 0}, u3[3] = {0, 0, 0}, v3[3] = {0, 0, 0};
-      yslope_rev_v(g, sy, r, lq0, lq1, lL, e3, u3, v3);
+      if constexpr (SEL)
+        yslope_rev_code(g, sy.F0, sy.F1, sy.F2, r, ycode, lq0, lq1, lL, e3, u3, v3);
+      else
+        yslope_rev_v(g, sy, r, lq0, lq1, lL, e3, u3, v3);
       ym[0] = e3[0], ym[1] = u3[0], ym[2] = v3[0];
       lR0 += e3[1], lR1 += u3[1], lR2 += v3[1];
       yp[0] = e3[2], yp[1] = u3[2], yp[2] = v3[2];
