@@ -509,7 +509,7 @@ __global__ void __launch_bounds__(256) nd_soar_b(const NoiseDriftItem* __restric
 // tiles of every member with model error (noise_tile: normal field, SOAR and
 // geostrophic balance of a 64 x 16 tile, the values of gpo_random_normal,
 // gpo_soar with scale 1 and gpo_geostrophic_balance): on the forward state
-// (TRAJ false), or on segment state `which` keeping its pre-noise copy in
+// (TRAJ false), or on segment state `which` keeping its pre-noise eta in
 // `keep` and the SOAR field in the segment's slot (TRAJ true).
 template <bool TRAJ>
 __global__ void __launch_bounds__(256) nd_noise_b(const NoiseDriftItem* __restrict__ tab,
@@ -527,10 +527,9 @@ __global__ void __launch_bounds__(256) nd_noise_b(const NoiseDriftItem* __restri
   float* base = TRAJ ? m.qs + (size_t)which * m.st : m.state;
   a.eta = base, a.hu = base + m.plane, a.hv = base + 2 * m.plane;
   a.Hi = m.Hi, a.f_cells = nullptr;
-  a.pre_e = a.pre_u = a.pre_v = nullptr, a.d_out = nullptr, a.d_row = m.g.nx + 4;
-  if (TRAJ) {
-    float* p = m.pre + (size_t)keep * m.st;
-    a.pre_e = p, a.pre_u = p + m.plane, a.pre_v = p + 2 * m.plane;
+  a.pre_e = nullptr, a.d_out = nullptr, a.d_row = m.g.nx + 4;
+  if (TRAJ) {  // the sweep reads the eta plane of the pre-noise copy (its dry-cell test)
+    a.pre_e = m.pre + (size_t)keep * m.st;
     a.d_out = nd_coarse(m, slot);
   }
   noise_tile<TRAJ, TRAJ>(a, 2 + kTileH * (t.y >> 16), 2 + kTileW * (t.y & 0xFFFF));

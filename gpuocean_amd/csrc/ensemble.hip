@@ -55,7 +55,7 @@ __global__ void __launch_bounds__(256) noise_fused(const NoiseMember* __restrict
   a.scale = m.scale, a.dx = m.dx, a.dy = m.dy, a.g = m.g, a.f = m.f, a.beta = m.beta;
   a.north_x = m.north_x, a.north_y = m.north_y, a.land = m.land;
   a.eta = m.eta, a.hu = m.hu, a.hv = m.hv, a.Hi = m.Hi, a.f_cells = m.f_cells;
-  a.pre_e = a.pre_u = a.pre_v = nullptr, a.d_out = nullptr, a.d_row = 0;
+  a.pre_e = nullptr, a.d_out = nullptr, a.d_row = 0;
   noise_tile<false, false>(a, 2 + kTileH * (t.y >> 16), 2 + kTileW * (t.y & 0xFFFF));
 }
 

@@ -2,8 +2,9 @@
 // (ensemble.hip, noise_fused) and the differentiable batched ensemble
 // (adjoint.hip): normal field, SOAR and geostrophic balance of one 64 x 16
 // tile of interior cells in one block.  The differentiable path also keeps
-// the state before the model error (KEEP_PRE) and the SOAR field itself
-// (STORE_D), which the reverse sweep reads.
+// eta before the model error (KEEP_PRE: the reverse of the substep that
+// produced it tests it for cells clamped dry) and the SOAR field itself
+// (STORE_D), which the reverse of the balance reads.
 #pragma once
 
 #include <cstdint>
@@ -23,7 +24,7 @@ struct NoiseTile {
   float *eta, *hu, *hv;
   const float* Hi;
   const float* f_cells;
-  float *pre_e, *pre_u, *pre_v;  // KEEP_PRE: the state before the model error (same pitch)
+  float* pre_e;                   // KEEP_PRE: eta before the model error (same pitch)
   float* d_out;                   // STORE_D: the SOAR field, dense rows of d_row elements
   int d_row;
 };
@@ -70,11 +71,11 @@ __device__ __forceinline__ void noise_tile(const NoiseTile& m, int T0, int S0) {
       in[k] = balance_load(m.f == 0.0f ? m.f_cells : nullptr, m.f_pitch, m.eta, m.hu, m.hv, m.pitch, m.Hi,
                            m.hi_pitch, tj, ti);
   }
-  if (KEEP_PRE) {  // the state before the model error
+  if (KEEP_PRE) {  // eta before the model error
 #pragma unroll
     for (int k = 0; k < kRowsPerThread; ++k) {
       const int tj = T0 + ly0 + 4 * k, ti = S0 + lx;
-      if (tj <= m.ny + 1 && ti <= m.nx + 1) balance_store(m.pre_e, m.pre_u, m.pre_v, m.pitch, tj, ti, in[k].e, in[k].u, in[k].v);
+      if (tj <= m.ny + 1 && ti <= m.nx + 1) mrow(m.pre_e, m.pitch, tj)[ti] = in[k].e;
     }
   }
 
