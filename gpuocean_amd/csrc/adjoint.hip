@@ -605,16 +605,20 @@ __global__ void __launch_bounds__(64) nd_drift_rev_b(const NoiseDriftItem* __res
 }
 
 // Reverse of the model error of step j on lambda buffer (j+1) & 1 (the
-// member's coarse field holds that step's SOAR field).
-__global__ void __launch_bounds__(256) nd_balance_rev_b(const NoiseDriftItem* __restrict__ tab, int j, int slot) {
-  const NoiseDriftItem& m = tab[blockIdx.z];
-  if (!m.noise) return;
+// member's coarse field holds that step's SOAR field), over the 64 x 16
+// tiles of the members with model error: four rows per thread.
+__global__ void __launch_bounds__(256) nd_balance_rev_b(const NoiseDriftItem* __restrict__ tab,
+                                                        const int2* __restrict__ tiles, int j, int slot) {
+  const int2 t = tiles[blockIdx.x];
+  const NoiseDriftItem& m = tab[t.x];
   float* l = m.lam + (size_t)((j + 1) & 1) * m.st;
-  Geo g = m.g;
-  balance_rev_cell<float>(g, (double)m.beta, nullptr, 0, nd_coarse(m, slot), 4 * (m.g.nx + 4), m.Hi, m.land, l,
-                          l + m.plane,
-                          l + 2 * m.plane, blockIdx.x * 64 + (threadIdx.x & 63) + 2,
-                          blockIdx.y * 4 + (threadIdx.x >> 6) + 2);
+  const Geo g = m.g;
+  const float* coarse = nd_coarse(m, slot);
+  const int T0 = 2 + kTileH * (t.y >> 16), ti = 2 + kTileW * (t.y & 0xFFFF) + (threadIdx.x & 63);
+#pragma unroll
+  for (int k = 0; k < kTileH / 4; ++k)
+    balance_rev_cell<float>(g, (double)m.beta, nullptr, 0, coarse, 4 * (g.nx + 4), m.Hi, m.land, l, l + m.plane,
+                            l + 2 * m.plane, ti, T0 + (threadIdx.x >> 6) + 4 * k);
 }
 
 // Drifter positions: mode 0 forward -> checkpoint seg, mode 1 checkpoint seg ->
@@ -1791,8 +1795,8 @@ int gpo_adj_batch_sweep(gpo_adj_batch h, int32_t m_steps, int32_t* flag, void* s
         hipLaunchKernelGGL(nd_xi_b, dim3((b->max_xi + 255) / 256, b->n), dim3(256), 0, s, b->d_nd, b->seg_start + j);
         hipLaunchKernelGGL(nd_soar_b, dim3((b->max_cells + 255) / 256, b->n), dim3(256), 0, s, b->d_nd, -1);
       }
-      hipLaunchKernelGGL(nd_balance_rev_b, dim3((b->max_w - 4 + 63) / 64, (b->max_h - 4 + 3) / 4, b->n), dim3(256),
-                         0, s, b->d_nd, j, b->coarse_saved ? j : -1);
+      hipLaunchKernelGGL(nd_balance_rev_b, dim3(b->n_ntiles), dim3(256), 0, s, b->d_nd, b->d_ntiles, j,
+                         b->coarse_saved ? j : -1);
       rc = hip_check(hipGetLastError(), "batched model-error reverse launch");
     }
     if (!rc) rc = adj_batch_stage(b, t1, flag, s, !b->any_noise);
