@@ -414,47 +414,119 @@ __global__ void drift_fwd(DriftGeo q, const T* e, const T* hu, const T* hv, cons
   (void)hh;
 }
 
-// Reverse of one drift step at positions pos (before the step) over the state
-// it read: lam_pos (in: at the new positions; out: at the old ones) and the
-// state sensitivities l_* (accumulated; cells whose values are stored
-// constants receive nothing).
-template <class T>
-__device__ void drift_rev_one(const DriftGeo& q, const T* e, const T* hu, const T* hv, const T* Bm, const T* pos,
-                              T* lpos, T* l_e, T* l_u, T* l_v, int d) {
-  if (d >= q.n) return;
-  const DriftAt<T> a = drift_at<T>(q.nx, q.ny, T(q.dx), T(q.dy), pos[2 * d], pos[2 * d + 1]);
-  if (!a.ok) return;  // the drifter stayed put: lam_pos passes through unchanged
-  const T s_dt = T(q.sens) * T(q.dt);
-  const T lx = lpos[2 * d], ly = lpos[2 * d + 1];
-  const T ga = s_dt * lx, gb = s_dt * ly;  // sensitivities of u and v
-  const T wq[4] = {(T(1) - a.xf) * (T(1) - a.yf), a.xf * (T(1) - a.yf), (T(1) - a.xf) * a.yf, a.xf * a.yf};
-  T uu[4], vv[4];
-  for (int c = 0; c < 4; ++c) {
-    const int cc = a.cx0 + (c & 1), rr = a.cy0 + (c >> 1);
-    const int mc = wrap_axis(cc, q.nx, q.per_ew), mr = wrap_axis(rr, q.ny, q.per_ns);
-    const bool dyn = mc >= 0 && mr >= 0;
-    const size_t ks = dyn ? (size_t)mr * q.sp + mc : (size_t)rr * q.sp + cc;
-    const T h = Bm[(size_t)rr * q.bp + cc] + e[ks];
-    uu[c] = hu[ks] / h;
-    vv[c] = hv[ks] / h;
-    if (dyn) {
-      const T a_u = ga * wq[c] / h, a_v = gb * wq[c] / h;
-      atomicAdd(&l_u[ks], a_u);
-      atomicAdd(&l_v[ks], a_v);
-      atomicAdd(&l_e[ks], -(a_u * uu[c] + a_v * vv[c]));
-    }
-  }
-  const T dudx = ((T(1) - a.yf) * (uu[1] - uu[0]) + a.yf * (uu[3] - uu[2])) / T(q.dx);
-  const T dudy = ((T(1) - a.xf) * (uu[2] - uu[0]) + a.xf * (uu[3] - uu[1])) / T(q.dy);
-  const T dvdx = ((T(1) - a.yf) * (vv[1] - vv[0]) + a.yf * (vv[3] - vv[2])) / T(q.dx);
-  const T dvdy = ((T(1) - a.xf) * (vv[2] - vv[0]) + a.xf * (vv[3] - vv[1])) / T(q.dy);
-  lpos[2 * d] = lx + ga * dudx + gb * dvdx;
-  lpos[2 * d + 1] = ly + ga * dudy + gb * dvdy;
+// In-order accumulation of one chunk of contributions by a block of 256
+// threads: item i (< kOrdItems) adds sv[i] = (u, v, eta) parts to cell
+// sk[i] >> 10 of (l_u, l_v, l_e); sk[i] = (cell << 10) | i, or kNoKey for an
+// item without a contribution, thread t owning items 4t..4t+3.  The items
+// are sorted by (cell, item) (bitonic: partners within a thread in
+// registers, within a wavefront by lane shuffles, across wavefronts through
+// shared memory), and the first item of each cell adds that cell's items in
+// item order: the sums do not depend on the order the hardware runs the
+// items in, as float atomics would.  Chunks added one after another keep the
+// order across chunks.
+constexpr int kOrdItems = 1024;
+constexpr uint64_t kNoKey = ~0ull;
+__device__ inline uint64_t shfl_xor_u64(uint64_t v, int m) {
+  const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, m), hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), m);
+  return ((uint64_t)hi << 32) | lo;
 }
 template <class T>
-__global__ void drift_rev(DriftGeo q, const T* e, const T* hu, const T* hv, const T* Bm, const T* pos, T* lpos,
-                          T* l_e, T* l_u, T* l_v) {
-  drift_rev_one<T>(q, e, hu, hv, Bm, pos, lpos, l_e, l_u, l_v, blockIdx.x * blockDim.x + threadIdx.x);
+__device__ void ordered_add(uint64_t* sk, const T (*sv)[3], T* l_u, T* l_v, T* l_e) {
+  const int t = threadIdx.x;
+  uint64_t key[4];
+  for (int r = 0; r < 4; ++r) key[r] = sk[4 * t + r];
+  for (int k = 2; k <= kOrdItems; k <<= 1)
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      uint64_t other[4];
+      if (j >= 256) {  // partner in another wavefront (items 256w..256w+255 belong to wavefront w)
+        __syncthreads();
+        for (int r = 0; r < 4; ++r) sk[4 * t + r] = key[r];
+        __syncthreads();
+        for (int r = 0; r < 4; ++r) other[r] = sk[(4 * t + r) ^ j];
+      } else if (j >= 4) {  // partner in lane t ^ (j / 4) of the same wavefront
+        for (int r = 0; r < 4; ++r) other[r] = shfl_xor_u64(key[r], j >> 2);
+      } else {
+        for (int r = 0; r < 4; ++r) other[r] = key[r ^ j];
+      }
+      for (int r = 0; r < 4; ++r) {  // the lower index of a pair keeps the minimum in an ascending run
+        const int x = 4 * t + r;
+        const bool keep_min = ((x & j) == 0) == ((x & k) == 0);
+        key[r] = keep_min ? (key[r] < other[r] ? key[r] : other[r]) : (key[r] < other[r] ? other[r] : key[r]);
+      }
+    }
+  __syncthreads();
+  for (int r = 0; r < 4; ++r) sk[4 * t + r] = key[r];
+  __syncthreads();
+  for (int i = threadIdx.x; i < kOrdItems; i += 256) {
+    const uint64_t cell = sk[i] >> 10;
+    if (sk[i] == kNoKey || (i > 0 && (sk[i - 1] >> 10) == cell)) continue;  // empty, or not the cell's first item
+    T u = l_u[cell], v = l_v[cell], e = l_e[cell];
+    for (int j = i; j < kOrdItems && sk[j] != kNoKey && (sk[j] >> 10) == cell; ++j) {
+      const int it = (int)(sk[j] & (kOrdItems - 1));
+      u = u + sv[it][0];
+      v = v + sv[it][1];
+      e = e + sv[it][2];
+    }
+    l_u[cell] = u;
+    l_v[cell] = v;
+    l_e[cell] = e;
+  }
+  __syncthreads();
+}
+
+// Reverse of one drift step at positions pos (before the step) over the state
+// it read, by one block of 256 threads for all q.n drifters (chunks of 256):
+// lam_pos (in: at the new positions; out: at the old ones) and the state
+// sensitivities l_* (accumulated in drifter order; cells whose values are
+// stored constants receive nothing).
+template <class T>
+__device__ void drift_rev_block(const DriftGeo& q, const T* e, const T* hu, const T* hv, const T* Bm, const T* pos,
+                                T* lpos, T* l_e, T* l_u, T* l_v) {
+  __shared__ uint64_t sk[kOrdItems];
+  __shared__ T sv[kOrdItems][3];
+  const int t = threadIdx.x;
+  for (int base = 0; base < q.n; base += 256) {
+    const int d = base + t;
+    for (int c = 0; c < 4; ++c) sk[4 * t + c] = kNoKey;
+    const DriftAt<T> a = d < q.n ? drift_at<T>(q.nx, q.ny, T(q.dx), T(q.dy), pos[2 * d], pos[2 * d + 1])
+                                 : DriftAt<T>{false, 0, 0, T(0), T(0)};
+    if (a.ok) {  // otherwise the drifter stayed put: lam_pos passes through unchanged
+      const T s_dt = T(q.sens) * T(q.dt);
+      const T lx = lpos[2 * d], ly = lpos[2 * d + 1];
+      const T ga = s_dt * lx, gb = s_dt * ly;  // sensitivities of u and v
+      const T wq[4] = {(T(1) - a.xf) * (T(1) - a.yf), a.xf * (T(1) - a.yf), (T(1) - a.xf) * a.yf, a.xf * a.yf};
+      T uu[4], vv[4];
+      for (int c = 0; c < 4; ++c) {
+        const int cc = a.cx0 + (c & 1), rr = a.cy0 + (c >> 1);
+        const int mc = wrap_axis(cc, q.nx, q.per_ew), mr = wrap_axis(rr, q.ny, q.per_ns);
+        const bool dyn = mc >= 0 && mr >= 0;
+        const size_t ks = dyn ? (size_t)mr * q.sp + mc : (size_t)rr * q.sp + cc;
+        const T h = Bm[(size_t)rr * q.bp + cc] + e[ks];
+        uu[c] = hu[ks] / h;
+        vv[c] = hv[ks] / h;
+        if (dyn) {
+          const T a_u = ga * wq[c] / h, a_v = gb * wq[c] / h;
+          sk[4 * t + c] = ((uint64_t)ks << 10) | (uint64_t)(4 * t + c);
+          sv[4 * t + c][0] = a_u;
+          sv[4 * t + c][1] = a_v;
+          sv[4 * t + c][2] = -(a_u * uu[c] + a_v * vv[c]);
+        }
+      }
+      const T dudx = ((T(1) - a.yf) * (uu[1] - uu[0]) + a.yf * (uu[3] - uu[2])) / T(q.dx);
+      const T dudy = ((T(1) - a.xf) * (uu[2] - uu[0]) + a.xf * (uu[3] - uu[1])) / T(q.dy);
+      const T dvdx = ((T(1) - a.yf) * (vv[1] - vv[0]) + a.yf * (vv[3] - vv[2])) / T(q.dx);
+      const T dvdy = ((T(1) - a.xf) * (vv[2] - vv[0]) + a.xf * (vv[3] - vv[1])) / T(q.dy);
+      lpos[2 * d] = lx + ga * dudx + gb * dvdx;
+      lpos[2 * d + 1] = ly + ga * dudy + gb * dvdy;
+    }
+    __syncthreads();
+    ordered_add<T>(sk, sv, l_u, l_v, l_e);
+  }
+}
+template <class T>
+__global__ void __launch_bounds__(256) drift_rev(DriftGeo q, const T* e, const T* hu, const T* hv, const T* Bm,
+                                                 const T* pos, T* lpos, T* l_e, T* l_u, T* l_v) {
+  drift_rev_block<T>(q, e, hu, hv, Bm, pos, lpos, l_e, l_u, l_v);
 }
 
 // ---- batched model error and drifters of the differentiable ensemble
@@ -592,16 +664,16 @@ __global__ void __launch_bounds__(64) nd_drift_b(const NoiseDriftItem* __restric
 }
 
 // Reverse of the drift of step j of the swept segment: positions pos_seg[j],
-// state qs[j+1], lambda buffer (j+1) & 1.
-__global__ void __launch_bounds__(64) nd_drift_rev_b(const NoiseDriftItem* __restrict__ tab, int j) {
+// state qs[j+1], lambda buffer (j+1) & 1; one block per member.
+__global__ void __launch_bounds__(256) nd_drift_rev_b(const NoiseDriftItem* __restrict__ tab, int j) {
   const NoiseDriftItem& m = tab[blockIdx.y];
   if (m.n_drifters == 0) return;
   const DriftGeo q{m.g.nx, m.g.ny, m.g.sp, m.bm_pitch, m.per_ns, m.per_ew, m.n_drifters, m.g.dx, m.g.dy,
                    (double)m.dt, 1.0};
   const float* s = m.qs + (size_t)(j + 1) * m.st;
   float* l = m.lam + (size_t)((j + 1) & 1) * m.st;
-  drift_rev_one<float>(q, s, s + m.plane, s + 2 * m.plane, m.Bm, m.pos_seg + (size_t)j * 2 * m.n_drifters,
-                       m.lam_pos, l, l + m.plane, l + 2 * m.plane, blockIdx.x * 64 + threadIdx.x);
+  drift_rev_block<float>(q, s, s + m.plane, s + 2 * m.plane, m.Bm, m.pos_seg + (size_t)j * 2 * m.n_drifters,
+                         m.lam_pos, l, l + m.plane, l + 2 * m.plane);
 }
 
 // Reverse of the model error of step j on lambda buffer (j+1) & 1 (the
@@ -637,42 +709,61 @@ __global__ void __launch_bounds__(64) nd_pos_copy_b(const NoiseDriftItem* __rest
 
 // Observation of the cell-centre velocity (u, v) = (hu, hv)/(eta + Hm),
 // bilinear between the four cell centres around each position (x, y).
-// J += sum (u - u_obs)^2 + (v - v_obs)^2; lambda += dJ/d(eta, hu, hv).
+// J += sum (u - u_obs)^2 + (v - v_obs)^2; lambda += dJ/d(eta, hu, hv).  One
+// block of 256 threads takes the points in chunks of 256; the sensitivities
+// are added in point order (ordered_add) and J in a fixed order, so both are
+// the same run to run.
 template <class T>
-__global__ void misfit(Geo g, int n, const float* pos, const float* obs, const T* e, const T* hu, const T* hv,
-                       const T* Hi, double* J, T* l_e, T* l_u, T* l_v) {
-  const int d = blockIdx.x * blockDim.x + threadIdx.x;
-  double jd = 0.0;
-  if (d < n) {
-    const T sx = T(pos[2 * d]) / T(g.dx) - T(0.5), sy = T(pos[2 * d + 1]) / T(g.dy) - T(0.5);
-    int i0 = (int)floor((double)sx), j0 = (int)floor((double)sy);
-    i0 = min(max(i0, 0), g.nx - 2);
-    j0 = min(max(j0, 0), g.ny - 2);
-    const T ax = sx - T(i0), ay = sy - T(j0);
-    const T wgt[4] = {(T(1) - ax) * (T(1) - ay), ax * (T(1) - ay), (T(1) - ax) * ay, ax * ay};
-    const int rr[4] = {j0 + 2, j0 + 2, j0 + 3, j0 + 3}, cc[4] = {i0 + 2, i0 + 3, i0 + 2, i0 + 3};
-    T u = 0, v = 0, hh[4];
-    for (int q = 0; q < 4; ++q) {
-      const size_t k = (size_t)rr[q] * g.sp + cc[q];
-      hh[q] = e[k] + cell_depth_land(g, Hi, rr[q], cc[q]);
-      u += wgt[q] * hu[k] / hh[q];
-      v += wgt[q] * hv[k] / hh[q];
-    }
-    const T du = u - T(obs[2 * d]), dv = v - T(obs[2 * d + 1]);
-    jd = (double)(du * du + dv * dv);
-    if (l_e) {
+__global__ void __launch_bounds__(256) misfit(Geo g, int n, const float* pos, const float* obs, const T* e,
+                                              const T* hu, const T* hv, const T* Hi, double* J, T* l_e, T* l_u,
+                                              T* l_v) {
+  __shared__ uint64_t sk[kOrdItems];
+  __shared__ T sv[kOrdItems][3];
+  __shared__ double sj[256];
+  const int t = threadIdx.x;
+  double jsum = 0.0;
+  for (int base = 0; base < n; base += 256) {
+    const int d = base + t;
+    for (int q = 0; q < 4; ++q) sk[4 * t + q] = kNoKey;
+    if (d < n) {
+      const T sx = T(pos[2 * d]) / T(g.dx) - T(0.5), sy = T(pos[2 * d + 1]) / T(g.dy) - T(0.5);
+      int i0 = (int)floor((double)sx), j0 = (int)floor((double)sy);
+      i0 = min(max(i0, 0), g.nx - 2);
+      j0 = min(max(j0, 0), g.ny - 2);
+      const T ax = sx - T(i0), ay = sy - T(j0);
+      const T wgt[4] = {(T(1) - ax) * (T(1) - ay), ax * (T(1) - ay), (T(1) - ax) * ay, ax * ay};
+      const int rr[4] = {j0 + 2, j0 + 2, j0 + 3, j0 + 3}, cc[4] = {i0 + 2, i0 + 3, i0 + 2, i0 + 3};
+      T u = 0, v = 0, hh[4];
       for (int q = 0; q < 4; ++q) {
         const size_t k = (size_t)rr[q] * g.sp + cc[q];
-        const T a = T(2) * du * wgt[q] / hh[q], b = T(2) * dv * wgt[q] / hh[q];
-        atomicAdd(&l_u[k], a);
-        atomicAdd(&l_v[k], b);
-        atomicAdd(&l_e[k], -(a * hu[k] + b * hv[k]) / hh[q]);
+        hh[q] = e[k] + cell_depth_land(g, Hi, rr[q], cc[q]);
+        u += wgt[q] * hu[k] / hh[q];
+        v += wgt[q] * hv[k] / hh[q];
+      }
+      const T du = u - T(obs[2 * d]), dv = v - T(obs[2 * d + 1]);
+      jsum += (double)(du * du + dv * dv);
+      if (l_e) {
+        for (int q = 0; q < 4; ++q) {
+          const size_t k = (size_t)rr[q] * g.sp + cc[q];
+          const T a = T(2) * du * wgt[q] / hh[q], b = T(2) * dv * wgt[q] / hh[q];
+          sk[4 * t + q] = ((uint64_t)k << 10) | (uint64_t)(4 * t + q);
+          sv[4 * t + q][0] = a;
+          sv[4 * t + q][1] = b;
+          sv[4 * t + q][2] = -(a * hu[k] + b * hv[k]) / hh[q];
+        }
       }
     }
+    __syncthreads();
+    if (l_e) ordered_add<T>(sk, sv, l_u, l_v, l_e);
   }
-  // wave sum, one atomic per wavefront
-  for (int off = 32; off > 0; off >>= 1) jd += __shfl_down(jd, off);
-  if ((threadIdx.x & 63) == 0) atomicAdd(J, jd);
+  // J: each thread's points in order, then a fixed tree over the threads
+  sj[t] = jsum;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) sj[t] += sj[t + w];
+    __syncthreads();
+  }
+  if (t == 0) *J = *J + sj[0];
 }
 
 // Copy of the interior cells of three state arrays (pitch sp).
@@ -1343,7 +1434,7 @@ int gpo_drift_adj_f32(int32_t nx, int32_t ny, float dx, float dy, float dt, cons
   if (!eta || !hu || !hv || !Bm || !positions || !lam_positions || !lam_eta || !lam_hu || !lam_hv)
     return bad_arg("gpo_drift_adj_f32: bad arguments");
   if (n == 0) return 0;
-  hipLaunchKernelGGL(drift_rev<float>, dim3((n + 63) / 64), dim3(64), 0, (hipStream_t)stream, q, eta, hu, hv, Bm,
+  hipLaunchKernelGGL(drift_rev<float>, dim3(1), dim3(256), 0, (hipStream_t)stream, q, eta, hu, hv, Bm,
                      positions, lam_positions, lam_eta, lam_hu, lam_hv);
   return hip_check(hipGetLastError(), "gpo_drift_adj_f32 launch");
 }
@@ -1358,7 +1449,7 @@ int gpo_drift_adj_f64(int32_t nx, int32_t ny, double dx, double dy, double dt, c
   if (!eta || !hu || !hv || !Bm || !positions || !lam_positions || !lam_eta || !lam_hu || !lam_hv)
     return bad_arg("gpo_drift_adj_f64: bad arguments");
   if (n == 0) return 0;
-  hipLaunchKernelGGL(drift_rev<double>, dim3((n + 63) / 64), dim3(64), 0, (hipStream_t)stream, q, eta, hu, hv, Bm,
+  hipLaunchKernelGGL(drift_rev<double>, dim3(1), dim3(256), 0, (hipStream_t)stream, q, eta, hu, hv, Bm,
                      positions, lam_positions, lam_eta, lam_hu, lam_hv);
   return hip_check(hipGetLastError(), "gpo_drift_adj_f64 launch");
 }
@@ -1377,7 +1468,7 @@ int gpo_point_misfit_f32(int32_t nx, int32_t ny, float dx, float dy, const float
   g.dx = dx;
   g.dy = dy;
   if (n == 0) return 0;
-  hipLaunchKernelGGL(misfit<float>, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, g, n, positions,
+  hipLaunchKernelGGL(misfit<float>, dim3(1), dim3(256), 0, (hipStream_t)stream, g, n, positions,
                      observed, eta, hu, hv, Hi, J, lam_eta, lam_hu, lam_hv);
   return hip_check(hipGetLastError(), "gpo_point_misfit_f32 launch");
 }
@@ -1396,7 +1487,7 @@ int gpo_point_misfit_f64(int32_t nx, int32_t ny, double dx, double dy, const dou
   g.dx = dx;
   g.dy = dy;
   if (n == 0) return 0;
-  hipLaunchKernelGGL(misfit<double>, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, g, n, positions,
+  hipLaunchKernelGGL(misfit<double>, dim3(1), dim3(256), 0, (hipStream_t)stream, g, n, positions,
                      observed, eta, hu, hv, Hi, J, lam_eta, lam_hu, lam_hv);
   return hip_check(hipGetLastError(), "gpo_point_misfit_f64 launch");
 }
@@ -1786,7 +1877,7 @@ int gpo_adj_batch_sweep(gpo_adj_batch h, int32_t m_steps, int32_t* flag, void* s
   for (int j = m_steps - 1; j >= 0 && !rc; --j) {
     const AdjItem<float>* t1 = b->d_items + ((size_t)2 * j + 1) * b->n;
     if (b->any_drift) {  // the drift read the state after the step
-      hipLaunchKernelGGL(nd_drift_rev_b, dim3((b->max_drifters + 63) / 64, b->n), dim3(64), 0, s, b->d_nd, j);
+      hipLaunchKernelGGL(nd_drift_rev_b, dim3(1, b->n), dim3(256), 0, s, b->d_nd, j);
       rc = hip_check(hipGetLastError(), "batched drifter reverse launch");
     }
     if (!rc && b->any_noise) {  // sponge blend, then the model error, in reverse

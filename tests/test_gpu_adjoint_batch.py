@@ -117,6 +117,57 @@ def test_batched_stochastic_drifter_gradients_equal_per_member(gpu_ctx, frs, isl
     ens.destroy()
 
 
+def test_crowded_drifters_and_points_give_the_same_gradient_every_run(gpu_ctx):
+    """Hundreds of drifters and observation points crowded into a few cells
+    (many share a cell, across chunks of 256): their sensitivities are added
+    in drifter / point order, so two batched runs and each member's own run
+    give the same misfits and gradients bit for bit."""
+    n_steps, k = 6, 3
+    print("crowded drifters and points: repeatability", flush=True)
+    sizes, bcs = [(64, 48), (80, 80)], [BCS[0], BCS[1]]
+    members, dpos, dobs = [], [], []
+    for i, ((nx, ny), bc) in enumerate(zip(sizes, bcs)):
+        sim, _, _, bg = _member(gpu_ctx, nx, ny, 120 + i, bc)
+        sim.setSOARModelError(1e-4, xorwow_seed=700 + i)
+        rng = np.random.default_rng(130 + i)
+        cx, cy = 0.5 * nx * sim.dx, 0.5 * ny * sim.dy
+        pts = np.stack([rng.uniform(cx - 2 * sim.dx, cx + 2 * sim.dx, 300),
+                        rng.uniform(cy - 2 * sim.dy, cy + 2 * sim.dy, 300)], 1).astype(np.float32)
+        pobs = (0.01 * rng.standard_normal((300, 2))).astype(np.float32)
+        x = np.stack([rng.uniform(cx - 3 * sim.dx, cx + 3 * sim.dx, 600),
+                      rng.uniform(cy - 3 * sim.dy, cy + 3 * sim.dy, 600)], 1).astype(np.float32)
+        members.append((sim, pts, pobs, bg))
+        dpos.append(x)
+        dobs.append((x + rng.normal(0, 30.0, x.shape)).astype(np.float32))
+
+    def batched():
+        ens = DifferentiableEnsemble([m[0] for m in members], [m[1] for m in members], [m[2] for m in members],
+                                     n_steps, checkpoint_every=k, model_error=True, drifter_positions=dpos,
+                                     drifter_observed=dobs)
+        J = ens.forward([m[3] for m in members])
+        g = ens.backward()
+        assert ens.flag_value() == 0
+        out = ([float(j) for j in J], [[g[i][q].clone() for q in range(3)] for i in range(len(members))])
+        ens.destroy()
+        return out
+
+    J_a, g_a = batched()
+    J_b, g_b = batched()
+    assert J_a == J_b
+    for i, (sim, pts, pobs, bg) in enumerate(members):
+        for q in range(3):
+            assert torch.equal(g_a[i][q], g_b[i][q]), f"member {i} field {q}: repeat"
+        one = DifferentiableCDKLM16(sim, pts, pobs, n_steps, checkpoint_every=k, model_error=True,
+                                    drifter_positions=dpos[i], drifter_observed=dobs[i])
+        J1, _ = one.forward(*bg)
+        g1 = one.backward()
+        assert J_a[i] == J1, f"member {i}"
+        for q in range(3):
+            assert np.array_equal(g_a[i][q][2:-2, 2:-2].cpu().numpy(), g1[q][2:-2, 2:-2].cpu().numpy()), \
+                f"member {i} field {q}"
+        assert g_a[i][0].abs().max().item() > 0
+
+
 @pytest.mark.parametrize("frs,island,noise", [(False, False, False), (False, True, False), (True, True, True)])
 def test_fused_reverse_equals_staged(gpu_ctx, frs, island, noise):
     """The fused strip-march reverse and the four staged reverse kernels give
