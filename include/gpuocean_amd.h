@@ -673,7 +673,9 @@ int gpo_drift_f64(int32_t nx, int32_t ny, double dx, double dy, double dt, const
  * (n pairs) holds the sensitivity of the new positions on entry and of the old
  * ones on return; lam_eta/hu/hv accumulate the sensitivity of the state the
  * drift read (periodic ghost cells fold onto the interior cells they copy;
- * other ghost cells are stored constants and receive nothing). */
+ * other ghost cells are stored constants and receive nothing), added cell by
+ * cell in drifter order, so the result repeats bit for bit.  One workgroup
+ * takes the drifters 256 at a time. */
 int gpo_drift_adj_f32(int32_t nx, int32_t ny, float dx, float dy, float dt, const float* eta, const float* hu,
                       const float* hv, int32_t pitch, const float* Bm, int32_t Bm_pitch, uint32_t boundary_conditions,
                       int32_t n, const float* positions, float* lam_positions, float* lam_eta, float* lam_hu,
@@ -687,7 +689,9 @@ int gpo_drift_adj_f64(int32_t nx, int32_t ny, double dx, double dy, double dt, c
  * (u, v) = (hu, hv)/(eta + Hm) interpolated bilinearly between the four cell
  * centres around positions[2d], positions[2d+1] (metres); observed holds
  * (u_obs, v_obs) pairs.  *J (device double) is accumulated; lam_* (may be
- * NULL) accumulate dJ/d(eta, hu, hv). */
+ * NULL) accumulate dJ/d(eta, hu, hv), cell by cell in point order (J in a
+ * fixed order too), so the results repeat bit for bit.  One workgroup takes
+ * the points 256 at a time. */
 int gpo_point_misfit_f32(int32_t nx, int32_t ny, float dx, float dy, const float* eta, const float* hu,
                          const float* hv, int32_t pitch, const float* Hi, int32_t Hi_pitch, int32_t n,
                          const float* positions, const float* observed, double* J, float* lam_eta, float* lam_hu,
