@@ -166,6 +166,21 @@ __global__ void __launch_bounds__(256) verify_divc(float c, float y, unsigned in
   if (local) atomicAdd(bad, local);
 }
 
+// Exhaustive check that sqrt_rsq equals the IEEE square root for every float
+// in [2^-96, FLT_MAX] (bit patterns 0x0F800000 .. 0x7F7FFFFF).
+constexpr uint32_t kSqrtLo = 0x0F800000u, kSqrtHi = 0x7F7FFFFFu;
+__global__ void __launch_bounds__(256) verify_sqrt(unsigned int* bad) {
+  const uint64_t base = ((uint64_t)blockIdx.x * 256u + threadIdx.x) * 64u;
+  unsigned int local = 0;
+  for (int k = 0; k < 64; ++k) {
+    const uint64_t i = base + (uint64_t)k;
+    if (i > (uint64_t)(kSqrtHi - kSqrtLo)) break;
+    const float x = __uint_as_float(kSqrtLo + (uint32_t)i);
+    local += (__float_as_uint(sqrt_rsq(x)) != __float_as_uint(__builtin_sqrtf(x))) ? 1u : 0u;
+  }
+  if (local) atomicAdd(bad, local);
+}
+
 // per_block_max_dt + max_dt_reduction (max_dt.cu:42-195).  Positive floats
 // order like their bit patterns, so the global minimum is an integer atomic.
 __global__ void __launch_bounds__(256) max_dt_kernel(int nx, int ny, float dx, float dy, float g, float land,
@@ -355,7 +370,7 @@ int make_step_args(const gpo_cdklm16_params* p, float dt, int32_t rk_step, const
   a.dxf = p->dx * p->f;
   a.dyf = p->dy * p->f;
   a.divc_ok = p->divc_ok & 7;
-  a.sqrt_ok = (p->g * p->depth_cutoff) >= 0x1p-96f;
+  a.sqrt_ok = (p->g * p->depth_cutoff) >= 0x1p-96f && (p->divc_ok & 8) != 0;
   a.in_e = in_eta;
   a.in_u = in_hu;
   a.in_v = in_hv;
@@ -398,6 +413,38 @@ extern "C" {
 
 const char* gpo_last_error(void) { return g_last_error.c_str(); }
 
+// sqrt_rsq verified exhaustively (once per process).
+int g_sqrt_state = -1;  // -1 unknown, 0 differs somewhere, 1 verified
+int sqrt_verified(hipStream_t s, bool* ok) {
+  {
+    std::lock_guard<std::mutex> lk(g_divc_mutex);
+    if (g_sqrt_state >= 0) {
+      *ok = g_sqrt_state == 1;
+      return 0;
+    }
+  }
+  *ok = false;
+  unsigned int* d_bad = nullptr;
+  int rc = check(hipMalloc(&d_bad, sizeof(unsigned int)), "gpo_cdklm16_prepare hipMalloc");
+  if (rc) return rc;
+  unsigned int h_bad = 0;
+  rc = check(hipMemcpyAsync(d_bad, &h_bad, sizeof h_bad, hipMemcpyHostToDevice, s), "gpo_cdklm16_prepare copy");
+  const uint64_t threads = ((uint64_t)(kSqrtHi - kSqrtLo) + 1 + 63) / 64;
+  const unsigned int blocks = (unsigned int)((threads + 255) / 256);
+  if (!rc) {
+    hipLaunchKernelGGL(verify_sqrt, dim3(blocks), dim3(256), 0, s, d_bad);
+    rc = check(hipGetLastError(), "gpo_cdklm16_prepare verify launch");
+  }
+  if (!rc) rc = check(hipMemcpyAsync(&h_bad, d_bad, sizeof h_bad, hipMemcpyDeviceToHost, s), "gpo_cdklm16_prepare copy");
+  if (!rc) rc = check(hipStreamSynchronize(s), "gpo_cdklm16_prepare sync");
+  (void)hipFree(d_bad);
+  if (rc) return rc;
+  *ok = (h_bad == 0);
+  std::lock_guard<std::mutex> lk(g_divc_mutex);
+  g_sqrt_state = *ok ? 1 : 0;
+  return 0;
+}
+
 const char* gpo_version(void) { return "gpuocean_amd 0.1 (gfx950)"; }
 
 static hipError_t launch_step(const gpo_cdklm16_params* p, const StepArgs& a, int exact, hipStream_t s) {
@@ -408,12 +455,13 @@ static hipError_t launch_step(const gpo_cdklm16_params* p, const StepArgs& a, in
 int gpo_cdklm16_prepare(gpo_cdklm16_params* p, void* stream) {
   if (!p) return bad("gpo_cdklm16_prepare: params is NULL");
   hipStream_t s = (hipStream_t)stream;
-  bool okx = false, oky = false, okg = false;
+  bool okx = false, oky = false, okg = false, oks = false;
   int rc = divc_verified(p->dx, s, &okx);
   if (!rc) rc = divc_verified(p->dy, s, &oky);
   if (!rc) rc = divc_verified(2.0f * p->g, s, &okg);
+  if (!rc) rc = sqrt_verified(s, &oks);
   if (rc) return rc;
-  p->divc_ok = (okx ? 1 : 0) | (oky ? 2 : 0) | (okg ? 4 : 0);
+  p->divc_ok = (okx ? 1 : 0) | (oky ? 2 : 0) | (okg ? 4 : 0) | (oks ? 8 : 0);
   return 0;
 }
 

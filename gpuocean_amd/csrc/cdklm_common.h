@@ -36,7 +36,7 @@ struct StepArgs {
   int bc_n, bc_e, bc_s, bc_w;
   int rk_order, rk_step, one_dim, has_forcing;
   int divc_ok;   // bit 0: Markstein division by dx verified, bit 1: by dy, bit 2: by 2g
-  bool sqrt_ok;  // g*depth_cutoff >= 2^-96: unscaled correctly rounded sqrt is valid
+  bool sqrt_ok;  // g*depth_cutoff >= 2^-96 and sqrt_rsq verified: the short square root is valid
   // constants as float32, derived on the host the same way the kernel would
   float dt, dx, dy, g, r, theta, desing_eps, flux_slope_eps, depth_cutoff, flux_balancer, land_value;
   float f, north_x, north_y, wind_stress_factor, rho_o;
@@ -56,10 +56,23 @@ struct StepArgs {
   gpo_forcing forcing;
 };
 
-// The exact build's fast divisions by dx, dy and 2g and its unscaled square
+// The exact build's fast divisions by dx, dy and 2g and its short square
 // root are all verified for this argument block (the step kernel then needs
 // no run-time flags for them).
 inline bool fast_arith_ok(const StepArgs& a) { return a.divc_ok == 7 && a.sqrt_ok; }
+
+// Square root from the hardware reciprocal square root y ~ 1/sqrt(x) and one
+// correction: g = x y, then g + (x - g^2) y/2 (Markstein's step for the square
+// root).  gpo_cdklm16_prepare checks on the device, for every float x in
+// [2^-96, FLT_MAX], that it equals the IEEE square root bit for bit before
+// the exact build uses it (bit 3 of gpo_cdklm16_params.divc_ok).
+__device__ __forceinline__ float sqrt_rsq(float x) {
+  const float y = __builtin_amdgcn_rsqf(x);
+  const float g = x * y;
+  const float h = 0.5f * y;
+  const float r = __builtin_fmaf(-g, g, x);
+  return __builtin_fmaf(r, h, g);
+}
 
 // Launchers of the two builds of the step kernel (fast / exact).
 hipError_t launch_cdklm_fast(const StepArgs& a, int has_land, int f_array, int angle0, hipStream_t s);

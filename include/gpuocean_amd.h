@@ -46,7 +46,7 @@ typedef struct {
   float land_value;               /* Bathymetry.mask_value (1e20) */
   float north_x, north_y;         /* sin(angle), cos(angle) for a constant angle */
   float wind_stress_factor, rho_o;/* WIND_STRESS_FACTOR, RHO_O */
-  int32_t divc_ok;                /* set by gpo_cdklm16_prepare (0 = IEEE division only) */
+  int32_t divc_ok;                /* set by gpo_cdklm16_prepare (0 = IEEE division and sqrt only) */
   int32_t waves_per_simd;         /* persistent substep grid, waves per SIMD (tuning; 0 = default) */
 } gpo_cdklm16_params;
 
@@ -66,9 +66,12 @@ typedef struct {
 /* Set-up of a parameter block before its first substep: verifies, on the
  * device and for every numerator in {+0} U +-[2^-96, 2^100], that division by
  * each of dx, dy and 2g equals one Markstein correction step with the
- * correctly rounded reciprocal, and records the result in p->divc_ok (used by
- * the exact build; results are cached per constant).  Synchronises `stream`
- * and allocates a 4-byte counter.  No reference counterpart: the reference
+ * correctly rounded reciprocal (bits 0-2 of p->divc_ok), and, for every float
+ * in [2^-96, FLT_MAX], that the square root from the hardware reciprocal
+ * square root and one correction step equals the IEEE square root (bit 3);
+ * the exact build uses these shorter sequences only where verified (results
+ * are cached per constant and per process).  Synchronises `stream` and
+ * allocates a 4-byte counter.  No reference counterpart: the reference
  * compiles its constants into the kernel (SWEsimulators/CDKLM16.py:242-292). */
 int gpo_cdklm16_prepare(gpo_cdklm16_params* p, void* stream);
 

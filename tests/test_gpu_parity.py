@@ -119,6 +119,17 @@ def test_general_march_equals_oracle(gpu_ctx, general, bc):
         assert np.array_equal(a, b), name
 
 
+def test_prepare_verifies_the_short_arithmetic(gpu_ctx):
+    """gpo_cdklm16_prepare's device checks pass on this GPU (Markstein
+    division by dx, dy and 2g; the reciprocal-square-root square root equal to
+    the IEEE one over every float in [2^-96, FLT_MAX]), so the exact build's
+    parity tests run its short sequences."""
+    from gpuocean_amd import CDKLM16
+    print("prepare: verified arithmetic", flush=True)
+    sim = CDKLM16(gpu_ctx, **cases.config_c0(n=64))
+    assert sim.params.divc_ok == 15
+
+
 @pytest.mark.parametrize("bathy", [False, True])
 def test_lake_at_rest(gpu_ctx, bathy):
     from gpuocean_amd import CDKLM16
