@@ -888,8 +888,8 @@ class DifferentiableEnsemble:
             P = _pitch_elems(m.nx + 4)
             rows = m.ny + 4
             # one allocation per member: state, stage, checkpoints, k+1 states, k stages, 3 lambda states,
-            # and with model error k pre-noise states
-            n_states = 2 + self.n_slots + (self.k + 1) + self.k + 3 + (self.k if m.gen is not None else 0)
+            # and with model error k pre-noise eta planes (three to a state)
+            n_states = 2 + self.n_slots + (self.k + 1) + self.k + 3 + ((self.k + 2) // 3 if m.gen is not None else 0)
             with torch.cuda.stream(self.stream):
                 buf = torch.zeros((n_states, 3, rows, P), dtype=torch.float32, device=self.dev)
                 ws = torch.empty(lib.gpo_adjoint_workspace_bytes(m.nx, m.ny, 4), dtype=torch.uint8, device=self.dev)

@@ -544,7 +544,7 @@ struct NoiseDriftItem {
   float *xi, *coarse;    // dense rows: xi_cols, nx + 4
   float* coarse_seg;     // k SOAR fields of the segment being recomputed (NULL: regenerated in the sweep)
   const float* Hi;
-  float *state, *qs, *pre, *lam;  // forward state; segment states; pre-noise outputs; lambda buffers
+  float *state, *qs, *pre, *lam;  // forward state; segment states; pre-noise eta planes; lambda buffers
   size_t plane, st;      // elements per plane and per state
   int n_drifters;
   const float* Bm;
@@ -601,7 +601,7 @@ __global__ void __launch_bounds__(256) nd_noise_b(const NoiseDriftItem* __restri
   a.Hi = m.Hi, a.f_cells = nullptr;
   a.pre_e = nullptr, a.d_out = nullptr, a.d_row = m.g.nx + 4;
   if (TRAJ) {  // the sweep reads the eta plane of the pre-noise copy (its dry-cell test)
-    a.pre_e = m.pre + (size_t)keep * m.st;
+    a.pre_e = m.pre + (size_t)keep * m.plane;
     a.d_out = nd_coarse(m, slot);
   }
   noise_tile<TRAJ, TRAJ>(a, 2 + kTileH * (t.y >> 16), 2 + kTileW * (t.y & 0xFFFF));
@@ -1645,7 +1645,7 @@ int gpo_adj_batch_create(const gpo_adj_member* members, int32_t n, int32_t k, in
         it.lie = gptr(li), it.liu = gptr(li + pl), it.liv = gptr(li + 2 * pl);
         if (rk) {
           // the second substep's own output: before the model error when there is one
-          it.ie = gptr(be), it.iu = gptr(bu), it.iv = gptr(bv), it.out_e = gptr(M.has_noise ? M.pre + j * st : ce);
+          it.ie = gptr(be), it.iu = gptr(bu), it.iv = gptr(bv), it.out_e = gptr(M.has_noise ? M.pre + j * pl : ce);
           float* lq = buf[j & 1];
           it.lqe = gptr(lq), it.lqu = gptr(lq + pl), it.lqv = gptr(lq + 2 * pl);
           it.accumulate = 0, it.lq_mode = 1;

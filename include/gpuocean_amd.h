@@ -586,7 +586,8 @@ typedef struct {
   float* coarse;                   /* (ny+4) x (nx+4), dense */
   float* coarse_seg;               /* k such fields: the swept segment's, kept from its recomputation
                                     * (NULL: regenerated in the sweep); all members or none */
-  float* pre;                      /* k states: second-substep outputs before the model error */
+  float* pre;                      /* k planes (pitch and plane stride of the states): eta of the
+                                    * second-substep outputs before the model error */
   /* drifters advected after every step (gpo_drift without wind); all members
    * or none */
   int32_t n_drifters;
