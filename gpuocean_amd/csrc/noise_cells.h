@@ -69,7 +69,13 @@ __device__ __forceinline__ void balance_compute(float dx, float dy, float g, flo
   const float h_mid = d + H_mid + b.e;
   const float ddx = (d_xp - d_xm) / (2.0f * dx);
   const float ddy = (d_yp - d_ym) / (2.0f * dy);
-  const float gc = g / coriolis;
+  // On an f-plane (beta = 0, f != 0) coriolis is f itself (f + 0 * x = f), so
+  // g / coriolis is the uniform g / f, which callers looping over cells hoist.
+  float gc;
+  if (beta == 0.0f && f != 0.0f)
+    gc = g / f;
+  else
+    gc = g / coriolis;
   const float d_hu = -gc * h_mid * ddy;
   const float d_hv = gc * h_mid * ddx;
   *e = dry ? 0.0f : b.e + d;
