@@ -88,8 +88,9 @@ __device__ __forceinline__ void noise_tile(const NoiseTile& m, int T0, int S0) {
     // work items (segment, row, quad): the second segment exists only where a
     // periodic window wraps, so most tiles keep every lane busy
     const int n_seg = seg_len0 < ncol ? 2 : 1;
-    for (int u = threadIdx.x; u < n_seg * nrow * kMaxQ; u += 256) {
-      const int seg = u / (nrow * kMaxQ), rem = u - seg * (nrow * kMaxQ), lr = rem / kMaxQ, j = rem - lr * kMaxQ;
+    const int per_seg = nrow * kMaxQ;
+    for (int u = threadIdx.x; u < n_seg * per_seg; u += 256) {
+      const int seg = u >= per_seg ? 1 : 0, rem = u - seg * per_seg, lr = rem / kMaxQ, j = rem - lr * kMaxQ;
       const int V = T0 - 1 + lr;
       const int r = m.per_ns ? wrap(V - 4, m.ny) : V;
       const int ca = seg == 0 ? a0 : 0;
